@@ -1,0 +1,228 @@
+/* The interface between libgail and its HIP (MI355X / gfx950) backend plugin.
+ *
+ * Drop-in counterpart of the reference's CUDA plugin table (src/cuda/cuda_plugin.h:144-195,
+ * entry gail_cuda_plugin_get at cuda_plugin.h:195): libgail dlopens libgail_hip.so from its
+ * own directory, calls gail_hip_plugin_get(GAIL_HIP_PLUGIN_ABI) and reaches the backend
+ * only through the table below. Only plain data crosses: no C++ types, no exceptions, no
+ * torch. A call that can fail returns a GailHipStatus and leaves its message in
+ * last_error() for the calling thread. Memory the plugin hands out (names, result views)
+ * stays valid until the next call on the same object or until it is destroyed.
+ *
+ * One GPU only: device_count() reports the devices HIP sees, and the library uses one.
+ */
+#ifndef GAIL_HIP_PLUGIN_H
+#define GAIL_HIP_PLUGIN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GAIL_HIP_PLUGIN_ABI 1u
+#define GAIL_HIP_PLUGIN_ENTRY "gail_hip_plugin_get"
+
+/* GailCudaStatus (cuda_plugin.h:52-63). */
+enum GailHipStatus {
+    GAIL_HIP_OK = 0,
+    GAIL_HIP_ERROR = 1,         /* std::runtime_error on the library's side */
+    GAIL_HIP_DEVICE_MEMORY = 2, /* gail::DeviceOutOfMemory: the device stays usable */
+    GAIL_HIP_HOST_MEMORY = 3,   /* std::bad_alloc */
+    GAIL_HIP_MISUSE = 4,        /* std::logic_error */
+    GAIL_HIP_NO_CODE = 5,       /* device_usable(): no code object for this GPU */
+    GAIL_HIP_UNSUPPORTED = 6    /* decoder entries: this backend encodes only */
+};
+
+typedef void (*GailHipLog)(void* user, int level, const char* message);
+
+typedef struct GailHipDevice GailHipDevice;
+typedef struct GailHipEncoder GailHipEncoder;
+typedef struct GailHipBatch GailHipBatch;
+
+/* GailCudaDeviceDesc (cuda_plugin.h:72-79); gfx arch instead of compute capability. */
+typedef struct GailHipDeviceDesc {
+    int32_t index;
+    int32_t computeUnits;
+    uint64_t memoryBytes;
+    char name[256];
+    char arch[64];      /* "gfx950" */
+    char pciBusId[32];  /* "0000:01:00.0" */
+} GailHipDeviceDesc;
+
+/* gail::Subband (layout.h:29-40), member for member. */
+typedef struct GailHipSubband {
+    int32_t comp, res, level, orient;
+    int32_t px0, py0, w, h;
+    int32_t ncbx, ncby, cblkBase;
+    int32_t eps, mu, Mb;
+    float delta;
+    double weight;
+} GailHipSubband;
+
+/* GailCudaLayout (cuda_plugin.h:83-88). profile: 0 CINEMA2K, 1 CINEMA4K. */
+typedef struct GailHipLayout {
+    int32_t profile, width, height, levels, guardBits;
+    int32_t numBands, numCblks;
+    double baseStep;
+    const GailHipSubband* bands;
+} GailHipLayout;
+
+/* gail::FrameFormat (color.h:47-73) plus the planar 12-bit X'Y'Z' layout this backend
+ * takes directly. kind: 0 RGB48 (interleaved 16-bit R'G'B'), 1 planar Y'CbCr / G'B'R',
+ * 2 planar X'Y'Z' (three planes of 16-bit words holding 12-bit codes). chroma: 0 left,
+ * 1 center, 2 top-left. */
+typedef struct GailHipFrameFormat {
+    int32_t kind, chroma;
+    int32_t width, height;
+    int32_t activeW, activeH, ox, oy;
+    int32_t sx, sy, rgb, bits, fullRange, pad;
+    double kr, kb;
+} GailHipFrameFormat;
+
+/* GailCudaColor (cuda_plugin.h:91-97), the members this backend honours: transfer
+ * (0 BT.1886, 1 sRGB, 2 PQ, 3 HLG, 4 linear, 5 gamma 2.6) and primaries (0 BT.709,
+ * 1 BT.2020, 2 P3-D65, 3 DCI-P3, 4 X'Y'Z'); HDR sources are taken in the "signal" mode. */
+typedef struct GailHipColor {
+    int32_t transfer, primaries, toneMap, toneNorm, gamut, lut3dSize;
+    double sdrWhite, hdrPeak, cinemaPeak, hdrRefWhite, cinemaRefWhite, knee, gamutThreshold;
+} GailHipColor;
+
+/* GailCudaEncodeSettings (cuda_plugin.h:100-105). */
+typedef struct GailHipEncodeSettings {
+    int32_t profile, width, height;
+    double baseStep, slopeFactor, bitsPerSymbol;
+    GailHipFrameFormat input;
+    GailHipColor color;
+} GailHipEncodeSettings;
+
+/* GailCudaMemory (cuda_plugin.h:113-116). */
+typedef struct GailHipMemory {
+    uint64_t total, available;
+    int32_t budgeted;
+} GailHipMemory;
+
+/* gail::CblkInfo (tier1_types.h:12-18). */
+typedef struct GailHipCblkInfo {
+    uint32_t nbps, numPasses, dataOffset, dataLen, passBase;
+} GailHipCblkInfo;
+
+/* GailCudaTier1 (cuda_plugin.h:119-125): a Tier1Frame as the encoder left it. */
+typedef struct GailHipTier1 {
+    const GailHipCblkInfo* info;
+    const uint32_t* passLen;
+    const int32_t* passDist;
+    const uint8_t* data;
+    uint64_t numInfo, numPasses, numData;
+} GailHipTier1;
+
+/* gail::FrameTimings (backend.h:33-37), milliseconds of device time per stage. */
+typedef struct GailHipFrameTimings {
+    double upload, color, dwt, quant, t1, t1cm, t1mq, download, gpu, total;
+    double symbols, codedBytes;
+} GailHipFrameTimings;
+
+/* GailCudaDebug (cuda_plugin.h:128-134). full: also coefficients and wavelet planes.
+ * xyz12 here is planar (3 planes of codes). */
+typedef struct GailHipDebug {
+    int32_t full;
+    const uint16_t* xyz12;
+    const uint32_t* coeffs;
+    const float* wavelet;
+    uint64_t numXyz12, numCoeffs, numWavelet;
+} GailHipDebug;
+
+/* Tuning of a batched job (no counterpart in the CUDA plugin). Zero fields take defaults. */
+typedef struct GailHipBatchOptions {
+    int32_t dwtTile;      /* 0 64x64, 1 64x32, 2 32x64, 3 32x32 */
+    int32_t cmWaves;      /* waves per context-modelling workgroup: 1, 2, 4, 8 */
+    int32_t mqGroup;      /* MQ workgroup size: 64, 128, 256 */
+    int32_t useGraph;     /* replay the launch chain as a HIP graph */
+    double poolBytesPerCoef;
+} GailHipBatchOptions;
+
+struct GailHipPlugin {
+    uint32_t abi;      /* GAIL_HIP_PLUGIN_ABI the plugin was built with */
+    uint32_t size;     /* sizeof(GailHipPlugin) */
+    uint64_t layout;   /* gail_hip_plugin_layout() of the plugin's build */
+    const char* name;     /* "gail_hip" */
+    const char* toolkit;  /* HIP runtime version it was built with */
+    const char* archs;    /* "gfx950" */
+
+    const char* (*last_error)(void);
+
+    /* devices (cuda_plugin.h:151-173) */
+    int (*device_count)(void);
+    int (*device_desc)(int index, GailHipDeviceDesc* out);
+    int (*device_usable)(int index);
+    int (*device_create)(int index, GailHipLog log, void* user, GailHipDevice** out);
+    void (*device_destroy)(GailHipDevice* d);
+    const char* (*device_name)(GailHipDevice* d);
+    int (*device_memory)(GailHipDevice* d, GailHipMemory* out);
+
+    /* encoder slots (cuda_plugin.h:175-183): one frame at a time */
+    uint64_t (*encoder_bytes)(GailHipDevice* d, const GailHipLayout* layout, const GailHipEncodeSettings* settings);
+    int (*encoder_create)(GailHipDevice* d, const GailHipLayout* layout, const GailHipEncodeSettings* settings,
+                          GailHipEncoder** out);
+    void (*encoder_destroy)(GailHipEncoder* e);
+    void* (*encoder_input)(GailHipEncoder* e);
+    void (*encoder_set_slope)(GailHipEncoder* e, double threshold);
+    /* frame: host memory of settings.input.bytes(). timings and debug may be NULL. */
+    int (*encoder_encode)(GailHipEncoder* e, const void* frame, GailHipTier1* out, GailHipFrameTimings* timings,
+                          GailHipDebug* debug);
+
+    /* decoder entries (cuda_plugin.h:185-191): GAIL_HIP_UNSUPPORTED */
+    uint64_t (*decoder_bytes)(GailHipDevice* d, const GailHipLayout* layout, const void* settings);
+    int (*decoder_create)(GailHipDevice* d, const GailHipLayout* layout, const void* settings, void** out);
+    void (*decoder_destroy)(void* dec);
+    int (*decoder_decode)(void* dec, const GailHipLayout* frame, const uint8_t* data, size_t size, const void* blocks,
+                          size_t numBlocks, void* dst, ptrdiff_t strideBytes, void* timings);
+
+    /* ---- batched form (no counterpart in the CUDA plugin) -------------------------------
+     * A batch is a fixed set of frame slots of any mix of geometries and input formats,
+     * encoded together: one DWT launch per level across all frames and one persistent
+     * Tier-1 launch across all their code-blocks. */
+    uint64_t (*batch_bytes)(GailHipDevice* d, const GailHipLayout* layouts, const GailHipEncodeSettings* settings,
+                            int count, const GailHipBatchOptions* options);
+    int (*batch_create)(GailHipDevice* d, const GailHipLayout* layouts, const GailHipEncodeSettings* settings,
+                        int count, const GailHipBatchOptions* options, GailHipBatch** out);
+    void (*batch_destroy)(GailHipBatch* b);
+    /* Pinned staging buffer of frame slot i. */
+    void* (*batch_input)(GailHipBatch* b, int i);
+    /* Copies frame slot i from host memory into HBM now (outside any timed run). */
+    int (*batch_upload)(GailHipBatch* b, int i, const void* frame);
+    void (*batch_set_slope)(GailHipBatch* b, int i, double threshold);
+    /* Encodes every slot. from_staging: copy each slot's staging buffer to HBM first;
+     * otherwise the frames already resident in HBM are encoded. outs[count]; timings (whole
+     * batch) may be NULL. */
+    int (*batch_encode)(GailHipBatch* b, int from_staging, GailHipTier1* outs, GailHipFrameTimings* timings);
+    /* The same split in two: launch returns once the work is queued on the batch's
+     * stream; wait blocks until the results are in host memory. */
+    int (*batch_launch)(GailHipBatch* b, int from_staging);
+    int (*batch_wait)(GailHipBatch* b, GailHipTier1* outs, GailHipFrameTimings* timings);
+    /* Debug taps of slot i after a run: planar codes, wavelet planes, coefficient words. */
+    int (*batch_debug)(GailHipBatch* b, int i, GailHipDebug* debug);
+};
+typedef struct GailHipPlugin GailHipPlugin;
+
+/* The one symbol the plugin exports (replaces gail_cuda_plugin_get, cuda_plugin.h:195).
+ * Returns the table, or NULL for another GAIL_HIP_PLUGIN_ABI. */
+__attribute__((visibility("default"))) const GailHipPlugin* gail_hip_plugin_get(uint32_t abi);
+
+/* Sizes of every structure above folded into one number (cuda_plugin.h:208-217), which
+ * the loader compares with the table's layout field. */
+static inline uint64_t gail_hip_plugin_layout(void) {
+    const uint64_t sizes[] = {sizeof(GailHipDeviceDesc), sizeof(GailHipSubband),      sizeof(GailHipLayout),
+                              sizeof(GailHipFrameFormat), sizeof(GailHipColor),       sizeof(GailHipEncodeSettings),
+                              sizeof(GailHipMemory),      sizeof(GailHipCblkInfo),    sizeof(GailHipTier1),
+                              sizeof(GailHipFrameTimings), sizeof(GailHipDebug),      sizeof(GailHipBatchOptions),
+                              sizeof(GailHipPlugin)};
+    uint64_t h = 1469598103934665603ull;
+    for (size_t i = 0; i < sizeof sizes / sizeof sizes[0]; ++i) h = (h ^ sizes[i]) * 1099511628211ull;
+    return h;
+}
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,364 @@
+"""Python access to the HIP encoder backend of GAIL (MI355X / gfx950).
+
+The compute path is native: lib/libgail_hip.so is the backend plugin (HIP kernels behind the
+plain-C table of include/gail_hip_plugin.h) and lib/libgail.so the host library. This module
+only binds them with ctypes, for tests and the benchmark; it raises if they are missing and
+never falls back to anything else.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+PLUGIN_PATH = os.path.join(LIB_DIR, "libgail_hip.so")
+HOST_PATH = os.path.join(LIB_DIR, "libgail.so")
+ABI = 1
+MAX_PASSES = 64
+
+
+class GailError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{message} (status {status})")
+        self.status = status
+
+
+class Subband(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("comp", "res", "level", "orient", "px0", "py0", "w", "h", "ncbx", "ncby",
+                                          "cblkBase", "eps", "mu", "Mb")] + [("delta", C.c_float),
+                                                                             ("weight", C.c_double)]
+
+
+class Layout(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("profile", "width", "height", "levels", "guardBits", "numBands",
+                                          "numCblks")] + [("baseStep", C.c_double), ("bands", C.POINTER(Subband))]
+
+
+class FrameFormat(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("kind", "chroma", "width", "height", "activeW", "activeH", "ox", "oy",
+                                          "sx", "sy", "rgb", "bits", "fullRange", "pad")] + [("kr", C.c_double),
+                                                                                             ("kb", C.c_double)]
+
+
+class Color(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("transfer", "primaries", "toneMap", "toneNorm", "gamut", "lut3dSize")] + [
+        (n, C.c_double) for n in ("sdrWhite", "hdrPeak", "cinemaPeak", "hdrRefWhite", "cinemaRefWhite", "knee",
+                                  "gamutThreshold")]
+
+
+class EncodeSettings(C.Structure):
+    _fields_ = [("profile", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("baseStep", C.c_double),
+                ("slopeFactor", C.c_double), ("bitsPerSymbol", C.c_double), ("input", FrameFormat),
+                ("color", Color)]
+
+
+class Memory(C.Structure):
+    _fields_ = [("total", C.c_uint64), ("available", C.c_uint64), ("budgeted", C.c_int32)]
+
+
+class DeviceDesc(C.Structure):
+    _fields_ = [("index", C.c_int32), ("computeUnits", C.c_int32), ("memoryBytes", C.c_uint64),
+                ("name", C.c_char * 256), ("arch", C.c_char * 64), ("pciBusId", C.c_char * 32)]
+
+
+class CblkInfo(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("nbps", "numPasses", "dataOffset", "dataLen", "passBase")]
+
+
+class Tier1(C.Structure):
+    _fields_ = [("info", C.POINTER(CblkInfo)), ("passLen", C.POINTER(C.c_uint32)),
+                ("passDist", C.POINTER(C.c_int32)), ("data", C.POINTER(C.c_uint8)), ("numInfo", C.c_uint64),
+                ("numPasses", C.c_uint64), ("numData", C.c_uint64)]
+
+
+class FrameTimings(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("upload", "color", "dwt", "quant", "t1", "t1cm", "t1mq", "download",
+                                          "gpu", "total", "symbols", "codedBytes")]
+
+
+class Debug(C.Structure):
+    _fields_ = [("full", C.c_int32), ("xyz12", C.POINTER(C.c_uint16)), ("coeffs", C.POINTER(C.c_uint32)),
+                ("wavelet", C.POINTER(C.c_float)), ("numXyz12", C.c_uint64), ("numCoeffs", C.c_uint64),
+                ("numWavelet", C.c_uint64)]
+
+
+class BatchOptions(C.Structure):
+    _fields_ = [("dwtTile", C.c_int32), ("cmWaves", C.c_int32), ("mqGroup", C.c_int32), ("useGraph", C.c_int32),
+                ("poolBytesPerCoef", C.c_double)]
+
+
+_vp = C.c_void_p
+_F = C.CFUNCTYPE
+
+
+class Plugin(C.Structure):
+    _fields_ = [
+        ("abi", C.c_uint32), ("size", C.c_uint32), ("layout", C.c_uint64), ("name", C.c_char_p),
+        ("toolkit", C.c_char_p), ("archs", C.c_char_p),
+        ("last_error", _F(C.c_char_p)),
+        ("device_count", _F(C.c_int)),
+        ("device_desc", _F(C.c_int, C.c_int, C.POINTER(DeviceDesc))),
+        ("device_usable", _F(C.c_int, C.c_int)),
+        ("device_create", _F(C.c_int, C.c_int, _vp, _vp, C.POINTER(_vp))),
+        ("device_destroy", _F(None, _vp)),
+        ("device_name", _F(C.c_char_p, _vp)),
+        ("device_memory", _F(C.c_int, _vp, C.POINTER(Memory))),
+        ("encoder_bytes", _F(C.c_uint64, _vp, C.POINTER(Layout), C.POINTER(EncodeSettings))),
+        ("encoder_create", _F(C.c_int, _vp, C.POINTER(Layout), C.POINTER(EncodeSettings), C.POINTER(_vp))),
+        ("encoder_destroy", _F(None, _vp)),
+        ("encoder_input", _F(_vp, _vp)),
+        ("encoder_set_slope", _F(None, _vp, C.c_double)),
+        ("encoder_encode", _F(C.c_int, _vp, _vp, C.POINTER(Tier1), C.POINTER(FrameTimings), C.POINTER(Debug))),
+        ("decoder_bytes", _F(C.c_uint64, _vp, C.POINTER(Layout), _vp)),
+        ("decoder_create", _F(C.c_int, _vp, C.POINTER(Layout), _vp, C.POINTER(_vp))),
+        ("decoder_destroy", _F(None, _vp)),
+        ("decoder_decode", _F(C.c_int, _vp, C.POINTER(Layout), _vp, C.c_size_t, _vp, C.c_size_t, _vp, C.c_ssize_t,
+                              _vp)),
+        ("batch_bytes", _F(C.c_uint64, _vp, C.POINTER(Layout), C.POINTER(EncodeSettings), C.c_int,
+                           C.POINTER(BatchOptions))),
+        ("batch_create", _F(C.c_int, _vp, C.POINTER(Layout), C.POINTER(EncodeSettings), C.c_int,
+                            C.POINTER(BatchOptions), C.POINTER(_vp))),
+        ("batch_destroy", _F(None, _vp)),
+        ("batch_input", _F(_vp, _vp, C.c_int)),
+        ("batch_upload", _F(C.c_int, _vp, C.c_int, _vp)),
+        ("batch_set_slope", _F(None, _vp, C.c_int, C.c_double)),
+        ("batch_encode", _F(C.c_int, _vp, C.c_int, C.POINTER(Tier1), C.POINTER(FrameTimings))),
+        ("batch_launch", _F(C.c_int, _vp, C.c_int)),
+        ("batch_wait", _F(C.c_int, _vp, C.POINTER(Tier1), C.POINTER(FrameTimings))),
+        ("batch_debug", _F(C.c_int, _vp, C.c_int, C.POINTER(Debug))),
+    ]
+
+
+# DCI containers: flat 1998x1080 / 3996x2160, scope 2048x858 / 4096x1716, full 2048x1080 / 4096x2160
+CONTAINERS = {"2k_flat": (1998, 1080, 0), "2k_scope": (2048, 858, 0), "2k_full": (2048, 1080, 0),
+              "4k_flat": (3996, 2160, 1), "4k_scope": (4096, 1716, 1), "4k_full": (4096, 2160, 1)}
+
+_host = None
+_plugin = None
+_plugin_lib = None
+
+
+def host():
+    """libgail.so (layout, synthetic frames). Raises if it was not built."""
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_PATH):
+            raise FileNotFoundError(f"{HOST_PATH} is missing: run __graft_entry__.build()")
+        h = C.CDLL(HOST_PATH)
+        h.gail_layout_make.restype = C.c_int
+        h.gail_layout_make.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(Subband), C.c_int,
+                                       C.POINTER(C.c_int)]
+        h.gail_layout_cblks.restype = C.c_int
+        h.gail_layout_cblks.argtypes = [C.c_int, C.c_int, C.c_int, _vp, C.c_int]
+        h.gail_synth_xyz12.restype = None
+        h.gail_synth_xyz12.argtypes = [C.c_uint64, C.c_int, C.c_int, _vp, C.c_int]
+        _host = h
+    return _host
+
+
+def plugin():
+    """The HIP plugin's table, loaded the way libgail loads it (by full path, ABI and
+    layout checked). Raises if the plugin is missing or from another build."""
+    global _plugin, _plugin_lib
+    if _plugin is None:
+        if not os.path.exists(PLUGIN_PATH):
+            raise FileNotFoundError(f"{PLUGIN_PATH} is missing: run __graft_entry__.build()")
+        lib = C.CDLL(PLUGIN_PATH, mode=os.RTLD_NOW | os.RTLD_LOCAL)
+        lib.gail_hip_plugin_get.restype = C.POINTER(Plugin)
+        lib.gail_hip_plugin_get.argtypes = [C.c_uint32]
+        p = lib.gail_hip_plugin_get(ABI)
+        if not p:
+            raise RuntimeError("libgail_hip.so was built for another plugin ABI")
+        t = p.contents
+        if t.abi != ABI or t.size != C.sizeof(Plugin):
+            raise RuntimeError("libgail_hip.so does not match this binding's plugin table")
+        _plugin_lib, _plugin = lib, t
+    return _plugin
+
+
+def synth_frame(seed, width, height, threads=8):
+    """Three planar 12-bit X'Y'Z' channels (uint16, shape (3, H, W)) from the seeded generator."""
+    a = np.zeros((3, height, width), np.uint16)
+    host().gail_synth_xyz12(C.c_uint64(seed), width, height, a.ctypes.data_as(_vp), threads)
+    return a
+
+
+class FrameLayout:
+    """makeLayout of a DCI frame in the plain form the plugin takes."""
+
+    def __init__(self, width, height, fourk, base_step=1.0):
+        self.width, self.height, self.fourk, self.base_step = width, height, int(fourk), base_step
+        self.bands = (Subband * 64)()
+        nb = C.c_int(0)
+        n = host().gail_layout_make(width, height, self.fourk, base_step, self.bands, 64, C.byref(nb))
+        if n < 0:
+            raise ValueError(f"invalid frame size {width}x{height}")
+        self.num_bands, self.num_cblks = nb.value, n
+        self.levels = 6 if fourk else 5
+        cb = np.zeros((n, 5), np.int32)
+        host().gail_layout_cblks(width, height, self.fourk, cb.ctypes.data_as(_vp), n)
+        self.cblks = cb
+
+    def plain(self):
+        return Layout(self.fourk, self.width, self.height, self.levels, 2 if self.fourk else 1, self.num_bands,
+                      self.num_cblks, self.base_step, C.cast(self.bands, C.POINTER(Subband)))
+
+
+def xyz_settings(layout):
+    s = EncodeSettings()
+    s.profile, s.width, s.height, s.baseStep = layout.fourk, layout.width, layout.height, layout.base_step
+    s.input.kind, s.input.width, s.input.height = 2, layout.width, layout.height
+    s.color.transfer, s.color.primaries, s.color.sdrWhite, s.color.cinemaPeak = 5, 4, 48.0, 48.0
+    return s
+
+
+def planar_settings(layout, bits, sx, sy, rgb, full_range, chroma=0, kr=0.2126, kb=0.0722, transfer=0,
+                    primaries=0, active=None, offset=(0, 0)):
+    s = EncodeSettings()
+    s.profile, s.width, s.height, s.baseStep = layout.fourk, layout.width, layout.height, layout.base_step
+    f = s.input
+    f.kind, f.width, f.height = 1, layout.width, layout.height
+    aw, ah = active if active else (layout.width, layout.height)
+    f.activeW, f.activeH, f.ox, f.oy = aw, ah, offset[0], offset[1]
+    f.sx, f.sy, f.rgb, f.bits, f.fullRange, f.chroma, f.kr, f.kb = sx, sy, rgb, bits, full_range, chroma, kr, kb
+    s.color.transfer, s.color.primaries, s.color.sdrWhite, s.color.cinemaPeak = transfer, primaries, 48.0, 48.0
+    return s
+
+
+def _check(status, what):
+    if status != 0:
+        raise GailError(status, f"{what}: {plugin().last_error().decode(errors='replace')}")
+
+
+class Tier1Result:
+    """A copy of one frame's Tier-1 output (gail::Tier1Frame)."""
+
+    def __init__(self, t):
+        n, npass, nd = t.numInfo, t.numPasses, t.numData
+        self.info = np.ctypeslib.as_array(C.cast(t.info, C.POINTER(C.c_uint32)), (n * 5,)).reshape(n, 5).copy() \
+            if n else np.zeros((0, 5), np.uint32)
+        self.pass_len = np.ctypeslib.as_array(t.passLen, (npass,)).copy() if npass else np.zeros(0, np.uint32)
+        self.pass_dist = np.ctypeslib.as_array(t.passDist, (npass,)).copy() if npass else np.zeros(0, np.int32)
+        self.data = np.ctypeslib.as_array(t.data, (nd,)).copy() if nd else np.zeros(0, np.uint8)
+
+    def block(self, i):
+        nbps, np_, off, ln, pb = (int(v) for v in self.info[i])
+        return dict(nbps=nbps, num_passes=np_, data=bytes(self.data[off:off + ln]),
+                    pass_len=[int(v) for v in self.pass_len[pb:pb + np_]],
+                    pass_dist=[int(v) for v in self.pass_dist[pb:pb + np_]])
+
+
+class Device:
+    def __init__(self, index=0):
+        p = plugin()
+        n = p.device_count()
+        if n <= 0:
+            raise GailError(1, "no HIP device")
+        _check(p.device_usable(index), "device_usable")
+        self.handle = _vp()
+        _check(p.device_create(index, None, None, C.byref(self.handle)), "device_create")
+
+    def name(self):
+        return plugin().device_name(self.handle).decode()
+
+    def memory(self):
+        m = Memory()
+        _check(plugin().device_memory(self.handle, C.byref(m)), "device_memory")
+        return m.total, m.available
+
+    def close(self):
+        if self.handle:
+            plugin().device_destroy(self.handle)
+            self.handle = None
+
+
+class Encoder:
+    """One frame at a time (the reference's EncoderSlot)."""
+
+    def __init__(self, device, layout, settings):
+        self.layout, self.settings = layout, settings
+        self._plain = layout.plain()
+        self.handle = _vp()
+        _check(plugin().encoder_create(device.handle, C.byref(self._plain), C.byref(settings), C.byref(self.handle)),
+               "encoder_create")
+
+    def set_slope(self, t):
+        plugin().encoder_set_slope(self.handle, t)
+
+    def encode(self, frame, debug=False, full=True):
+        frame = np.ascontiguousarray(frame)
+        out, tm = Tier1(), FrameTimings()
+        dbg = Debug(1 if full else 0) if debug else None
+        _check(plugin().encoder_encode(self.handle, frame.ctypes.data_as(_vp), C.byref(out), C.byref(tm),
+                                       C.byref(dbg) if dbg else None), "encoder_encode")
+        res = Tier1Result(out)
+        res.timings = {k: getattr(tm, k) for k, _ in FrameTimings._fields_}
+        if dbg:
+            res.debug = _debug_arrays(dbg, self.layout)
+        return res
+
+    def close(self):
+        if self.handle:
+            plugin().encoder_destroy(self.handle)
+            self.handle = None
+
+
+def _debug_arrays(dbg, layout):
+    W, H = layout.width, layout.height
+    d = {"codes": np.ctypeslib.as_array(dbg.xyz12, (dbg.numXyz12,)).reshape(3, H, W).copy()}
+    if dbg.full:
+        d["wavelet"] = np.ctypeslib.as_array(dbg.wavelet, (dbg.numWavelet,)).reshape(3, H, W).copy()
+        d["coeffs"] = np.ctypeslib.as_array(dbg.coeffs, (dbg.numCoeffs,)).reshape(-1, 1024).copy()
+    return d
+
+
+class Batch:
+    """Many frame slots of any geometry encoded together (the batched form)."""
+
+    def __init__(self, device, layouts, settings, dwt_tile=0, cm_waves=4, mq_group=256, use_graph=False,
+                 pool_bytes_per_coef=0.0):
+        self.layouts, self.settings = list(layouts), list(settings)
+        n = len(self.layouts)
+        self._plain = (Layout * n)(*[l.plain() for l in self.layouts])
+        self._settings = (EncodeSettings * n)(*self.settings)
+        self.options = BatchOptions(dwt_tile, cm_waves, mq_group, int(use_graph), pool_bytes_per_coef)
+        self.handle = _vp()
+        _check(plugin().batch_create(device.handle, self._plain, self._settings, n, C.byref(self.options),
+                                     C.byref(self.handle)), "batch_create")
+        self.n = n
+
+    def upload(self, i, frame):
+        frame = np.ascontiguousarray(frame)
+        _check(plugin().batch_upload(self.handle, i, frame.ctypes.data_as(_vp)), "batch_upload")
+
+    def stage(self, i, frame):
+        frame = np.ascontiguousarray(frame)
+        dst = plugin().batch_input(self.handle, i)
+        C.memmove(dst, frame.ctypes.data, frame.nbytes)
+
+    def set_slope(self, i, t):
+        plugin().batch_set_slope(self.handle, i, t)
+
+    def launch(self, from_staging=False):
+        _check(plugin().batch_launch(self.handle, int(from_staging)), "batch_launch")
+
+    def wait(self, copy=True):
+        outs, tm = (Tier1 * self.n)(), FrameTimings()
+        _check(plugin().batch_wait(self.handle, outs if copy else None, C.byref(tm)), "batch_wait")
+        timings = {k: getattr(tm, k) for k, _ in FrameTimings._fields_}
+        return ([Tier1Result(o) for o in outs] if copy else None), timings
+
+    def encode(self, from_staging=False, copy=True):
+        self.launch(from_staging)
+        return self.wait(copy)
+
+    def debug(self, i, full=True):
+        dbg = Debug(1 if full else 0)
+        _check(plugin().batch_debug(self.handle, i, C.byref(dbg)), "batch_debug")
+        return _debug_arrays(dbg, self.layouts[i])
+
+    def close(self):
+        if self.handle:
+            plugin().batch_destroy(self.handle)
+            self.handle = None

@@ -1,0 +1,590 @@
+// Device engine of the HIP backend (see engine.h). Host code compiled by hipcc.
+#include "engine.h"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <stdexcept>
+
+#include "errors.h"
+
+namespace gail_amd {
+
+void hipCheck(hipError_t e, const char* what) {
+    if (e == hipSuccess) return;
+    (void)hipGetLastError();
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        throw DeviceOutOfMemory(std::string(what) + ": " + hipGetErrorString(e));
+    throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+size_t InputFormat::bytes() const {
+    switch (kind) {
+    case RGB48: return (size_t)width * height * 6;
+    case XyzPlanar: return (size_t)width * height * 6;
+    default: {
+        const size_t bps = bits > 8 ? 2 : 1;
+        const size_t cw = (size_t)((aw + sx) >> sx), ch = (size_t)((ah + sy) >> sy);
+        return ((size_t)aw * ah + 2 * cw * ch) * bps;
+    }
+    }
+}
+
+// ---------------------------------------------------------------- device context
+DeviceContext& DeviceContext::get(int device) {
+    static std::mutex m;
+    static std::map<int, std::unique_ptr<DeviceContext>> ctx;
+    std::lock_guard<std::mutex> l(m);
+    auto& p = ctx[device];
+    if (!p) p.reset(new DeviceContext(device));
+    return *p;
+}
+
+DeviceContext::DeviceContext(int device) : dev_(device) {
+    hipCheck(hipSetDevice(device), "hipSetDevice");
+    hipDeviceProp_t prop;
+    hipCheck(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    cus_ = prop.multiProcessorCount;
+    name_ = prop.name;
+    arch_ = prop.gcnArchName;
+    hipCheck(t1_tables_init(), "Tier-1 tables");
+    std::vector<float> t;
+    buildLog2Table(t, kLog2Max);
+    hipCheck(hipMalloc(&log2_, t.size() * sizeof(float)), "log2 table");
+    hipCheck(hipMemcpy(log2_, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice), "log2 table upload");
+    buildCodeThresholds(t);
+    hipCheck(hipMalloc(&thr_, t.size() * sizeof(float)), "code thresholds");
+    hipCheck(hipMemcpy(thr_, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice), "code thresholds upload");
+}
+
+// ---------------------------------------------------------------- job
+namespace {
+constexpr size_t kAlign = 256;
+size_t alignUp(size_t v, size_t a = kAlign) { return (v + a - 1) / a * a; }
+
+struct Sizes {
+    size_t pool = 0, out = 0;
+};
+
+Sizes t1Sizes(size_t coefs, uint32_t n, double perCoef) {
+    Sizes z;
+    z.pool = std::max<size_t>(64ull << 20, (size_t)(perCoef * (double)coefs));
+    z.pool = std::max<size_t>(z.pool, (size_t)n * 1400);
+    z.pool = alignUp(z.pool, 1 << 20);
+    z.out = alignUp(z.pool / 2 + (size_t)n * 68);
+    return z;
+}
+}  // namespace
+
+template <class T> T* Job::devAlloc(size_t count) {
+    void* p = nullptr;
+    const size_t b = alignUp(std::max<size_t>(count * sizeof(T), 16));
+    hipCheck(hipMalloc(&p, b), "device allocation");
+    allocs_.push_back(p);
+    bytes_ += b;
+    return (T*)p;
+}
+
+size_t Job::estimateBytes(const std::vector<FrameConfig>& frames, const JobOptions& opt) {
+    size_t total = 0, coefs = 0;
+    uint32_t n = 0;
+    for (const FrameConfig& f : frames) {
+        const size_t W = f.layout.width, H = f.layout.height, npix = W * H;
+        total += alignUp(f.input.bytes()) + (f.input.kind == InputFormat::XyzPlanar ? 0 : alignUp(npix * 6));
+        total += alignUp(npix * 12) + alignUp(3 * ((W + 1) / 2) * ((H + 1) / 2) * 4) + alignUp(3 * ((W + 3) / 4) * ((H + 3) / 4) * 4);
+        if (f.input.kind != InputFormat::XyzPlanar) total += 65536 * 4;
+        coefs += 3 * npix;
+        n += (uint32_t)f.layout.numCblks();
+    }
+    const Sizes z = t1Sizes(coefs, n, opt.poolBytesPerCoef);
+    total += (size_t)n * (16 + 16 + 3 * kMaxPasses * 4 + kMaxPlanes * 8 + kMaxPasses * 16 + 4 + 8 + 8 + 4);
+    total += z.pool + z.out;
+    return total;
+}
+
+Job::Job(int device, std::vector<FrameConfig> frames, const JobOptions& opt)
+    : dev_(device), ctx_(DeviceContext::get(device)), opt_(opt), frames_(std::move(frames)) {
+    if (frames_.empty()) throw std::logic_error("a job needs at least one frame");
+    try {
+        allocate();
+    } catch (...) {
+        release();
+        throw;
+    }
+}
+
+Job::~Job() { release(); }
+
+void Job::release() {
+    hipSetDevice(dev_);
+    if (stream_) hipStreamSynchronize(stream_);
+    if (graph_) hipGraphExecDestroy(graph_);
+    graph_ = nullptr;
+    for (void* p : allocs_) hipFree(p);
+    allocs_.clear();
+    for (void* p : hostIn_)
+        if (p) hipHostFree(p);
+    hostIn_.clear();
+    for (void* p : {(void*)hFrames_, (void*)hInfo_, (void*)hPassLen_, (void*)hPassDist_, (void*)hData_, (void*)hSmall_})
+        if (p) hipHostFree(p);
+    hFrames_ = nullptr;
+    hInfo_ = nullptr;
+    hPassLen_ = nullptr;
+    hPassDist_ = nullptr;
+    hData_ = nullptr;
+    hSmall_ = nullptr;
+    for (auto& e : ev_)
+        if (e) {
+            hipEventDestroy(e);
+            e = nullptr;
+        }
+    if (stream_) hipStreamDestroy(stream_);
+    stream_ = nullptr;
+    bytes_ = 0;
+}
+
+static void hostAlloc(void** p, size_t bytes) {
+    hipCheck(hipHostMalloc(p, std::max<size_t>(bytes, 16), hipHostMallocDefault), "pinned host memory");
+}
+
+void Job::allocate() {
+    hipCheck(hipSetDevice(dev_), "hipSetDevice");
+    hipCheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream");
+    for (auto& e : ev_) hipCheck(hipEventCreate(&e), "event");
+    const int nf = (int)frames_.size();
+    fd_.resize(nf);
+    hostIn_.assign(nf, nullptr);
+    slope_.assign(nf, 0.f);
+    std::vector<T1Block> blocks;
+    std::vector<T1Band> bands;
+    int tx, ty;
+    dwt_tile_size(opt_.dwtTile, tx, ty);
+    int maxLevels = 0;
+    for (int f = 0; f < nf; ++f) {
+        const FrameConfig& fc = frames_[f];
+        const Layout& L = fc.layout;
+        FrameDev& d = fd_[f];
+        const size_t W = L.width, H = L.height;
+        d.npix = W * H;
+        d.inBytes = fc.input.bytes();
+        if (fc.input.width != L.width || fc.input.height != L.height)
+            throw std::logic_error("input format does not match the frame size");
+        d.in = devAlloc<uint8_t>(d.inBytes);
+        d.codes = fc.input.kind == InputFormat::XyzPlanar ? (uint16_t*)d.in : devAlloc<uint16_t>(3 * d.npix);
+        d.coef = devAlloc<float>(3 * d.npix);
+        d.llA = devAlloc<float>(3 * ((W + 1) / 2) * ((H + 1) / 2));
+        d.llB = devAlloc<float>(3 * ((W + 3) / 4) * ((H + 3) / 4));
+        if (fc.input.kind != InputFormat::XyzPlanar) {
+            std::vector<float> lut;
+            buildColorTables(fc.color, lut, d.matrix);
+            d.lut = devAlloc<float>(65536);
+            hipCheck(hipMemcpy(d.lut, lut.data(), 65536 * 4, hipMemcpyHostToDevice), "EOTF table upload");
+        }
+        hostAlloc(&hostIn_[f], d.inBytes);
+        d.firstBlock = (uint32_t)blocks.size();
+        d.numBlocks = (uint32_t)L.numCblks();
+        const uint32_t bandBase = (uint32_t)bands.size();
+        for (const Subband& b : L.bands)
+            bands.push_back(T1Band{1.0f / b.delta, ((1u << b.Mb) << 6) - 1u, (float)((double)b.delta * b.delta * b.weight), 0u});
+        for (int i = 0; i < L.numCblks(); ++i) {
+            const CodeBlock& cb = L.cblks[i];
+            const Subband& b = L.bands[cb.band];
+            T1Block t{};
+            t.coef_off = (uint32_t)((size_t)b.comp * d.npix + (size_t)(b.py0 + cb.y0) * W + (size_t)(b.px0 + cb.x0));
+            t.frame = (uint16_t)f;
+            t.band = (uint16_t)(bandBase + cb.band);
+            t.w = (uint8_t)cb.w;
+            t.h = (uint8_t)cb.h;
+            t.orient = (uint8_t)b.orient;
+            t.frame_block = (uint32_t)i;
+            blocks.push_back(t);
+        }
+        totalCoefs_ += 3 * d.npix;
+        maxLevels = std::max(maxLevels, L.levels);
+    }
+    nblocks_ = (uint32_t)blocks.size();
+    if (bands.size() > 65535 || nf > 65535) throw std::logic_error("job too large");
+    dBlocks_ = devAlloc<T1Block>(nblocks_);
+    hipCheck(hipMemcpy(dBlocks_, blocks.data(), blocks.size() * sizeof(T1Block), hipMemcpyHostToDevice), "block table");
+    dBands_ = devAlloc<T1Band>(bands.size());
+    hipCheck(hipMemcpy(dBands_, bands.data(), bands.size() * sizeof(T1Band), hipMemcpyHostToDevice), "band table");
+    dFrames_ = devAlloc<T1FrameRef>(nf);
+    hostAlloc((void**)&hFrames_, sizeof(T1FrameRef) * nf);
+    for (int f = 0; f < nf; ++f) hFrames_[f] = T1FrameRef{fd_[f].coef, (uint32_t)frames_[f].layout.width, 0.f};
+
+    // DWT level descriptors: level l of every frame that has it, tiles numbered across frames.
+    levelDescs_.assign(maxLevels, {});
+    levelTiles_.assign(maxLevels, 0);
+    for (int f = 0; f < nf; ++f) {
+        const Layout& L = frames_[f].layout;
+        FrameDev& d = fd_[f];
+        int w = L.width, h = L.height;
+        for (int l = 0; l < L.levels; ++l) {
+            DwtLevelDesc ds{};
+            const int w2 = (w + 1) / 2, h2 = (h + 1) / 2;
+            if (l == 0) {
+                ds.src = d.codes;
+                ds.src_stride = (uint32_t)L.width;
+                ds.src_plane = (uint32_t)d.npix;
+            } else {
+                ds.src = ((l - 1) % 2 == 0) ? d.llA : d.llB;
+                ds.src_stride = (uint32_t)w;
+                ds.src_plane = (uint32_t)(w * h);
+            }
+            ds.coef = d.coef;
+            ds.coef_stride = (uint32_t)L.width;
+            ds.coef_plane = (uint32_t)d.npix;
+            if (l == L.levels - 1) {
+                ds.ll_dst = d.coef;
+                ds.ll_stride = (uint32_t)L.width;
+                ds.ll_plane = (uint32_t)d.npix;
+            } else {
+                ds.ll_dst = (l % 2 == 0) ? d.llA : d.llB;
+                ds.ll_stride = (uint32_t)w2;
+                ds.ll_plane = (uint32_t)(w2 * h2);
+            }
+            ds.w = w;
+            ds.h = h;
+            ds.tiles_x = (w + tx - 1) / tx;
+            ds.tile_base = (uint32_t)levelTiles_[l];
+            levelTiles_[l] += ds.tiles_x * ((h + ty - 1) / ty);
+            levelDescs_[l].push_back(ds);
+            w = w2;
+            h = h2;
+        }
+    }
+    for (int l = 0; l < maxLevels; ++l) {
+        DwtLevelDesc* p = devAlloc<DwtLevelDesc>(levelDescs_[l].size());
+        hipCheck(hipMemcpy(p, levelDescs_[l].data(), levelDescs_[l].size() * sizeof(DwtLevelDesc), hipMemcpyHostToDevice),
+                 "DWT descriptors");
+        dLevelDescs_.push_back(p);
+    }
+
+    const uint32_t n = nblocks_;
+    dInfo_ = devAlloc<T1Info>(n);
+    dPassEnd_ = devAlloc<uint32_t>((size_t)n * kMaxPasses);
+    dPassDist_ = devAlloc<int32_t>((size_t)n * kMaxPasses);
+    dPassLen_ = devAlloc<uint32_t>((size_t)n * kMaxPasses);
+    dChunks_ = devAlloc<uint2>((size_t)n * kMaxPlanes);
+    dSnaps_ = devAlloc<MqSnapshot>((size_t)n * kMaxPasses);
+    dPerm_ = devAlloc<uint32_t>(n);
+    dHist_ = devAlloc<uint32_t>(128);
+    dOutOff_ = devAlloc<unsigned long long>(n);
+    dDataOff_ = devAlloc<unsigned long long>(n);
+    dPartial_ = devAlloc<unsigned long long>(n / 2048 + 4);
+    dCodeLen_ = devAlloc<uint32_t>(n);
+    dCounters_ = devAlloc<JobCounters>(1);
+    dWork_ = devAlloc<uint32_t>(4);
+    hostAlloc((void**)&hSmall_, 256);
+    hostAlloc((void**)&hInfo_, (size_t)n * 20);
+    hPassCap_ = (size_t)n * kMaxPasses;
+    hostAlloc((void**)&hPassLen_, hPassCap_ * 4);
+    hostAlloc((void**)&hPassDist_, hPassCap_ * 4);
+    hDataCap_ = std::max<size_t>(16ull << 20, totalCoefs_);
+    hostAlloc((void**)&hData_, hDataCap_);
+    const Sizes z = t1Sizes(totalCoefs_, n, opt_.poolBytesPerCoef);
+    poolCap_ = z.pool;
+    outCap_ = z.out;
+    dPool_ = devAlloc<uint8_t>(poolCap_);
+    dOut_ = devAlloc<uint8_t>(outCap_);
+    fixedInfo_.assign(nf, {});
+    frameDataBase_.assign(nf, 0);
+    framePassBase_.assign(nf, 0);
+}
+
+// Replaces the symbol pool and / or the codeword buffer with ones twice as large (after a
+// run overflowed them); the chain is then run again from context modelling.
+void Job::growT1(bool pool, bool out) {
+    hipCheck(hipStreamSynchronize(stream_), "sync");
+    auto replace = [&](uint8_t*& p, size_t& cap) {
+        auto it = std::find(allocs_.begin(), allocs_.end(), (void*)p);
+        if (it != allocs_.end()) allocs_.erase(it);
+        hipFree(p);
+        bytes_ -= alignUp(cap);
+        cap *= 2;
+        p = devAlloc<uint8_t>(cap);
+    };
+    // Both grow together: the dense results live in the pool behind room for every
+    // codeword, so the pool must stay larger than the codeword buffer.
+    (void)pool;
+    (void)out;
+    replace(dPool_, poolCap_);
+    replace(dOut_, outCap_);
+    if (graph_) {
+        hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
+}
+
+void Job::upload(int i, const void* frame) {
+    hipCheck(hipSetDevice(dev_), "hipSetDevice");
+    hipCheck(hipMemcpy(fd_[i].in, frame, fd_[i].inBytes, hipMemcpyHostToDevice), "frame upload");
+}
+
+void Job::enqueueT1(hipStream_t s) {
+    const uint32_t n = nblocks_;
+    hipCheck(launch_clear_counters(dCounters_, dWork_, 4, s), "clear counters");
+    CmParams cm{};
+    cm.blocks = dBlocks_;
+    cm.bands = dBands_;
+    cm.frames = dFrames_;
+    cm.nblocks = n;
+    cm.max_passes = kMaxPasses;
+    cm.info = dInfo_;
+    cm.pass_end = dPassEnd_;
+    cm.pass_dist = dPassDist_;
+    cm.chunks = dChunks_;
+    cm.pool = dPool_;
+    cm.pool_cap = poolCap_;
+    cm.counters = dCounters_;
+    cm.work_next = dWork_;
+    cm.log2tab = ctx_.log2Table();
+    const int waves = opt_.cmWaves;
+    const uint32_t pairs = (n + 1) / 2;
+    const int cmGroups = (int)std::max<uint32_t>(
+        1, std::min<uint32_t>((pairs + waves - 1) / waves, (uint32_t)(ctx_.computeUnits() * opt_.cmGroupsPerCU)));
+    hipCheck(launch_t1_cm(cm, waves, cmGroups, s), "context modelling");
+    if (!capturing_) hipCheck(hipEventRecord(ev_[4], s), "event");
+    hipCheck(launch_sort_blocks(dInfo_, n, dHist_, dPerm_, s), "block ordering");
+    hipCheck(launch_out_offsets(dInfo_, n, dOutOff_, dPartial_, s), "codeword offsets");
+    MqParams mq{};
+    mq.info = dInfo_;
+    mq.pass_end = dPassEnd_;
+    mq.chunks = dChunks_;
+    mq.pool = dPool_;
+    mq.perm = dPerm_;
+    mq.out_off = dOutOff_;
+    mq.out = dOut_;
+    mq.out_cap = outCap_;
+    mq.snaps = dSnaps_;
+    mq.pass_len = dPassLen_;
+    mq.code_len = dCodeLen_;
+    mq.counters = dCounters_;
+    mq.nblocks = n;
+    mq.work_next = dWork_ + 1;
+    const int mqGroups = (int)std::max<uint32_t>(
+        1, std::min<uint32_t>((n + opt_.mqGroup - 1) / opt_.mqGroup, (uint32_t)(ctx_.computeUnits() * opt_.mqGroupsPerCU)));
+    hipCheck(launch_t1_mq(mq, opt_.mqGroup, mqGroups, s), "MQ coding");
+    if (!capturing_) hipCheck(hipEventRecord(ev_[5], s), "event");
+    hipCheck(launch_compact_offsets(dInfo_, dCodeLen_, n, dDataOff_, nullptr, dPartial_, s), "compaction offsets");
+    // dense results live in the (now consumed) symbol pool
+    const size_t dataBytes = alignUp(outCap_);
+    dDenseData_ = dPool_;
+    dDenseInfo_ = (uint32_t*)(dPool_ + dataBytes);
+    dDensePassLen_ = (uint32_t*)(dPool_ + dataBytes + alignUp((size_t)n * 20));
+    dDensePassDist_ = (int32_t*)(dPool_ + dataBytes + alignUp((size_t)n * 20) + alignUp((size_t)n * kMaxPasses * 4));
+    CompactParams cp{};
+    cp.info = dInfo_;
+    cp.code_len = dCodeLen_;
+    cp.out_off = dOutOff_;
+    cp.out = dOut_;
+    cp.pass_len = dPassLen_;
+    cp.pass_dist = dPassDist_;
+    cp.data_off = dDataOff_;
+    cp.dense_info = dDenseInfo_;
+    cp.dense_pass_len = dDensePassLen_;
+    cp.dense_pass_dist = dDensePassDist_;
+    cp.dense_data = dDenseData_;
+    cp.counters = dCounters_;
+    cp.nblocks = n;
+    hipCheck(launch_compact(cp, s), "compaction");
+    if (!capturing_) hipCheck(hipEventRecord(ev_[6], s), "event");
+    // flags and totals for finish()
+    const int groups = (int)((n + 2047) / 2048);
+    hipCheck(hipMemcpyAsync(hSmall_, dCounters_, sizeof(JobCounters), hipMemcpyDeviceToHost, s), "counters");
+    hipCheck(hipMemcpyAsync(hSmall_ + 16, dPartial_ + groups, 8, hipMemcpyDeviceToHost, s), "totals");
+}
+
+void Job::enqueueChain(bool uploadInputs) {
+    hipStream_t s = stream_;
+    if (!capturing_) hipCheck(hipEventRecord(ev_[0], s), "event");
+    if (uploadInputs)
+        for (int f = 0; f < numFrames(); ++f)
+            hipCheck(hipMemcpyAsync(fd_[f].in, hostIn_[f], fd_[f].inBytes, hipMemcpyHostToDevice, s), "frame upload");
+    hipCheck(hipMemcpyAsync(dFrames_, hFrames_, sizeof(T1FrameRef) * numFrames(), hipMemcpyHostToDevice, s), "frame table");
+    if (!capturing_) hipCheck(hipEventRecord(ev_[1], s), "event");
+    for (int f = 0; f < numFrames(); ++f) {
+        const InputFormat& in = frames_[f].input;
+        if (in.kind == InputFormat::XyzPlanar) continue;
+        ColorDesc c{};
+        c.src = fd_[f].in;
+        c.codes = fd_[f].codes;
+        c.lut = fd_[f].lut;
+        c.thresholds = ctx_.thresholds();
+        std::memcpy(c.matrix, fd_[f].matrix, sizeof c.matrix);
+        c.width = in.width;
+        c.height = in.height;
+        c.kind = in.kind == InputFormat::RGB48 ? 0 : 1;
+        c.aw = in.aw;
+        c.ah = in.ah;
+        c.ox = in.ox;
+        c.oy = in.oy;
+        c.sx = in.sx;
+        c.sy = in.sy;
+        c.bits = in.bits;
+        c.rgb = in.rgb;
+        c.full_range = in.fullRange;
+        c.chroma = in.chroma;
+        const double kg = 1.0 - in.kr - in.kb;
+        c.crR = (float)(2 * (1 - in.kr));
+        c.cbB = (float)(2 * (1 - in.kb));
+        c.cbG = (float)(2 * in.kb * (1 - in.kb) / kg);
+        c.crG = (float)(2 * in.kr * (1 - in.kr) / kg);
+        hipCheck(launch_color(c, s), "colour");
+    }
+    if (!capturing_) hipCheck(hipEventRecord(ev_[2], s), "event");
+    for (size_t l = 0; l < levelDescs_.size(); ++l)
+        hipCheck(launch_dwt_level(l == 0, opt_.dwtTile, dLevelDescs_[l], (int)levelDescs_[l].size(), levelTiles_[l], s),
+                 "DWT level");
+    if (!capturing_) hipCheck(hipEventRecord(ev_[3], s), "event");
+    enqueueT1(s);
+}
+
+void Job::launch(bool uploadInputs) {
+    hipCheck(hipSetDevice(dev_), "hipSetDevice");
+    for (int f = 0; f < numFrames(); ++f) hFrames_[f].slope_thresh = slope_[f];
+    if (opt_.useGraph) {
+        if (!graph_ || graphUpload_ != uploadInputs) {
+            if (graph_) hipGraphExecDestroy(graph_);
+            graph_ = nullptr;
+            hipGraph_t g = nullptr;
+            hipCheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "graph capture");
+            capturing_ = true;
+            try {
+                enqueueChain(uploadInputs);
+                capturing_ = false;
+            } catch (...) {
+                capturing_ = false;
+                hipStreamEndCapture(stream_, &g);
+                if (g) hipGraphDestroy(g);
+                throw;
+            }
+            hipCheck(hipStreamEndCapture(stream_, &g), "graph capture");
+            hipCheck(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0), "graph instantiate");
+            hipGraphDestroy(g);
+            graphUpload_ = uploadInputs;
+        }
+        hipCheck(hipEventRecord(ev_[0], stream_), "event");
+        hipCheck(hipGraphLaunch(graph_, stream_), "graph launch");
+        hipCheck(hipEventRecord(ev_[6], stream_), "event");
+        return;
+    }
+    enqueueChain(uploadInputs);
+}
+
+void Job::finish(StageTimes* times) {
+    hipCheck(hipStreamSynchronize(stream_), "encode");
+    int reruns = 0;
+    for (;;) {
+        const JobCounters* c = (const JobCounters*)hSmall_;
+        const uint32_t flags = c->flags;
+        if (flags & 4u) throw std::runtime_error("code-block with more than 24 magnitude bit-planes");
+        if (!(flags & 3u)) break;
+        if (++reruns > 6) throw std::runtime_error("Tier-1 buffers keep overflowing");
+        growT1((flags & 1u) != 0, (flags & 2u) != 0);
+        enqueueT1(stream_);
+        hipCheck(hipStreamSynchronize(stream_), "encode (re-run)");
+    }
+    const unsigned long long packed = hSmall_[16];
+    numDataTotal_ = (size_t)(packed & ((1ull << 36) - 1));
+    numPassesTotal_ = (size_t)(packed >> 36);
+    if (numDataTotal_ > hDataCap_) {
+        hipHostFree(hData_);
+        hData_ = nullptr;
+        hDataCap_ = numDataTotal_ + (numDataTotal_ >> 2);
+        hostAlloc((void**)&hData_, hDataCap_);
+    }
+    hipStream_t s = stream_;
+    hipCheck(hipMemcpyAsync(hInfo_, dDenseInfo_, (size_t)nblocks_ * 20, hipMemcpyDeviceToHost, s), "result download");
+    hipCheck(hipMemcpyAsync(hPassLen_, dDensePassLen_, numPassesTotal_ * 4, hipMemcpyDeviceToHost, s), "result download");
+    hipCheck(hipMemcpyAsync(hPassDist_, dDensePassDist_, numPassesTotal_ * 4, hipMemcpyDeviceToHost, s), "result download");
+    hipCheck(hipMemcpyAsync(hData_, dDenseData_, numDataTotal_, hipMemcpyDeviceToHost, s), "result download");
+    hipCheck(hipEventRecord(ev_[7], s), "event");
+    hipCheck(hipStreamSynchronize(s), "result download");
+    // frame-local offsets
+    for (int f = 0; f < numFrames(); ++f) {
+        uint32_t* inf = hInfo_ + (size_t)fd_[f].firstBlock * 5;
+        const uint32_t d0 = inf[2], p0 = inf[4];
+        frameDataBase_[f] = d0;
+        framePassBase_[f] = p0;
+        for (uint32_t b = 0; b < fd_[f].numBlocks; ++b) {
+            inf[5 * b + 2] -= d0;
+            inf[5 * b + 4] -= p0;
+        }
+    }
+    if (times) {
+        StageTimes t;
+        auto ms = [&](int a, int b) {
+            float v = 0;
+            return hipEventElapsedTime(&v, ev_[a], ev_[b]) == hipSuccess ? v : 0.f;
+        };
+        if (!opt_.useGraph) {
+            t.upload = ms(0, 1);
+            t.color = ms(1, 2);
+            t.dwt = ms(2, 3);
+            t.t1cm = ms(3, 4);
+            t.t1mq = ms(4, 5);
+        }
+        t.compact = opt_.useGraph ? 0.f : ms(5, 6);
+        t.download = ms(6, 7);
+        t.gpu = ms(0, 7);
+        t.bytes = (double)numDataTotal_;
+        t.reruns = reruns;
+        *times = t;
+    }
+}
+
+void Job::encode(bool uploadInputs, StageTimes* times) {
+    launch(uploadInputs);
+    finish(times);
+}
+
+Tier1View Job::result(int i) const {
+    const FrameDev& d = fd_[i];
+    const size_t next = i + 1 < numFrames() ? frameDataBase_[i + 1] : numDataTotal_;
+    const size_t nextP = i + 1 < numFrames() ? framePassBase_[i + 1] : numPassesTotal_;
+    Tier1View v;
+    v.info5 = hInfo_ + (size_t)d.firstBlock * 5;
+    v.passLen = hPassLen_ + framePassBase_[i];
+    v.passDist = hPassDist_ + framePassBase_[i];
+    v.data = hData_ + frameDataBase_[i];
+    v.numBlocks = d.numBlocks;
+    v.numPasses = nextP - framePassBase_[i];
+    v.numData = next - frameDataBase_[i];
+    return v;
+}
+
+void Job::debugCodes(int i, std::vector<uint16_t>& out) {
+    out.resize(3 * fd_[i].npix);
+    hipCheck(hipStreamSynchronize(stream_), "sync");
+    hipCheck(hipMemcpy(out.data(), fd_[i].codes, out.size() * 2, hipMemcpyDeviceToHost), "codes download");
+}
+
+void Job::debugWavelet(int i, std::vector<float>& out) {
+    out.resize(3 * fd_[i].npix);
+    hipCheck(hipStreamSynchronize(stream_), "sync");
+    hipCheck(hipMemcpy(out.data(), fd_[i].coef, out.size() * 4, hipMemcpyDeviceToHost), "wavelet download");
+}
+
+void Job::debugCoeffs(int i, std::vector<uint32_t>& out) {
+    const FrameDev& d = fd_[i];
+    uint32_t* tmp = nullptr;
+    hipCheck(hipMalloc(&tmp, (size_t)d.numBlocks * 4096), "debug buffer");
+    hipCheck(launch_quant_debug(dBlocks_ + d.firstBlock, dBands_, dFrames_, d.numBlocks, tmp, stream_), "quantiser");
+    out.resize((size_t)d.numBlocks * 1024);
+    hipCheck(hipStreamSynchronize(stream_), "sync");
+    const hipError_t e = hipMemcpy(out.data(), tmp, out.size() * 4, hipMemcpyDeviceToHost);
+    hipFree(tmp);
+    hipCheck(e, "coefficient download");
+}
+
+void Job::debugInfo(int i, std::vector<T1Info>& out) {
+    const FrameDev& d = fd_[i];
+    out.resize(d.numBlocks);
+    hipCheck(hipStreamSynchronize(stream_), "sync");
+    hipCheck(hipMemcpy(out.data(), dInfo_ + d.firstBlock, d.numBlocks * sizeof(T1Info), hipMemcpyDeviceToHost), "info");
+}
+
+}  // namespace gail_amd

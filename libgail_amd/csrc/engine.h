@@ -1,0 +1,207 @@
+// Device engine of the HIP backend: plans a job (one frame, or many frames of mixed
+// geometry), owns its HBM buffers and stream, and runs colour -> DWT -> fused quantisation
+// + context modelling -> ordering -> MQ -> compaction on it.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "color_tables.h"
+#include "device_types.h"
+#include "kernels.h"
+#include "layout.h"
+
+namespace gail_amd {
+
+// How frames are handed over.
+struct InputFormat {
+    enum Kind : int { RGB48 = 0, Planar = 1, XyzPlanar = 2 };
+    int kind = XyzPlanar;
+    int width = 0, height = 0;                     // container
+    int aw = 0, ah = 0, ox = 0, oy = 0;            // planar: active picture and its offset
+    int sx = 1, sy = 1, bits = 8, rgb = 0, fullRange = 0, chroma = 0;
+    double kr = 0.2126, kb = 0.0722;
+    size_t bytes() const;
+};
+
+struct FrameConfig {
+    Layout layout;
+    InputFormat input;
+    ColorSpec color;
+};
+
+struct JobOptions {
+    int dwtTile = 0;          // tile shape index (see dwt_tile_size)
+    int cmWaves = 4;          // waves per context-modelling workgroup
+    int cmGroupsPerCU = 8;    // persistent grid size per compute unit
+    int mqGroup = 256;        // MQ workgroup size
+    int mqGroupsPerCU = 8;
+    double poolBytesPerCoef = 8.0;  // symbol pool sizing (grown on overflow)
+    bool useGraph = false;    // replay the launch chain as a HIP graph
+};
+
+// Per-stage device time (ms) of the last run, from HIP events on the job's stream.
+struct StageTimes {
+    float upload = 0, color = 0, dwt = 0, t1cm = 0, t1mq = 0, compact = 0, download = 0, gpu = 0;
+    double symbols = 0, bytes = 0;
+    int reruns = 0;
+};
+
+// One frame's Tier-1 result, in the job's pinned host buffers (valid until the next run).
+struct Tier1View {
+    const uint32_t* info5;  // per block: nbps, passes, data offset, data length, pass base
+    const uint32_t* passLen;
+    const int32_t* passDist;
+    const uint8_t* data;
+    size_t numBlocks, numPasses, numData;
+};
+
+// Device-wide state shared by every job on a device: the constant Tier-1 tables and the
+// colour / entropy lookup tables.
+class DeviceContext {
+public:
+    static DeviceContext& get(int device);
+    int device() const { return dev_; }
+    const float* log2Table() const { return log2_; }
+    const float* thresholds() const { return thr_; }
+    int computeUnits() const { return cus_; }
+    std::string name() const { return name_; }
+    std::string arch() const { return arch_; }
+
+private:
+    explicit DeviceContext(int device);
+    int dev_ = 0, cus_ = 0;
+    float* log2_ = nullptr;
+    float* thr_ = nullptr;
+    std::string name_, arch_;
+};
+
+class Job {
+public:
+    Job(int device, std::vector<FrameConfig> frames, const JobOptions& opt);
+    ~Job();
+    Job(const Job&) = delete;
+    Job& operator=(const Job&) = delete;
+
+    int numFrames() const { return (int)frames_.size(); }
+    const FrameConfig& frame(int i) const { return frames_[i]; }
+    hipStream_t stream() const { return stream_; }
+    // Device memory this job holds.
+    size_t deviceBytes() const { return bytes_; }
+    // Device memory a job with these frames would take (before any growth).
+    static size_t estimateBytes(const std::vector<FrameConfig>& frames, const JobOptions& opt);
+
+    // Pinned host staging buffer of frame i (input.bytes()).
+    void* hostInput(int i) { return hostIn_[i]; }
+    // Copies frame i from host memory into its device input buffer (synchronously).
+    void upload(int i, const void* frame);
+    // Early termination threshold of frame i (0: code every pass).
+    void setSlope(int i, double t) { slope_[i] = (float)t; }
+
+    // Runs the whole chain. uploadInputs: copy every frame's pinned staging buffer to the
+    // device first (inside the timed chain); otherwise the frames already resident in HBM
+    // are encoded. Blocks until the results are in the job's pinned host buffers.
+    void encode(bool uploadInputs, StageTimes* times = nullptr);
+    // The same in two halves, so that host work can overlap the device: launch() enqueues
+    // the chain, finish() waits for it and fetches the results.
+    void launch(bool uploadInputs);
+    void finish(StageTimes* times = nullptr);
+
+    Tier1View result(int i) const;
+
+    // Debug taps (synchronous): X'Y'Z' codes (3 planes), the DWT output (3 Mallat planes),
+    // and the quantised coefficient words (1,024 per block) of frame i.
+    void debugCodes(int i, std::vector<uint16_t>& out);
+    void debugWavelet(int i, std::vector<float>& out);
+    void debugCoeffs(int i, std::vector<uint32_t>& out);
+    // The raw per-block symbol counts and pass tables of frame i (for tests).
+    void debugInfo(int i, std::vector<T1Info>& out);
+
+private:
+    struct FrameDev {
+        size_t inBytes = 0, npix = 0;
+        void* in = nullptr;          // device input (u16 codes for XyzPlanar)
+        uint16_t* codes = nullptr;   // 3 planes of codes (== in for XyzPlanar)
+        float* coef = nullptr;       // 3 Mallat planes
+        float* llA = nullptr;        // LL scratch, 3 x ceil(W/2) x ceil(H/2)
+        float* llB = nullptr;        // LL scratch, 3 x ceil(W/4) x ceil(H/4)
+        float* lut = nullptr;        // EOTF table (colour inputs only)
+        float matrix[12] = {};
+        uint32_t firstBlock = 0, numBlocks = 0;
+    };
+
+    void allocate();
+    void release();
+    void growT1(bool pool, bool out);
+    void enqueueChain(bool uploadInputs);
+    void enqueueT1(hipStream_t s);
+    template <class T> T* devAlloc(size_t count);
+
+    int dev_;
+    DeviceContext& ctx_;
+    JobOptions opt_;
+    std::vector<FrameConfig> frames_;
+    std::vector<FrameDev> fd_;
+    std::vector<void*> hostIn_;
+    std::vector<float> slope_;
+    std::vector<void*> allocs_;
+    size_t bytes_ = 0;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev_[8] = {};
+    hipGraphExec_t graph_ = nullptr;
+    bool graphUpload_ = false;
+    bool capturing_ = false;
+
+    uint32_t nblocks_ = 0;
+    size_t totalCoefs_ = 0;
+    // job-wide tables
+    T1Block* dBlocks_ = nullptr;
+    T1Band* dBands_ = nullptr;
+    T1FrameRef* dFrames_ = nullptr;
+    T1FrameRef* hFrames_ = nullptr;  // pinned, copied before each run
+    std::vector<std::vector<DwtLevelDesc>> levelDescs_;
+    std::vector<DwtLevelDesc*> dLevelDescs_;
+    std::vector<int> levelTiles_;
+    // Tier-1 arrays
+    T1Info* dInfo_ = nullptr;
+    uint32_t* dPassEnd_ = nullptr;
+    int32_t* dPassDist_ = nullptr;
+    uint32_t* dPassLen_ = nullptr;
+    uint2* dChunks_ = nullptr;
+    MqSnapshot* dSnaps_ = nullptr;
+    uint32_t* dPerm_ = nullptr;
+    uint32_t* dHist_ = nullptr;
+    unsigned long long* dOutOff_ = nullptr;
+    unsigned long long* dDataOff_ = nullptr;
+    unsigned long long* dPartial_ = nullptr;
+    uint32_t* dCodeLen_ = nullptr;
+    JobCounters* dCounters_ = nullptr;
+    uint32_t* dWork_ = nullptr;
+    uint8_t* dPool_ = nullptr;
+    size_t poolCap_ = 0;
+    uint8_t* dOut_ = nullptr;
+    size_t outCap_ = 0;
+    // dense results alias the pool once MQ has consumed it
+    uint32_t* dDenseInfo_ = nullptr;
+    uint32_t* dDensePassLen_ = nullptr;
+    int32_t* dDensePassDist_ = nullptr;
+    uint8_t* dDenseData_ = nullptr;
+    // pinned host results
+    uint32_t* hInfo_ = nullptr;
+    uint32_t* hPassLen_ = nullptr;
+    int32_t* hPassDist_ = nullptr;
+    uint8_t* hData_ = nullptr;
+    size_t hDataCap_ = 0, hPassCap_ = 0;
+    unsigned long long* hSmall_ = nullptr;  // counters + totals
+    size_t numPassesTotal_ = 0, numDataTotal_ = 0;
+    std::vector<size_t> frameDataBase_, framePassBase_;
+    std::vector<std::vector<uint32_t>> fixedInfo_;
+};
+
+// Throws with the HIP error text.
+void hipCheck(hipError_t e, const char* what);
+
+}  // namespace gail_amd

@@ -1,0 +1,279 @@
+// Colour conversion and the irreversible 9/7 DWT for CDNA4 (gfx950, wave64).
+//
+// DWT: one kernel launch per decomposition level covers every frame of a job. A
+// workgroup owns a TX x TY tile of the level's LL region and all three components: it
+// stages the tile plus a 4-sample mirrored halo in LDS (coalesced row loads), lifts the
+// columns in registers (one thread per column, the whole column unrolled), then the rows
+// (one thread per row), and writes the de-interleaved results back with coalesced row
+// stores: the LL quarter to the next level's scratch plane, HL/LH/HH straight into the
+// frame's Mallat-layout coefficient planes. The order of operations on every sample is the
+// CPU reference's (columns, then rows; lifting steps alpha..delta; low * (1/K), high * K),
+// and the kernels are built with -ffp-contract=off, so results are bit-identical to it.
+// Level 0 reads the frame's 12-bit X'Y'Z' codes and applies the DC shift and forward ICT
+// on load, so the ICT planes never exist in HBM.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_types.h"
+#include "kernels.h"
+
+namespace gail_amd {
+
+namespace {
+
+// ISO/IEC 15444-1 Table F.4, rounded to float exactly as the CPU reference rounds them.
+constexpr float kAlpha = (float)-1.586134342059924;
+constexpr float kBeta = (float)-0.052980118572961;
+constexpr float kGamma = (float)0.882911075530934;
+constexpr float kDelta = (float)0.443506852043971;
+constexpr float kLowScale = (float)(1.0 / 1.230174104914001);
+constexpr float kHighScale = (float)1.230174104914001;
+// Forward ICT (G.3).
+constexpr float kY0[3] = {(float)0.299, (float)0.587, (float)0.114};
+constexpr float kY1[3] = {(float)-0.16875, (float)-0.33126, (float)0.5};
+constexpr float kY2[3] = {(float)0.5, (float)-0.41869, (float)-0.08131};
+
+// Whole-sample symmetric extension of index i into [0, n).
+__device__ __forceinline__ int mirror(int i, int n) {
+    if (n == 1) return 0;
+    const int period = 2 * (n - 1);
+    i %= period;
+    if (i < 0) i += period;
+    return i < n ? i : period - i;
+}
+
+// Lifting of N samples held in registers, sample k at parity k & 1 (the tile origin is
+// even). The first and last samples have no neighbour on one side and are left alone;
+// the 4-sample halo keeps the damage outside the outputs.
+template <int N> __device__ __forceinline__ void lift97(float (&v)[N]) {
+#pragma unroll
+    for (int k = 1; k + 1 < N; k += 2) v[k] += kAlpha * (v[k - 1] + v[k + 1]);
+#pragma unroll
+    for (int k = 2; k + 1 < N; k += 2) v[k] += kBeta * (v[k - 1] + v[k + 1]);
+#pragma unroll
+    for (int k = 1; k + 1 < N; k += 2) v[k] += kGamma * (v[k - 1] + v[k + 1]);
+#pragma unroll
+    for (int k = 2; k + 1 < N; k += 2) v[k] += kDelta * (v[k - 1] + v[k + 1]);
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] *= (k & 1) ? kHighScale : kLowScale;
+}
+
+__device__ __forceinline__ int find_desc(const DwtLevelDesc* d, int n, int tile) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((int)d[mid].tile_base <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <bool kLevel0, int TX, int TY>
+__global__ void __launch_bounds__(256) dwt_level_kernel(const DwtLevelDesc* __restrict__ descs, int ndesc) {
+    constexpr int RX = TX + 8, RY = TY + 8, PITCH = RX + 1;
+    __shared__ float tile[3][RY][PITCH];
+    const int di = find_desc(descs, ndesc, blockIdx.x);
+    const DwtLevelDesc d = descs[di];
+    const int t = blockIdx.x - d.tile_base;
+    const int x0 = (t % d.tiles_x) * TX, y0 = (t / d.tiles_x) * TY;
+    const int w = d.w, h = d.h;
+    const int tid = threadIdx.x;
+
+    // Stage the tile and its mirrored halo.
+    for (int i = tid; i < RY * RX; i += 256) {
+        const int ry = i / RX, rx = i - ry * RX;
+        const int gy = mirror(y0 - 4 + ry, h), gx = mirror(x0 - 4 + rx, w);
+        const size_t at = (size_t)gy * d.src_stride + gx;
+        if constexpr (kLevel0) {
+            const uint16_t* s = (const uint16_t*)d.src;
+            const float x = (float)((int)s[at] - 2048);
+            const float y = (float)((int)s[d.src_plane + at] - 2048);
+            const float z = (float)((int)s[2 * (size_t)d.src_plane + at] - 2048);
+            tile[0][ry][rx] = kY0[0] * x + kY0[1] * y + kY0[2] * z;
+            tile[1][ry][rx] = kY1[0] * x + kY1[1] * y + kY1[2] * z;
+            tile[2][ry][rx] = kY2[0] * x + kY2[1] * y + kY2[2] * z;
+        } else {
+            const float* s = (const float*)d.src;
+            tile[0][ry][rx] = s[at];
+            tile[1][ry][rx] = s[d.src_plane + at];
+            tile[2][ry][rx] = s[2 * (size_t)d.src_plane + at];
+        }
+    }
+    __syncthreads();
+    // Columns (vertical lifting), one thread per column of each component.
+    if (h >= 2) {
+        for (int i = tid; i < 3 * RX; i += 256) {
+            const int c = i / RX, rx = i - c * RX;
+            float v[RY];
+#pragma unroll
+            for (int k = 0; k < RY; ++k) v[k] = tile[c][k][rx];
+            lift97(v);
+#pragma unroll
+            for (int k = 4; k < TY + 4; ++k) tile[c][k][rx] = v[k];
+        }
+        __syncthreads();
+    }
+    // Rows (horizontal lifting), one thread per output row of each component.
+    if (w >= 2) {
+        for (int i = tid; i < 3 * TY; i += 256) {
+            const int c = i / TY, ry = 4 + (i - c * TY);
+            float v[RX];
+#pragma unroll
+            for (int k = 0; k < RX; ++k) v[k] = tile[c][ry][k];
+            lift97(v);
+#pragma unroll
+            for (int k = 4; k < TX + 4; ++k) tile[c][ry][k] = v[k];
+        }
+        __syncthreads();
+    }
+    // De-interleaved stores: per tile row, the even samples (low) and the odd samples
+    // (high) each form a contiguous run of TX / 2 outputs.
+    const int snx = (w + 1) >> 1, sny = (h + 1) >> 1;
+    constexpr int HALF = TX / 2;
+    for (int i = tid; i < 3 * TY * TX; i += 256) {
+        const int c = i / (TY * TX);
+        const int r = (i / TX) % TY;
+        const int seg = (i / HALF) & 1, k = i % HALF;
+        const int gy = y0 + r, gx = x0 + 2 * k + seg;
+        if (gy >= h || gx >= w) continue;
+        const float v = tile[c][4 + r][4 + 2 * k + seg];
+        const int oy = (gy & 1) ? sny + (gy >> 1) : (gy >> 1);
+        const int ox = seg ? snx + (gx >> 1) : (gx >> 1);
+        if (!seg && !(gy & 1)) d.ll_dst[(size_t)c * d.ll_plane + (size_t)oy * d.ll_stride + ox] = v;
+        else d.coef[(size_t)c * d.coef_plane + (size_t)oy * d.coef_stride + ox] = v;
+    }
+}
+
+// ---------------------------------------------------------------- colour
+// Source R'G'B' sample (0..1 after range handling) to the EOTF table index, as the CPU
+// reference forms it: rounded to a 16-bit code (color.cpp:136-139), then code / 65535.
+__device__ __forceinline__ int lut_index16(float v) {
+    v = v < 0.f ? 0.f : (v > 1.f ? 1.f : v);
+    const float s = v * 65535.f;
+    const int r = (int)s;
+    const int code = r + ((s - (float)r) >= 0.5f ? 1 : 0);
+    const float e = (float)code / 65535.f;
+    const float q = (e < 0.f ? 0.f : (e > 1.f ? 1.f : e)) * 65535.0f + 0.5f;
+    return (int)floorf(q);
+}
+
+__device__ __forceinline__ int lut_index_code(uint32_t code) {
+    const float e = (float)code / 65535.f;
+    return (int)floorf((e < 0.f ? 0.f : (e > 1.f ? 1.f : e)) * 65535.0f + 0.5f);
+}
+
+// cd/m^2 -> 12-bit code: the number of code thresholds at or below v. The thresholds are
+// where the host's floor(4095 * (v / 52.37)^(1 / 2.6) + 0.5) steps up, which is monotone
+// in v, so this reproduces the CPU reference's codes exactly.
+__device__ __forceinline__ uint32_t encode_code(const float* __restrict__ thr, float v) {
+    uint32_t lo = 0;
+#pragma unroll
+    for (uint32_t step = 2048; step; step >>= 1)
+        if (lo + step <= 4095u && thr[lo + step] <= v) lo += step;
+    return lo;
+}
+
+__device__ __forceinline__ float sample(const void* src, int bits, size_t i) {
+    return bits > 8 ? (float)((const uint16_t*)src)[i] : (float)((const uint8_t*)src)[i];
+}
+
+}  // namespace
+
+// Any input format -> planar 12-bit X'Y'Z' codes (one thread per container pixel).
+__global__ void __launch_bounds__(256) color_kernel(ColorDesc d) {
+    __shared__ float thr[4096];
+    for (int i = threadIdx.x; i < 4096; i += 256) thr[i] = d.thresholds[i];
+    __syncthreads();
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t npix = (uint32_t)d.width * d.height;
+    if (i >= npix) return;
+    const int x = (int)(i % (uint32_t)d.width), y = (int)(i / (uint32_t)d.width);
+    int idx[3] = {0, 0, 0};
+    if (d.kind == 0) {  // interleaved 16-bit R'G'B'
+        const uint16_t* s = (const uint16_t*)d.src + 3 * (size_t)i;
+        for (int c = 0; c < 3; ++c) idx[c] = lut_index_code(s[c]);
+    } else {  // planar Y'CbCr or G'B'R' of the active picture, black around it
+        const int ax = x - d.ox, ay = y - d.oy;
+        float f[3] = {0.f, 0.f, 0.f};
+        if (ax >= 0 && ay >= 0 && ax < d.aw && ay < d.ah) {
+            const int cw = (d.aw + d.sx) >> d.sx, ch = (d.ah + d.sy) >> d.sy;
+            const size_t uoff = (size_t)d.aw * d.ah, voff = uoff + (size_t)cw * ch;
+            float cyp = !d.sy ? (float)ay : d.chroma == 2 ? ay * 0.5f : (ay - 0.5f) * 0.5f;
+            cyp = cyp < 0.f ? 0.f : (cyp > (float)(ch - 1) ? (float)(ch - 1) : cyp);
+            float cxp = !d.sx ? (float)ax : d.chroma == 1 ? (ax - 0.5f) * 0.5f : ax * 0.5f;
+            cxp = cxp < 0.f ? 0.f : (cxp > (float)(cw - 1) ? (float)(cw - 1) : cxp);
+            const int yy0 = (int)cyp, yy1 = min(yy0 + 1, ch - 1), xx0 = (int)cxp, xx1 = min(xx0 + 1, cw - 1);
+            const float fy = cyp - yy0, fx = cxp - xx0;
+            float uv[2];
+            for (int k = 0; k < 2; ++k) {
+                const size_t off = k ? voff : uoff;
+                const float a = sample(d.src, d.bits, off + (size_t)yy0 * cw + xx0);
+                const float b = sample(d.src, d.bits, off + (size_t)yy0 * cw + xx1);
+                const float c = sample(d.src, d.bits, off + (size_t)yy1 * cw + xx0);
+                const float e = sample(d.src, d.bits, off + (size_t)yy1 * cw + xx1);
+                uv[k] = (a * (1 - fx) + b * fx) * (1 - fy) + (c * (1 - fx) + e * fx) * fy;
+            }
+            float Y = sample(d.src, d.bits, (size_t)ay * d.aw + ax), U = uv[0], V = uv[1];
+            const float maxv = (float)((1 << d.bits) - 1), scale = (float)(1 << (d.bits - 8));
+            if (d.rgb) {
+                f[0] = V / maxv;
+                f[1] = Y / maxv;
+                f[2] = U / maxv;
+            } else {
+                if (d.full_range) {
+                    Y /= maxv;
+                    U = (U - 128.f * scale) / maxv;
+                    V = (V - 128.f * scale) / maxv;
+                } else {
+                    Y = (Y - 16.f * scale) / (219.f * scale);
+                    U = (U - 128.f * scale) / (224.f * scale);
+                    V = (V - 128.f * scale) / (224.f * scale);
+                }
+                f[0] = Y + d.crR * V;
+                f[1] = Y - d.cbG * U - d.crG * V;
+                f[2] = Y + d.cbB * U;
+            }
+        }
+        for (int c = 0; c < 3; ++c) idx[c] = lut_index16(f[c]);
+    }
+    const float R = d.lut[idx[0]], G = d.lut[idx[1]], B = d.lut[idx[2]];
+    const float* m = d.matrix;
+    const float X = m[0] * R + m[1] * G + m[2] * B;
+    const float Yv = m[4] * R + m[5] * G + m[6] * B;
+    const float Z = m[8] * R + m[9] * G + m[10] * B;
+    d.codes[i] = (uint16_t)encode_code(thr, X);
+    d.codes[npix + i] = (uint16_t)encode_code(thr, Yv);
+    d.codes[2 * (size_t)npix + i] = (uint16_t)encode_code(thr, Z);
+}
+
+// ---------------------------------------------------------------- launchers
+template <bool L0, int TX, int TY> static hipError_t launch_dwt(const DwtLevelDesc* descs, int n, int tiles, hipStream_t s) {
+    hipLaunchKernelGGL((dwt_level_kernel<L0, TX, TY>), dim3(tiles), dim3(256), 0, s, descs, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_dwt_level(bool level0, int tile_shape, const DwtLevelDesc* descs, int ndesc, int tiles,
+                            hipStream_t s) {
+    switch (tile_shape) {
+    case 0: return level0 ? launch_dwt<true, 64, 64>(descs, ndesc, tiles, s) : launch_dwt<false, 64, 64>(descs, ndesc, tiles, s);
+    case 1: return level0 ? launch_dwt<true, 64, 32>(descs, ndesc, tiles, s) : launch_dwt<false, 64, 32>(descs, ndesc, tiles, s);
+    case 2: return level0 ? launch_dwt<true, 32, 64>(descs, ndesc, tiles, s) : launch_dwt<false, 32, 64>(descs, ndesc, tiles, s);
+    default: return level0 ? launch_dwt<true, 32, 32>(descs, ndesc, tiles, s) : launch_dwt<false, 32, 32>(descs, ndesc, tiles, s);
+    }
+}
+
+void dwt_tile_size(int tile_shape, int& tx, int& ty) {
+    static const int shapes[4][2] = {{64, 64}, {64, 32}, {32, 64}, {32, 32}};
+    const int k = tile_shape >= 0 && tile_shape < 4 ? tile_shape : 3;
+    tx = shapes[k][0];
+    ty = shapes[k][1];
+}
+
+hipError_t launch_color(const ColorDesc& d, hipStream_t s) {
+    const uint32_t npix = (uint32_t)d.width * d.height;
+    hipLaunchKernelGGL(color_kernel, dim3((npix + 255) / 256), dim3(256), 0, s, d);
+    return hipGetLastError();
+}
+
+}  // namespace gail_amd

@@ -1,0 +1,931 @@
+// EBCOT Tier-1 for CDNA4 (gfx950, wave64): fused dead-zone quantisation + context
+// modelling, block ordering, MQ coding with exact truncation lengths, and compaction.
+//
+// Context modelling: one wave codes two 32x32 code-blocks, lanes 0-31 and 32-63; lane x
+// owns column x and keeps its column's state as 32-bit row masks (significant, refined,
+// visited, current magnitude bit) plus its 32 quantised magnitudes in registers. Left and
+// right neighbours come from lane shuffles, above and below are bit shifts. Per 4-row
+// stripe the significance-propagation pass is solved as a monotone fixpoint over the
+// half-wave (no change anywhere => the sequential scan's result), cleanup needs no
+// fixpoint because every remaining sample is coded. Each lane packs its column's
+// symbols of a stripe (at most 10, 6 bits each) in a 64-bit register; ballots + mbcnt
+// give the exclusive prefix over the columns, so symbols land in scan order in a per-block
+// LDS staging buffer, copied to an HBM pool once per bit-plane.
+//
+// MQ coding: one lane per code-block, blocks ordered largest first so a wave's lanes
+// finish together. The coder keeps A, C, CT and the pending byte in registers and the 19
+// context states in LDS; at each pass end it snapshots its state, and after the final
+// flush it derives every pass's truncation length from the snapshots.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_types.h"
+#include "kernels.h"
+
+namespace gail_amd {
+
+namespace {
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ uint32_t bitv(uint32_t m, int r) { return (m >> r) & 1u; }
+// Bit r-1 and r+1 of m, zero outside 0..31.
+__device__ __forceinline__ uint32_t above(uint32_t m, int r) { return r > 0 ? (m >> (r - 1)) & 1u : 0u; }
+__device__ __forceinline__ uint32_t below(uint32_t m, int r) { return r < 31 ? (m >> (r + 1)) & 1u : 0u; }
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Value of the lane one column to the left / right within the same half-wave; zero at
+// the block's edges.
+__device__ __forceinline__ uint32_t from_left(uint32_t v, int x) {
+    const uint32_t r = (uint32_t)__shfl_up((int)v, 1);
+    return x == 0 ? 0u : r;
+}
+__device__ __forceinline__ uint32_t from_right(uint32_t v, int x) {
+    const uint32_t r = (uint32_t)__shfl_down((int)v, 1);
+    return x == 31 ? 0u : r;
+}
+
+// Any set bit among the 8 neighbours of each row, from the column's own mask S and the
+// left / right columns' masks.
+__device__ __forceinline__ uint32_t neighbour_any(uint32_t S, uint32_t L, uint32_t R) {
+    return (S << 1) | (S >> 1) | L | (L << 1) | (L >> 1) | R | (R << 1) | (R >> 1);
+}
+
+__device__ __forceinline__ int half_sum_i(int v) {
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+    return v;
+}
+__device__ __forceinline__ uint32_t half_max_u(uint32_t v) {
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) {
+        const uint32_t t = (uint32_t)__shfl_xor((int)v, o, 32);
+        v = t > v ? t : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Table D.1 for the four orientations, indexed [orient][h * 15 + v * 5 + d].
+__constant__ uint8_t kZcTable[4 * 45];
+// Table D.3: context | xor << 8, indexed [(hc + 1) * 3 + (vc + 1)].
+__constant__ uint16_t kScTable[9] = {13 | 256, 12 | 256, 11 | 256, 10 | 256, 9, 10, 11, 12, 13};
+
+struct Emitter {
+    unsigned long long acc;
+    uint32_t n;
+    __device__ __forceinline__ void put(uint32_t cx, uint32_t d) {
+        acc |= (unsigned long long)((cx << 1) | d) << (6 * n);
+        ++n;
+    }
+};
+
+// Sign coding of a sample that just became significant; contributions come from the
+// neighbours' significance at the time of coding (CL, CR, CU, CD) and their signs.
+__device__ __forceinline__ void code_sign(Emitter& em, int r, uint32_t CL, uint32_t CR, uint32_t CU, uint32_t CD,
+                                          uint32_t sL, uint32_t sR, uint32_t sC) {
+    int hc = (bitv(CL, r) ? (bitv(sL, r) ? -1 : 1) : 0) + (bitv(CR, r) ? (bitv(sR, r) ? -1 : 1) : 0);
+    int vc = (above(CU, r) ? (above(sC, r) ? -1 : 1) : 0) + (below(CD, r) ? (below(sC, r) ? -1 : 1) : 0);
+    hc = hc < -1 ? -1 : (hc > 1 ? 1 : hc);
+    vc = vc < -1 ? -1 : (vc > 1 ? 1 : vc);
+    const uint32_t sc = kScTable[(hc + 1) * 3 + vc + 1];
+    em.put(sc & 0xFFu, bitv(sC, r) ^ (sc >> 8));
+}
+
+__device__ __forceinline__ uint32_t zc_context(const uint8_t* zc, int r, uint32_t CL, uint32_t CR, uint32_t CU,
+                                               uint32_t CD) {
+    const uint32_t h = bitv(CL, r) + bitv(CR, r);
+    const uint32_t v = above(CU, r) + below(CD, r);
+    const uint32_t d = above(CL, r) + below(CL, r) + above(CR, r) + below(CR, r);
+    return zc[h * 15 + v * 5 + d];
+}
+
+constexpr int kChunkBytes = 65536;  // symbol pool grabbed per wave at a time
+
+}  // namespace
+
+// ---------------------------------------------------------------- counters
+__global__ void clear_counters_kernel(JobCounters* c, uint32_t* work, int nwork) {
+    const int t = threadIdx.x;
+    if (t < (int)(sizeof(JobCounters) / 4)) ((uint32_t*)c)[t] = 0;
+    if (t < nwork) work[t] = 0;
+}
+
+hipError_t launch_clear_counters(JobCounters* c, uint32_t* work, int nwork, hipStream_t s) {
+    hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, s, c, work, nwork);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- debug quantiser
+// The coefficient words the context-modelling kernel quantises for itself, materialised
+// (1,024 per block, row-major, zero-padded) for the debug taps.
+__global__ void __launch_bounds__(256) quant_debug_kernel(const T1Block* blocks, const T1Band* bands,
+                                                          const T1FrameRef* frames, uint32_t n, uint32_t* out) {
+    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const uint32_t cb = (uint32_t)(gid >> 10);
+    if (cb >= n) return;
+    const int k = (int)(gid & 1023), r = k >> 5, x = k & 31;
+    const T1Block b = blocks[cb];
+    uint32_t word = 0;
+    if (x < b.w && r < b.h) {
+        const T1Band band = bands[b.band];
+        const T1FrameRef fr = frames[b.frame];
+        const float v = fr.coef[b.coef_off + (size_t)r * fr.stride + x];
+        const float a = (v < 0.f ? -v : v) * band.inv_delta * 64.0f;
+        const uint32_t q = a >= (float)band.max_mag ? band.max_mag : (uint32_t)a;
+        word = q | ((v < 0.f && q >= 64u) ? 0x80000000u : 0u);
+    }
+    out[gid] = word;
+}
+
+hipError_t launch_quant_debug(const T1Block* blocks, const T1Band* bands, const T1FrameRef* frames, uint32_t n,
+                              uint32_t* out, hipStream_t s) {
+    hipLaunchKernelGGL(quant_debug_kernel, dim3((unsigned)(((size_t)n * 1024 + 255) / 256)), dim3(256), 0, s, blocks,
+                       bands, frames, n, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- context modelling
+template <int WAVES>
+__global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
+    __shared__ __attribute__((aligned(16))) uint8_t s_sym[WAVES][2][kPlaneSymCap];
+    __shared__ uint32_t s_hist[WAVES][2][20];
+    __shared__ uint8_t s_zc[4 * 45];
+    for (int i = threadIdx.x; i < 4 * 45; i += WAVES * 64) s_zc[i] = kZcTable[i];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, half = lane >> 5, x = lane & 31;
+    uint8_t* const sym = s_sym[wv][half];
+    uint32_t* const hist = s_hist[wv][half];
+    unsigned long long chunk_cur = 0, chunk_end = 0;  // wave-uniform sub-allocator
+
+    for (;;) {
+        uint32_t pair = 0;
+        if (lane == 0) pair = atomicAdd(P.work_next, 1u);
+        pair = __shfl((int)pair, 0);
+        if (2 * pair >= P.nblocks) break;
+        const uint32_t cb = 2 * pair + half;
+        const bool has = cb < P.nblocks;
+        T1Block blk{};
+        if (has) blk = P.blocks[cb];
+        const int w = has ? blk.w : 0, h = has ? blk.h : 0;
+        const T1Band band = P.bands[has ? blk.band : 0];
+        const T1FrameRef fr = P.frames[has ? blk.frame : 0];
+        const uint8_t* zc = s_zc + 45 * (has ? blk.orient : 0);
+
+        // Fused dead-zone quantisation of the block's column (quant.h:20-25).
+        uint32_t m[32];
+        uint32_t sgn = 0, colmax = 0;
+        {
+            const float* src = fr.coef + blk.coef_off + x;
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                uint32_t q = 0;
+                if (x < w && r < h) {
+                    const float v = src[(size_t)r * fr.stride];
+                    const float a = (v < 0.f ? -v : v) * band.inv_delta * 64.0f;
+                    q = a >= (float)band.max_mag ? band.max_mag : (uint32_t)a;
+                    sgn |= (v < 0.f && q >= 64u ? 1u : 0u) << r;
+                }
+                m[r] = q;
+                colmax = q > colmax ? q : colmax;
+            }
+        }
+        const uint32_t bmax = half_max_u(colmax) >> 6;
+        const int nbps = bmax ? 32 - __clz((int)bmax) : 0;
+        const int nbps_lo = __shfl(nbps, 0), nbps_hi = __shfl(nbps, 32);
+        const int jmax = nbps_lo > nbps_hi ? nbps_lo : nbps_hi;
+        const int h_lo = __shfl(h, 0), h_hi = __shfl(h, 32);
+        const int nstripes = ((h_lo > h_hi ? h_lo : h_hi) + 3) >> 2;
+        const uint32_t sgnL = from_left(sgn, x), sgnR = from_right(sgn, x);
+        const uint32_t rowsValid = h >= 32 ? 0xFFFFFFFFu : ((1u << h) - 1u);
+        const float thr = fr.slope_thresh;
+        const bool track = thr > 0.f;
+        if (x < 20) hist[x] = 0;
+
+        // Bit-plane masks of the column: pl[k] holds bit (p + k) of every row's magnitude
+        // (bits p..p+6: the current magnitude bit is pl[6], the six below it feed the
+        // distortion estimates). Moving to the next plane shifts the register down by one
+        // and extracts a single new mask.
+        uint32_t pl[7];
+        {
+            const int p0 = nbps > 0 ? nbps - 1 : 0;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                uint32_t mk = 0;
+#pragma unroll
+                for (int r = 0; r < 32; ++r) mk |= ((m[r] >> (p0 + k)) & 1u) << r;
+                pl[k] = mk;
+            }
+        }
+        // Sum of the 7-bit windows (bits p..p+6) of the rows in mask s.
+        auto window_sum = [&](uint32_t s) {
+            int t = 0;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) t += __popc(pl[k] & s) << k;
+            return t;
+        };
+
+        uint32_t S = 0, R = 0;  // significant, refined before
+        uint32_t np = 0, sym_total = 0, planes = 0;
+        bool done = !has || nbps == 0 || nbps > kMaxPlanes;
+        for (int j = 0; j < jmax; ++j) {
+            const bool act = !done && j < nbps;
+            const int p = nbps - 1 - j;  // current bit-plane (valid when act)
+            if (j > 0) {
+#pragma unroll
+                for (int k = 6; k > 0; --k) pl[k] = pl[k - 1];
+                const int sh = p > 0 ? p : 0;
+                uint32_t mk = 0;
+#pragma unroll
+                for (int r = 0; r < 32; ++r) mk |= ((m[r] >> sh) & 1u) << r;
+                pl[0] = act ? mk : 0u;
+            }
+            const uint32_t Bp = pl[6];
+            const uint32_t sig0 = S;
+            uint32_t V = 0, plane_count = 0;
+            int plane_dist = 0;
+            for (int type = (j == 0 ? 2 : 0); type < 3; ++type) {
+                const bool pact = act && np < P.max_passes;
+                const uint32_t colMask = (pact && x < w) ? rowsValid : 0u;
+                uint32_t new_sig = 0;  // samples that became significant in this pass
+                int dist = 0;
+                if (type == 1) {
+                    // ---- magnitude refinement (D.3.3): samples significant before this plane
+                    const uint32_t nAny = neighbour_any(S, from_left(S, x), from_right(S, x));
+                    const uint32_t coded_all = sig0 & colMask;
+                    for (int s = 0; s < nstripes; ++s) {
+                        const int r0 = 4 * s;
+                        const uint32_t coded = coded_all & (0xFu << r0);
+                        Emitter em{0, 0};
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const int r = r0 + k;
+                            if (!bitv(coded, r)) continue;
+                            em.put(bitv(R, r) ? 16u : (bitv(nAny, r) ? 15u : 14u), bitv(Bp, r));
+                        }
+                        const unsigned long long b0 = __ballot(em.n & 1u), b1 = __ballot(em.n & 2u);
+                        const unsigned long long b2 = __ballot(em.n & 4u);
+                        const uint32_t lot = __popcll(b0 & 0xFFFFFFFFull) + 2 * __popcll(b1 & 0xFFFFFFFFull) +
+                                             4 * __popcll(b2 & 0xFFFFFFFFull);
+                        const uint32_t all = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
+                        uint32_t excl = lanes_below(b0) + 2 * lanes_below(b1) + 4 * lanes_below(b2);
+                        if (half) excl -= lot;
+                        for (uint32_t k = 0; k < em.n; ++k) {
+                            const uint32_t sv = (uint32_t)(em.acc >> (6 * k)) & 63u;
+                            sym[plane_count + excl + k] = (uint8_t)sv;
+                            if (track) atomicAdd(&hist[sv >> 1], (sv & 1u) ? 0x10000u : 1u);
+                        }
+                        plane_count += half ? all - lot : lot;
+                    }
+                    // t1_dist_ref summed: 64 w - 5120 where bit p+6 is set, 3072 - 64 w elsewhere
+                    const uint32_t s1 = coded_all & Bp, s0 = coded_all & ~Bp;
+                    dist = 64 * window_sum(s1) - 5120 * __popc(s1) + 3072 * __popc(s0) - 64 * window_sum(s0);
+                    R |= coded_all;
+                } else {
+                    for (int s = 0; s < nstripes; ++s) {
+                        const int r0 = 4 * s;
+                        const uint32_t SM = (0xFu << r0) & colMask;
+                        const uint32_t SL = from_left(S, x), SR = from_right(S, x);
+                        Emitter em{0, 0};
+                        if (type == 0) {
+                            // ---- significance propagation (D.3.1)
+                            const uint32_t cand = ~S & SM;
+                            const uint32_t nS = neighbour_any(S, SL, SR);
+                            if (__any((cand & nS) != 0)) {
+                                // least fixpoint of "eligible given the news to the left and above"
+                                uint32_t NEW = 0, E = 0;
+                                for (;;) {
+                                    const uint32_t NL = from_left(NEW, x);
+                                    uint32_t e = cand & (nS | NL | (NL << 1) | (NL >> 1));
+                                    uint32_t nn = e & Bp;
+                                    e |= cand & (nn << 1);
+                                    nn = e & Bp;
+                                    e |= cand & (nn << 1);
+                                    nn = e & Bp;
+                                    e |= cand & (nn << 1);
+                                    nn = e & Bp;
+                                    const bool changed = nn != NEW;
+                                    NEW = nn;
+                                    E = e;
+                                    if (!__any(changed)) break;
+                                }
+                                const uint32_t NL = from_left(NEW, x);
+                                const uint32_t CL = SL | NL, CR = SR, CU = S | NEW, CD = S;
+#pragma unroll
+                                for (int k = 0; k < 4; ++k) {
+                                    const int r = r0 + k;
+                                    if (!bitv(E, r)) continue;
+                                    const uint32_t b = bitv(Bp, r);
+                                    em.put(zc_context(zc, r, CL, CR, CU, CD), b);
+                                    if (b) code_sign(em, r, CL, CR, CU, CD, sgnL, sgnR, sgn);
+                                }
+                                S |= NEW;
+                                V |= E;
+                                new_sig |= NEW;
+                            }
+                        } else {
+                            // ---- cleanup with run-length mode (D.3.4)
+                            const uint32_t Cc = SM & ~S & ~V;
+                            if (__any(Cc != 0)) {
+                                const uint32_t NEWc = Cc & Bp;
+                                const uint32_t NLc = from_left(NEWc, x);
+                                const uint32_t CL = SL | NLc, CR = SR, CU = S | NEWc, CD = S;
+                                bool run = ((Cc >> r0) & 0xFu) == 0xFu && r0 + 3 < h;
+                                if (run) {
+                                    // no significant neighbour in the 3 x 6 window around the column's stripe
+                                    const uint32_t win = (r0 == 0 ? 0x1Fu : (0x3Fu << (r0 - 1)));
+                                    run = ((CL | CR) & win) == 0 && above(S, r0) == 0 && below(S, r0 + 3) == 0;
+                                }
+                                int first = 4;
+                                if (run) {
+                                    const uint32_t nb = (NEWc >> r0) & 0xFu;
+                                    if (nb) first = __builtin_ctz(nb);
+                                    em.put(17u, nb ? 1u : 0u);
+                                    if (nb) {
+                                        em.put(18u, (uint32_t)first >> 1);
+                                        em.put(18u, (uint32_t)first & 1u);
+                                    }
+                                }
+#pragma unroll
+                                for (int k = 0; k < 4; ++k) {
+                                    const int r = r0 + k;
+                                    if (run && k < first) continue;
+                                    if (!bitv(Cc, r)) continue;
+                                    const uint32_t b = bitv(Bp, r);
+                                    if (!(run && k == first)) em.put(zc_context(zc, r, CL, CR, CU, CD), b);
+                                    if (b) code_sign(em, r, CL, CR, CU, CD, sgnL, sgnR, sgn);
+                                }
+                                S |= NEWc;
+                                new_sig |= NEWc;
+                            }
+                        }
+                        // the stripe's symbols, in column order, after the plane's earlier ones
+                        const unsigned long long b0 = __ballot(em.n & 1u), b1 = __ballot(em.n & 2u);
+                        const unsigned long long b2 = __ballot(em.n & 4u), b3 = __ballot(em.n & 8u);
+                        const uint32_t lot = __popcll(b0 & 0xFFFFFFFFull) + 2 * __popcll(b1 & 0xFFFFFFFFull) +
+                                             4 * __popcll(b2 & 0xFFFFFFFFull) + 8 * __popcll(b3 & 0xFFFFFFFFull);
+                        const uint32_t all = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2) + 8 * __popcll(b3);
+                        uint32_t excl = lanes_below(b0) + 2 * lanes_below(b1) + 4 * lanes_below(b2) + 8 * lanes_below(b3);
+                        if (half) excl -= lot;
+                        for (uint32_t k = 0; k < em.n; ++k) {
+                            const uint32_t sv = (uint32_t)(em.acc >> (6 * k)) & 63u;
+                            sym[plane_count + excl + k] = (uint8_t)sv;
+                            if (track) atomicAdd(&hist[sv >> 1], (sv & 1u) ? 0x10000u : 1u);
+                        }
+                        plane_count += half ? all - lot : lot;
+                    }
+                    // t1_dist_sig summed over the new significances: 192 w - 9216 each
+                    dist = 192 * window_sum(new_sig) - 9216 * __popc(new_sig);
+                }
+                const int total = half_sum_i(dist);
+                if (pact) {
+                    if (x == 0) {
+                        P.pass_end[(size_t)cb * kMaxPasses + np] = sym_total + plane_count;
+                        P.pass_dist[(size_t)cb * kMaxPasses + np] = total;
+                    }
+                    ++np;
+                    plane_dist += total;
+                }
+            }
+            // ---- the plane's symbols to the pool (wave-uniform sub-allocation)
+            wave_sync();
+            const uint32_t cnt_lo = __shfl((int)plane_count, 0), cnt_hi = __shfl((int)plane_count, 32);
+            const uint32_t al_lo = (cnt_lo + 15u) & ~15u, al_hi = (cnt_hi + 15u) & ~15u;
+            if (chunk_cur + al_lo + al_hi > chunk_end) {
+                const unsigned long long want = kChunkBytes > al_lo + al_hi ? kChunkBytes : al_lo + al_hi;
+                unsigned long long got = 0;
+                if (lane == 0) got = atomicAdd(&P.counters->sym_next, want);
+                got = __shfl((long long)got, 0);
+                chunk_cur = got;
+                chunk_end = got + want;
+                if (chunk_end > P.pool_cap && lane == 0) atomicOr(&P.counters->flags, 1u);
+            }
+            const unsigned long long off = chunk_cur + (half ? al_lo : 0u);
+            chunk_cur += al_lo + al_hi;
+            const bool fits = chunk_end <= P.pool_cap;
+            if (act && fits) {
+                for (uint32_t i = (uint32_t)x * 16u; i < plane_count; i += 512u)
+                    *(uint4*)(P.pool + off + i) = *(const uint4*)(sym + i);
+                if (x == 0) P.chunks[(size_t)cb * kMaxPlanes + j] = make_uint2((uint32_t)(off >> 4), plane_count);
+            }
+            if (act) {
+                sym_total += plane_count;
+                ++planes;
+            }
+            wave_sync();
+            // ---- early termination after plane p > 0 (t1_ref.cpp:128-162)
+            bool stop_now = false;
+            if (__any(track)) {
+                float term = 0.f;
+                if (track && x < 19) {
+                    const uint32_t hv = hist[x];
+                    const uint32_t n0 = hv & 0xFFFFu, n1 = hv >> 16, n = n0 + n1;
+                    if (n > 0 && n <= kLog2Max) {
+                        const float* row = P.log2tab + (size_t)n * (n + 1) / 2;
+                        const float a = (float)n0 * row[n0];
+                        const float b = (float)n1 * row[n1];
+                        term = -(a + b) + 2.f;
+                    }
+                }
+                wave_sync();
+                if (track && x < 20) hist[x] = 0;
+                float bits = 0.f;
+                for (int c = 0; c < 19; ++c) bits += __shfl(term, (half << 5) | c);
+                const float bytes = bits * 0.125f + 1.f;
+                bool cand = track && act && p > 0;
+                float dabs = 0.f;
+                if (cand) dabs = (float)plane_dist * ldexpf(1.0f, 2 * p) * (1.f / 4096.f) * band.dist_scale;
+                cand = cand && dabs < thr * bytes;
+                if (__any(cand)) {
+                    // remaining squared error, summed in the CPU's row-major order
+                    const int pp = act ? p : 0;
+                    const float step = ldexpf(1.0f, pp);
+                    float e = 0.f;
+                    float* scratch = (float*)sym;
+#pragma unroll
+                    for (int part = 0; part < 2; ++part) {
+#pragma unroll
+                        for (int rr = 0; rr < 16; ++rr) {
+                            const uint32_t mm = m[part * 16 + rr];
+                            const float v = (float)mm * (1.f / 64.f);
+                            const uint32_t q = mm >> (pp + 6);
+                            const float rec = q == 0 ? 0.f : ((float)q + 0.5f) * step;
+                            const float dd = v - rec;
+                            scratch[rr * 32 + x] = dd * dd;
+                        }
+                        wave_sync();
+                        if (x == 0 && cand) {
+                            const int r1 = h < part * 16 + 16 ? h : part * 16 + 16;
+                            for (int r = part * 16; r < r1; ++r)
+                                for (int c = 0; c < w; ++c) e += scratch[(r - part * 16) * 32 + c];
+                        }
+                        wave_sync();
+                    }
+                    e = __shfl(e, half << 5);
+                    stop_now = cand && e * band.dist_scale < thr * bytes;
+                }
+            }
+            if (act && (stop_now || np >= P.max_passes)) done = true;
+            if (!act) done = true;
+        }
+        if (has && x == 0) {
+            T1Info inf;
+            inf.nbps = (uint32_t)(nbps <= kMaxPlanes ? nbps : 0);
+            inf.passes = np;
+            inf.planes = planes;
+            inf.symbols = sym_total;
+            P.info[cb] = inf;
+            if (nbps > kMaxPlanes) atomicOr(&P.counters->flags, 4u);
+        }
+    }
+}
+
+static hipError_t upload_zc_table() {
+    {
+        // Table D.1 (ISO/IEC 15444-1) for LL/LH, HL (h and v swapped) and HH.
+        uint8_t t[4 * 45];
+        for (int o = 0; o < 4; ++o)
+            for (int h = 0; h < 3; ++h)
+                for (int v = 0; v < 3; ++v)
+                    for (int d = 0; d < 5; ++d) {
+                        int hh = h, vv = v, c;
+                        if (o == 1) {
+                            hh = v;
+                            vv = h;
+                        }
+                        if (o == 3) {
+                            const int hv = hh + vv;
+                            c = d >= 3 ? 8 : d == 2 ? (hv ? 7 : 6) : d == 1 ? (hv >= 2 ? 5 : hv ? 4 : 3) : (hv >= 2 ? 2 : hv);
+                        } else {
+                            c = hh == 2 ? 8 : hh == 1 ? (vv ? 7 : d ? 6 : 5) : vv ? (vv == 2 ? 4 : 3) : (d >= 2 ? 2 : d);
+                        }
+                        t[o * 45 + h * 15 + v * 5 + d] = (uint8_t)c;
+                    }
+        return hipMemcpyToSymbol(HIP_SYMBOL(kZcTable), t, sizeof t);
+    }
+}
+
+hipError_t launch_t1_cm(const CmParams& p, int waves, int groups, hipStream_t s) {
+    switch (waves) {
+    case 1: hipLaunchKernelGGL(t1_cm_kernel<1>, dim3(groups), dim3(64), 0, s, p); break;
+    case 2: hipLaunchKernelGGL(t1_cm_kernel<2>, dim3(groups), dim3(128), 0, s, p); break;
+    case 8: hipLaunchKernelGGL(t1_cm_kernel<8>, dim3(groups), dim3(512), 0, s, p); break;
+    default: hipLaunchKernelGGL(t1_cm_kernel<4>, dim3(groups), dim3(256), 0, s, p); break;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- ordering
+// Counting sort of the blocks by symbol count, largest first: 128 buckets, four per
+// octave. Kernel 1 histograms, kernel 2 turns the histogram into bucket starts, kernel 3
+// scatters (one global reservation per bucket and workgroup).
+namespace {
+constexpr int kBuckets = 128;
+constexpr int kSortSpan = 4096;  // blocks per workgroup
+__device__ __forceinline__ uint32_t bucket_of(uint32_t s) {
+    if (s < 4) return kBuckets - 1;
+    const int l = 31 - __clz((int)s);
+    const uint32_t sub = (s >> (l - 2)) & 3u;
+    const uint32_t b = (uint32_t)l * 4 + sub;
+    return kBuckets - 1 - (b < kBuckets - 1 ? b : kBuckets - 1);
+}
+}  // namespace
+
+__global__ void __launch_bounds__(256) sort_hist_kernel(const T1Info* info, uint32_t n, uint32_t* hist) {
+    __shared__ uint32_t h[kBuckets];
+    for (int i = threadIdx.x; i < kBuckets; i += 256) h[i] = 0;
+    __syncthreads();
+    const uint32_t b0 = blockIdx.x * kSortSpan, b1 = min(n, b0 + kSortSpan);
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += 256) atomicAdd(&h[bucket_of(info[i].symbols)], 1u);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kBuckets; i += 256)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+__global__ void sort_scan_kernel(uint32_t* hist) {
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int b = 0; b < kBuckets; ++b) {
+            const uint32_t c = hist[b];
+            hist[b] = run;
+            run += c;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) sort_scatter_kernel(const T1Info* info, uint32_t n, uint32_t* cursor,
+                                                           uint32_t* perm) {
+    __shared__ uint32_t h[kBuckets], base[kBuckets];
+    for (int i = threadIdx.x; i < kBuckets; i += 256) h[i] = 0;
+    __syncthreads();
+    const uint32_t b0 = blockIdx.x * kSortSpan, b1 = min(n, b0 + kSortSpan);
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += 256) atomicAdd(&h[bucket_of(info[i].symbols)], 1u);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kBuckets; i += 256) {
+        base[i] = h[i] ? atomicAdd(&cursor[i], h[i]) : 0u;
+        h[i] = 0;
+    }
+    __syncthreads();
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += 256) {
+        const uint32_t b = bucket_of(info[i].symbols);
+        perm[base[b] + atomicAdd(&h[b], 1u)] = i;
+    }
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+hipError_t launch_sort_blocks(const T1Info* info, uint32_t n, uint32_t* hist, uint32_t* perm, hipStream_t s) {
+    const uint32_t groups = (n + kSortSpan - 1) / kSortSpan;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(128), 0, s, hist, kBuckets);
+    hipLaunchKernelGGL(sort_hist_kernel, dim3(groups), dim3(256), 0, s, info, n, hist);
+    hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(64), 0, s, hist);
+    hipLaunchKernelGGL(sort_scatter_kernel, dim3(groups), dim3(256), 0, s, info, n, hist, perm);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- exclusive scans
+// Three-phase exclusive scan over per-block values (kScanSpan blocks per workgroup):
+// per-group totals, a one-group scan of the totals, then local scans plus the group base.
+namespace {
+constexpr int kScanSpan = 2048;
+
+__device__ __forceinline__ uint32_t out_capacity(const T1Info& f) {
+    // A block's codeword never needs more than half a byte per symbol plus the flush; a
+    // block that still overflows is reported and the job re-run with more room.
+    return f.passes ? ((f.symbols / 2u + 64u + 3u) & ~3u) : 0u;
+}
+
+template <class F>
+__device__ void block_scan_u64(uint32_t n, unsigned long long* out, unsigned long long* partial, int phase, F value) {
+    __shared__ unsigned long long s[256];
+    const uint32_t b0 = blockIdx.x * kScanSpan, b1 = min(n, b0 + kScanSpan);
+    const uint32_t per = kScanSpan / 256;
+    const uint32_t i0 = b0 + threadIdx.x * per, i1 = min(b1, i0 + per);
+    unsigned long long sum = 0;
+    for (uint32_t i = i0; i < i1; ++i) sum += value(i);
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        const unsigned long long v = threadIdx.x >= (unsigned)d ? s[threadIdx.x - d] : 0ull;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    if (phase == 0) {
+        if (threadIdx.x == 255) partial[blockIdx.x] = s[255];
+        return;
+    }
+    unsigned long long run = partial[blockIdx.x] + s[threadIdx.x] - sum;
+    for (uint32_t i = i0; i < i1; ++i) {
+        out[i] = run;
+        run += value(i);
+    }
+}
+}  // namespace
+
+__global__ void __launch_bounds__(256) out_offsets_kernel(const T1Info* info, uint32_t n, unsigned long long* out,
+                                                          unsigned long long* partial, int phase) {
+    block_scan_u64(n, out, partial, phase, [&](uint32_t i) { return (unsigned long long)out_capacity(info[i]); });
+}
+
+__global__ void scan_partials_kernel(unsigned long long* partial, int groups, unsigned long long* total) {
+    if (threadIdx.x == 0) {
+        unsigned long long run = 0;
+        for (int g = 0; g < groups; ++g) {
+            const unsigned long long c = partial[g];
+            partial[g] = run;
+            run += c;
+        }
+        if (total) *total = run;
+    }
+}
+
+hipError_t launch_out_offsets(const T1Info* info, uint32_t n, unsigned long long* out_off, unsigned long long* partial,
+                              hipStream_t s) {
+    const int groups = (int)((n + kScanSpan - 1) / kScanSpan);
+    hipLaunchKernelGGL(out_offsets_kernel, dim3(groups), dim3(256), 0, s, info, n, out_off, partial, 0);
+    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(64), 0, s, partial, groups, partial + groups);
+    hipLaunchKernelGGL(out_offsets_kernel, dim3(groups), dim3(256), 0, s, info, n, out_off, partial, 1);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) data_offsets_kernel(const T1Info* info, const uint32_t* len, uint32_t n,
+                                                           unsigned long long* out, unsigned long long* partial,
+                                                           int phase) {
+    block_scan_u64(n, out, partial, phase, [&](uint32_t i) {
+        return (unsigned long long)((len[i] + 3u) & ~3u) | ((unsigned long long)info[i].passes << 36);
+    });
+}
+
+hipError_t launch_compact_offsets(const T1Info* info, const uint32_t* code_len, uint32_t n,
+                                  unsigned long long* data_off, uint32_t* pass_off, unsigned long long* partial,
+                                  hipStream_t s) {
+    (void)pass_off;
+    const int groups = (int)((n + kScanSpan - 1) / kScanSpan);
+    hipLaunchKernelGGL(data_offsets_kernel, dim3(groups), dim3(256), 0, s, info, code_len, n, data_off, partial, 0);
+    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(64), 0, s, partial, groups, partial + groups);
+    hipLaunchKernelGGL(data_offsets_kernel, dim3(groups), dim3(256), 0, s, info, code_len, n, data_off, partial, 1);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- MQ coding
+namespace {
+// Table C.2 packed as Qe | NMPS << 16 | NLPS << 22 | SWITCH << 28. A context's state word is
+// its entry of this table with its MPS in bit 31.
+__constant__ uint32_t kMqTable[47];
+
+struct Mq {
+    uint32_t A, C;
+    int CT, bp;
+    uint32_t B;
+};
+
+__device__ __forceinline__ void mq_byteout(Mq& e, uint8_t* out, uint32_t cap, bool& overflow) {
+    bool stuff = e.B == 0xFFu;
+    if (!stuff && (e.C & 0x8000000u)) {
+        e.B += 1u;
+        if (e.B == 0xFFu) {
+            e.C &= 0x7FFFFFFu;
+            stuff = true;
+        }
+    }
+    if (e.bp >= 0) {
+        if ((uint32_t)e.bp < cap) out[e.bp] = (uint8_t)e.B;
+        else overflow = true;
+    }
+    ++e.bp;
+    if (stuff) {
+        e.B = e.C >> 20;
+        e.C &= 0xFFFFFu;
+        e.CT = 7;
+    } else {
+        e.B = (e.C >> 19) & 0xFFu;
+        e.C &= 0x7FFFFu;
+        e.CT = 8;
+    }
+}
+
+__device__ __forceinline__ void mq_renorm(Mq& e, uint8_t* out, uint32_t cap, bool& overflow) {
+    int n = __clz((int)e.A) - 16;
+    while (n > 0) {
+        const int s = n < e.CT ? n : e.CT;
+        e.A <<= s;
+        e.C <<= s;
+        e.CT -= s;
+        n -= s;
+        if (e.CT == 0) mq_byteout(e, out, cap, overflow);
+    }
+}
+}  // namespace
+
+template <int GROUP>
+__global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
+    __shared__ uint32_t s_ctx[19][GROUP];
+    __shared__ uint32_t s_tab[48];
+    if (threadIdx.x < 47) s_tab[threadIdx.x] = kMqTable[threadIdx.x];
+    __syncthreads();
+    // A full symbol pool left some chunk tables unwritten: nothing to code, the host
+    // re-runs Tier-1 with a larger pool.
+    if (P.counters->flags & 1u) return;
+    const int t = threadIdx.x;
+    for (;;) {
+        // one wave-sized group of blocks at a time, in largest-first order
+        uint32_t base = 0;
+        if ((t & 63) == 0) base = atomicAdd(P.work_next, 64u);
+        base = __shfl((int)base, 0);
+        if (base >= P.nblocks) break;
+        const uint32_t gid = base + (t & 63);
+        if (gid >= P.nblocks) continue;
+        const uint32_t cb = P.perm[gid];
+        const T1Info inf = P.info[cb];
+        const uint32_t np = inf.passes;
+        if (np == 0) {
+            P.code_len[cb] = 0;
+            continue;
+        }
+        const unsigned long long obase = P.out_off[cb];
+        uint8_t* out = P.out + obase;
+        uint32_t cap = (inf.symbols / 2u + 64u + 3u) & ~3u;
+        bool overflow = obase + cap > P.out_cap;
+        if (overflow) cap = 0;
+        for (int i = 0; i < 19; ++i) s_ctx[i][t] = s_tab[0];
+        s_ctx[0][t] = s_tab[4];
+        s_ctx[17][t] = s_tab[3];
+        s_ctx[18][t] = s_tab[46];
+        Mq e{0x8000u, 0u, 12, -1, 0u};
+        const uint32_t* pend = P.pass_end + (size_t)cb * kMaxPasses;
+        MqSnapshot* snaps = P.snaps + (size_t)cb * kMaxPasses;
+        uint32_t k = 0, pos = 0, next_end = pend[0];
+        auto pass_ends = [&]() {
+            while (k < np && next_end == pos) {
+                snaps[k] = MqSnapshot{e.bp, e.B | ((uint32_t)e.CT << 16), e.C, e.A};
+                ++k;
+                next_end = k < np ? pend[k] : 0xFFFFFFFFu;
+            }
+        };
+        for (uint32_t j = 0; j < inf.planes; ++j) {
+            const uint2 ch = P.chunks[(size_t)cb * kMaxPlanes + j];
+            const uint4* src = (const uint4*)(P.pool + ((unsigned long long)ch.x << 4));
+            for (uint32_t i = 0; i < ch.y; i += 16) {
+                const uint4 v = src[i >> 4];
+                const uint32_t words[4] = {v.x, v.y, v.z, v.w};
+                const uint32_t cnt = ch.y - i < 16u ? ch.y - i : 16u;
+#pragma unroll
+                for (uint32_t b = 0; b < 16; ++b) {
+                    if (b >= cnt) break;
+                    pass_ends();
+                    const uint32_t sv = (words[b >> 2] >> (8 * (b & 3))) & 0xFFu;
+                    const uint32_t cx = sv >> 1, d = sv & 1u;
+                    const uint32_t st = s_ctx[cx][t];
+                    const uint32_t qe = st & 0xFFFFu, mps = st >> 31;
+                    const uint32_t a = e.A - qe;
+                    const bool is_mps = d == mps;
+                    const bool add_c = is_mps != (a < qe);  // conditional exchange (C.2.5 / C.2.6)
+                    e.C += add_c ? qe : 0u;
+                    e.A = add_c ? a : qe;
+                    if (!is_mps || !(a & 0x8000u)) {
+                        const uint32_t nidx = is_mps ? ((st >> 16) & 63u) : ((st >> 22) & 63u);
+                        const uint32_t nmps = is_mps ? mps : (mps ^ ((st >> 28) & 1u));
+                        s_ctx[cx][t] = s_tab[nidx] | (nmps << 31);
+                        mq_renorm(e, out, cap, overflow);
+                    }
+                    ++pos;
+                }
+            }
+        }
+        pass_ends();
+        // FLUSH (C.2.9)
+        const uint32_t tempc = e.C + e.A;
+        e.C |= 0xFFFFu;
+        if (e.C >= tempc) e.C -= 0x8000u;
+        e.C <<= e.CT;
+        mq_byteout(e, out, cap, overflow);
+        e.C <<= e.CT;
+        mq_byteout(e, out, cap, overflow);
+        uint32_t len = (uint32_t)e.bp;
+        if (e.B != 0xFFu) {
+            if ((uint32_t)e.bp < cap) out[e.bp] = (uint8_t)e.B;
+            else overflow = true;
+            len = (uint32_t)e.bp + 1;
+        }
+        if (overflow) {
+            atomicOr(&P.counters->flags, 2u);
+            P.code_len[cb] = 0;
+            continue;
+        }
+        // Truncation length of every pass but the last (mq_encoder.h:146-169): the shortest
+        // prefix, never ending in 0xFF, that with the decoder's 1-fill stays inside the
+        // interval the coder had at the pass end.
+        uint32_t* plen = P.pass_len + (size_t)cb * kMaxPasses;
+        uint32_t prev = 1;
+        for (uint32_t q = 0; q + 1 < np; ++q) {
+            const MqSnapshot sn = snaps[q];
+            const int ct = (int)(sn.b_ct >> 16);
+            const int base_sh = 43 - ct;
+            const unsigned long long low =
+                ((unsigned long long)(sn.b_ct & 0xFFFFu) << base_sh) + ((unsigned long long)sn.c << 16);
+            const unsigned long long high = low + ((unsigned long long)sn.a << 16);
+            unsigned long long prefix = 0;
+            int rel = 0;
+            uint32_t L = len;
+            for (int i = sn.bp; i < (int)len; ++i) {
+                const uint32_t byte = i >= 0 ? out[i] : 0u;
+                const int sh = base_sh - rel;
+                if (sh < 0) break;
+                prefix += (unsigned long long)byte << sh;
+                const unsigned long long tail = prefix + (1ull << sh);
+                if ((uint32_t)(i + 1) >= prev && byte != 0xFFu && tail <= high && tail > low) {
+                    L = (uint32_t)(i + 1);
+                    break;
+                }
+                rel += byte == 0xFFu ? 7 : 8;
+            }
+            plen[q] = L;
+            prev = L > prev ? L : prev;
+        }
+        plen[np - 1] = len;
+        P.code_len[cb] = len;
+    }
+}
+
+static hipError_t upload_mq_table() {
+    {
+        static const uint16_t qe[47] = {
+            0x5601, 0x3401, 0x1801, 0x0AC1, 0x0521, 0x0221, 0x5601, 0x5401, 0x4801, 0x3801, 0x3001, 0x2401,
+            0x1C01, 0x1601, 0x5601, 0x5401, 0x5101, 0x4801, 0x3801, 0x3401, 0x3001, 0x2801, 0x2401, 0x2201,
+            0x1C01, 0x1801, 0x1601, 0x1401, 0x1201, 0x1101, 0x0AC1, 0x09C1, 0x08A1, 0x0521, 0x0441, 0x02A1,
+            0x0221, 0x0141, 0x0111, 0x0085, 0x0049, 0x0025, 0x0015, 0x0009, 0x0005, 0x0001, 0x5601};
+        static const uint8_t nm[47] = {1,  2,  3,  4,  5,  38, 7,  8,  9,  10, 11, 12, 13, 29, 15, 16,
+                                       17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31, 32,
+                                       33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 44, 45, 45, 46};
+        static const uint8_t nl[47] = {1,  6,  9,  12, 29, 33, 6,  14, 14, 14, 17, 18, 20, 21, 14, 14,
+                                       15, 16, 17, 18, 19, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29,
+                                       30, 31, 32, 33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 46};
+        uint32_t tab[47];
+        for (int i = 0; i < 47; ++i)
+            tab[i] = qe[i] | ((uint32_t)nm[i] << 16) | ((uint32_t)nl[i] << 22) | ((i == 0 || i == 6 || i == 14) ? 1u << 28 : 0u);
+        return hipMemcpyToSymbol(HIP_SYMBOL(kMqTable), tab, sizeof tab);
+    }
+}
+
+hipError_t t1_tables_init() {
+    const hipError_t e = upload_zc_table();
+    return e != hipSuccess ? e : upload_mq_table();
+}
+
+hipError_t launch_t1_mq(const MqParams& p, int group, int groups, hipStream_t s) {
+    switch (group) {
+    case 64: hipLaunchKernelGGL(t1_mq_kernel<64>, dim3(groups), dim3(64), 0, s, p); break;
+    case 128: hipLaunchKernelGGL(t1_mq_kernel<128>, dim3(groups), dim3(128), 0, s, p); break;
+    default: hipLaunchKernelGGL(t1_mq_kernel<256>, dim3(groups), dim3(256), 0, s, p); break;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- compaction
+// One wave per block: info record, pass tables and the codeword into dense arrays.
+__global__ void __launch_bounds__(256) compact_kernel(CompactParams P) {
+    if (P.counters->flags & 3u) return;  // the run is repeated with larger buffers
+    const uint32_t cb = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t l = threadIdx.x & 63;
+    if (cb >= P.nblocks) return;
+    const T1Info inf = P.info[cb];
+    const uint32_t len = P.code_len[cb];
+    const unsigned long long packed = P.data_off[cb];
+    const unsigned long long doff = packed & ((1ull << 36) - 1);
+    const uint32_t poff = (uint32_t)(packed >> 36);
+    if (l == 0) {
+        uint32_t* di = P.dense_info + 5 * (size_t)cb;
+        di[0] = inf.nbps;
+        di[1] = inf.passes;
+        di[2] = (uint32_t)doff;
+        di[3] = len;
+        di[4] = poff;
+    }
+    if (l < inf.passes) {
+        P.dense_pass_len[poff + l] = P.pass_len[(size_t)cb * kMaxPasses + l];
+        P.dense_pass_dist[poff + l] = P.pass_dist[(size_t)cb * kMaxPasses + l];
+    }
+    const uint32_t words = (len + 3) >> 2;
+    const uint32_t* s = (const uint32_t*)(P.out + P.out_off[cb]);
+    uint32_t* d = (uint32_t*)(P.dense_data + doff);
+    for (uint32_t i = l; i < words; i += 64) d[i] = s[i];
+}
+
+hipError_t launch_compact(const CompactParams& p, hipStream_t s) {
+    hipLaunchKernelGGL(compact_kernel, dim3((p.nblocks + 3) / 4), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+}  // namespace gail_amd

@@ -1,0 +1,387 @@
+// The plugin table of libgail_hip.so (include/gail_hip_plugin.h): plain-C entry points over
+// the device engine, with exceptions turned into status codes and a thread-local message
+// (the pattern of the reference's cuda_plugin.cpp:48-70, 220-249).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "engine.h"
+#include "errors.h"
+#include "gail_hip_plugin.h"
+
+using namespace gail_amd;
+
+struct GailHipDevice {
+    int index = 0;
+    std::string name;
+    GailHipLog log = nullptr;
+    void* user = nullptr;
+};
+
+struct GailHipEncoder {
+    std::unique_ptr<Job> job;
+    std::vector<uint16_t> dbgCodes;
+    std::vector<float> dbgWavelet;
+    std::vector<uint32_t> dbgCoeffs;
+};
+
+struct GailHipBatch {
+    std::unique_ptr<Job> job;
+    std::vector<uint16_t> dbgCodes;
+    std::vector<float> dbgWavelet;
+    std::vector<uint32_t> dbgCoeffs;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+template <class F> int guarded(F&& f) {
+    try {
+        f();
+        return GAIL_HIP_OK;
+    } catch (const DeviceOutOfMemory& e) {
+        g_error = e.what();
+        return GAIL_HIP_DEVICE_MEMORY;
+    } catch (const std::bad_alloc&) {
+        g_error = "out of host memory";
+        return GAIL_HIP_HOST_MEMORY;
+    } catch (const std::logic_error& e) {
+        g_error = e.what();
+        return GAIL_HIP_MISUSE;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return GAIL_HIP_ERROR;
+    } catch (...) {
+        g_error = "unknown error";
+        return GAIL_HIP_ERROR;
+    }
+}
+
+// GailHipLayout -> Layout: the geometry is rebuilt from the size and levels, the subbands
+// (which carry the step sizes) are taken as they are (cuda_plugin.h:245-253).
+Layout toLayout(const GailHipLayout& p) {
+    if (!p.bands || p.numBands <= 0) throw std::logic_error("frame layout is empty");
+    Layout L = makeGeometry(p.width, p.height, p.profile == 1, p.levels, p.guardBits);
+    if ((int)L.bands.size() != p.numBands || L.numCblks() != p.numCblks)
+        throw std::logic_error("frame layout does not have the geometry of its size and levels");
+    L.baseStep = p.baseStep;
+    for (int i = 0; i < p.numBands; ++i) {
+        const GailHipSubband& s = p.bands[i];
+        Subband& b = L.bands[i];
+        if (s.comp != b.comp || s.res != b.res || s.orient != b.orient || s.w != b.w || s.h != b.h)
+            throw std::logic_error("subband does not match the frame geometry");
+        if (s.Mb < 0 || s.Mb > 22) throw std::logic_error("subband with more than 22 magnitude bit-planes");
+        b.eps = s.eps;
+        b.mu = s.mu;
+        b.Mb = s.Mb;
+        b.delta = s.delta;
+        b.weight = s.weight;
+    }
+    return L;
+}
+
+FrameConfig toFrame(const GailHipLayout& l, const GailHipEncodeSettings& s) {
+    FrameConfig f;
+    f.layout = toLayout(l);
+    const GailHipFrameFormat& in = s.input;
+    if (in.kind < 0 || in.kind > 2) throw std::logic_error("unknown input layout");
+    f.input.kind = in.kind;
+    f.input.width = in.width ? in.width : l.width;
+    f.input.height = in.height ? in.height : l.height;
+    f.input.aw = in.activeW;
+    f.input.ah = in.activeH;
+    f.input.ox = in.ox;
+    f.input.oy = in.oy;
+    f.input.sx = in.sx;
+    f.input.sy = in.sy;
+    f.input.bits = in.bits;
+    f.input.rgb = in.rgb;
+    f.input.fullRange = in.fullRange;
+    f.input.chroma = in.chroma;
+    f.input.kr = in.kr;
+    f.input.kb = in.kb;
+    if (in.kind == 1) {
+        if (in.bits < 8 || in.bits > 16) throw std::logic_error("planar input bit depth out of range");
+        if (in.activeW <= 0 || in.activeH <= 0 || in.ox < 0 || in.oy < 0 || in.ox + in.activeW > f.input.width ||
+            in.oy + in.activeH > f.input.height)
+            throw std::logic_error("planar picture does not fit the container");
+    }
+    f.color.transfer = s.color.transfer;
+    f.color.primaries = s.color.primaries;
+    f.color.sdrWhite = s.color.sdrWhite > 0 ? s.color.sdrWhite : 48.0;
+    f.color.cinemaPeak = s.color.cinemaPeak > 0 ? s.color.cinemaPeak : 48.0;
+    return f;
+}
+
+JobOptions toOptions(const GailHipBatchOptions* o) {
+    JobOptions j;
+    if (!o) return j;
+    if (o->dwtTile >= 0 && o->dwtTile <= 3) j.dwtTile = o->dwtTile;
+    if (o->cmWaves == 1 || o->cmWaves == 2 || o->cmWaves == 4 || o->cmWaves == 8) j.cmWaves = o->cmWaves;
+    if (o->mqGroup == 64 || o->mqGroup == 128 || o->mqGroup == 256) j.mqGroup = o->mqGroup;
+    j.useGraph = o->useGraph != 0;
+    if (o->poolBytesPerCoef > 0) j.poolBytesPerCoef = o->poolBytesPerCoef;
+    return j;
+}
+
+void fillTier1(const Job& job, int i, GailHipTier1* out) {
+    const Tier1View v = job.result(i);
+    out->info = reinterpret_cast<const GailHipCblkInfo*>(v.info5);
+    out->passLen = v.passLen;
+    out->passDist = v.passDist;
+    out->data = v.data;
+    out->numInfo = v.numBlocks;
+    out->numPasses = v.numPasses;
+    out->numData = v.numData;
+}
+
+void fillTimings(const StageTimes& t, GailHipFrameTimings* out) {
+    std::memset(out, 0, sizeof *out);
+    out->upload = t.upload;
+    out->color = t.color;
+    out->dwt = t.dwt;
+    out->t1cm = t.t1cm;
+    out->t1mq = t.t1mq + t.compact;
+    out->t1 = t.t1cm + t.t1mq + t.compact;
+    out->download = t.download;
+    out->gpu = t.gpu;
+    out->total = t.gpu;
+    out->codedBytes = t.bytes;
+}
+
+template <class T> void debugTaps(T* e, int i, GailHipDebug* dbg) {
+    e->job->debugCodes(i, e->dbgCodes);
+    dbg->xyz12 = e->dbgCodes.data();
+    dbg->numXyz12 = e->dbgCodes.size();
+    if (dbg->full) {
+        e->job->debugWavelet(i, e->dbgWavelet);
+        e->job->debugCoeffs(i, e->dbgCoeffs);
+        dbg->wavelet = e->dbgWavelet.data();
+        dbg->numWavelet = e->dbgWavelet.size();
+        dbg->coeffs = e->dbgCoeffs.data();
+        dbg->numCoeffs = e->dbgCoeffs.size();
+    }
+}
+
+// ---- table entries
+const char* lastError() { return g_error.c_str(); }
+
+int deviceCount() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    return n;
+}
+
+int deviceDesc(int index, GailHipDeviceDesc* out) {
+    return guarded([&] {
+        hipDeviceProp_t p;
+        hipCheck(hipGetDeviceProperties(&p, index), "hipGetDeviceProperties");
+        std::memset(out, 0, sizeof *out);
+        out->index = index;
+        out->computeUnits = p.multiProcessorCount;
+        out->memoryBytes = p.totalGlobalMem;
+        std::snprintf(out->name, sizeof out->name, "%s", p.name);
+        std::snprintf(out->arch, sizeof out->arch, "%s", p.gcnArchName);
+        std::snprintf(out->pciBusId, sizeof out->pciBusId, "%04x:%02x:%02x.0", p.pciDomainID, p.pciBusID, p.pciDeviceID);
+    });
+}
+
+int deviceUsable(int index) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, index) != hipSuccess) {
+        (void)hipGetLastError();
+        g_error = "no such device";
+        return GAIL_HIP_ERROR;
+    }
+    if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0) {
+        g_error = std::string("no code object for ") + p.gcnArchName + " (built for gfx950)";
+        return GAIL_HIP_NO_CODE;
+    }
+    return guarded([&] { DeviceContext::get(index); });
+}
+
+int deviceCreate(int index, GailHipLog log, void* user, GailHipDevice** out) {
+    return guarded([&] {
+        DeviceContext& c = DeviceContext::get(index);
+        auto d = std::make_unique<GailHipDevice>();
+        d->index = index;
+        d->name = c.name() + " (HIP " + c.arch() + ")";
+        d->log = log;
+        d->user = user;
+        if (log) log(user, 2, ("hip: device " + std::to_string(index) + " " + d->name).c_str());
+        *out = d.release();
+    });
+}
+
+void deviceDestroy(GailHipDevice* d) { delete d; }
+const char* deviceName(GailHipDevice* d) { return d->name.c_str(); }
+
+int deviceMemory(GailHipDevice* d, GailHipMemory* out) {
+    return guarded([&] {
+        size_t f = 0, t = 0;
+        hipCheck(hipSetDevice(d->index), "hipSetDevice");
+        hipCheck(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+        out->total = t;
+        out->available = f;
+        out->budgeted = 1;
+    });
+}
+
+uint64_t encoderBytes(GailHipDevice*, const GailHipLayout* l, const GailHipEncodeSettings* s) {
+    try {
+        return Job::estimateBytes({toFrame(*l, *s)}, JobOptions{});
+    } catch (...) {
+        return 0;
+    }
+}
+
+int encoderCreate(GailHipDevice* d, const GailHipLayout* l, const GailHipEncodeSettings* s, GailHipEncoder** out) {
+    return guarded([&] {
+        auto e = std::make_unique<GailHipEncoder>();
+        e->job = std::make_unique<Job>(d->index, std::vector<FrameConfig>{toFrame(*l, *s)}, JobOptions{});
+        *out = e.release();
+    });
+}
+
+void encoderDestroy(GailHipEncoder* e) { delete e; }
+void* encoderInput(GailHipEncoder* e) { return e->job->hostInput(0); }
+void encoderSetSlope(GailHipEncoder* e, double t) { e->job->setSlope(0, t); }
+
+int encoderEncode(GailHipEncoder* e, const void* frame, GailHipTier1* out, GailHipFrameTimings* timings,
+                  GailHipDebug* debug) {
+    return guarded([&] {
+        if (!frame || !out) throw std::logic_error("encode needs a frame and an output");
+        Job& j = *e->job;
+        if (frame != j.hostInput(0)) std::memcpy(j.hostInput(0), frame, j.frame(0).input.bytes());
+        StageTimes t;
+        j.encode(true, &t);
+        fillTier1(j, 0, out);
+        if (timings) fillTimings(t, timings);
+        if (debug) debugTaps(e, 0, debug);
+    });
+}
+
+uint64_t decoderBytes(GailHipDevice*, const GailHipLayout*, const void*) { return 0; }
+int decoderCreate(GailHipDevice*, const GailHipLayout*, const void*, void**) {
+    g_error = "the HIP backend does not decode";
+    return GAIL_HIP_UNSUPPORTED;
+}
+void decoderDestroy(void*) {}
+int decoderDecode(void*, const GailHipLayout*, const uint8_t*, size_t, const void*, size_t, void*, ptrdiff_t, void*) {
+    g_error = "the HIP backend does not decode";
+    return GAIL_HIP_UNSUPPORTED;
+}
+
+std::vector<FrameConfig> toFrames(const GailHipLayout* l, const GailHipEncodeSettings* s, int count) {
+    if (count <= 0 || !l || !s) throw std::logic_error("a batch needs at least one frame");
+    std::vector<FrameConfig> f;
+    for (int i = 0; i < count; ++i) f.push_back(toFrame(l[i], s[i]));
+    return f;
+}
+
+uint64_t batchBytes(GailHipDevice*, const GailHipLayout* l, const GailHipEncodeSettings* s, int count,
+                    const GailHipBatchOptions* o) {
+    try {
+        return Job::estimateBytes(toFrames(l, s, count), toOptions(o));
+    } catch (...) {
+        return 0;
+    }
+}
+
+int batchCreate(GailHipDevice* d, const GailHipLayout* l, const GailHipEncodeSettings* s, int count,
+                const GailHipBatchOptions* o, GailHipBatch** out) {
+    return guarded([&] {
+        auto b = std::make_unique<GailHipBatch>();
+        b->job = std::make_unique<Job>(d->index, toFrames(l, s, count), toOptions(o));
+        *out = b.release();
+    });
+}
+
+void batchDestroy(GailHipBatch* b) { delete b; }
+void* batchInput(GailHipBatch* b, int i) {
+    return i >= 0 && i < b->job->numFrames() ? b->job->hostInput(i) : nullptr;
+}
+int batchUpload(GailHipBatch* b, int i, const void* frame) {
+    return guarded([&] {
+        if (i < 0 || i >= b->job->numFrames() || !frame) throw std::logic_error("no such frame slot");
+        b->job->upload(i, frame);
+    });
+}
+void batchSetSlope(GailHipBatch* b, int i, double t) {
+    if (i >= 0 && i < b->job->numFrames()) b->job->setSlope(i, t);
+}
+int batchLaunch(GailHipBatch* b, int fromStaging) {
+    return guarded([&] { b->job->launch(fromStaging != 0); });
+}
+int batchWait(GailHipBatch* b, GailHipTier1* outs, GailHipFrameTimings* timings) {
+    return guarded([&] {
+        StageTimes t;
+        b->job->finish(&t);
+        if (outs)
+            for (int i = 0; i < b->job->numFrames(); ++i) fillTier1(*b->job, i, &outs[i]);
+        if (timings) fillTimings(t, timings);
+    });
+}
+int batchEncode(GailHipBatch* b, int fromStaging, GailHipTier1* outs, GailHipFrameTimings* timings) {
+    const int r = batchLaunch(b, fromStaging);
+    return r != GAIL_HIP_OK ? r : batchWait(b, outs, timings);
+}
+int batchDebug(GailHipBatch* b, int i, GailHipDebug* dbg) {
+    return guarded([&] {
+        if (i < 0 || i >= b->job->numFrames()) throw std::logic_error("no such frame slot");
+        debugTaps(b, i, dbg);
+    });
+}
+
+#define GAIL_STR2(x) #x
+#define GAIL_STR(x) GAIL_STR2(x)
+
+const GailHipPlugin g_plugin = {
+    GAIL_HIP_PLUGIN_ABI,
+    (uint32_t)sizeof(GailHipPlugin),
+    gail_hip_plugin_layout(),
+    "gail_hip",
+    "HIP " GAIL_STR(HIP_VERSION_MAJOR) "." GAIL_STR(HIP_VERSION_MINOR),
+    "gfx950",
+    lastError,
+    deviceCount,
+    deviceDesc,
+    deviceUsable,
+    deviceCreate,
+    deviceDestroy,
+    deviceName,
+    deviceMemory,
+    encoderBytes,
+    encoderCreate,
+    encoderDestroy,
+    encoderInput,
+    encoderSetSlope,
+    encoderEncode,
+    decoderBytes,
+    decoderCreate,
+    decoderDestroy,
+    decoderDecode,
+    batchBytes,
+    batchCreate,
+    batchDestroy,
+    batchInput,
+    batchUpload,
+    batchSetSlope,
+    batchEncode,
+    batchLaunch,
+    batchWait,
+    batchDebug,
+};
+
+}  // namespace
+
+extern "C" const GailHipPlugin* gail_hip_plugin_get(uint32_t abi) { return abi == GAIL_HIP_PLUGIN_ABI ? &g_plugin : nullptr; }

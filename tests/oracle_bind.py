@@ -1,0 +1,268 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so) and, where it was built, of the
+compiled reference hot path (oracle/_ref/libgailref.so). Test infrastructure: imported by
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg only."""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libgailref.so")
+
+MAX_PASSES = 64
+BLOCK_BUF = 1024 * 4 * 24 + 64
+
+
+class Band(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("comp", "res", "level", "orient", "px0", "py0", "w", "h", "ncbx",
+                                          "ncby", "cblk_base", "eps", "mu", "mb")] + [
+        ("delta", C.c_float), ("pad", C.c_float), ("weight", C.c_double)]
+
+
+class Cblk(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("band", "x0", "y0", "w", "h")]
+
+
+class T1Result(C.Structure):
+    _fields_ = [("nbps", C.c_int32), ("num_passes", C.c_int32), ("len", C.c_uint32),
+                ("pass_len", C.c_uint32 * MAX_PASSES), ("pass_dist", C.c_int32 * MAX_PASSES)]
+
+
+class CblkInfo(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("nbps", "num_passes", "data_offset", "data_len", "pass_base")]
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class _Lib:
+    """The oracle and the reference shim share one calling convention (prefix or_/ref_)."""
+
+    def __init__(self, path, prefix):
+        self.lib = C.CDLL(path)
+        self.prefix = prefix
+        L = self.lib
+        f = getattr(L, prefix + "layout" if prefix == "ref_" else "or_layout_make")
+        f.restype = C.c_int
+        f.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(Band), C.POINTER(C.c_int),
+                      C.POINTER(Cblk), C.c_int]
+        self._layout = f
+        f = getattr(L, prefix + "t1_encode")
+        f.restype = C.c_int
+        f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_void_p,
+                      C.c_uint32, C.POINTER(T1Result), C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32),
+                      C.c_void_p]
+        self._t1 = f
+        f = getattr(L, prefix + "fdwt97_2d" if prefix == "ref_" else "or_fdwt97_2d_f32")
+        f.restype = None
+        f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        self._dwt = f
+        f = getattr(L, prefix + "color_tables")
+        f.restype = None
+        f.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p]
+        self._tables = f
+        f = getattr(L, prefix + "planar_to_rgb48")
+        f.restype = None
+        f.argtypes = [C.c_void_p] + [C.c_int] * 8 + [C.c_double, C.c_double] + [C.c_int] * 4 + [C.c_void_p]
+        self._planar = f
+        f = getattr(L, prefix + "encode_frame_xyz")
+        f.restype = C.c_int64
+        f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_float, C.c_int, C.c_void_p,
+                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]
+        self._frame = f
+
+    def layout(self, W, H, fourk, base_step=1.0):
+        bands = (Band * 64)()
+        nb = C.c_int(0)
+        n = self._layout(W, H, int(fourk), base_step, bands, C.byref(nb), None, 0)
+        if n < 0:
+            raise ValueError("invalid layout")
+        cbl = (Cblk * n)()
+        self._layout(W, H, int(fourk), base_step, bands, C.byref(nb), cbl, n)
+        return list(bands[:nb.value]), cbl
+
+    def t1_encode(self, coeffs, w, h, orient, max_passes=MAX_PASSES, thr=0.0, scale=1.0, trace=False):
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.uint32)
+        out = np.zeros(BLOCK_BUF, np.uint8)
+        res = T1Result()
+        syms = np.zeros(1 << 17, np.uint8) if trace else None
+        pe = np.zeros(MAX_PASSES, np.uint32) if trace else None
+        ns = C.c_uint32(0)
+        rc = self._t1(_p(coeffs), w, h, orient, max_passes, thr, scale, _p(out), BLOCK_BUF, C.byref(res),
+                      _p(syms), 1 << 17 if trace else 0, C.byref(ns), _p(pe))
+        if rc:
+            raise RuntimeError("t1 encode failed")
+        np_ = res.num_passes
+        r = dict(nbps=res.nbps, num_passes=np_, data=bytes(out[:res.len]),
+                 pass_len=list(res.pass_len[:np_]), pass_dist=list(res.pass_dist[:np_]))
+        if trace:
+            r["symbols"] = bytes(syms[:ns.value])
+            r["pass_end"] = list(pe[:np_])
+        return r
+
+    def fdwt(self, plane, levels):
+        a = np.array(plane, dtype=np.float32, copy=True, order="C")
+        self._dwt(_p(a), a.shape[1], a.shape[0], levels)
+        return a
+
+    def color_tables(self, transfer, primaries, sdr_white=48.0, cinema_peak=48.0):
+        lut = np.zeros(65536, np.float32)
+        cp = np.zeros(48, np.float32)
+        self._tables(transfer, primaries, sdr_white, cinema_peak, _p(lut), _p(cp))
+        return lut, cp
+
+    def planar_to_rgb48(self, src, bits, aw, ah, sx, sy, rgb, full_range, chroma, kr, kb, W, H, ox, oy):
+        src = np.ascontiguousarray(src)
+        out = np.zeros((H, W, 3), np.uint16)
+        self._planar(_p(src), bits, aw, ah, sx, sy, rgb, full_range, chroma, kr, kb, W, H, ox, oy, _p(out))
+        return out
+
+    def encode_frame_xyz(self, xyz, fourk, thr=0.0, threads=8, base_step=1.0, want_planes=False):
+        xyz = np.ascontiguousarray(xyz, dtype=np.uint16)
+        _, H, W = xyz.shape
+        bands, cbl = self.layout(W, H, fourk, base_step)
+        n = len(cbl)
+        info = np.zeros((n, 5), np.uint32)
+        pl = np.zeros(n * MAX_PASSES, np.uint32)
+        pd = np.zeros(n * MAX_PASSES, np.int32)
+        cap = n * 4096 + (64 << 20)
+        data = np.zeros(cap, np.uint8)
+        coeffs = np.zeros(n * 1024, np.uint32)
+        planes = np.zeros((3, H, W), np.float32) if want_planes else None
+        total = self._frame(_p(xyz), W, H, int(fourk), base_step, thr, threads, _p(info), _p(pl), _p(pd),
+                            _p(data), cap, _p(coeffs), _p(planes))
+        if total < 0:
+            raise RuntimeError("frame encode failed")
+        npass = int(info[:, 1].sum())
+        return dict(info=info, pass_len=pl[:npass], pass_dist=pd[:npass], data=data[:total], coeffs=coeffs,
+                    planes=planes)
+
+
+class Oracle(_Lib):
+    def __init__(self, path=ORACLE_SO):
+        super().__init__(path, "or_")
+        L = self.lib
+        L.or_t1_decode.restype = C.c_int
+        L.or_t1_decode.argtypes = [C.c_void_p, C.c_uint32] + [C.c_int] * 5 + [C.c_void_p] * 3
+        L.or_fdwt97_2d_f64.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.or_idwt97_2d_f64.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.or_quantize_coeff.restype = C.c_uint32
+        L.or_quantize_coeff.argtypes = [C.c_float, C.c_float, C.c_uint32]
+        L.or_quantize_block.argtypes = [C.c_void_p] + [C.c_int] * 5 + [C.c_float, C.c_int, C.c_void_p]
+        L.or_synthesis_norm.restype = C.c_double
+        L.or_synthesis_norm.argtypes = [C.c_int, C.c_int]
+        L.or_convert_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                       C.c_void_p]
+        L.or_xyz_planar_to_ict.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        L.or_color_pixel.argtypes = [C.c_void_p, C.c_void_p, C.c_uint16, C.c_uint16, C.c_uint16, C.c_void_p]
+        L.or_xyz12_to_16.restype = C.c_uint16
+        L.or_xyz12_to_16.argtypes = [C.c_uint32]
+
+    def t1_decode(self, data, w, h, orient, nbps, passes, length=None):
+        buf = np.frombuffer(bytes(data), np.uint8).copy() if len(data) else np.zeros(1, np.uint8)
+        n = len(data) if length is None else length
+        mag = np.zeros(w * h, np.uint32)
+        sign = np.zeros(w * h, np.uint8)
+        sig = np.zeros(w * h, np.uint8)
+        self.lib.or_t1_decode(_p(buf), n, w, h, orient, nbps, passes, _p(mag), _p(sign), _p(sig))
+        return mag.reshape(h, w), sign.reshape(h, w), sig.reshape(h, w)
+
+    def fdwt_f64(self, plane, levels):
+        a = np.array(plane, dtype=np.float64, copy=True, order="C")
+        self.lib.or_fdwt97_2d_f64(_p(a), a.shape[1], a.shape[0], levels)
+        return a
+
+    def idwt_f64(self, plane, levels):
+        a = np.array(plane, dtype=np.float64, copy=True, order="C")
+        self.lib.or_idwt97_2d_f64(_p(a), a.shape[1], a.shape[0], levels)
+        return a
+
+    def quantize_coeff(self, v, inv_delta, max_mag):
+        return self.lib.or_quantize_coeff(v, inv_delta, max_mag)
+
+    def quantize_block(self, plane, x0, y0, w, h, delta, mb):
+        plane = np.ascontiguousarray(plane, dtype=np.float32)
+        out = np.zeros(1024, np.uint32)
+        self.lib.or_quantize_block(_p(plane), plane.shape[1], x0, y0, w, h, delta, mb, _p(out))
+        return out
+
+    def synthesis_norm(self, level, orient):
+        return self.lib.or_synthesis_norm(level, orient)
+
+    def convert_frame(self, lut, cp, rgb48):
+        rgb48 = np.ascontiguousarray(rgb48, dtype=np.uint16)
+        H, W, _ = rgb48.shape
+        planes = np.zeros((3, H, W), np.float32)
+        xyz = np.zeros((H, W, 3), np.uint16)
+        self.lib.or_convert_frame(_p(lut), _p(cp), _p(rgb48), W, H, _p(planes), _p(xyz))
+        return planes, xyz
+
+    def xyz_to_ict(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.uint16)
+        _, H, W = xyz.shape
+        planes = np.zeros((3, H, W), np.float32)
+        self.lib.or_xyz_planar_to_ict(_p(xyz), W, H, _p(planes))
+        return planes
+
+    def color_pixel(self, lut, cp, r, g, b):
+        out = np.zeros(3, np.int32)
+        self.lib.or_color_pixel(_p(lut), _p(cp), r, g, b, _p(out))
+        return tuple(int(v) for v in out)
+
+
+class Reference(_Lib):
+    def __init__(self, path=REF_SO):
+        super().__init__(path, "ref_")
+        L = self.lib
+        L.ref_synthesis_norm.restype = C.c_double
+        L.ref_synthesis_norm.argtypes = [C.c_int, C.c_int]
+        L.ref_quantize_coeff.restype = C.c_uint32
+        L.ref_quantize_coeff.argtypes = [C.c_float, C.c_float, C.c_uint32]
+        L.ref_convert_frame.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_int, C.c_int,
+                                        C.c_void_p, C.c_void_p]
+
+    def synthesis_norm(self, level, orient):
+        return self.lib.ref_synthesis_norm(level, orient)
+
+    def quantize_coeff(self, v, inv_delta, max_mag):
+        return self.lib.ref_quantize_coeff(v, inv_delta, max_mag)
+
+    def convert_frame(self, transfer, primaries, rgb48, sdr_white=48.0, cinema_peak=48.0):
+        rgb48 = np.ascontiguousarray(rgb48, dtype=np.uint16)
+        H, W, _ = rgb48.shape
+        planes = np.zeros((3, H, W), np.float32)
+        xyz = np.zeros((H, W, 3), np.uint16)
+        self.lib.ref_convert_frame(transfer, primaries, sdr_white, cinema_peak, _p(rgb48), W, H, _p(planes),
+                                   _p(xyz))
+        return planes, xyz
+
+
+def oracle():
+    return Oracle()
+
+
+def reference():
+    """The compiled reference, or None where it was not built."""
+    return Reference() if os.path.exists(REF_SO) else None
+
+
+def random_block(rng, trial):
+    """A seeded test block in the manner of the reference's Tier-1 test (ragged sizes, four
+    orientations, three magnitude regimes: small, sparse and large)."""
+    w = 1 + int(rng.integers(32)) if trial % 7 == 0 else 32
+    h = 1 + int(rng.integers(32)) if trial % 5 == 0 else 32
+    orient = trial % 4
+    mode = trial % 3
+    sd = (20.0, 300.0, 3000.0)[mode]
+    v = rng.normal(0, sd, size=(h, w))
+    if mode == 1:
+        v = np.where(rng.random((h, w)) < 0.7, v * 0.05, v)
+    v = v.astype(np.float32)
+    c = np.zeros((32, 32), np.uint32)
+    a = np.abs(v) * np.float32(64.0)
+    m = np.where(a >= np.float32((1 << 26) - 1), (1 << 26) - 1, a.astype(np.uint32))
+    m = m | np.where((v < 0) & (m >= 64), np.uint32(0x80000000), np.uint32(0))
+    c[:h, :w] = m
+    return c, w, h, orient

@@ -1,0 +1,31 @@
+"""Comparison of a Tier-1 frame (info rows, pass tables, codewords) against the oracle's,
+block by block, with a readable report of the first differences."""
+import numpy as np
+
+
+def oracle_blocks(ref):
+    info = ref["info"]
+    out = []
+    for i in range(len(info)):
+        nbps, np_, off, ln, pb = (int(v) for v in info[i])
+        out.append(dict(nbps=nbps, num_passes=np_, data=bytes(ref["data"][off:off + ln]),
+                        pass_len=[int(v) for v in ref["pass_len"][pb:pb + np_]],
+                        pass_dist=[int(v) for v in ref["pass_dist"][pb:pb + np_]]))
+    return out
+
+
+def compare_tier1(gpu, ref, max_report=5):
+    """Returns (number of blocks that differ, report lines)."""
+    want = oracle_blocks(ref)
+    bad, lines = 0, []
+    if len(gpu.info) != len(want):
+        return len(want), [f"block count {len(gpu.info)} != {len(want)}"]
+    for i, w in enumerate(want):
+        g = gpu.block(i)
+        if g != w:
+            bad += 1
+            if len(lines) < max_report:
+                diff = [k for k in w if g[k] != w[k]]
+                lines.append(f"block {i}: differs in {diff}; gpu nbps/np/len {g['nbps']}/{g['num_passes']}/"
+                             f"{len(g['data'])} oracle {w['nbps']}/{w['num_passes']}/{len(w['data'])}")
+    return bad, lines
