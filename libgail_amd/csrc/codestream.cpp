@@ -1,0 +1,676 @@
+// Tier-2, rate control and the DCI structural check (see codestream.h).
+#include "codestream.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+namespace gail_amd {
+
+namespace {
+
+// ---------------------------------------------------------------- packet-header bits
+// B.10.1: bits fill each byte MSB first; a byte after 0xFF carries 7 bits, so no marker
+// can appear in a header. With a null destination it only counts.
+class HeaderBits {
+public:
+    explicit HeaderBits(std::vector<uint8_t>* dst) : dst_(dst) {}
+    void bit(uint32_t b) { bits(b & 1u, 1); }
+    void bits(uint32_t v, int n) {
+        while (n > 0) {
+            const int take = std::min(n, room_ - used_);
+            n -= take;
+            acc_ = (acc_ << take) | ((v >> n) & ((1u << take) - 1u));
+            used_ += take;
+            if (used_ == room_) emit();
+        }
+    }
+    void zeros(int n) {
+        for (; n > 24; n -= 24) bits(0, 24);
+        bits(0, n);
+    }
+    // Pads the last byte; a header ending in 0xFF still owes the stuffed zero byte.
+    size_t finish() {
+        if (used_ > 0 || count_ == 0) {
+            acc_ <<= room_ - used_;
+            emit();
+        }
+        if (last_ == 0xFFu) {
+            acc_ = 0;
+            emit();
+        }
+        return count_;
+    }
+
+private:
+    void emit() {
+        last_ = acc_ & 0xFFu;
+        if (dst_) dst_->push_back((uint8_t)last_);
+        ++count_;
+        room_ = last_ == 0xFFu ? 7 : 8;
+        acc_ = 0;
+        used_ = 0;
+    }
+    std::vector<uint8_t>* dst_;
+    uint32_t acc_ = 0, last_ = 0;
+    int used_ = 0, room_ = 8;
+    size_t count_ = 0;
+};
+
+// ---------------------------------------------------------------- tag trees (B.10.2)
+class TagTree {
+public:
+    void reset(int w, int h) {
+        if (w != w_ || h != h_) {
+            w_ = w;
+            h_ = h;
+            parent_.clear();
+            int lw = w, lh = h, base = 0;
+            std::vector<std::array<int, 3>> levels;
+            for (;;) {
+                levels.push_back({base, lw, lh});
+                base += lw * lh;
+                if (lw == 1 && lh == 1) break;
+                lw = (lw + 1) / 2;
+                lh = (lh + 1) / 2;
+            }
+            parent_.assign(base, -1);
+            for (size_t l = 0; l + 1 < levels.size(); ++l) {
+                const auto& a = levels[l];
+                const auto& b = levels[l + 1];
+                for (int y = 0; y < a[2]; ++y)
+                    for (int x = 0; x < a[1]; ++x) parent_[a[0] + y * a[1] + x] = b[0] + (y / 2) * b[1] + x / 2;
+            }
+            value_.resize(base);
+            low_.resize(base);
+            known_.resize(base);
+        }
+        std::fill(value_.begin(), value_.end(), 1 << 30);
+        std::fill(low_.begin(), low_.end(), 0);
+        std::fill(known_.begin(), known_.end(), 0);
+    }
+    // Leaf values, then every inner node takes the minimum of its children.
+    void set(int leaf, int v) {
+        for (int i = leaf; i >= 0; i = parent_[i]) value_[i] = std::min(value_[i], v);
+    }
+    void encode(HeaderBits& bw, int leaf, int threshold) {
+        int path[32], depth = 0;
+        for (int i = leaf; i >= 0; i = parent_[i]) path[depth++] = i;
+        int low = 0;
+        while (depth > 0) {
+            const int n = path[--depth];
+            low = std::max(low, low_[n]);
+            while (low < threshold) {
+                if (low >= value_[n]) {
+                    if (!known_[n]) {
+                        bw.bit(1);
+                        known_[n] = 1;
+                    }
+                    break;
+                }
+                bw.bit(0);
+                ++low;
+            }
+            low_[n] = low;
+        }
+    }
+
+private:
+    int w_ = 0, h_ = 0;
+    std::vector<int> parent_, value_, low_;
+    std::vector<uint8_t> known_;
+};
+
+int floorLog2(uint32_t v) { return 31 - __builtin_clz(v); }
+
+// Table B.4: codewords for the number of coding passes.
+void passCount(HeaderBits& bw, int n) {
+    if (n == 1) bw.bits(0, 1);
+    else if (n == 2) bw.bits(2, 2);
+    else if (n <= 5) bw.bits(0xC | (n - 3), 4);
+    else if (n <= 36) bw.bits(0x1E0 | (n - 6), 9);
+    else if (n <= 164) bw.bits(0xFF80 | (n - 37), 16);
+    else throw std::runtime_error("too many coding passes");
+}
+
+struct Packets {
+    const Layout& L;
+    const Tier1Ref& t1;
+    const uint8_t* passes;
+    TagTree incl, zbp;
+
+    // One packet (one precinct of one resolution, single layer); appends it to dst when
+    // given and returns its size.
+    size_t write(int comp, int res, int prec, std::vector<uint8_t>* dst) {
+        const Precinct& P = L.resolution(comp, res).precincts[prec];
+        bool any = false;
+        for (int k = 0; k < P.nb && !any; ++k) {
+            const PrecinctBand& pb = P.bands[k];
+            const Subband& b = L.bands[pb.band];
+            for (int j = 0; j < pb.ncby && !any; ++j)
+                for (int i = 0; i < pb.ncbx && !any; ++i)
+                    any = passes[b.cblkBase + (pb.cby0 + j) * b.ncbx + pb.cbx0 + i] != 0;
+        }
+        HeaderBits bw(dst);
+        if (!any) {
+            bw.bit(0);
+            return bw.finish();
+        }
+        bw.bit(1);
+        size_t body = 0;
+        for (int k = 0; k < P.nb; ++k) {
+            const PrecinctBand& pb = P.bands[k];
+            if (!pb.ncbx) continue;
+            const Subband& b = L.bands[pb.band];
+            incl.reset(pb.ncbx, pb.ncby);
+            zbp.reset(pb.ncbx, pb.ncby);
+            for (int j = 0; j < pb.ncby; ++j)
+                for (int i = 0; i < pb.ncbx; ++i) {
+                    const int ci = b.cblkBase + (pb.cby0 + j) * b.ncbx + pb.cbx0 + i;
+                    incl.set(j * pb.ncbx + i, passes[ci] ? 0 : 1);
+                    zbp.set(j * pb.ncbx + i, b.Mb - (int)t1.nbps(ci));
+                }
+            for (int j = 0; j < pb.ncby; ++j)
+                for (int i = 0; i < pb.ncbx; ++i) {
+                    const int leaf = j * pb.ncbx + i;
+                    const int ci = b.cblkBase + (pb.cby0 + j) * b.ncbx + pb.cbx0 + i;
+                    incl.encode(bw, leaf, 1);
+                    const int np = passes[ci];
+                    if (!np) continue;
+                    zbp.encode(bw, leaf, 1 << 20);
+                    passCount(bw, np);
+                    const uint32_t len = t1.len(ci, np - 1);
+                    const int extra = floorLog2((uint32_t)np);
+                    const int need = len ? floorLog2(len) + 1 : 1;
+                    const int incr = std::max(0, need - (3 + extra));
+                    bw.bits(((1u << incr) - 1u) << 1, incr + 1);  // Lblock increment: incr ones, then 0
+                    bw.bits(len, 3 + incr + extra);
+                    body += len;
+                }
+        }
+        const size_t hdr = bw.finish();
+        if (dst) {
+            for (int k = 0; k < P.nb; ++k) {
+                const PrecinctBand& pb = P.bands[k];
+                if (!pb.ncbx) continue;
+                const Subband& b = L.bands[pb.band];
+                for (int j = 0; j < pb.ncby; ++j)
+                    for (int i = 0; i < pb.ncbx; ++i) {
+                        const int ci = b.cblkBase + (pb.cby0 + j) * b.ncbx + pb.cbx0 + i;
+                        if (!passes[ci]) continue;
+                        const uint8_t* c = t1.code(ci);
+                        dst->insert(dst->end(), c, c + t1.len(ci, passes[ci] - 1));
+                    }
+            }
+        }
+        return hdr + body;
+    }
+};
+
+// CPRL progression volumes (progression.h:20-26): one for 2K; for 4K the POC puts the 2K
+// resolutions first and the 4K resolution last.
+struct Volume {
+    int rs, re;
+};
+int volumes(const Layout& L, Volume v[2]) {
+    v[0] = {0, L.fourk ? L.levels : L.levels + 1};
+    v[1] = {L.levels, L.levels + 1};
+    return L.fourk ? 2 : 1;
+}
+
+// Packets of component comp within a volume, CPRL order (B.12.1.5, origin 0, one tile).
+template <class F> void forEachPacket(const Layout& L, Volume v, int comp, F&& f) {
+    const int NL = L.levels;
+    int xs = 1 << 30, ys = 1 << 30;
+    for (int r = v.rs; r < v.re; ++r) {
+        const Resolution& R = L.resolution(0, r);
+        xs = std::min(xs, 1 << (R.ppx + NL - r));
+        ys = std::min(ys, 1 << (R.ppy + NL - r));
+    }
+    for (int y = 0; y < L.height; y += ys)
+        for (int x = 0; x < L.width; x += xs)
+            for (int r = v.rs; r < v.re; ++r) {
+                const Resolution& R = L.resolution(comp, r);
+                const int xr = 1 << (R.ppx + NL - r), yr = 1 << (R.ppy + NL - r);
+                if ((x & (xr - 1)) || (y & (yr - 1))) continue;
+                const int px = x / xr, py = y / yr;
+                if (px >= R.npx || py >= R.npy) continue;
+                f(r, py * R.npx + px);
+            }
+}
+
+void put16(std::vector<uint8_t>& v, uint32_t x) {
+    v.push_back((uint8_t)(x >> 8));
+    v.push_back((uint8_t)x);
+}
+void put32(std::vector<uint8_t>& v, uint32_t x) {
+    put16(v, x >> 16);
+    put16(v, x & 0xFFFFu);
+}
+
+enum : uint16_t {
+    SOC = 0xFF4F, SIZ = 0xFF51, COD = 0xFF52, TLM = 0xFF55, QCD = 0xFF5C, POC = 0xFF5F, COM = 0xFF64,
+    SOT = 0xFF90, SOD = 0xFF93, EOC = 0xFFD9
+};
+
+// Main header of a DCI frame (t2.cpp:126-183): SIZ (Rsiz 3 / 4), COD (CPRL, one layer,
+// ICT, 32x32 blocks, 9/7, precincts 128 / 256), QCD (scalar expounded), the 4K POC, COM,
+// and a TLM patched once the tile-parts are known (kept last).
+void mainHeader(const Layout& L, std::vector<uint8_t>& v, int ntp, const std::string& comment) {
+    const int nres = L.numRes();
+    put16(v, SOC);
+    put16(v, SIZ);
+    put16(v, 38 + 3 * kComps);
+    put16(v, L.fourk ? 4 : 3);
+    put32(v, L.width);
+    put32(v, L.height);
+    put32(v, 0);
+    put32(v, 0);
+    put32(v, L.width);
+    put32(v, L.height);
+    put32(v, 0);
+    put32(v, 0);
+    put16(v, kComps);
+    for (int c = 0; c < kComps; ++c) {
+        v.push_back(kBitDepth - 1);
+        v.push_back(1);
+        v.push_back(1);
+    }
+    put16(v, COD);
+    put16(v, 12 + nres);
+    v.push_back(0x01);
+    v.push_back(4);
+    put16(v, 1);
+    v.push_back(1);
+    v.push_back((uint8_t)L.levels);
+    v.push_back(kCblkLog2 - 2);
+    v.push_back(kCblkLog2 - 2);
+    v.push_back(0);
+    v.push_back(0);
+    for (int r = 0; r < nres; ++r) v.push_back(r == 0 ? 0x77 : 0x88);
+    put16(v, QCD);
+    const int nb = L.bandsPerComp();
+    put16(v, 3 + 2 * nb);
+    v.push_back((uint8_t)((L.guardBits << 5) | 2));
+    for (int i = 0; i < nb; ++i) put16(v, (uint32_t)((L.bands[i].eps << 11) | L.bands[i].mu));
+    if (L.fourk) {
+        put16(v, POC);
+        put16(v, 2 + 7 * 2);
+        const int NL = L.levels;
+        const uint8_t e[2][7] = {{0, 0, 0, 1, (uint8_t)NL, 3, 4}, {(uint8_t)NL, 0, 0, 1, (uint8_t)(NL + 1), 3, 4}};
+        for (auto& p : e) v.insert(v.end(), p, p + 7);
+    }
+    put16(v, COM);
+    put16(v, (uint32_t)(4 + comment.size()));
+    put16(v, 1);
+    v.insert(v.end(), comment.begin(), comment.end());
+    put16(v, TLM);
+    put16(v, 4 + 5 * ntp);
+    v.push_back(0);
+    v.push_back(0x50);
+    for (int i = 0; i < ntp; ++i) {
+        v.push_back(0);
+        put32(v, 0);
+    }
+}
+
+}  // namespace
+
+const char* codestreamLabel() { return "GAIL 1.1.0"; }
+
+void buildCodestream(const Layout& L, const Tier1Ref& t1, const uint8_t* passes, std::vector<uint8_t>* out,
+                     CodestreamSizes* sizes, const std::string& tag) {
+    if (tag.size() > kMaxDeviceTagBytes) throw std::logic_error("device tag too long");
+    const int ntp = L.fourk ? 6 : 3;
+    Packets pw{L, t1, passes, {}, {}};
+    const std::string comment = tag.empty() ? std::string(codestreamLabel()) : std::string(codestreamLabel()) + " " + tag;
+    const size_t slack = std::strlen(codestreamLabel()) + 1 + kMaxDeviceTagBytes - comment.size();
+    std::vector<uint8_t> hdr;
+    mainHeader(L, hdr, ntp, comment);
+    const size_t tlmPos = hdr.size() - 5 * ntp;
+    if (out) *out = hdr;
+    Volume vols[2];
+    const int nv = volumes(L, vols);
+    CodestreamSizes sz;
+    sz.numTileParts = ntp;
+    size_t total = hdr.size() + slack;
+    for (int vi = 0; vi < nv; ++vi)
+        for (int c = 0; c < kComps; ++c) {
+            const int tp = vi * 3 + c;
+            size_t sotPos = 0;
+            if (out) {
+                sotPos = out->size();
+                put16(*out, SOT);
+                put16(*out, 10);
+                put16(*out, 0);
+                put32(*out, 0);
+                out->push_back((uint8_t)tp);
+                out->push_back((uint8_t)ntp);
+                put16(*out, SOD);
+            }
+            size_t bytes = 14;
+            forEachPacket(L, vols[vi], c, [&](int r, int prec) { bytes += pw.write(c, r, prec, out); });
+            if (out) {
+                const uint32_t psot = (uint32_t)(out->size() - sotPos);
+                if (psot != bytes) throw std::logic_error("tile-part size mismatch");
+                for (int b = 0; b < 4; ++b) {
+                    (*out)[sotPos + 6 + b] = (uint8_t)(psot >> (24 - 8 * b));
+                    (*out)[tlmPos + 5 * tp + 1 + b] = (uint8_t)(psot >> (24 - 8 * b));
+                }
+            }
+            sz.tilePart[tp] = bytes;
+            sz.comp[c] += bytes;
+            total += bytes;
+        }
+    total += 2;
+    if (out) put16(*out, EOC);
+    sz.total = total;
+    if (out && out->size() + slack != total) throw std::logic_error("codestream size mismatch");
+    if (sizes) *sizes = sz;
+}
+
+size_t emptyCodestreamBytes(const Layout& L) {
+    std::vector<uint32_t> info((size_t)L.numCblks() * 5, 0);
+    const Tier1Ref t1{info.data(), nullptr, nullptr, nullptr, (size_t)L.numCblks()};
+    const std::vector<uint8_t> passes((size_t)L.numCblks(), 0);
+    CodestreamSizes sz;
+    buildCodestream(L, t1, passes.data(), nullptr, &sz);
+    return sz.total;
+}
+
+RateLimits dciLimits(double fps, double maxMbps, double compMbps) {
+    auto budget = [fps](double mbps) {
+        const double b = std::floor(mbps * 1e6 / (8.0 * fps)) - 16;
+        return std::isfinite(b) && b > 0 ? (size_t)b : (size_t)0;
+    };
+    return RateLimits{budget(maxMbps), budget(compMbps)};
+}
+
+// ---------------------------------------------------------------- rate control
+namespace {
+
+// Lower convex hull of each block's (rate, distortion) points, as start offsets plus the
+// passes, rates, slopes and distortion gains of the hull points (rate_control.cpp:38-86).
+struct Hull {
+    std::vector<uint32_t> start;
+    std::vector<uint8_t> passes;
+    std::vector<uint32_t> rate;
+    std::vector<double> slope, gain;
+};
+
+Hull buildHull(const Layout& L, const Tier1Ref& t1) {
+    const int n = L.numCblks();
+    Hull H;
+    H.start.resize(n + 1);
+    std::vector<uint32_t> R;
+    std::vector<double> D;
+    std::vector<int> K;
+    for (int i = 0; i < n; ++i) {
+        H.start[i] = (uint32_t)H.passes.size();
+        const int np = (int)t1.passes(i);
+        if (!np) continue;
+        const Subband& b = L.bands[L.cblks[i].band];
+        const double scale = (double)b.delta * b.delta * b.weight;
+        R.assign(1, 0);
+        D.assign(1, 0.0);
+        K.assign(1, 0);
+        double acc = 0;
+        for (int k = 0; k < np; ++k) {
+            acc += t1.dist(i, k) * passPlaneScale((int)t1.nbps(i), k) * scale;
+            const uint32_t r = t1.len(i, k);
+            if (acc <= D.back()) continue;
+            while (K.size() >= 2) {
+                const size_t m = K.size() - 1;
+                const double sNew = r == R[m] ? INFINITY : (acc - D[m]) / (double)(r - R[m]);
+                const double sLast = R[m] == R[m - 1] ? INFINITY : (D[m] - D[m - 1]) / (double)(R[m] - R[m - 1]);
+                if (sNew < sLast) break;
+                R.pop_back();
+                D.pop_back();
+                K.pop_back();
+            }
+            R.push_back(r);
+            D.push_back(acc);
+            K.push_back(k + 1);
+        }
+        for (size_t m = 1; m < K.size(); ++m) {
+            H.passes.push_back((uint8_t)K[m]);
+            H.rate.push_back(R[m]);
+            const double dr = (double)R[m] - (double)R[m - 1];
+            H.slope.push_back(dr <= 0 ? INFINITY : (D[m] - D[m - 1]) / dr);
+            H.gain.push_back(D[m] - D[m - 1]);
+        }
+    }
+    H.start[n] = (uint32_t)H.passes.size();
+    return H;
+}
+
+void select(const Layout& L, const Hull& H, const double lambda[3], std::vector<uint8_t>& passes) {
+    for (int i = 0; i < L.numCblks(); ++i) {
+        const int c = L.bands[L.cblks[i].band].comp;
+        uint8_t np = 0;
+        for (uint32_t h = H.start[i]; h < H.start[i + 1]; ++h) {
+            if (H.slope[h] < lambda[c]) break;
+            np = H.passes[h];
+        }
+        passes[i] = np;
+    }
+}
+
+// Hull points binned by log2(slope) at 1/1024 octave with suffix sums: data bytes and
+// included-block counts for any threshold in O(1) (rate_control.cpp:117-162).
+class SlopeHistogram {
+public:
+    SlopeHistogram(const Layout& L, const Hull& H, double loLog, double hiLog) : lo_(loLog) {
+        nb_ = std::max(16, (int)std::ceil((hiLog - loLog) * 1024.0) + 2);
+        for (int c = 0; c < 3; ++c) {
+            rate_[c].assign(nb_ + 1, 0.0);
+            incl_[c].assign(nb_ + 1, 0.0);
+        }
+        for (int i = 0; i < L.numCblks(); ++i) {
+            const int c = L.bands[L.cblks[i].band].comp;
+            uint32_t prev = 0;
+            for (uint32_t h = H.start[i]; h < H.start[i + 1]; ++h) {
+                const int b = bin(H.slope[h]);
+                rate_[c][b] += (double)H.rate[h] - prev;
+                if (h == H.start[i]) incl_[c][b] += 1;
+                prev = H.rate[h];
+            }
+        }
+        for (int c = 0; c < 3; ++c)
+            for (int b = nb_ - 1; b >= 0; --b) {
+                rate_[c][b] += rate_[c][b + 1];
+                incl_[c][b] += incl_[c][b + 1];
+            }
+    }
+    void eval(const double lam[3], double data[3], double incl[3]) const {
+        for (int c = 0; c < 3; ++c) {
+            const int b = lam[c] <= 0 ? 0 : std::min(nb_, bin(lam[c]));
+            data[c] = rate_[c][b];
+            incl[c] = incl_[c][b];
+        }
+    }
+
+private:
+    int bin(double s) const {
+        if (!std::isfinite(s)) return nb_ - 1;
+        return std::clamp((int)std::floor((std::log2(s) - lo_) * 1024.0), 0, nb_ - 1);
+    }
+    double lo_;
+    int nb_;
+    std::vector<double> rate_[3], incl_[3];
+};
+
+}  // namespace
+
+RateResult allocateRate(const Layout& L, const Tier1Ref& t1, const RateLimits& lim) {
+    const Hull H = buildHull(L, t1);
+    RateResult res;
+    res.passes.assign(L.numCblks(), 0);
+    double lam[3] = {0, 0, 0};
+    auto measure = [&](const double l[3]) {
+        select(L, H, l, res.passes);
+        buildCodestream(L, t1, res.passes.data(), nullptr, &res.sizes);
+        ++res.iterations;
+        return res.sizes;
+    };
+    double maxSlope = 0, minSlope = INFINITY;
+    for (double s : H.slope)
+        if (std::isfinite(s) && s > 0) {
+            maxSlope = std::max(maxSlope, s);
+            minSlope = std::min(minSlope, s);
+        }
+    if (maxSlope == 0) maxSlope = minSlope = 1;
+    const double loB = std::log2(minSlope) - 1, hiB = std::log2(maxSlope) + 1;
+    const SlopeHistogram hist(L, H, loB, hiB);
+    CodestreamSizes sz = measure(lam);
+    // header bytes per component ~ calibrated + beta * (included blocks - calibrated count)
+    double hdrCal[3], inclCal[3], beta[3];
+    auto calibrate = [&](const double l[3], const CodestreamSizes& ex) {
+        double data[3], incl[3];
+        hist.eval(l, data, incl);
+        for (int c = 0; c < 3; ++c) {
+            hdrCal[c] = (double)ex.comp[c] - data[c];
+            inclCal[c] = incl[c];
+            beta[c] = incl[c] > 0 ? std::max(1.0, std::min(8.0, hdrCal[c] / incl[c])) : 3.0;
+        }
+    };
+    auto model = [&](const double l[3], double comp[3]) {
+        double data[3], incl[3];
+        hist.eval(l, data, incl);
+        double tot = (double)(sz.total - sz.comp[0] - sz.comp[1] - sz.comp[2]);
+        for (int c = 0; c < 3; ++c) {
+            comp[c] = data[c] + hdrCal[c] + beta[c] * (incl[c] - inclCal[c]);
+            tot += comp[c];
+        }
+        return tot;
+    };
+    auto search = [&](auto pred, double lo, double hi) {
+        for (int it = 0; it < 48 && hi - lo > 1e-5; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            if (pred(mid)) hi = mid;
+            else lo = mid;
+        }
+        return hi;
+    };
+    if (sz.total > lim.maxTotal) {
+        calibrate(lam, sz);
+        for (int round = 0; round < 6; ++round) {
+            const double lg = search(
+                [&](double m) {
+                    const double l = std::exp2(m), ll[3] = {l, l, l};
+                    double comp[3];
+                    return model(ll, comp) <= (double)lim.maxTotal;
+                },
+                loB, hiB);
+            lam[0] = lam[1] = lam[2] = std::exp2(lg);
+            sz = measure(lam);
+            if (sz.total <= lim.maxTotal && sz.total + lim.maxTotal / 500 >= lim.maxTotal) break;
+            calibrate(lam, sz);
+        }
+        while (sz.total > lim.maxTotal && lam[0] <= maxSlope) {
+            for (double& l : lam) l = l > 0 ? l * 1.002 : minSlope;
+            sz = measure(lam);
+        }
+    }
+    for (int c = 0; c < kComps; ++c) {
+        if (sz.comp[c] <= lim.maxComp) continue;
+        calibrate(lam, sz);
+        for (int round = 0; round < 6; ++round) {
+            const double lg = search(
+                [&](double m) {
+                    double ll[3] = {lam[0], lam[1], lam[2]};
+                    ll[c] = std::exp2(m);
+                    double comp[3];
+                    model(ll, comp);
+                    return comp[c] <= (double)lim.maxComp;
+                },
+                std::log2(std::max(lam[c], minSlope)), hiB);
+            lam[c] = std::exp2(lg);
+            sz = measure(lam);
+            if (sz.comp[c] <= lim.maxComp && sz.comp[c] + lim.maxComp / 500 >= lim.maxComp) break;
+            calibrate(lam, sz);
+        }
+        while (sz.comp[c] > lim.maxComp && lam[c] <= maxSlope) {
+            lam[c] = lam[c] > 0 ? lam[c] * 1.002 : minSlope;
+            sz = measure(lam);
+        }
+    }
+    res.lambda = std::max({lam[0], lam[1], lam[2]});
+    for (int c = 0; c < 3; ++c) res.lambdaComp[c] = lam[c];
+    double dist = 0;
+    for (int i = 0; i < L.numCblks(); ++i)
+        for (uint32_t h = H.start[i]; h < H.start[i + 1]; ++h)
+            if (H.passes[h] > res.passes[i]) dist += H.gain[h];
+    res.distortion = dist;
+    res.sizes = sz;
+    return res;
+}
+
+// ---------------------------------------------------------------- DCI check
+std::string checkDciCodestream(const uint8_t* d, size_t n, double fps, bool fourk) {
+    auto u16 = [&](size_t i) { return i + 1 < n ? (uint32_t)d[i] << 8 | d[i + 1] : 0u; };
+    auto u32 = [&](size_t i) { return u16(i) << 16 | u16(i + 2); };
+    if (n < 4 || u16(0) != SOC) return "no SOC";
+    if (u16(n - 2) != EOC) return "no EOC";
+    const double limit = std::rint(250e6 / (8.0 * fps));
+    if ((double)n > limit) return "frame larger than the DCI limit";
+    size_t p = 2;
+    bool siz = false, cod = false, qcd = false, tlm = false, poc = false;
+    std::vector<uint32_t> tlmLen;
+    while (p + 4 <= n && u16(p) != SOT) {
+        const uint32_t m = u16(p), len = u16(p + 2);
+        if (len < 2 || p + 2 + len > n) return "truncated marker segment";
+        if (m == SIZ) {
+            siz = true;
+            if (u16(p + 4) != (fourk ? 4u : 3u)) return "Rsiz is not the DCI profile";
+            const uint32_t W = u32(p + 6), H = u32(p + 10);
+            if (fourk ? (W > 4096 || H > 2160) : (W > 2048 || H > 1080)) return "picture exceeds the container";
+            if (u16(p + 38) != 3) return "not three components";
+            for (int c = 0; c < 3; ++c)
+                if (d[p + 40 + 3 * c] != 11) return "component is not 12-bit";
+        } else if (m == COD) {
+            cod = true;
+            if (d[p + 5] != 4) return "progression is not CPRL";
+            if (u16(p + 6) != 1) return "more than one layer";
+            if (d[p + 10] != 3 || d[p + 11] != 3) return "code-blocks are not 32x32";
+            if (d[p + 12] != 0) return "code-block style is not 0";
+            if (d[p + 13] != 0) return "transform is not 9/7";
+            const int nl = d[p + 9];
+            if (nl != (fourk ? 6 : 5)) return "unexpected decomposition levels";
+        } else if (m == QCD) {
+            qcd = true;
+            if ((d[p + 4] & 0x1F) != 2) return "quantisation is not scalar expounded";
+        } else if (m == POC) {
+            poc = true;
+        } else if (m == TLM) {
+            tlm = true;
+            if (d[p + 5] != 0x50) return "TLM is not 8/32-bit";
+            for (size_t q = p + 6; q + 5 <= p + 2 + len; q += 5) tlmLen.push_back(u32(q + 1));
+        }
+        p += 2 + len;
+    }
+    if (!siz || !cod || !qcd || !tlm) return "missing main-header marker";
+    if (fourk && !poc) return "4K frame without POC";
+    const size_t ntp = fourk ? 6 : 3;
+    if (tlmLen.size() != ntp) return "TLM does not list every tile-part";
+    const double compLimit = 200e6 / (8.0 * fps);
+    size_t comp[3] = {0, 0, 0};
+    for (size_t t = 0; t < ntp; ++t) {
+        if (u16(p) != SOT) return "tile-part without SOT";
+        const uint32_t psot = u32(p + 6);
+        if (psot != tlmLen[t]) return "Psot does not match TLM";
+        if (d[p + 10] != t || d[p + 11] != ntp) return "tile-part numbering";
+        if (u16(p + 12) != SOD) return "no SOD";
+        comp[t % 3] += psot;
+        p += psot;
+        if (p > n - 2) return "tile-part runs past the end";
+    }
+    if (p != n - 2) return "bytes after the last tile-part";
+    for (size_t c : comp)
+        if ((double)c > compLimit) return "component exceeds 200 Mbit/s";
+    return "";
+}
+
+}  // namespace gail_amd
