@@ -1,0 +1,169 @@
+"""Benchmark of the GAIL HIP encoder path on one MI355X.
+
+Workload (default, BASELINE configs[1]/[2]): DCI 4K flat 3996x2160 12-bit X'Y'Z' frames of
+the seeded synthetic generator, NL 6, 32x32 code-blocks. A step encodes one batch of B frames
+(per-frame seeds) through colour/ICT -> 9/7 DWT -> quantisation -> Tier-1 (context modelling,
+MQ, truncation lengths) to Tier-1 frames in host memory, with the inputs already resident in
+HBM. Early termination uses 0.25 x the rate-control lambda of the preceding frames, as the
+reference pipeline does. The plain one-frame-at-a-time path is timed beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def dwt_level0_bytes(W, H):
+    """Algorithmic HBM bytes of DWT level 0 per frame: 3 planes of u16 codes read, 3 float
+    planes written (the ICT is applied on load)."""
+    return 3 * (2 + 4) * W * H
+
+
+def dwt_all_bytes(W, H, levels):
+    b = dwt_level0_bytes(W, H)
+    w, h = (W + 1) // 2, (H + 1) // 2
+    for _ in range(1, levels):
+        b += 3 * 8 * w * h
+        w, h = (w + 1) // 2, (h + 1) // 2
+    return b
+
+
+def cpu_baseline(frames, fourk, thr, budget_s=20.0):
+    """The reference's CPU flow (compiled from its sources into oracle/_ref when present,
+    else the oracle port) on this host's cores, over a bounded sample of the same frames."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import Oracle, reference
+    ref = reference()
+    kind = "reference" if ref is not None else "port"
+    lib = ref if ref is not None else Oracle()
+    cores = len(os.sched_getaffinity(0))
+    done, t0 = 0, time.perf_counter()
+    while done < len(frames) and (time.perf_counter() - t0) < budget_s:
+        lib.encode_frame_xyz(frames[done], fourk, thr=thr, threads=cores)
+        done += 1
+    dt = time.perf_counter() - t0
+    return {"value": round(done / dt, 3), "unit": "frames/s", "cores": cores, "kind": kind,
+            "sample": f"{done} 4K flat frames, colour/ICT + DWT + quantisation + Tier-1 with the same early "
+                      f"termination threshold, {cores} threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--profile", default="4k_flat")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-frames", type=int, default=8)
+    ap.add_argument("--graph", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU figures are not part of it "
+              "(frames shard across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import libgail_amd as g
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+    W, H, fourk = g.CONTAINERS[args.profile]
+    B = args.batch
+    layout = g.FrameLayout(W, H, fourk)
+    print(f"[bench] generating {B} frames {W}x{H}", file=sys.stderr, flush=True)
+    frames = [g.synth_frame(1000 + i, W, H, threads=16) for i in range(B)]
+    dev = g.Device(0)
+
+    # Early-termination threshold: 0.25 x lambda of a preceding frame (pipeline.cpp:194),
+    # taken here from the rate control of the first frame coded without early termination.
+    thr = 0.0
+    try:
+        import ctypes as C
+        h = g.host()
+        if hasattr(h, "gail_encode_codestream"):
+            enc = g.Encoder(dev, layout, g.xyz_settings(layout))
+            r = enc.encode(frames[0])
+            enc.close()
+            thr = 0.25 * g.rate_lambda(layout, r, fps=24.0, mbps=250.0)
+    except Exception as e:  # noqa: BLE001
+        print(f"[bench] no rate-control lambda ({e}); early termination off", file=sys.stderr)
+    print(f"[bench] early termination threshold {thr:.4g}", file=sys.stderr, flush=True)
+
+    # ---- batched form: B frames resident in HBM, one chain per step
+    batch = g.Batch(dev, [layout] * B, [g.xyz_settings(layout)] * B, use_graph=bool(args.graph))
+    for i, f in enumerate(frames):
+        batch.upload(i, f)
+        batch.set_slope(i, thr)
+    for _ in range(args.warmup):
+        batch.encode(copy=False)
+    stage_acc = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.encode(copy=False)
+        stage_acc.append(batch.stage_ms())
+    dt = time.perf_counter() - t0
+    ms_step = 1000 * dt / args.steps
+    fps = B * args.steps / dt
+    print(f"[bench] batched: {fps:.1f} frames/s, {ms_step:.2f} ms per batch of {B}", file=sys.stderr, flush=True)
+    med = {k: statistics.median(s[k] for s in stage_acc) for k in stage_acc[0]}
+    batch.close()
+
+    # ---- plain path: one frame at a time, upload included as in the reference
+    enc = g.Encoder(dev, layout, g.xyz_settings(layout))
+    enc.set_slope(thr)
+    lat = []
+    for i in range(args.plain_frames + 1):
+        t1 = time.perf_counter()
+        enc.encode(frames[i % B])
+        if i:
+            lat.append(1000 * (time.perf_counter() - t1))
+    enc.close()
+    plain_fps = 1000.0 / statistics.mean(lat)
+
+    # roofline: DWT level 0 of the whole batch in one launch (HIP events on the batch stream)
+    stage_note = "graph replay: per-stage events off" if args.graph else ""
+    roof = None
+    if med.get("dwt0", 0) > 0:
+        achieved = B * dwt_level0_bytes(W, H) / (med["dwt0"] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "dwt_level_kernel<level 0>"}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(frames[:4], fourk, thr)
+
+    line = {
+        "metric": "4K DCI encode fps, batched/served vs plain, 1 MI355X; p50/p99 frame latency",
+        "value": round(fps, 2), "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+int", "data": "synthetic",
+        "config": {"workload": f"batched {args.profile} {W}x{H} x{B} frames, Tier-1 to host", "frames_per_step": B,
+                   "levels": layout.levels, "code_blocks_per_frame": layout.num_cblks,
+                   "slope_threshold": thr, "graph": bool(args.graph)},
+        "plain_fps": round(plain_fps, 2),
+        "batched_over_plain": round(fps / plain_fps, 3),
+        "plain_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                             "p99": round(float(np.percentile(lat, 99)), 3)},
+        "stage_ms_per_batch": {k: round(v, 3) for k, v in med.items()},
+        "tier1_symbols_per_s": round(med["symbols"] / (ms_step * 1e-3), 1) if med.get("symbols") else None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if stage_note:
+        line["note"] = stage_note
+    dev.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,261 @@
+/* GAIL encoder library API on the HIP backend (libgail.so of this build).
+ *
+ * The encoder half of the reference's public header (include/gail/gail.h:1-362), with the
+ * same names, structure layouts and enum values, so applications written against the
+ * reference compile and run unchanged: pull (send / receive), callback (on_packet) and
+ * blocking (gail_encoder_encode) encodes. Additions, as gail.h:82 allows ("enum values and
+ * functions may be added"): GAIL_BACKEND_HIP, GAIL_PIX_XYZ12_PLANAR, and the helper entry
+ * points at the end. The decoder half (gail.h:385-538) is not part of this build.
+ */
+#ifndef GAIL_AMD_H
+#define GAIL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__) || defined(__clang__)
+#define GAIL_API __attribute__((visibility("default")))
+#else
+#define GAIL_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GAIL_VERSION_MAJOR 1
+#define GAIL_VERSION_MINOR 1
+#define GAIL_VERSION_PATCH 0
+#define GAIL_VERSION ((GAIL_VERSION_MAJOR << 16) | (GAIL_VERSION_MINOR << 8) | GAIL_VERSION_PATCH)
+
+/* gail.h:85-95 */
+typedef enum gail_status {
+    GAIL_OK = 0,
+    GAIL_EAGAIN = 1,
+    GAIL_EOF = 2,
+    GAIL_ERROR_INVALID = -1,
+    GAIL_ERROR_UNSUPPORTED = -2,
+    GAIL_ERROR_NOMEM = -3,
+    GAIL_ERROR_FAILED = -4,
+    GAIL_ERROR_STATE = -5,
+    GAIL_ERROR_CANCELLED = -6
+} gail_status;
+
+GAIL_API const char* gail_last_error(void);
+GAIL_API const char* gail_status_string(int status);
+GAIL_API unsigned gail_version(void);
+GAIL_API const char* gail_version_string(void);
+
+/* gail.h:107-112 */
+typedef enum gail_profile { GAIL_PROFILE_2K = 0, GAIL_PROFILE_4K = 1 } gail_profile;
+typedef enum gail_container { GAIL_CONTAINER_FLAT = 0, GAIL_CONTAINER_SCOPE = 1, GAIL_CONTAINER_FULL = 2 } gail_container;
+GAIL_API int gail_container_size(gail_profile profile, gail_container container, int* width, int* height);
+
+/* gail.h:124-142; GAIL_PIX_XYZ12_PLANAR: three planes (data[0..2] = X', Y', Z') of 16-bit
+ * words holding 12-bit codes in the low bits, encoded exactly. */
+typedef enum gail_pixel_format {
+    GAIL_PIX_RGB48 = 0,
+    GAIL_PIX_XYZ12 = 1,
+    GAIL_PIX_XYZ12_MSB = 2,
+    GAIL_PIX_YUV420P = 3,
+    GAIL_PIX_YUV420P10 = 4,
+    GAIL_PIX_YUV420P12 = 5,
+    GAIL_PIX_YUV422P = 6,
+    GAIL_PIX_YUV422P10 = 7,
+    GAIL_PIX_YUV422P12 = 8,
+    GAIL_PIX_YUV444P = 9,
+    GAIL_PIX_YUV444P10 = 10,
+    GAIL_PIX_YUV444P12 = 11,
+    GAIL_PIX_GBRP = 12,
+    GAIL_PIX_GBRP10 = 13,
+    GAIL_PIX_GBRP12 = 14,
+    GAIL_PIX_GBRP16 = 15,
+    GAIL_PIX_XYZ12_PLANAR = 16
+} gail_pixel_format;
+
+typedef enum gail_matrix { GAIL_MATRIX_BT709 = 0, GAIL_MATRIX_BT2020 = 1, GAIL_MATRIX_BT601 = 2 } gail_matrix;
+typedef enum gail_chroma_location {
+    GAIL_CHROMA_LEFT = 0,
+    GAIL_CHROMA_CENTER = 1,
+    GAIL_CHROMA_TOPLEFT = 2
+} gail_chroma_location;
+
+/* gail.h:152-161 */
+typedef struct gail_input {
+    gail_pixel_format format;
+    int width, height; /* picture size; 0 = the container. A smaller picture is centred. */
+    gail_matrix matrix;
+    int full_range;
+    gail_chroma_location chroma_location;
+    uint64_t reserved[4];
+} gail_input;
+
+/* gail.h:165-232 */
+typedef enum gail_color_mode { GAIL_COLOR_MANAGED = 0, GAIL_COLOR_LUT3D = 1 } gail_color_mode;
+typedef enum gail_transfer {
+    GAIL_TRANSFER_BT1886 = 0,
+    GAIL_TRANSFER_SRGB = 1,
+    GAIL_TRANSFER_PQ = 2,
+    GAIL_TRANSFER_HLG = 3,
+    GAIL_TRANSFER_LINEAR = 4,
+    GAIL_TRANSFER_GAMMA26 = 5
+} gail_transfer;
+typedef enum gail_primaries {
+    GAIL_PRIMARIES_BT709 = 0,
+    GAIL_PRIMARIES_BT2020 = 1,
+    GAIL_PRIMARIES_P3D65 = 2,
+    GAIL_PRIMARIES_DCIP3 = 3
+} gail_primaries;
+typedef enum gail_tone_map {
+    GAIL_TONE_MAP_SIGNAL = 0,
+    GAIL_TONE_MAP_SHOULDER = 1,
+    GAIL_TONE_MAP_BT2390 = 2,
+    GAIL_TONE_MAP_NONE = 3
+} gail_tone_map;
+typedef enum gail_tone_norm {
+    GAIL_TONE_NORM_CHANNEL = 0,
+    GAIL_TONE_NORM_MAXRGB = 1,
+    GAIL_TONE_NORM_LUMINANCE = 2
+} gail_tone_norm;
+typedef enum gail_gamut { GAIL_GAMUT_NONE = 0, GAIL_GAMUT_P3D65 = 1, GAIL_GAMUT_DCIP3 = 2 } gail_gamut;
+
+typedef struct gail_color {
+    gail_color_mode mode;
+    gail_transfer transfer;
+    gail_primaries primaries;
+    gail_tone_map tone_map;
+    gail_tone_norm tone_norm;
+    double sdr_white;
+    double hdr_peak;
+    double hdr_ref_white;
+    double cinema_ref_white;
+    double cinema_peak;
+    double knee;
+    gail_gamut gamut;
+    double gamut_threshold;
+    const float* lut3d;
+    int lut3d_size;
+    uint64_t reserved[8];
+} gail_color;
+
+typedef enum gail_preset {
+    GAIL_PRESET_SDR_BT709 = 0,
+    GAIL_PRESET_SDR_BT2020 = 1,
+    GAIL_PRESET_HDR_PQ_BT2020 = 2,
+    GAIL_PRESET_HDR_HLG_BT2020 = 3
+} gail_preset;
+GAIL_API void gail_color_preset(gail_preset preset, gail_color* c);
+
+/* gail.h:245-250, plus the HIP backend */
+typedef enum gail_backend {
+    GAIL_BACKEND_AUTO = 0,
+    GAIL_BACKEND_CUDA = 1,
+    GAIL_BACKEND_VULKAN = 2,
+    GAIL_BACKEND_CPU = 3,
+    GAIL_BACKEND_HIP = 4
+} gail_backend;
+
+typedef enum gail_log_level { GAIL_LOG_ERROR = 0, GAIL_LOG_WARNING = 1, GAIL_LOG_INFO = 2, GAIL_LOG_DEBUG = 3 } gail_log_level;
+
+/* gail.h:254-263 */
+typedef struct gail_packet {
+    uint8_t* data;
+    size_t size;
+    int64_t index;
+    void* opaque;
+    int device;
+    int compliant;
+    double psnr_estimate;
+    uint64_t reserved[4];
+} gail_packet;
+GAIL_API void gail_packet_free(gail_packet* pkt);
+
+/* gail.h:267-272 */
+typedef struct gail_frame {
+    const void* data[3];
+    ptrdiff_t stride[3];
+    void* opaque;
+    uint64_t reserved[4];
+} gail_frame;
+
+typedef void (*gail_packet_cb)(void* user, const gail_packet* pkt);
+typedef void (*gail_log_cb)(void* user, gail_log_level level, const char* message);
+
+/* gail.h:277-321 */
+typedef struct gail_config {
+    size_t struct_size;
+    gail_profile profile;
+    gail_container container;
+    double fps;
+    double max_mbps;
+    int stereo;
+    gail_input input;
+    gail_color color;
+    gail_backend backend;
+    const int* devices;
+    int num_devices;
+    int frames_in_flight;
+    int check;
+    gail_packet_cb on_packet;
+    void* user;
+    gail_log_cb log;
+    void* log_user;
+    gail_log_level log_level;
+    const char* const* device_ids;
+    int num_device_ids;
+    uint64_t max_device_memory;
+    int cpu_threads;
+    int low_priority;
+} gail_config;
+
+GAIL_API void gail_config_init(gail_config* c, size_t struct_size);
+static inline void gail_config_defaults(gail_config* c) { gail_config_init(c, sizeof(gail_config)); }
+
+typedef struct gail_encoder gail_encoder;
+
+/* gail.h:329-362 */
+GAIL_API int gail_encoder_create(const gail_config* c, gail_encoder** encoder);
+GAIL_API void gail_encoder_destroy(gail_encoder* e);
+GAIL_API void gail_encoder_frame_size(const gail_encoder* e, int* width, int* height);
+GAIL_API const char* gail_encoder_devices(const gail_encoder* e);
+GAIL_API int gail_encoder_send(gail_encoder* e, const gail_frame* frame);
+GAIL_API int gail_encoder_receive(gail_encoder* e, gail_packet* pkt, int wait);
+GAIL_API int gail_encoder_encode(gail_encoder* e, const gail_frame* frame, gail_packet* pkt);
+GAIL_API int gail_encoder_finish(gail_encoder* e);
+GAIL_API void gail_encoder_cancel(gail_encoder* e);
+GAIL_API const char* gail_encoder_error(const gail_encoder* e);
+
+/* gail.h:364-383 */
+typedef struct gail_device_info {
+    gail_backend backend;
+    int index;
+    char name[256];
+    char uuid[48];
+    char pci_bus_id[32];
+    uint64_t memory_bytes;
+    uint64_t reserved[8];
+} gail_device_info;
+GAIL_API int gail_enumerate_devices(gail_backend backend, gail_device_info* devices, int max_devices);
+
+/* ---- helpers of this build (no counterpart in gail.h) --------------------------------- */
+struct GailHipSubband;
+/* makeLayout (layout.cpp:181-210) as plain subbands; returns the code-block count. */
+GAIL_API int gail_layout_make(int width, int height, int fourk, double base_step, struct GailHipSubband* bands,
+                              int cap, int* nbands);
+/* The code-blocks of that layout as (band, x0, y0, w, h) rows. */
+GAIL_API int gail_layout_cblks(int width, int height, int fourk, int* out5, int cap);
+/* Rate control + Tier-2 of one frame's Tier-1 output (pipeline.cpp:210-226). */
+GAIL_API int64_t gail_encode_codestream(int width, int height, int fourk, double base_step, const uint32_t* info5,
+                                        const uint32_t* pass_len, const int32_t* pass_dist, const uint8_t* data,
+                                        uint64_t num_blocks, double fps, double mbps, uint8_t* out, uint64_t cap,
+                                        double* lambda, double* mse, const char* tag);
+/* DCI structural check (codestream_check.cpp:43-280); 0 when it passes. */
+GAIL_API int gail_check_codestream(const uint8_t* data, uint64_t size, double fps, int fourk, char* msg, int cap);
+GAIL_API uint64_t gail_empty_codestream_bytes(int width, int height, int fourk, double base_step);
+/* The seeded synthetic test frame: three planes of 12-bit X'Y'Z' codes. */
+GAIL_API void gail_synth_xyz12(uint64_t seed, int width, int height, uint16_t* out, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -202,6 +202,11 @@ struct GailHipPlugin {
     int (*batch_wait)(GailHipBatch* b, GailHipTier1* outs, GailHipFrameTimings* timings);
     /* Debug taps of slot i after a run: planar codes, wavelet planes, coefficient words. */
     int (*batch_debug)(GailHipBatch* b, int i, GailHipDebug* debug);
+    /* Device time (ms, HIP events on the batch's stream) of the last run's stages:
+     * [0] upload, [1] colour, [2] DWT (all levels), [3] DWT level 0, [4] context modelling,
+     * [5] ordering + MQ, [6] compaction, [7] download, [8] whole run; [9] Tier-1 symbols,
+     * [10] codeword bytes, [11] re-runs after a buffer overflow. Fills up to n values. */
+    int (*batch_stage_ms)(GailHipBatch* b, double* out, int n);
 };
 typedef struct GailHipPlugin GailHipPlugin;
 

@@ -127,6 +127,7 @@ class Plugin(C.Structure):
         ("batch_launch", _F(C.c_int, _vp, C.c_int)),
         ("batch_wait", _F(C.c_int, _vp, C.POINTER(Tier1), C.POINTER(FrameTimings))),
         ("batch_debug", _F(C.c_int, _vp, C.c_int, C.POINTER(Debug))),
+        ("batch_stage_ms", _F(C.c_int, _vp, C.POINTER(C.c_double), C.c_int)),
     ]
 
 
@@ -153,6 +154,14 @@ def host():
         h.gail_layout_cblks.argtypes = [C.c_int, C.c_int, C.c_int, _vp, C.c_int]
         h.gail_synth_xyz12.restype = None
         h.gail_synth_xyz12.argtypes = [C.c_uint64, C.c_int, C.c_int, _vp, C.c_int]
+        h.gail_encode_codestream.restype = C.c_int64
+        h.gail_encode_codestream.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, _vp, C.c_uint64,
+                                             C.c_double, C.c_double, _vp, C.c_uint64, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_double), C.c_char_p]
+        h.gail_check_codestream.restype = C.c_int
+        h.gail_check_codestream.argtypes = [_vp, C.c_uint64, C.c_double, C.c_int, C.c_char_p, C.c_int]
+        h.gail_empty_codestream_bytes.restype = C.c_uint64
+        h.gail_empty_codestream_bytes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double]
         _host = h
     return _host
 
@@ -358,7 +367,52 @@ class Batch:
         _check(plugin().batch_debug(self.handle, i, C.byref(dbg)), "batch_debug")
         return _debug_arrays(dbg, self.layouts[i])
 
+    STAGES = ("upload", "color", "dwt", "dwt0", "t1cm", "t1mq", "compact", "download", "gpu", "symbols", "bytes",
+              "reruns")
+
+    def stage_ms(self):
+        v = (C.c_double * 12)()
+        _check(plugin().batch_stage_ms(self.handle, v, 12), "batch_stage_ms")
+        return dict(zip(self.STAGES, list(v)))
+
     def close(self):
         if self.handle:
             plugin().batch_destroy(self.handle)
             self.handle = None
+
+
+def _t1_arrays(t1):
+    """(info5, pass_len, pass_dist, data) of a Tier1Result or an oracle frame dict."""
+    if isinstance(t1, dict):
+        return (np.ascontiguousarray(t1["info"], np.uint32), np.ascontiguousarray(t1["pass_len"], np.uint32),
+                np.ascontiguousarray(t1["pass_dist"], np.int32), np.ascontiguousarray(t1["data"], np.uint8))
+    return t1.info, t1.pass_len, t1.pass_dist, t1.data
+
+
+def encode_codestream(layout, t1, fps=24.0, mbps=250.0, tag="hip7.2", build=True):
+    """Rate control + Tier-2 of one frame (libgail.so). Returns (codestream bytes or None,
+    lambda, estimated mse)."""
+    info, pl, pd, data = _t1_arrays(t1)
+    lam, mse = C.c_double(0), C.c_double(0)
+    cap = int(fps and 250e6 / (8 * fps)) + 4096 if build else 0
+    out = np.zeros(max(cap, 1), np.uint8) if build else None
+    n = host().gail_encode_codestream(layout.width, layout.height, layout.fourk, layout.base_step,
+                                      info.ctypes.data_as(_vp), pl.ctypes.data_as(_vp), pd.ctypes.data_as(_vp),
+                                      data.ctypes.data_as(_vp), len(info), fps, mbps,
+                                      out.ctypes.data_as(_vp) if build else None, cap, C.byref(lam), C.byref(mse),
+                                      tag.encode())
+    if n < 0:
+        raise GailError(int(n), "rate control / Tier-2 failed")
+    return (bytes(out[:n]) if build else None), lam.value, mse.value
+
+
+def rate_lambda(layout, t1, fps=24.0, mbps=250.0):
+    return encode_codestream(layout, t1, fps, mbps, build=False)[1]
+
+
+def check_codestream(cs, fps, fourk):
+    """'' when the codestream passes the DCI structural check, else the reason."""
+    buf = np.frombuffer(cs, np.uint8)
+    msg = C.create_string_buffer(256)
+    host().gail_check_codestream(buf.ctypes.data_as(_vp), len(buf), fps, int(fourk), msg, 256)
+    return msg.value.decode()

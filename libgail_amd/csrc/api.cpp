@@ -1,0 +1,738 @@
+// The encoder half of the GAIL library API (include/gail_amd.h) on the HIP backend plugin,
+// with a single-device frame loop in place of the reference's multi-device pull scheduler
+// (pipeline.cpp:42-292): frames handed over by the app go into a bounded queue; one worker
+// thread per frame slot (an encoder of the plugin) codes Tier-1 on the GPU, then runs rate
+// control and Tier-2 on the CPU; a writer thread delivers codestreams in send order to the
+// waiter, the callback or the receive queue (gail_api.cpp:70-134, 323-417, 566-806).
+#include <dlfcn.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <condition_variable>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "codestream.h"
+#include "gail_amd.h"
+#include "gail_hip_plugin.h"
+#include "layout.h"
+
+using namespace gail_amd;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int status, const std::string& msg) {
+    g_error = msg;
+    return status;
+}
+
+struct ApiError : std::runtime_error {
+    int status;
+    ApiError(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+// ---- the plugin, loaded by full path from this library's own directory (cuda_host.cpp:82-93)
+struct PluginHandle {
+    const GailHipPlugin* table = nullptr;
+    std::string error;
+};
+
+const PluginHandle& loadPlugin() {
+    static PluginHandle h;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        Dl_info info;
+        if (!dladdr((void*)&loadPlugin, &info) || !info.dli_fname) {
+            h.error = "cannot locate libgail.so";
+            return;
+        }
+        std::string dir = info.dli_fname;
+        dir = dir.substr(0, dir.find_last_of('/') + 1);
+        void* lib = dlopen((dir + "libgail_hip.so").c_str(), RTLD_NOW | RTLD_LOCAL);
+        if (!lib) {
+            h.error = std::string("HIP backend plugin not loaded: ") + dlerror();
+            return;
+        }
+        auto get = (const GailHipPlugin* (*)(uint32_t))dlsym(lib, GAIL_HIP_PLUGIN_ENTRY);
+        const GailHipPlugin* t = get ? get(GAIL_HIP_PLUGIN_ABI) : nullptr;
+        if (!t || t->abi != GAIL_HIP_PLUGIN_ABI || t->size != sizeof(GailHipPlugin) ||
+            t->layout != gail_hip_plugin_layout()) {
+            h.error = "libgail_hip.so is from another build";
+            return;
+        }
+        h.table = t;
+    });
+    return h;
+}
+
+bool allZero(const uint64_t* p, int n) {
+    for (int i = 0; i < n; ++i)
+        if (p[i]) return false;
+    return true;
+}
+
+struct PlanarInfo {
+    int bits, sx, sy, rgb;
+};
+
+bool planarInfo(gail_pixel_format f, PlanarInfo& p) {
+    switch (f) {
+    case GAIL_PIX_YUV420P: p = {8, 1, 1, 0}; return true;
+    case GAIL_PIX_YUV420P10: p = {10, 1, 1, 0}; return true;
+    case GAIL_PIX_YUV420P12: p = {12, 1, 1, 0}; return true;
+    case GAIL_PIX_YUV422P: p = {8, 1, 0, 0}; return true;
+    case GAIL_PIX_YUV422P10: p = {10, 1, 0, 0}; return true;
+    case GAIL_PIX_YUV422P12: p = {12, 1, 0, 0}; return true;
+    case GAIL_PIX_YUV444P: p = {8, 0, 0, 0}; return true;
+    case GAIL_PIX_YUV444P10: p = {10, 0, 0, 0}; return true;
+    case GAIL_PIX_YUV444P12: p = {12, 0, 0, 0}; return true;
+    case GAIL_PIX_GBRP: p = {8, 0, 0, 1}; return true;
+    case GAIL_PIX_GBRP10: p = {10, 0, 0, 1}; return true;
+    case GAIL_PIX_GBRP12: p = {12, 0, 0, 1}; return true;
+    case GAIL_PIX_GBRP16: p = {16, 0, 0, 1}; return true;
+    default: return false;
+    }
+}
+
+struct Waiter {
+    std::mutex m;
+    std::condition_variable cv;
+    bool done = false;
+    int status = GAIL_OK;
+    gail_packet pkt{};
+};
+
+struct Item {
+    long long index = 0;
+    std::vector<uint8_t> buf;
+    void* opaque = nullptr;
+    Waiter* waiter = nullptr;
+};
+
+struct Done {
+    std::vector<uint8_t> cs;
+    void* opaque = nullptr;
+    Waiter* waiter = nullptr;
+    bool compliant = true;
+    double mse = 0;
+};
+
+}  // namespace
+
+struct gail_encoder {
+    gail_config cfg{};
+    gail_color color{};
+    const GailHipPlugin* P = nullptr;
+    GailHipDevice* dev = nullptr;
+    std::vector<GailHipEncoder*> slots;
+    Layout L;
+    std::vector<GailHipSubband> bands;
+    GailHipLayout plainLayout{};
+    GailHipEncodeSettings settings{};
+    RateLimits limits;
+    int width = 0, height = 0, picW = 0, picH = 0, ox = 0, oy = 0;
+    size_t inBytes = 0;
+    std::string devices, error;
+    std::atomic<bool> failed{false}, canceled{false};
+    std::atomic<double> lastLambda{0.0};
+    double slopeFactor = 0.25;  // gail_api.cpp:607
+
+    // input queue (bounded)
+    std::mutex qm;
+    std::condition_variable qcv;
+    std::deque<Item> queue;
+    size_t qcap = 4;
+    long long nextIndex = 0;
+    bool ended = false;
+    // results
+    std::mutex om;
+    std::condition_variable ocv;
+    std::map<long long, Done> pending;
+    std::deque<gail_packet> out;
+    long long nextOut = 0;
+    int activeWorkers = 0;
+    bool writerDone = false;
+    std::vector<std::thread> workers;
+    std::thread writer;
+
+    void log(gail_log_level lvl, const char* fmt, ...) {
+        if (!cfg.log || lvl > cfg.log_level) return;
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        std::vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        cfg.log(cfg.log_user, lvl, buf);
+    }
+
+    void stop(const std::string& why) {
+        {
+            std::lock_guard<std::mutex> l(om);
+            if (!failed.exchange(true)) error = why;
+        }
+        qcv.notify_all();
+        ocv.notify_all();
+    }
+
+    // Copies one app frame into the plugin's input layout (gail_api.cpp:363-396).
+    void fill(const gail_frame& f, uint8_t* dst) const {
+        const gail_pixel_format fmt = cfg.input.format;
+        PlanarInfo pi;
+        if (planarInfo(fmt, pi)) {
+            const size_t bps = pi.bits > 8 ? 2 : 1;
+            const int cw = (picW + pi.sx) >> pi.sx, ch = (picH + pi.sy) >> pi.sy;
+            uint8_t* plane = dst;
+            for (int c = 0; c < 3; ++c) {
+                const int w = c ? cw : picW, h = c ? ch : picH;
+                const size_t row = (size_t)w * bps;
+                const ptrdiff_t stride = f.stride[c] ? f.stride[c] : (ptrdiff_t)row;
+                const uint8_t* s = static_cast<const uint8_t*>(f.data[c]);
+                for (int y = 0; y < h; ++y) std::memcpy(plane + y * row, s + y * stride, row);
+                plane += row * h;
+            }
+            return;
+        }
+        if (fmt == GAIL_PIX_RGB48) {
+            const size_t row = (size_t)picW * 6;
+            const ptrdiff_t stride = f.stride[0] ? f.stride[0] : (ptrdiff_t)row;
+            for (int y = 0; y < picH; ++y)
+                std::memcpy(dst + ((size_t)(y + oy) * width + ox) * 6, (const uint8_t*)f.data[0] + y * stride, row);
+            return;
+        }
+        // X'Y'Z': planar 12-bit codes, the picture centred, code 0 around it
+        uint16_t* planes[3] = {(uint16_t*)dst, (uint16_t*)dst + (size_t)width * height,
+                               (uint16_t*)dst + 2 * (size_t)width * height};
+        for (int y = 0; y < picH; ++y) {
+            const size_t o = (size_t)(y + oy) * width + ox;
+            if (fmt == GAIL_PIX_XYZ12_PLANAR) {
+                for (int c = 0; c < 3; ++c) {
+                    const ptrdiff_t stride = f.stride[c] ? f.stride[c] : (ptrdiff_t)picW * 2;
+                    const uint16_t* s = (const uint16_t*)((const uint8_t*)f.data[c] + y * stride);
+                    for (int x = 0; x < picW; ++x) planes[c][o + x] = (uint16_t)std::min<uint32_t>(s[x], 4095u);
+                }
+                continue;
+            }
+            const ptrdiff_t stride = f.stride[0] ? f.stride[0] : (ptrdiff_t)picW * 6;
+            const uint16_t* s = (const uint16_t*)((const uint8_t*)f.data[0] + y * stride);
+            const bool msb = fmt == GAIL_PIX_XYZ12_MSB;
+            for (int x = 0; x < picW; ++x)
+                for (int c = 0; c < 3; ++c) {
+                    const uint32_t v = msb ? s[3 * x + c] >> 4 : s[3 * x + c];
+                    planes[c][o + x] = (uint16_t)std::min<uint32_t>(v, 4095u);
+                }
+        }
+    }
+
+    int push(const gail_frame* f, Waiter* w) {
+        if (canceled.load()) return fail(GAIL_ERROR_CANCELLED, "canceled");
+        if (failed.load()) return fail(GAIL_ERROR_FAILED, error);
+        if (!allZero(f->reserved, 4)) return fail(GAIL_ERROR_INVALID, "gail_frame.reserved must be zero");
+        PlanarInfo pi;
+        const int planes = (planarInfo(cfg.input.format, pi) || cfg.input.format == GAIL_PIX_XYZ12_PLANAR) ? 3 : 1;
+        for (int c = 0; c < planes; ++c)
+            if (!f->data[c]) return fail(GAIL_ERROR_INVALID, "frame plane missing");
+        Item it;
+        it.buf.assign(inBytes, 0);
+        fill(*f, it.buf.data());
+        it.opaque = f->opaque;
+        it.waiter = w;
+        std::unique_lock<std::mutex> l(qm);
+        qcv.wait(l, [&] { return queue.size() < qcap || failed.load() || canceled.load() || ended; });
+        if (canceled.load()) return fail(GAIL_ERROR_CANCELLED, "canceled");
+        if (failed.load()) return fail(GAIL_ERROR_FAILED, error);
+        if (ended && !w) return fail(GAIL_ERROR_STATE, "the stream has already ended");
+        it.index = nextIndex++;
+        queue.push_back(std::move(it));
+        qcv.notify_all();
+        return GAIL_OK;
+    }
+
+    void workerLoop(GailHipEncoder* slot) {
+        const std::string tag = "hip";
+        for (;;) {
+            Item it;
+            {
+                std::unique_lock<std::mutex> l(qm);
+                qcv.wait(l, [&] { return !queue.empty() || ended || failed.load() || canceled.load(); });
+                if (failed.load() || canceled.load() || queue.empty()) break;
+                it = std::move(queue.front());
+                queue.pop_front();
+                qcv.notify_all();
+            }
+            Done d;
+            d.opaque = it.opaque;
+            d.waiter = it.waiter;
+            try {
+                const double thr = lastLambda.load() * slopeFactor;
+                P->encoder_set_slope(slot, thr);
+                GailHipTier1 t1{};
+                if (P->encoder_encode(slot, it.buf.data(), &t1, nullptr, nullptr) != GAIL_HIP_OK)
+                    throw std::runtime_error(P->last_error());
+                Tier1Ref ref{(const uint32_t*)t1.info, t1.passLen, t1.passDist, t1.data, (size_t)t1.numInfo};
+                RateResult rr = allocateRate(L, ref, limits);
+                double lam = std::min({rr.lambdaComp[0], rr.lambdaComp[1], rr.lambdaComp[2]});
+                if (thr > 0 && lam < thr) {
+                    // an easier frame than the last: passes the rate control would keep may
+                    // have been skipped; code it again without early termination
+                    P->encoder_set_slope(slot, 0.0);
+                    if (P->encoder_encode(slot, it.buf.data(), &t1, nullptr, nullptr) != GAIL_HIP_OK)
+                        throw std::runtime_error(P->last_error());
+                    ref = Tier1Ref{(const uint32_t*)t1.info, t1.passLen, t1.passDist, t1.data, (size_t)t1.numInfo};
+                    rr = allocateRate(L, ref, limits);
+                    lam = std::min({rr.lambdaComp[0], rr.lambdaComp[1], rr.lambdaComp[2]});
+                }
+                buildCodestream(L, ref, rr.passes.data(), &d.cs, nullptr, tag);
+                lastLambda.store(lam);
+                d.mse = rr.distortion / ((double)width * height * 3);
+                if (cfg.check) {
+                    const std::string why = checkDciCodestream(d.cs.data(), d.cs.size(), cfg.fps * (cfg.stereo ? 2 : 1),
+                                                               cfg.profile == GAIL_PROFILE_4K);
+                    d.compliant = why.empty();
+                    if (!d.compliant) log(GAIL_LOG_WARNING, "frame %lld failed the DCI check: %s", it.index, why.c_str());
+                }
+            } catch (const std::exception& e) {
+                stop(e.what());
+                if (it.waiter) finishWaiter(it.waiter, GAIL_ERROR_FAILED, nullptr);
+                break;
+            }
+            {
+                std::lock_guard<std::mutex> l(om);
+                pending.emplace(it.index, std::move(d));
+            }
+            ocv.notify_all();
+        }
+        {
+            std::lock_guard<std::mutex> l(om);
+            --activeWorkers;
+        }
+        ocv.notify_all();
+    }
+
+    static void finishWaiter(Waiter* w, int status, const gail_packet* p) {
+        std::lock_guard<std::mutex> l(w->m);
+        w->status = status;
+        if (p) w->pkt = *p;
+        w->done = true;
+        w->cv.notify_all();
+    }
+
+    void writerLoop() {
+        for (;;) {
+            Done d;
+            long long idx;
+            {
+                std::unique_lock<std::mutex> l(om);
+                ocv.wait(l, [&] { return pending.count(nextOut) || activeWorkers == 0 || failed.load() || canceled.load(); });
+                if (failed.load() || canceled.load()) break;
+                auto it = pending.find(nextOut);
+                if (it == pending.end()) break;
+                d = std::move(it->second);
+                pending.erase(it);
+                idx = nextOut++;
+            }
+            gail_packet p{};
+            p.size = d.cs.size();
+            p.index = idx;
+            p.opaque = d.opaque;
+            p.device = 0;
+            p.compliant = d.compliant ? 1 : 0;
+            p.psnr_estimate = d.mse > 0 ? 10 * std::log10(4095.0 * 4095.0 / d.mse) : 99.0;
+            if (!d.waiter && cfg.on_packet) {
+                p.data = d.cs.data();
+                cfg.on_packet(cfg.user, &p);
+                continue;
+            }
+            p.data = (uint8_t*)std::malloc(std::max<size_t>(1, d.cs.size()));
+            if (!p.data) {
+                stop("out of host memory");
+                break;
+            }
+            std::memcpy(p.data, d.cs.data(), d.cs.size());
+            if (d.waiter) {
+                finishWaiter(d.waiter, GAIL_OK, &p);
+                continue;
+            }
+            std::lock_guard<std::mutex> l(om);
+            out.push_back(p);
+            ocv.notify_all();
+        }
+        // wake blocked waiters of frames that will not be delivered
+        std::lock_guard<std::mutex> l(om);
+        for (auto& kv : pending)
+            if (kv.second.waiter) finishWaiter(kv.second.waiter, canceled.load() ? GAIL_ERROR_CANCELLED : GAIL_ERROR_FAILED, nullptr);
+        writerDone = true;
+        ocv.notify_all();
+    }
+
+    void shutdown() {
+        {
+            std::lock_guard<std::mutex> l(qm);
+            ended = true;
+        }
+        qcv.notify_all();
+        for (auto& t : workers)
+            if (t.joinable()) t.join();
+        ocv.notify_all();
+        if (writer.joinable()) writer.join();
+        for (auto* s : slots) P->encoder_destroy(s);
+        slots.clear();
+        if (dev) P->device_destroy(dev);
+        dev = nullptr;
+        std::lock_guard<std::mutex> l(om);
+        for (auto& p : out) std::free(p.data);
+        out.clear();
+    }
+};
+
+extern "C" {
+
+GAIL_API const char* gail_last_error(void) { return g_error.c_str(); }
+
+GAIL_API const char* gail_status_string(int s) {
+    switch (s) {
+    case GAIL_OK: return "ok";
+    case GAIL_EAGAIN: return "try again";
+    case GAIL_EOF: return "end of stream";
+    case GAIL_ERROR_INVALID: return "invalid argument";
+    case GAIL_ERROR_UNSUPPORTED: return "unsupported";
+    case GAIL_ERROR_NOMEM: return "out of memory";
+    case GAIL_ERROR_FAILED: return "failed";
+    case GAIL_ERROR_STATE: return "not allowed now";
+    case GAIL_ERROR_CANCELLED: return "cancelled";
+    default: return "unknown status";
+    }
+}
+
+GAIL_API unsigned gail_version(void) { return GAIL_VERSION; }
+GAIL_API const char* gail_version_string(void) { return "1.1.0 (hip; gail_hip)"; }
+
+GAIL_API int gail_container_size(gail_profile p, gail_container c, int* w, int* h) {
+    if ((p != GAIL_PROFILE_2K && p != GAIL_PROFILE_4K) || c < GAIL_CONTAINER_FLAT || c > GAIL_CONTAINER_FULL)
+        return fail(GAIL_ERROR_INVALID, "invalid profile or container");
+    int W, H;
+    containerSize(p == GAIL_PROFILE_4K, (int)c, W, H);
+    if (w) *w = W;
+    if (h) *h = H;
+    return GAIL_OK;
+}
+
+GAIL_API void gail_color_preset(gail_preset preset, gail_color* c) {
+    std::memset(c, 0, sizeof *c);
+    c->mode = GAIL_COLOR_MANAGED;
+    c->transfer = GAIL_TRANSFER_BT1886;
+    c->primaries = GAIL_PRIMARIES_BT709;
+    c->tone_map = GAIL_TONE_MAP_SIGNAL;
+    c->tone_norm = GAIL_TONE_NORM_CHANNEL;
+    c->sdr_white = 48.0;
+    c->hdr_peak = 1000.0;
+    c->hdr_ref_white = 203.0;
+    c->cinema_ref_white = 16.0;
+    c->cinema_peak = 48.0;
+    c->knee = 0.25;
+    c->gamut = GAIL_GAMUT_NONE;
+    c->gamut_threshold = 0.8;
+    switch (preset) {
+    case GAIL_PRESET_SDR_BT2020: c->primaries = GAIL_PRIMARIES_BT2020; break;
+    case GAIL_PRESET_HDR_PQ_BT2020:
+        c->transfer = GAIL_TRANSFER_PQ;
+        c->primaries = GAIL_PRIMARIES_BT2020;
+        break;
+    case GAIL_PRESET_HDR_HLG_BT2020:
+        c->transfer = GAIL_TRANSFER_HLG;
+        c->primaries = GAIL_PRIMARIES_BT2020;
+        break;
+    default: break;
+    }
+}
+
+GAIL_API void gail_config_init(gail_config* c, size_t size) {
+    gail_config d;
+    std::memset(&d, 0, sizeof d);
+    d.struct_size = std::min(size, sizeof d);
+    d.profile = GAIL_PROFILE_2K;
+    d.container = GAIL_CONTAINER_FLAT;
+    d.fps = 24.0;
+    d.max_mbps = 230.0;
+    d.input.format = GAIL_PIX_RGB48;
+    gail_color_preset(GAIL_PRESET_SDR_BT709, &d.color);
+    d.backend = GAIL_BACKEND_AUTO;
+    d.check = 1;
+    d.log_level = GAIL_LOG_WARNING;
+    std::memcpy(c, &d, std::min(size, sizeof d));
+}
+
+GAIL_API void gail_packet_free(gail_packet* p) {
+    if (!p) return;
+    std::free(p->data);
+    p->data = nullptr;
+    p->size = 0;
+}
+
+GAIL_API int gail_encoder_create(const gail_config* c, gail_encoder** encoder) {
+    if (!c || !encoder) return fail(GAIL_ERROR_INVALID, "null argument");
+    *encoder = nullptr;
+    auto e = std::make_unique<gail_encoder>();
+    try {
+        gail_config_init(&e->cfg, sizeof e->cfg);
+        std::memcpy(&e->cfg, c, std::min(c->struct_size ? c->struct_size : sizeof *c, sizeof e->cfg));
+        gail_config& cfg = e->cfg;
+        if (cfg.backend != GAIL_BACKEND_AUTO && cfg.backend != GAIL_BACKEND_HIP)
+            throw ApiError(GAIL_ERROR_UNSUPPORTED, "this library has the HIP backend only");
+        if (!(cfg.fps > 0) || !(cfg.max_mbps > 0) || cfg.max_mbps > 250.0)
+            throw ApiError(GAIL_ERROR_INVALID, "fps or max_mbps out of range");
+        if (!allZero(cfg.input.reserved, 4) || !allZero(cfg.color.reserved, 8))
+            throw ApiError(GAIL_ERROR_INVALID, "reserved fields must be zero");
+        if (cfg.color.mode == GAIL_COLOR_LUT3D)
+            throw ApiError(GAIL_ERROR_UNSUPPORTED, "3D LUT colour is not supported by the HIP backend");
+        if (gail_container_size(cfg.profile, cfg.container, &e->width, &e->height) != GAIL_OK)
+            throw ApiError(GAIL_ERROR_INVALID, "invalid profile or container");
+        e->picW = cfg.input.width ? cfg.input.width : e->width;
+        e->picH = cfg.input.height ? cfg.input.height : e->height;
+        if (e->picW <= 0 || e->picH <= 0 || e->picW > e->width || e->picH > e->height)
+            throw ApiError(GAIL_ERROR_INVALID, "picture larger than the container");
+        e->ox = (e->width - e->picW) / 2;
+        e->oy = (e->height - e->picH) / 2;
+        const bool fourk = cfg.profile == GAIL_PROFILE_4K;
+        e->L = makeLayout(e->width, e->height, fourk, 1.0);
+        for (const Subband& b : e->L.bands) {
+            GailHipSubband s{};
+            s.comp = b.comp;
+            s.res = b.res;
+            s.level = b.level;
+            s.orient = b.orient;
+            s.px0 = b.px0;
+            s.py0 = b.py0;
+            s.w = b.w;
+            s.h = b.h;
+            s.ncbx = b.ncbx;
+            s.ncby = b.ncby;
+            s.cblkBase = b.cblkBase;
+            s.eps = b.eps;
+            s.mu = b.mu;
+            s.Mb = b.Mb;
+            s.delta = b.delta;
+            s.weight = b.weight;
+            e->bands.push_back(s);
+        }
+        e->plainLayout = GailHipLayout{fourk ? 1 : 0, e->width, e->height, e->L.levels, e->L.guardBits,
+                                       (int)e->bands.size(), e->L.numCblks(), 1.0, e->bands.data()};
+        GailHipEncodeSettings& s = e->settings;
+        s.profile = fourk ? 1 : 0;
+        s.width = e->width;
+        s.height = e->height;
+        s.baseStep = 1.0;
+        s.slopeFactor = e->slopeFactor;
+        s.bitsPerSymbol = 0.5;
+        s.input.width = e->width;
+        s.input.height = e->height;
+        PlanarInfo pi;
+        const gail_pixel_format fmt = cfg.input.format;
+        if (planarInfo(fmt, pi)) {
+            s.input.kind = 1;
+            s.input.activeW = e->picW;
+            s.input.activeH = e->picH;
+            s.input.ox = e->ox;
+            s.input.oy = e->oy;
+            s.input.sx = pi.sx;
+            s.input.sy = pi.sy;
+            s.input.rgb = pi.rgb;
+            s.input.bits = pi.bits;
+            s.input.fullRange = pi.rgb ? 1 : (cfg.input.full_range ? 1 : 0);
+            s.input.chroma = (int)cfg.input.chroma_location;
+            const double kr[3] = {0.2126, 0.2627, 0.299}, kb[3] = {0.0722, 0.0593, 0.114};
+            if (cfg.input.matrix < GAIL_MATRIX_BT709 || cfg.input.matrix > GAIL_MATRIX_BT601)
+                throw ApiError(GAIL_ERROR_INVALID, "invalid matrix");
+            s.input.kr = kr[cfg.input.matrix];
+            s.input.kb = kb[cfg.input.matrix];
+            const int cw = (e->picW + pi.sx) >> pi.sx, ch = (e->picH + pi.sy) >> pi.sy;
+            e->inBytes = ((size_t)e->picW * e->picH + 2 * (size_t)cw * ch) * (pi.bits > 8 ? 2 : 1);
+        } else if (fmt == GAIL_PIX_RGB48) {
+            s.input.kind = 0;
+            e->inBytes = (size_t)e->width * e->height * 6;
+        } else if (fmt == GAIL_PIX_XYZ12 || fmt == GAIL_PIX_XYZ12_MSB || fmt == GAIL_PIX_XYZ12_PLANAR) {
+            s.input.kind = 2;
+            e->inBytes = (size_t)e->width * e->height * 6;
+        } else {
+            throw ApiError(GAIL_ERROR_INVALID, "invalid pixel format");
+        }
+        const bool xyz = s.input.kind == 2;
+        s.color.transfer = xyz ? 5 : (int)cfg.color.transfer;
+        s.color.primaries = xyz ? 4 : (int)cfg.color.primaries;
+        s.color.sdrWhite = cfg.color.sdr_white;
+        s.color.cinemaPeak = cfg.color.cinema_peak;
+        if (!xyz && (cfg.color.transfer == GAIL_TRANSFER_PQ || cfg.color.transfer == GAIL_TRANSFER_HLG) &&
+            cfg.color.tone_map != GAIL_TONE_MAP_SIGNAL)
+            throw ApiError(GAIL_ERROR_UNSUPPORTED, "HDR tone mapping other than SIGNAL is not supported by the HIP backend");
+        const double eyeFps = cfg.fps * (cfg.stereo ? 2 : 1);
+        e->limits = dciLimits(eyeFps, cfg.max_mbps, 200.0);
+        if (emptyCodestreamBytes(e->L) > e->limits.maxTotal)
+            throw ApiError(GAIL_ERROR_INVALID, "the frame budget is below the codestream headers");
+
+        const PluginHandle& ph = loadPlugin();
+        if (!ph.table) throw ApiError(GAIL_ERROR_UNSUPPORTED, ph.error);
+        e->P = ph.table;
+        const int n = e->P->device_count();
+        if (n <= 0) throw ApiError(GAIL_ERROR_UNSUPPORTED, "no HIP device");
+        const int index = (cfg.devices && cfg.num_devices > 0) ? cfg.devices[0] : 0;
+        if (index < 0 || index >= n) throw ApiError(GAIL_ERROR_INVALID, "no such device");
+        if (e->P->device_usable(index) != GAIL_HIP_OK) throw ApiError(GAIL_ERROR_UNSUPPORTED, e->P->last_error());
+        if (e->P->device_create(index, nullptr, nullptr, &e->dev) != GAIL_HIP_OK)
+            throw ApiError(GAIL_ERROR_FAILED, e->P->last_error());
+        e->devices = e->P->device_name(e->dev);
+        // frames in flight: as asked (default 2), fewer when device memory is short
+        int want = cfg.frames_in_flight > 0 ? cfg.frames_in_flight : 2;
+        const uint64_t per = e->P->encoder_bytes(e->dev, &e->plainLayout, &e->settings);
+        GailHipMemory mem{};
+        if (per && e->P->device_memory(e->dev, &mem) == GAIL_HIP_OK && mem.budgeted) {
+            const uint64_t headroom = std::max<uint64_t>(512ull << 20, mem.total / 16);
+            uint64_t avail = mem.available > headroom ? mem.available - headroom : 0;
+            if (cfg.max_device_memory) avail = std::min<uint64_t>(avail, cfg.max_device_memory);
+            want = (int)std::max<uint64_t>(1, std::min<uint64_t>(want, avail / (per + per / 10)));
+        }
+        for (int i = 0; i < want; ++i) {
+            GailHipEncoder* slot = nullptr;
+            const int st = e->P->encoder_create(e->dev, &e->plainLayout, &e->settings, &slot);
+            if (st != GAIL_HIP_OK) {
+                if (e->slots.empty())
+                    throw ApiError(st == GAIL_HIP_DEVICE_MEMORY ? GAIL_ERROR_NOMEM : GAIL_ERROR_FAILED, e->P->last_error());
+                e->log(GAIL_LOG_WARNING, "frames in flight reduced to %d: %s", (int)e->slots.size(), e->P->last_error());
+                break;
+            }
+            e->slots.push_back(slot);
+        }
+        e->qcap = e->slots.size() + 2;
+        e->activeWorkers = (int)e->slots.size();
+        for (auto* slot : e->slots) e->workers.emplace_back([ep = e.get(), slot] { ep->workerLoop(slot); });
+        e->writer = std::thread([ep = e.get()] { ep->writerLoop(); });
+        *encoder = e.release();
+        return GAIL_OK;
+    } catch (const ApiError& x) {
+        e->shutdown();
+        return fail(x.status, x.what());
+    } catch (const std::exception& x) {
+        e->shutdown();
+        return fail(GAIL_ERROR_FAILED, x.what());
+    }
+}
+
+GAIL_API void gail_encoder_destroy(gail_encoder* e) {
+    if (!e) return;
+    e->shutdown();
+    delete e;
+}
+
+GAIL_API void gail_encoder_frame_size(const gail_encoder* e, int* w, int* h) {
+    if (w) *w = e ? e->width : 0;
+    if (h) *h = e ? e->height : 0;
+}
+
+GAIL_API const char* gail_encoder_devices(const gail_encoder* e) { return e ? e->devices.c_str() : ""; }
+
+GAIL_API int gail_encoder_send(gail_encoder* e, const gail_frame* f) {
+    if (!e) return fail(GAIL_ERROR_INVALID, "null encoder");
+    if (!f) {
+        {
+            std::lock_guard<std::mutex> l(e->qm);
+            e->ended = true;
+        }
+        e->qcv.notify_all();
+        return GAIL_OK;
+    }
+    {
+        std::lock_guard<std::mutex> l(e->qm);
+        if (e->ended) return fail(GAIL_ERROR_STATE, "the stream has already ended");
+    }
+    return e->push(f, nullptr);
+}
+
+GAIL_API int gail_encoder_receive(gail_encoder* e, gail_packet* pkt, int wait) {
+    if (!e || !pkt) return fail(GAIL_ERROR_INVALID, "null argument");
+    if (e->cfg.on_packet) return fail(GAIL_ERROR_STATE, "packets go to on_packet");
+    std::unique_lock<std::mutex> l(e->om);
+    for (;;) {
+        if (!e->out.empty()) {
+            *pkt = e->out.front();
+            e->out.pop_front();
+            return GAIL_OK;
+        }
+        if (e->canceled.load()) return fail(GAIL_ERROR_CANCELLED, "canceled");
+        if (e->failed.load()) return fail(GAIL_ERROR_FAILED, e->error);
+        if (e->writerDone) return GAIL_EOF;
+        if (!wait) return GAIL_EAGAIN;
+        e->ocv.wait(l);
+    }
+}
+
+GAIL_API int gail_encoder_encode(gail_encoder* e, const gail_frame* f, gail_packet* pkt) {
+    if (!e || !f || !pkt) return fail(GAIL_ERROR_INVALID, "null argument");
+    Waiter w;
+    const int r = e->push(f, &w);
+    if (r != GAIL_OK) return r;
+    std::unique_lock<std::mutex> l(w.m);
+    w.cv.wait(l, [&] { return w.done; });
+    if (w.status != GAIL_OK) return fail(w.status, e->failed.load() ? e->error : "canceled");
+    *pkt = w.pkt;
+    return GAIL_OK;
+}
+
+GAIL_API int gail_encoder_finish(gail_encoder* e) {
+    if (!e) return fail(GAIL_ERROR_INVALID, "null encoder");
+    {
+        std::lock_guard<std::mutex> l(e->qm);
+        e->ended = true;
+    }
+    e->qcv.notify_all();
+    std::unique_lock<std::mutex> l(e->om);
+    e->ocv.wait(l, [&] { return e->writerDone; });
+    if (e->canceled.load()) return fail(GAIL_ERROR_CANCELLED, "canceled");
+    if (e->failed.load()) return fail(GAIL_ERROR_FAILED, e->error);
+    return GAIL_OK;
+}
+
+GAIL_API void gail_encoder_cancel(gail_encoder* e) {
+    if (!e || e->canceled.exchange(true)) return;
+    {
+        std::lock_guard<std::mutex> l(e->qm);
+        e->queue.clear();
+    }
+    e->qcv.notify_all();
+    e->ocv.notify_all();
+}
+
+GAIL_API const char* gail_encoder_error(const gail_encoder* e) {
+    return e && e->failed.load() ? e->error.c_str() : nullptr;
+}
+
+GAIL_API int gail_enumerate_devices(gail_backend backend, gail_device_info* out, int max) {
+    if (backend != GAIL_BACKEND_AUTO && backend != GAIL_BACKEND_HIP) return 0;
+    const PluginHandle& ph = loadPlugin();
+    if (!ph.table) return 0;
+    const int n = ph.table->device_count();
+    for (int i = 0; i < n && i < max; ++i) {
+        GailHipDeviceDesc d{};
+        if (ph.table->device_desc(i, &d) != GAIL_HIP_OK) continue;
+        gail_device_info& o = out[i];
+        std::memset(&o, 0, sizeof o);
+        o.backend = GAIL_BACKEND_HIP;
+        o.index = i;
+        std::snprintf(o.name, sizeof o.name, "%s", d.name);
+        std::snprintf(o.pci_bus_id, sizeof o.pci_bus_id, "%s", d.pciBusId);
+        o.memory_bytes = d.memoryBytes;
+    }
+    return std::max(n, 0);
+}
+
+}  // extern "C"

@@ -55,7 +55,7 @@ struct JobCounters {
     unsigned long long sym_next;  // bump allocator of the symbol pool
     uint32_t flags;               // 1: symbol pool full, 2: codeword buffer full
     uint32_t pad;
-    unsigned long long out_total; // bytes of codeword capacity handed out
+    unsigned long long symbols;   // Tier-1 symbols of the job
     uint32_t pad2[10];
 };
 

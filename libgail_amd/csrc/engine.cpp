@@ -347,7 +347,7 @@ void Job::enqueueT1(hipStream_t s) {
         1, std::min<uint32_t>((pairs + waves - 1) / waves, (uint32_t)(ctx_.computeUnits() * opt_.cmGroupsPerCU)));
     hipCheck(launch_t1_cm(cm, waves, cmGroups, s), "context modelling");
     if (!capturing_) hipCheck(hipEventRecord(ev_[4], s), "event");
-    hipCheck(launch_sort_blocks(dInfo_, n, dHist_, dPerm_, s), "block ordering");
+    hipCheck(launch_sort_blocks(dInfo_, n, dHist_, dPerm_, dCounters_, s), "block ordering");
     hipCheck(launch_out_offsets(dInfo_, n, dOutOff_, dPartial_, s), "codeword offsets");
     MqParams mq{};
     mq.info = dInfo_;
@@ -435,9 +435,11 @@ void Job::enqueueChain(bool uploadInputs) {
         hipCheck(launch_color(c, s), "colour");
     }
     if (!capturing_) hipCheck(hipEventRecord(ev_[2], s), "event");
-    for (size_t l = 0; l < levelDescs_.size(); ++l)
+    for (size_t l = 0; l < levelDescs_.size(); ++l) {
         hipCheck(launch_dwt_level(l == 0, opt_.dwtTile, dLevelDescs_[l], (int)levelDescs_[l].size(), levelTiles_[l], s),
                  "DWT level");
+        if (l == 0 && !capturing_) hipCheck(hipEventRecord(ev_[8], s), "event");
+    }
     if (!capturing_) hipCheck(hipEventRecord(ev_[3], s), "event");
     enqueueT1(s);
 }
@@ -514,7 +516,7 @@ void Job::finish(StageTimes* times) {
             inf[5 * b + 4] -= p0;
         }
     }
-    if (times) {
+    {
         StageTimes t;
         auto ms = [&](int a, int b) {
             float v = 0;
@@ -524,6 +526,7 @@ void Job::finish(StageTimes* times) {
             t.upload = ms(0, 1);
             t.color = ms(1, 2);
             t.dwt = ms(2, 3);
+            t.dwt0 = ms(2, 8);
             t.t1cm = ms(3, 4);
             t.t1mq = ms(4, 5);
         }
@@ -531,8 +534,10 @@ void Job::finish(StageTimes* times) {
         t.download = ms(6, 7);
         t.gpu = ms(0, 7);
         t.bytes = (double)numDataTotal_;
+        t.symbols = (double)((const JobCounters*)hSmall_)->symbols;
         t.reruns = reruns;
-        *times = t;
+        last_ = t;
+        if (times) *times = t;
     }
 }
 

@@ -45,7 +45,7 @@ struct JobOptions {
 
 // Per-stage device time (ms) of the last run, from HIP events on the job's stream.
 struct StageTimes {
-    float upload = 0, color = 0, dwt = 0, t1cm = 0, t1mq = 0, compact = 0, download = 0, gpu = 0;
+    float upload = 0, color = 0, dwt = 0, dwt0 = 0, t1cm = 0, t1mq = 0, compact = 0, download = 0, gpu = 0;
     double symbols = 0, bytes = 0;
     int reruns = 0;
 };
@@ -111,6 +111,8 @@ public:
     void finish(StageTimes* times = nullptr);
 
     Tier1View result(int i) const;
+    // Stage times of the last run (ms) and its symbol count.
+    const StageTimes& lastTimes() const { return last_; }
 
     // Debug taps (synchronous): X'Y'Z' codes (3 planes), the DWT output (3 Mallat planes),
     // and the quantised coefficient words (1,024 per block) of frame i.
@@ -150,10 +152,11 @@ private:
     std::vector<void*> allocs_;
     size_t bytes_ = 0;
     hipStream_t stream_ = nullptr;
-    hipEvent_t ev_[8] = {};
+    hipEvent_t ev_[9] = {};  // ev_[8]: after DWT level 0
     hipGraphExec_t graph_ = nullptr;
     bool graphUpload_ = false;
     bool capturing_ = false;
+    StageTimes last_;
 
     uint32_t nblocks_ = 0;
     size_t totalCoefs_ = 0;

@@ -96,7 +96,8 @@ hipError_t launch_color(const ColorDesc& d, hipStream_t s);
 hipError_t t1_tables_init();
 hipError_t launch_clear_counters(JobCounters* c, uint32_t* work, int nwork, hipStream_t s);
 hipError_t launch_t1_cm(const CmParams& p, int waves_per_group, int groups, hipStream_t s);
-hipError_t launch_sort_blocks(const T1Info* info, uint32_t n, uint32_t* hist, uint32_t* perm, hipStream_t s);
+hipError_t launch_sort_blocks(const T1Info* info, uint32_t n, uint32_t* hist, uint32_t* perm, JobCounters* counters,
+                              hipStream_t s);
 hipError_t launch_out_offsets(const T1Info* info, uint32_t n, unsigned long long* out_off, unsigned long long* partial,
                               hipStream_t s);
 hipError_t launch_t1_mq(const MqParams& p, int group, int groups, hipStream_t s);

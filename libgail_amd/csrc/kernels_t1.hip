@@ -540,15 +540,25 @@ __device__ __forceinline__ uint32_t bucket_of(uint32_t s) {
 }
 }  // namespace
 
-__global__ void __launch_bounds__(256) sort_hist_kernel(const T1Info* info, uint32_t n, uint32_t* hist) {
+__global__ void __launch_bounds__(256) sort_hist_kernel(const T1Info* info, uint32_t n, uint32_t* hist,
+                                                        JobCounters* counters) {
     __shared__ uint32_t h[kBuckets];
+    __shared__ unsigned long long total;
     for (int i = threadIdx.x; i < kBuckets; i += 256) h[i] = 0;
+    if (threadIdx.x == 0) total = 0;
     __syncthreads();
     const uint32_t b0 = blockIdx.x * kSortSpan, b1 = min(n, b0 + kSortSpan);
-    for (uint32_t i = b0 + threadIdx.x; i < b1; i += 256) atomicAdd(&h[bucket_of(info[i].symbols)], 1u);
+    unsigned long long mine = 0;
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += 256) {
+        const uint32_t s = info[i].symbols;
+        mine += s;
+        atomicAdd(&h[bucket_of(s)], 1u);
+    }
+    atomicAdd(&total, mine);
     __syncthreads();
     for (int i = threadIdx.x; i < kBuckets; i += 256)
         if (h[i]) atomicAdd(&hist[i], h[i]);
+    if (threadIdx.x == 0) atomicAdd(&counters->symbols, total);
 }
 
 __global__ void sort_scan_kernel(uint32_t* hist) {
@@ -585,10 +595,11 @@ __global__ void zero_u32_kernel(uint32_t* p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
-hipError_t launch_sort_blocks(const T1Info* info, uint32_t n, uint32_t* hist, uint32_t* perm, hipStream_t s) {
+hipError_t launch_sort_blocks(const T1Info* info, uint32_t n, uint32_t* hist, uint32_t* perm, JobCounters* counters,
+                              hipStream_t s) {
     const uint32_t groups = (n + kSortSpan - 1) / kSortSpan;
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(128), 0, s, hist, kBuckets);
-    hipLaunchKernelGGL(sort_hist_kernel, dim3(groups), dim3(256), 0, s, info, n, hist);
+    hipLaunchKernelGGL(sort_hist_kernel, dim3(groups), dim3(256), 0, s, info, n, hist, counters);
     hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(64), 0, s, hist);
     hipLaunchKernelGGL(sort_scatter_kernel, dim3(groups), dim3(256), 0, s, info, n, hist, perm);
     return hipGetLastError();

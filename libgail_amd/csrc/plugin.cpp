@@ -342,6 +342,15 @@ int batchDebug(GailHipBatch* b, int i, GailHipDebug* dbg) {
     });
 }
 
+int batchStageMs(GailHipBatch* b, double* out, int n) {
+    return guarded([&] {
+        const StageTimes& t = b->job->lastTimes();
+        const double v[12] = {t.upload, t.color, t.dwt, t.dwt0, t.t1cm, t.t1mq, t.compact, t.download, t.gpu,
+                              t.symbols, t.bytes, (double)t.reruns};
+        for (int i = 0; i < n && i < 12; ++i) out[i] = v[i];
+    });
+}
+
 #define GAIL_STR2(x) #x
 #define GAIL_STR(x) GAIL_STR2(x)
 
@@ -380,6 +389,7 @@ const GailHipPlugin g_plugin = {
     batchLaunch,
     batchWait,
     batchDebug,
+    batchStageMs,
 };
 
 }  // namespace
