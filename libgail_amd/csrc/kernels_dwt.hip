@@ -36,7 +36,11 @@ constexpr float kY2[3] = {(float)0.5, (float)-0.41869, (float)-0.08131};
 
 // Whole-sample symmetric extension of index i into [0, n).
 __device__ __forceinline__ int mirror(int i, int n) {
+    if ((unsigned)i < (unsigned)n) return i;
     if (n == 1) return 0;
+    // one reflection covers every position within n - 1 of the signal
+    const int r = i < 0 ? -i : 2 * (n - 1) - i;
+    if ((unsigned)r < (unsigned)n) return r;
     const int period = 2 * (n - 1);
     i %= period;
     if (i < 0) i += period;

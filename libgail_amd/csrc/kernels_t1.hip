@@ -41,12 +41,14 @@ __device__ __forceinline__ void wave_sync() {
 
 // Value of the lane one column to the left / right within the same half-wave; zero at
 // the block's edges.
+// DPP wave_shr:1 / wave_shl:1 move a register one lane up / down across the whole wave in
+// one VALU operation; the lanes at a block's edge (x = 0 / 31) see zero.
 __device__ __forceinline__ uint32_t from_left(uint32_t v, int x) {
-    const uint32_t r = (uint32_t)__shfl_up((int)v, 1);
+    const uint32_t r = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, true);
     return x == 0 ? 0u : r;
 }
 __device__ __forceinline__ uint32_t from_right(uint32_t v, int x) {
-    const uint32_t r = (uint32_t)__shfl_down((int)v, 1);
+    const uint32_t r = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xF, 0xF, true);
     return x == 31 ? 0u : r;
 }
 
@@ -696,48 +698,6 @@ namespace {
 // its entry of this table with its MPS in bit 31.
 __constant__ uint32_t kMqTable[47];
 
-struct Mq {
-    uint32_t A, C;
-    int CT, bp;
-    uint32_t B;
-};
-
-__device__ __forceinline__ void mq_byteout(Mq& e, uint8_t* out, uint32_t cap, bool& overflow) {
-    bool stuff = e.B == 0xFFu;
-    if (!stuff && (e.C & 0x8000000u)) {
-        e.B += 1u;
-        if (e.B == 0xFFu) {
-            e.C &= 0x7FFFFFFu;
-            stuff = true;
-        }
-    }
-    if (e.bp >= 0) {
-        if ((uint32_t)e.bp < cap) out[e.bp] = (uint8_t)e.B;
-        else overflow = true;
-    }
-    ++e.bp;
-    if (stuff) {
-        e.B = e.C >> 20;
-        e.C &= 0xFFFFFu;
-        e.CT = 7;
-    } else {
-        e.B = (e.C >> 19) & 0xFFu;
-        e.C &= 0x7FFFFu;
-        e.CT = 8;
-    }
-}
-
-__device__ __forceinline__ void mq_renorm(Mq& e, uint8_t* out, uint32_t cap, bool& overflow) {
-    int n = __clz((int)e.A) - 16;
-    while (n > 0) {
-        const int s = n < e.CT ? n : e.CT;
-        e.A <<= s;
-        e.C <<= s;
-        e.CT -= s;
-        n -= s;
-        if (e.CT == 0) mq_byteout(e, out, cap, overflow);
-    }
-}
 }  // namespace
 
 template <int GROUP>
@@ -774,61 +734,126 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
         s_ctx[0][t] = s_tab[4];
         s_ctx[17][t] = s_tab[3];
         s_ctx[18][t] = s_tab[46];
-        Mq e{0x8000u, 0u, 12, -1, 0u};
+        uint32_t A = 0x8000u, C = 0u, B = 0u;
+        int CT = 12, bp = -1;
+        uint32_t word = 0;  // finished bytes of the current aligned 4-byte group
+        // BYTEOUT (C.2.7) with the pending byte B kept in a register: it is final once the
+        // next byte starts, and finished bytes go out four at a time.
+        auto byteout = [&]() {
+            bool stuff = B == 0xFFu;
+            if (!stuff && (C & 0x8000000u)) {
+                B += 1u;
+                if (B == 0xFFu) {
+                    C &= 0x7FFFFFFu;
+                    stuff = true;
+                }
+            }
+            if (bp >= 0) {
+                word |= B << (8 * (bp & 3));
+                if ((bp & 3) == 3) {
+                    if ((uint32_t)bp < cap) *(uint32_t*)(out + (bp & ~3)) = word;
+                    else overflow = true;
+                    word = 0;
+                }
+            }
+            ++bp;
+            if (stuff) {
+                B = C >> 20;
+                C &= 0xFFFFFu;
+                CT = 7;
+            } else {
+                B = (C >> 19) & 0xFFu;
+                C &= 0x7FFFFu;
+                CT = 8;
+            }
+        };
         const uint32_t* pend = P.pass_end + (size_t)cb * kMaxPasses;
         MqSnapshot* snaps = P.snaps + (size_t)cb * kMaxPasses;
         uint32_t k = 0, pos = 0, next_end = pend[0];
-        auto pass_ends = [&]() {
-            while (k < np && next_end == pos) {
-                snaps[k] = MqSnapshot{e.bp, e.B | ((uint32_t)e.CT << 16), e.C, e.A};
-                ++k;
-                next_end = k < np ? pend[k] : 0xFFFFFFFFu;
-            }
-        };
-        for (uint32_t j = 0; j < inf.planes; ++j) {
-            const uint2 ch = P.chunks[(size_t)cb * kMaxPlanes + j];
-            const uint4* src = (const uint4*)(P.pool + ((unsigned long long)ch.x << 4));
+        const uint2* chunk = P.chunks + (size_t)cb * kMaxPlanes;
+        const uint32_t planes = inf.planes;
+        uint2 ch = chunk[0];
+        uint4 nextv = *(const uint4*)(P.pool + ((unsigned long long)ch.x << 4));
+        uint32_t t_cur = s_ctx[(nextv.x & 0xFFu) >> 1][t];
+        for (uint32_t j = 0; j < planes; ++j) {
+            const uint2 chn = j + 1 < planes ? chunk[j + 1] : make_uint2(0u, 0u);
+            const uint8_t* src = P.pool + ((unsigned long long)ch.x << 4);
+            // first symbol of the next plane, for the context prefetch at this plane's end
+            const uint32_t first_next = j + 1 < planes ? P.pool[(unsigned long long)chn.x << 4] : 0u;
             for (uint32_t i = 0; i < ch.y; i += 16) {
-                const uint4 v = src[i >> 4];
-                const uint32_t words[4] = {v.x, v.y, v.z, v.w};
-                const uint32_t cnt = ch.y - i < 16u ? ch.y - i : 16u;
+                const uint4 cur = nextv;
+                if (i + 16 < ch.y) nextv = *(const uint4*)(src + i + 16);
+                else if (j + 1 < planes) nextv = *(const uint4*)(P.pool + ((unsigned long long)chn.x << 4));
+                const uint32_t m = ch.y - i < 16u ? ch.y - i : 16u;
+                const uint32_t w[5] = {cur.x, cur.y, cur.z, cur.w, nextv.x};
 #pragma unroll
                 for (uint32_t b = 0; b < 16; ++b) {
-                    if (b >= cnt) break;
-                    pass_ends();
-                    const uint32_t sv = (words[b >> 2] >> (8 * (b & 3))) & 0xFFu;
-                    const uint32_t cx = sv >> 1, d = sv & 1u;
-                    const uint32_t st = s_ctx[cx][t];
+                    if (b >= m) break;
+                    // pass ends: the coder state is the truncation reference of the pass
+                    while (pos == next_end) {
+                        snaps[k] = MqSnapshot{bp, B | ((uint32_t)CT << 16), C, A};
+                        ++k;
+                        next_end = k < np ? pend[k] : 0xFFFFFFFFu;
+                    }
+                    const uint32_t sv = (w[b >> 2] >> (8 * (b & 3))) & 0xFFu;
+                    const uint32_t sn = b + 1 < m ? (w[(b + 1) >> 2] >> (8 * ((b + 1) & 3))) & 0xFFu
+                                                  : (i + 16 < ch.y ? (nextv.x & 0xFFu) : first_next);
+                    const uint32_t cx = sv >> 1, d = sv & 1u, cxn = sn >> 1;
+                    // the next symbol's context state, read before this symbol may change it
+                    const uint32_t t_next = s_ctx[cxn][t];
+                    const uint32_t st = t_cur;
                     const uint32_t qe = st & 0xFFFFu, mps = st >> 31;
-                    const uint32_t a = e.A - qe;
+                    const uint32_t a = A - qe;
                     const bool is_mps = d == mps;
                     const bool add_c = is_mps != (a < qe);  // conditional exchange (C.2.5 / C.2.6)
-                    e.C += add_c ? qe : 0u;
-                    e.A = add_c ? a : qe;
+                    C += add_c ? qe : 0u;
+                    A = add_c ? a : qe;
+                    uint32_t st_new = st;
                     if (!is_mps || !(a & 0x8000u)) {
                         const uint32_t nidx = is_mps ? ((st >> 16) & 63u) : ((st >> 22) & 63u);
                         const uint32_t nmps = is_mps ? mps : (mps ^ ((st >> 28) & 1u));
-                        s_ctx[cx][t] = s_tab[nidx] | (nmps << 31);
-                        mq_renorm(e, out, cap, overflow);
+                        st_new = s_tab[nidx] | (nmps << 31);
+                        s_ctx[cx][t] = st_new;
+                        int n = __clz((int)A) - 16;  // RENORME: shifts until A >= 0x8000
+                        while (n > 0) {
+                            const int s = n < CT ? n : CT;
+                            A <<= s;
+                            C <<= s;
+                            CT -= s;
+                            n -= s;
+                            if (CT == 0) byteout();
+                        }
                     }
+                    t_cur = cxn == cx ? st_new : t_next;
                     ++pos;
                 }
             }
+            ch = chn;
         }
-        pass_ends();
+        while (pos == next_end) {
+            snaps[k] = MqSnapshot{bp, B | ((uint32_t)CT << 16), C, A};
+            ++k;
+            next_end = k < np ? pend[k] : 0xFFFFFFFFu;
+        }
         // FLUSH (C.2.9)
-        const uint32_t tempc = e.C + e.A;
-        e.C |= 0xFFFFu;
-        if (e.C >= tempc) e.C -= 0x8000u;
-        e.C <<= e.CT;
-        mq_byteout(e, out, cap, overflow);
-        e.C <<= e.CT;
-        mq_byteout(e, out, cap, overflow);
-        uint32_t len = (uint32_t)e.bp;
-        if (e.B != 0xFFu) {
-            if ((uint32_t)e.bp < cap) out[e.bp] = (uint8_t)e.B;
+        const uint32_t tempc = C + A;
+        C |= 0xFFFFu;
+        if (C >= tempc) C -= 0x8000u;
+        C <<= CT;
+        byteout();
+        C <<= CT;
+        byteout();
+        uint32_t len = (uint32_t)bp;
+        if (B != 0xFFu) {
+            word |= B << (8 * (bp & 3));
+            len = (uint32_t)bp + 1;
+        }
+        if (len & 3) {  // the last, partial group of bytes
+            if ((len - 1) < cap) *(uint32_t*)(out + ((len - 1) & ~3u)) = word;
             else overflow = true;
-            len = (uint32_t)e.bp + 1;
+        } else if (B != 0xFFu) {
+            if ((uint32_t)bp < cap) *(uint32_t*)(out + (bp & ~3)) = word;
+            else overflow = true;
         }
         if (overflow) {
             atomicOr(&P.counters->flags, 2u);

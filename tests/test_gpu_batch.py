@@ -1,0 +1,156 @@
+"""Batched form, colour inputs, graph replays and the overflow re-run, against the oracle.
+Every comparison is bit-exact (integer results and the float32 DWT planes alike)."""
+import numpy as np
+import pytest
+
+import libgail_amd as g
+from t1_compare import compare_tier1
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_frame(res, ref, label):
+    bad, lines = compare_tier1(res, ref)
+    for ln in lines:
+        print(f"[{label}]", ln, flush=True)
+    assert bad == 0, f"{label}: {bad} code-blocks differ"
+
+
+def test_mixed_geometry_batch(device, oracle):
+    """Config 3 in small: 2K flat, 2K scope, 4K scope and 4K full frames with their own
+    seeds and early-termination thresholds in one batch (one DWT launch per level, one
+    Tier-1 launch for all code-blocks)."""
+    names = ["2k_flat", "2k_scope", "4k_scope", "4k_full"]
+    thr = [0.0, 40.0, 0.0, 400.0]
+    layouts, frames = [], []
+    for i, n in enumerate(names):
+        W, H, k = g.CONTAINERS[n]
+        layouts.append(g.FrameLayout(W, H, k))
+        frames.append(g.synth_frame(100 + i, W, H))
+    b = g.Batch(device, layouts, [g.xyz_settings(L) for L in layouts])
+    try:
+        for i, f in enumerate(frames):
+            b.upload(i, f)
+            b.set_slope(i, thr[i])
+        print("[batch] encode 4 mixed frames", flush=True)
+        outs, _ = b.encode()
+    finally:
+        b.close()
+    for i, n in enumerate(names):
+        k = g.CONTAINERS[n][2]
+        ref = oracle.encode_frame_xyz(frames[i], k, thr=thr[i], threads=16)
+        _check_frame(outs[i], ref, n)
+
+
+def test_same_alone_and_in_the_mix(device):
+    """A frame's Tier-1 output is the same encoded alone and inside a batch of others."""
+    W, H, k = g.CONTAINERS["2k_flat"]
+    L = g.FrameLayout(W, H, k)
+    x = g.synth_frame(77, W, H)
+    enc = g.Encoder(device, L, g.xyz_settings(L))
+    try:
+        enc.set_slope(25.0)
+        alone = enc.encode(x)
+    finally:
+        enc.close()
+    W4, H4, k4 = g.CONTAINERS["4k_flat"]
+    L4 = g.FrameLayout(W4, H4, k4)
+    b = g.Batch(device, [L4, L, L4], [g.xyz_settings(L4), g.xyz_settings(L), g.xyz_settings(L4)])
+    try:
+        b.upload(0, g.synth_frame(1, W4, H4))
+        b.upload(1, x)
+        b.upload(2, g.synth_frame(2, W4, H4))
+        b.set_slope(0, 300.0)
+        b.set_slope(1, 25.0)
+        outs, _ = b.encode()
+    finally:
+        b.close()
+    mix = outs[1]
+    assert np.array_equal(alone.info, mix.info)
+    assert np.array_equal(alone.pass_len, mix.pass_len)
+    assert np.array_equal(alone.pass_dist, mix.pass_dist)
+    assert np.array_equal(alone.data, mix.data)
+
+
+def test_graph_replays_match_eager(device):
+    """The captured chain replayed several times, with eager work and syncs in between,
+    gives the eager result every time."""
+    W, H, k = g.CONTAINERS["2k_scope"]
+    L = g.FrameLayout(W, H, k)
+    frames = [g.synth_frame(300 + i, W, H) for i in range(2)]
+    eager = g.Batch(device, [L, L], [g.xyz_settings(L)] * 2)
+    graph = g.Batch(device, [L, L], [g.xyz_settings(L)] * 2, use_graph=True)
+    try:
+        for i, f in enumerate(frames):
+            eager.upload(i, f)
+            graph.upload(i, f)
+            eager.set_slope(i, 30.0)
+            graph.set_slope(i, 30.0)
+        want, _ = eager.encode()
+        for rep in range(4):
+            print(f"[graph] replay {rep}", flush=True)
+            got, _ = graph.encode()
+            for i in range(2):
+                assert np.array_equal(got[i].data, want[i].data), f"replay {rep} frame {i}"
+                assert np.array_equal(got[i].pass_len, want[i].pass_len)
+                assert np.array_equal(got[i].info, want[i].info)
+            eager.encode()  # eager work between replays
+    finally:
+        eager.close()
+        graph.close()
+
+
+def test_pool_overflow_reruns_on_gpu(device, oracle):
+    """A symbol pool far too small is detected on the device; the job re-runs Tier-1 on the
+    GPU with larger buffers and the result still matches the oracle."""
+    W, H, k = g.CONTAINERS["2k_flat"]
+    L = g.FrameLayout(W, H, k)
+    x = g.synth_frame(5, W, H)
+    b = g.Batch(device, [L], [g.xyz_settings(L)], pool_bytes_per_coef=0.02)
+    try:
+        b.upload(0, x)
+        outs, _ = b.encode()
+        st = b.stage_ms()
+    finally:
+        b.close()
+    assert st["reruns"] >= 1
+    _check_frame(outs[0], oracle.encode_frame_xyz(x, k, threads=16), "overflow")
+
+
+def _planar_from_synth(x, bits, sx, sy, rgb):
+    """A planar picture derived from a synthetic frame: G'B'R' (rgb) or Y'CbCr with chroma
+    subsampled by 2^sx x 2^sy, at the given bit depth."""
+    shift = 12 - bits
+    if rgb:
+        planes = [x[1], x[2], x[0]]
+        return np.concatenate([(p >> shift if shift >= 0 else p << -shift).astype(np.uint16).ravel() for p in planes])
+    Y = (x[1] >> shift).astype(np.uint16)
+    def sub(p):
+        p = p[::1 << sy, ::1 << sx]
+        return (p >> shift).astype(np.uint16)
+    return np.concatenate([Y.ravel(), sub(x[0]).ravel(), sub(x[2]).ravel()])
+
+
+@pytest.mark.parametrize("kind", ["gbrp12_bt709", "yuv420p10_bt709"])
+def test_colour_inputs(device, oracle, kind):
+    """Config 4 inputs: 12-bit G'B'R' and 10-bit Y'CbCr 4:2:0 (BT.709, limited range, chroma
+    left), a 1920x1080 picture centred in the 2K flat container."""
+    W, H, k = g.CONTAINERS["2k_flat"]
+    aw, ah, ox, oy = 1920, 1080, 39, 0
+    bits, sx, sy, rgb, fr = (12, 0, 0, 1, 1) if kind.startswith("gbrp") else (10, 1, 1, 0, 0)
+    src = _planar_from_synth(g.synth_frame(55, aw, ah), bits, sx, sy, rgb)
+    L = g.FrameLayout(W, H, k)
+    s = g.planar_settings(L, bits, sx, sy, rgb, fr, active=(aw, ah), offset=(ox, oy))
+    enc = g.Encoder(device, L, s)
+    try:
+        res = enc.encode(src, debug=True)
+    finally:
+        enc.close()
+    rgb48 = oracle.planar_to_rgb48(src, bits, aw, ah, sx, sy, rgb, fr, 0, 0.2126, 0.0722, W, H, ox, oy)
+    lut, cp = oracle.color_tables(0, 0)
+    _, xyz = oracle.convert_frame(lut, cp, rgb48)
+    codes = np.ascontiguousarray(xyz.transpose(2, 0, 1))
+    diff = np.count_nonzero(res.debug["codes"] != codes)
+    print(f"[colour {kind}] codes differing from the CPU path: {diff}", flush=True)
+    assert diff == 0
+    _check_frame(res, oracle.encode_frame_xyz(codes, k, threads=16), kind)
