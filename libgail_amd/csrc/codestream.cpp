@@ -400,50 +400,63 @@ struct Hull {
     std::vector<double> slope, gain;
 };
 
-Hull buildHull(const Layout& L, const Tier1Ref& t1) {
+// Fills H (its buffers are reused from call to call, so a worker thread that codes many
+// frames does not fault fresh pages in for every one).
+void buildHull(const Layout& L, const Tier1Ref& t1, Hull& H) {
+    constexpr int kMaxBlockPasses = 64;
     const int n = L.numCblks();
-    Hull H;
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) total += t1.passes(i);
     H.start.resize(n + 1);
-    std::vector<uint32_t> R;
-    std::vector<double> D;
-    std::vector<int> K;
+    H.passes.resize(total);
+    H.rate.resize(total);
+    H.slope.resize(total);
+    H.gain.resize(total);
+    uint32_t R[kMaxBlockPasses + 1];
+    double D[kMaxBlockPasses + 1];
+    int K[kMaxBlockPasses + 1];
+    size_t out = 0;
     for (int i = 0; i < n; ++i) {
-        H.start[i] = (uint32_t)H.passes.size();
+        H.start[i] = (uint32_t)out;
         const int np = (int)t1.passes(i);
         if (!np) continue;
+        if (np > kMaxBlockPasses) throw std::runtime_error("code-block with more than 64 coding passes");
         const Subband& b = L.bands[L.cblks[i].band];
         const double scale = (double)b.delta * b.delta * b.weight;
-        R.assign(1, 0);
-        D.assign(1, 0.0);
-        K.assign(1, 0);
+        const int nbps = (int)t1.nbps(i);
+        int m = 0;  // index of the last hull point; point 0 is the origin
+        R[0] = 0;
+        D[0] = 0.0;
+        K[0] = 0;
         double acc = 0;
         for (int k = 0; k < np; ++k) {
-            acc += t1.dist(i, k) * passPlaneScale((int)t1.nbps(i), k) * scale;
+            acc += t1.dist(i, k) * passPlaneScale(nbps, k) * scale;
             const uint32_t r = t1.len(i, k);
-            if (acc <= D.back()) continue;
-            while (K.size() >= 2) {
-                const size_t m = K.size() - 1;
+            if (acc <= D[m]) continue;
+            while (m >= 1) {
                 const double sNew = r == R[m] ? INFINITY : (acc - D[m]) / (double)(r - R[m]);
                 const double sLast = R[m] == R[m - 1] ? INFINITY : (D[m] - D[m - 1]) / (double)(R[m] - R[m - 1]);
                 if (sNew < sLast) break;
-                R.pop_back();
-                D.pop_back();
-                K.pop_back();
+                --m;
             }
-            R.push_back(r);
-            D.push_back(acc);
-            K.push_back(k + 1);
+            ++m;
+            R[m] = r;
+            D[m] = acc;
+            K[m] = k + 1;
         }
-        for (size_t m = 1; m < K.size(); ++m) {
-            H.passes.push_back((uint8_t)K[m]);
-            H.rate.push_back(R[m]);
-            const double dr = (double)R[m] - (double)R[m - 1];
-            H.slope.push_back(dr <= 0 ? INFINITY : (D[m] - D[m - 1]) / dr);
-            H.gain.push_back(D[m] - D[m - 1]);
+        for (int j = 1; j <= m; ++j, ++out) {
+            H.passes[out] = (uint8_t)K[j];
+            H.rate[out] = R[j];
+            const double dr = (double)R[j] - (double)R[j - 1];
+            H.slope[out] = dr <= 0 ? INFINITY : (D[j] - D[j - 1]) / dr;
+            H.gain[out] = D[j] - D[j - 1];
         }
     }
-    H.start[n] = (uint32_t)H.passes.size();
-    return H;
+    H.start[n] = (uint32_t)out;
+    H.passes.resize(out);
+    H.rate.resize(out);
+    H.slope.resize(out);
+    H.gain.resize(out);
 }
 
 void select(const Layout& L, const Hull& H, const double lambda[3], std::vector<uint8_t>& passes) {
@@ -462,7 +475,9 @@ void select(const Layout& L, const Hull& H, const double lambda[3], std::vector<
 // included-block counts for any threshold in O(1) (rate_control.cpp:117-162).
 class SlopeHistogram {
 public:
-    SlopeHistogram(const Layout& L, const Hull& H, double loLog, double hiLog) : lo_(loLog) {
+    // Buffers are reused across frames by the thread that owns the histogram.
+    void build(const Layout& L, const Hull& H, double loLog, double hiLog) {
+        lo_ = loLog;
         nb_ = std::max(16, (int)std::ceil((hiLog - loLog) * 1024.0) + 2);
         for (int c = 0; c < 3; ++c) {
             rate_[c].assign(nb_ + 1, 0.0);
@@ -470,10 +485,11 @@ public:
         }
         for (int i = 0; i < L.numCblks(); ++i) {
             const int c = L.bands[L.cblks[i].band].comp;
+            double* rate = rate_[c].data();
             uint32_t prev = 0;
             for (uint32_t h = H.start[i]; h < H.start[i + 1]; ++h) {
                 const int b = bin(H.slope[h]);
-                rate_[c][b] += (double)H.rate[h] - prev;
+                rate[b] += (double)H.rate[h] - prev;
                 if (h == H.start[i]) incl_[c][b] += 1;
                 prev = H.rate[h];
             }
@@ -497,15 +513,22 @@ private:
         if (!std::isfinite(s)) return nb_ - 1;
         return std::clamp((int)std::floor((std::log2(s) - lo_) * 1024.0), 0, nb_ - 1);
     }
-    double lo_;
-    int nb_;
+    double lo_ = 0;
+    int nb_ = 0;
     std::vector<double> rate_[3], incl_[3];
+};
+
+struct RateScratch {
+    Hull hull;
+    SlopeHistogram hist;
 };
 
 }  // namespace
 
 RateResult allocateRate(const Layout& L, const Tier1Ref& t1, const RateLimits& lim) {
-    const Hull H = buildHull(L, t1);
+    thread_local RateScratch scratch;
+    const Hull& H = scratch.hull;
+    buildHull(L, t1, scratch.hull);
     RateResult res;
     res.passes.assign(L.numCblks(), 0);
     double lam[3] = {0, 0, 0};
@@ -523,7 +546,8 @@ RateResult allocateRate(const Layout& L, const Tier1Ref& t1, const RateLimits& l
         }
     if (maxSlope == 0) maxSlope = minSlope = 1;
     const double loB = std::log2(minSlope) - 1, hiB = std::log2(maxSlope) + 1;
-    const SlopeHistogram hist(L, H, loB, hiB);
+    SlopeHistogram& hist = scratch.hist;
+    hist.build(L, H, loB, hiB);
     CodestreamSizes sz = measure(lam);
     // header bytes per component ~ calibrated + beta * (included blocks - calibrated count)
     double hdrCal[3], inclCal[3], beta[3];
@@ -564,7 +588,10 @@ RateResult allocateRate(const Layout& L, const Tier1Ref& t1, const RateLimits& l
                     return model(ll, comp) <= (double)lim.maxTotal;
                 },
                 loB, hiB);
+            const double prev = lam[0];
             lam[0] = lam[1] = lam[2] = std::exp2(lg);
+            // the same lambda again measures the same sizes: the model has settled
+            if (round > 0 && lam[0] == prev) break;
             sz = measure(lam);
             if (sz.total <= lim.maxTotal && sz.total + lim.maxTotal / 500 >= lim.maxTotal) break;
             calibrate(lam, sz);
@@ -587,7 +614,9 @@ RateResult allocateRate(const Layout& L, const Tier1Ref& t1, const RateLimits& l
                     return comp[c] <= (double)lim.maxComp;
                 },
                 std::log2(std::max(lam[c], minSlope)), hiB);
+            const double prev = lam[c];
             lam[c] = std::exp2(lg);
+            if (round > 0 && lam[c] == prev) break;
             sz = measure(lam);
             if (sz.comp[c] <= lim.maxComp && sz.comp[c] + lim.maxComp / 500 >= lim.maxComp) break;
             calibrate(lam, sz);
