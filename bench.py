@@ -56,6 +56,107 @@ def cpu_baseline(frames, fourk, thr, budget_s=20.0):
                       f"termination threshold, {cores} threads, {dt:.1f} s"}
 
 
+def _planar_picture(g, np, seed, W, H, fmt):
+    """A planar picture derived from a synthetic frame, planes back to back (uint16):
+    gbrp12 (G, B, R at 12 bits) or yuv420p10 (Y full size, Cb/Cr halved, 10 bits)."""
+    x = g.synth_frame(seed, W, H, threads=16)
+    if fmt == "gbrp12":
+        return [x[1], x[2], x[0]]
+    return [(x[1] >> 2).astype(np.uint16), np.ascontiguousarray((x[0] >> 2)[::2, ::2]).astype(np.uint16),
+            np.ascontiguousarray((x[2] >> 2)[::2, ::2]).astype(np.uint16)]
+
+
+def _pin(g, np, planes):
+    """Copies planes back to back into pinned host memory (the DMA input path)."""
+    buf = g.pinned_empty(sum(p.nbytes for p in planes))
+    out, off = [], 0
+    for p in planes:
+        v = buf[off:off + p.nbytes].view(np.uint16).reshape(p.shape)
+        v[...] = p
+        out.append(v)
+        off += p.nbytes
+    return out
+
+
+def served_run(g, np, offered_fps=300.0, seconds=5.0, seed=7):
+    """Served form: Poisson arrivals at offered_fps, profiles uniform over the six DCI sizes,
+    inputs alternating 12-bit G'B'R' BT.709 and 10-bit Y'CbCr 4:2:0 (limited range, chroma
+    left) at the container size; automatic early termination per stream."""
+    import random
+    import threading
+    rng = random.Random(seed)
+    srv = g.Server(slots_per_job=8, max_jobs=4, check=True)
+    try:
+        names = list(g.CONTAINERS)
+        streams, pool = {}, {}
+        for si, name in enumerate(names):
+            W, H, _ = g.CONTAINERS[name]
+            for fi, fmt in enumerate(("gbrp12", "yuv420p10")):
+                sid = si * 2 + fi
+                streams[sid] = g.stream_config(name, fmt, matrix=0, full_range=1 if fmt == "gbrp12" else 0,
+                                               chroma=0, preset=0)
+                pool[sid] = [_pin(g, np, _planar_picture(g, np, 5000 + sid * 10 + k, W, H, fmt)) for k in range(2)]
+        n = int(offered_fps * seconds)
+        t, arrivals = 0.0, []
+        for i in range(n):
+            t += rng.expovariate(offered_fps)
+            arrivals.append((t, rng.randrange(len(streams)), rng.randrange(2)))
+        print(f"[bench] served: {n} frames, Poisson {offered_fps} fps offered", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        for i, (at, sid, k) in enumerate(arrivals):
+            now = time.perf_counter() - t0
+            if at > now:
+                time.sleep(at - now)
+            srv.submit(i, sid, streams[sid], pool[sid][k], -1.0)
+        srv.drain()
+        st = srv.stats()
+        bad = sum(1 for r in srv.results.values() if r["status"] != 0 or not r.get("compliant", 0))
+        return {"offered_fps": offered_fps, "frames": st["frames"], "fps": round(st["fps"], 2),
+                "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
+                "queued_p99_ms": round(st["queued_p99_ms"], 2), "mean_batch": round(st["mean_batch"], 2),
+                "jobs": st["jobs"], "slots_per_job": st["slots_per_job"],
+                "max_frames_in_flight": st["max_frames_in_flight"],
+                "slot_gb": round(st["slot_bytes"] / 1e9, 3), "recoded": st["recoded"], "failed_or_noncompliant": bad}
+    finally:
+        srv.close()
+
+
+def stress_run(g, np, frames=512, in_flight=256, distinct=8):
+    """4K flat 3996x2160 X'Y'Z' stream at 48 fps rate control (651,025-byte budget): PCRD
+    truncation on every frame, up to in_flight frames admitted at once (16 jobs x 16 slots)."""
+    slots = 16
+    srv = g.Server(slots_per_job=slots, max_jobs=max(1, in_flight // slots), check=True)
+    try:
+        W, H, _ = g.CONTAINERS["4k_flat"]
+        cfg = g.stream_config("4k_flat", "xyz12_planar", fps=48.0)
+        src = []
+        for k in range(distinct):
+            x = g.synth_frame(1000 + k, W, H, threads=16)
+            src.append(_pin(g, np, [x[0], x[1], x[2]]))
+        # warm-up: one frame per job so every job has run once
+        for i in range(srv.stats()["jobs"]):
+            srv.submit(10 ** 6 + i, 1, cfg, src[i % distinct], -1.0)
+        srv.drain()
+        st0 = srv.stats()
+        print(f"[bench] stress: {frames} 4K frames at 48 fps rate control, {st0['jobs']} jobs x "
+              f"{st0['slots_per_job']} slots", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        for i in range(frames):
+            srv.submit(i, 1, cfg, src[i % distinct], -1.0)
+        srv.drain()
+        dt = time.perf_counter() - t0
+        lat = sorted(srv.results[i]["latency_ms"] for i in range(frames))
+        sizes = [srv.results[i]["size"] for i in range(frames)]
+        bad = sum(1 for i in range(frames) if srv.results[i]["status"] != 0 or not srv.results[i]["compliant"])
+        st = srv.stats()
+        return {"frames": frames, "fps": round(frames / dt, 2), "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                "p99_ms": round(float(np.percentile(lat, 99)), 2), "max_bytes": max(sizes), "cap_bytes": 651025,
+                "jobs": st["jobs"], "slots_per_job": st["slots_per_job"], "max_frames_in_flight": in_flight,
+                "recoded": st["recoded"] - st0["recoded"], "failed_or_noncompliant": bad}
+    finally:
+        srv.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +168,8 @@ def main():
     ap.add_argument("--plain-frames", type=int, default=8)
     ap.add_argument("--graph", type=int, default=0)
     ap.add_argument("--inflight", type=int, default=2)
+    ap.add_argument("--served-seconds", type=float, default=5.0)
+    ap.add_argument("--stress-frames", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU figures are not part of it "
@@ -157,6 +260,14 @@ def main():
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "dwt_level_kernel<level 0>"}
 
+    served = stress = None
+    if args.served_seconds > 0:
+        served = served_run(g, np, seconds=args.served_seconds)
+        print(f"[bench] served: {json.dumps(served)}", file=sys.stderr, flush=True)
+    if args.stress_frames > 0:
+        stress = stress_run(g, np, frames=args.stress_frames)
+        print(f"[bench] stress: {json.dumps(stress)}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(frames[:4], fourk, thr)
@@ -178,6 +289,8 @@ def main():
         "tier1_symbols_per_s": round(med["symbols"] / (ms_step * 1e-3), 1) if med.get("symbols") else None,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "served": served,
+        "stress_4k_48fps": stress,
     }
     if stage_note:
         line["note"] = stage_note

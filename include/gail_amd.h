@@ -237,6 +237,71 @@ typedef struct gail_device_info {
 } gail_device_info;
 GAIL_API int gail_enumerate_devices(gail_backend backend, gail_device_info* devices, int max_devices);
 
+/* ---- served form (no counterpart in gail.h) --------------------------------------------
+ * A request loop on one device for frames of many streams, each with its own profile,
+ * container, input format and colour. Frames are admitted into device jobs as job slots free
+ * up (the number of slots is set by a device-memory budget), the frames admitted together are
+ * coded in one job, several jobs run on the device at once, and rate control + Tier-2 run on a
+ * host thread pool. Every frame's result is delivered through the callback with its latency. */
+typedef struct gail_server gail_server;
+
+typedef struct gail_server_result {
+    uint64_t id;                /* the id given to gail_server_submit */
+    uint32_t stream;
+    int status;                 /* GAIL_OK, or why this frame failed */
+    const gail_packet* packet;  /* the codestream; valid during the callback only */
+    double queued_ms;           /* submit -> admitted into a device job */
+    double latency_ms;          /* submit -> codestream ready */
+    int batch_frames;           /* frames in the device job the frame was coded in */
+    int recoded;                /* coded again without early termination (lambda < threshold) */
+    double lambda;              /* the frame's rate-control slope */
+} gail_server_result;
+
+typedef void (*gail_server_cb)(void* user, const gail_server_result* result);
+
+typedef struct gail_server_config {
+    size_t struct_size;
+    int device;
+    int slots_per_job;      /* frames per device job (default 8) */
+    int max_jobs;           /* device jobs (default 4); fewer if the budget is short */
+    uint64_t memory_budget; /* device bytes for the jobs; 0 = free memory less a reserve */
+    int cpu_threads;        /* rate control + Tier-2 + input threads (0 = usable cores - 1) */
+    int check;              /* run the DCI codestream check on every frame */
+    int batch_window_us;    /* an idle job waits up to this long for more frames (0: launch at once) */
+    gail_server_cb on_result; /* called from pool threads, one call at a time */
+    void* user;
+} gail_server_config;
+
+typedef struct gail_server_stats {
+    uint64_t frames, failed, recoded;
+    double seconds;       /* first submit -> last result */
+    double fps;
+    double p50_ms, p99_ms, max_ms;          /* latency */
+    double queued_p50_ms, queued_p99_ms;
+    double mean_batch;    /* frames per device job */
+    int jobs, slots_per_job, max_frames_in_flight;
+    uint64_t job_bytes;   /* device memory of one job */
+    uint64_t slot_bytes;  /* device memory per frame slot (one 4K frame) */
+} gail_server_stats;
+
+GAIL_API void gail_server_config_init(gail_server_config* c, size_t struct_size);
+GAIL_API int gail_server_create(const gail_server_config* c, gail_server** server);
+/* Queues one frame of a stream. stream_cfg describes the stream (profile, container, fps,
+ * max_mbps, stereo, input, color; the other fields are ignored); frames of one stream id must
+ * use the same description. The frame's planes are read when it is admitted: keep them valid
+ * until its result arrives. slope: early-termination threshold of this frame, or negative for
+ * 0.25 x the stream's last lambda (pipeline.cpp:194-218). Never blocks on the device. */
+GAIL_API int gail_server_submit(gail_server* s, uint64_t id, uint32_t stream, const gail_config* stream_cfg,
+                                const gail_frame* frame, double slope);
+/* Waits until every submitted frame has been delivered. */
+GAIL_API int gail_server_drain(gail_server* s);
+GAIL_API int gail_server_get_stats(gail_server* s, gail_server_stats* out);
+GAIL_API void gail_server_destroy(gail_server* s);
+/* Pinned host memory for frames: a frame whose planes lie back to back in the device's input
+ * layout is copied to the device without a host copy, by DMA when it is in such memory. */
+GAIL_API void* gail_host_alloc(size_t bytes);
+GAIL_API void gail_host_free(void* p);
+
 /* ---- helpers of this build (no counterpart in gail.h) --------------------------------- */
 struct GailHipSubband;
 /* makeLayout (layout.cpp:181-210) as plain subbands; returns the code-block count. */

@@ -141,6 +141,12 @@ typedef struct GailHipBatchOptions {
     double poolBytesPerCoef;
 } GailHipBatchOptions;
 
+/* Room of one frame slot of a batch made by slots_create (no counterpart in the CUDA
+ * plugin): frames up to maxWidth x maxHeight; colour != 0 also takes planar / RGB48 inputs. */
+typedef struct GailHipSlot {
+    int32_t maxWidth, maxHeight, colour, reserved;
+} GailHipSlot;
+
 struct GailHipPlugin {
     uint32_t abi;      /* GAIL_HIP_PLUGIN_ABI the plugin was built with */
     uint32_t size;     /* sizeof(GailHipPlugin) */
@@ -207,6 +213,26 @@ struct GailHipPlugin {
      * [5] ordering + MQ, [6] compaction, [7] download, [8] whole run; [9] Tier-1 symbols,
      * [10] codeword bytes, [11] re-runs after a buffer overflow. Fills up to n values. */
     int (*batch_stage_ms)(GailHipBatch* b, double* out, int n);
+
+    /* Batches whose frame slots are bound per run (the served form). slots_create makes
+     * count empty slots; batch_bind assigns frames 0..count-1 (layouts and settings of any
+     * mix that fits the slots) for the following runs; it waits for the batch's previous run.
+     * batch_input(b, i) is valid for every slot; the other batch entries act on the bound
+     * frames. */
+    uint64_t (*slots_bytes)(GailHipDevice* d, const GailHipSlot* slots, int count, const GailHipBatchOptions* options);
+    int (*slots_create)(GailHipDevice* d, const GailHipSlot* slots, int count, const GailHipBatchOptions* options,
+                        GailHipBatch** out);
+    int (*batch_bind)(GailHipBatch* b, const GailHipLayout* layouts, const GailHipEncodeSettings* settings, int count);
+    /* 1 once the last launched run has finished on the device (batch_wait still fetches
+     * its results), 0 while it runs, a negative status on error. */
+    int (*batch_query)(GailHipBatch* b);
+    /* Queues the copy of bound frame i from host memory src (its input layout, bytes as the
+     * frame's settings give) on the batch's stream, ahead of the next batch_launch(b, 0).
+     * src must stay valid until that run is done; pinned memory (host_alloc) is copied by DMA. */
+    int (*batch_upload_from)(GailHipBatch* b, int i, const void* src);
+    /* Pinned host memory on the plugin's runtime; NULL when none is left. */
+    void* (*host_alloc)(uint64_t bytes);
+    void (*host_free)(void* p);
 };
 typedef struct GailHipPlugin GailHipPlugin;
 
@@ -221,7 +247,7 @@ static inline uint64_t gail_hip_plugin_layout(void) {
                               sizeof(GailHipFrameFormat), sizeof(GailHipColor),       sizeof(GailHipEncodeSettings),
                               sizeof(GailHipMemory),      sizeof(GailHipCblkInfo),    sizeof(GailHipTier1),
                               sizeof(GailHipFrameTimings), sizeof(GailHipDebug),      sizeof(GailHipBatchOptions),
-                              sizeof(GailHipPlugin)};
+                              sizeof(GailHipSlot),        sizeof(GailHipPlugin)};
     uint64_t h = 1469598103934665603ull;
     for (size_t i = 0; i < sizeof sizes / sizeof sizes[0]; ++i) h = (h ^ sizes[i]) * 1099511628211ull;
     return h;

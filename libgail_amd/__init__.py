@@ -88,6 +88,10 @@ class BatchOptions(C.Structure):
                 ("poolBytesPerCoef", C.c_double)]
 
 
+class Slot(C.Structure):
+    _fields_ = [("maxWidth", C.c_int32), ("maxHeight", C.c_int32), ("colour", C.c_int32), ("reserved", C.c_int32)]
+
+
 _vp = C.c_void_p
 _F = C.CFUNCTYPE
 
@@ -128,6 +132,13 @@ class Plugin(C.Structure):
         ("batch_wait", _F(C.c_int, _vp, C.POINTER(Tier1), C.POINTER(FrameTimings))),
         ("batch_debug", _F(C.c_int, _vp, C.c_int, C.POINTER(Debug))),
         ("batch_stage_ms", _F(C.c_int, _vp, C.POINTER(C.c_double), C.c_int)),
+        ("slots_bytes", _F(C.c_uint64, _vp, C.POINTER(Slot), C.c_int, C.POINTER(BatchOptions))),
+        ("slots_create", _F(C.c_int, _vp, C.POINTER(Slot), C.c_int, C.POINTER(BatchOptions), C.POINTER(_vp))),
+        ("batch_bind", _F(C.c_int, _vp, C.POINTER(Layout), C.POINTER(EncodeSettings), C.c_int)),
+        ("batch_query", _F(C.c_int, _vp)),
+        ("batch_upload_from", _F(C.c_int, _vp, C.c_int, _vp)),
+        ("host_alloc", _F(_vp, C.c_uint64)),
+        ("host_free", _F(None, _vp)),
     ]
 
 
@@ -323,19 +334,50 @@ def _debug_arrays(dbg, layout):
 
 
 class Batch:
-    """Many frame slots of any geometry encoded together (the batched form)."""
+    """Many frame slots of any geometry encoded together (the batched form). With `slots`
+    (a list of (max_width, max_height, colour)) the batch starts empty and bind() assigns
+    frames to its slots for each run (the served form)."""
 
-    def __init__(self, device, layouts, settings, dwt_tile=0, cm_waves=4, mq_group=256, use_graph=False,
-                 pool_bytes_per_coef=0.0):
+    def __init__(self, device, layouts=None, settings=None, dwt_tile=0, cm_waves=4, mq_group=256, use_graph=False,
+                 pool_bytes_per_coef=0.0, slots=None):
+        self.options = BatchOptions(dwt_tile, cm_waves, mq_group, int(use_graph), pool_bytes_per_coef)
+        self.handle = _vp()
+        if slots is not None:
+            self._slots = (Slot * len(slots))(*[Slot(w, h, int(c), 0) for w, h, c in slots])
+            self.num_slots = len(slots)
+            _check(plugin().slots_create(device.handle, self._slots, len(slots), C.byref(self.options),
+                                         C.byref(self.handle)), "slots_create")
+            self.layouts, self.settings, self.n = [], [], 0
+            return
         self.layouts, self.settings = list(layouts), list(settings)
         n = len(self.layouts)
         self._plain = (Layout * n)(*[l.plain() for l in self.layouts])
         self._settings = (EncodeSettings * n)(*self.settings)
-        self.options = BatchOptions(dwt_tile, cm_waves, mq_group, int(use_graph), pool_bytes_per_coef)
-        self.handle = _vp()
         _check(plugin().batch_create(device.handle, self._plain, self._settings, n, C.byref(self.options),
                                      C.byref(self.handle)), "batch_create")
+        self.n = self.num_slots = n
+
+    @staticmethod
+    def slots_bytes(device, slots, **options):
+        arr = (Slot * len(slots))(*[Slot(w, h, int(c), 0) for w, h, c in slots])
+        opt = BatchOptions(options.get("dwt_tile", 0), options.get("cm_waves", 4), options.get("mq_group", 256),
+                           int(options.get("use_graph", False)), options.get("pool_bytes_per_coef", 0.0))
+        return plugin().slots_bytes(device.handle, arr, len(slots), C.byref(opt))
+
+    def bind(self, layouts, settings):
+        """Assigns frames to slots 0..n-1 for the following runs."""
+        self.layouts, self.settings = list(layouts), list(settings)
+        n = len(self.layouts)
+        self._plain = (Layout * n)(*[l.plain() for l in self.layouts])
+        self._settings = (EncodeSettings * n)(*self.settings)
+        _check(plugin().batch_bind(self.handle, self._plain, self._settings, n), "batch_bind")
         self.n = n
+
+    def query(self):
+        r = plugin().batch_query(self.handle)
+        if r < 0:
+            _check(-r, "batch_query")
+        return r == 1
 
     def upload(self, i, frame):
         frame = np.ascontiguousarray(frame)
@@ -416,3 +458,192 @@ def check_codestream(cs, fps, fourk):
     msg = C.create_string_buffer(256)
     host().gail_check_codestream(buf.ctypes.data_as(_vp), len(buf), fps, int(fourk), msg, 256)
     return msg.value.decode()
+
+
+# ---------------------------------------------------------------- library API (gail_amd.h)
+class GailInput(C.Structure):
+    _fields_ = [("format", C.c_int), ("width", C.c_int), ("height", C.c_int), ("matrix", C.c_int),
+                ("full_range", C.c_int), ("chroma_location", C.c_int), ("reserved", C.c_uint64 * 4)]
+
+
+class GailColor(C.Structure):
+    _fields_ = [("mode", C.c_int), ("transfer", C.c_int), ("primaries", C.c_int), ("tone_map", C.c_int),
+                ("tone_norm", C.c_int), ("sdr_white", C.c_double), ("hdr_peak", C.c_double),
+                ("hdr_ref_white", C.c_double), ("cinema_ref_white", C.c_double), ("cinema_peak", C.c_double),
+                ("knee", C.c_double), ("gamut", C.c_int), ("gamut_threshold", C.c_double), ("lut3d", _vp),
+                ("lut3d_size", C.c_int), ("reserved", C.c_uint64 * 8)]
+
+
+class GailConfig(C.Structure):
+    _fields_ = [("struct_size", C.c_size_t), ("profile", C.c_int), ("container", C.c_int), ("fps", C.c_double),
+                ("max_mbps", C.c_double), ("stereo", C.c_int), ("input", GailInput), ("color", GailColor),
+                ("backend", C.c_int), ("devices", _vp), ("num_devices", C.c_int), ("frames_in_flight", C.c_int),
+                ("check", C.c_int), ("on_packet", _vp), ("user", _vp), ("log", _vp), ("log_user", _vp),
+                ("log_level", C.c_int), ("device_ids", _vp), ("num_device_ids", C.c_int),
+                ("max_device_memory", C.c_uint64), ("cpu_threads", C.c_int), ("low_priority", C.c_int)]
+
+
+class GailFrame(C.Structure):
+    _fields_ = [("data", _vp * 3), ("stride", C.c_ssize_t * 3), ("opaque", _vp), ("reserved", C.c_uint64 * 4)]
+
+
+class GailPacket(C.Structure):
+    _fields_ = [("data", _vp), ("size", C.c_size_t), ("index", C.c_int64), ("opaque", _vp), ("device", C.c_int),
+                ("compliant", C.c_int), ("psnr_estimate", C.c_double), ("reserved", C.c_uint64 * 4)]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("stream", C.c_uint32), ("status", C.c_int), ("packet", C.POINTER(GailPacket)),
+                ("queued_ms", C.c_double), ("latency_ms", C.c_double), ("batch_frames", C.c_int),
+                ("recoded", C.c_int), ("lambda_", C.c_double)]
+
+
+SERVER_CB = C.CFUNCTYPE(None, _vp, C.POINTER(ServerResult))
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("struct_size", C.c_size_t), ("device", C.c_int), ("slots_per_job", C.c_int), ("max_jobs", C.c_int),
+                ("memory_budget", C.c_uint64), ("cpu_threads", C.c_int), ("check", C.c_int),
+                ("batch_window_us", C.c_int), ("on_result", SERVER_CB), ("user", _vp)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("frames", C.c_uint64), ("failed", C.c_uint64), ("recoded", C.c_uint64), ("seconds", C.c_double),
+                ("fps", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
+                ("queued_p50_ms", C.c_double), ("queued_p99_ms", C.c_double), ("mean_batch", C.c_double),
+                ("jobs", C.c_int), ("slots_per_job", C.c_int), ("max_frames_in_flight", C.c_int),
+                ("job_bytes", C.c_uint64), ("slot_bytes", C.c_uint64)]
+
+
+PIX = {"rgb48": 0, "xyz12": 1, "xyz12_msb": 2, "yuv420p": 3, "yuv420p10": 4, "yuv420p12": 5, "yuv422p": 6,
+       "yuv422p10": 7, "yuv422p12": 8, "yuv444p": 9, "yuv444p10": 10, "yuv444p12": 11, "gbrp": 12, "gbrp10": 13,
+       "gbrp12": 14, "gbrp16": 15, "xyz12_planar": 16}
+CONTAINER_IDS = {"2k_flat": (0, 0), "2k_scope": (0, 1), "2k_full": (0, 2), "4k_flat": (1, 0), "4k_scope": (1, 1),
+                 "4k_full": (1, 2)}
+
+
+def _lib_api():
+    h = host()
+    if not hasattr(h, "_server_bound"):
+        h.gail_config_init.restype = None
+        h.gail_config_init.argtypes = [C.POINTER(GailConfig), C.c_size_t]
+        h.gail_color_preset.restype = None
+        h.gail_color_preset.argtypes = [C.c_int, C.POINTER(GailColor)]
+        h.gail_last_error.restype = C.c_char_p
+        h.gail_server_config_init.restype = None
+        h.gail_server_config_init.argtypes = [C.POINTER(ServerConfig), C.c_size_t]
+        h.gail_server_create.restype = C.c_int
+        h.gail_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(_vp)]
+        h.gail_server_submit.restype = C.c_int
+        h.gail_server_submit.argtypes = [_vp, C.c_uint64, C.c_uint32, C.POINTER(GailConfig), C.POINTER(GailFrame),
+                                         C.c_double]
+        h.gail_server_drain.restype = C.c_int
+        h.gail_server_drain.argtypes = [_vp]
+        h.gail_server_get_stats.restype = C.c_int
+        h.gail_server_get_stats.argtypes = [_vp, C.POINTER(ServerStats)]
+        h.gail_host_alloc.restype = _vp
+        h.gail_host_alloc.argtypes = [C.c_size_t]
+        h.gail_host_free.restype = None
+        h.gail_host_free.argtypes = [_vp]
+        h.gail_server_destroy.restype = None
+        h.gail_server_destroy.argtypes = [_vp]
+        h._server_bound = True
+    return h
+
+
+def stream_config(container="4k_flat", fmt="xyz12_planar", fps=24.0, mbps=250.0, picture=None, matrix=0,
+                  full_range=0, chroma=0, preset=0):
+    """A gail_config describing one stream (profile, container, rate, input, colour)."""
+    h = _lib_api()
+    c = GailConfig()
+    h.gail_config_init(C.byref(c), C.sizeof(c))
+    c.profile, c.container = CONTAINER_IDS[container]
+    c.fps, c.max_mbps = fps, mbps
+    c.input.format = PIX[fmt]
+    if picture:
+        c.input.width, c.input.height = picture
+    c.input.matrix, c.input.full_range, c.input.chroma_location = matrix, full_range, chroma
+    h.gail_color_preset(preset, C.byref(c.color))
+    return c
+
+
+class ServerError(RuntimeError):
+    pass
+
+
+def pinned_empty(nbytes):
+    """A uint8 numpy array over pinned host memory (gail_host_alloc), freed with the array."""
+    import weakref
+    h = _lib_api()
+    p = h.gail_host_alloc(nbytes)
+    if not p:
+        raise ServerError(f"pinned allocation of {nbytes} bytes failed: {h.gail_last_error().decode()}")
+    buf = (C.c_uint8 * nbytes).from_address(p)
+    weakref.finalize(buf, h.gail_host_free, p)
+    return np.frombuffer(buf, np.uint8)
+
+
+class Server:
+    """The served form (gail_server_*): frames of many streams admitted into device jobs as
+    slots free up. Results land in self.results[id] as dicts; with keep=True each also holds
+    the codestream bytes."""
+
+    def __init__(self, device=0, slots_per_job=8, max_jobs=4, memory_budget=0, cpu_threads=0, check=False,
+                 keep=False, batch_window_us=0):
+        h = _lib_api()
+        self.keep = keep
+        self.results = {}
+        self._frames = {}
+        self._lock = __import__("threading").Lock()
+        cfg = ServerConfig()
+        h.gail_server_config_init(C.byref(cfg), C.sizeof(cfg))
+        cfg.device, cfg.slots_per_job, cfg.max_jobs = device, slots_per_job, max_jobs
+        cfg.memory_budget, cfg.cpu_threads, cfg.check = memory_budget, cpu_threads, int(check)
+        cfg.batch_window_us = int(batch_window_us)
+        self._cb = SERVER_CB(self._on_result)
+        cfg.on_result = self._cb
+        self.handle = _vp()
+        if h.gail_server_create(C.byref(cfg), C.byref(self.handle)) != 0:
+            raise ServerError(h.gail_last_error().decode())
+
+    def _on_result(self, _user, rp):
+        r = rp.contents
+        d = {"status": r.status, "latency_ms": r.latency_ms, "queued_ms": r.queued_ms, "batch": r.batch_frames,
+             "recoded": r.recoded, "lambda": r.lambda_, "stream": r.stream}
+        if r.packet:
+            p = r.packet.contents
+            d["size"], d["compliant"], d["psnr"] = p.size, p.compliant, p.psnr_estimate
+            if self.keep:
+                d["codestream"] = C.string_at(p.data, p.size)
+        with self._lock:
+            self.results[r.id] = d
+            self._frames.pop(r.id, None)
+
+    def submit(self, fid, stream, cfg, planes, slope=-1.0):
+        """planes: list of three numpy arrays (planar formats) or one array; kept alive until
+        the frame's result arrives."""
+        f = GailFrame()
+        arrs = planes if isinstance(planes, (list, tuple)) else [planes]
+        for i, a in enumerate(arrs):
+            f.data[i] = a.ctypes.data
+            f.stride[i] = a.strides[0] if a.ndim > 1 else 0
+        with self._lock:
+            self._frames[fid] = (arrs, cfg)
+        st = _lib_api().gail_server_submit(self.handle, fid, stream, C.byref(cfg), C.byref(f), slope)
+        if st != 0:
+            with self._lock:
+                self._frames.pop(fid, None)
+            raise ServerError(f"submit: {_lib_api().gail_last_error().decode()}")
+
+    def drain(self):
+        _lib_api().gail_server_drain(self.handle)
+
+    def stats(self):
+        s = ServerStats()
+        _lib_api().gail_server_get_stats(self.handle, C.byref(s))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.handle:
+            _lib_api().gail_server_destroy(self.handle)
+            self.handle = None

@@ -27,84 +27,15 @@
 #include "gail_amd.h"
 #include "gail_hip_plugin.h"
 #include "layout.h"
+#include "stream.h"
 
 using namespace gail_amd;
 
 namespace {
 
-thread_local std::string g_error;
-
 int fail(int status, const std::string& msg) {
-    g_error = msg;
+    lastErrorText() = msg;
     return status;
-}
-
-struct ApiError : std::runtime_error {
-    int status;
-    ApiError(int s, const std::string& m) : std::runtime_error(m), status(s) {}
-};
-
-// ---- the plugin, loaded by full path from this library's own directory (cuda_host.cpp:82-93)
-struct PluginHandle {
-    const GailHipPlugin* table = nullptr;
-    std::string error;
-};
-
-const PluginHandle& loadPlugin() {
-    static PluginHandle h;
-    static std::once_flag once;
-    std::call_once(once, [] {
-        Dl_info info;
-        if (!dladdr((void*)&loadPlugin, &info) || !info.dli_fname) {
-            h.error = "cannot locate libgail.so";
-            return;
-        }
-        std::string dir = info.dli_fname;
-        dir = dir.substr(0, dir.find_last_of('/') + 1);
-        void* lib = dlopen((dir + "libgail_hip.so").c_str(), RTLD_NOW | RTLD_LOCAL);
-        if (!lib) {
-            h.error = std::string("HIP backend plugin not loaded: ") + dlerror();
-            return;
-        }
-        auto get = (const GailHipPlugin* (*)(uint32_t))dlsym(lib, GAIL_HIP_PLUGIN_ENTRY);
-        const GailHipPlugin* t = get ? get(GAIL_HIP_PLUGIN_ABI) : nullptr;
-        if (!t || t->abi != GAIL_HIP_PLUGIN_ABI || t->size != sizeof(GailHipPlugin) ||
-            t->layout != gail_hip_plugin_layout()) {
-            h.error = "libgail_hip.so is from another build";
-            return;
-        }
-        h.table = t;
-    });
-    return h;
-}
-
-bool allZero(const uint64_t* p, int n) {
-    for (int i = 0; i < n; ++i)
-        if (p[i]) return false;
-    return true;
-}
-
-struct PlanarInfo {
-    int bits, sx, sy, rgb;
-};
-
-bool planarInfo(gail_pixel_format f, PlanarInfo& p) {
-    switch (f) {
-    case GAIL_PIX_YUV420P: p = {8, 1, 1, 0}; return true;
-    case GAIL_PIX_YUV420P10: p = {10, 1, 1, 0}; return true;
-    case GAIL_PIX_YUV420P12: p = {12, 1, 1, 0}; return true;
-    case GAIL_PIX_YUV422P: p = {8, 1, 0, 0}; return true;
-    case GAIL_PIX_YUV422P10: p = {10, 1, 0, 0}; return true;
-    case GAIL_PIX_YUV422P12: p = {12, 1, 0, 0}; return true;
-    case GAIL_PIX_YUV444P: p = {8, 0, 0, 0}; return true;
-    case GAIL_PIX_YUV444P10: p = {10, 0, 0, 0}; return true;
-    case GAIL_PIX_YUV444P12: p = {12, 0, 0, 0}; return true;
-    case GAIL_PIX_GBRP: p = {8, 0, 0, 1}; return true;
-    case GAIL_PIX_GBRP10: p = {10, 0, 0, 1}; return true;
-    case GAIL_PIX_GBRP12: p = {12, 0, 0, 1}; return true;
-    case GAIL_PIX_GBRP16: p = {16, 0, 0, 1}; return true;
-    default: return false;
-    }
 }
 
 struct Waiter {
@@ -133,22 +64,14 @@ struct Done {
 }  // namespace
 
 struct gail_encoder {
+    std::unique_ptr<StreamSpec> spec;
     gail_config cfg{};
-    gail_color color{};
     const GailHipPlugin* P = nullptr;
     GailHipDevice* dev = nullptr;
     std::vector<GailHipEncoder*> slots;
-    Layout L;
-    std::vector<GailHipSubband> bands;
-    GailHipLayout plainLayout{};
-    GailHipEncodeSettings settings{};
-    RateLimits limits;
-    int width = 0, height = 0, picW = 0, picH = 0, ox = 0, oy = 0;
-    size_t inBytes = 0;
     std::string devices, error;
     std::atomic<bool> failed{false}, canceled{false};
     std::atomic<double> lastLambda{0.0};
-    double slopeFactor = 0.25;  // gail_api.cpp:607
 
     // input queue (bounded)
     std::mutex qm;
@@ -187,66 +110,15 @@ struct gail_encoder {
         ocv.notify_all();
     }
 
-    // Copies one app frame into the plugin's input layout (gail_api.cpp:363-396).
-    void fill(const gail_frame& f, uint8_t* dst) const {
-        const gail_pixel_format fmt = cfg.input.format;
-        PlanarInfo pi;
-        if (planarInfo(fmt, pi)) {
-            const size_t bps = pi.bits > 8 ? 2 : 1;
-            const int cw = (picW + pi.sx) >> pi.sx, ch = (picH + pi.sy) >> pi.sy;
-            uint8_t* plane = dst;
-            for (int c = 0; c < 3; ++c) {
-                const int w = c ? cw : picW, h = c ? ch : picH;
-                const size_t row = (size_t)w * bps;
-                const ptrdiff_t stride = f.stride[c] ? f.stride[c] : (ptrdiff_t)row;
-                const uint8_t* s = static_cast<const uint8_t*>(f.data[c]);
-                for (int y = 0; y < h; ++y) std::memcpy(plane + y * row, s + y * stride, row);
-                plane += row * h;
-            }
-            return;
-        }
-        if (fmt == GAIL_PIX_RGB48) {
-            const size_t row = (size_t)picW * 6;
-            const ptrdiff_t stride = f.stride[0] ? f.stride[0] : (ptrdiff_t)row;
-            for (int y = 0; y < picH; ++y)
-                std::memcpy(dst + ((size_t)(y + oy) * width + ox) * 6, (const uint8_t*)f.data[0] + y * stride, row);
-            return;
-        }
-        // X'Y'Z': planar 12-bit codes, the picture centred, code 0 around it
-        uint16_t* planes[3] = {(uint16_t*)dst, (uint16_t*)dst + (size_t)width * height,
-                               (uint16_t*)dst + 2 * (size_t)width * height};
-        for (int y = 0; y < picH; ++y) {
-            const size_t o = (size_t)(y + oy) * width + ox;
-            if (fmt == GAIL_PIX_XYZ12_PLANAR) {
-                for (int c = 0; c < 3; ++c) {
-                    const ptrdiff_t stride = f.stride[c] ? f.stride[c] : (ptrdiff_t)picW * 2;
-                    const uint16_t* s = (const uint16_t*)((const uint8_t*)f.data[c] + y * stride);
-                    for (int x = 0; x < picW; ++x) planes[c][o + x] = (uint16_t)std::min<uint32_t>(s[x], 4095u);
-                }
-                continue;
-            }
-            const ptrdiff_t stride = f.stride[0] ? f.stride[0] : (ptrdiff_t)picW * 6;
-            const uint16_t* s = (const uint16_t*)((const uint8_t*)f.data[0] + y * stride);
-            const bool msb = fmt == GAIL_PIX_XYZ12_MSB;
-            for (int x = 0; x < picW; ++x)
-                for (int c = 0; c < 3; ++c) {
-                    const uint32_t v = msb ? s[3 * x + c] >> 4 : s[3 * x + c];
-                    planes[c][o + x] = (uint16_t)std::min<uint32_t>(v, 4095u);
-                }
-        }
-    }
-
     int push(const gail_frame* f, Waiter* w) {
         if (canceled.load()) return fail(GAIL_ERROR_CANCELLED, "canceled");
         if (failed.load()) return fail(GAIL_ERROR_FAILED, error);
-        if (!allZero(f->reserved, 4)) return fail(GAIL_ERROR_INVALID, "gail_frame.reserved must be zero");
-        PlanarInfo pi;
-        const int planes = (planarInfo(cfg.input.format, pi) || cfg.input.format == GAIL_PIX_XYZ12_PLANAR) ? 3 : 1;
-        for (int c = 0; c < planes; ++c)
-            if (!f->data[c]) return fail(GAIL_ERROR_INVALID, "frame plane missing");
+        std::string why;
+        const int st = spec->checkFrame(*f, why);
+        if (st != GAIL_OK) return fail(st, why);
         Item it;
-        it.buf.assign(inBytes, 0);
-        fill(*f, it.buf.data());
+        it.buf.resize(spec->inBytes);
+        spec->fill(*f, it.buf.data());
         it.opaque = f->opaque;
         it.waiter = w;
         std::unique_lock<std::mutex> l(qm);
@@ -276,13 +148,13 @@ struct gail_encoder {
             d.opaque = it.opaque;
             d.waiter = it.waiter;
             try {
-                const double thr = lastLambda.load() * slopeFactor;
+                const double thr = lastLambda.load() * spec->slopeFactor;
                 P->encoder_set_slope(slot, thr);
                 GailHipTier1 t1{};
                 if (P->encoder_encode(slot, it.buf.data(), &t1, nullptr, nullptr) != GAIL_HIP_OK)
                     throw std::runtime_error(P->last_error());
                 Tier1Ref ref{(const uint32_t*)t1.info, t1.passLen, t1.passDist, t1.data, (size_t)t1.numInfo};
-                RateResult rr = allocateRate(L, ref, limits);
+                RateResult rr = allocateRate(spec->L, ref, spec->limits);
                 double lam = std::min({rr.lambdaComp[0], rr.lambdaComp[1], rr.lambdaComp[2]});
                 if (thr > 0 && lam < thr) {
                     // an easier frame than the last: passes the rate control would keep may
@@ -291,15 +163,14 @@ struct gail_encoder {
                     if (P->encoder_encode(slot, it.buf.data(), &t1, nullptr, nullptr) != GAIL_HIP_OK)
                         throw std::runtime_error(P->last_error());
                     ref = Tier1Ref{(const uint32_t*)t1.info, t1.passLen, t1.passDist, t1.data, (size_t)t1.numInfo};
-                    rr = allocateRate(L, ref, limits);
+                    rr = allocateRate(spec->L, ref, spec->limits);
                     lam = std::min({rr.lambdaComp[0], rr.lambdaComp[1], rr.lambdaComp[2]});
                 }
-                buildCodestream(L, ref, rr.passes.data(), &d.cs, nullptr, tag);
+                buildCodestream(spec->L, ref, rr.passes.data(), &d.cs, nullptr, tag);
                 lastLambda.store(lam);
-                d.mse = rr.distortion / ((double)width * height * 3);
+                d.mse = rr.distortion / ((double)spec->width * spec->height * 3);
                 if (cfg.check) {
-                    const std::string why = checkDciCodestream(d.cs.data(), d.cs.size(), cfg.fps * (cfg.stereo ? 2 : 1),
-                                                               cfg.profile == GAIL_PROFILE_4K);
+                    const std::string why = checkDciCodestream(d.cs.data(), d.cs.size(), spec->eyeFps(), spec->fourk());
                     d.compliant = why.empty();
                     if (!d.compliant) log(GAIL_LOG_WARNING, "frame %lld failed the DCI check: %s", it.index, why.c_str());
                 }
@@ -399,7 +270,7 @@ struct gail_encoder {
 
 extern "C" {
 
-GAIL_API const char* gail_last_error(void) { return g_error.c_str(); }
+GAIL_API const char* gail_last_error(void) { return lastErrorText().c_str(); }
 
 GAIL_API const char* gail_status_string(int s) {
     switch (s) {
@@ -486,101 +357,9 @@ GAIL_API int gail_encoder_create(const gail_config* c, gail_encoder** encoder) {
     *encoder = nullptr;
     auto e = std::make_unique<gail_encoder>();
     try {
-        gail_config_init(&e->cfg, sizeof e->cfg);
-        std::memcpy(&e->cfg, c, std::min(c->struct_size ? c->struct_size : sizeof *c, sizeof e->cfg));
+        e->spec = StreamSpec::make(*c);
+        e->cfg = e->spec->cfg;
         gail_config& cfg = e->cfg;
-        if (cfg.backend != GAIL_BACKEND_AUTO && cfg.backend != GAIL_BACKEND_HIP)
-            throw ApiError(GAIL_ERROR_UNSUPPORTED, "this library has the HIP backend only");
-        if (!(cfg.fps > 0) || !(cfg.max_mbps > 0) || cfg.max_mbps > 250.0)
-            throw ApiError(GAIL_ERROR_INVALID, "fps or max_mbps out of range");
-        if (!allZero(cfg.input.reserved, 4) || !allZero(cfg.color.reserved, 8))
-            throw ApiError(GAIL_ERROR_INVALID, "reserved fields must be zero");
-        if (cfg.color.mode == GAIL_COLOR_LUT3D)
-            throw ApiError(GAIL_ERROR_UNSUPPORTED, "3D LUT colour is not supported by the HIP backend");
-        if (gail_container_size(cfg.profile, cfg.container, &e->width, &e->height) != GAIL_OK)
-            throw ApiError(GAIL_ERROR_INVALID, "invalid profile or container");
-        e->picW = cfg.input.width ? cfg.input.width : e->width;
-        e->picH = cfg.input.height ? cfg.input.height : e->height;
-        if (e->picW <= 0 || e->picH <= 0 || e->picW > e->width || e->picH > e->height)
-            throw ApiError(GAIL_ERROR_INVALID, "picture larger than the container");
-        e->ox = (e->width - e->picW) / 2;
-        e->oy = (e->height - e->picH) / 2;
-        const bool fourk = cfg.profile == GAIL_PROFILE_4K;
-        e->L = makeLayout(e->width, e->height, fourk, 1.0);
-        for (const Subband& b : e->L.bands) {
-            GailHipSubband s{};
-            s.comp = b.comp;
-            s.res = b.res;
-            s.level = b.level;
-            s.orient = b.orient;
-            s.px0 = b.px0;
-            s.py0 = b.py0;
-            s.w = b.w;
-            s.h = b.h;
-            s.ncbx = b.ncbx;
-            s.ncby = b.ncby;
-            s.cblkBase = b.cblkBase;
-            s.eps = b.eps;
-            s.mu = b.mu;
-            s.Mb = b.Mb;
-            s.delta = b.delta;
-            s.weight = b.weight;
-            e->bands.push_back(s);
-        }
-        e->plainLayout = GailHipLayout{fourk ? 1 : 0, e->width, e->height, e->L.levels, e->L.guardBits,
-                                       (int)e->bands.size(), e->L.numCblks(), 1.0, e->bands.data()};
-        GailHipEncodeSettings& s = e->settings;
-        s.profile = fourk ? 1 : 0;
-        s.width = e->width;
-        s.height = e->height;
-        s.baseStep = 1.0;
-        s.slopeFactor = e->slopeFactor;
-        s.bitsPerSymbol = 0.5;
-        s.input.width = e->width;
-        s.input.height = e->height;
-        PlanarInfo pi;
-        const gail_pixel_format fmt = cfg.input.format;
-        if (planarInfo(fmt, pi)) {
-            s.input.kind = 1;
-            s.input.activeW = e->picW;
-            s.input.activeH = e->picH;
-            s.input.ox = e->ox;
-            s.input.oy = e->oy;
-            s.input.sx = pi.sx;
-            s.input.sy = pi.sy;
-            s.input.rgb = pi.rgb;
-            s.input.bits = pi.bits;
-            s.input.fullRange = pi.rgb ? 1 : (cfg.input.full_range ? 1 : 0);
-            s.input.chroma = (int)cfg.input.chroma_location;
-            const double kr[3] = {0.2126, 0.2627, 0.299}, kb[3] = {0.0722, 0.0593, 0.114};
-            if (cfg.input.matrix < GAIL_MATRIX_BT709 || cfg.input.matrix > GAIL_MATRIX_BT601)
-                throw ApiError(GAIL_ERROR_INVALID, "invalid matrix");
-            s.input.kr = kr[cfg.input.matrix];
-            s.input.kb = kb[cfg.input.matrix];
-            const int cw = (e->picW + pi.sx) >> pi.sx, ch = (e->picH + pi.sy) >> pi.sy;
-            e->inBytes = ((size_t)e->picW * e->picH + 2 * (size_t)cw * ch) * (pi.bits > 8 ? 2 : 1);
-        } else if (fmt == GAIL_PIX_RGB48) {
-            s.input.kind = 0;
-            e->inBytes = (size_t)e->width * e->height * 6;
-        } else if (fmt == GAIL_PIX_XYZ12 || fmt == GAIL_PIX_XYZ12_MSB || fmt == GAIL_PIX_XYZ12_PLANAR) {
-            s.input.kind = 2;
-            e->inBytes = (size_t)e->width * e->height * 6;
-        } else {
-            throw ApiError(GAIL_ERROR_INVALID, "invalid pixel format");
-        }
-        const bool xyz = s.input.kind == 2;
-        s.color.transfer = xyz ? 5 : (int)cfg.color.transfer;
-        s.color.primaries = xyz ? 4 : (int)cfg.color.primaries;
-        s.color.sdrWhite = cfg.color.sdr_white;
-        s.color.cinemaPeak = cfg.color.cinema_peak;
-        if (!xyz && (cfg.color.transfer == GAIL_TRANSFER_PQ || cfg.color.transfer == GAIL_TRANSFER_HLG) &&
-            cfg.color.tone_map != GAIL_TONE_MAP_SIGNAL)
-            throw ApiError(GAIL_ERROR_UNSUPPORTED, "HDR tone mapping other than SIGNAL is not supported by the HIP backend");
-        const double eyeFps = cfg.fps * (cfg.stereo ? 2 : 1);
-        e->limits = dciLimits(eyeFps, cfg.max_mbps, 200.0);
-        if (emptyCodestreamBytes(e->L) > e->limits.maxTotal)
-            throw ApiError(GAIL_ERROR_INVALID, "the frame budget is below the codestream headers");
-
         const PluginHandle& ph = loadPlugin();
         if (!ph.table) throw ApiError(GAIL_ERROR_UNSUPPORTED, ph.error);
         e->P = ph.table;
@@ -594,7 +373,7 @@ GAIL_API int gail_encoder_create(const gail_config* c, gail_encoder** encoder) {
         e->devices = e->P->device_name(e->dev);
         // frames in flight: as asked (default 2), fewer when device memory is short
         int want = cfg.frames_in_flight > 0 ? cfg.frames_in_flight : 2;
-        const uint64_t per = e->P->encoder_bytes(e->dev, &e->plainLayout, &e->settings);
+        const uint64_t per = e->P->encoder_bytes(e->dev, &e->spec->plainLayout, &e->spec->settings);
         GailHipMemory mem{};
         if (per && e->P->device_memory(e->dev, &mem) == GAIL_HIP_OK && mem.budgeted) {
             const uint64_t headroom = std::max<uint64_t>(512ull << 20, mem.total / 16);
@@ -604,7 +383,7 @@ GAIL_API int gail_encoder_create(const gail_config* c, gail_encoder** encoder) {
         }
         for (int i = 0; i < want; ++i) {
             GailHipEncoder* slot = nullptr;
-            const int st = e->P->encoder_create(e->dev, &e->plainLayout, &e->settings, &slot);
+            const int st = e->P->encoder_create(e->dev, &e->spec->plainLayout, &e->spec->settings, &slot);
             if (st != GAIL_HIP_OK) {
                 if (e->slots.empty())
                     throw ApiError(st == GAIL_HIP_DEVICE_MEMORY ? GAIL_ERROR_NOMEM : GAIL_ERROR_FAILED, e->P->last_error());
@@ -635,8 +414,8 @@ GAIL_API void gail_encoder_destroy(gail_encoder* e) {
 }
 
 GAIL_API void gail_encoder_frame_size(const gail_encoder* e, int* w, int* h) {
-    if (w) *w = e ? e->width : 0;
-    if (h) *h = e ? e->height : 0;
+    if (w) *w = e && e->spec ? e->spec->width : 0;
+    if (h) *h = e && e->spec ? e->spec->height : 0;
 }
 
 GAIL_API const char* gail_encoder_devices(const gail_encoder* e) { return e ? e->devices.c_str() : ""; }

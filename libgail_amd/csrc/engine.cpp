@@ -87,32 +87,79 @@ template <class T> T* Job::devAlloc(size_t count) {
     return (T*)p;
 }
 
-size_t Job::estimateBytes(const std::vector<FrameConfig>& frames, const JobOptions& opt) {
+namespace {
+// Largest number of code-blocks and levels of any DCI layout (2K or 4K decomposition) of a
+// frame no larger than W x H: block counts only grow with the frame.
+struct SlotLimits {
+    uint32_t blocks = 0, bands = 0;
+    int levels = 0;
+};
+SlotLimits slotLimits(const SlotCapacity& c) {
+    SlotLimits r;
+    for (bool fourk : {false, true}) {
+        const Layout L = makeGeometry(c.maxWidth, c.maxHeight, fourk, fourk ? 6 : 5, fourk ? 2 : 1);
+        r.blocks = std::max<uint32_t>(r.blocks, (uint32_t)L.numCblks());
+        r.bands = std::max<uint32_t>(r.bands, (uint32_t)L.bands.size());
+        r.levels = std::max(r.levels, L.levels);
+    }
+    return r;
+}
+}  // namespace
+
+bool SlotCapacity::fits(const FrameConfig& f) const {
+    if (f.layout.width > maxWidth || f.layout.height > maxHeight) return false;
+    if (f.input.kind != InputFormat::XyzPlanar && !colour) return false;
+    return f.input.bytes() <= (size_t)maxWidth * maxHeight * 6;
+}
+
+size_t Job::estimateBytes(const std::vector<SlotCapacity>& slots, const JobOptions& opt) {
     size_t total = 0, coefs = 0;
-    uint32_t n = 0;
-    for (const FrameConfig& f : frames) {
-        const size_t W = f.layout.width, H = f.layout.height, npix = W * H;
-        total += alignUp(f.input.bytes()) + (f.input.kind == InputFormat::XyzPlanar ? 0 : alignUp(npix * 6));
+    uint32_t n = 0, nb = 0;
+    int lv = 0;
+    for (const SlotCapacity& c : slots) {
+        const size_t W = c.maxWidth, H = c.maxHeight, npix = W * H;
+        total += alignUp(npix * 6) + (c.colour ? alignUp(npix * 6) + alignUp(65536 * 4) : 0);
         total += alignUp(npix * 12) + alignUp(3 * ((W + 1) / 2) * ((H + 1) / 2) * 4) + alignUp(3 * ((W + 3) / 4) * ((H + 3) / 4) * 4);
-        if (f.input.kind != InputFormat::XyzPlanar) total += 65536 * 4;
         coefs += 3 * npix;
-        n += (uint32_t)f.layout.numCblks();
+        const SlotLimits lim = slotLimits(c);
+        n += lim.blocks;
+        nb += lim.bands;
+        lv = std::max(lv, lim.levels);
     }
     const Sizes z = t1Sizes(coefs, n, opt.poolBytesPerCoef);
     total += (size_t)n * (16 + 16 + 3 * kMaxPasses * 4 + kMaxPlanes * 8 + kMaxPasses * 16 + 4 + 8 + 8 + 4);
+    total += (size_t)nb * sizeof(T1Band) + (size_t)lv * slots.size() * sizeof(DwtLevelDesc);
     total += z.pool + z.out;
     return total;
 }
 
-Job::Job(int device, std::vector<FrameConfig> frames, const JobOptions& opt)
-    : dev_(device), ctx_(DeviceContext::get(device)), opt_(opt), frames_(std::move(frames)) {
-    if (frames_.empty()) throw std::logic_error("a job needs at least one frame");
+static std::vector<SlotCapacity> exactSlots(const std::vector<FrameConfig>& frames) {
+    std::vector<SlotCapacity> s;
+    for (const FrameConfig& f : frames)
+        s.push_back(SlotCapacity{f.layout.width, f.layout.height, f.input.kind != InputFormat::XyzPlanar});
+    return s;
+}
+
+size_t Job::estimateBytes(const std::vector<FrameConfig>& frames, const JobOptions& opt) {
+    return estimateBytes(exactSlots(frames), opt);
+}
+
+Job::Job(int device, std::vector<SlotCapacity> slots, const JobOptions& opt)
+    : dev_(device), ctx_(DeviceContext::get(device)), opt_(opt), slots_(std::move(slots)) {
+    if (slots_.empty()) throw std::logic_error("a job needs at least one frame slot");
+    for (const SlotCapacity& c : slots_)
+        if (c.maxWidth <= 0 || c.maxHeight <= 0 || c.maxWidth > 65535 || c.maxHeight > 65535)
+            throw std::logic_error("invalid frame slot size");
     try {
         allocate();
     } catch (...) {
         release();
         throw;
     }
+}
+
+Job::Job(int device, std::vector<FrameConfig> frames, const JobOptions& opt) : Job(device, exactSlots(frames), opt) {
+    bind(std::move(frames));
 }
 
 Job::~Job() { release(); }
@@ -127,19 +174,24 @@ void Job::release() {
     for (void* p : hostIn_)
         if (p) hipHostFree(p);
     hostIn_.clear();
-    for (void* p : {(void*)hFrames_, (void*)hInfo_, (void*)hPassLen_, (void*)hPassDist_, (void*)hData_, (void*)hSmall_})
-        if (p) hipHostFree(p);
-    hFrames_ = nullptr;
-    hInfo_ = nullptr;
-    hPassLen_ = nullptr;
-    hPassDist_ = nullptr;
-    hData_ = nullptr;
-    hSmall_ = nullptr;
+    for (SlotDev& d : sd_)
+        if (d.hostLut) {
+            hipHostFree(d.hostLut);
+            d.hostLut = nullptr;
+        }
+    for (void** p : {(void**)&hFrames_, (void**)&hInfo_, (void**)&hPassLen_, (void**)&hPassDist_, (void**)&hData_,
+                     (void**)&hSmall_, (void**)&hBlocks_, (void**)&hBands_, (void**)&hLevels_})
+        if (*p) {
+            hipHostFree(*p);
+            *p = nullptr;
+        }
     for (auto& e : ev_)
         if (e) {
             hipEventDestroy(e);
             e = nullptr;
         }
+    if (evDone_) hipEventDestroy(evDone_);
+    evDone_ = nullptr;
     if (stream_) hipStreamDestroy(stream_);
     stream_ = nullptr;
     bytes_ = 0;
@@ -153,41 +205,120 @@ void Job::allocate() {
     hipCheck(hipSetDevice(dev_), "hipSetDevice");
     hipCheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream");
     for (auto& e : ev_) hipCheck(hipEventCreate(&e), "event");
-    const int nf = (int)frames_.size();
-    fd_.resize(nf);
-    hostIn_.assign(nf, nullptr);
-    slope_.assign(nf, 0.f);
-    std::vector<T1Block> blocks;
-    std::vector<T1Band> bands;
+    hipCheck(hipEventCreateWithFlags(&evDone_, hipEventDisableTiming), "event");
+    const int ns = (int)slots_.size();
+    sd_.resize(ns);
+    hostIn_.assign(ns, nullptr);
+    slope_.assign(ns, 0.f);
+    for (int i = 0; i < ns; ++i) {
+        const SlotCapacity& c = slots_[i];
+        SlotDev& d = sd_[i];
+        const size_t W = c.maxWidth, H = c.maxHeight;
+        d.pixCap = W * H;
+        d.inCap = 6 * d.pixCap;
+        d.in = devAlloc<uint8_t>(d.inCap);
+        if (c.colour) {
+            d.codesBuf = devAlloc<uint16_t>(3 * d.pixCap);
+            d.lut = devAlloc<float>(65536);
+            hostAlloc((void**)&d.hostLut, 65536 * sizeof(float));
+        }
+        d.coef = devAlloc<float>(3 * d.pixCap);
+        d.llA = devAlloc<float>(3 * ((W + 1) / 2) * ((H + 1) / 2));
+        d.llB = devAlloc<float>(3 * ((W + 3) / 4) * ((H + 3) / 4));
+        const SlotLimits lim = slotLimits(c);
+        blockCap_ += lim.blocks;
+        bandCap_ += lim.bands;
+        levelCap_ = std::max(levelCap_, lim.levels);
+        coefCap_ += 3 * d.pixCap;
+    }
+    if (bandCap_ > 65535 || ns > 65535) throw std::logic_error("job too large");
+    dBlocks_ = devAlloc<T1Block>(blockCap_);
+    hostAlloc((void**)&hBlocks_, sizeof(T1Block) * blockCap_);
+    dBands_ = devAlloc<T1Band>(bandCap_);
+    hostAlloc((void**)&hBands_, sizeof(T1Band) * bandCap_);
+    dFrames_ = devAlloc<T1FrameRef>(ns);
+    hostAlloc((void**)&hFrames_, sizeof(T1FrameRef) * ns);
+    dLevels_ = devAlloc<DwtLevelDesc>((size_t)levelCap_ * ns);
+    hostAlloc((void**)&hLevels_, sizeof(DwtLevelDesc) * levelCap_ * ns);
+
+    const uint32_t n = blockCap_;
+    dInfo_ = devAlloc<T1Info>(n);
+    dPassEnd_ = devAlloc<uint32_t>((size_t)n * kMaxPasses);
+    dPassDist_ = devAlloc<int32_t>((size_t)n * kMaxPasses);
+    dPassLen_ = devAlloc<uint32_t>((size_t)n * kMaxPasses);
+    dChunks_ = devAlloc<uint2>((size_t)n * kMaxPlanes);
+    dSnaps_ = devAlloc<MqSnapshot>((size_t)n * kMaxPasses);
+    dPerm_ = devAlloc<uint32_t>(n);
+    dHist_ = devAlloc<uint32_t>(128);
+    dOutOff_ = devAlloc<unsigned long long>(n);
+    dDataOff_ = devAlloc<unsigned long long>(n);
+    dPartial_ = devAlloc<unsigned long long>(n / 2048 + 4);
+    dCodeLen_ = devAlloc<uint32_t>(n);
+    dCounters_ = devAlloc<JobCounters>(1);
+    dWork_ = devAlloc<uint32_t>(4);
+    hostAlloc((void**)&hSmall_, 256);
+    hostAlloc((void**)&hInfo_, (size_t)n * 20);
+    // result buffers start at typical sizes and grow when a run needs more
+    hPassCap_ = (size_t)n * 32;
+    hostAlloc((void**)&hPassLen_, hPassCap_ * 4);
+    hostAlloc((void**)&hPassDist_, hPassCap_ * 4);
+    hDataCap_ = std::max<size_t>(16ull << 20, coefCap_ / 2);
+    hostAlloc((void**)&hData_, hDataCap_);
+    const Sizes z = t1Sizes(coefCap_, n, opt_.poolBytesPerCoef);
+    poolCap_ = z.pool;
+    outCap_ = z.out;
+    dPool_ = devAlloc<uint8_t>(poolCap_);
+    dOut_ = devAlloc<uint8_t>(outCap_);
+}
+
+void Job::bind(std::vector<FrameConfig> frames) {
+    const int nf = (int)frames.size();
+    if (nf < 1 || nf > numSlots()) throw std::logic_error("more frames than the job has slots");
+    for (int f = 0; f < nf; ++f) {
+        const FrameConfig& fc = frames[f];
+        if (fc.input.width != fc.layout.width || fc.input.height != fc.layout.height)
+            throw std::logic_error("input format does not match the frame size");
+        if (!slots_[f].fits(fc)) throw std::logic_error("frame does not fit its slot");
+    }
+    hipCheck(hipSetDevice(dev_), "hipSetDevice");
+    hipCheck(hipStreamSynchronize(stream_), "previous run");  // the staging tables are reused
+    // Until the new tables are complete nothing is bound (a failure leaves the job unbound).
+    frames_.clear();
+    nblocks_ = 0;
+    launched_ = false;
     int tx, ty;
     dwt_tile_size(opt_.dwtTile, tx, ty);
+    uint32_t nblocks = 0, nbands = 0;
     int maxLevels = 0;
     for (int f = 0; f < nf; ++f) {
-        const FrameConfig& fc = frames_[f];
+        const FrameConfig& fc = frames[f];
         const Layout& L = fc.layout;
-        FrameDev& d = fd_[f];
+        SlotDev& d = sd_[f];
         const size_t W = L.width, H = L.height;
         d.npix = W * H;
         d.inBytes = fc.input.bytes();
-        if (fc.input.width != L.width || fc.input.height != L.height)
-            throw std::logic_error("input format does not match the frame size");
-        d.in = devAlloc<uint8_t>(d.inBytes);
-        d.codes = fc.input.kind == InputFormat::XyzPlanar ? (uint16_t*)d.in : devAlloc<uint16_t>(3 * d.npix);
-        d.coef = devAlloc<float>(3 * d.npix);
-        d.llA = devAlloc<float>(3 * ((W + 1) / 2) * ((H + 1) / 2));
-        d.llB = devAlloc<float>(3 * ((W + 3) / 4) * ((H + 3) / 4));
+        d.codes = fc.input.kind == InputFormat::XyzPlanar ? (uint16_t*)d.in : d.codesBuf;
         if (fc.input.kind != InputFormat::XyzPlanar) {
-            std::vector<float> lut;
-            buildColorTables(fc.color, lut, d.matrix);
-            d.lut = devAlloc<float>(65536);
-            hipCheck(hipMemcpy(d.lut, lut.data(), 65536 * 4, hipMemcpyHostToDevice), "EOTF table upload");
+            const ColorSpec& cs = fc.color;
+            if (!d.lutValid || cs.transfer != d.lutSpec.transfer || cs.primaries != d.lutSpec.primaries ||
+                cs.sdrWhite != d.lutSpec.sdrWhite || cs.cinemaPeak != d.lutSpec.cinemaPeak) {
+                std::vector<float> lut;
+                buildColorTables(cs, lut, d.matrix);
+                std::memcpy(d.hostLut, lut.data(), 65536 * sizeof(float));
+                hipCheck(hipMemcpyAsync(d.lut, d.hostLut, 65536 * sizeof(float), hipMemcpyHostToDevice, stream_),
+                         "EOTF table upload");
+                d.lutSpec = cs;
+                d.lutValid = true;
+            }
         }
-        hostAlloc(&hostIn_[f], d.inBytes);
-        d.firstBlock = (uint32_t)blocks.size();
+        d.firstBlock = nblocks;
         d.numBlocks = (uint32_t)L.numCblks();
-        const uint32_t bandBase = (uint32_t)bands.size();
+        if (nblocks + d.numBlocks > blockCap_ || nbands + L.bands.size() > bandCap_)
+            throw std::logic_error("frame layout exceeds the job's block capacity");
+        const uint32_t bandBase = nbands;
         for (const Subband& b : L.bands)
-            bands.push_back(T1Band{1.0f / b.delta, ((1u << b.Mb) << 6) - 1u, (float)((double)b.delta * b.delta * b.weight), 0u});
+            hBands_[nbands++] = T1Band{1.0f / b.delta, ((1u << b.Mb) << 6) - 1u, (float)((double)b.delta * b.delta * b.weight), 0u};
+        T1Block* out = hBlocks_ + nblocks;
         for (int i = 0; i < L.numCblks(); ++i) {
             const CodeBlock& cb = L.cblks[i];
             const Subband& b = L.bands[cb.band];
@@ -199,27 +330,19 @@ void Job::allocate() {
             t.h = (uint8_t)cb.h;
             t.orient = (uint8_t)b.orient;
             t.frame_block = (uint32_t)i;
-            blocks.push_back(t);
+            out[i] = t;
         }
-        totalCoefs_ += 3 * d.npix;
+        nblocks += d.numBlocks;
         maxLevels = std::max(maxLevels, L.levels);
+        hFrames_[f] = T1FrameRef{d.coef, (uint32_t)W, 0.f};
     }
-    nblocks_ = (uint32_t)blocks.size();
-    if (bands.size() > 65535 || nf > 65535) throw std::logic_error("job too large");
-    dBlocks_ = devAlloc<T1Block>(nblocks_);
-    hipCheck(hipMemcpy(dBlocks_, blocks.data(), blocks.size() * sizeof(T1Block), hipMemcpyHostToDevice), "block table");
-    dBands_ = devAlloc<T1Band>(bands.size());
-    hipCheck(hipMemcpy(dBands_, bands.data(), bands.size() * sizeof(T1Band), hipMemcpyHostToDevice), "band table");
-    dFrames_ = devAlloc<T1FrameRef>(nf);
-    hostAlloc((void**)&hFrames_, sizeof(T1FrameRef) * nf);
-    for (int f = 0; f < nf; ++f) hFrames_[f] = T1FrameRef{fd_[f].coef, (uint32_t)frames_[f].layout.width, 0.f};
-
+    if (maxLevels > levelCap_) throw std::logic_error("frame has more levels than its slot");
     // DWT level descriptors: level l of every frame that has it, tiles numbered across frames.
-    levelDescs_.assign(maxLevels, {});
+    levelCount_.assign(maxLevels, 0);
     levelTiles_.assign(maxLevels, 0);
     for (int f = 0; f < nf; ++f) {
-        const Layout& L = frames_[f].layout;
-        FrameDev& d = fd_[f];
+        const Layout& L = frames[f].layout;
+        SlotDev& d = sd_[f];
         int w = L.width, h = L.height;
         for (int l = 0; l < L.levels; ++l) {
             DwtLevelDesc ds{};
@@ -250,48 +373,26 @@ void Job::allocate() {
             ds.tiles_x = (w + tx - 1) / tx;
             ds.tile_base = (uint32_t)levelTiles_[l];
             levelTiles_[l] += ds.tiles_x * ((h + ty - 1) / ty);
-            levelDescs_[l].push_back(ds);
+            hLevels_[(size_t)l * numSlots() + levelCount_[l]++] = ds;
             w = w2;
             h = h2;
         }
     }
-    for (int l = 0; l < maxLevels; ++l) {
-        DwtLevelDesc* p = devAlloc<DwtLevelDesc>(levelDescs_[l].size());
-        hipCheck(hipMemcpy(p, levelDescs_[l].data(), levelDescs_[l].size() * sizeof(DwtLevelDesc), hipMemcpyHostToDevice),
-                 "DWT descriptors");
-        dLevelDescs_.push_back(p);
-    }
-
-    const uint32_t n = nblocks_;
-    dInfo_ = devAlloc<T1Info>(n);
-    dPassEnd_ = devAlloc<uint32_t>((size_t)n * kMaxPasses);
-    dPassDist_ = devAlloc<int32_t>((size_t)n * kMaxPasses);
-    dPassLen_ = devAlloc<uint32_t>((size_t)n * kMaxPasses);
-    dChunks_ = devAlloc<uint2>((size_t)n * kMaxPlanes);
-    dSnaps_ = devAlloc<MqSnapshot>((size_t)n * kMaxPasses);
-    dPerm_ = devAlloc<uint32_t>(n);
-    dHist_ = devAlloc<uint32_t>(128);
-    dOutOff_ = devAlloc<unsigned long long>(n);
-    dDataOff_ = devAlloc<unsigned long long>(n);
-    dPartial_ = devAlloc<unsigned long long>(n / 2048 + 4);
-    dCodeLen_ = devAlloc<uint32_t>(n);
-    dCounters_ = devAlloc<JobCounters>(1);
-    dWork_ = devAlloc<uint32_t>(4);
-    hostAlloc((void**)&hSmall_, 256);
-    hostAlloc((void**)&hInfo_, (size_t)n * 20);
-    hPassCap_ = (size_t)n * kMaxPasses;
-    hostAlloc((void**)&hPassLen_, hPassCap_ * 4);
-    hostAlloc((void**)&hPassDist_, hPassCap_ * 4);
-    hDataCap_ = std::max<size_t>(16ull << 20, totalCoefs_);
-    hostAlloc((void**)&hData_, hDataCap_);
-    const Sizes z = t1Sizes(totalCoefs_, n, opt_.poolBytesPerCoef);
-    poolCap_ = z.pool;
-    outCap_ = z.out;
-    dPool_ = devAlloc<uint8_t>(poolCap_);
-    dOut_ = devAlloc<uint8_t>(outCap_);
-    fixedInfo_.assign(nf, {});
+    hipStream_t s = stream_;
+    hipCheck(hipMemcpyAsync(dBlocks_, hBlocks_, sizeof(T1Block) * nblocks, hipMemcpyHostToDevice, s), "block table");
+    hipCheck(hipMemcpyAsync(dBands_, hBands_, sizeof(T1Band) * nbands, hipMemcpyHostToDevice, s), "band table");
+    hipCheck(hipMemcpyAsync(dLevels_, hLevels_, sizeof(DwtLevelDesc) * maxLevels * numSlots(), hipMemcpyHostToDevice, s),
+             "DWT descriptors");
+    nblocks_ = nblocks;
+    numLevels_ = maxLevels;
+    frames_ = std::move(frames);
     frameDataBase_.assign(nf, 0);
     framePassBase_.assign(nf, 0);
+    launched_ = false;
+    if (graph_) {  // the captured chain belongs to the previous composition
+        hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
 }
 
 // Replaces the symbol pool and / or the codeword buffer with ones twice as large (after a
@@ -318,9 +419,30 @@ void Job::growT1(bool pool, bool out) {
     }
 }
 
+void* Job::hostInput(int i) {
+    if (i < 0 || i >= numSlots()) throw std::logic_error("no such frame slot");
+    if (!hostIn_[i]) hostAlloc(&hostIn_[i], sd_[i].inCap);
+    return hostIn_[i];
+}
+
 void Job::upload(int i, const void* frame) {
+    if (i < 0 || i >= numFrames()) throw std::logic_error("no frame bound to this slot");
     hipCheck(hipSetDevice(dev_), "hipSetDevice");
-    hipCheck(hipMemcpy(fd_[i].in, frame, fd_[i].inBytes, hipMemcpyHostToDevice), "frame upload");
+    hipCheck(hipStreamSynchronize(stream_), "previous run");
+    hipCheck(hipMemcpy(sd_[i].in, frame, sd_[i].inBytes, hipMemcpyHostToDevice), "frame upload");
+}
+
+void Job::uploadFrom(int i, const void* src) {
+    if (i < 0 || i >= numFrames()) throw std::logic_error("no frame bound to this slot");
+    hipCheck(hipSetDevice(dev_), "hipSetDevice");
+    if (launched_) hipCheck(hipStreamSynchronize(stream_), "previous run");
+    hipCheck(hipMemcpyAsync(sd_[i].in, src, sd_[i].inBytes, hipMemcpyHostToDevice, stream_), "frame upload");
+}
+
+void Job::uploadAsync(int i) {
+    if (i < 0 || i >= numFrames()) throw std::logic_error("no frame bound to this slot");
+    hipCheck(hipSetDevice(dev_), "hipSetDevice");
+    hipCheck(hipMemcpyAsync(sd_[i].in, hostInput(i), sd_[i].inBytes, hipMemcpyHostToDevice, stream_), "frame upload");
 }
 
 void Job::enqueueT1(hipStream_t s) {
@@ -402,18 +524,18 @@ void Job::enqueueChain(bool uploadInputs) {
     if (!capturing_) hipCheck(hipEventRecord(ev_[0], s), "event");
     if (uploadInputs)
         for (int f = 0; f < numFrames(); ++f)
-            hipCheck(hipMemcpyAsync(fd_[f].in, hostIn_[f], fd_[f].inBytes, hipMemcpyHostToDevice, s), "frame upload");
+            hipCheck(hipMemcpyAsync(sd_[f].in, hostInput(f), sd_[f].inBytes, hipMemcpyHostToDevice, s), "frame upload");
     hipCheck(hipMemcpyAsync(dFrames_, hFrames_, sizeof(T1FrameRef) * numFrames(), hipMemcpyHostToDevice, s), "frame table");
     if (!capturing_) hipCheck(hipEventRecord(ev_[1], s), "event");
     for (int f = 0; f < numFrames(); ++f) {
         const InputFormat& in = frames_[f].input;
         if (in.kind == InputFormat::XyzPlanar) continue;
         ColorDesc c{};
-        c.src = fd_[f].in;
-        c.codes = fd_[f].codes;
-        c.lut = fd_[f].lut;
+        c.src = sd_[f].in;
+        c.codes = sd_[f].codes;
+        c.lut = sd_[f].lut;
         c.thresholds = ctx_.thresholds();
-        std::memcpy(c.matrix, fd_[f].matrix, sizeof c.matrix);
+        std::memcpy(c.matrix, sd_[f].matrix, sizeof c.matrix);
         c.width = in.width;
         c.height = in.height;
         c.kind = in.kind == InputFormat::RGB48 ? 0 : 1;
@@ -435,8 +557,8 @@ void Job::enqueueChain(bool uploadInputs) {
         hipCheck(launch_color(c, s), "colour");
     }
     if (!capturing_) hipCheck(hipEventRecord(ev_[2], s), "event");
-    for (size_t l = 0; l < levelDescs_.size(); ++l) {
-        hipCheck(launch_dwt_level(l == 0, opt_.dwtTile, dLevelDescs_[l], (int)levelDescs_[l].size(), levelTiles_[l], s),
+    for (int l = 0; l < numLevels_; ++l) {
+        hipCheck(launch_dwt_level(l == 0, opt_.dwtTile, dLevels_ + (size_t)l * numSlots(), levelCount_[l], levelTiles_[l], s),
                  "DWT level");
         if (l == 0 && !capturing_) hipCheck(hipEventRecord(ev_[8], s), "event");
     }
@@ -445,8 +567,14 @@ void Job::enqueueChain(bool uploadInputs) {
 }
 
 void Job::launch(bool uploadInputs) {
+    if (frames_.empty()) throw std::logic_error("no frames bound to the job");
     hipCheck(hipSetDevice(dev_), "hipSetDevice");
+    // hFrames_ is staged from the host: a run still in flight may be reading it
+    if (launched_) hipCheck(hipStreamSynchronize(stream_), "previous run");
     for (int f = 0; f < numFrames(); ++f) hFrames_[f].slope_thresh = slope_[f];
+    launched_ = true;
+    if (uploadInputs)  // staging buffers exist before any capture records copies from them
+        for (int f = 0; f < numFrames(); ++f) hostInput(f);
     if (opt_.useGraph) {
         if (!graph_ || graphUpload_ != uploadInputs) {
             if (graph_) hipGraphExecDestroy(graph_);
@@ -471,13 +599,25 @@ void Job::launch(bool uploadInputs) {
         hipCheck(hipEventRecord(ev_[0], stream_), "event");
         hipCheck(hipGraphLaunch(graph_, stream_), "graph launch");
         hipCheck(hipEventRecord(ev_[6], stream_), "event");
+        hipCheck(hipEventRecord(evDone_, stream_), "event");
         return;
     }
     enqueueChain(uploadInputs);
+    hipCheck(hipEventRecord(evDone_, stream_), "event");
+}
+
+bool Job::done() const {
+    if (!launched_) return true;
+    const hipError_t e = hipEventQuery(evDone_);
+    if (e == hipErrorNotReady) return false;
+    hipCheck(e, "encode");
+    return true;
 }
 
 void Job::finish(StageTimes* times) {
+    if (!launched_) throw std::logic_error("finish() without a launched run");
     hipCheck(hipStreamSynchronize(stream_), "encode");
+    launched_ = false;
     int reruns = 0;
     for (;;) {
         const JobCounters* c = (const JobCounters*)hSmall_;
@@ -492,6 +632,15 @@ void Job::finish(StageTimes* times) {
     const unsigned long long packed = hSmall_[16];
     numDataTotal_ = (size_t)(packed & ((1ull << 36) - 1));
     numPassesTotal_ = (size_t)(packed >> 36);
+    if (numPassesTotal_ > hPassCap_) {
+        hipHostFree(hPassLen_);
+        hipHostFree(hPassDist_);
+        hPassLen_ = nullptr;
+        hPassDist_ = nullptr;
+        hPassCap_ = std::min<size_t>((size_t)blockCap_ * kMaxPasses, numPassesTotal_ + (numPassesTotal_ >> 2));
+        hostAlloc((void**)&hPassLen_, hPassCap_ * 4);
+        hostAlloc((void**)&hPassDist_, hPassCap_ * 4);
+    }
     if (numDataTotal_ > hDataCap_) {
         hipHostFree(hData_);
         hData_ = nullptr;
@@ -507,11 +656,11 @@ void Job::finish(StageTimes* times) {
     hipCheck(hipStreamSynchronize(s), "result download");
     // frame-local offsets
     for (int f = 0; f < numFrames(); ++f) {
-        uint32_t* inf = hInfo_ + (size_t)fd_[f].firstBlock * 5;
+        uint32_t* inf = hInfo_ + (size_t)sd_[f].firstBlock * 5;
         const uint32_t d0 = inf[2], p0 = inf[4];
         frameDataBase_[f] = d0;
         framePassBase_[f] = p0;
-        for (uint32_t b = 0; b < fd_[f].numBlocks; ++b) {
+        for (uint32_t b = 0; b < sd_[f].numBlocks; ++b) {
             inf[5 * b + 2] -= d0;
             inf[5 * b + 4] -= p0;
         }
@@ -547,7 +696,7 @@ void Job::encode(bool uploadInputs, StageTimes* times) {
 }
 
 Tier1View Job::result(int i) const {
-    const FrameDev& d = fd_[i];
+    const SlotDev& d = sd_[i];
     const size_t next = i + 1 < numFrames() ? frameDataBase_[i + 1] : numDataTotal_;
     const size_t nextP = i + 1 < numFrames() ? framePassBase_[i + 1] : numPassesTotal_;
     Tier1View v;
@@ -562,19 +711,19 @@ Tier1View Job::result(int i) const {
 }
 
 void Job::debugCodes(int i, std::vector<uint16_t>& out) {
-    out.resize(3 * fd_[i].npix);
+    out.resize(3 * sd_[i].npix);
     hipCheck(hipStreamSynchronize(stream_), "sync");
-    hipCheck(hipMemcpy(out.data(), fd_[i].codes, out.size() * 2, hipMemcpyDeviceToHost), "codes download");
+    hipCheck(hipMemcpy(out.data(), sd_[i].codes, out.size() * 2, hipMemcpyDeviceToHost), "codes download");
 }
 
 void Job::debugWavelet(int i, std::vector<float>& out) {
-    out.resize(3 * fd_[i].npix);
+    out.resize(3 * sd_[i].npix);
     hipCheck(hipStreamSynchronize(stream_), "sync");
-    hipCheck(hipMemcpy(out.data(), fd_[i].coef, out.size() * 4, hipMemcpyDeviceToHost), "wavelet download");
+    hipCheck(hipMemcpy(out.data(), sd_[i].coef, out.size() * 4, hipMemcpyDeviceToHost), "wavelet download");
 }
 
 void Job::debugCoeffs(int i, std::vector<uint32_t>& out) {
-    const FrameDev& d = fd_[i];
+    const SlotDev& d = sd_[i];
     uint32_t* tmp = nullptr;
     hipCheck(hipMalloc(&tmp, (size_t)d.numBlocks * 4096), "debug buffer");
     hipCheck(launch_quant_debug(dBlocks_ + d.firstBlock, dBands_, dFrames_, d.numBlocks, tmp, stream_), "quantiser");
@@ -586,7 +735,7 @@ void Job::debugCoeffs(int i, std::vector<uint32_t>& out) {
 }
 
 void Job::debugInfo(int i, std::vector<T1Info>& out) {
-    const FrameDev& d = fd_[i];
+    const SlotDev& d = sd_[i];
     out.resize(d.numBlocks);
     hipCheck(hipStreamSynchronize(stream_), "sync");
     hipCheck(hipMemcpy(out.data(), dInfo_ + d.firstBlock, d.numBlocks * sizeof(T1Info), hipMemcpyDeviceToHost), "info");

@@ -79,36 +79,65 @@ private:
     std::string name_, arch_;
 };
 
+// Room one frame slot of a job holds: any frame up to maxWidth x maxHeight, and (colour)
+// the code planes and EOTF table that planar or RGB48 inputs need.
+struct SlotCapacity {
+    int maxWidth = 0, maxHeight = 0;
+    bool colour = false;
+    bool fits(const FrameConfig& f) const;
+};
+
 class Job {
 public:
+    // A job with slots of the given capacities and nothing bound yet.
+    Job(int device, std::vector<SlotCapacity> slots, const JobOptions& opt);
+    // A job whose slots are exactly these frames, bound.
     Job(int device, std::vector<FrameConfig> frames, const JobOptions& opt);
     ~Job();
     Job(const Job&) = delete;
     Job& operator=(const Job&) = delete;
 
+    // Assigns frames to slots 0..frames.size()-1 for the following runs (frame i must fit
+    // slot i). Waits for the job's previous run; the tables go to the device on the job's
+    // stream ahead of the next launch.
+    void bind(std::vector<FrameConfig> frames);
+
+    int numSlots() const { return (int)slots_.size(); }
+    const SlotCapacity& slot(int i) const { return slots_[i]; }
     int numFrames() const { return (int)frames_.size(); }
     const FrameConfig& frame(int i) const { return frames_[i]; }
     hipStream_t stream() const { return stream_; }
     // Device memory this job holds.
     size_t deviceBytes() const { return bytes_; }
-    // Device memory a job with these frames would take (before any growth).
+    // Device memory a job with these slots (or frames) would take before any growth.
+    static size_t estimateBytes(const std::vector<SlotCapacity>& slots, const JobOptions& opt);
     static size_t estimateBytes(const std::vector<FrameConfig>& frames, const JobOptions& opt);
 
-    // Pinned host staging buffer of frame i (input.bytes()).
-    void* hostInput(int i) { return hostIn_[i]; }
-    // Copies frame i from host memory into its device input buffer (synchronously).
+    // Pinned host staging buffer of slot i (room for any input the slot takes), allocated
+    // on first use.
+    void* hostInput(int i);
+    // Copies the bound frame i from host memory into its device input buffer (synchronously).
     void upload(int i, const void* frame);
+    // The same on the job's stream from the slot's pinned staging buffer, ahead of the
+    // next launch (the staging buffer must hold the frame until that run is done).
+    void uploadAsync(int i);
+    // The same from any host buffer (kept valid by the caller until the run is done).
+    void uploadFrom(int i, const void* src);
     // Early termination threshold of frame i (0: code every pass).
     void setSlope(int i, double t) { slope_[i] = (float)t; }
 
-    // Runs the whole chain. uploadInputs: copy every frame's pinned staging buffer to the
-    // device first (inside the timed chain); otherwise the frames already resident in HBM
-    // are encoded. Blocks until the results are in the job's pinned host buffers.
+    // Runs the whole chain on the bound frames. uploadInputs: copy every frame's pinned
+    // staging buffer to the device first (inside the timed chain); otherwise the frames
+    // already resident in HBM are encoded. Blocks until the results are in the job's pinned
+    // host buffers.
     void encode(bool uploadInputs, StageTimes* times = nullptr);
     // The same in two halves, so that host work can overlap the device: launch() enqueues
     // the chain, finish() waits for it and fetches the results.
     void launch(bool uploadInputs);
     void finish(StageTimes* times = nullptr);
+    // True once the last launched chain has completed on the device (finish() still has to
+    // be called to fetch its results).
+    bool done() const;
 
     Tier1View result(int i) const;
     // Stage times of the last run (ms) and its symbol count.
@@ -123,14 +152,20 @@ public:
     void debugInfo(int i, std::vector<T1Info>& out);
 
 private:
-    struct FrameDev {
-        size_t inBytes = 0, npix = 0;
-        void* in = nullptr;          // device input (u16 codes for XyzPlanar)
-        uint16_t* codes = nullptr;   // 3 planes of codes (== in for XyzPlanar)
+    struct SlotDev {
+        size_t inCap = 0, pixCap = 0;
+        void* in = nullptr;          // device input
+        uint16_t* codesBuf = nullptr;  // 3 planes of codes for colour inputs
         float* coef = nullptr;       // 3 Mallat planes
         float* llA = nullptr;        // LL scratch, 3 x ceil(W/2) x ceil(H/2)
         float* llB = nullptr;        // LL scratch, 3 x ceil(W/4) x ceil(H/4)
-        float* lut = nullptr;        // EOTF table (colour inputs only)
+        float* lut = nullptr;        // EOTF table (colour slots only)
+        float* hostLut = nullptr;    // pinned copy of the table last uploaded
+        bool lutValid = false;
+        ColorSpec lutSpec;
+        // bound frame
+        size_t inBytes = 0, npix = 0;
+        uint16_t* codes = nullptr;   // == in for X'Y'Z' planar input
         float matrix[12] = {};
         uint32_t firstBlock = 0, numBlocks = 0;
     };
@@ -145,29 +180,39 @@ private:
     int dev_;
     DeviceContext& ctx_;
     JobOptions opt_;
+    std::vector<SlotCapacity> slots_;
+    std::vector<SlotDev> sd_;
     std::vector<FrameConfig> frames_;
-    std::vector<FrameDev> fd_;
     std::vector<void*> hostIn_;
     std::vector<float> slope_;
     std::vector<void*> allocs_;
     size_t bytes_ = 0;
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_[9] = {};  // ev_[8]: after DWT level 0
+    hipEvent_t evDone_ = nullptr;
     hipGraphExec_t graph_ = nullptr;
     bool graphUpload_ = false;
     bool capturing_ = false;
+    bool launched_ = false;
     StageTimes last_;
 
+    // capacities
+    uint32_t blockCap_ = 0, bandCap_ = 0;
+    size_t coefCap_ = 0;
+    int levelCap_ = 0;
+    // bound composition
     uint32_t nblocks_ = 0;
-    size_t totalCoefs_ = 0;
-    // job-wide tables
+    int numLevels_ = 0;
+    std::vector<int> levelCount_, levelTiles_;
+    // job-wide tables (device, and pinned staging for their upload)
     T1Block* dBlocks_ = nullptr;
+    T1Block* hBlocks_ = nullptr;
     T1Band* dBands_ = nullptr;
+    T1Band* hBands_ = nullptr;
     T1FrameRef* dFrames_ = nullptr;
-    T1FrameRef* hFrames_ = nullptr;  // pinned, copied before each run
-    std::vector<std::vector<DwtLevelDesc>> levelDescs_;
-    std::vector<DwtLevelDesc*> dLevelDescs_;
-    std::vector<int> levelTiles_;
+    T1FrameRef* hFrames_ = nullptr;  // copied before each run (it carries the thresholds)
+    DwtLevelDesc* dLevels_ = nullptr;  // levelCap_ x slots
+    DwtLevelDesc* hLevels_ = nullptr;
     // Tier-1 arrays
     T1Info* dInfo_ = nullptr;
     uint32_t* dPassEnd_ = nullptr;
@@ -201,7 +246,6 @@ private:
     unsigned long long* hSmall_ = nullptr;  // counters + totals
     size_t numPassesTotal_ = 0, numDataTotal_ = 0;
     std::vector<size_t> frameDataBase_, framePassBase_;
-    std::vector<std::vector<uint32_t>> fixedInfo_;
 };
 
 // Throws with the HIP error text.

@@ -90,10 +90,11 @@ __global__ void __launch_bounds__(256) dwt_level_kernel(const DwtLevelDesc* __re
         const int gy = mirror(y0 - 4 + ry, h), gx = mirror(x0 - 4 + rx, w);
         const size_t at = (size_t)gy * d.src_stride + gx;
         if constexpr (kLevel0) {
+            // 12-bit codes (words above 4095 read as 4095, as the host copy clips them)
             const uint16_t* s = (const uint16_t*)d.src;
-            const float x = (float)((int)s[at] - 2048);
-            const float y = (float)((int)s[d.src_plane + at] - 2048);
-            const float z = (float)((int)s[2 * (size_t)d.src_plane + at] - 2048);
+            const float x = (float)(min((int)s[at], 4095) - 2048);
+            const float y = (float)(min((int)s[d.src_plane + at], 4095) - 2048);
+            const float z = (float)(min((int)s[2 * (size_t)d.src_plane + at], 4095) - 2048);
             tile[0][ry][rx] = kY0[0] * x + kY0[1] * y + kY0[2] * z;
             tile[1][ry][rx] = kY1[0] * x + kY1[1] * y + kY1[2] * z;
             tile[2][ry][rx] = kY2[0] * x + kY2[1] * y + kY2[2] * z;

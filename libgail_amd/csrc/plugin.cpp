@@ -5,7 +5,10 @@
 
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <tuple>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -65,9 +68,22 @@ template <class F> int guarded(F&& f) {
 
 // GailHipLayout -> Layout: the geometry is rebuilt from the size and levels, the subbands
 // (which carry the step sizes) are taken as they are (cuda_plugin.h:245-253).
+// Geometry of a size and decomposition, built once per process (the served form binds the
+// same few DCI sizes over and over).
+const Layout& cachedGeometry(int width, int height, bool fourk, int levels, int guardBits) {
+    static std::mutex m;
+    static std::map<std::tuple<int, int, bool, int, int>, std::unique_ptr<Layout>> cache;
+    std::lock_guard<std::mutex> l(m);
+    auto& p = cache[std::make_tuple(width, height, fourk, levels, guardBits)];
+    if (!p) p = std::make_unique<Layout>(makeGeometry(width, height, fourk, levels, guardBits));
+    return *p;
+}
+
 Layout toLayout(const GailHipLayout& p) {
     if (!p.bands || p.numBands <= 0) throw std::logic_error("frame layout is empty");
-    Layout L = makeGeometry(p.width, p.height, p.profile == 1, p.levels, p.guardBits);
+    if (p.width <= 0 || p.height <= 0 || p.width > 65535 || p.height > 65535 || p.levels < 1 || p.levels > 8)
+        throw std::logic_error("frame layout size or levels out of range");
+    Layout L = cachedGeometry(p.width, p.height, p.profile == 1, p.levels, p.guardBits);
     if ((int)L.bands.size() != p.numBands || L.numCblks() != p.numCblks)
         throw std::logic_error("frame layout does not have the geometry of its size and levels");
     L.baseStep = p.baseStep;
@@ -253,7 +269,11 @@ int encoderCreate(GailHipDevice* d, const GailHipLayout* l, const GailHipEncodeS
 }
 
 void encoderDestroy(GailHipEncoder* e) { delete e; }
-void* encoderInput(GailHipEncoder* e) { return e->job->hostInput(0); }
+void* encoderInput(GailHipEncoder* e) {
+    void* p = nullptr;
+    guarded([&] { p = e->job->hostInput(0); });
+    return p;
+}
 void encoderSetSlope(GailHipEncoder* e, double t) { e->job->setSlope(0, t); }
 
 int encoderEncode(GailHipEncoder* e, const void* frame, GailHipTier1* out, GailHipFrameTimings* timings,
@@ -308,7 +328,9 @@ int batchCreate(GailHipDevice* d, const GailHipLayout* l, const GailHipEncodeSet
 
 void batchDestroy(GailHipBatch* b) { delete b; }
 void* batchInput(GailHipBatch* b, int i) {
-    return i >= 0 && i < b->job->numFrames() ? b->job->hostInput(i) : nullptr;
+    void* p = nullptr;
+    guarded([&] { p = b->job->hostInput(i); });
+    return p;
 }
 int batchUpload(GailHipBatch* b, int i, const void* frame) {
     return guarded([&] {
@@ -351,6 +373,60 @@ int batchStageMs(GailHipBatch* b, double* out, int n) {
     });
 }
 
+std::vector<SlotCapacity> toSlots(const GailHipSlot* s, int count) {
+    if (count <= 0 || !s) throw std::logic_error("a batch needs at least one frame slot");
+    std::vector<SlotCapacity> v;
+    for (int i = 0; i < count; ++i) v.push_back(SlotCapacity{s[i].maxWidth, s[i].maxHeight, s[i].colour != 0});
+    return v;
+}
+
+uint64_t slotsBytes(GailHipDevice*, const GailHipSlot* s, int count, const GailHipBatchOptions* o) {
+    try {
+        return Job::estimateBytes(toSlots(s, count), toOptions(o));
+    } catch (...) {
+        return 0;
+    }
+}
+
+int slotsCreate(GailHipDevice* d, const GailHipSlot* s, int count, const GailHipBatchOptions* o, GailHipBatch** out) {
+    return guarded([&] {
+        auto b = std::make_unique<GailHipBatch>();
+        b->job = std::make_unique<Job>(d->index, toSlots(s, count), toOptions(o));
+        *out = b.release();
+    });
+}
+
+int batchBind(GailHipBatch* b, const GailHipLayout* l, const GailHipEncodeSettings* s, int count) {
+    return guarded([&] { b->job->bind(toFrames(l, s, count)); });
+}
+
+int batchQuery(GailHipBatch* b) {
+    int r = 0;
+    const int st = guarded([&] { r = b->job->done() ? 1 : 0; });
+    return st == GAIL_HIP_OK ? r : -st;
+}
+
+int batchUploadFrom(GailHipBatch* b, int i, const void* src) {
+    return guarded([&] {
+        if (!src) throw std::logic_error("no source frame");
+        b->job->uploadFrom(i, src);
+    });
+}
+
+void* hostAllocEntry(uint64_t bytes) {
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes ? bytes : 16, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        g_error = "out of pinned host memory";
+        return nullptr;
+    }
+    return p;
+}
+
+void hostFreeEntry(void* p) {
+    if (p) hipHostFree(p);
+}
+
 #define GAIL_STR2(x) #x
 #define GAIL_STR(x) GAIL_STR2(x)
 
@@ -390,6 +466,13 @@ const GailHipPlugin g_plugin = {
     batchWait,
     batchDebug,
     batchStageMs,
+    slotsBytes,
+    slotsCreate,
+    batchBind,
+    batchQuery,
+    batchUploadFrom,
+    hostAllocEntry,
+    hostFreeEntry,
 };
 
 }  // namespace

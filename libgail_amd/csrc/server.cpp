@@ -1,0 +1,563 @@
+// Served form of libgail.so (gail_server_*, include/gail_amd.h): a request loop on one device
+// for frames of many streams with mixed profiles and input formats.
+//
+// Device memory is held by a few jobs of the plugin (batches of frame slots, each slot room
+// for one 4K frame of any input); their number comes from the memory budget. A scheduler
+// thread admits waiting frames (oldest first) into whichever job is idle, as many as it has
+// slots, binds them, queues their inputs (DMA straight from pinned app memory when the frame
+// already has the device layout, else a host copy on the pool) and launches the job; jobs run
+// concurrently on their own streams. A waiter thread per job collects its Tier-1 results and
+// hands each frame to the host pool for rate control, Tier-2 and delivery; the job is idle
+// again once all of its frames are done. A frame whose lambda falls below its early-
+// termination threshold is queued again, first in line, to be coded without it
+// (pipeline.cpp:194-218).
+#include <sched.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "codestream.h"
+#include "gail_amd.h"
+#include "gail_hip_plugin.h"
+#include "stream.h"
+
+using namespace gail_amd;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+int sfail(int status, const std::string& m) {
+    lastErrorText() = m;
+    return status;
+}
+
+double msSince(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); }
+
+// Fixed set of host threads running queued tasks.
+class Pool {
+public:
+    explicit Pool(int n) {
+        for (int i = 0; i < n; ++i) threads_.emplace_back([this] { run(); });
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> l(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : threads_) t.join();
+    }
+    void post(std::function<void()> f, bool urgent = false) {
+        {
+            std::lock_guard<std::mutex> l(m_);
+            if (urgent) q_.push_front(std::move(f));
+            else q_.push_back(std::move(f));
+        }
+        cv_.notify_one();
+    }
+
+private:
+    void run() {
+        for (;;) {
+            std::function<void()> f;
+            {
+                std::unique_lock<std::mutex> l(m_);
+                cv_.wait(l, [&] { return stop_ || !q_.empty(); });
+                if (q_.empty()) return;
+                f = std::move(q_.front());
+                q_.pop_front();
+            }
+            f();
+        }
+    }
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::deque<std::function<void()>> q_;
+    std::vector<std::thread> threads_;
+    bool stop_ = false;
+};
+
+struct StreamState {
+    std::unique_ptr<StreamSpec> spec;
+    std::atomic<double> lastLambda{0.0};
+};
+
+struct Request {
+    uint64_t id = 0;
+    uint32_t stream = 0;
+    StreamState* st = nullptr;
+    gail_frame frame{};
+    double slope = 0;
+    bool autoSlope = false;
+    bool recoded = false;
+    Clock::time_point tSubmit, tAdmit;
+    int batchFrames = 0;
+};
+
+struct JobState {
+    GailHipBatch* batch = nullptr;
+    std::vector<std::unique_ptr<Request>> reqs;
+    std::vector<GailHipTier1> t1;
+    int rcLeft = 0;
+    bool busy = false;
+    bool launched = false;  // waiter: a run is on the device
+    std::thread waiter;
+};
+
+}  // namespace
+
+struct gail_server {
+    gail_server_config cfg{};
+    const GailHipPlugin* P = nullptr;
+    GailHipDevice* dev = nullptr;
+    std::unique_ptr<Pool> pool;
+    std::vector<std::unique_ptr<JobState>> jobs;
+    int slotsPerJob = 0;
+    uint64_t jobBytes = 0, slotBytes = 0;
+
+    std::mutex m;  // everything below
+    std::condition_variable cv;
+    std::deque<std::unique_ptr<Request>> pending;
+    std::map<uint32_t, std::unique_ptr<StreamState>> streams;
+    bool stop = false;
+    uint64_t submitted = 0, delivered = 0, failedCount = 0, recodedCount = 0, jobRuns = 0, jobFrames = 0;
+    int inFlight = 0, maxInFlight = 0;
+    bool haveFirst = false;
+    Clock::time_point tFirst, tLast;
+    std::vector<double> latency, queued;
+    std::thread scheduler;
+    std::mutex deliverMutex;  // one callback at a time
+
+    void deliver(Request& r, int status, const gail_packet* pkt, double lambda) {
+        const Clock::time_point now = Clock::now();
+        gail_server_result res{};
+        res.id = r.id;
+        res.stream = r.stream;
+        res.status = status;
+        res.packet = pkt;
+        res.queued_ms = msSince(r.tSubmit, r.tAdmit);
+        res.latency_ms = msSince(r.tSubmit, now);
+        res.batch_frames = r.batchFrames;
+        res.recoded = r.recoded ? 1 : 0;
+        res.lambda = lambda;
+        if (cfg.on_result) {
+            std::lock_guard<std::mutex> l(deliverMutex);
+            cfg.on_result(cfg.user, &res);
+        }
+        std::lock_guard<std::mutex> l(m);
+        ++delivered;
+        --inFlight;
+        if (status != GAIL_OK) ++failedCount;
+        if (r.recoded) ++recodedCount;
+        latency.push_back(res.latency_ms);
+        queued.push_back(res.queued_ms);
+        tLast = now;
+        cv.notify_all();
+    }
+
+    // Rate control, Tier-2 and delivery of frame i of job j (pool thread).
+    void finishFrame(JobState& j, int i) {
+        Request& r = *j.reqs[i];
+        const StreamSpec& sp = *r.st->spec;
+        bool requeue = false;
+        try {
+            const GailHipTier1& t = j.t1[i];
+            const Tier1Ref ref{(const uint32_t*)t.info, t.passLen, t.passDist, t.data, (size_t)t.numInfo};
+            RateResult rr = allocateRate(sp.L, ref, sp.limits);
+            const double lam = std::min({rr.lambdaComp[0], rr.lambdaComp[1], rr.lambdaComp[2]});
+            if (r.autoSlope && r.slope > 0 && lam < r.slope) {
+                requeue = true;  // passes the rate control keeps may have been skipped
+            } else {
+                std::vector<uint8_t> cs;
+                buildCodestream(sp.L, ref, rr.passes.data(), &cs, nullptr, "hip");
+                r.st->lastLambda.store(lam);
+                gail_packet p{};
+                p.data = cs.data();
+                p.size = cs.size();
+                p.index = (int64_t)r.id;
+                p.opaque = r.frame.opaque;
+                p.compliant = 1;
+                if (cfg.check) p.compliant = checkDciCodestream(cs.data(), cs.size(), sp.eyeFps(), sp.fourk()).empty();
+                const double mse = rr.distortion / ((double)sp.width * sp.height * 3);
+                p.psnr_estimate = mse > 0 ? 10 * std::log10(4095.0 * 4095.0 / mse) : 99.0;
+                deliver(r, GAIL_OK, &p, lam);
+            }
+        } catch (const std::exception& e) {
+            lastErrorText() = e.what();
+            deliver(r, GAIL_ERROR_FAILED, nullptr, 0);
+        }
+        std::lock_guard<std::mutex> l(m);
+        if (requeue) {
+            j.reqs[i]->slope = 0;
+            j.reqs[i]->recoded = true;
+            pending.push_front(std::move(j.reqs[i]));
+        }
+        if (--j.rcLeft == 0) {
+            j.reqs.clear();
+            j.busy = false;
+        }
+        cv.notify_all();
+    }
+
+    void waiterLoop(JobState* j) {
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> l(m);
+                cv.wait(l, [&] { return j->launched || (stop && !j->busy); });
+                if (!j->launched) return;
+                j->launched = false;
+            }
+            const int n = (int)j->reqs.size();
+            j->t1.assign(n, GailHipTier1{});
+            if (P->batch_wait(j->batch, j->t1.data(), nullptr) != GAIL_HIP_OK) {
+                const std::string why = P->last_error();
+                for (auto& r : j->reqs) {
+                    lastErrorText() = why;
+                    deliver(*r, GAIL_ERROR_FAILED, nullptr, 0);
+                }
+                std::lock_guard<std::mutex> l(m);
+                j->reqs.clear();
+                j->busy = false;
+                cv.notify_all();
+                continue;
+            }
+            {
+                std::lock_guard<std::mutex> l(m);
+                j->rcLeft = n;
+            }
+            for (int i = 0; i < n; ++i) pool->post([this, j, i] { finishFrame(*j, i); });
+        }
+    }
+
+    // Binds, fills and launches job j with the frames taken from the queue (scheduler).
+    void admit(JobState& j) {
+        const int n = (int)j.reqs.size();
+        std::vector<GailHipLayout> layouts(n);
+        std::vector<GailHipEncodeSettings> settings(n);
+        for (int i = 0; i < n; ++i) {
+            layouts[i] = j.reqs[i]->st->spec->plainLayout;
+            settings[i] = j.reqs[i]->st->spec->settings;
+        }
+        auto failAll = [&](const std::string& why) {
+            for (auto& r : j.reqs) {
+                lastErrorText() = why;
+                deliver(*r, GAIL_ERROR_FAILED, nullptr, 0);
+            }
+            std::lock_guard<std::mutex> l(m);
+            j.reqs.clear();
+            j.busy = false;
+            cv.notify_all();
+        };
+        if (P->batch_bind(j.batch, layouts.data(), settings.data(), n) != GAIL_HIP_OK) return failAll(P->last_error());
+        // host copies of frames not already in the device layout, in parallel on the pool
+        std::vector<const void*> src(n, nullptr);
+        std::string stagingError;
+        std::mutex fm;
+        std::condition_variable fcv;
+        int fills = 0;
+        for (int i = 0; i < n; ++i) {
+            Request& r = *j.reqs[i];
+            const void* direct = r.st->spec->directInput(r.frame);
+            if (direct) {
+                src[i] = direct;
+                continue;
+            }
+            void* dst = P->batch_input(j.batch, i);
+            if (!dst) {
+                if (stagingError.empty()) stagingError = P->last_error();
+                continue;
+            }
+            src[i] = dst;
+            ++fills;
+            // ahead of queued rate control: the device waits for these
+            pool->post(
+                [&, dst, rp = &r] {
+                    rp->st->spec->fill(rp->frame, (uint8_t*)dst);
+                    std::lock_guard<std::mutex> l(fm);
+                    if (--fills == 0) fcv.notify_all();
+                },
+                true);
+        }
+        {
+            std::unique_lock<std::mutex> l(fm);
+            fcv.wait(l, [&] { return fills == 0; });
+        }
+        if (!stagingError.empty()) return failAll(stagingError);
+        for (int i = 0; i < n; ++i) {
+            if (P->batch_upload_from(j.batch, i, src[i]) != GAIL_HIP_OK) return failAll(P->last_error());
+            P->batch_set_slope(j.batch, i, j.reqs[i]->slope);
+        }
+        if (P->batch_launch(j.batch, 0) != GAIL_HIP_OK) return failAll(P->last_error());
+        std::lock_guard<std::mutex> l(m);
+        j.launched = true;
+        ++jobRuns;
+        jobFrames += n;
+        cv.notify_all();
+    }
+
+    void schedulerLoop() {
+        for (;;) {
+            JobState* j = nullptr;
+            {
+                std::unique_lock<std::mutex> l(m);
+                auto idle = [&]() -> JobState* {
+                    for (auto& p : jobs)
+                        if (!p->busy) return p.get();
+                    return nullptr;
+                };
+                cv.wait(l, [&] { return (stop && pending.empty()) || (!pending.empty() && idle()); });
+                if (pending.empty()) return;
+                // batching window: a part-full job waits for more frames, up to the window
+                // counted from the oldest waiting frame's arrival
+                if (cfg.batch_window_us > 0 && !stop) {
+                    const Clock::time_point until = pending.front()->tSubmit + std::chrono::microseconds(cfg.batch_window_us);
+                    cv.wait_until(l, until, [&] { return stop || (int)pending.size() >= slotsPerJob; });
+                    if (pending.empty()) continue;
+                }
+                j = idle();
+                if (!j) continue;
+                j->busy = true;
+                const Clock::time_point now = Clock::now();
+                const int take = std::min<int>(slotsPerJob, (int)pending.size());
+                for (int i = 0; i < take; ++i) {
+                    auto r = std::move(pending.front());
+                    pending.pop_front();
+                    r->tAdmit = now;
+                    r->batchFrames = take;
+                    if (r->autoSlope && !r->recoded) r->slope = r->st->spec->slopeFactor * r->st->lastLambda.load();
+                    j->reqs.push_back(std::move(r));
+                }
+            }
+            admit(*j);
+        }
+    }
+
+    void shutdown() {
+        {
+            std::lock_guard<std::mutex> l(m);
+            stop = true;
+        }
+        cv.notify_all();
+        if (scheduler.joinable()) scheduler.join();
+        for (auto& j : jobs) {
+            {
+                std::unique_lock<std::mutex> l(m);
+                cv.wait(l, [&] { return !j->busy; });
+            }
+            cv.notify_all();
+            if (j->waiter.joinable()) j->waiter.join();
+        }
+        pool.reset();
+        for (auto& j : jobs)
+            if (j->batch) P->batch_destroy(j->batch);
+        jobs.clear();
+        if (dev) P->device_destroy(dev);
+        dev = nullptr;
+    }
+};
+
+extern "C" {
+
+GAIL_API void gail_server_config_init(gail_server_config* c, size_t size) {
+    if (!c) return;
+    gail_server_config d{};
+    d.struct_size = sizeof d;
+    d.slots_per_job = 8;
+    d.max_jobs = 4;
+    std::memcpy(c, &d, std::min(size ? size : sizeof d, sizeof d));
+}
+
+GAIL_API int gail_server_create(const gail_server_config* c, gail_server** out) {
+    if (!c || !out) return sfail(GAIL_ERROR_INVALID, "null argument");
+    *out = nullptr;
+    auto s = std::make_unique<gail_server>();
+    gail_server_config_init(&s->cfg, sizeof s->cfg);
+    std::memcpy(&s->cfg, c, std::min(c->struct_size ? c->struct_size : sizeof *c, sizeof s->cfg));
+    gail_server_config& cfg = s->cfg;
+    if (cfg.slots_per_job <= 0) cfg.slots_per_job = 8;
+    if (cfg.max_jobs <= 0) cfg.max_jobs = 4;
+    if (cfg.slots_per_job > 256 || cfg.max_jobs > 64) return sfail(GAIL_ERROR_INVALID, "slots_per_job or max_jobs too large");
+    const PluginHandle& ph = loadPlugin();
+    if (!ph.table) return sfail(GAIL_ERROR_UNSUPPORTED, ph.error);
+    s->P = ph.table;
+    const GailHipPlugin* P = s->P;
+    const int n = P->device_count();
+    if (n <= 0) return sfail(GAIL_ERROR_UNSUPPORTED, "no HIP device");
+    if (cfg.device < 0 || cfg.device >= n) return sfail(GAIL_ERROR_INVALID, "no such device");
+    if (P->device_usable(cfg.device) != GAIL_HIP_OK) return sfail(GAIL_ERROR_UNSUPPORTED, P->last_error());
+    if (P->device_create(cfg.device, nullptr, nullptr, &s->dev) != GAIL_HIP_OK) return sfail(GAIL_ERROR_FAILED, P->last_error());
+    // every slot takes any DCI frame (up to 4096 x 2160) and any input format
+    const GailHipSlot one{4096, 2160, 1, 0};
+    s->slotBytes = P->slots_bytes(s->dev, &one, 1, nullptr);
+    uint64_t budget = cfg.memory_budget;
+    GailHipMemory mem{};
+    if (P->device_memory(s->dev, &mem) == GAIL_HIP_OK) {
+        const uint64_t reserve = std::max<uint64_t>(2ull << 30, mem.total / 16);
+        const uint64_t avail = mem.available > reserve ? mem.available - reserve : 0;
+        budget = budget ? std::min(budget, avail) : avail;
+    }
+    s->slotsPerJob = cfg.slots_per_job;
+    std::vector<GailHipSlot> slots(s->slotsPerJob, one);
+    s->jobBytes = P->slots_bytes(s->dev, slots.data(), s->slotsPerJob, nullptr);
+    while (s->slotsPerJob > 1 && s->jobBytes > budget) {
+        --s->slotsPerJob;
+        s->jobBytes = P->slots_bytes(s->dev, slots.data(), s->slotsPerJob, nullptr);
+    }
+    const int njobs = (int)std::min<uint64_t>(cfg.max_jobs, s->jobBytes ? budget / s->jobBytes : 0);
+    if (njobs < 1) {
+        s->shutdown();
+        return sfail(GAIL_ERROR_NOMEM, "the device memory budget holds no frame slot");
+    }
+    for (int i = 0; i < njobs; ++i) {
+        auto j = std::make_unique<JobState>();
+        if (P->slots_create(s->dev, slots.data(), s->slotsPerJob, nullptr, &j->batch) != GAIL_HIP_OK) {
+            if (s->jobs.empty()) {
+                const std::string why = P->last_error();
+                s->shutdown();
+                return sfail(GAIL_ERROR_NOMEM, why);
+            }
+            break;
+        }
+        s->jobs.push_back(std::move(j));
+    }
+    int threads = cfg.cpu_threads;
+    if (threads <= 0) {  // the cores this process may run on (usableCpuCount, cpu_backend.cpp:180-192)
+        cpu_set_t set;
+        CPU_ZERO(&set);
+        const int cores = sched_getaffinity(0, sizeof set, &set) == 0 ? CPU_COUNT(&set)
+                                                                      : (int)std::thread::hardware_concurrency();
+        threads = std::max(2, cores - 1);
+    }
+    s->pool = std::make_unique<Pool>(threads);
+    for (auto& j : s->jobs) j->waiter = std::thread([sp = s.get(), jp = j.get()] { sp->waiterLoop(jp); });
+    s->scheduler = std::thread([sp = s.get()] { sp->schedulerLoop(); });
+    *out = s.release();
+    return GAIL_OK;
+}
+
+GAIL_API int gail_server_submit(gail_server* s, uint64_t id, uint32_t stream, const gail_config* c, const gail_frame* f,
+                                double slope) {
+    if (!s || !c || !f) return sfail(GAIL_ERROR_INVALID, "null argument");
+    auto r = std::make_unique<Request>();
+    r->id = id;
+    r->stream = stream;
+    r->frame = *f;
+    r->tSubmit = Clock::now();
+    {
+        std::unique_lock<std::mutex> l(s->m);
+        if (s->stop) return sfail(GAIL_ERROR_STATE, "the server is shutting down");
+        auto& slot = s->streams[stream];
+        if (!slot) {
+            l.unlock();
+            auto fresh = std::make_unique<StreamState>();
+            try {
+                fresh->spec = StreamSpec::make(*c);
+            } catch (const ApiError& e) {
+                return sfail(e.status, e.what());
+            } catch (const std::exception& e) {
+                return sfail(GAIL_ERROR_INVALID, e.what());
+            }
+            l.lock();
+            auto& again = s->streams[stream];
+            if (!again) again = std::move(fresh);
+        }
+        StreamState* sp = s->streams[stream].get();
+        const gail_config& k = sp->spec->cfg;
+        if (k.profile != c->profile || k.container != c->container || k.input.format != c->input.format ||
+            k.fps != c->fps || k.max_mbps != c->max_mbps)
+            return sfail(GAIL_ERROR_INVALID, "a stream's frames must keep its configuration");
+        std::string why;
+        const int chk = sp->spec->checkFrame(*f, why);
+        if (chk != GAIL_OK) return sfail(chk, why);
+        r->st = sp;
+        r->autoSlope = slope < 0;
+        r->slope = slope < 0 ? 0 : slope;
+        if (!s->haveFirst) {
+            s->haveFirst = true;
+            s->tFirst = r->tSubmit;
+        }
+        ++s->submitted;
+        s->maxInFlight = std::max(s->maxInFlight, ++s->inFlight);
+        s->pending.push_back(std::move(r));
+    }
+    s->cv.notify_all();
+    return GAIL_OK;
+}
+
+GAIL_API int gail_server_drain(gail_server* s) {
+    if (!s) return sfail(GAIL_ERROR_INVALID, "null argument");
+    std::unique_lock<std::mutex> l(s->m);
+    s->cv.wait(l, [&] { return s->delivered >= s->submitted; });
+    return GAIL_OK;
+}
+
+GAIL_API int gail_server_get_stats(gail_server* s, gail_server_stats* out) {
+    if (!s || !out) return sfail(GAIL_ERROR_INVALID, "null argument");
+    std::lock_guard<std::mutex> l(s->m);
+    gail_server_stats r{};
+    r.frames = s->delivered;
+    r.failed = s->failedCount;
+    r.recoded = s->recodedCount;
+    r.seconds = s->haveFirst ? msSince(s->tFirst, s->tLast) / 1000.0 : 0;
+    r.fps = r.seconds > 0 ? (double)(s->delivered - s->failedCount) / r.seconds : 0;
+    auto pct = [](std::vector<double> v, double q) {
+        if (v.empty()) return 0.0;
+        std::sort(v.begin(), v.end());
+        const double pos = q * (double)(v.size() - 1);
+        const size_t i = (size_t)pos;
+        const double f = pos - (double)i;
+        return i + 1 < v.size() ? v[i] * (1 - f) + v[i + 1] * f : v[i];
+    };
+    r.p50_ms = pct(s->latency, 0.5);
+    r.p99_ms = pct(s->latency, 0.99);
+    r.max_ms = s->latency.empty() ? 0 : *std::max_element(s->latency.begin(), s->latency.end());
+    r.queued_p50_ms = pct(s->queued, 0.5);
+    r.queued_p99_ms = pct(s->queued, 0.99);
+    r.mean_batch = s->jobRuns ? (double)s->jobFrames / (double)s->jobRuns : 0;
+    r.jobs = (int)s->jobs.size();
+    r.slots_per_job = s->slotsPerJob;
+    r.max_frames_in_flight = s->maxInFlight;
+    r.job_bytes = s->jobBytes;
+    r.slot_bytes = s->slotBytes;
+    *out = r;
+    return GAIL_OK;
+}
+
+GAIL_API void* gail_host_alloc(size_t bytes) {
+    const PluginHandle& ph = loadPlugin();
+    if (!ph.table) {
+        lastErrorText() = ph.error;
+        return nullptr;
+    }
+    void* p = ph.table->host_alloc(bytes);
+    if (!p) lastErrorText() = "out of pinned host memory";
+    return p;
+}
+
+GAIL_API void gail_host_free(void* p) {
+    const PluginHandle& ph = loadPlugin();
+    if (p && ph.table) ph.table->host_free(p);
+}
+
+GAIL_API void gail_server_destroy(gail_server* s) {
+    if (!s) return;
+    gail_server_drain(s);
+    s->shutdown();
+    delete s;
+}
+
+}  // extern "C"

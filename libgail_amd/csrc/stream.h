@@ -1,0 +1,71 @@
+// What libgail.so's encoder and server share: the backend plugin, loaded from this
+// library's directory, and the per-stream set-up taken from a gail_config (frame layout,
+// the plugin's layout and settings, rate limits, and the copy of an app frame into the
+// plugin's input layout, gail_api.cpp:363-396, 566-739).
+#pragma once
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "codestream.h"
+#include "gail_amd.h"
+#include "gail_hip_plugin.h"
+#include "layout.h"
+
+namespace gail_amd {
+
+// The thread's last error message (gail_last_error).
+std::string& lastErrorText();
+
+struct ApiError : std::runtime_error {
+    int status;
+    ApiError(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+struct PluginHandle {
+    const GailHipPlugin* table = nullptr;
+    std::string error;
+};
+// libgail_hip.so next to this library, ABI / size / layout checked (cuda_host.cpp:82-93, 386-420).
+const PluginHandle& loadPlugin();
+
+bool allZero(const uint64_t* p, int n);
+
+struct PlanarInfo {
+    int bits, sx, sy, rgb;
+};
+bool planarInfo(gail_pixel_format f, PlanarInfo& p);
+
+// One stream's geometry, input and limits. Never copied or moved once made (plainLayout
+// points into bands).
+struct StreamSpec {
+    gail_config cfg{};
+    Layout L;
+    std::vector<GailHipSubband> bands;
+    GailHipLayout plainLayout{};
+    GailHipEncodeSettings settings{};
+    RateLimits limits;
+    int width = 0, height = 0, picW = 0, picH = 0, ox = 0, oy = 0;
+    size_t inBytes = 0;
+    double slopeFactor = 0.25;  // gail_api.cpp:607
+
+    // Validates c and builds the stream (throws ApiError).
+    static std::unique_ptr<StreamSpec> make(const gail_config& c);
+    StreamSpec(const StreamSpec&) = delete;
+    StreamSpec& operator=(const StreamSpec&) = delete;
+    // Checks that a frame has the planes this stream reads.
+    int checkFrame(const gail_frame& f, std::string& why) const;
+    // Copies an app frame into the plugin's input layout (inBytes at dst).
+    void fill(const gail_frame& f, uint8_t* dst) const;
+    // When the frame already is the plugin's input layout in one block of memory, that
+    // block (inBytes long); else nullptr.
+    const void* directInput(const gail_frame& f) const;
+    bool fourk() const { return cfg.profile == GAIL_PROFILE_4K; }
+    double eyeFps() const { return cfg.fps * (cfg.stereo ? 2 : 1); }
+
+private:
+    StreamSpec() = default;
+};
+
+}  // namespace gail_amd

@@ -29,3 +29,18 @@ def compare_tier1(gpu, ref, max_report=5):
                 lines.append(f"block {i}: differs in {diff}; gpu nbps/np/len {g['nbps']}/{g['num_passes']}/"
                              f"{len(g['data'])} oracle {w['nbps']}/{w['num_passes']}/{len(w['data'])}")
     return bad, lines
+
+
+def planar_from_synth(x, bits, sx, sy, rgb):
+    """A planar picture derived from a synthetic frame: G'B'R' (rgb) or Y'CbCr with chroma
+    subsampled by 2^sx x 2^sy, at the given bit depth, planes back to back (uint16)."""
+    shift = 12 - bits
+    if rgb:
+        planes = [x[1], x[2], x[0]]
+        return np.concatenate([(p >> shift if shift >= 0 else p << -shift).astype(np.uint16).ravel() for p in planes])
+    Y = (x[1] >> shift).astype(np.uint16)
+
+    def sub(p):
+        p = p[::1 << sy, ::1 << sx]
+        return (p >> shift).astype(np.uint16)
+    return np.concatenate([Y.ravel(), sub(x[0]).ravel(), sub(x[2]).ravel()])

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import libgail_amd as g
-from t1_compare import compare_tier1
+from t1_compare import compare_tier1, planar_from_synth as _planar_from_synth
 
 pytestmark = pytest.mark.gpu
 
@@ -117,20 +117,6 @@ def test_pool_overflow_reruns_on_gpu(device, oracle):
     _check_frame(outs[0], oracle.encode_frame_xyz(x, k, threads=16), "overflow")
 
 
-def _planar_from_synth(x, bits, sx, sy, rgb):
-    """A planar picture derived from a synthetic frame: G'B'R' (rgb) or Y'CbCr with chroma
-    subsampled by 2^sx x 2^sy, at the given bit depth."""
-    shift = 12 - bits
-    if rgb:
-        planes = [x[1], x[2], x[0]]
-        return np.concatenate([(p >> shift if shift >= 0 else p << -shift).astype(np.uint16).ravel() for p in planes])
-    Y = (x[1] >> shift).astype(np.uint16)
-    def sub(p):
-        p = p[::1 << sy, ::1 << sx]
-        return (p >> shift).astype(np.uint16)
-    return np.concatenate([Y.ravel(), sub(x[0]).ravel(), sub(x[2]).ravel()])
-
-
 @pytest.mark.parametrize("kind", ["gbrp12_bt709", "yuv420p10_bt709"])
 def test_colour_inputs(device, oracle, kind):
     """Config 4 inputs: 12-bit G'B'R' and 10-bit Y'CbCr 4:2:0 (BT.709, limited range, chroma
@@ -154,3 +140,51 @@ def test_colour_inputs(device, oracle, kind):
     print(f"[colour {kind}] codes differing from the CPU path: {diff}", flush=True)
     assert diff == 0
     _check_frame(res, oracle.encode_frame_xyz(codes, k, threads=16), kind)
+
+
+def test_slot_job_rebinds(device, oracle):
+    """One job with three 4K-sized slots, bound to three different mixes in a row (2K and 4K
+    containers, X'Y'Z' and Y'CbCr inputs, fewer frames than slots): every frame's Tier-1
+    output equals the oracle's, and a frame bound again later gives the same bytes."""
+    b = g.Batch(device, slots=[(4096, 2160, True)] * 3)
+    W2, H2, k2 = g.CONTAINERS["2k_flat"]
+    L2 = g.FrameLayout(W2, H2, k2)
+    W4, H4, k4 = g.CONTAINERS["4k_scope"]
+    L4 = g.FrameLayout(W4, H4, k4)
+    Ls, _, ks = g.CONTAINERS["2k_scope"]
+    L2s = g.FrameLayout(Ls, g.CONTAINERS["2k_scope"][1], ks)
+    x2 = g.synth_frame(501, W2, H2)
+    x4 = g.synth_frame(502, W4, H4)
+    x2s = g.synth_frame(503, L2s.width, L2s.height)
+    try:
+        print("[slots] run 1: 2k_flat + 4k_scope", flush=True)
+        b.bind([L2, L4], [g.xyz_settings(L2), g.xyz_settings(L4)])
+        b.stage(0, x2)
+        b.stage(1, x4)
+        b.set_slope(0, 0.0)
+        b.set_slope(1, 60.0)
+        run1, _ = b.encode(from_staging=True)
+        _check_frame(run1[0], oracle.encode_frame_xyz(x2, k2, threads=16), "slots run1 2k_flat")
+        _check_frame(run1[1], oracle.encode_frame_xyz(x4, k4, thr=60.0, threads=16), "slots run1 4k_scope")
+
+        print("[slots] run 2: 2k_scope + yuv420p10 2k_flat + 2k_flat", flush=True)
+        aw, ah, ox, oy = 1920, 1080, 39, 0
+        src = _planar_from_synth(g.synth_frame(504, aw, ah), 10, 1, 1, 0)
+        sp = g.planar_settings(L2, 10, 1, 1, 0, 0, active=(aw, ah), offset=(ox, oy))
+        b.bind([L2s, L2, L2], [g.xyz_settings(L2s), sp, g.xyz_settings(L2)])
+        b.stage(0, x2s)
+        b.stage(1, src)
+        b.stage(2, x2)
+        for i in range(3):
+            b.set_slope(i, 0.0)
+        run2, _ = b.encode(from_staging=True)
+        _check_frame(run2[0], oracle.encode_frame_xyz(x2s, ks, threads=16), "slots run2 2k_scope")
+        rgb48 = oracle.planar_to_rgb48(src, 10, aw, ah, 1, 1, 0, 0, 0, 0.2126, 0.0722, W2, H2, ox, oy)
+        lut, cp = oracle.color_tables(0, 0)
+        _, xyz = oracle.convert_frame(lut, cp, rgb48)
+        _check_frame(run2[1], oracle.encode_frame_xyz(np.ascontiguousarray(xyz.transpose(2, 0, 1)), k2, threads=16),
+                     "slots run2 yuv420p10")
+        for name in ("info", "pass_len", "pass_dist", "data"):
+            assert np.array_equal(getattr(run2[2], name), getattr(run1[0], name)), name
+    finally:
+        b.close()

@@ -57,3 +57,30 @@ def test_check_catches_corruption(frame_2k):
     cs = bytearray(g.encode_codestream(L, t1)[0])
     cs[-1] = 0
     assert g.check_codestream(bytes(cs), 24.0, 0) != ""
+
+
+def _encoder_create_status(cfg):
+    import ctypes as C
+    h = g._lib_api()
+    h.gail_encoder_create.restype = C.c_int
+    h.gail_encoder_create.argtypes = [C.POINTER(g.GailConfig), C.POINTER(C.c_void_p)]
+    h.gail_encoder_destroy.argtypes = [C.c_void_p]
+    enc = C.c_void_p()
+    st = h.gail_encoder_create(C.byref(cfg), C.byref(enc))
+    if enc:
+        h.gail_encoder_destroy(enc)
+    return st, h.gail_last_error().decode()
+
+
+def test_stream_configs_validate():
+    """The stream descriptions the served form takes are valid for every DCI size (the
+    configuration is checked before any device is looked for), and a picture larger than
+    its container is refused."""
+    for name, (W, H, _) in g.CONTAINERS.items():
+        for fmt in ("xyz12_planar", "gbrp12", "yuv420p10"):
+            pic = None if fmt == "xyz12_planar" else (min(1920, W), min(1080, H))
+            st, msg = _encoder_create_status(g.stream_config(name, fmt, picture=pic))
+            assert st in (0, -2), (name, fmt, st, msg)  # OK on a GPU, UNSUPPORTED (no device) here
+            assert "picture" not in msg and "invalid" not in msg, (name, fmt, msg)
+    st, msg = _encoder_create_status(g.stream_config("2k_scope", "yuv420p10", picture=(1920, 1080)))
+    assert st == -1 and "picture" in msg
