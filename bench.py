@@ -206,9 +206,19 @@ def main():
     # ---- batched form: B frames resident in HBM per batch, `inflight` batches on their own
     # streams so that one batch's Tier-1 tail overlaps the next batch's DWT and context modelling
     nb = max(1, args.inflight)
-    batches = [g.Batch(dev, [layout] * B, [g.xyz_settings(layout)] * B, use_graph=bool(args.graph))
-               for _ in range(nb)]
-    for b in batches:
+    tiles = {"64x64": 0, "64x32": 1, "32x64": 2, "32x32": 3}
+    # autotuner: the fastest DWT tile shape, context-modelling and MQ workgroup sizes for
+    # this batch, measured on it
+    first = g.Batch(dev, [layout] * B, [g.xyz_settings(layout)] * B, use_graph=bool(args.graph))
+    for i, f in enumerate(frames):
+        first.upload(i, f)
+        first.set_slope(i, thr)
+    chosen, tune_ms = first.autotune(reps=2)
+    print(f"[bench] autotuner: {chosen} {json.dumps(tune_ms)}", file=sys.stderr, flush=True)
+    batches = [first] + [g.Batch(dev, [layout] * B, [g.xyz_settings(layout)] * B, use_graph=bool(args.graph),
+                                 dwt_tile=tiles[chosen["dwt_tile"]], cm_waves=chosen["cm_waves"],
+                                 mq_group=chosen["mq_group"]) for _ in range(nb - 1)]
+    for b in batches[1:]:
         for i, f in enumerate(frames):
             b.upload(i, f)
             b.set_slope(i, thr)
@@ -280,13 +290,14 @@ def main():
         "config": {"workload": f"batched {args.profile} {W}x{H} x{B} frames, Tier-1 to host", "frames_per_step": B,
                    "batches_in_flight": nb,
                    "levels": layout.levels, "code_blocks_per_frame": layout.num_cblks,
-                   "slope_threshold": thr, "graph": bool(args.graph)},
+                   "slope_threshold": thr, "graph": bool(args.graph), "autotuned": chosen},
         "plain_fps": round(plain_fps, 2),
         "batched_over_plain": round(fps / plain_fps, 3),
         "plain_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                              "p99": round(float(np.percentile(lat, 99)), 3)},
         "stage_ms_per_batch": {k: round(v, 3) for k, v in med.items()},
         "tier1_symbols_per_s": round(med["symbols"] / (ms_step * 1e-3), 1) if med.get("symbols") else None,
+        "autotune_ms": tune_ms,
         "roofline": roof,
         "cpu_baseline": cpu,
         "served": served,

@@ -233,6 +233,12 @@ struct GailHipPlugin {
     /* Pinned host memory on the plugin's runtime; NULL when none is left. */
     void* (*host_alloc)(uint64_t bytes);
     void (*host_free)(void* p);
+    /* Autotuner: times every DWT tile shape (64x64, 64x32, 32x64, 32x32), context-modelling
+     * waves per workgroup (1, 2, 4, 8) and MQ lanes per workgroup (64, 128, 256) on the bound
+     * frames, whose inputs must already be on the device; keeps the fastest of each for the
+     * batch's later runs. chosen (may be NULL) receives them; times (may be NULL) the 11
+     * measured stage times in ms in that order. */
+    int (*batch_autotune)(GailHipBatch* b, int reps, GailHipBatchOptions* chosen, double* times, int ntimes);
 };
 typedef struct GailHipPlugin GailHipPlugin;
 

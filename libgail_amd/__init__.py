@@ -139,6 +139,7 @@ class Plugin(C.Structure):
         ("batch_upload_from", _F(C.c_int, _vp, C.c_int, _vp)),
         ("host_alloc", _F(_vp, C.c_uint64)),
         ("host_free", _F(None, _vp)),
+        ("batch_autotune", _F(C.c_int, _vp, C.c_int, C.POINTER(BatchOptions), C.POINTER(C.c_double), C.c_int)),
     ]
 
 
@@ -372,6 +373,19 @@ class Batch:
         self._settings = (EncodeSettings * n)(*self.settings)
         _check(plugin().batch_bind(self.handle, self._plain, self._settings, n), "batch_bind")
         self.n = n
+
+    TUNE = ("dwt_64x64", "dwt_64x32", "dwt_32x64", "dwt_32x32", "cm_1", "cm_2", "cm_4", "cm_8", "mq_64", "mq_128",
+            "mq_256")
+
+    def autotune(self, reps=2):
+        """Times the kernel variants on the bound, uploaded frames and keeps the fastest.
+        Returns (chosen options dict, stage times dict in ms)."""
+        o = BatchOptions()
+        t = (C.c_double * 11)()
+        _check(plugin().batch_autotune(self.handle, reps, C.byref(o), t, 11), "batch_autotune")
+        chosen = {"dwt_tile": ["64x64", "64x32", "32x64", "32x32"][o.dwtTile], "cm_waves": o.cmWaves,
+                  "mq_group": o.mqGroup}
+        return chosen, {k: round(v, 4) for k, v in zip(self.TUNE, list(t))}
 
     def query(self):
         r = plugin().batch_query(self.handle)

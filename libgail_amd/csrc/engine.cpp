@@ -286,8 +286,6 @@ void Job::bind(std::vector<FrameConfig> frames) {
     frames_.clear();
     nblocks_ = 0;
     launched_ = false;
-    int tx, ty;
-    dwt_tile_size(opt_.dwtTile, tx, ty);
     uint32_t nblocks = 0, nbands = 0;
     int maxLevels = 0;
     for (int f = 0; f < nf; ++f) {
@@ -337,11 +335,56 @@ void Job::bind(std::vector<FrameConfig> frames) {
         hFrames_[f] = T1FrameRef{d.coef, (uint32_t)W, 0.f};
     }
     if (maxLevels > levelCap_) throw std::logic_error("frame has more levels than its slot");
-    // DWT level descriptors: level l of every frame that has it, tiles numbered across frames.
-    levelCount_.assign(maxLevels, 0);
-    levelTiles_.assign(maxLevels, 0);
+    hipStream_t s = stream_;
+    hipCheck(hipMemcpyAsync(dBlocks_, hBlocks_, sizeof(T1Block) * nblocks, hipMemcpyHostToDevice, s), "block table");
+    hipCheck(hipMemcpyAsync(dBands_, hBands_, sizeof(T1Band) * nbands, hipMemcpyHostToDevice, s), "band table");
+    nblocks_ = nblocks;
+    numLevels_ = maxLevels;
+    frames_ = std::move(frames);
+    planLevels();
+    frameDataBase_.assign(nf, 0);
+    framePassBase_.assign(nf, 0);
+    launched_ = false;
+    if (graph_) {  // the captured chain belongs to the previous composition
+        hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
+}
+
+// Replaces the symbol pool and / or the codeword buffer with ones twice as large (after a
+// run overflowed them); the chain is then run again from context modelling.
+void Job::growT1(bool pool, bool out) {
+    hipCheck(hipStreamSynchronize(stream_), "sync");
+    auto replace = [&](uint8_t*& p, size_t& cap) {
+        auto it = std::find(allocs_.begin(), allocs_.end(), (void*)p);
+        if (it != allocs_.end()) allocs_.erase(it);
+        hipFree(p);
+        bytes_ -= alignUp(cap);
+        cap *= 2;
+        p = devAlloc<uint8_t>(cap);
+    };
+    // Both grow together: the dense results live in the pool behind room for every
+    // codeword, so the pool must stay larger than the codeword buffer.
+    (void)pool;
+    (void)out;
+    replace(dPool_, poolCap_);
+    replace(dOut_, outCap_);
+    if (graph_) {
+        hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
+}
+
+// DWT level descriptors of the bound frames for the current tile shape: level l of every
+// frame that has it, tiles numbered across frames; queued to the device on the stream.
+void Job::planLevels() {
+    int tx, ty;
+    dwt_tile_size(opt_.dwtTile, tx, ty);
+    const int nf = numFrames();
+    levelCount_.assign(numLevels_, 0);
+    levelTiles_.assign(numLevels_, 0);
     for (int f = 0; f < nf; ++f) {
-        const Layout& L = frames[f].layout;
+        const Layout& L = frames_[f].layout;
         SlotDev& d = sd_[f];
         int w = L.width, h = L.height;
         for (int l = 0; l < L.levels; ++l) {
@@ -378,45 +421,114 @@ void Job::bind(std::vector<FrameConfig> frames) {
             h = h2;
         }
     }
-    hipStream_t s = stream_;
-    hipCheck(hipMemcpyAsync(dBlocks_, hBlocks_, sizeof(T1Block) * nblocks, hipMemcpyHostToDevice, s), "block table");
-    hipCheck(hipMemcpyAsync(dBands_, hBands_, sizeof(T1Band) * nbands, hipMemcpyHostToDevice, s), "band table");
-    hipCheck(hipMemcpyAsync(dLevels_, hLevels_, sizeof(DwtLevelDesc) * maxLevels * numSlots(), hipMemcpyHostToDevice, s),
+    hipCheck(hipMemcpyAsync(dLevels_, hLevels_, sizeof(DwtLevelDesc) * numLevels_ * numSlots(), hipMemcpyHostToDevice,
+                            stream_),
              "DWT descriptors");
-    nblocks_ = nblocks;
-    numLevels_ = maxLevels;
-    frames_ = std::move(frames);
-    frameDataBase_.assign(nf, 0);
-    framePassBase_.assign(nf, 0);
-    launched_ = false;
-    if (graph_) {  // the captured chain belongs to the previous composition
+    if (graph_) {  // the captured chain has the old launch shapes
         hipGraphExecDestroy(graph_);
         graph_ = nullptr;
     }
 }
 
-// Replaces the symbol pool and / or the codeword buffer with ones twice as large (after a
-// run overflowed them); the chain is then run again from context modelling.
-void Job::growT1(bool pool, bool out) {
-    hipCheck(hipStreamSynchronize(stream_), "sync");
-    auto replace = [&](uint8_t*& p, size_t& cap) {
-        auto it = std::find(allocs_.begin(), allocs_.end(), (void*)p);
-        if (it != allocs_.end()) allocs_.erase(it);
-        hipFree(p);
-        bytes_ -= alignUp(cap);
-        cap *= 2;
-        p = devAlloc<uint8_t>(cap);
+void Job::enqueueDwt(hipStream_t s) {
+    for (int l = 0; l < numLevels_; ++l) {
+        hipCheck(launch_dwt_level(l == 0, opt_.dwtTile, dLevels_ + (size_t)l * numSlots(), levelCount_[l], levelTiles_[l], s),
+                 "DWT level");
+        if (l == 0 && !capturing_) hipCheck(hipEventRecord(ev_[8], s), "event");
+    }
+}
+
+JobOptions Job::autotune(int reps, TuneTimes* out) {
+    if (frames_.empty()) throw std::logic_error("no frames bound to the job");
+    hipCheck(hipSetDevice(dev_), "hipSetDevice");
+    hipCheck(hipStreamSynchronize(stream_), "previous run");
+    launched_ = false;
+    reps = std::max(1, reps);
+    hipStream_t s = stream_;
+    for (int f = 0; f < numFrames(); ++f) hFrames_[f].slope_thresh = slope_[f];
+    hipCheck(hipMemcpyAsync(dFrames_, hFrames_, sizeof(T1FrameRef) * numFrames(), hipMemcpyHostToDevice, s), "frame table");
+    auto elapsed = [&](hipEvent_t a, hipEvent_t b) {
+        hipCheck(hipEventSynchronize(b), "autotune");
+        float ms = 0;
+        hipCheck(hipEventElapsedTime(&ms, a, b), "autotune");
+        return (double)ms;
     };
-    // Both grow together: the dense results live in the pool behind room for every
-    // codeword, so the pool must stay larger than the codeword buffer.
-    (void)pool;
-    (void)out;
-    replace(dPool_, poolCap_);
-    replace(dOut_, outCap_);
+    TuneTimes t;
+    // DWT tile shape: the whole DWT of the bound frames, best of reps
+    int bestTile = opt_.dwtTile;
+    double bestDwt = 1e30;
+    for (int shape = 0; shape < 4; ++shape) {
+        opt_.dwtTile = shape;
+        planLevels();
+        double best = 1e30;
+        for (int r = 0; r <= reps; ++r) {  // the first run warms up
+            hipCheck(hipEventRecord(ev_[2], s), "event");
+            enqueueDwt(s);
+            hipCheck(hipEventRecord(ev_[3], s), "event");
+            const double ms = elapsed(ev_[2], ev_[3]);
+            if (r) best = std::min(best, ms);
+        }
+        t.dwt[shape] = best;
+        if (best < bestDwt) {
+            bestDwt = best;
+            bestTile = shape;
+        }
+    }
+    opt_.dwtTile = bestTile;
+    planLevels();
+    enqueueDwt(s);  // the planes the Tier-1 runs below read
+    // context modelling waves per workgroup, then MQ workgroup size (whole Tier-1 runs)
+    const int waves[4] = {1, 2, 4, 8}, groups[3] = {64, 128, 256};
+    int bestWaves = opt_.cmWaves, bestGroup = opt_.mqGroup;
+    double bestCm = 1e30, bestMq = 1e30;
+    auto runT1 = [&]() {  // a run that overflowed its buffers skipped work: grow and repeat
+        for (int attempt = 0;; ++attempt) {
+            hipCheck(hipEventRecord(ev_[3], s), "event");
+            enqueueT1(s);
+            hipCheck(hipStreamSynchronize(s), "autotune");
+            const uint32_t flags = ((const JobCounters*)hSmall_)->flags;
+            if (flags & 4u) throw std::runtime_error("code-block with more than 24 magnitude bit-planes");
+            if (!(flags & 3u)) return;
+            if (attempt >= 6) throw std::runtime_error("Tier-1 buffers keep overflowing");
+            growT1(true, true);
+        }
+    };
+    for (int i = 0; i < 4; ++i) {
+        opt_.cmWaves = waves[i];
+        double best = 1e30;
+        for (int r = 0; r <= reps; ++r) {
+            runT1();
+            const double ms = elapsed(ev_[3], ev_[4]);
+            if (r) best = std::min(best, ms);
+        }
+        t.cm[i] = best;
+        if (best < bestCm) {
+            bestCm = best;
+            bestWaves = waves[i];
+        }
+    }
+    opt_.cmWaves = bestWaves;
+    for (int i = 0; i < 3; ++i) {
+        opt_.mqGroup = groups[i];
+        double best = 1e30;
+        for (int r = 0; r <= reps; ++r) {
+            runT1();
+            const double ms = elapsed(ev_[4], ev_[5]);
+            if (r) best = std::min(best, ms);
+        }
+        t.mq[i] = best;
+        if (best < bestMq) {
+            bestMq = best;
+            bestGroup = groups[i];
+        }
+    }
+    opt_.mqGroup = bestGroup;
     if (graph_) {
         hipGraphExecDestroy(graph_);
         graph_ = nullptr;
     }
+    if (out) *out = t;
+    return opt_;
 }
 
 void* Job::hostInput(int i) {
@@ -557,11 +669,7 @@ void Job::enqueueChain(bool uploadInputs) {
         hipCheck(launch_color(c, s), "colour");
     }
     if (!capturing_) hipCheck(hipEventRecord(ev_[2], s), "event");
-    for (int l = 0; l < numLevels_; ++l) {
-        hipCheck(launch_dwt_level(l == 0, opt_.dwtTile, dLevels_ + (size_t)l * numSlots(), levelCount_[l], levelTiles_[l], s),
-                 "DWT level");
-        if (l == 0 && !capturing_) hipCheck(hipEventRecord(ev_[8], s), "event");
-    }
+    enqueueDwt(s);
     if (!capturing_) hipCheck(hipEventRecord(ev_[3], s), "event");
     enqueueT1(s);
 }

@@ -79,6 +79,13 @@ private:
     std::string name_, arch_;
 };
 
+// Stage times (ms) the autotuner measured for each choice: DWT tile shapes 64x64, 64x32,
+// 32x64, 32x32; context-modelling waves per workgroup 1, 2, 4, 8; MQ lanes per workgroup
+// 64, 128, 256.
+struct TuneTimes {
+    double dwt[4] = {}, cm[4] = {}, mq[3] = {};
+};
+
 // Room one frame slot of a job holds: any frame up to maxWidth x maxHeight, and (colour)
 // the code planes and EOTF table that planar or RGB48 inputs need.
 struct SlotCapacity {
@@ -142,6 +149,11 @@ public:
     Tier1View result(int i) const;
     // Stage times of the last run (ms) and its symbol count.
     const StageTimes& lastTimes() const { return last_; }
+    // Times every DWT tile shape, context-modelling workgroup size and MQ workgroup size on
+    // the bound frames (their inputs must be on the device), keeps the fastest of each for
+    // the following runs and returns them. Each choice runs reps times after a warm-up.
+    JobOptions autotune(int reps = 2, TuneTimes* times = nullptr);
+    const JobOptions& options() const { return opt_; }
 
     // Debug taps (synchronous): X'Y'Z' codes (3 planes), the DWT output (3 Mallat planes),
     // and the quantised coefficient words (1,024 per block) of frame i.
@@ -174,6 +186,8 @@ private:
     void release();
     void growT1(bool pool, bool out);
     void enqueueChain(bool uploadInputs);
+    void planLevels();
+    void enqueueDwt(hipStream_t s);
     void enqueueT1(hipStream_t s);
     template <class T> T* devAlloc(size_t count);
 

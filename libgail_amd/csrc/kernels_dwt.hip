@@ -76,6 +76,8 @@ __device__ __forceinline__ int find_desc(const DwtLevelDesc* d, int n, int tile)
 template <bool kLevel0, int TX, int TY>
 __global__ void __launch_bounds__(256) dwt_level_kernel(const DwtLevelDesc* __restrict__ descs, int ndesc) {
     constexpr int RX = TX + 8, RY = TY + 8, PITCH = RX + 1;
+    constexpr int NPOS = RX * RY;
+    constexpr int PER = (NPOS + 255) / 256;  // staged positions per thread
     __shared__ float tile[3][RY][PITCH];
     const int di = find_desc(descs, ndesc, blockIdx.x);
     const DwtLevelDesc d = descs[di];
@@ -83,26 +85,69 @@ __global__ void __launch_bounds__(256) dwt_level_kernel(const DwtLevelDesc* __re
     const int x0 = (t % d.tiles_x) * TX, y0 = (t / d.tiles_x) * TY;
     const int w = d.w, h = d.h;
     const int tid = threadIdx.x;
+    // interior tiles need no symmetric extension
+    const bool inside = x0 >= 4 && y0 >= 4 && x0 + TX + 4 <= w && y0 + TY + 4 <= h;
 
-    // Stage the tile and its mirrored halo.
-    for (int i = tid; i < RY * RX; i += 256) {
+    // Stage the tile and its mirrored halo: every load of the thread is issued before the
+    // first one is used, so they overlap.
+    uint32_t at[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int i = tid + 256 * j;
         const int ry = i / RX, rx = i - ry * RX;
-        const int gy = mirror(y0 - 4 + ry, h), gx = mirror(x0 - 4 + rx, w);
-        const size_t at = (size_t)gy * d.src_stride + gx;
-        if constexpr (kLevel0) {
-            // 12-bit codes (words above 4095 read as 4095, as the host copy clips them)
-            const uint16_t* s = (const uint16_t*)d.src;
-            const float x = (float)(min((int)s[at], 4095) - 2048);
-            const float y = (float)(min((int)s[d.src_plane + at], 4095) - 2048);
-            const float z = (float)(min((int)s[2 * (size_t)d.src_plane + at], 4095) - 2048);
-            tile[0][ry][rx] = kY0[0] * x + kY0[1] * y + kY0[2] * z;
-            tile[1][ry][rx] = kY1[0] * x + kY1[1] * y + kY1[2] * z;
-            tile[2][ry][rx] = kY2[0] * x + kY2[1] * y + kY2[2] * z;
-        } else {
-            const float* s = (const float*)d.src;
-            tile[0][ry][rx] = s[at];
-            tile[1][ry][rx] = s[d.src_plane + at];
-            tile[2][ry][rx] = s[2 * (size_t)d.src_plane + at];
+        const int gy = inside ? y0 - 4 + ry : mirror(y0 - 4 + ry, h);
+        const int gx = inside ? x0 - 4 + rx : mirror(x0 - 4 + rx, w);
+        at[j] = (uint32_t)gy * d.src_stride + (uint32_t)gx;
+    }
+    if constexpr (kLevel0) {
+        const uint16_t* s0 = (const uint16_t*)d.src;
+        const uint16_t* s1 = s0 + d.src_plane;
+        const uint16_t* s2 = s1 + d.src_plane;
+        uint32_t cx[PER], cy[PER], cz[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            if (NPOS % 256 == 0 || tid + 256 * j < NPOS) {
+                cx[j] = s0[at[j]];
+                cy[j] = s1[at[j]];
+                cz[j] = s2[at[j]];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int i = tid + 256 * j;
+            if (NPOS % 256 == 0 || i < NPOS) {
+                const int ry = i / RX, rx = i - ry * RX;
+                // 12-bit codes (words above 4095 read as 4095, as the host copy clips them)
+                const float x = (float)((int)min(cx[j], 4095u) - 2048);
+                const float y = (float)((int)min(cy[j], 4095u) - 2048);
+                const float z = (float)((int)min(cz[j], 4095u) - 2048);
+                tile[0][ry][rx] = kY0[0] * x + kY0[1] * y + kY0[2] * z;
+                tile[1][ry][rx] = kY1[0] * x + kY1[1] * y + kY1[2] * z;
+                tile[2][ry][rx] = kY2[0] * x + kY2[1] * y + kY2[2] * z;
+            }
+        }
+    } else {
+        const float* s0 = (const float*)d.src;
+        const float* s1 = s0 + d.src_plane;
+        const float* s2 = s1 + d.src_plane;
+        float vx[PER], vy[PER], vz[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            if (NPOS % 256 == 0 || tid + 256 * j < NPOS) {
+                vx[j] = s0[at[j]];
+                vy[j] = s1[at[j]];
+                vz[j] = s2[at[j]];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int i = tid + 256 * j;
+            if (NPOS % 256 == 0 || i < NPOS) {
+                const int ry = i / RX, rx = i - ry * RX;
+                tile[0][ry][rx] = vx[j];
+                tile[1][ry][rx] = vy[j];
+                tile[2][ry][rx] = vz[j];
+            }
         }
     }
     __syncthreads();
@@ -133,18 +178,22 @@ __global__ void __launch_bounds__(256) dwt_level_kernel(const DwtLevelDesc* __re
         __syncthreads();
     }
     // De-interleaved stores: per tile row, the even samples (low) and the odd samples
-    // (high) each form a contiguous run of TX / 2 outputs.
+    // (high) each form a contiguous run of TX / 2 outputs. Thread tid always writes output
+    // k of segment seg; the 256 threads cover 256 / TX rows per step.
     const int snx = (w + 1) >> 1, sny = (h + 1) >> 1;
-    constexpr int HALF = TX / 2;
-    for (int i = tid; i < 3 * TY * TX; i += 256) {
-        const int c = i / (TY * TX);
-        const int r = (i / TX) % TY;
-        const int seg = (i / HALF) & 1, k = i % HALF;
-        const int gy = y0 + r, gx = x0 + 2 * k + seg;
-        if (gy >= h || gx >= w) continue;
+    constexpr int HALF = TX / 2, ROWS = 256 / TX, STEPS = 3 * TY / ROWS;
+    const int k = tid % HALF, seg = (tid / HALF) & 1, r0 = tid / TX;
+    const int gx = x0 + 2 * k + seg;
+    const int ox = seg ? snx + (gx >> 1) : (gx >> 1);
+    const bool colOk = gx < w;
+#pragma unroll 4
+    for (int j = 0; j < STEPS; ++j) {
+        const int c = j / (TY / ROWS);
+        const int r = r0 + ROWS * (j % (TY / ROWS));
+        const int gy = y0 + r;
+        if (!colOk || gy >= h) continue;
         const float v = tile[c][4 + r][4 + 2 * k + seg];
         const int oy = (gy & 1) ? sny + (gy >> 1) : (gy >> 1);
-        const int ox = seg ? snx + (gx >> 1) : (gx >> 1);
         if (!seg && !(gy & 1)) d.ll_dst[(size_t)c * d.ll_plane + (size_t)oy * d.ll_stride + ox] = v;
         else d.coef[(size_t)c * d.coef_plane + (size_t)oy * d.coef_stride + ox] = v;
     }

@@ -427,6 +427,23 @@ void hostFreeEntry(void* p) {
     if (p) hipHostFree(p);
 }
 
+int batchAutotune(GailHipBatch* b, int reps, GailHipBatchOptions* chosen, double* times, int ntimes) {
+    return guarded([&] {
+        TuneTimes t;
+        const JobOptions o = b->job->autotune(reps, &t);
+        if (chosen) {
+            chosen->dwtTile = o.dwtTile;
+            chosen->cmWaves = o.cmWaves;
+            chosen->mqGroup = o.mqGroup;
+            chosen->useGraph = o.useGraph ? 1 : 0;
+            chosen->poolBytesPerCoef = o.poolBytesPerCoef;
+        }
+        const double v[11] = {t.dwt[0], t.dwt[1], t.dwt[2], t.dwt[3], t.cm[0], t.cm[1],
+                              t.cm[2],  t.cm[3],  t.mq[0],  t.mq[1],  t.mq[2]};
+        for (int i = 0; times && i < ntimes && i < 11; ++i) times[i] = v[i];
+    });
+}
+
 #define GAIL_STR2(x) #x
 #define GAIL_STR(x) GAIL_STR2(x)
 
@@ -473,6 +490,7 @@ const GailHipPlugin g_plugin = {
     batchUploadFrom,
     hostAllocEntry,
     hostFreeEntry,
+    batchAutotune,
 };
 
 }  // namespace

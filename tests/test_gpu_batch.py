@@ -188,3 +188,41 @@ def test_slot_job_rebinds(device, oracle):
             assert np.array_equal(getattr(run2[2], name), getattr(run1[0], name)), name
     finally:
         b.close()
+
+
+@pytest.mark.parametrize("opts", [dict(dwt_tile=1), dict(dwt_tile=2), dict(dwt_tile=3), dict(cm_waves=1),
+                                  dict(cm_waves=2), dict(cm_waves=8), dict(mq_group=64), dict(mq_group=128)])
+def test_kernel_variants_bit_exact(device, oracle, opts):
+    """Every DWT tile shape, context-modelling and MQ workgroup size the autotuner can pick
+    gives the oracle's Tier-1 output (2K flat, early termination on)."""
+    W, H, k = g.CONTAINERS["2k_flat"]
+    L = g.FrameLayout(W, H, k)
+    x = g.synth_frame(61, W, H)
+    b = g.Batch(device, [L], [g.xyz_settings(L)], **opts)
+    try:
+        b.upload(0, x)
+        b.set_slope(0, 45.0)
+        outs, _ = b.encode()
+    finally:
+        b.close()
+    _check_frame(outs[0], oracle.encode_frame_xyz(x, k, thr=45.0, threads=16), f"variant {opts}")
+
+
+def test_autotune_then_encode(device, oracle):
+    """After the autotuner has run every variant on a batch, the batch's output still equals
+    the oracle's."""
+    W, H, k = g.CONTAINERS["2k_scope"]
+    L = g.FrameLayout(W, H, k)
+    frames = [g.synth_frame(70 + i, W, H) for i in range(2)]
+    b = g.Batch(device, [L, L], [g.xyz_settings(L)] * 2)
+    try:
+        for i, f in enumerate(frames):
+            b.upload(i, f)
+            b.set_slope(i, 0.0)
+        chosen, times = b.autotune(reps=1)
+        print("[autotune]", chosen, times, flush=True)
+        outs, _ = b.encode()
+    finally:
+        b.close()
+    for i in range(2):
+        _check_frame(outs[i], oracle.encode_frame_xyz(frames[i], k, threads=16), f"autotuned frame {i}")
