@@ -138,6 +138,8 @@ typedef struct GailHipBatchOptions {
     int32_t cmWaves;      /* waves per context-modelling workgroup: 1, 2, 4, 8 */
     int32_t mqGroup;      /* MQ workgroup size: 64, 128, 256 */
     int32_t useGraph;     /* replay the launch chain as a HIP graph */
+    int32_t mqRead;       /* symbol bytes an MQ lane reads at a time: 16 or 64 */
+    int32_t reserved;
     double poolBytesPerCoef;
 } GailHipBatchOptions;
 
@@ -236,8 +238,9 @@ struct GailHipPlugin {
     /* Autotuner: times every DWT tile shape (64x64, 64x32, 32x64, 32x32), context-modelling
      * waves per workgroup (1, 2, 4, 8) and MQ lanes per workgroup (64, 128, 256) on the bound
      * frames, whose inputs must already be on the device; keeps the fastest of each for the
-     * batch's later runs. chosen (may be NULL) receives them; times (may be NULL) the 11
-     * measured stage times in ms in that order. */
+     * batch's later runs, and the MQ symbol read width (16 or 64 bytes). chosen (may be NULL)
+     * receives them; times (may be NULL) the 14 measured stage times in ms: 4 tile shapes,
+     * 4 workgroup sizes, 3 MQ workgroup sizes with 16-byte reads, the same with 64-byte. */
     int (*batch_autotune)(GailHipBatch* b, int reps, GailHipBatchOptions* chosen, double* times, int ntimes);
 };
 typedef struct GailHipPlugin GailHipPlugin;

@@ -85,7 +85,7 @@ class Debug(C.Structure):
 
 class BatchOptions(C.Structure):
     _fields_ = [("dwtTile", C.c_int32), ("cmWaves", C.c_int32), ("mqGroup", C.c_int32), ("useGraph", C.c_int32),
-                ("poolBytesPerCoef", C.c_double)]
+                ("mqRead", C.c_int32), ("reserved", C.c_int32), ("poolBytesPerCoef", C.c_double)]
 
 
 class Slot(C.Structure):
@@ -340,8 +340,8 @@ class Batch:
     frames to its slots for each run (the served form)."""
 
     def __init__(self, device, layouts=None, settings=None, dwt_tile=0, cm_waves=4, mq_group=256, use_graph=False,
-                 pool_bytes_per_coef=0.0, slots=None):
-        self.options = BatchOptions(dwt_tile, cm_waves, mq_group, int(use_graph), pool_bytes_per_coef)
+                 pool_bytes_per_coef=0.0, slots=None, mq_read=16):
+        self.options = BatchOptions(dwt_tile, cm_waves, mq_group, int(use_graph), mq_read, 0, pool_bytes_per_coef)
         self.handle = _vp()
         if slots is not None:
             self._slots = (Slot * len(slots))(*[Slot(w, h, int(c), 0) for w, h, c in slots])
@@ -362,7 +362,8 @@ class Batch:
     def slots_bytes(device, slots, **options):
         arr = (Slot * len(slots))(*[Slot(w, h, int(c), 0) for w, h, c in slots])
         opt = BatchOptions(options.get("dwt_tile", 0), options.get("cm_waves", 4), options.get("mq_group", 256),
-                           int(options.get("use_graph", False)), options.get("pool_bytes_per_coef", 0.0))
+                           int(options.get("use_graph", False)), options.get("mq_read", 16), 0,
+                           options.get("pool_bytes_per_coef", 0.0))
         return plugin().slots_bytes(device.handle, arr, len(slots), C.byref(opt))
 
     def bind(self, layouts, settings):
@@ -375,16 +376,16 @@ class Batch:
         self.n = n
 
     TUNE = ("dwt_64x64", "dwt_64x32", "dwt_32x64", "dwt_32x32", "cm_1", "cm_2", "cm_4", "cm_8", "mq_64", "mq_128",
-            "mq_256")
+            "mq_256", "mq_64_r64", "mq_128_r64", "mq_256_r64")
 
     def autotune(self, reps=2):
         """Times the kernel variants on the bound, uploaded frames and keeps the fastest.
         Returns (chosen options dict, stage times dict in ms)."""
         o = BatchOptions()
-        t = (C.c_double * 11)()
-        _check(plugin().batch_autotune(self.handle, reps, C.byref(o), t, 11), "batch_autotune")
+        t = (C.c_double * 14)()
+        _check(plugin().batch_autotune(self.handle, reps, C.byref(o), t, 14), "batch_autotune")
         chosen = {"dwt_tile": ["64x64", "64x32", "32x64", "32x32"][o.dwtTile], "cm_waves": o.cmWaves,
-                  "mq_group": o.mqGroup}
+                  "mq_group": o.mqGroup, "mq_read": o.mqRead}
         return chosen, {k: round(v, 4) for k, v in zip(self.TUNE, list(t))}
 
     def query(self):
@@ -603,9 +604,10 @@ class Server:
     the codestream bytes."""
 
     def __init__(self, device=0, slots_per_job=8, max_jobs=4, memory_budget=0, cpu_threads=0, check=False,
-                 keep=False, batch_window_us=0):
+                 keep=False, batch_window_us=0, on_done=None):
         h = _lib_api()
         self.keep = keep
+        self.on_done = on_done  # called with (id, result dict) after each result is stored
         self.results = {}
         self._frames = {}
         self._lock = __import__("threading").Lock()
@@ -632,6 +634,8 @@ class Server:
         with self._lock:
             self.results[r.id] = d
             self._frames.pop(r.id, None)
+        if self.on_done:
+            self.on_done(r.id, d)
 
     def submit(self, fid, stream, cfg, planes, slope=-1.0):
         """planes: list of three numpy arrays (planar formats) or one array; kept alive until

@@ -479,7 +479,7 @@ JobOptions Job::autotune(int reps, TuneTimes* out) {
     enqueueDwt(s);  // the planes the Tier-1 runs below read
     // context modelling waves per workgroup, then MQ workgroup size (whole Tier-1 runs)
     const int waves[4] = {1, 2, 4, 8}, groups[3] = {64, 128, 256};
-    int bestWaves = opt_.cmWaves, bestGroup = opt_.mqGroup;
+    int bestWaves = opt_.cmWaves, bestGroup = opt_.mqGroup, bestRead = opt_.mqRead;
     double bestCm = 1e30, bestMq = 1e30;
     auto runT1 = [&]() {  // a run that overflowed its buffers skipped work: grow and repeat
         for (int attempt = 0;; ++attempt) {
@@ -508,8 +508,9 @@ JobOptions Job::autotune(int reps, TuneTimes* out) {
         }
     }
     opt_.cmWaves = bestWaves;
-    for (int i = 0; i < 3; ++i) {
-        opt_.mqGroup = groups[i];
+    for (int i = 0; i < 6; ++i) {
+        opt_.mqGroup = groups[i % 3];
+        opt_.mqRead = i < 3 ? 16 : 64;
         double best = 1e30;
         for (int r = 0; r <= reps; ++r) {
             runT1();
@@ -519,10 +520,12 @@ JobOptions Job::autotune(int reps, TuneTimes* out) {
         t.mq[i] = best;
         if (best < bestMq) {
             bestMq = best;
-            bestGroup = groups[i];
+            bestGroup = opt_.mqGroup;
+            bestRead = opt_.mqRead;
         }
     }
     opt_.mqGroup = bestGroup;
+    opt_.mqRead = bestRead;
     if (graph_) {
         hipGraphExecDestroy(graph_);
         graph_ = nullptr;
@@ -600,7 +603,7 @@ void Job::enqueueT1(hipStream_t s) {
     mq.work_next = dWork_ + 1;
     const int mqGroups = (int)std::max<uint32_t>(
         1, std::min<uint32_t>((n + opt_.mqGroup - 1) / opt_.mqGroup, (uint32_t)(ctx_.computeUnits() * opt_.mqGroupsPerCU)));
-    hipCheck(launch_t1_mq(mq, opt_.mqGroup, mqGroups, s), "MQ coding");
+    hipCheck(launch_t1_mq(mq, opt_.mqGroup, opt_.mqRead, mqGroups, s), "MQ coding");
     if (!capturing_) hipCheck(hipEventRecord(ev_[5], s), "event");
     hipCheck(launch_compact_offsets(dInfo_, dCodeLen_, n, dDataOff_, nullptr, dPartial_, s), "compaction offsets");
     // dense results live in the (now consumed) symbol pool

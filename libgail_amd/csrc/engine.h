@@ -34,10 +34,11 @@ struct FrameConfig {
 };
 
 struct JobOptions {
-    int dwtTile = 0;          // tile shape index (see dwt_tile_size)
+    int dwtTile = 1;          // tile shape index (see dwt_tile_size): 64 x 32
     int cmWaves = 4;          // waves per context-modelling workgroup
     int cmGroupsPerCU = 8;    // persistent grid size per compute unit
-    int mqGroup = 256;        // MQ workgroup size
+    int mqGroup = 128;        // MQ workgroup size
+    int mqRead = 16;          // symbol bytes each MQ lane reads at a time (16 or 64)
     int mqGroupsPerCU = 8;
     double poolBytesPerCoef = 8.0;  // symbol pool sizing (grown on overflow)
     bool useGraph = false;    // replay the launch chain as a HIP graph
@@ -81,9 +82,9 @@ private:
 
 // Stage times (ms) the autotuner measured for each choice: DWT tile shapes 64x64, 64x32,
 // 32x64, 32x32; context-modelling waves per workgroup 1, 2, 4, 8; MQ lanes per workgroup
-// 64, 128, 256.
+// 64, 128, 256 with 16-byte symbol reads, then the same with 64-byte reads.
 struct TuneTimes {
-    double dwt[4] = {}, cm[4] = {}, mq[3] = {};
+    double dwt[4] = {}, cm[4] = {}, mq[6] = {};  // mq: 16-byte reads x 64/128/256 lanes, then 64-byte
 };
 
 // Room one frame slot of a job holds: any frame up to maxWidth x maxHeight, and (colour)

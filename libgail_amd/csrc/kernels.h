@@ -100,7 +100,8 @@ hipError_t launch_sort_blocks(const T1Info* info, uint32_t n, uint32_t* hist, ui
                               hipStream_t s);
 hipError_t launch_out_offsets(const T1Info* info, uint32_t n, unsigned long long* out_off, unsigned long long* partial,
                               hipStream_t s);
-hipError_t launch_t1_mq(const MqParams& p, int group, int groups, hipStream_t s);
+// group: lanes per workgroup (64, 128, 256); read_bytes: symbols read per lane at a time (16 or 64).
+hipError_t launch_t1_mq(const MqParams& p, int group, int read_bytes, int groups, hipStream_t s);
 hipError_t launch_compact_offsets(const T1Info* info, const uint32_t* code_len, uint32_t n,
                                   unsigned long long* data_off, uint32_t* pass_off, unsigned long long* partial,
                                   hipStream_t s);

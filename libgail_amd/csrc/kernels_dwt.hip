@@ -79,9 +79,14 @@ __global__ void __launch_bounds__(256) dwt_level_kernel(const DwtLevelDesc* __re
     constexpr int NPOS = RX * RY;
     constexpr int PER = (NPOS + 255) / 256;  // staged positions per thread
     __shared__ float tile[3][RY][PITCH];
-    const int di = find_desc(descs, ndesc, blockIdx.x);
+    // Workgroup b runs on XCD b % 8; each XCD takes a contiguous range of tiles, so tiles
+    // that share halo rows and cache lines meet in the same L2.
+    const int nb = (int)gridDim.x, q = nb >> 3, rem = nb & 7;
+    const int xcd = (int)blockIdx.x & 7, j = (int)blockIdx.x >> 3;
+    const int tileIdx = xcd * q + min(xcd, rem) + j;
+    const int di = find_desc(descs, ndesc, tileIdx);
     const DwtLevelDesc d = descs[di];
-    const int t = blockIdx.x - d.tile_base;
+    const int t = tileIdx - d.tile_base;
     const int x0 = (t % d.tiles_x) * TX, y0 = (t / d.tiles_x) * TY;
     const int w = d.w, h = d.h;
     const int tid = threadIdx.x;

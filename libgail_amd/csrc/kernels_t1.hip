@@ -401,7 +401,8 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
             // ---- the plane's symbols to the pool (wave-uniform sub-allocation)
             wave_sync();
             const uint32_t cnt_lo = __shfl((int)plane_count, 0), cnt_hi = __shfl((int)plane_count, 32);
-            const uint32_t al_lo = (cnt_lo + 15u) & ~15u, al_hi = (cnt_hi + 15u) & ~15u;
+            // 64-byte aligned: the MQ coder reads a plane in 64-byte groups
+            const uint32_t al_lo = (cnt_lo + 63u) & ~63u, al_hi = (cnt_hi + 63u) & ~63u;
             if (chunk_cur + al_lo + al_hi > chunk_end) {
                 const unsigned long long want = kChunkBytes > al_lo + al_hi ? kChunkBytes : al_lo + al_hi;
                 unsigned long long got = 0;
@@ -616,7 +617,7 @@ constexpr int kScanSpan = 2048;
 __device__ __forceinline__ uint32_t out_capacity(const T1Info& f) {
     // A block's codeword never needs more than half a byte per symbol plus the flush; a
     // block that still overflows is reported and the job re-run with more room.
-    return f.passes ? ((f.symbols / 2u + 64u + 3u) & ~3u) : 0u;
+    return f.passes ? ((f.symbols / 2u + 64u + 63u) & ~63u) : 0u;
 }
 
 template <class F>
@@ -700,7 +701,9 @@ __constant__ uint32_t kMqTable[47];
 
 }  // namespace
 
-template <int GROUP>
+// GROUP lanes per workgroup; NQ: 16-byte words of symbols read per lane at a time (1 or 4),
+// with the following NQ in flight while these are coded.
+template <int GROUP, int NQ>
 __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
     __shared__ uint32_t s_ctx[19][GROUP];
     __shared__ uint32_t s_tab[48];
@@ -709,6 +712,7 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
     // A full symbol pool left some chunk tables unwritten: nothing to code, the host
     // re-runs Tier-1 with a larger pool.
     if (P.counters->flags & 1u) return;
+    constexpr uint32_t kGroupBytes = 16u * NQ;
     const int t = threadIdx.x;
     for (;;) {
         // one wave-sized group of blocks at a time, in largest-first order
@@ -727,7 +731,7 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
         }
         const unsigned long long obase = P.out_off[cb];
         uint8_t* out = P.out + obase;
-        uint32_t cap = (inf.symbols / 2u + 64u + 3u) & ~3u;
+        uint32_t cap = out_capacity(inf);  // a multiple of 64, as every block's offset
         bool overflow = obase + cap > P.out_cap;
         if (overflow) cap = 0;
         for (int i = 0; i < 19; ++i) s_ctx[i][t] = s_tab[0];
@@ -736,9 +740,11 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
         s_ctx[18][t] = s_tab[46];
         uint32_t A = 0x8000u, C = 0u, B = 0u;
         int CT = 12, bp = -1;
-        uint32_t word = 0;  // finished bytes of the current aligned 4-byte group
+        uint32_t word = 0;              // finished bytes of the current aligned 4-byte group
+        uint4 ob = make_uint4(0u, 0u, 0u, 0u);  // finished words of the current 16 bytes
+        uint32_t flushed = 0;           // codeword bytes already written out (a multiple of 16)
         // BYTEOUT (C.2.7) with the pending byte B kept in a register: it is final once the
-        // next byte starts, and finished bytes go out four at a time.
+        // next byte starts. Finished bytes go out 16 at a time.
         auto byteout = [&]() {
             bool stuff = B == 0xFFu;
             if (!stuff && (C & 0x8000000u)) {
@@ -751,9 +757,18 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
             if (bp >= 0) {
                 word |= B << (8 * (bp & 3));
                 if ((bp & 3) == 3) {
-                    if ((uint32_t)bp < cap) *(uint32_t*)(out + (bp & ~3)) = word;
-                    else overflow = true;
+                    const int wi = (bp >> 2) & 3;
+                    ob.x = wi == 0 ? word : ob.x;
+                    ob.y = wi == 1 ? word : ob.y;
+                    ob.z = wi == 2 ? word : ob.z;
+                    ob.w = wi == 3 ? word : ob.w;
                     word = 0;
+                    if (wi == 3) {
+                        const uint32_t at = (uint32_t)bp & ~15u;
+                        if (at + 16u <= cap) *(uint4*)(out + at) = ob;
+                        else overflow = true;
+                        flushed = at + 16u;
+                    }
                 }
             }
             ++bp;
@@ -773,62 +788,83 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
         const uint2* chunk = P.chunks + (size_t)cb * kMaxPlanes;
         const uint32_t planes = inf.planes;
         uint2 ch = chunk[0];
-        uint4 nextv = *(const uint4*)(P.pool + ((unsigned long long)ch.x << 4));
-        uint32_t t_cur = s_ctx[(nextv.x & 0xFFu) >> 1][t];
+        // A plane's symbols start 64-byte aligned and take a multiple of 64 bytes of the
+        // pool, so whole groups can be read.
+        const uint8_t* src = P.pool + ((unsigned long long)ch.x << 4);
+        uint4 q[NQ];
+#pragma unroll
+        for (int e = 0; e < NQ; ++e) q[e] = *(const uint4*)(src + 16 * e);
+        uint32_t t_cur = s_ctx[(q[0].x & 0xFFu) >> 1][t];
         for (uint32_t j = 0; j < planes; ++j) {
             const uint2 chn = j + 1 < planes ? chunk[j + 1] : make_uint2(0u, 0u);
-            const uint8_t* src = P.pool + ((unsigned long long)ch.x << 4);
-            // first symbol of the next plane, for the context prefetch at this plane's end
-            const uint32_t first_next = j + 1 < planes ? P.pool[(unsigned long long)chn.x << 4] : 0u;
-            for (uint32_t i = 0; i < ch.y; i += 16) {
-                const uint4 cur = nextv;
-                if (i + 16 < ch.y) nextv = *(const uint4*)(src + i + 16);
-                else if (j + 1 < planes) nextv = *(const uint4*)(P.pool + ((unsigned long long)chn.x << 4));
-                const uint32_t m = ch.y - i < 16u ? ch.y - i : 16u;
-                const uint32_t w[5] = {cur.x, cur.y, cur.z, cur.w, nextv.x};
+            const uint8_t* srcn = P.pool + ((unsigned long long)chn.x << 4);
+            const uint32_t ngroups = (ch.y + kGroupBytes - 1u) / kGroupBytes;
+            if (ngroups == 0 && j + 1 < planes) {  // an empty plane: the next plane's group now
 #pragma unroll
-                for (uint32_t b = 0; b < 16; ++b) {
-                    if (b >= m) break;
-                    // pass ends: the coder state is the truncation reference of the pass
-                    while (pos == next_end) {
-                        snaps[k] = MqSnapshot{bp, B | ((uint32_t)CT << 16), C, A};
-                        ++k;
-                        next_end = k < np ? pend[k] : 0xFFFFFFFFu;
-                    }
-                    const uint32_t sv = (w[b >> 2] >> (8 * (b & 3))) & 0xFFu;
-                    const uint32_t sn = b + 1 < m ? (w[(b + 1) >> 2] >> (8 * ((b + 1) & 3))) & 0xFFu
-                                                  : (i + 16 < ch.y ? (nextv.x & 0xFFu) : first_next);
-                    const uint32_t cx = sv >> 1, d = sv & 1u, cxn = sn >> 1;
-                    // the next symbol's context state, read before this symbol may change it
-                    const uint32_t t_next = s_ctx[cxn][t];
-                    const uint32_t st = t_cur;
-                    const uint32_t qe = st & 0xFFFFu, mps = st >> 31;
-                    const uint32_t a = A - qe;
-                    const bool is_mps = d == mps;
-                    const bool add_c = is_mps != (a < qe);  // conditional exchange (C.2.5 / C.2.6)
-                    C += add_c ? qe : 0u;
-                    A = add_c ? a : qe;
-                    uint32_t st_new = st;
-                    if (!is_mps || !(a & 0x8000u)) {
-                        const uint32_t nidx = is_mps ? ((st >> 16) & 63u) : ((st >> 22) & 63u);
-                        const uint32_t nmps = is_mps ? mps : (mps ^ ((st >> 28) & 1u));
-                        st_new = s_tab[nidx] | (nmps << 31);
-                        s_ctx[cx][t] = st_new;
-                        int n = __clz((int)A) - 16;  // RENORME: shifts until A >= 0x8000
-                        while (n > 0) {
-                            const int s = n < CT ? n : CT;
-                            A <<= s;
-                            C <<= s;
-                            CT -= s;
-                            n -= s;
-                            if (CT == 0) byteout();
+                for (int e = 0; e < NQ; ++e) q[e] = *(const uint4*)(srcn + 16 * e);
+            }
+            for (uint32_t g = 0; g < ngroups; ++g) {
+                const uint8_t* nsrc = g + 1 < ngroups ? src + kGroupBytes * (g + 1) : (j + 1 < planes ? srcn : nullptr);
+                uint4 n[NQ];
+#pragma unroll
+                for (int e = 0; e < NQ; ++e) n[e] = nsrc ? *(const uint4*)(nsrc + 16 * e) : make_uint4(0u, 0u, 0u, 0u);
+                const uint32_t mg = ch.y - kGroupBytes * g < kGroupBytes ? ch.y - kGroupBytes * g : kGroupBytes;
+#pragma unroll
+                for (int sub = 0; sub < NQ; ++sub) {
+                    if (16u * sub < mg) {
+                        const uint32_t m = mg - 16u * sub < 16u ? mg - 16u * sub : 16u;
+                        // the symbol after this run of m: in the group, or the first of the next
+                        const uint32_t after = 16u * sub + 16u < mg ? (q[sub + 1 < NQ ? sub + 1 : 0].x & 0xFFu)
+                                                                   : (n[0].x & 0xFFu);
+                        const uint4 cur = q[sub];
+                        const uint32_t w[5] = {cur.x, cur.y, cur.z, cur.w, after};
+#pragma unroll
+                        for (uint32_t b = 0; b < 16; ++b) {
+                            if (b >= m) break;
+                            // pass ends: the coder state is the truncation reference of the pass
+                            while (pos == next_end) {
+                                snaps[k] = MqSnapshot{bp, B | ((uint32_t)CT << 16), C, A};
+                                ++k;
+                                next_end = k < np ? pend[k] : 0xFFFFFFFFu;
+                            }
+                            const uint32_t sv = (w[b >> 2] >> (8 * (b & 3))) & 0xFFu;
+                            const uint32_t sn = b + 1 < m ? (w[(b + 1) >> 2] >> (8 * ((b + 1) & 3))) & 0xFFu : w[4];
+                            const uint32_t cx = sv >> 1, d = sv & 1u, cxn = sn >> 1;
+                            // the next symbol's context state, read before this symbol may change it
+                            const uint32_t t_next = s_ctx[cxn][t];
+                            const uint32_t st = t_cur;
+                            const uint32_t qe = st & 0xFFFFu, mps = st >> 31;
+                            const uint32_t a = A - qe;
+                            const bool is_mps = d == mps;
+                            const bool add_c = is_mps != (a < qe);  // conditional exchange (C.2.5 / C.2.6)
+                            C += add_c ? qe : 0u;
+                            A = add_c ? a : qe;
+                            uint32_t st_new = st;
+                            if (!is_mps || !(a & 0x8000u)) {
+                                const uint32_t nidx = is_mps ? ((st >> 16) & 63u) : ((st >> 22) & 63u);
+                                const uint32_t nmps = is_mps ? mps : (mps ^ ((st >> 28) & 1u));
+                                st_new = s_tab[nidx] | (nmps << 31);
+                                s_ctx[cx][t] = st_new;
+                                int nsh = __clz((int)A) - 16;  // RENORME: shifts until A >= 0x8000
+                                while (nsh > 0) {
+                                    const int sh = nsh < CT ? nsh : CT;
+                                    A <<= sh;
+                                    C <<= sh;
+                                    CT -= sh;
+                                    nsh -= sh;
+                                    if (CT == 0) byteout();
+                                }
+                            }
+                            t_cur = cxn == cx ? st_new : t_next;
+                            ++pos;
                         }
                     }
-                    t_cur = cxn == cx ? st_new : t_next;
-                    ++pos;
                 }
+#pragma unroll
+                for (int e = 0; e < NQ; ++e) q[e] = n[e];
             }
             ch = chn;
+            src = srcn;
         }
         while (pos == next_end) {
             snaps[k] = MqSnapshot{bp, B | ((uint32_t)CT << 16), C, A};
@@ -848,12 +884,18 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
             word |= B << (8 * (bp & 3));
             len = (uint32_t)bp + 1;
         }
-        if (len & 3) {  // the last, partial group of bytes
-            if ((len - 1) < cap) *(uint32_t*)(out + ((len - 1) & ~3u)) = word;
-            else overflow = true;
-        } else if (B != 0xFFu) {
-            if ((uint32_t)bp < cap) *(uint32_t*)(out + (bp & ~3)) = word;
-            else overflow = true;
+        // the words after the last full 16 bytes: the finished ones in ob, then the partial one
+        if (len > cap) {
+            overflow = true;
+        } else {
+            const uint32_t nw = (len + 3u) >> 2;
+            for (uint32_t wi = flushed >> 2; wi < nw; ++wi) {
+                const uint32_t r = wi & 3u;
+                const uint32_t v = wi + 1 == nw && (len & 3u || B != 0xFFu)
+                                       ? word
+                                       : (r == 0 ? ob.x : r == 1 ? ob.y : r == 2 ? ob.z : ob.w);
+                *(uint32_t*)(out + 4u * wi) = v;
+            }
         }
         if (overflow) {
             atomicOr(&P.counters->flags, 2u);
@@ -920,11 +962,21 @@ hipError_t t1_tables_init() {
     return e != hipSuccess ? e : upload_mq_table();
 }
 
-hipError_t launch_t1_mq(const MqParams& p, int group, int groups, hipStream_t s) {
+hipError_t launch_t1_mq(const MqParams& p, int group, int read_bytes, int groups, hipStream_t s) {
+    const bool wide = read_bytes >= 64;
     switch (group) {
-    case 64: hipLaunchKernelGGL(t1_mq_kernel<64>, dim3(groups), dim3(64), 0, s, p); break;
-    case 128: hipLaunchKernelGGL(t1_mq_kernel<128>, dim3(groups), dim3(128), 0, s, p); break;
-    default: hipLaunchKernelGGL(t1_mq_kernel<256>, dim3(groups), dim3(256), 0, s, p); break;
+    case 64:
+        if (wide) hipLaunchKernelGGL((t1_mq_kernel<64, 4>), dim3(groups), dim3(64), 0, s, p);
+        else hipLaunchKernelGGL((t1_mq_kernel<64, 1>), dim3(groups), dim3(64), 0, s, p);
+        break;
+    case 128:
+        if (wide) hipLaunchKernelGGL((t1_mq_kernel<128, 4>), dim3(groups), dim3(128), 0, s, p);
+        else hipLaunchKernelGGL((t1_mq_kernel<128, 1>), dim3(groups), dim3(128), 0, s, p);
+        break;
+    default:
+        if (wide) hipLaunchKernelGGL((t1_mq_kernel<256, 4>), dim3(groups), dim3(256), 0, s, p);
+        else hipLaunchKernelGGL((t1_mq_kernel<256, 1>), dim3(groups), dim3(256), 0, s, p);
+        break;
     }
     return hipGetLastError();
 }
