@@ -301,6 +301,12 @@ GAIL_API void gail_server_destroy(gail_server* s);
  * layout is copied to the device without a host copy, by DMA when it is in such memory. */
 GAIL_API void* gail_host_alloc(size_t bytes);
 GAIL_API void gail_host_free(void* p);
+/* Device memory (device 0) for frames kept resident in HBM: a frame whose planes lie there,
+ * back to back in the device's input layout, is copied within HBM into its slot. gail_copy
+ * copies in any direction (synchronously). */
+GAIL_API void* gail_device_alloc(size_t bytes);
+GAIL_API void gail_device_free(void* p);
+GAIL_API int gail_copy(void* dst, const void* src, size_t bytes);
 
 /* ---- helpers of this build (no counterpart in gail.h) --------------------------------- */
 struct GailHipSubband;

@@ -228,9 +228,10 @@ struct GailHipPlugin {
     /* 1 once the last launched run has finished on the device (batch_wait still fetches
      * its results), 0 while it runs, a negative status on error. */
     int (*batch_query)(GailHipBatch* b);
-    /* Queues the copy of bound frame i from host memory src (its input layout, bytes as the
-     * frame's settings give) on the batch's stream, ahead of the next batch_launch(b, 0).
-     * src must stay valid until that run is done; pinned memory (host_alloc) is copied by DMA. */
+    /* Queues the copy of bound frame i from src (its input layout, bytes as the frame's
+     * settings give) on the batch's stream, ahead of the next batch_launch(b, 0). src must stay
+     * valid until that run is done: pinned host memory (host_alloc) is copied by DMA, device
+     * memory (device_alloc) within HBM. */
     int (*batch_upload_from)(GailHipBatch* b, int i, const void* src);
     /* Pinned host memory on the plugin's runtime; NULL when none is left. */
     void* (*host_alloc)(uint64_t bytes);
@@ -242,6 +243,13 @@ struct GailHipPlugin {
      * receives them; times (may be NULL) the 14 measured stage times in ms: 4 tile shapes,
      * 4 workgroup sizes, 3 MQ workgroup sizes with 16-byte reads, the same with 64-byte. */
     int (*batch_autotune)(GailHipBatch* b, int reps, GailHipBatchOptions* chosen, double* times, int ntimes);
+    /* Device memory for frames kept resident in HBM (a frame whose planes lie there is
+     * copied device to device into its slot), a synchronous copy in any direction, and
+     * whether an address is device memory (1), host memory (0). */
+    void* (*device_alloc)(GailHipDevice* d, uint64_t bytes);
+    void (*device_free)(void* p);
+    int (*copy)(void* dst, const void* src, uint64_t bytes);
+    int (*is_device_pointer)(const void* p);
 };
 typedef struct GailHipPlugin GailHipPlugin;
 

@@ -140,6 +140,10 @@ class Plugin(C.Structure):
         ("host_alloc", _F(_vp, C.c_uint64)),
         ("host_free", _F(None, _vp)),
         ("batch_autotune", _F(C.c_int, _vp, C.c_int, C.POINTER(BatchOptions), C.POINTER(C.c_double), C.c_int)),
+        ("device_alloc", _F(_vp, _vp, C.c_uint64)),
+        ("device_free", _F(None, _vp)),
+        ("copy", _F(C.c_int, _vp, _vp, C.c_uint64)),
+        ("is_device_pointer", _F(C.c_int, _vp)),
     ]
 
 
@@ -556,6 +560,12 @@ def _lib_api():
         h.gail_server_drain.argtypes = [_vp]
         h.gail_server_get_stats.restype = C.c_int
         h.gail_server_get_stats.argtypes = [_vp, C.POINTER(ServerStats)]
+        h.gail_device_alloc.restype = _vp
+        h.gail_device_alloc.argtypes = [C.c_size_t]
+        h.gail_device_free.restype = None
+        h.gail_device_free.argtypes = [_vp]
+        h.gail_copy.restype = C.c_int
+        h.gail_copy.argtypes = [_vp, _vp, C.c_size_t]
         h.gail_host_alloc.restype = _vp
         h.gail_host_alloc.argtypes = [C.c_size_t]
         h.gail_host_free.restype = None
@@ -596,6 +606,32 @@ def pinned_empty(nbytes):
     buf = (C.c_uint8 * nbytes).from_address(p)
     weakref.finalize(buf, h.gail_host_free, p)
     return np.frombuffer(buf, np.uint8)
+
+
+class DeviceFrame:
+    """A frame's planes uploaded, back to back, into device memory (gail_device_alloc), for
+    submitting frames that are already resident in HBM."""
+
+    def __init__(self, planes):
+        h = _lib_api()
+        arrs = [np.ascontiguousarray(p) for p in planes]
+        total = sum(a.nbytes for a in arrs)
+        self.ptr = h.gail_device_alloc(total)
+        if not self.ptr:
+            raise ServerError(f"device allocation failed: {h.gail_last_error().decode()}")
+        self.ptrs, self.strides, off = [], [], 0
+        for a in arrs:
+            if h.gail_copy(self.ptr + off, a.ctypes.data, a.nbytes) != 0:
+                raise ServerError(f"upload failed: {h.gail_last_error().decode()}")
+            self.ptrs.append(self.ptr + off)
+            self.strides.append(a.strides[0] if a.ndim > 1 else 0)
+            off += a.nbytes
+        self.nbytes = total
+
+    def close(self):
+        if self.ptr:
+            _lib_api().gail_device_free(self.ptr)
+            self.ptr = None
 
 
 class Server:
@@ -641,10 +677,15 @@ class Server:
         """planes: list of three numpy arrays (planar formats) or one array; kept alive until
         the frame's result arrives."""
         f = GailFrame()
-        arrs = planes if isinstance(planes, (list, tuple)) else [planes]
-        for i, a in enumerate(arrs):
-            f.data[i] = a.ctypes.data
-            f.stride[i] = a.strides[0] if a.ndim > 1 else 0
+        if isinstance(planes, DeviceFrame):
+            arrs = planes
+            for i, (ptr, st) in enumerate(zip(planes.ptrs, planes.strides)):
+                f.data[i], f.stride[i] = ptr, st
+        else:
+            arrs = planes if isinstance(planes, (list, tuple)) else [planes]
+            for i, a in enumerate(arrs):
+                f.data[i] = a.ctypes.data
+                f.stride[i] = a.strides[0] if a.ndim > 1 else 0
         with self._lock:
             self._frames[fid] = (arrs, cfg)
         st = _lib_api().gail_server_submit(self.handle, fid, stream, C.byref(cfg), C.byref(f), slope)

@@ -551,7 +551,8 @@ void Job::uploadFrom(int i, const void* src) {
     if (i < 0 || i >= numFrames()) throw std::logic_error("no frame bound to this slot");
     hipCheck(hipSetDevice(dev_), "hipSetDevice");
     if (launched_) hipCheck(hipStreamSynchronize(stream_), "previous run");
-    hipCheck(hipMemcpyAsync(sd_[i].in, src, sd_[i].inBytes, hipMemcpyHostToDevice, stream_), "frame upload");
+    // host or device source: the runtime tells them apart (a device source stays in HBM)
+    hipCheck(hipMemcpyAsync(sd_[i].in, src, sd_[i].inBytes, hipMemcpyDefault, stream_), "frame upload");
 }
 
 void Job::uploadAsync(int i) {

@@ -129,7 +129,8 @@ public:
     // The same on the job's stream from the slot's pinned staging buffer, ahead of the
     // next launch (the staging buffer must hold the frame until that run is done).
     void uploadAsync(int i);
-    // The same from any host buffer (kept valid by the caller until the run is done).
+    // The same from any host or device buffer (kept valid by the caller until the run is
+    // done); a device buffer is copied within HBM.
     void uploadFrom(int i, const void* src);
     // Early termination threshold of frame i (0: code every pass).
     void setSlope(int i, double t) { slope_[i] = (float)t; }

@@ -446,6 +446,32 @@ int batchAutotune(GailHipBatch* b, int reps, GailHipBatchOptions* chosen, double
     });
 }
 
+void* deviceAllocEntry(GailHipDevice* d, uint64_t bytes) {
+    void* p = nullptr;
+    const int st = guarded([&] {
+        hipCheck(hipSetDevice(d->index), "hipSetDevice");
+        hipCheck(hipMalloc(&p, bytes ? bytes : 16), "device allocation");
+    });
+    return st == GAIL_HIP_OK ? p : nullptr;
+}
+
+void deviceFreeEntry(void* p) {
+    if (p) hipFree(p);
+}
+
+int copyEntry(void* dst, const void* src, uint64_t bytes) {
+    return guarded([&] { hipCheck(hipMemcpy(dst, src, bytes, hipMemcpyDefault), "copy"); });
+}
+
+int isDevicePointer(const void* p) {
+    hipPointerAttribute_t a{};
+    if (!p || hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return a.type == hipMemoryTypeDevice ? 1 : 0;
+}
+
 #define GAIL_STR2(x) #x
 #define GAIL_STR(x) GAIL_STR2(x)
 
@@ -493,6 +519,10 @@ const GailHipPlugin g_plugin = {
     hostAllocEntry,
     hostFreeEntry,
     batchAutotune,
+    deviceAllocEntry,
+    deviceFreeEntry,
+    copyEntry,
+    isDevicePointer,
 };
 
 }  // namespace

@@ -104,6 +104,7 @@ struct Request {
     double slope = 0;
     bool autoSlope = false;
     bool recoded = false;
+    bool onDevice = false;  // the planes are in device memory
     Clock::time_point tSubmit, tAdmit;
     int batchFrames = 0;
 };
@@ -274,6 +275,10 @@ struct gail_server {
             const void* direct = r.st->spec->directInput(r.frame);
             if (direct) {
                 src[i] = direct;
+                continue;
+            }
+            if (r.onDevice) {  // checked at submit; the host cannot read device planes
+                if (stagingError.empty()) stagingError = "a device-resident frame must have the device input layout";
                 continue;
             }
             void* dst = P->batch_input(j.batch, i);
@@ -483,6 +488,10 @@ GAIL_API int gail_server_submit(gail_server* s, uint64_t id, uint32_t stream, co
         std::string why;
         const int chk = sp->spec->checkFrame(*f, why);
         if (chk != GAIL_OK) return sfail(chk, why);
+        r->onDevice = s->P->is_device_pointer(f->data[0]) == 1;
+        if (r->onDevice && !sp->spec->directInput(*f))
+            return sfail(GAIL_ERROR_INVALID, "a device-resident frame must have the device input layout "
+                                             "(planes back to back, no padding, the container size for X'Y'Z')");
         r->st = sp;
         r->autoSlope = slope < 0;
         r->slope = slope < 0 ? 0 : slope;
@@ -546,6 +555,35 @@ GAIL_API void* gail_host_alloc(size_t bytes) {
     void* p = ph.table->host_alloc(bytes);
     if (!p) lastErrorText() = "out of pinned host memory";
     return p;
+}
+
+GAIL_API void* gail_device_alloc(size_t bytes) {
+    const PluginHandle& ph = loadPlugin();
+    if (!ph.table) {
+        lastErrorText() = ph.error;
+        return nullptr;
+    }
+    GailHipDevice* d = nullptr;
+    if (ph.table->device_count() <= 0 || ph.table->device_create(0, nullptr, nullptr, &d) != GAIL_HIP_OK) {
+        lastErrorText() = "no HIP device";
+        return nullptr;
+    }
+    void* p = ph.table->device_alloc(d, bytes);
+    if (!p) lastErrorText() = ph.table->last_error();
+    ph.table->device_destroy(d);
+    return p;
+}
+
+GAIL_API void gail_device_free(void* p) {
+    const PluginHandle& ph = loadPlugin();
+    if (p && ph.table) ph.table->device_free(p);
+}
+
+GAIL_API int gail_copy(void* dst, const void* src, size_t bytes) {
+    const PluginHandle& ph = loadPlugin();
+    if (!ph.table) return sfail(GAIL_ERROR_UNSUPPORTED, ph.error);
+    if (ph.table->copy(dst, src, bytes) != GAIL_HIP_OK) return sfail(GAIL_ERROR_FAILED, ph.table->last_error());
+    return GAIL_OK;
 }
 
 GAIL_API void gail_host_free(void* p) {

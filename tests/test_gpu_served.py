@@ -138,3 +138,31 @@ def test_served_stream_feedback(device):
             assert r["size"] <= 1302067
     finally:
         srv.close()
+
+
+def test_served_device_resident_frames(device):
+    """A frame resident in HBM (copied within HBM into its slot) codes to the same codestream
+    as the same frame handed over from host memory; a device frame that is not in the device
+    input layout is refused at submit (the host cannot rearrange device memory)."""
+    W, H, _ = g.CONTAINERS["2k_flat"]
+    x = g.synth_frame(88, W, H)
+    cfg = g.stream_config("2k_flat", "xyz12_planar")
+    dev_frame = g.DeviceFrame([x[0], x[1], x[2]])
+    srv = g.Server(slots_per_job=2, max_jobs=1, check=True, keep=True)
+    try:
+        srv.submit(1, 1, cfg, [x[0].copy(), x[1].copy(), x[2].copy()], 35.0)
+        srv.submit(2, 1, cfg, dev_frame, 35.0)
+        srv.drain()
+        a, b = srv.results[1], srv.results[2]
+        print(f"[served] host frame {a['status']} {a.get('size')}, device frame {b['status']} {b.get('size')}", flush=True)
+        assert a["status"] == 0 and b["status"] == 0
+        assert a["codestream"] == b["codestream"]
+        # planes not back to back: rejected
+        bad = g.DeviceFrame([x[0], x[1], x[2]])
+        bad.ptrs = [bad.ptrs[0], bad.ptrs[2], bad.ptrs[1]]
+        with pytest.raises(g.ServerError):
+            srv.submit(3, 1, cfg, bad, 35.0)
+        bad.close()
+    finally:
+        srv.close()
+        dev_frame.close()
