@@ -159,7 +159,9 @@ hipError_t launch_quant_debug(const T1Block* blocks, const T1Band* bands, const 
 template <int WAVES>
 __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
     __shared__ __attribute__((aligned(16))) uint8_t s_sym[WAVES][2][kPlaneSymCap];
-    __shared__ uint32_t s_hist[WAVES][2][20];
+    // Early-termination counts per context (d = 0 low half, d = 1 high half): one column per
+    // lane, so every lane updates its own words without atomics; summed at the plane end.
+    __shared__ uint32_t s_hist[WAVES][2][20 * 32];
     __shared__ uint8_t s_zc[4 * 45];
     for (int i = threadIdx.x; i < 4 * 45; i += WAVES * 64) s_zc[i] = kZcTable[i];
     __syncthreads();
@@ -211,7 +213,8 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
         const uint32_t rowsValid = h >= 32 ? 0xFFFFFFFFu : ((1u << h) - 1u);
         const float thr = fr.slope_thresh;
         const bool track = thr > 0.f;
-        if (x < 20) hist[x] = 0;
+#pragma unroll 4
+        for (int c = 0; c < 20; ++c) hist[c * 32 + x] = 0;
 
         // Bit-plane masks of the column: pl[k] holds bit (p + k) of every row's magnitude
         // (bits p..p+6: the current magnitude bit is pl[6], the six below it feed the
@@ -284,7 +287,7 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                         for (uint32_t k = 0; k < em.n; ++k) {
                             const uint32_t sv = (uint32_t)(em.acc >> (6 * k)) & 63u;
                             sym[plane_count + excl + k] = (uint8_t)sv;
-                            if (track) atomicAdd(&hist[sv >> 1], (sv & 1u) ? 0x10000u : 1u);
+                            if (track) hist[(sv >> 1) * 32 + x] += (sv & 1u) ? 0x10000u : 1u;
                         }
                         plane_count += half ? all - lot : lot;
                     }
@@ -381,7 +384,7 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                         for (uint32_t k = 0; k < em.n; ++k) {
                             const uint32_t sv = (uint32_t)(em.acc >> (6 * k)) & 63u;
                             sym[plane_count + excl + k] = (uint8_t)sv;
-                            if (track) atomicAdd(&hist[sv >> 1], (sv & 1u) ? 0x10000u : 1u);
+                            if (track) hist[(sv >> 1) * 32 + x] += (sv & 1u) ? 0x10000u : 1u;
                         }
                         plane_count += half ? all - lot : lot;
                     }
@@ -430,7 +433,9 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
             if (__any(track)) {
                 float term = 0.f;
                 if (track && x < 19) {
-                    const uint32_t hv = hist[x];
+                    uint32_t hv = 0;  // the context's counts over the half-wave's 32 columns
+#pragma unroll 4
+                    for (int l = 0; l < 32; ++l) hv += hist[x * 32 + ((l + x) & 31)];
                     const uint32_t n0 = hv & 0xFFFFu, n1 = hv >> 16, n = n0 + n1;
                     if (n > 0 && n <= kLog2Max) {
                         const float* row = P.log2tab + (size_t)n * (n + 1) / 2;
@@ -440,7 +445,10 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                     }
                 }
                 wave_sync();
-                if (track && x < 20) hist[x] = 0;
+                if (track) {
+#pragma unroll 4
+                    for (int c = 0; c < 20; ++c) hist[c * 32 + x] = 0;
+                }
                 float bits = 0.f;
                 for (int c = 0; c < 19; ++c) bits += __shfl(term, (half << 5) | c);
                 const float bytes = bits * 0.125f + 1.f;

@@ -84,3 +84,20 @@ def test_stream_configs_validate():
             assert "picture" not in msg and "invalid" not in msg, (name, fmt, msg)
     st, msg = _encoder_create_status(g.stream_config("2k_scope", "yuv420p10", picture=(1920, 1080)))
     assert st == -1 and "picture" in msg
+
+
+def test_server_api_without_device():
+    """The served form's entry points fail cleanly, with a message, where there is no GPU
+    (or refuse bad arguments before touching one)."""
+    import ctypes as C
+    h = g._lib_api()
+    out = C.c_void_p()
+    cfg = g.ServerConfig()
+    h.gail_server_config_init(C.byref(cfg), C.sizeof(cfg))
+    assert cfg.slots_per_job == 8 and cfg.max_jobs == 4
+    assert h.gail_server_create(None, C.byref(out)) == -1  # GAIL_ERROR_INVALID
+    cfg.slots_per_job = 100000
+    assert h.gail_server_create(C.byref(cfg), C.byref(out)) == -1
+    assert b"too large" in h.gail_last_error()
+    assert h.gail_server_submit(None, 0, 0, None, None, 0.0) == -1
+    assert h.gail_server_drain(None) == -1
