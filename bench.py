@@ -127,6 +127,76 @@ def served_run(g, np, offered_fps=300.0, seconds=5.0, seed=7):
         srv.close()
 
 
+def mixed_batch_run(g, np, dev, steps=6, per_profile=16):
+    """Batched form over a 64-frame mix: 16 frames each of 2K flat, 2K scope, 4K scope and 4K
+    full (per-frame seeds) in one job, one DWT launch per level and one persistent Tier-1
+    launch for all their code-blocks, the chain replayed as a HIP graph, tile shapes and
+    workgroup sizes autotuned on the batch. Tier-1 output to host memory, two jobs in flight."""
+    names = ["2k_flat", "2k_scope", "4k_scope", "4k_full"]
+    layouts, frames = [], []
+    for n in names:
+        W, H, k = g.CONTAINERS[n]
+        L = g.FrameLayout(W, H, k)
+        for i in range(per_profile):
+            layouts.append(L)
+            frames.append(g.synth_frame(7000 + 100 * names.index(n) + i, W, H, threads=16))
+    settings = [g.xyz_settings(L) for L in layouts]
+    first = g.Batch(dev, layouts, settings, use_graph=True)
+    for i, f in enumerate(frames):
+        first.upload(i, f)
+        first.set_slope(i, 0.0)
+    chosen, _ = first.autotune(reps=1)
+    tiles = {"64x64": 0, "64x32": 1, "32x64": 2, "32x32": 3}
+    second = g.Batch(dev, layouts, settings, use_graph=True, dwt_tile=tiles[chosen["dwt_tile"]],
+                     cm_waves=chosen["cm_waves"], mq_group=chosen["mq_group"], mq_read=chosen["mq_read"])
+    for i, f in enumerate(frames):
+        second.upload(i, f)
+        second.set_slope(i, 0.0)
+    bs = [first, second]
+    for b in bs:  # warm-up (captures the graphs)
+        b.encode(copy=False)
+    t0 = time.perf_counter()
+    for b in bs[:min(2, steps)]:
+        b.launch()
+    for s in range(steps):  # steps runs in all, two in flight
+        bs[s % 2].wait(copy=False)
+        if s + 2 < steps:
+            bs[s % 2].launch()
+    dt = time.perf_counter() - t0
+    blocks = sum(L.num_cblks for L in layouts)
+    px = sum(L.width * L.height for L in layouts)
+    fps = len(frames) * steps / dt
+    for b in bs:
+        b.close()
+    return {"frames_per_job": len(frames), "profiles": names, "jobs_timed": steps, "fps": round(fps, 2),
+            "fps_4k_equivalent": round(fps * (px / len(frames)) / (3996 * 2160), 2),
+            "code_blocks_per_launch": blocks, "graph": True, "autotuned": chosen,
+            "ms_per_job": round(1000 * dt / steps, 2), "early_termination": "off"}
+
+
+def cpu_config1(g, np):
+    """One 2K flat 1998x1080 frame through the CPU restatement of the reference encoder:
+    colour/ICT, DWT, quantisation and Tier-1 (oracle, on the host's cores), then this build's
+    rate control and Tier-2 on one thread: wall time and the codestream against the cap."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import reference, Oracle
+    ref = reference()
+    lib = ref if ref is not None else Oracle()
+    W, H, k = g.CONTAINERS["2k_flat"]
+    x = g.synth_frame(1, W, H, threads=16)
+    cores = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    t1 = lib.encode_frame_xyz(x, k, threads=cores)
+    t_mid = time.perf_counter()
+    cs, _, _ = g.encode_codestream(g.FrameLayout(W, H, k), t1, fps=24.0, mbps=250.0, tag="cpu")
+    t_end = time.perf_counter()
+    return {"frame": "2K flat 1998x1080, seed 1", "wall_ms": round(1000 * (t_end - t0), 1),
+            "tier1_ms": round(1000 * (t_mid - t0), 1), "rate_control_tier2_ms": round(1000 * (t_end - t_mid), 1),
+            "fps": round(1.0 / (t_end - t0), 2), "cores": cores, "bytes": len(cs), "cap_bytes": 1302067,
+            "dci_check": g.check_codestream(cs, 24.0, 0) or "pass",
+            "kind": "reference" if ref is not None else "port"}
+
+
 class StreamBench:
     """One server with 16 jobs x 16 frame slots (256 4K frames, ~160 GB of HBM) for the 4K
     flat streams: a closed loop with at most a given number of frames in flight."""
@@ -232,6 +302,7 @@ def main():
     ap.add_argument("--stream-frames", type=int, default=768)
     ap.add_argument("--stress-frames", type=int, default=512)
     ap.add_argument("--served-seconds", type=float, default=5.0)
+    ap.add_argument("--mixed-steps", type=int, default=6)
     ap.add_argument("--pmc-probe", action="store_true", help="a 1 GiB copy and one batch, for counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -329,6 +400,10 @@ def main():
     plain_fps = 1000.0 / statistics.mean(lat)
     print(f"[bench] plain: {plain_fps:.1f} frames/s (Tier-1 {statistics.mean(lat_t1):.2f} ms of "
           f"{statistics.mean(lat):.2f} ms)", file=sys.stderr, flush=True)
+    mixed = None
+    if args.mixed_steps > 0:
+        mixed = mixed_batch_run(g, np, dev, steps=args.mixed_steps)
+        print(f"[bench] mixed 64-frame batch: {json.dumps(mixed)}", file=sys.stderr, flush=True)
     dev.close()
 
     # ---- served 4K flat streams (the headline) and the 48 fps stress stream
@@ -353,9 +428,11 @@ def main():
         served = served_run(g, np, seconds=args.served_seconds)
         print(f"[bench] served: {json.dumps(served)}", file=sys.stderr, flush=True)
 
-    cpu = None
+    cpu = cfg1 = None
     if not args.no_cpu:
         cpu = cpu_baseline(frames[:4], fourk, thr)
+        cfg1 = cpu_config1(g, np)
+        print(f"[bench] CPU 2K frame: {json.dumps(cfg1)}", file=sys.stderr, flush=True)
 
     value = main_run["fps"]
     line = {
@@ -383,8 +460,10 @@ def main():
         "autotune_ms": tune_ms,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched_mixed_64": mixed,
         "stress_4k_48fps": stress,
         "served": served,
+        "cpu_2k_frame": cfg1,
     }
     print(json.dumps(line))
 

@@ -320,6 +320,9 @@ GAIL_API int64_t gail_encode_codestream(int width, int height, int fourk, double
                                         const uint32_t* pass_len, const int32_t* pass_dist, const uint8_t* data,
                                         uint64_t num_blocks, double fps, double mbps, uint8_t* out, uint64_t cap,
                                         double* lambda, double* mse, const char* tag);
+/* Rate control's slope grid: floor(1024 log2 s), and the smallest double of bin k. */
+GAIL_API int64_t gail_slope_bin(double s);
+GAIL_API double gail_slope_bin_edge(int64_t k);
 /* DCI structural check (codestream_check.cpp:43-280); 0 when it passes. */
 GAIL_API int gail_check_codestream(const uint8_t* data, uint64_t size, double fps, int fourk, char* msg, int cap);
 GAIL_API uint64_t gail_empty_codestream_bytes(int width, int height, int fourk, double base_step);

@@ -471,14 +471,77 @@ void select(const Layout& L, const Hull& H, const double lambda[3], std::vector<
     }
 }
 
-// Hull points binned by log2(slope) at 1/1024 octave with suffix sums: data bytes and
-// included-block counts for any threshold in O(1) (rate_control.cpp:117-162).
+}  // namespace
+
+// Slopes on the absolute 1/1024-octave grid: bin k holds the slopes s with
+// floor(1024 log2 s) = k. The bin of a double comes from its binary exponent and a table of
+// the 1,024 steps of one octave (integer arithmetic only), and a bin's lower edge is a
+// double read from the same table, so any implementation that shares the table makes the
+// same choices.
+const OctaveSteps& octaveSteps() {
+    static const OctaveSteps t = [] {
+        OctaveSteps o{};
+        for (int j = 0; j < 1024; ++j) {
+            const double v = std::exp2(j / 1024.0);  // in [1, 2)
+            uint64_t bits;
+            std::memcpy(&bits, &v, 8);
+            o.mantissa[j] = bits & ((1ull << 52) - 1);
+        }
+        o.mantissa[1024] = 1ull << 52;
+        // the first step at or above each of 1,024 equal mantissa ranges
+        for (int r = 0, j = 0; r < 1024; ++r) {
+            const uint64_t start = (uint64_t)r << 42;
+            while (j < 1023 && o.mantissa[j + 1] <= start) ++j;
+            o.first[r] = (uint16_t)j;
+        }
+        return o;
+    }();
+    return t;
+}
+
+namespace {
+
+}  // namespace
+
+int64_t slopeBin(double s) {  // s finite and > 0
+    const OctaveSteps& t = octaveSteps();
+    uint64_t bits;
+    std::memcpy(&bits, &s, 8);
+    int e = (int)((bits >> 52) & 0x7ff);
+    uint64_t man = bits & ((1ull << 52) - 1);
+    if (e == 0) {  // subnormal: normalise
+        while (!(man & (1ull << 52))) {
+            man <<= 1;
+            --e;
+        }
+        man &= (1ull << 52) - 1;
+        e += 1;
+    }
+    int j = t.first[man >> 42];  // then the largest j with mantissa[j] <= man
+    while (j < 1023 && t.mantissa[j + 1] <= man) ++j;
+    return (int64_t)(e - 1023) * 1024 + j;
+}
+
+double binEdge(int64_t k) {  // the smallest double of bin k
+    const OctaveSteps& t = octaveSteps();
+    const int64_t oct = k >= 0 ? k / 1024 : -((-k + 1023) / 1024);
+    const int j = (int)(k - oct * 1024);
+    const uint64_t bits = ((uint64_t)(oct + 1023) << 52) | t.mantissa[j];
+    double v;
+    std::memcpy(&v, &bits, 8);
+    return v;
+}
+
+namespace {
+
+// Hull points binned by slope with suffix sums: data bytes and included-block counts for any
+// bin threshold in O(1) (rate_control.cpp:117-162).
 class SlopeHistogram {
 public:
     // Buffers are reused across frames by the thread that owns the histogram.
-    void build(const Layout& L, const Hull& H, double loLog, double hiLog) {
-        lo_ = loLog;
-        nb_ = std::max(16, (int)std::ceil((hiLog - loLog) * 1024.0) + 2);
+    void build(const Layout& L, const Hull& H, int64_t kLo, int nb) {
+        kLo_ = kLo;
+        nb_ = nb;
         for (int c = 0; c < 3; ++c) {
             rate_[c].assign(nb_ + 1, 0.0);
             incl_[c].assign(nb_ + 1, 0.0);
@@ -500,20 +563,19 @@ public:
                 incl_[c][b] += incl_[c][b + 1];
             }
     }
-    void eval(const double lam[3], double data[3], double incl[3]) const {
-        for (int c = 0; c < 3; ++c) {
-            const int b = lam[c] <= 0 ? 0 : std::min(nb_, bin(lam[c]));
-            data[c] = rate_[c][b];
-            incl[c] = incl_[c][b];
-        }
-    }
-
-private:
+    // Bytes and included blocks of component c with every hull point of bin >= b.
+    double data(int c, int b) const { return rate_[c][std::clamp(b, 0, nb_)]; }
+    double incl(int c, int b) const { return incl_[c][std::clamp(b, 0, nb_)]; }
     int bin(double s) const {
         if (!std::isfinite(s)) return nb_ - 1;
-        return std::clamp((int)std::floor((std::log2(s) - lo_) * 1024.0), 0, nb_ - 1);
+        if (!(s > 0)) return 0;
+        return (int)std::clamp<int64_t>(slopeBin(s) - kLo_, 0, nb_ - 1);
     }
-    double lo_ = 0;
+    int bins() const { return nb_; }
+    double edge(int b) const { return binEdge(kLo_ + b); }
+
+private:
+    int64_t kLo_ = 0;
     int nb_ = 0;
     std::vector<double> rate_[3], incl_[3];
 };
@@ -545,56 +607,60 @@ RateResult allocateRate(const Layout& L, const Tier1Ref& t1, const RateLimits& l
             minSlope = std::min(minSlope, s);
         }
     if (maxSlope == 0) maxSlope = minSlope = 1;
-    const double loB = std::log2(minSlope) - 1, hiB = std::log2(maxSlope) + 1;
+    // bins one octave beyond the slopes on either side
+    const int64_t kLo = slopeBin(minSlope) - 1024;
+    const int nb = (int)(slopeBin(maxSlope) + 1024 - kLo + 2);
     SlopeHistogram& hist = scratch.hist;
-    hist.build(L, H, loB, hiB);
+    hist.build(L, H, kLo, nb);
+    // a threshold of bin b; bin -1 stands for 0 (every pass)
+    int bin[3] = {-1, -1, -1};
+    auto apply = [&]() {
+        for (int c = 0; c < 3; ++c) lam[c] = bin[c] < 0 ? 0.0 : hist.edge(bin[c]);
+    };
     CodestreamSizes sz = measure(lam);
     // header bytes per component ~ calibrated + beta * (included blocks - calibrated count)
     double hdrCal[3], inclCal[3], beta[3];
-    auto calibrate = [&](const double l[3], const CodestreamSizes& ex) {
-        double data[3], incl[3];
-        hist.eval(l, data, incl);
+    auto calibrate = [&](const CodestreamSizes& ex) {
         for (int c = 0; c < 3; ++c) {
-            hdrCal[c] = (double)ex.comp[c] - data[c];
-            inclCal[c] = incl[c];
-            beta[c] = incl[c] > 0 ? std::max(1.0, std::min(8.0, hdrCal[c] / incl[c])) : 3.0;
+            const double data = hist.data(c, bin[c]), incl = hist.incl(c, bin[c]);
+            hdrCal[c] = (double)ex.comp[c] - data;
+            inclCal[c] = incl;
+            beta[c] = incl > 0 ? std::max(1.0, std::min(8.0, hdrCal[c] / incl)) : 3.0;
         }
     };
-    auto model = [&](const double l[3], double comp[3]) {
-        double data[3], incl[3];
-        hist.eval(l, data, incl);
+    auto model = [&](const int b[3], double comp[3]) {
         double tot = (double)(sz.total - sz.comp[0] - sz.comp[1] - sz.comp[2]);
         for (int c = 0; c < 3; ++c) {
-            comp[c] = data[c] + hdrCal[c] + beta[c] * (incl[c] - inclCal[c]);
+            comp[c] = hist.data(c, b[c]) + hdrCal[c] + beta[c] * (hist.incl(c, b[c]) - inclCal[c]);
             tot += comp[c];
         }
         return tot;
     };
-    auto search = [&](auto pred, double lo, double hi) {
-        for (int it = 0; it < 48 && hi - lo > 1e-5; ++it) {
-            const double mid = 0.5 * (lo + hi);
+    // smallest bin in [lo, hi] that satisfies a predicate that holds from some bin on
+    auto search = [&](auto pred, int lo, int hi) {
+        while (lo < hi) {
+            const int mid = lo + (hi - lo) / 2;
             if (pred(mid)) hi = mid;
-            else lo = mid;
+            else lo = mid + 1;
         }
-        return hi;
+        return lo;
     };
     if (sz.total > lim.maxTotal) {
-        calibrate(lam, sz);
+        calibrate(sz);
         for (int round = 0; round < 6; ++round) {
-            const double lg = search(
-                [&](double m) {
-                    const double l = std::exp2(m), ll[3] = {l, l, l};
+            const int b = search(
+                [&](int m) {
+                    const int bb[3] = {m, m, m};
                     double comp[3];
-                    return model(ll, comp) <= (double)lim.maxTotal;
+                    return model(bb, comp) <= (double)lim.maxTotal;
                 },
-                loB, hiB);
-            const double prev = lam[0];
-            lam[0] = lam[1] = lam[2] = std::exp2(lg);
-            // the same lambda again measures the same sizes: the model has settled
-            if (round > 0 && lam[0] == prev) break;
+                0, hist.bins());
+            if (round > 0 && b == bin[0]) break;  // the model has settled: the same sizes again
+            bin[0] = bin[1] = bin[2] = b;
+            apply();
             sz = measure(lam);
             if (sz.total <= lim.maxTotal && sz.total + lim.maxTotal / 500 >= lim.maxTotal) break;
-            calibrate(lam, sz);
+            calibrate(sz);
         }
         while (sz.total > lim.maxTotal && lam[0] <= maxSlope) {
             for (double& l : lam) l = l > 0 ? l * 1.002 : minSlope;
@@ -603,23 +669,25 @@ RateResult allocateRate(const Layout& L, const Tier1Ref& t1, const RateLimits& l
     }
     for (int c = 0; c < kComps; ++c) {
         if (sz.comp[c] <= lim.maxComp) continue;
-        calibrate(lam, sz);
+        // the bins follow the nudged thresholds
+        for (int k = 0; k < 3; ++k) bin[k] = lam[k] > 0 ? hist.bin(lam[k]) : -1;
+        calibrate(sz);
         for (int round = 0; round < 6; ++round) {
-            const double lg = search(
-                [&](double m) {
-                    double ll[3] = {lam[0], lam[1], lam[2]};
-                    ll[c] = std::exp2(m);
+            const int b = search(
+                [&](int m) {
+                    int bb[3] = {bin[0], bin[1], bin[2]};
+                    bb[c] = m;
                     double comp[3];
-                    model(ll, comp);
+                    model(bb, comp);
                     return comp[c] <= (double)lim.maxComp;
                 },
-                std::log2(std::max(lam[c], minSlope)), hiB);
-            const double prev = lam[c];
-            lam[c] = std::exp2(lg);
-            if (round > 0 && lam[c] == prev) break;
+                std::max(bin[c], 0), hist.bins());
+            if (round > 0 && b == bin[c]) break;
+            bin[c] = b;
+            lam[c] = hist.edge(b);
             sz = measure(lam);
             if (sz.comp[c] <= lim.maxComp && sz.comp[c] + lim.maxComp / 500 >= lim.maxComp) break;
-            calibrate(lam, sz);
+            calibrate(sz);
         }
         while (sz.comp[c] > lim.maxComp && lam[c] <= maxSlope) {
             lam[c] = lam[c] > 0 ? lam[c] * 1.002 : minSlope;

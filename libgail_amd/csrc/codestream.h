@@ -63,6 +63,17 @@ struct RateResult {
     int iterations = 0;
 };
 
+// The 1,024 steps of one octave as the 52-bit mantissas of exp2(j / 1024), j = 0..1023
+// (entry 1024 = 2^52): the grid rate control bins slopes on.
+struct OctaveSteps {
+    uint64_t mantissa[1025];
+    uint16_t first[1024];  // largest j with mantissa[j] <= r << 42, for r = 0..1023
+};
+const OctaveSteps& octaveSteps();
+// floor(1024 log2 s) for a finite s > 0, and the smallest double of that bin.
+int64_t slopeBin(double s);
+double binEdge(int64_t k);
+
 // PCRD-opt under the limits (rate_control.cpp:165-279).
 RateResult allocateRate(const Layout& L, const Tier1Ref& t1, const RateLimits& lim);
 

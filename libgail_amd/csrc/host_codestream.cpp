@@ -2,6 +2,7 @@
 // allocation and Tier-2 assembly of one frame's Tier-1 output into a DCI codestream
 // (what the reference's pipeline worker does after EncoderSlot::encode,
 // pipeline.cpp:210-226), and the DCI structural check.
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -57,6 +58,10 @@ extern "C" GAIL_AMD_EXPORT int64_t gail_encode_codestream(int W, int H, int four
         return -1;
     }
 }
+
+// The rate-control slope grid (floor(1024 log2 s) and a bin's lowest double), for tests.
+extern "C" GAIL_AMD_EXPORT int64_t gail_slope_bin(double s) { return s > 0 && std::isfinite(s) ? slopeBin(s) : 0; }
+extern "C" GAIL_AMD_EXPORT double gail_slope_bin_edge(int64_t k) { return binEdge(k); }
 
 // "" (returned as 0) when the codestream passes, else a message copied into msg.
 extern "C" GAIL_AMD_EXPORT int gail_check_codestream(const uint8_t* data, uint64_t size, double fps, int fourk,

@@ -101,3 +101,26 @@ def test_server_api_without_device():
     assert b"too large" in h.gail_last_error()
     assert h.gail_server_submit(None, 0, 0, None, None, 0.0) == -1
     assert h.gail_server_drain(None) == -1
+
+
+def test_slope_grid():
+    """Rate control bins slopes on floor(1024 log2 s) with integer arithmetic only: the bin
+    agrees with the logarithm away from bin edges, and every bin's edge lies in its bin with
+    the double just below it in the bin before (so no slope falls between bins)."""
+    import ctypes as C
+    import math
+    h = g.host()
+    h.gail_slope_bin.restype = C.c_int64
+    h.gail_slope_bin.argtypes = [C.c_double]
+    h.gail_slope_bin_edge.restype = C.c_double
+    h.gail_slope_bin_edge.argtypes = [C.c_int64]
+    rng = np.random.default_rng(3)
+    for s in np.exp2(rng.uniform(-40, 60, 20000)):
+        f = 1024 * math.log2(s)
+        if abs(f - round(f)) > 1e-6:
+            assert h.gail_slope_bin(float(s)) == math.floor(f), s
+    for k in list(range(-5000, 5000, 7)) + [0, 1, 1023, 1024, -1, -1024, -1025]:
+        e = h.gail_slope_bin_edge(k)
+        assert h.gail_slope_bin(e) == k, k
+        assert h.gail_slope_bin(np.nextafter(e, 0.0)) == k - 1, k
+        assert abs(math.log2(e) * 1024 - k) < 1e-6
