@@ -320,9 +320,19 @@ GAIL_API int64_t gail_encode_codestream(int width, int height, int fourk, double
                                         const uint32_t* pass_len, const int32_t* pass_dist, const uint8_t* data,
                                         uint64_t num_blocks, double fps, double mbps, uint8_t* out, uint64_t cap,
                                         double* lambda, double* mse, const char* tag);
+/* Frame sizes from buildCodestream and from the device's packet-size code (run on the host)
+ * over `trials` pass choices; the number of choices where they differ, -1 on error. */
+GAIL_API int gail_rc_measure_check(int width, int height, int fourk, double base_step, const uint32_t* info5,
+                                   const uint32_t* pass_len, const int32_t* pass_dist, const uint8_t* data,
+                                   uint64_t num_blocks, int trials, uint64_t seed);
 /* Rate control's slope grid: floor(1024 log2 s), and the smallest double of bin k. */
 GAIL_API int64_t gail_slope_bin(double s);
 GAIL_API double gail_slope_bin_edge(int64_t k);
+/* Tier-2 of a Tier-1 frame whose passes were already chosen (each block keeps its coded
+ * passes, as the device rate control leaves them); the codestream size, or < 0. */
+GAIL_API int64_t gail_build_codestream(int width, int height, int fourk, double base_step, const uint32_t* info5,
+                                       const uint32_t* pass_len, const int32_t* pass_dist, const uint8_t* data,
+                                       uint64_t num_blocks, uint8_t* out, uint64_t cap, const char* tag);
 /* DCI structural check (codestream_check.cpp:43-280); 0 when it passes. */
 GAIL_API int gail_check_codestream(const uint8_t* data, uint64_t size, double fps, int fourk, char* msg, int cap);
 GAIL_API uint64_t gail_empty_codestream_bytes(int width, int height, int fourk, double base_step);

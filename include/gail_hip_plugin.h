@@ -94,6 +94,10 @@ typedef struct GailHipEncodeSettings {
     double baseStep, slopeFactor, bitsPerSymbol;
     GailHipFrameFormat input;
     GailHipColor color;
+    /* Rate control on the device (no counterpart in the CUDA plugin): the frame's byte budget
+     * and per-component cap (rate_control.cpp:11-26); 0 leaves rate control to the host. The
+     * frame's Tier-1 output then holds only the chosen passes, and GailHipRate its result. */
+    uint64_t rcMaxTotal, rcMaxComp;
 } GailHipEncodeSettings;
 
 /* GailCudaMemory (cuda_plugin.h:113-116). */
@@ -115,6 +119,14 @@ typedef struct GailHipTier1 {
     const uint8_t* data;
     uint64_t numInfo, numPasses, numData;
 } GailHipTier1;
+
+/* Result of the device rate control of one frame (no counterpart in the CUDA plugin). */
+typedef struct GailHipRate {
+    double lambda[3];          /* slope threshold per component */
+    uint64_t total, comp[3], tilePart[6];  /* codestream bytes, as Tier-2 will write them */
+    double distortion;         /* weighted squared error left (rate-control units) */
+    uint32_t iterations, numTileParts;
+} GailHipRate;
 
 /* gail::FrameTimings (backend.h:33-37), milliseconds of device time per stage. */
 typedef struct GailHipFrameTimings {
@@ -250,6 +262,9 @@ struct GailHipPlugin {
     void (*device_free)(void* p);
     int (*copy)(void* dst, const void* src, uint64_t bytes);
     int (*is_device_pointer)(const void* p);
+    /* Device rate control result of frame i of the last run (GAIL_HIP_MISUSE when the frame's
+     * settings left rate control to the host). */
+    int (*batch_rate)(GailHipBatch* b, int i, GailHipRate* out);
 };
 typedef struct GailHipPlugin GailHipPlugin;
 
@@ -264,7 +279,7 @@ static inline uint64_t gail_hip_plugin_layout(void) {
                               sizeof(GailHipFrameFormat), sizeof(GailHipColor),       sizeof(GailHipEncodeSettings),
                               sizeof(GailHipMemory),      sizeof(GailHipCblkInfo),    sizeof(GailHipTier1),
                               sizeof(GailHipFrameTimings), sizeof(GailHipDebug),      sizeof(GailHipBatchOptions),
-                              sizeof(GailHipSlot),        sizeof(GailHipPlugin)};
+                              sizeof(GailHipSlot),        sizeof(GailHipRate),        sizeof(GailHipPlugin)};
     uint64_t h = 1469598103934665603ull;
     for (size_t i = 0; i < sizeof sizes / sizeof sizes[0]; ++i) h = (h ^ sizes[i]) * 1099511628211ull;
     return h;

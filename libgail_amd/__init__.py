@@ -50,7 +50,13 @@ class Color(C.Structure):
 class EncodeSettings(C.Structure):
     _fields_ = [("profile", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("baseStep", C.c_double),
                 ("slopeFactor", C.c_double), ("bitsPerSymbol", C.c_double), ("input", FrameFormat),
-                ("color", Color)]
+                ("color", Color), ("rcMaxTotal", C.c_uint64), ("rcMaxComp", C.c_uint64)]
+
+
+class Rate(C.Structure):
+    _fields_ = [("lambda_", C.c_double * 3), ("total", C.c_uint64), ("comp", C.c_uint64 * 3),
+                ("tilePart", C.c_uint64 * 6), ("distortion", C.c_double), ("iterations", C.c_uint32),
+                ("numTileParts", C.c_uint32)]
 
 
 class Memory(C.Structure):
@@ -144,6 +150,7 @@ class Plugin(C.Structure):
         ("device_free", _F(None, _vp)),
         ("copy", _F(C.c_int, _vp, _vp, C.c_uint64)),
         ("is_device_pointer", _F(C.c_int, _vp)),
+        ("batch_rate", _F(C.c_int, _vp, C.c_int, C.POINTER(Rate))),
     ]
 
 
@@ -392,6 +399,14 @@ class Batch:
                   "mq_group": o.mqGroup, "mq_read": o.mqRead}
         return chosen, {k: round(v, 4) for k, v in zip(self.TUNE, list(t))}
 
+    def rate(self, i):
+        """Device rate control result of frame i of the last run."""
+        r = Rate()
+        _check(plugin().batch_rate(self.handle, i, C.byref(r)), "batch_rate")
+        return {"lambda": list(r.lambda_), "total": r.total, "comp": list(r.comp),
+                "tile_parts": list(r.tilePart)[:r.numTileParts], "distortion": r.distortion,
+                "iterations": r.iterations}
+
     def query(self):
         r = plugin().batch_query(self.handle)
         if r < 0:
@@ -465,6 +480,37 @@ def encode_codestream(layout, t1, fps=24.0, mbps=250.0, tag="hip7.2", build=True
     if n < 0:
         raise GailError(int(n), "rate control / Tier-2 failed")
     return (bytes(out[:n]) if build else None), lam.value, mse.value
+
+
+def build_codestream(layout, t1, tag="hip", cap=8 << 20):
+    """Tier-2 of a Tier-1 frame whose passes are already chosen (device rate control)."""
+    info, pl, pd, data = _t1_arrays(t1)
+    h = host()
+    h.gail_build_codestream.restype = C.c_int64
+    h.gail_build_codestream.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, _vp, C.c_uint64, _vp,
+                                        C.c_uint64, C.c_char_p]
+    out = np.zeros(cap, np.uint8)
+    n = h.gail_build_codestream(layout.width, layout.height, layout.fourk, layout.base_step, info.ctypes.data_as(_vp),
+                                pl.ctypes.data_as(_vp), pd.ctypes.data_as(_vp), data.ctypes.data_as(_vp), len(info),
+                                out.ctypes.data_as(_vp), cap, tag.encode())
+    if n < 0:
+        raise GailError(int(n), "Tier-2 failed")
+    return bytes(out[:n])
+
+
+def dci_budget(fps, mbps=250.0, comp_mbps=200.0):
+    """(frame budget, component budget) in bytes: floor(Mbit/s * 1e6 / (8 fps)) - 16
+    (rate_control.cpp:11-26)."""
+    import math
+    return math.floor(mbps * 1e6 / (8 * fps)) - 16, math.floor(comp_mbps * 1e6 / (8 * fps)) - 16
+
+
+def with_rate_control(settings, fps=24.0, mbps=250.0):
+    """A copy of encode settings with rate control on the device at fps / mbps."""
+    s = EncodeSettings()
+    C.pointer(s)[0] = settings
+    s.rcMaxTotal, s.rcMaxComp = dci_budget(fps, mbps)
+    return s
 
 
 def rate_lambda(layout, t1, fps=24.0, mbps=250.0):

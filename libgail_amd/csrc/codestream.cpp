@@ -1,5 +1,6 @@
 // Tier-2, rate control and the DCI structural check (see codestream.h).
 #include "codestream.h"
+#include "rc_shared.h"
 
 #include <algorithm>
 #include <array>
@@ -371,6 +372,65 @@ void buildCodestream(const Layout& L, const Tier1Ref& t1, const uint8_t* passes,
     if (sizes) *sizes = sz;
 }
 
+RcTables makeRcTables(const Layout& L) {
+    RcTables T;
+    const int ntp = L.fourk ? 6 : 3;
+    T.ntp = (uint32_t)ntp;
+    std::vector<uint8_t> hdr;
+    mainHeader(L, hdr, ntp, codestreamLabel());
+    const size_t slack = 1 + kMaxDeviceTagBytes;  // the COM text at its longest
+    T.fixedBytes = hdr.size() + slack + 2;
+    Volume vols[2];
+    const int nv = volumes(L, vols);
+    for (int vi = 0; vi < nv; ++vi)
+        for (int c = 0; c < kComps; ++c) {
+            const int tp = vi * 3 + c;
+            T.tpComp[tp] = c;
+            forEachPacket(L, vols[vi], c, [&](int r, int prec) {
+                const Precinct& P = L.resolution(c, r).precincts[prec];
+                RcPacket pk{(uint32_t)T.pbands.size(), (uint16_t)P.nb, (uint16_t)tp};
+                for (int k = 0; k < P.nb; ++k) {
+                    const PrecinctBand& pb = P.bands[k];
+                    const Subband& b = L.bands[pb.band];
+                    if (pb.ncbx > 8 || pb.ncby > 8) throw std::logic_error("precinct-band wider than 8 code-blocks");
+                    RcPBand q{};
+                    q.block0 = (uint32_t)(b.cblkBase + pb.cby0 * b.ncbx + pb.cbx0);
+                    q.stride = (uint16_t)b.ncbx;
+                    q.ncbx = (uint8_t)pb.ncbx;
+                    q.ncby = (uint8_t)pb.ncby;
+                    q.mb = (uint8_t)b.Mb;
+                    T.pbands.push_back(q);
+                }
+                T.packets.push_back(pk);
+            });
+        }
+    return T;
+}
+
+CodestreamSizes rcMeasure(const Layout& L, const RcTables& T, const Tier1Ref& t1, const uint8_t* passes) {
+    struct View {
+        const Tier1Ref* t1;
+        const uint8_t* p;
+        uint32_t passes(uint32_t i) const { return p[i]; }
+        uint32_t nbps(uint32_t i) const { return t1->nbps((int)i); }
+        uint32_t len(uint32_t i) const { return p[i] ? t1->len((int)i, p[i] - 1) : 0u; }
+    } view{&t1, passes};
+    (void)L;
+    CodestreamSizes sz;
+    sz.numTileParts = (int)T.ntp;
+    unsigned long long tp[6] = {};
+    for (uint32_t k = 0; k < T.ntp; ++k) tp[k] = 14;
+    RcTagTree incl, zbp;
+    for (const RcPacket& pk : T.packets) tp[pk.tp] += rc_packet_bytes(pk, T.pbands.data(), view, incl, zbp);
+    sz.total = T.fixedBytes;
+    for (uint32_t k = 0; k < T.ntp; ++k) {
+        sz.tilePart[k] = tp[k];
+        sz.comp[T.tpComp[k]] += tp[k];
+        sz.total += tp[k];
+    }
+    return sz;
+}
+
 size_t emptyCodestreamBytes(const Layout& L) {
     std::vector<uint32_t> info((size_t)L.numCblks() * 5, 0);
     const Tier1Ref t1{info.data(), nullptr, nullptr, nullptr, (size_t)L.numCblks()};
@@ -391,83 +451,40 @@ RateLimits dciLimits(double fps, double maxMbps, double compMbps) {
 // ---------------------------------------------------------------- rate control
 namespace {
 
-// Lower convex hull of each block's (rate, distortion) points, as start offsets plus the
-// passes, rates, slopes and distortion gains of the hull points (rate_control.cpp:38-86).
+// Lower convex hull of each block's (rate, distortion) points (rate_control.cpp:38-86), as
+// start offsets into one array of hull points.
 struct Hull {
     std::vector<uint32_t> start;
-    std::vector<uint8_t> passes;
-    std::vector<uint32_t> rate;
-    std::vector<double> slope, gain;
+    std::vector<RcPoint> pts;
 };
 
 // Fills H (its buffers are reused from call to call, so a worker thread that codes many
 // frames does not fault fresh pages in for every one).
 void buildHull(const Layout& L, const Tier1Ref& t1, Hull& H) {
-    constexpr int kMaxBlockPasses = 64;
     const int n = L.numCblks();
     size_t total = 0;
     for (int i = 0; i < n; ++i) total += t1.passes(i);
     H.start.resize(n + 1);
-    H.passes.resize(total);
-    H.rate.resize(total);
-    H.slope.resize(total);
-    H.gain.resize(total);
-    uint32_t R[kMaxBlockPasses + 1];
-    double D[kMaxBlockPasses + 1];
-    int K[kMaxBlockPasses + 1];
+    H.pts.resize(total);
     size_t out = 0;
     for (int i = 0; i < n; ++i) {
         H.start[i] = (uint32_t)out;
         const int np = (int)t1.passes(i);
         if (!np) continue;
-        if (np > kMaxBlockPasses) throw std::runtime_error("code-block with more than 64 coding passes");
+        if (np > kMaxPasses) throw std::runtime_error("code-block with more than 64 coding passes");
         const Subband& b = L.bands[L.cblks[i].band];
         const double scale = (double)b.delta * b.delta * b.weight;
-        const int nbps = (int)t1.nbps(i);
-        int m = 0;  // index of the last hull point; point 0 is the origin
-        R[0] = 0;
-        D[0] = 0.0;
-        K[0] = 0;
-        double acc = 0;
-        for (int k = 0; k < np; ++k) {
-            acc += t1.dist(i, k) * passPlaneScale(nbps, k) * scale;
-            const uint32_t r = t1.len(i, k);
-            if (acc <= D[m]) continue;
-            while (m >= 1) {
-                const double sNew = r == R[m] ? INFINITY : (acc - D[m]) / (double)(r - R[m]);
-                const double sLast = R[m] == R[m - 1] ? INFINITY : (D[m] - D[m - 1]) / (double)(R[m] - R[m - 1]);
-                if (sNew < sLast) break;
-                --m;
-            }
-            ++m;
-            R[m] = r;
-            D[m] = acc;
-            K[m] = k + 1;
-        }
-        for (int j = 1; j <= m; ++j, ++out) {
-            H.passes[out] = (uint8_t)K[j];
-            H.rate[out] = R[j];
-            const double dr = (double)R[j] - (double)R[j - 1];
-            H.slope[out] = dr <= 0 ? INFINITY : (D[j] - D[j - 1]) / dr;
-            H.gain[out] = D[j] - D[j - 1];
-        }
+        out += rc_block_hull(np, (int)t1.nbps(i), scale, &t1.passLen[t1.info5[5 * i + 4]],
+                             &t1.passDist[t1.info5[5 * i + 4]], &H.pts[out]);
     }
     H.start[n] = (uint32_t)out;
-    H.passes.resize(out);
-    H.rate.resize(out);
-    H.slope.resize(out);
-    H.gain.resize(out);
+    H.pts.resize(out);
 }
 
 void select(const Layout& L, const Hull& H, const double lambda[3], std::vector<uint8_t>& passes) {
     for (int i = 0; i < L.numCblks(); ++i) {
         const int c = L.bands[L.cblks[i].band].comp;
-        uint8_t np = 0;
-        for (uint32_t h = H.start[i]; h < H.start[i + 1]; ++h) {
-            if (H.slope[h] < lambda[c]) break;
-            np = H.passes[h];
-        }
-        passes[i] = np;
+        passes[i] = (uint8_t)rc_select(&H.pts[H.start[i]], (int)(H.start[i + 1] - H.start[i]), lambda[c]);
     }
 }
 
@@ -499,62 +516,36 @@ const OctaveSteps& octaveSteps() {
     return t;
 }
 
-namespace {
-
-}  // namespace
-
 int64_t slopeBin(double s) {  // s finite and > 0
     const OctaveSteps& t = octaveSteps();
-    uint64_t bits;
-    std::memcpy(&bits, &s, 8);
-    int e = (int)((bits >> 52) & 0x7ff);
-    uint64_t man = bits & ((1ull << 52) - 1);
-    if (e == 0) {  // subnormal: normalise
-        while (!(man & (1ull << 52))) {
-            man <<= 1;
-            --e;
-        }
-        man &= (1ull << 52) - 1;
-        e += 1;
-    }
-    int j = t.first[man >> 42];  // then the largest j with mantissa[j] <= man
-    while (j < 1023 && t.mantissa[j + 1] <= man) ++j;
-    return (int64_t)(e - 1023) * 1024 + j;
+    return rc_slope_bin(s, t.mantissa, t.first);
 }
 
-double binEdge(int64_t k) {  // the smallest double of bin k
-    const OctaveSteps& t = octaveSteps();
-    const int64_t oct = k >= 0 ? k / 1024 : -((-k + 1023) / 1024);
-    const int j = (int)(k - oct * 1024);
-    const uint64_t bits = ((uint64_t)(oct + 1023) << 52) | t.mantissa[j];
-    double v;
-    std::memcpy(&v, &bits, 8);
-    return v;
-}
+double binEdge(int64_t k) { return rc_bin_edge(k, octaveSteps().mantissa); }
 
 namespace {
 
-// Hull points binned by slope with suffix sums: data bytes and included-block counts for any
-// bin threshold in O(1) (rate_control.cpp:117-162).
+// Hull points binned by slope with suffix sums over the bins: data bytes and included-block
+// counts for any bin threshold in O(1) (rate_control.cpp:117-162). Integer sums.
 class SlopeHistogram {
 public:
     // Buffers are reused across frames by the thread that owns the histogram.
     void build(const Layout& L, const Hull& H, int64_t kLo, int nb) {
-        kLo_ = kLo;
+        const OctaveSteps& t = octaveSteps();
         nb_ = nb;
         for (int c = 0; c < 3; ++c) {
-            rate_[c].assign(nb_ + 1, 0.0);
-            incl_[c].assign(nb_ + 1, 0.0);
+            rate_[c].assign(nb_ + 1, 0ull);
+            incl_[c].assign(nb_ + 1, 0ull);
         }
         for (int i = 0; i < L.numCblks(); ++i) {
             const int c = L.bands[L.cblks[i].band].comp;
-            double* rate = rate_[c].data();
+            unsigned long long* rate = rate_[c].data();
             uint32_t prev = 0;
             for (uint32_t h = H.start[i]; h < H.start[i + 1]; ++h) {
-                const int b = bin(H.slope[h]);
-                rate[b] += (double)H.rate[h] - prev;
+                const int b = rc_bin_of(H.pts[h].slope, kLo, nb, t.mantissa, t.first);
+                rate[b] += H.pts[h].rate - prev;
                 if (h == H.start[i]) incl_[c][b] += 1;
-                prev = H.rate[h];
+                prev = H.pts[h].rate;
             }
         }
         for (int c = 0; c < 3; ++c)
@@ -563,21 +554,12 @@ public:
                 incl_[c][b] += incl_[c][b + 1];
             }
     }
-    // Bytes and included blocks of component c with every hull point of bin >= b.
-    double data(int c, int b) const { return rate_[c][std::clamp(b, 0, nb_)]; }
-    double incl(int c, int b) const { return incl_[c][std::clamp(b, 0, nb_)]; }
-    int bin(double s) const {
-        if (!std::isfinite(s)) return nb_ - 1;
-        if (!(s > 0)) return 0;
-        return (int)std::clamp<int64_t>(slopeBin(s) - kLo_, 0, nb_ - 1);
-    }
-    int bins() const { return nb_; }
-    double edge(int b) const { return binEdge(kLo_ + b); }
+    double data(int c, int b) const { return (double)rate_[c][b]; }
+    double incl(int c, int b) const { return (double)incl_[c][b]; }
 
 private:
-    int64_t kLo_ = 0;
     int nb_ = 0;
-    std::vector<double> rate_[3], incl_[3];
+    std::vector<unsigned long long> rate_[3], incl_[3];
 };
 
 struct RateScratch {
@@ -593,115 +575,40 @@ RateResult allocateRate(const Layout& L, const Tier1Ref& t1, const RateLimits& l
     buildHull(L, t1, scratch.hull);
     RateResult res;
     res.passes.assign(L.numCblks(), 0);
-    double lam[3] = {0, 0, 0};
+    const OctaveSteps& t = octaveSteps();
+    double maxSlope = 0, minSlope = INFINITY;
+    for (const RcPoint& p : H.pts)
+        if (std::isfinite(p.slope) && p.slope > 0) {
+            maxSlope = std::max(maxSlope, p.slope);
+            minSlope = std::min(minSlope, p.slope);
+        }
+    if (maxSlope == 0) maxSlope = minSlope = 1;
+    RcSearch S;
+    S.init();
+    rc_bin_range(minSlope, maxSlope, t.mantissa, t.first, S.kLo, S.nb);
+    S.minSlope = minSlope;
+    S.maxSlope = maxSlope;
+    S.maxTotal = lim.maxTotal;
+    S.maxComp = lim.maxComp;
+    S.mantissa = t.mantissa;
+    S.first = t.first;
+    SlopeHistogram& hist = scratch.hist;
+    hist.build(L, H, S.kLo, S.nb);
     auto measure = [&](const double l[3]) {
         select(L, H, l, res.passes);
         buildCodestream(L, t1, res.passes.data(), nullptr, &res.sizes);
-        ++res.iterations;
-        return res.sizes;
+        return RcSizes{res.sizes.total, {res.sizes.comp[0], res.sizes.comp[1], res.sizes.comp[2]}};
     };
-    double maxSlope = 0, minSlope = INFINITY;
-    for (double s : H.slope)
-        if (std::isfinite(s) && s > 0) {
-            maxSlope = std::max(maxSlope, s);
-            minSlope = std::min(minSlope, s);
-        }
-    if (maxSlope == 0) maxSlope = minSlope = 1;
-    // bins one octave beyond the slopes on either side
-    const int64_t kLo = slopeBin(minSlope) - 1024;
-    const int nb = (int)(slopeBin(maxSlope) + 1024 - kLo + 2);
-    SlopeHistogram& hist = scratch.hist;
-    hist.build(L, H, kLo, nb);
-    // a threshold of bin b; bin -1 stands for 0 (every pass)
-    int bin[3] = {-1, -1, -1};
-    auto apply = [&]() {
-        for (int c = 0; c < 3; ++c) lam[c] = bin[c] < 0 ? 0.0 : hist.edge(bin[c]);
-    };
-    CodestreamSizes sz = measure(lam);
-    // header bytes per component ~ calibrated + beta * (included blocks - calibrated count)
-    double hdrCal[3], inclCal[3], beta[3];
-    auto calibrate = [&](const CodestreamSizes& ex) {
-        for (int c = 0; c < 3; ++c) {
-            const double data = hist.data(c, bin[c]), incl = hist.incl(c, bin[c]);
-            hdrCal[c] = (double)ex.comp[c] - data;
-            inclCal[c] = incl;
-            beta[c] = incl > 0 ? std::max(1.0, std::min(8.0, hdrCal[c] / incl)) : 3.0;
-        }
-    };
-    auto model = [&](const int b[3], double comp[3]) {
-        double tot = (double)(sz.total - sz.comp[0] - sz.comp[1] - sz.comp[2]);
-        for (int c = 0; c < 3; ++c) {
-            comp[c] = hist.data(c, b[c]) + hdrCal[c] + beta[c] * (hist.incl(c, b[c]) - inclCal[c]);
-            tot += comp[c];
-        }
-        return tot;
-    };
-    // smallest bin in [lo, hi] that satisfies a predicate that holds from some bin on
-    auto search = [&](auto pred, int lo, int hi) {
-        while (lo < hi) {
-            const int mid = lo + (hi - lo) / 2;
-            if (pred(mid)) hi = mid;
-            else lo = mid + 1;
-        }
-        return lo;
-    };
-    if (sz.total > lim.maxTotal) {
-        calibrate(sz);
-        for (int round = 0; round < 6; ++round) {
-            const int b = search(
-                [&](int m) {
-                    const int bb[3] = {m, m, m};
-                    double comp[3];
-                    return model(bb, comp) <= (double)lim.maxTotal;
-                },
-                0, hist.bins());
-            if (round > 0 && b == bin[0]) break;  // the model has settled: the same sizes again
-            bin[0] = bin[1] = bin[2] = b;
-            apply();
-            sz = measure(lam);
-            if (sz.total <= lim.maxTotal && sz.total + lim.maxTotal / 500 >= lim.maxTotal) break;
-            calibrate(sz);
-        }
-        while (sz.total > lim.maxTotal && lam[0] <= maxSlope) {
-            for (double& l : lam) l = l > 0 ? l * 1.002 : minSlope;
-            sz = measure(lam);
-        }
-    }
-    for (int c = 0; c < kComps; ++c) {
-        if (sz.comp[c] <= lim.maxComp) continue;
-        // the bins follow the nudged thresholds
-        for (int k = 0; k < 3; ++k) bin[k] = lam[k] > 0 ? hist.bin(lam[k]) : -1;
-        calibrate(sz);
-        for (int round = 0; round < 6; ++round) {
-            const int b = search(
-                [&](int m) {
-                    int bb[3] = {bin[0], bin[1], bin[2]};
-                    bb[c] = m;
-                    double comp[3];
-                    model(bb, comp);
-                    return comp[c] <= (double)lim.maxComp;
-                },
-                std::max(bin[c], 0), hist.bins());
-            if (round > 0 && b == bin[c]) break;
-            bin[c] = b;
-            lam[c] = hist.edge(b);
-            sz = measure(lam);
-            if (sz.comp[c] <= lim.maxComp && sz.comp[c] + lim.maxComp / 500 >= lim.maxComp) break;
-            calibrate(sz);
-        }
-        while (sz.comp[c] > lim.maxComp && lam[c] <= maxSlope) {
-            lam[c] = lam[c] > 0 ? lam[c] * 1.002 : minSlope;
-            sz = measure(lam);
-        }
-    }
-    res.lambda = std::max({lam[0], lam[1], lam[2]});
-    for (int c = 0; c < 3; ++c) res.lambdaComp[c] = lam[c];
+    RcSizes sz = measure(S.lambda);
+    while (S.next(hist, sz)) sz = measure(S.lambda);
+    res.iterations = (int)S.iterations;
+    res.lambda = std::max({S.lambda[0], S.lambda[1], S.lambda[2]});
+    for (int c = 0; c < 3; ++c) res.lambdaComp[c] = S.lambda[c];
     double dist = 0;
     for (int i = 0; i < L.numCblks(); ++i)
         for (uint32_t h = H.start[i]; h < H.start[i + 1]; ++h)
-            if (H.passes[h] > res.passes[i]) dist += H.gain[h];
+            if (H.pts[h].passes > res.passes[i]) dist += H.pts[h].gain;
     res.distortion = dist;
-    res.sizes = sz;
     return res;
 }
 

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "device_types.h"
 #include "layout.h"
 
 namespace gail_amd {
@@ -66,13 +67,27 @@ struct RateResult {
 // The 1,024 steps of one octave as the 52-bit mantissas of exp2(j / 1024), j = 0..1023
 // (entry 1024 = 2^52): the grid rate control bins slopes on.
 struct OctaveSteps {
-    uint64_t mantissa[1025];
+    unsigned long long mantissa[1025];
     uint16_t first[1024];  // largest j with mantissa[j] <= r << 42, for r = 0..1023
 };
 const OctaveSteps& octaveSteps();
 // floor(1024 log2 s) for a finite s > 0, and the smallest double of that bin.
 int64_t slopeBin(double s);
 double binEdge(int64_t k);
+
+// A layout's packets for the device rate control: every packet with its precinct-bands
+// and tile-part, and the bytes that do not depend on the coded data.
+struct RcTables {
+    std::vector<RcPacket> packets;
+    std::vector<RcPBand> pbands;
+    uint32_t ntp = 0;
+    int tpComp[6] = {};
+    unsigned long long fixedBytes = 0;  // main header, COM reserve, EOC
+};
+RcTables makeRcTables(const Layout& L);
+// Codestream sizes from the tables and rc_packet_bytes (what the device computes) for
+// passes[i] passes of block i.
+CodestreamSizes rcMeasure(const Layout& L, const RcTables& T, const Tier1Ref& t1, const uint8_t* passes);
 
 // PCRD-opt under the limits (rate_control.cpp:165-279).
 RateResult allocateRate(const Layout& L, const Tier1Ref& t1, const RateLimits& lim);

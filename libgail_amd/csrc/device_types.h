@@ -25,7 +25,7 @@ struct T1Band {
     float inv_delta;    // 1 / delta, as the quantiser multiplies
     uint32_t max_mag;   // ((1 << Mb) << 6) - 1
     float dist_scale;   // delta^2 * weight: distortion units of the rate control
-    uint32_t pad;
+    uint32_t comp_mb;   // component | Mb << 8
 };
 
 // Per-frame device pointers of a job.
@@ -57,6 +57,47 @@ struct JobCounters {
     uint32_t pad;
     unsigned long long symbols;   // Tier-1 symbols of the job
     uint32_t pad2[10];
+};
+
+// ---------------------------------------------------------------- rate control on the device
+constexpr int kRcMaxBins = 65536;  // slope histogram bins per component (64 octaves)
+
+// One packet of a frame's codestream (8 bytes): precinct-bands [first, first + nband) and
+// the tile-part it belongs to.
+struct RcPacket {
+    uint32_t first;
+    uint16_t nband, tp;
+};
+
+// One precinct-band (12 bytes): the code-blocks block0 + j * stride + i (frame-local) for
+// i < ncbx, j < ncby, of a subband with Mb magnitude bit-planes.
+struct RcPBand {
+    uint32_t block0;
+    uint16_t stride;
+    uint8_t ncbx, ncby, mb, pad[3];
+};
+
+// A point of a code-block's rate-distortion hull (24 bytes).
+struct RcPoint {
+    double slope, gain;
+    uint32_t rate, passes;
+};
+
+// Rate control of one frame of a job: its parameters and, after the run, its result.
+struct RcFrame {
+    const RcPacket* packets;
+    const RcPBand* pbands;
+    uint32_t npackets, ntp;
+    uint32_t first_block, nblocks;       // the frame's blocks in the job
+    unsigned long long fixed_bytes;      // main header, COM reserve and EOC
+    unsigned long long max_total, max_comp;  // budgets; max_total 0: no rate control
+    int32_t tp_comp[6];                  // component of each tile-part
+    // result
+    double lambda[3];
+    double min_slope, max_slope;
+    unsigned long long total, comp[3], tile_part[6];
+    double distortion;
+    uint32_t iterations, status;         // status 0: done
 };
 
 }  // namespace gail_amd

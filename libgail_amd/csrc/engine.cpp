@@ -8,6 +8,7 @@
 #include <new>
 #include <stdexcept>
 
+#include "codestream.h"
 #include "errors.h"
 
 namespace gail_amd {
@@ -57,6 +58,37 @@ DeviceContext::DeviceContext(int device) : dev_(device) {
     buildCodeThresholds(t);
     hipCheck(hipMalloc(&thr_, t.size() * sizeof(float)), "code thresholds");
     hipCheck(hipMemcpy(thr_, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice), "code thresholds upload");
+    const OctaveSteps& o = octaveSteps();
+    hipCheck(hipMalloc(&octM_, sizeof o.mantissa), "slope grid");
+    hipCheck(hipMemcpy(octM_, o.mantissa, sizeof o.mantissa, hipMemcpyHostToDevice), "slope grid upload");
+    hipCheck(hipMalloc(&octF_, sizeof o.first), "slope grid");
+    hipCheck(hipMemcpy(octF_, o.first, sizeof o.first, hipMemcpyHostToDevice), "slope grid upload");
+}
+
+const DevRcTables& DeviceContext::rcTables(const Layout& L) {
+    // keyed by the geometry and every subband's Mb (the only layout data the tables hold)
+    std::string key = std::to_string(L.width) + "x" + std::to_string(L.height) + (L.fourk ? "k" : "s") +
+                      std::to_string(L.levels) + ":";
+    for (const Subband& b : L.bands) key += (char)('A' + b.Mb);
+    std::lock_guard<std::mutex> lk(rcMutex_);
+    auto it = rc_.find(key);
+    if (it != rc_.end()) return it->second;
+    const RcTables T = makeRcTables(L);
+    DevRcTables d;
+    RcPacket* pk = nullptr;
+    RcPBand* pb = nullptr;
+    hipCheck(hipSetDevice(dev_), "hipSetDevice");
+    hipCheck(hipMalloc(&pk, std::max<size_t>(16, T.packets.size() * sizeof(RcPacket))), "packet table");
+    hipCheck(hipMemcpy(pk, T.packets.data(), T.packets.size() * sizeof(RcPacket), hipMemcpyHostToDevice), "packet table");
+    hipCheck(hipMalloc(&pb, std::max<size_t>(16, T.pbands.size() * sizeof(RcPBand))), "precinct table");
+    hipCheck(hipMemcpy(pb, T.pbands.data(), T.pbands.size() * sizeof(RcPBand), hipMemcpyHostToDevice), "precinct table");
+    d.packets = pk;
+    d.pbands = pb;
+    d.npackets = (uint32_t)T.packets.size();
+    d.ntp = T.ntp;
+    for (int k = 0; k < 6; ++k) d.tpComp[k] = T.tpComp[k];
+    d.fixedBytes = T.fixedBytes;
+    return rc_.emplace(key, d).first->second;
 }
 
 // ---------------------------------------------------------------- job
@@ -129,6 +161,9 @@ size_t Job::estimateBytes(const std::vector<SlotCapacity>& slots, const JobOptio
     const Sizes z = t1Sizes(coefs, n, opt.poolBytesPerCoef);
     total += (size_t)n * (16 + 16 + 3 * kMaxPasses * 4 + kMaxPlanes * 8 + kMaxPasses * 16 + 4 + 8 + 8 + 4);
     total += (size_t)nb * sizeof(T1Band) + (size_t)lv * slots.size() * sizeof(DwtLevelDesc);
+    // device rate control: hull points, chosen passes, one histogram per slot
+    total += (size_t)n * (kMaxPasses * sizeof(RcPoint) + 4 + 1) + (size_t)nb * 8;
+    total += slots.size() * (6 * (size_t)(kRcMaxBins + 1) * 8 + sizeof(RcFrame));
     total += z.pool + z.out;
     return total;
 }
@@ -180,7 +215,8 @@ void Job::release() {
             d.hostLut = nullptr;
         }
     for (void** p : {(void**)&hFrames_, (void**)&hInfo_, (void**)&hPassLen_, (void**)&hPassDist_, (void**)&hData_,
-                     (void**)&hSmall_, (void**)&hBlocks_, (void**)&hBands_, (void**)&hLevels_})
+                     (void**)&hSmall_, (void**)&hBlocks_, (void**)&hBands_, (void**)&hLevels_, (void**)&hBandScale_,
+                     (void**)&hRc_})
         if (*p) {
             hipHostFree(*p);
             *p = nullptr;
@@ -240,6 +276,15 @@ void Job::allocate() {
     hostAlloc((void**)&hFrames_, sizeof(T1FrameRef) * ns);
     dLevels_ = devAlloc<DwtLevelDesc>((size_t)levelCap_ * ns);
     hostAlloc((void**)&hLevels_, sizeof(DwtLevelDesc) * levelCap_ * ns);
+    dBandScale_ = devAlloc<double>(bandCap_);
+    hostAlloc((void**)&hBandScale_, sizeof(double) * bandCap_);
+    dRc_ = devAlloc<RcFrame>(ns);
+    hostAlloc((void**)&hRc_, sizeof(RcFrame) * ns);
+    std::memset(hRc_, 0, sizeof(RcFrame) * ns);
+    dHull_ = devAlloc<RcPoint>((size_t)blockCap_ * kMaxPasses);
+    dHullN_ = devAlloc<uint32_t>(blockCap_);
+    dChosen_ = devAlloc<uint8_t>(blockCap_);
+    dRcHist_ = devAlloc<unsigned long long>((size_t)ns * 6 * (kRcMaxBins + 1));
 
     const uint32_t n = blockCap_;
     dInfo_ = devAlloc<T1Info>(n);
@@ -288,6 +333,7 @@ void Job::bind(std::vector<FrameConfig> frames) {
     launched_ = false;
     uint32_t nblocks = 0, nbands = 0;
     int maxLevels = 0;
+    bool anyRc = false;
     for (int f = 0; f < nf; ++f) {
         const FrameConfig& fc = frames[f];
         const Layout& L = fc.layout;
@@ -314,8 +360,10 @@ void Job::bind(std::vector<FrameConfig> frames) {
         if (nblocks + d.numBlocks > blockCap_ || nbands + L.bands.size() > bandCap_)
             throw std::logic_error("frame layout exceeds the job's block capacity");
         const uint32_t bandBase = nbands;
+        for (const Subband& b : L.bands) hBandScale_[nbands + (&b - L.bands.data())] = (double)b.delta * b.delta * b.weight;
         for (const Subband& b : L.bands)
-            hBands_[nbands++] = T1Band{1.0f / b.delta, ((1u << b.Mb) << 6) - 1u, (float)((double)b.delta * b.delta * b.weight), 0u};
+            hBands_[nbands++] = T1Band{1.0f / b.delta, ((1u << b.Mb) << 6) - 1u, (float)((double)b.delta * b.delta * b.weight),
+                                       (uint32_t)b.comp | ((uint32_t)b.Mb << 8)};
         T1Block* out = hBlocks_ + nblocks;
         for (int i = 0; i < L.numCblks(); ++i) {
             const CodeBlock& cb = L.cblks[i];
@@ -330,6 +378,22 @@ void Job::bind(std::vector<FrameConfig> frames) {
             t.frame_block = (uint32_t)i;
             out[i] = t;
         }
+        RcFrame& rc = hRc_[f];
+        std::memset(&rc, 0, sizeof rc);
+        if (fc.rcMaxTotal) {
+            const DevRcTables& T = ctx_.rcTables(L);
+            rc.packets = T.packets;
+            rc.pbands = T.pbands;
+            rc.npackets = T.npackets;
+            rc.ntp = T.ntp;
+            rc.first_block = nblocks;
+            rc.nblocks = d.numBlocks;
+            rc.fixed_bytes = T.fixedBytes;
+            rc.max_total = fc.rcMaxTotal;
+            rc.max_comp = fc.rcMaxComp ? fc.rcMaxComp : fc.rcMaxTotal;
+            for (int k = 0; k < 6; ++k) rc.tp_comp[k] = T.tpComp[k];
+            anyRc = true;
+        }
         nblocks += d.numBlocks;
         maxLevels = std::max(maxLevels, L.levels);
         hFrames_[f] = T1FrameRef{d.coef, (uint32_t)W, 0.f};
@@ -338,6 +402,9 @@ void Job::bind(std::vector<FrameConfig> frames) {
     hipStream_t s = stream_;
     hipCheck(hipMemcpyAsync(dBlocks_, hBlocks_, sizeof(T1Block) * nblocks, hipMemcpyHostToDevice, s), "block table");
     hipCheck(hipMemcpyAsync(dBands_, hBands_, sizeof(T1Band) * nbands, hipMemcpyHostToDevice, s), "band table");
+    hipCheck(hipMemcpyAsync(dBandScale_, hBandScale_, sizeof(double) * nbands, hipMemcpyHostToDevice, s), "band scales");
+    hipCheck(hipMemcpyAsync(dRc_, hRc_, sizeof(RcFrame) * nf, hipMemcpyHostToDevice, s), "rate-control frames");
+    anyRc_ = anyRc;
     nblocks_ = nblocks;
     numLevels_ = maxLevels;
     frames_ = std::move(frames);
@@ -606,6 +673,25 @@ void Job::enqueueT1(hipStream_t s) {
         1, std::min<uint32_t>((n + opt_.mqGroup - 1) / opt_.mqGroup, (uint32_t)(ctx_.computeUnits() * opt_.mqGroupsPerCU)));
     hipCheck(launch_t1_mq(mq, opt_.mqGroup, opt_.mqRead, mqGroups, s), "MQ coding");
     if (!capturing_) hipCheck(hipEventRecord(ev_[5], s), "event");
+    if (anyRc_) {
+        RcParams rp{};
+        rp.frames = dRc_;
+        rp.blocks = dBlocks_;
+        rp.bands = dBands_;
+        rp.band_scale = dBandScale_;
+        rp.info = dInfo_;
+        rp.pass_len = dPassLen_;
+        rp.pass_dist = dPassDist_;
+        rp.code_len = dCodeLen_;
+        rp.hull = dHull_;
+        rp.hull_n = dHullN_;
+        rp.hist = dRcHist_;
+        rp.mantissa = ctx_.octaveMantissa();
+        rp.first = ctx_.octaveFirst();
+        rp.chosen = dChosen_;
+        rp.counters = dCounters_;
+        hipCheck(launch_rc(rp, numFrames(), s), "rate control");
+    }
     hipCheck(launch_compact_offsets(dInfo_, dCodeLen_, n, dDataOff_, nullptr, dPartial_, s), "compaction offsets");
     // dense results live in the (now consumed) symbol pool
     const size_t dataBytes = alignUp(outCap_);
@@ -764,6 +850,8 @@ void Job::finish(StageTimes* times) {
     hipCheck(hipMemcpyAsync(hPassLen_, dDensePassLen_, numPassesTotal_ * 4, hipMemcpyDeviceToHost, s), "result download");
     hipCheck(hipMemcpyAsync(hPassDist_, dDensePassDist_, numPassesTotal_ * 4, hipMemcpyDeviceToHost, s), "result download");
     hipCheck(hipMemcpyAsync(hData_, dDenseData_, numDataTotal_, hipMemcpyDeviceToHost, s), "result download");
+    if (anyRc_)
+        hipCheck(hipMemcpyAsync(hRc_, dRc_, sizeof(RcFrame) * numFrames(), hipMemcpyDeviceToHost, s), "rate-control results");
     hipCheck(hipEventRecord(ev_[7], s), "event");
     hipCheck(hipStreamSynchronize(s), "result download");
     // frame-local offsets
@@ -805,6 +893,11 @@ void Job::finish(StageTimes* times) {
 void Job::encode(bool uploadInputs, StageTimes* times) {
     launch(uploadInputs);
     finish(times);
+}
+
+const RcFrame* Job::rateResult(int i) const {
+    if (i < 0 || i >= numFrames() || !hRc_[i].max_total) return nullptr;
+    return &hRc_[i];
 }
 
 Tier1View Job::result(int i) const {

@@ -5,7 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -31,6 +33,8 @@ struct FrameConfig {
     Layout layout;
     InputFormat input;
     ColorSpec color;
+    // rate control on the device: the frame's byte budget and component cap (0: on the host)
+    unsigned long long rcMaxTotal = 0, rcMaxComp = 0;
 };
 
 struct JobOptions {
@@ -62,12 +66,24 @@ struct Tier1View {
 
 // Device-wide state shared by every job on a device: the constant Tier-1 tables and the
 // colour / entropy lookup tables.
+// A layout's packet tables on the device (built once per layout and device).
+struct DevRcTables {
+    const RcPacket* packets = nullptr;
+    const RcPBand* pbands = nullptr;
+    uint32_t npackets = 0, ntp = 0;
+    int tpComp[6] = {};
+    unsigned long long fixedBytes = 0;
+};
+
 class DeviceContext {
 public:
     static DeviceContext& get(int device);
     int device() const { return dev_; }
     const float* log2Table() const { return log2_; }
     const float* thresholds() const { return thr_; }
+    const unsigned long long* octaveMantissa() const { return octM_; }
+    const uint16_t* octaveFirst() const { return octF_; }
+    const DevRcTables& rcTables(const Layout& L);
     int computeUnits() const { return cus_; }
     std::string name() const { return name_; }
     std::string arch() const { return arch_; }
@@ -77,6 +93,10 @@ private:
     int dev_ = 0, cus_ = 0;
     float* log2_ = nullptr;
     float* thr_ = nullptr;
+    unsigned long long* octM_ = nullptr;
+    uint16_t* octF_ = nullptr;
+    std::mutex rcMutex_;
+    std::map<std::string, DevRcTables> rc_;
     std::string name_, arch_;
 };
 
@@ -149,6 +169,8 @@ public:
     bool done() const;
 
     Tier1View result(int i) const;
+    // Device rate control result of frame i of the last run (nullptr when it had none).
+    const RcFrame* rateResult(int i) const;
     // Stage times of the last run (ms) and its symbol count.
     const StageTimes& lastTimes() const { return last_; }
     // Times every DWT tile shape, context-modelling workgroup size and MQ workgroup size on
@@ -229,6 +251,16 @@ private:
     T1FrameRef* hFrames_ = nullptr;  // copied before each run (it carries the thresholds)
     DwtLevelDesc* dLevels_ = nullptr;  // levelCap_ x slots
     DwtLevelDesc* hLevels_ = nullptr;
+    // device rate control
+    bool anyRc_ = false;
+    double* dBandScale_ = nullptr;
+    double* hBandScale_ = nullptr;
+    RcFrame* dRc_ = nullptr;
+    RcFrame* hRc_ = nullptr;         // parameters staged by bind(), results after finish()
+    RcPoint* dHull_ = nullptr;
+    uint32_t* dHullN_ = nullptr;
+    uint8_t* dChosen_ = nullptr;
+    unsigned long long* dRcHist_ = nullptr;
     // Tier-1 arrays
     T1Info* dInfo_ = nullptr;
     uint32_t* dPassEnd_ = nullptr;

@@ -59,9 +59,66 @@ extern "C" GAIL_AMD_EXPORT int64_t gail_encode_codestream(int W, int H, int four
     }
 }
 
+// Sizes of the same frame measured two ways, buildCodestream and the device's packet-size
+// code run on the host, over `trials` pass choices (all passes, none, and random truncations
+// from seed); returns the number of choices where any size differs, or -1 on error.
+extern "C" GAIL_AMD_EXPORT int gail_rc_measure_check(int W, int H, int fourk, double baseStep, const uint32_t* info5,
+                                                     const uint32_t* passLen, const int32_t* passDist,
+                                                     const uint8_t* data, uint64_t nblocks, int trials, uint64_t seed) {
+    try {
+        const Layout& L = cachedLayout(W, H, fourk != 0, baseStep);
+        if ((uint64_t)L.numCblks() != nblocks) return -1;
+        const Tier1Ref t1{info5, passLen, passDist, data, (size_t)nblocks};
+        const RcTables T = makeRcTables(L);
+        std::vector<uint8_t> passes(nblocks);
+        uint64_t x = seed * 0x9E3779B97F4A7C15ull + 1;
+        int bad = 0;
+        for (int t = 0; t < trials; ++t) {
+            for (uint64_t i = 0; i < nblocks; ++i) {
+                const uint32_t np = t1.passes((int)i);
+                x ^= x << 13;
+                x ^= x >> 7;
+                x ^= x << 17;
+                passes[i] = (uint8_t)(t == 0 ? np : t == 1 ? 0 : (np ? x % (np + 1) : 0));
+            }
+            CodestreamSizes a, b = rcMeasure(L, T, t1, passes.data());
+            buildCodestream(L, t1, passes.data(), nullptr, &a);
+            bool same = a.total == b.total && a.numTileParts == b.numTileParts;
+            for (int c = 0; c < 3; ++c) same = same && a.comp[c] == b.comp[c];
+            for (int k = 0; k < a.numTileParts; ++k) same = same && a.tilePart[k] == b.tilePart[k];
+            if (!same) ++bad;
+        }
+        return bad;
+    } catch (...) {
+        return -1;
+    }
+}
+
 // The rate-control slope grid (floor(1024 log2 s) and a bin's lowest double), for tests.
 extern "C" GAIL_AMD_EXPORT int64_t gail_slope_bin(double s) { return s > 0 && std::isfinite(s) ? slopeBin(s) : 0; }
 extern "C" GAIL_AMD_EXPORT double gail_slope_bin_edge(int64_t k) { return binEdge(k); }
+
+// Tier-2 of a frame whose passes were already chosen (each block keeps t1.passes(i), as the
+// device rate control leaves them): the codestream bytes, or -1 / -2 (too small a buffer).
+extern "C" GAIL_AMD_EXPORT int64_t gail_build_codestream(int W, int H, int fourk, double baseStep, const uint32_t* info5,
+                                                         const uint32_t* passLen, const int32_t* passDist,
+                                                         const uint8_t* data, uint64_t nblocks, uint8_t* out, uint64_t cap,
+                                                         const char* tag) {
+    try {
+        const Layout& L = cachedLayout(W, H, fourk != 0, baseStep);
+        if ((uint64_t)L.numCblks() != nblocks) return -1;
+        const Tier1Ref t1{info5, passLen, passDist, data, (size_t)nblocks};
+        std::vector<uint8_t> passes(nblocks);
+        for (uint64_t i = 0; i < nblocks; ++i) passes[i] = (uint8_t)t1.passes((int)i);
+        std::vector<uint8_t> cs;
+        buildCodestream(L, t1, passes.data(), &cs, nullptr, tag ? tag : "");
+        if (cs.size() > cap) return -2;
+        std::memcpy(out, cs.data(), cs.size());
+        return (int64_t)cs.size();
+    } catch (...) {
+        return -1;
+    }
+}
 
 // "" (returned as 0) when the codestream passes, else a message copied into msg.
 extern "C" GAIL_AMD_EXPORT int gail_check_codestream(const uint8_t* data, uint64_t size, double fps, int fourk,

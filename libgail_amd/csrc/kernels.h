@@ -87,6 +87,26 @@ struct CompactParams {
     uint32_t nblocks;
 };
 
+// Device rate control (kernels_rc.hip): one workgroup per frame, after MQ, before compaction.
+struct RcParams {
+    RcFrame* frames;
+    const T1Block* blocks;
+    const T1Band* bands;
+    const double* band_scale;   // [band] delta^2 * weight in double
+    T1Info* info;               // passes overwritten with the chosen ones
+    const uint32_t* pass_len;
+    const int32_t* pass_dist;
+    uint32_t* code_len;         // overwritten with the chosen passes' length
+    RcPoint* hull;              // [block][kMaxPasses]
+    uint32_t* hull_n;           // [block]
+    unsigned long long* hist;   // [frame][6][kRcMaxBins + 1]
+    const unsigned long long* mantissa;  // [1025] the octave's steps
+    const uint16_t* first;               // [1024]
+    uint8_t* chosen;            // [block]
+    const JobCounters* counters;
+};
+hipError_t launch_rc(const RcParams& p, int frames, hipStream_t s);
+
 hipError_t launch_dwt_level(bool level0, int tile_shape, const DwtLevelDesc* descs, int ndesc, int tiles,
                             hipStream_t s);
 void dwt_tile_size(int tile_shape, int& tx, int& ty);

@@ -128,6 +128,8 @@ FrameConfig toFrame(const GailHipLayout& l, const GailHipEncodeSettings& s) {
             in.oy + in.activeH > f.input.height)
             throw std::logic_error("planar picture does not fit the container");
     }
+    f.rcMaxTotal = s.rcMaxTotal;
+    f.rcMaxComp = s.rcMaxComp;
     f.color.transfer = s.color.transfer;
     f.color.primaries = s.color.primaries;
     f.color.sdrWhite = s.color.sdrWhite > 0 ? s.color.sdrWhite : 48.0;
@@ -472,6 +474,22 @@ int isDevicePointer(const void* p) {
     return a.type == hipMemoryTypeDevice ? 1 : 0;
 }
 
+int batchRate(GailHipBatch* b, int i, GailHipRate* out) {
+    return guarded([&] {
+        const RcFrame* r = b->job->rateResult(i);
+        if (!r || !out) throw std::logic_error("the frame had no rate control on the device");
+        for (int c = 0; c < 3; ++c) {
+            out->lambda[c] = r->lambda[c];
+            out->comp[c] = r->comp[c];
+        }
+        out->total = r->total;
+        for (int k = 0; k < 6; ++k) out->tilePart[k] = r->tile_part[k];
+        out->distortion = r->distortion;
+        out->iterations = r->iterations;
+        out->numTileParts = r->ntp;
+    });
+}
+
 #define GAIL_STR2(x) #x
 #define GAIL_STR(x) GAIL_STR2(x)
 
@@ -523,6 +541,7 @@ const GailHipPlugin g_plugin = {
     deviceFreeEntry,
     copyEntry,
     isDevicePointer,
+    batchRate,
 };
 
 }  // namespace

@@ -28,7 +28,7 @@ def test_plugin_table():
     # layout hash over the plain structures, computed the way gail_hip_plugin_layout() does
     sizes = [C.sizeof(t) for t in (g.DeviceDesc, g.Subband, g.Layout, g.FrameFormat, g.Color, g.EncodeSettings,
                                    g.Memory, g.CblkInfo, g.Tier1, g.FrameTimings, g.Debug, g.BatchOptions, g.Slot,
-                                   g.Plugin)]
+                                   g.Rate, g.Plugin)]
     h = 1469598103934665603
     for s in sizes:
         h = ((h ^ s) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
