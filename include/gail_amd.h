@@ -268,6 +268,8 @@ typedef struct gail_server_config {
     int cpu_threads;        /* rate control + Tier-2 + input threads (0 = usable cores - 1) */
     int check;              /* run the DCI codestream check on every frame */
     int batch_window_us;    /* an idle job waits up to this long for more frames (0: launch at once) */
+    int host_rate_control;  /* 1: PCRD on the host pool; 0 (default): on the GPU, only the kept
+                               bytes of each frame come back and the pool writes Tier-2 */
     gail_server_cb on_result; /* called from pool threads, one call at a time */
     void* user;
 } gail_server_config;

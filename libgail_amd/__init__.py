@@ -569,7 +569,8 @@ SERVER_CB = C.CFUNCTYPE(None, _vp, C.POINTER(ServerResult))
 class ServerConfig(C.Structure):
     _fields_ = [("struct_size", C.c_size_t), ("device", C.c_int), ("slots_per_job", C.c_int), ("max_jobs", C.c_int),
                 ("memory_budget", C.c_uint64), ("cpu_threads", C.c_int), ("check", C.c_int),
-                ("batch_window_us", C.c_int), ("on_result", SERVER_CB), ("user", _vp)]
+                ("batch_window_us", C.c_int), ("host_rate_control", C.c_int), ("on_result", SERVER_CB),
+                ("user", _vp)]
 
 
 class ServerStats(C.Structure):
@@ -686,7 +687,7 @@ class Server:
     the codestream bytes."""
 
     def __init__(self, device=0, slots_per_job=8, max_jobs=4, memory_budget=0, cpu_threads=0, check=False,
-                 keep=False, batch_window_us=0, on_done=None):
+                 keep=False, batch_window_us=0, on_done=None, host_rate_control=False):
         h = _lib_api()
         self.keep = keep
         self.on_done = on_done  # called with (id, result dict) after each result is stored
@@ -698,6 +699,7 @@ class Server:
         cfg.device, cfg.slots_per_job, cfg.max_jobs = device, slots_per_job, max_jobs
         cfg.memory_budget, cfg.cpu_threads, cfg.check = memory_budget, cpu_threads, int(check)
         cfg.batch_window_us = int(batch_window_us)
+        cfg.host_rate_control = int(host_rate_control)
         self._cb = SERVER_CB(self._on_result)
         cfg.on_result = self._cb
         self.handle = _vp()

@@ -178,7 +178,17 @@ struct gail_server {
         try {
             const GailHipTier1& t = j.t1[i];
             const Tier1Ref ref{(const uint32_t*)t.info, t.passLen, t.passDist, t.data, (size_t)t.numInfo};
-            RateResult rr = allocateRate(sp.L, ref, sp.limits);
+            RateResult rr;
+            GailHipRate dr{};
+            if (!cfg.host_rate_control && P->batch_rate(j.batch, i, &dr) == GAIL_HIP_OK) {
+                // the device kept each block's chosen passes and only their bytes
+                rr.passes.resize(ref.numBlocks);
+                for (size_t b = 0; b < ref.numBlocks; ++b) rr.passes[b] = (uint8_t)ref.passes((int)b);
+                for (int c = 0; c < 3; ++c) rr.lambdaComp[c] = dr.lambda[c];
+                rr.distortion = dr.distortion;
+            } else {
+                rr = allocateRate(sp.L, ref, sp.limits);
+            }
             const double lam = std::min({rr.lambdaComp[0], rr.lambdaComp[1], rr.lambdaComp[2]});
             if (r.autoSlope && r.slope > 0 && lam < r.slope) {
                 requeue = true;  // passes the rate control keeps may have been skipped
@@ -250,8 +260,13 @@ struct gail_server {
         std::vector<GailHipLayout> layouts(n);
         std::vector<GailHipEncodeSettings> settings(n);
         for (int i = 0; i < n; ++i) {
-            layouts[i] = j.reqs[i]->st->spec->plainLayout;
-            settings[i] = j.reqs[i]->st->spec->settings;
+            const StreamSpec& sp = *j.reqs[i]->st->spec;
+            layouts[i] = sp.plainLayout;
+            settings[i] = sp.settings;
+            if (!cfg.host_rate_control) {  // rate control on the device, the stream's budgets
+                settings[i].rcMaxTotal = sp.limits.maxTotal;
+                settings[i].rcMaxComp = sp.limits.maxComp;
+            }
         }
         auto failAll = [&](const std::string& why) {
             for (auto& r : j.reqs) {
