@@ -392,7 +392,8 @@ RcTables makeRcTables(const Layout& L) {
                 for (int k = 0; k < P.nb; ++k) {
                     const PrecinctBand& pb = P.bands[k];
                     const Subband& b = L.bands[pb.band];
-                    if (pb.ncbx > 8 || pb.ncby > 8) throw std::logic_error("precinct-band wider than 8 code-blocks");
+                    if (pb.ncbx > 4 || pb.ncby > 4) throw std::logic_error("precinct-band wider than 4 code-blocks");
+                    if (b.Mb > 31) throw std::logic_error("subband with more than 31 magnitude bit-planes");
                     RcPBand q{};
                     q.block0 = (uint32_t)(b.cblkBase + pb.cby0 * b.ncbx + pb.cbx0);
                     q.stride = (uint16_t)b.ncbx;
@@ -420,8 +421,7 @@ CodestreamSizes rcMeasure(const Layout& L, const RcTables& T, const Tier1Ref& t1
     sz.numTileParts = (int)T.ntp;
     unsigned long long tp[6] = {};
     for (uint32_t k = 0; k < T.ntp; ++k) tp[k] = 14;
-    RcTagTree incl, zbp;
-    for (const RcPacket& pk : T.packets) tp[pk.tp] += rc_packet_bytes(pk, T.pbands.data(), view, incl, zbp);
+    for (const RcPacket& pk : T.packets) tp[pk.tp] += rc_packet_bytes(pk, T.pbands.data(), view);
     sz.total = T.fixedBytes;
     for (uint32_t k = 0; k < T.ntp; ++k) {
         sz.tilePart[k] = tp[k];

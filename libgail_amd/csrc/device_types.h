@@ -92,6 +92,9 @@ struct RcFrame {
     unsigned long long fixed_bytes;      // main header, COM reserve and EOC
     unsigned long long max_total, max_comp;  // budgets; max_total 0: no rate control
     int32_t tp_comp[6];                  // component of each tile-part
+    // slope histogram bins [k_lo, k_lo + nb) of the run
+    int64_t k_lo;
+    int32_t nb, pad;
     // result
     double lambda[3];
     double min_slope, max_slope;

@@ -161,9 +161,11 @@ size_t Job::estimateBytes(const std::vector<SlotCapacity>& slots, const JobOptio
     const Sizes z = t1Sizes(coefs, n, opt.poolBytesPerCoef);
     total += (size_t)n * (16 + 16 + 3 * kMaxPasses * 4 + kMaxPlanes * 8 + kMaxPasses * 16 + 4 + 8 + 8 + 4);
     total += (size_t)nb * sizeof(T1Band) + (size_t)lv * slots.size() * sizeof(DwtLevelDesc);
-    // device rate control: hull points, chosen passes, one histogram per slot
-    total += (size_t)n * (kMaxPasses * sizeof(RcPoint) + 4 + 1) + (size_t)nb * 8;
-    total += slots.size() * (6 * (size_t)(kRcMaxBins + 1) * 8 + sizeof(RcFrame));
+    // device rate control: hull points and their slope bins, chosen passes, slope ranges,
+    // one histogram per slot
+    total += (size_t)n * (kMaxPasses * (8 + 8 + 4 + 4) + 4 + 4) + (size_t)nb * 8;
+    total += 16 * rcPartEntries(n, slots.size()) + 4 * rc_segsum_entries(slots.size());
+    total += slots.size() * (6 * (size_t)(kRcMaxBins + 1) * 4 + sizeof(RcFrame));
     total += z.pool + z.out;
     return total;
 }
@@ -281,10 +283,17 @@ void Job::allocate() {
     dRc_ = devAlloc<RcFrame>(ns);
     hostAlloc((void**)&hRc_, sizeof(RcFrame) * ns);
     std::memset(hRc_, 0, sizeof(RcFrame) * ns);
-    dHull_ = devAlloc<RcPoint>((size_t)blockCap_ * kMaxPasses);
+    dHullSlope_ = devAlloc<double>((size_t)blockCap_ * kMaxPasses);
+    dHullGain_ = devAlloc<double>((size_t)blockCap_ * kMaxPasses);
+    dHullSel_ = devAlloc<uint32_t>((size_t)blockCap_ * kMaxPasses);
+    dHullKbin_ = devAlloc<int32_t>((size_t)blockCap_ * kMaxPasses);
     dHullN_ = devAlloc<uint32_t>(blockCap_);
-    dChosen_ = devAlloc<uint8_t>(blockCap_);
-    dRcHist_ = devAlloc<unsigned long long>((size_t)ns * 6 * (kRcMaxBins + 1));
+    dRcSel_ = devAlloc<uint32_t>(blockCap_);
+    dRcPart_ = devAlloc<unsigned long long>(2 * rcPartEntries(blockCap_, ns));
+    dRcSegsum_ = devAlloc<uint32_t>(rc_segsum_entries(ns));
+    // the rate control leaves its histogram zero after every run; it starts zero
+    dRcHist_ = devAlloc<uint32_t>((size_t)ns * 6 * (kRcMaxBins + 1));
+    hipCheck(hipMemset(dRcHist_, 0, sizeof(uint32_t) * ns * 6 * (kRcMaxBins + 1)), "histogram clear");
 
     const uint32_t n = blockCap_;
     dInfo_ = devAlloc<T1Info>(n);
@@ -334,6 +343,7 @@ void Job::bind(std::vector<FrameConfig> frames) {
     uint32_t nblocks = 0, nbands = 0;
     int maxLevels = 0;
     bool anyRc = false;
+    uint32_t rcMaxBlocks = 0;
     for (int f = 0; f < nf; ++f) {
         const FrameConfig& fc = frames[f];
         const Layout& L = fc.layout;
@@ -393,6 +403,7 @@ void Job::bind(std::vector<FrameConfig> frames) {
             rc.max_comp = fc.rcMaxComp ? fc.rcMaxComp : fc.rcMaxTotal;
             for (int k = 0; k < 6; ++k) rc.tp_comp[k] = T.tpComp[k];
             anyRc = true;
+            rcMaxBlocks = std::max(rcMaxBlocks, d.numBlocks);
         }
         nblocks += d.numBlocks;
         maxLevels = std::max(maxLevels, L.levels);
@@ -405,6 +416,7 @@ void Job::bind(std::vector<FrameConfig> frames) {
     hipCheck(hipMemcpyAsync(dBandScale_, hBandScale_, sizeof(double) * nbands, hipMemcpyHostToDevice, s), "band scales");
     hipCheck(hipMemcpyAsync(dRc_, hRc_, sizeof(RcFrame) * nf, hipMemcpyHostToDevice, s), "rate-control frames");
     anyRc_ = anyRc;
+    rcMaxBlocks_ = rcMaxBlocks;
     nblocks_ = nblocks;
     numLevels_ = maxLevels;
     frames_ = std::move(frames);
@@ -683,14 +695,20 @@ void Job::enqueueT1(hipStream_t s) {
         rp.pass_len = dPassLen_;
         rp.pass_dist = dPassDist_;
         rp.code_len = dCodeLen_;
-        rp.hull = dHull_;
+        rp.stride = blockCap_;
+        rp.h_slope = dHullSlope_;
+        rp.h_gain = dHullGain_;
+        rp.h_sel = dHullSel_;
+        rp.h_kbin = dHullKbin_;
         rp.hull_n = dHullN_;
+        rp.part = dRcPart_;
         rp.hist = dRcHist_;
+        rp.segsum = dRcSegsum_;
         rp.mantissa = ctx_.octaveMantissa();
         rp.first = ctx_.octaveFirst();
-        rp.chosen = dChosen_;
+        rp.sel = dRcSel_;
         rp.counters = dCounters_;
-        hipCheck(launch_rc(rp, numFrames(), s), "rate control");
+        hipCheck(launch_rc(rp, numFrames(), rcMaxBlocks_, s), "rate control");
     }
     hipCheck(launch_compact_offsets(dInfo_, dCodeLen_, n, dDataOff_, nullptr, dPartial_, s), "compaction offsets");
     // dense results live in the (now consumed) symbol pool

@@ -257,10 +257,16 @@ private:
     double* hBandScale_ = nullptr;
     RcFrame* dRc_ = nullptr;
     RcFrame* hRc_ = nullptr;         // parameters staged by bind(), results after finish()
-    RcPoint* dHull_ = nullptr;
+    double* dHullSlope_ = nullptr;   // hull points, point-major
+    double* dHullGain_ = nullptr;
+    uint32_t* dHullSel_ = nullptr;
+    int32_t* dHullKbin_ = nullptr;
     uint32_t* dHullN_ = nullptr;
-    uint8_t* dChosen_ = nullptr;
-    unsigned long long* dRcHist_ = nullptr;
+    uint32_t* dRcSel_ = nullptr;
+    unsigned long long* dRcPart_ = nullptr;
+    uint32_t* dRcSegsum_ = nullptr;
+    uint32_t rcMaxBlocks_ = 0;       // most blocks of a rate-controlled frame
+    uint32_t* dRcHist_ = nullptr;
     // Tier-1 arrays
     T1Info* dInfo_ = nullptr;
     uint32_t* dPassEnd_ = nullptr;

@@ -79,7 +79,10 @@ extern "C" GAIL_AMD_EXPORT int gail_rc_measure_check(int W, int H, int fourk, do
                 x ^= x << 13;
                 x ^= x >> 7;
                 x ^= x << 17;
-                passes[i] = (uint8_t)(t == 0 ? np : t == 1 ? 0 : (np ? x % (np + 1) : 0));
+                // all passes, none, random truncations, and sparse inclusion (about one block
+                // in four) that leaves whole quads and precinct-bands empty
+                const uint32_t r = np ? (uint32_t)(x % (np + 1)) : 0u;
+                passes[i] = (uint8_t)(t == 0 ? np : t == 1 ? 0 : (t & 1) ? r : ((x >> 40) % 4 == 0 ? r : 0u));
             }
             CodestreamSizes a, b = rcMeasure(L, T, t1, passes.data());
             buildCodestream(L, t1, passes.data(), nullptr, &a);

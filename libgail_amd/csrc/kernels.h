@@ -87,7 +87,8 @@ struct CompactParams {
     uint32_t nblocks;
 };
 
-// Device rate control (kernels_rc.hip): one workgroup per frame, after MQ, before compaction.
+// Device rate control (kernels_rc.hip), after MQ, before compaction: a hull launch over
+// every block, then one workgroup per frame.
 struct RcParams {
     RcFrame* frames;
     const T1Block* blocks;
@@ -97,15 +98,25 @@ struct RcParams {
     const uint32_t* pass_len;
     const int32_t* pass_dist;
     uint32_t* code_len;         // overwritten with the chosen passes' length
-    RcPoint* hull;              // [block][kMaxPasses]
-    uint32_t* hull_n;           // [block]
-    unsigned long long* hist;   // [frame][6][kRcMaxBins + 1]
+    // hull points, point-major: point h of block b at [h * stride + b]
+    size_t stride;              // the job's block capacity
+    double* h_slope;
+    double* h_gain;
+    uint32_t* h_sel;            // passes << 24 | truncation length
+    int32_t* h_kbin;            // floor(1024 log2 slope)
+    uint32_t* hull_n;           // [block] hull points | component << 8
+    unsigned long long* part;   // [2 * rcPartEntries] slope range of each hull workgroup
+    uint32_t* hist;             // [frame][6][kRcMaxBins + 1], zero between runs
+    uint32_t* segsum;           // [rc_segsum_entries] scan segment totals
     const unsigned long long* mantissa;  // [1025] the octave's steps
     const uint16_t* first;               // [1024]
-    uint8_t* chosen;            // [block]
+    uint32_t* sel;              // [block] chosen passes << 24 | their length
     const JobCounters* counters;
 };
-hipError_t launch_rc(const RcParams& p, int frames, hipStream_t s);
+// Entries of RcParams::part for a job of `blocks` code-blocks in `frames` frames.
+inline size_t rcPartEntries(size_t blocks, size_t frames) { return blocks / 256 + frames + 2; }
+size_t rc_segsum_entries(size_t frames);
+hipError_t launch_rc(const RcParams& p, int frames, uint32_t max_frame_blocks, hipStream_t s);
 
 hipError_t launch_dwt_level(bool level0, int tile_shape, const DwtLevelDesc* descs, int ndesc, int tiles,
                             hipStream_t s);

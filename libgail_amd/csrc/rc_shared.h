@@ -11,8 +11,10 @@
 
 #if defined(__HIPCC__)
 #define GAIL_HD __host__ __device__ inline
+#define GAIL_UNROLL _Pragma("unroll")
 #else
 #define GAIL_HD inline
+#define GAIL_UNROLL _Pragma("GCC unroll 16")
 #endif
 
 namespace gail_amd {
@@ -28,8 +30,10 @@ GAIL_HD double rc_plane_scale(int nbps, int k) {
 
 // Lower convex hull of a block's (rate, distortion) points (rate_control.cpp:38-86): len and
 // dist are the block's cumulative truncation lengths and per-pass distortion reductions,
-// scale is delta^2 * weight of its subband. Writes the points, returns their number.
-GAIL_HD int rc_block_hull(int np, int nbps, double scale, const uint32_t* len, const int32_t* dist, RcPoint* out) {
+// scale is delta^2 * weight of its subband. Hands point j to out(j, point) and returns the
+// number of points; their slopes fall strictly (the first may be infinite).
+template <class Sink>
+GAIL_HD int rc_block_hull_to(int np, int nbps, double scale, const uint32_t* len, const int32_t* dist, const Sink& out) {
     uint32_t R[kMaxPasses + 1];
     double D[kMaxPasses + 1];
     int K[kMaxPasses + 1];
@@ -55,12 +59,23 @@ GAIL_HD int rc_block_hull(int np, int nbps, double scale, const uint32_t* len, c
     }
     for (int j = 1; j <= m; ++j) {
         const double dr = (double)R[j] - (double)R[j - 1];
-        out[j - 1].slope = dr <= 0 ? rc_inf() : (D[j] - D[j - 1]) / dr;
-        out[j - 1].gain = D[j] - D[j - 1];
-        out[j - 1].rate = R[j];
-        out[j - 1].passes = (uint32_t)K[j];
+        RcPoint q;
+        q.slope = dr <= 0 ? rc_inf() : (D[j] - D[j - 1]) / dr;
+        q.gain = D[j] - D[j - 1];
+        q.rate = R[j];
+        q.passes = (uint32_t)K[j];
+        out(j - 1, q);
     }
     return m;
+}
+
+struct RcPointArray {
+    RcPoint* p;
+    GAIL_HD void operator()(int j, const RcPoint& q) const { p[j] = q; }
+};
+
+GAIL_HD int rc_block_hull(int np, int nbps, double scale, const uint32_t* len, const int32_t* dist, RcPoint* out) {
+    return rc_block_hull_to(np, nbps, scale, len, dist, RcPointArray{out});
 }
 
 // Passes of a block kept at threshold lambda: up to its last hull point of slope >= lambda.
@@ -157,65 +172,6 @@ struct RcBits {
     }
 };
 
-// Tag tree (B.10.2) over at most 8 x 8 leaves.
-struct RcTagTree {
-    static constexpr int kNodes = 85;
-    int lw[5], lh[5], base[5], nlev;
-    int value[kNodes], low[kNodes];
-    uint8_t known[kNodes];
-    GAIL_HD void reset(int w, int h) {
-        nlev = 0;
-        int b = 0;
-        for (;;) {
-            lw[nlev] = w;
-            lh[nlev] = h;
-            base[nlev] = b;
-            b += w * h;
-            ++nlev;
-            if (w == 1 && h == 1) break;
-            w = (w + 1) / 2;
-            h = (h + 1) / 2;
-        }
-        for (int i = 0; i < b; ++i) {
-            value[i] = 1 << 30;
-            low[i] = 0;
-            known[i] = 0;
-        }
-    }
-    GAIL_HD int parent(int node) const {
-        for (int l = 0; l + 1 < nlev; ++l)
-            if (node < base[l + 1]) {
-                const int r = node - base[l], y = r / lw[l], x = r - y * lw[l];
-                return base[l + 1] + (y >> 1) * lw[l + 1] + (x >> 1);
-            }
-        return -1;
-    }
-    GAIL_HD void set(int leaf, int v) {
-        for (int i = leaf; i >= 0; i = parent(i)) value[i] = value[i] < v ? value[i] : v;
-    }
-    GAIL_HD void encode(RcBits& bw, int leaf, int threshold) {
-        int path[8], depth = 0;
-        for (int i = leaf; i >= 0; i = parent(i)) path[depth++] = i;
-        int lo = 0;
-        while (depth > 0) {
-            const int n = path[--depth];
-            lo = lo > low[n] ? lo : low[n];
-            while (lo < threshold) {
-                if (lo >= value[n]) {
-                    if (!known[n]) {
-                        bw.bits(1, 1);
-                        known[n] = 1;
-                    }
-                    break;
-                }
-                bw.bits(0, 1);
-                ++lo;
-            }
-            low[n] = lo;
-        }
-    }
-};
-
 GAIL_HD int rc_floor_log2(uint32_t v) { return 31 - __builtin_clz(v); }
 
 // Table B.4: codewords for the number of coding passes.
@@ -230,52 +186,116 @@ GAIL_HD void rc_pass_count(RcBits& bw, int n) {
 // Bytes of one packet (header and body) with blk.passes(i) passes of frame-local block i
 // (one layer; t2.cpp:31-115). blk gives passes(i), nbps(i) and len(i), the truncation
 // length after its passes.
+//
+// The two tag trees of a precinct-band (B.10.2) are coded without node arrays. A DCI
+// precinct-band holds at most 4 x 4 code-blocks, so a tree has at most three levels: the
+// leaves, the 2 x 2 quads and the root. With one quality layer the coded bits follow from
+// the node values and from which nodes were already visited (a bit mask per tree):
+//   inclusion (threshold 1): walking root to leaf, a node not yet visited emits 1 if some
+//     block below it is included (value 0) and 0 otherwise; the walk stops below a node of
+//     value 1;
+//   zero bit-planes (coded to the end, included blocks only): a node not yet visited emits
+//     value - parent value zeros and a 1.
+// That is what the general tag-tree coder emits for these thresholds. Everything else
+// lives in registers (the leaves of a band are unrolled), so the device sizes a packet in
+// one thread without memory traffic beyond its blocks.
 template <class Blocks>
-GAIL_HD unsigned long long rc_packet_bytes(const RcPacket& P, const RcPBand* pbands, const Blocks& blk, RcTagTree& incl,
-                                           RcTagTree& zbp) {
-    bool any = false;
-    for (int k = 0; k < P.nband && !any; ++k) {
-        const RcPBand& pb = pbands[P.first + k];
-        for (int j = 0; j < pb.ncby && !any; ++j)
-            for (int i = 0; i < pb.ncbx && !any; ++i) any = blk.passes(pb.block0 + j * pb.stride + i) != 0;
-    }
+GAIL_HD unsigned long long rc_packet_bytes(const RcPacket& P, const RcPBand* pbands, const Blocks& blk) {
     RcBits bw;
     bw.init();
-    if (!any) {
-        bw.bits(0, 1);
-        return bw.finish();
-    }
-    bw.bits(1, 1);
+    bw.bits(1, 1);  // non-empty; an empty packet is the single byte 0x00 (below)
+    bool any = false;
     unsigned long long body = 0;
     for (int k = 0; k < P.nband; ++k) {
-        const RcPBand& pb = pbands[P.first + k];
-        if (!pb.ncbx) continue;
-        incl.reset(pb.ncbx, pb.ncby);
-        zbp.reset(pb.ncbx, pb.ncby);
-        for (int j = 0; j < pb.ncby; ++j)
-            for (int i = 0; i < pb.ncbx; ++i) {
-                const uint32_t ci = pb.block0 + j * pb.stride + i;
-                incl.set(j * pb.ncbx + i, blk.passes(ci) ? 0 : 1);
-                zbp.set(j * pb.ncbx + i, (int)pb.mb - (int)blk.nbps(ci));
+        const RcPBand pb = pbands[P.first + k];
+        const int w = pb.ncbx, h = pb.ncby;
+        if (!w) continue;
+        uint32_t np[16], len[16];
+        int zv[16];
+        uint32_t incl = 0;  // included leaves, bit y * 4 + x
+GAIL_UNROLL
+        for (int q = 0; q < 16; ++q) {
+            np[q] = 0;
+            len[q] = 0;
+            zv[q] = 255;
+            if ((q & 3) < w && (q >> 2) < h) {
+                const uint32_t ci = pb.block0 + (uint32_t)(q >> 2) * pb.stride + (uint32_t)(q & 3);
+                np[q] = blk.passes(ci);
+                len[q] = blk.len(ci);
+                zv[q] = (int)pb.mb - (int)blk.nbps(ci);
+                if (np[q]) incl |= 1u << q;
             }
-        for (int j = 0; j < pb.ncby; ++j)
-            for (int i = 0; i < pb.ncbx; ++i) {
-                const int leaf = j * pb.ncbx + i;
-                const uint32_t ci = pb.block0 + j * pb.stride + i;
-                incl.encode(bw, leaf, 1);
-                const int np = (int)blk.passes(ci);
-                if (!np) continue;
-                zbp.encode(bw, leaf, 1 << 20);
-                rc_pass_count(bw, np);
-                const uint32_t len = blk.len(ci);
-                const int extra = rc_floor_log2((uint32_t)np);
-                const int need = len ? rc_floor_log2(len) + 1 : 1;
-                const int incr = need - (3 + extra) > 0 ? need - (3 + extra) : 0;
-                bw.bits(((1u << incr) - 1u) << 1, incr + 1);  // Lblock increment: incr ones, then 0
-                bw.bits(len, 3 + incr + extra);
-                body += len;
+        }
+        any = any || incl != 0;
+        int zq[4];
+GAIL_UNROLL
+        for (int m = 0; m < 4; ++m) {
+            const int q0 = (m >> 1) * 8 + (m & 1) * 2;
+            const int a = zv[q0] < zv[q0 + 1] ? zv[q0] : zv[q0 + 1];
+            const int b = zv[q0 + 4] < zv[q0 + 5] ? zv[q0 + 4] : zv[q0 + 5];
+            zq[m] = a < b ? a : b;
+        }
+        const int zr0 = zq[0] < zq[1] ? zq[0] : zq[1], zr1 = zq[2] < zq[3] ? zq[2] : zq[3];
+        const int zroot = zr0 < zr1 ? zr0 : zr1;
+        const int levels = (w == 1 && h == 1) ? 1 : ((w <= 2 && h <= 2) ? 2 : 3);
+        uint32_t ivis = 0, zvis = 0;  // visited nodes: leaves 0-15, quads 16-19, root 20
+GAIL_UNROLL
+        for (int q = 0; q < 16; ++q) {
+            if ((q & 3) >= w || (q >> 2) >= h) continue;
+            const int m = (q >> 3) * 2 + ((q >> 1) & 1);
+            // inclusion tag tree
+            bool go = true;
+            if (levels >= 2) {
+                const bool v0 = incl != 0;
+                if (!(ivis & (1u << 20))) {
+                    ivis |= 1u << 20;
+                    bw.bits(v0 ? 1u : 0u, 1);
+                }
+                go = v0;
             }
+            if (go && levels == 3) {
+                const bool v0 = (incl & (0x33u << ((m >> 1) * 8 + (m & 1) * 2))) != 0;
+                if (!(ivis & (1u << (16 + m)))) {
+                    ivis |= 1u << (16 + m);
+                    bw.bits(v0 ? 1u : 0u, 1);
+                }
+                go = v0;
+            }
+            if (go && !(ivis & (1u << q))) {
+                ivis |= 1u << q;
+                bw.bits(np[q] ? 1u : 0u, 1);
+            }
+            if (!np[q]) continue;
+            // zero bit-planes tag tree
+            int prev = 0;
+            if (levels >= 2) {
+                if (!(zvis & (1u << 20))) {
+                    zvis |= 1u << 20;
+                    bw.bits(1, zroot - prev + 1);
+                }
+                prev = zroot;
+            }
+            if (levels == 3) {
+                if (!(zvis & (1u << (16 + m)))) {
+                    zvis |= 1u << (16 + m);
+                    bw.bits(1, zq[m] - prev + 1);
+                }
+                prev = zq[m];
+            }
+            if (!(zvis & (1u << q))) {
+                zvis |= 1u << q;
+                bw.bits(1, zv[q] - prev + 1);
+            }
+            rc_pass_count(bw, (int)np[q]);
+            const int extra = rc_floor_log2(np[q]);
+            const int need = len[q] ? rc_floor_log2(len[q]) + 1 : 1;
+            const int incr = need - (3 + extra) > 0 ? need - (3 + extra) : 0;
+            bw.bits(((1u << incr) - 1u) << 1, incr + 1);  // Lblock increment: incr ones, then 0
+            bw.bits(len[q], 3 + incr + extra);
+            body += len[q];
+        }
     }
+    if (!any) return 1;
     return bw.finish() + body;
 }
 

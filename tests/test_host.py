@@ -126,11 +126,12 @@ def test_slope_grid():
         assert abs(math.log2(e) * 1024 - k) < 1e-6
 
 
-@pytest.mark.parametrize("name", ["2k_flat", "4k_scope"])
+@pytest.mark.parametrize("name", ["2k_flat", "2k_scope", "4k_scope"])
 def test_device_packet_sizes_match_tier2(oracle, name):
     """The packet-size code the GPU rate control runs (tag trees, pass counts, Lblock, byte
     stuffing of B.10) gives exactly the sizes of the host's Tier-2 builder, per component
-    and tile-part, for all passes, none, and random truncations of an oracle frame."""
+    and tile-part, for all passes, none, random truncations and sparse inclusion of an
+    oracle frame."""
     import ctypes as C
     W, H, k = g.CONTAINERS[name]
     t1 = oracle.encode_frame_xyz(g.synth_frame(12, W, H), k, thr=0.0, threads=8)
@@ -140,5 +141,5 @@ def test_device_packet_sizes_match_tier2(oracle, name):
     h.gail_rc_measure_check.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_uint64, C.c_int, C.c_uint64]
     bad = h.gail_rc_measure_check(W, H, k, 1.0, info.ctypes.data, pl.ctypes.data, pd.ctypes.data, data.ctypes.data,
-                                  len(info), 8, 5)
+                                  len(info), 12, 5)
     assert bad == 0
