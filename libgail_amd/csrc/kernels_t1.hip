@@ -78,8 +78,6 @@ __device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
 
 // Table D.1 for the four orientations, indexed [orient][h * 15 + v * 5 + d].
 __constant__ uint8_t kZcTable[4 * 45];
-// Table D.3: context | xor << 8, indexed [(hc + 1) * 3 + (vc + 1)].
-__constant__ uint16_t kScTable[9] = {13 | 256, 12 | 256, 11 | 256, 10 | 256, 9, 10, 11, 12, 13};
 
 struct Emitter {
     unsigned long long acc;
@@ -90,16 +88,29 @@ struct Emitter {
     }
 };
 
-// Sign coding of a sample that just became significant; contributions come from the
-// neighbours' significance at the time of coding (CL, CR, CU, CD) and their signs.
-__device__ __forceinline__ void code_sign(Emitter& em, int r, uint32_t CL, uint32_t CR, uint32_t CU, uint32_t CD,
-                                          uint32_t sL, uint32_t sR, uint32_t sC) {
-    int hc = (bitv(CL, r) ? (bitv(sL, r) ? -1 : 1) : 0) + (bitv(CR, r) ? (bitv(sR, r) ? -1 : 1) : 0);
-    int vc = (above(CU, r) ? (above(sC, r) ? -1 : 1) : 0) + (below(CD, r) ? (below(sC, r) ? -1 : 1) : 0);
-    hc = hc < -1 ? -1 : (hc > 1 ? 1 : hc);
-    vc = vc < -1 ? -1 : (vc > 1 ? 1 : vc);
-    const uint32_t sc = kScTable[(hc + 1) * 3 + vc + 1];
-    em.put(sc & 0xFFu, bitv(sC, r) ^ (sc >> 8));
+// Sign-coding contexts (D.3.2, Table D.3) of every row of a column at once, from the
+// neighbours' significance at the time of coding (CL, CR, CU, CD: row masks of the left,
+// right, above and below neighbours; CU / CD are read one row up / down) and the signs of
+// the left, right and own columns (sU / sD: own signs moved one row down / up). Per row,
+// with hc, vc the clamped horizontal and vertical contributions:
+//   context 9 + (c0 | c1 << 1 | c2 << 2): 13 when hc = vc != 0, 12 when hc != 0 = vc,
+//   11 when hc = -vc != 0, 10 when hc = 0 != vc, 9 when both are 0;
+//   coded bit d = sign ^ (hc < 0 or (hc = 0 and vc < 0)).
+struct SignMasks {
+    uint32_t c0, c1, c2, d;
+};
+__device__ __forceinline__ SignMasks sign_masks(uint32_t CL, uint32_t CR, uint32_t CU, uint32_t CD, uint32_t sL,
+                                                uint32_t sR, uint32_t sC, uint32_t sU, uint32_t sD) {
+    const uint32_t lp = CL & ~sL, ln = CL & sL, rp = CR & ~sR, rn = CR & sR;
+    const uint32_t hP = (lp & ~rn) | (rp & ~ln), hN = (ln & ~rp) | (rn & ~lp);
+    const uint32_t U = CU << 1, D = CD >> 1;
+    const uint32_t up = U & ~sU, un = U & sU, dp = D & ~sD, dn = D & sD;
+    const uint32_t vP = (up & ~dn) | (dp & ~un), vN = (un & ~dp) | (dn & ~up);
+    const uint32_t H = hP | hN, V = vP | vN, same = (hP & vP) | (hN & vN);
+    return SignMasks{H ^ V, H & ~same, same, sC ^ (hN | (~H & vN))};
+}
+__device__ __forceinline__ void code_sign(Emitter& em, int r, const SignMasks& m) {
+    em.put(9u + (bitv(m.c0, r) | bitv(m.c1, r) << 1 | bitv(m.c2, r) << 2), bitv(m.d, r));
 }
 
 __device__ __forceinline__ uint32_t zc_context(const uint8_t* zc, int r, uint32_t CL, uint32_t CR, uint32_t CU,
@@ -210,6 +221,7 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
         const int h_lo = __shfl(h, 0), h_hi = __shfl(h, 32);
         const int nstripes = ((h_lo > h_hi ? h_lo : h_hi) + 3) >> 2;
         const uint32_t sgnL = from_left(sgn, x), sgnR = from_right(sgn, x);
+        const uint32_t sgnU = sgn << 1, sgnD = sgn >> 1;
         const uint32_t rowsValid = h >= 32 ? 0xFFFFFFFFu : ((1u << h) - 1u);
         const float thr = fr.slope_thresh;
         const bool track = thr > 0.f;
@@ -270,6 +282,7 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                     for (int s = 0; s < nstripes; ++s) {
                         const int r0 = 4 * s;
                         const uint32_t coded = coded_all & (0xFu << r0);
+                        if (!__any(coded != 0)) continue;
                         Emitter em{0, 0};
 #pragma unroll
                         for (int k = 0; k < 4; ++k) {
@@ -287,7 +300,7 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                         for (uint32_t k = 0; k < em.n; ++k) {
                             const uint32_t sv = (uint32_t)(em.acc >> (6 * k)) & 63u;
                             sym[plane_count + excl + k] = (uint8_t)sv;
-                            if (track) hist[(sv >> 1) * 32 + x] += (sv & 1u) ? 0x10000u : 1u;
+                            if (track) atomicAdd(&hist[(sv >> 1) * 32 + x], (sv & 1u) ? 0x10000u : 1u);
                         }
                         plane_count += half ? all - lot : lot;
                     }
@@ -301,11 +314,13 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                         const uint32_t SM = (0xFu << r0) & colMask;
                         const uint32_t SL = from_left(S, x), SR = from_right(S, x);
                         Emitter em{0, 0};
+                        bool coded = false;  // wave-uniform: some lane codes in this stripe
                         if (type == 0) {
                             // ---- significance propagation (D.3.1)
                             const uint32_t cand = ~S & SM;
                             const uint32_t nS = neighbour_any(S, SL, SR);
                             if (__any((cand & nS) != 0)) {
+                                coded = true;
                                 // least fixpoint of "eligible given the news to the left and above"
                                 uint32_t NEW = 0, E = 0;
                                 for (;;) {
@@ -325,13 +340,14 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                                 }
                                 const uint32_t NL = from_left(NEW, x);
                                 const uint32_t CL = SL | NL, CR = SR, CU = S | NEW, CD = S;
+                                const SignMasks sm = sign_masks(CL, CR, CU, CD, sgnL, sgnR, sgn, sgnU, sgnD);
 #pragma unroll
                                 for (int k = 0; k < 4; ++k) {
                                     const int r = r0 + k;
                                     if (!bitv(E, r)) continue;
                                     const uint32_t b = bitv(Bp, r);
                                     em.put(zc_context(zc, r, CL, CR, CU, CD), b);
-                                    if (b) code_sign(em, r, CL, CR, CU, CD, sgnL, sgnR, sgn);
+                                    if (b) code_sign(em, r, sm);
                                 }
                                 S |= NEW;
                                 V |= E;
@@ -341,9 +357,11 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                             // ---- cleanup with run-length mode (D.3.4)
                             const uint32_t Cc = SM & ~S & ~V;
                             if (__any(Cc != 0)) {
+                                coded = true;
                                 const uint32_t NEWc = Cc & Bp;
                                 const uint32_t NLc = from_left(NEWc, x);
                                 const uint32_t CL = SL | NLc, CR = SR, CU = S | NEWc, CD = S;
+                                const SignMasks sm = sign_masks(CL, CR, CU, CD, sgnL, sgnR, sgn, sgnU, sgnD);
                                 bool run = ((Cc >> r0) & 0xFu) == 0xFu && r0 + 3 < h;
                                 if (run) {
                                     // no significant neighbour in the 3 x 6 window around the column's stripe
@@ -367,12 +385,13 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                                     if (!bitv(Cc, r)) continue;
                                     const uint32_t b = bitv(Bp, r);
                                     if (!(run && k == first)) em.put(zc_context(zc, r, CL, CR, CU, CD), b);
-                                    if (b) code_sign(em, r, CL, CR, CU, CD, sgnL, sgnR, sgn);
+                                    if (b) code_sign(em, r, sm);
                                 }
                                 S |= NEWc;
                                 new_sig |= NEWc;
                             }
                         }
+                        if (!coded) continue;
                         // the stripe's symbols, in column order, after the plane's earlier ones
                         const unsigned long long b0 = __ballot(em.n & 1u), b1 = __ballot(em.n & 2u);
                         const unsigned long long b2 = __ballot(em.n & 4u), b3 = __ballot(em.n & 8u);
@@ -384,7 +403,7 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                         for (uint32_t k = 0; k < em.n; ++k) {
                             const uint32_t sv = (uint32_t)(em.acc >> (6 * k)) & 63u;
                             sym[plane_count + excl + k] = (uint8_t)sv;
-                            if (track) hist[(sv >> 1) * 32 + x] += (sv & 1u) ? 0x10000u : 1u;
+                            if (track) atomicAdd(&hist[(sv >> 1) * 32 + x], (sv & 1u) ? 0x10000u : 1u);
                         }
                         plane_count += half ? all - lot : lot;
                     }
