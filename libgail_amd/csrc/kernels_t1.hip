@@ -836,56 +836,60 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
 #pragma unroll
                 for (int e = 0; e < NQ; ++e) n[e] = nsrc ? *(const uint4*)(nsrc + 16 * e) : make_uint4(0u, 0u, 0u, 0u);
                 const uint32_t mg = ch.y - kGroupBytes * g < kGroupBytes ? ch.y - kGroupBytes * g : kGroupBytes;
-#pragma unroll
+                // one copy of the 16 unrolled symbol steps: the group's 16-byte words move
+                // to the front of q in turn
+#pragma unroll 1
                 for (int sub = 0; sub < NQ; ++sub) {
                     if (16u * sub < mg) {
                         const uint32_t m = mg - 16u * sub < 16u ? mg - 16u * sub : 16u;
                         // the symbol after this run of m: in the group, or the first of the next
-                        const uint32_t after = 16u * sub + 16u < mg ? (q[sub + 1 < NQ ? sub + 1 : 0].x & 0xFFu)
-                                                                   : (n[0].x & 0xFFu);
-                        const uint4 cur = q[sub];
+                        const uint32_t after = 16u * sub + 16u < mg ? (q[NQ > 1 ? 1 : 0].x & 0xFFu) : (n[0].x & 0xFFu);
+                        const uint4 cur = q[0];
                         const uint32_t w[5] = {cur.x, cur.y, cur.z, cur.w, after};
 #pragma unroll
                         for (uint32_t b = 0; b < 16; ++b) {
-                            if (b >= m) break;
-                            // pass ends: the coder state is the truncation reference of the pass
-                            while (pos == next_end) {
-                                snaps[k] = MqSnapshot{bp, B | ((uint32_t)CT << 16), C, A};
-                                ++k;
-                                next_end = k < np ? pend[k] : 0xFFFFFFFFu;
-                            }
-                            const uint32_t sv = (w[b >> 2] >> (8 * (b & 3))) & 0xFFu;
-                            const uint32_t sn = b + 1 < m ? (w[(b + 1) >> 2] >> (8 * ((b + 1) & 3))) & 0xFFu : w[4];
-                            const uint32_t cx = sv >> 1, d = sv & 1u, cxn = sn >> 1;
-                            // the next symbol's context state, read before this symbol may change it
-                            const uint32_t t_next = s_ctx[cxn][t];
-                            const uint32_t st = t_cur;
-                            const uint32_t qe = st & 0xFFFFu, mps = st >> 31;
-                            const uint32_t a = A - qe;
-                            const bool is_mps = d == mps;
-                            const bool add_c = is_mps != (a < qe);  // conditional exchange (C.2.5 / C.2.6)
-                            C += add_c ? qe : 0u;
-                            A = add_c ? a : qe;
-                            uint32_t st_new = st;
-                            if (!is_mps || !(a & 0x8000u)) {
-                                const uint32_t nidx = is_mps ? ((st >> 16) & 63u) : ((st >> 22) & 63u);
-                                const uint32_t nmps = is_mps ? mps : (mps ^ ((st >> 28) & 1u));
-                                st_new = s_tab[nidx] | (nmps << 31);
-                                s_ctx[cx][t] = st_new;
-                                int nsh = __clz((int)A) - 16;  // RENORME: shifts until A >= 0x8000
-                                while (nsh > 0) {
-                                    const int sh = nsh < CT ? nsh : CT;
-                                    A <<= sh;
-                                    C <<= sh;
-                                    CT -= sh;
-                                    nsh -= sh;
-                                    if (CT == 0) byteout();
+                            if (b < m) {  // a guard, not an exit, so the 16 steps unroll
+                                // pass ends: the coder state is the truncation reference of the pass
+                                while (pos == next_end) {
+                                    snaps[k] = MqSnapshot{bp, B | ((uint32_t)CT << 16), C, A};
+                                    ++k;
+                                    next_end = k < np ? pend[k] : 0xFFFFFFFFu;
                                 }
+                                const uint32_t sv = (w[b >> 2] >> (8 * (b & 3))) & 0xFFu;
+                                const uint32_t sn = b + 1 < m ? (w[(b + 1) >> 2] >> (8 * ((b + 1) & 3))) & 0xFFu : w[4];
+                                const uint32_t cx = sv >> 1, d = sv & 1u, cxn = sn >> 1;
+                                // the next symbol's context state, read before this symbol may change it
+                                const uint32_t t_next = s_ctx[cxn][t];
+                                const uint32_t st = t_cur;
+                                const uint32_t qe = st & 0xFFFFu, mps = st >> 31;
+                                const uint32_t a = A - qe;
+                                const bool is_mps = d == mps;
+                                const bool add_c = is_mps != (a < qe);  // conditional exchange (C.2.5 / C.2.6)
+                                C += add_c ? qe : 0u;
+                                A = add_c ? a : qe;
+                                uint32_t st_new = st;
+                                if (!is_mps || !(a & 0x8000u)) {
+                                    const uint32_t nidx = is_mps ? ((st >> 16) & 63u) : ((st >> 22) & 63u);
+                                    const uint32_t nmps = is_mps ? mps : (mps ^ ((st >> 28) & 1u));
+                                    st_new = s_tab[nidx] | (nmps << 31);
+                                    s_ctx[cx][t] = st_new;
+                                    int nsh = __clz((int)A) - 16;  // RENORME: shifts until A >= 0x8000
+                                    while (nsh > 0) {
+                                        const int sh = nsh < CT ? nsh : CT;
+                                        A <<= sh;
+                                        C <<= sh;
+                                        CT -= sh;
+                                        nsh -= sh;
+                                        if (CT == 0) byteout();
+                                    }
+                                }
+                                t_cur = cxn == cx ? st_new : t_next;
+                                ++pos;
                             }
-                            t_cur = cxn == cx ? st_new : t_next;
-                            ++pos;
                         }
                     }
+#pragma unroll
+                    for (int e = 0; e + 1 < NQ; ++e) q[e] = q[e + 1];
                 }
 #pragma unroll
                 for (int e = 0; e < NQ; ++e) q[e] = n[e];
