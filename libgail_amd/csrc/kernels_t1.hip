@@ -759,7 +759,7 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
         const unsigned long long obase = P.out_off[cb];
         uint8_t* out = P.out + obase;
         uint32_t cap = out_capacity(inf);  // a multiple of 64, as every block's offset
-        bool overflow = obase + cap > P.out_cap;
+        uint32_t overflow = obase + cap > P.out_cap ? 1u : 0u;  // a word, not a lane mask, in the hot loop
         if (overflow) cap = 0;
         for (int i = 0; i < 19; ++i) s_ctx[i][t] = s_tab[0];
         s_ctx[0][t] = s_tab[4];
@@ -771,43 +771,33 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
         uint4 ob = make_uint4(0u, 0u, 0u, 0u);  // finished words of the current 16 bytes
         uint32_t flushed = 0;           // codeword bytes already written out (a multiple of 16)
         // BYTEOUT (C.2.7) with the pending byte B kept in a register: it is final once the
-        // next byte starts. Finished bytes go out 16 at a time.
+        // next byte starts. Branch-free but for the 16-byte flush: every lane of the wave
+        // runs it whenever one lane needs it. Finished bytes go out 16 at a time.
         auto byteout = [&]() {
-            bool stuff = B == 0xFFu;
-            if (!stuff && (C & 0x8000000u)) {
-                B += 1u;
-                if (B == 0xFFu) {
-                    C &= 0x7FFFFFFu;
-                    stuff = true;
-                }
-            }
-            if (bp >= 0) {
-                word |= B << (8 * (bp & 3));
-                if ((bp & 3) == 3) {
-                    const int wi = (bp >> 2) & 3;
-                    ob.x = wi == 0 ? word : ob.x;
-                    ob.y = wi == 1 ? word : ob.y;
-                    ob.z = wi == 2 ? word : ob.z;
-                    ob.w = wi == 3 ? word : ob.w;
-                    word = 0;
-                    if (wi == 3) {
-                        const uint32_t at = (uint32_t)bp & ~15u;
-                        if (at + 16u <= cap) *(uint4*)(out + at) = ob;
-                        else overflow = true;
-                        flushed = at + 16u;
-                    }
-                }
+            // a carry into B unless B is 0xFF; a B of 0xFF (before or after the carry) is
+            // followed by a stuffed byte of 7 bits
+            const uint32_t carry = B != 0xFFu ? (C >> 27) & 1u : 0u;
+            B += carry;
+            const bool stuff = B == 0xFFu;
+            C = carry && stuff ? C & 0x7FFFFFFu : C;
+            word |= (bp >= 0 ? B : 0u) << (8 * (bp & 3));
+            const bool full = bp >= 0 && (bp & 3) == 3;
+            const int wi = (bp >> 2) & 3;
+            ob.x = full && wi == 0 ? word : ob.x;
+            ob.y = full && wi == 1 ? word : ob.y;
+            ob.z = full && wi == 2 ? word : ob.z;
+            ob.w = full && wi == 3 ? word : ob.w;
+            word = full ? 0u : word;
+            if (full && wi == 3) {
+                const uint32_t at = (uint32_t)bp & ~15u;
+                if (at + 16u <= cap) *(uint4*)(out + at) = ob;
+                else overflow = 1u;
+                flushed = at + 16u;
             }
             ++bp;
-            if (stuff) {
-                B = C >> 20;
-                C &= 0xFFFFFu;
-                CT = 7;
-            } else {
-                B = (C >> 19) & 0xFFu;
-                C &= 0x7FFFFu;
-                CT = 8;
-            }
+            B = stuff ? C >> 20 : (C >> 19) & 0xFFu;
+            C &= stuff ? 0xFFFFFu : 0x7FFFFu;
+            CT = stuff ? 7 : 8;
         };
         const uint32_t* pend = P.pass_end + (size_t)cb * kMaxPasses;
         MqSnapshot* snaps = P.snaps + (size_t)cb * kMaxPasses;
@@ -873,10 +863,17 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
                                     const uint32_t nmps = is_mps ? mps : (mps ^ ((st >> 28) & 1u));
                                     st_new = s_tab[nidx] | (nmps << 31);
                                     s_ctx[cx][t] = st_new;
-                                    int nsh = __clz((int)A) - 16;  // RENORME: shifts until A >= 0x8000
+                                    // RENORME: A and C shift until A >= 0x8000 (at most 15 bits), a
+                                    // byte out whenever CT runs out; a second one is rare
+                                    int nsh = __clz((int)A) - 16;
+                                    A <<= nsh;
+                                    int sh = nsh < CT ? nsh : CT;
+                                    C <<= sh;
+                                    CT -= sh;
+                                    nsh -= sh;
+                                    if (CT == 0) byteout();
                                     while (nsh > 0) {
-                                        const int sh = nsh < CT ? nsh : CT;
-                                        A <<= sh;
+                                        sh = nsh < CT ? nsh : CT;
                                         C <<= sh;
                                         CT -= sh;
                                         nsh -= sh;
@@ -917,7 +914,7 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
         }
         // the words after the last full 16 bytes: the finished ones in ob, then the partial one
         if (len > cap) {
-            overflow = true;
+            overflow = 1u;
         } else {
             const uint32_t nw = (len + 3u) >> 2;
             for (uint32_t wi = flushed >> 2; wi < nw; ++wi) {
