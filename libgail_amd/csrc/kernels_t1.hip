@@ -18,6 +18,7 @@
 // flush it derives every pass's truncation length from the snapshots.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "device_types.h"
@@ -30,8 +31,8 @@ namespace {
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ uint32_t bitv(uint32_t m, int r) { return (m >> r) & 1u; }
 // Bit r-1 and r+1 of m, zero outside 0..31.
-__device__ __forceinline__ uint32_t above(uint32_t m, int r) { return r > 0 ? (m >> (r - 1)) & 1u : 0u; }
-__device__ __forceinline__ uint32_t below(uint32_t m, int r) { return r < 31 ? (m >> (r + 1)) & 1u : 0u; }
+__device__ __forceinline__ uint32_t above(uint32_t m, int r) { return ((m << 1) >> r) & 1u; }
+__device__ __forceinline__ uint32_t below(uint32_t m, int r) { return ((m >> 1) >> r) & 1u; }
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -63,6 +64,11 @@ __device__ __forceinline__ int half_sum_i(int v) {
     for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
     return v;
 }
+__device__ __forceinline__ double half_sum_d(double v) {
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+    return v;
+}
 __device__ __forceinline__ uint32_t half_max_u(uint32_t v) {
 #pragma unroll
     for (int o = 16; o > 0; o >>= 1) {
@@ -76,14 +82,16 @@ __device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Table D.1 for the four orientations, indexed [orient][h * 15 + v * 5 + d].
-__constant__ uint8_t kZcTable[4 * 45];
+// Table D.1 for the four orientations, indexed [orient][code] by the 7-bit neighbourhood
+// code of zc_codes (h = h1 + 2 h2, v = v1 + 2 v2, d = d0 + 2 d1 + 4 d2).
+__constant__ uint8_t kZcTable[4 * 128];
 
 struct Emitter {
     unsigned long long acc;
     uint32_t n;
-    __device__ __forceinline__ void put(uint32_t cx, uint32_t d) {
-        acc |= (unsigned long long)((cx << 1) | d) << (6 * n);
+    __device__ __forceinline__ void put(uint32_t cx, uint32_t d) { put_sym((cx << 1) | d); }
+    __device__ __forceinline__ void put_sym(uint32_t sym) {
+        acc |= (unsigned long long)sym << (6 * n);
         ++n;
     }
 };
@@ -109,16 +117,25 @@ __device__ __forceinline__ SignMasks sign_masks(uint32_t CL, uint32_t CR, uint32
     const uint32_t H = hP | hN, V = vP | vN, same = (hP & vP) | (hN & vN);
     return SignMasks{H ^ V, H & ~same, same, sC ^ (hN | (~H & vN))};
 }
-__device__ __forceinline__ void code_sign(Emitter& em, int r, const SignMasks& m) {
-    em.put(9u + (bitv(m.c0, r) | bitv(m.c1, r) << 1 | bitv(m.c2, r) << 2), bitv(m.d, r));
+
+// Rows r0..r0+3 of mask m moved to bits 0, 8, 16 and 24 (one byte per row of a stripe).
+__device__ __forceinline__ uint32_t spread4(uint32_t m, int r0) { return (((m >> r0) & 0xFu) * 0x204081u) & 0x01010101u; }
+
+// Zero-coding neighbourhoods (D.3.1) of the four rows of stripe r0, one 7-bit code per byte:
+// h1 | h2 << 1 (horizontal count h), v1 | v2 << 2 (vertical v), d0 | d1 << 4 | d2 << 5
+// (diagonal d), each a bit-sliced sum over whole row masks.
+__device__ __forceinline__ uint32_t zc_codes(uint32_t CL, uint32_t CR, uint32_t CU, uint32_t CD, int r0) {
+    const uint32_t U = CU << 1, D = CD >> 1;
+    const uint32_t a = CL << 1, b = CL >> 1, c = CR << 1, e = CR >> 1;
+    const uint32_t s1 = a ^ b, c1 = a & b, s2 = c ^ e, c2 = c & e;  // d = s1 + s2 + 2 (c1 + c2)
+    return spread4(CL ^ CR, r0) | spread4(CL & CR, r0) << 1 | spread4(U ^ D, r0) << 2 | spread4(U & D, r0) << 3 |
+           spread4(s1 ^ s2, r0) << 4 | spread4(c1 ^ c2 ^ (s1 & s2), r0) << 5 | spread4(c1 & c2, r0) << 6;
 }
 
-__device__ __forceinline__ uint32_t zc_context(const uint8_t* zc, int r, uint32_t CL, uint32_t CR, uint32_t CU,
-                                               uint32_t CD) {
-    const uint32_t h = bitv(CL, r) + bitv(CR, r);
-    const uint32_t v = above(CU, r) + below(CD, r);
-    const uint32_t d = above(CL, r) + below(CL, r) + above(CR, r) + below(CR, r);
-    return zc[h * 15 + v * 5 + d];
+// Sign-coding symbols of the four rows of stripe r0, one per byte: 18 + the byte is the
+// symbol (context 9 + c, bit d).
+__device__ __forceinline__ uint32_t sign_codes(const SignMasks& m, int r0) {
+    return spread4(m.d, r0) | spread4(m.c0, r0) << 1 | spread4(m.c1, r0) << 2 | spread4(m.c2, r0) << 3;
 }
 
 constexpr int kChunkBytes = 65536;  // symbol pool grabbed per wave at a time
@@ -173,8 +190,8 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
     // Early-termination counts per context (d = 0 low half, d = 1 high half): one column per
     // lane, so every lane updates its own words without atomics; summed at the plane end.
     __shared__ uint32_t s_hist[WAVES][2][20 * 32];
-    __shared__ uint8_t s_zc[4 * 45];
-    for (int i = threadIdx.x; i < 4 * 45; i += WAVES * 64) s_zc[i] = kZcTable[i];
+    __shared__ uint8_t s_zc[4 * 128];
+    for (int i = threadIdx.x; i < 4 * 128; i += WAVES * 64) s_zc[i] = kZcTable[i];
     __syncthreads();
 
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, half = lane >> 5, x = lane & 31;
@@ -194,7 +211,7 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
         const int w = has ? blk.w : 0, h = has ? blk.h : 0;
         const T1Band band = P.bands[has ? blk.band : 0];
         const T1FrameRef fr = P.frames[has ? blk.frame : 0];
-        const uint8_t* zc = s_zc + 45 * (has ? blk.orient : 0);
+        const uint8_t* zc = s_zc + 128 * (has ? blk.orient : 0);
 
         // Fused dead-zone quantisation of the block's column (quant.h:20-25).
         uint32_t m[32];
@@ -340,14 +357,15 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                                 }
                                 const uint32_t NL = from_left(NEW, x);
                                 const uint32_t CL = SL | NL, CR = SR, CU = S | NEW, CD = S;
-                                const SignMasks sm = sign_masks(CL, CR, CU, CD, sgnL, sgnR, sgn, sgnU, sgnD);
+                                const uint32_t Z = zc_codes(CL, CR, CU, CD, r0);
+                                const uint32_t G = sign_codes(sign_masks(CL, CR, CU, CD, sgnL, sgnR, sgn, sgnU, sgnD), r0);
 #pragma unroll
                                 for (int k = 0; k < 4; ++k) {
                                     const int r = r0 + k;
                                     if (!bitv(E, r)) continue;
                                     const uint32_t b = bitv(Bp, r);
-                                    em.put(zc_context(zc, r, CL, CR, CU, CD), b);
-                                    if (b) code_sign(em, r, sm);
+                                    em.put(zc[(Z >> (8 * k)) & 0x7Fu], b);
+                                    if (b) em.put_sym(18u + ((G >> (8 * k)) & 0xFu));
                                 }
                                 S |= NEW;
                                 V |= E;
@@ -361,7 +379,8 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                                 const uint32_t NEWc = Cc & Bp;
                                 const uint32_t NLc = from_left(NEWc, x);
                                 const uint32_t CL = SL | NLc, CR = SR, CU = S | NEWc, CD = S;
-                                const SignMasks sm = sign_masks(CL, CR, CU, CD, sgnL, sgnR, sgn, sgnU, sgnD);
+                                const uint32_t Z = zc_codes(CL, CR, CU, CD, r0);
+                                const uint32_t G = sign_codes(sign_masks(CL, CR, CU, CD, sgnL, sgnR, sgn, sgnU, sgnD), r0);
                                 bool run = ((Cc >> r0) & 0xFu) == 0xFu && r0 + 3 < h;
                                 if (run) {
                                     // no significant neighbour in the 3 x 6 window around the column's stripe
@@ -384,8 +403,8 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                                     if (run && k < first) continue;
                                     if (!bitv(Cc, r)) continue;
                                     const uint32_t b = bitv(Bp, r);
-                                    if (!(run && k == first)) em.put(zc_context(zc, r, CL, CR, CU, CD), b);
-                                    if (b) code_sign(em, r, sm);
+                                    if (!(run && k == first)) em.put(zc[(Z >> (8 * k)) & 0x7Fu], b);
+                                    if (b) em.put_sym(18u + ((G >> (8 * k)) & 0xFu));
                                 }
                                 S |= NEWc;
                                 new_sig |= NEWc;
@@ -476,32 +495,54 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                 if (cand) dabs = (float)plane_dist * ldexpf(1.0f, 2 * p) * (1.f / 4096.f) * band.dist_scale;
                 cand = cand && dabs < thr * bytes;
                 if (__any(cand)) {
-                    // remaining squared error, summed in the CPU's row-major order
+                    // Remaining squared error (t1_ref.cpp:142-154): the CPU sums the samples'
+                    // float terms in row-major order. That float sum of at most 1,024
+                    // non-negative terms is within 1023 * 2^-24 (< 6.2e-5, relative) of their
+                    // exact sum; the half-wave's sum (float column sums, within 31 * 2^-24,
+                    // then a double shuffle tree) is within 2e-6 of it. Unless that sum lies
+                    // within 1e-4 of the threshold it decides the comparison the same way;
+                    // only then is the row-major float sum formed.
                     const int pp = act ? p : 0;
                     const float step = ldexpf(1.0f, pp);
-                    float e = 0.f;
-                    float* scratch = (float*)sym;
+                    auto term = [&](uint32_t mm) {
+                        const float v = (float)mm * (1.f / 64.f);
+                        const uint32_t q = mm >> (pp + 6);
+                        const float rec = q == 0 ? 0.f : ((float)q + 0.5f) * step;
+                        const float dd = v - rec;
+                        return dd * dd;
+                    };
+                    float col = 0.f;
 #pragma unroll
-                    for (int part = 0; part < 2; ++part) {
+                    for (int r = 0; r < 32; ++r)
+                        if (x < w && r < h) col += term(m[r]);
+                    const double lhs = half_sum_d((double)col) * (double)band.dist_scale;
+                    const double T = (double)(thr * bytes);
+                    const bool low = lhs * (1.0 + 1e-4) < T, high = lhs * (1.0 - 1e-4) >= T;
+                    const bool unsure = cand && !low && !high;
+                    bool stop = cand && low;
+                    if (__any(unsure)) {
+                        float e = 0.f;
+                        float* scratch = (float*)sym;
 #pragma unroll
-                        for (int rr = 0; rr < 16; ++rr) {
-                            const uint32_t mm = m[part * 16 + rr];
-                            const float v = (float)mm * (1.f / 64.f);
-                            const uint32_t q = mm >> (pp + 6);
-                            const float rec = q == 0 ? 0.f : ((float)q + 0.5f) * step;
-                            const float dd = v - rec;
-                            scratch[rr * 32 + x] = dd * dd;
+                        for (int part = 0; part < 2; ++part) {
+#pragma unroll
+                            for (int rr = 0; rr < 16; ++rr) {
+                                uint32_t mm = m[part * 16 + rr];
+                                asm volatile("" : "+v"(mm));  // recomputed here, not kept from the fast path
+                                scratch[rr * 32 + x] = term(mm);
+                            }
+                            wave_sync();
+                            if (x == 0 && unsure) {
+                                const int r1 = h < part * 16 + 16 ? h : part * 16 + 16;
+                                for (int r = part * 16; r < r1; ++r)
+                                    for (int c = 0; c < w; ++c) e += scratch[(r - part * 16) * 32 + c];
+                            }
+                            wave_sync();
                         }
-                        wave_sync();
-                        if (x == 0 && cand) {
-                            const int r1 = h < part * 16 + 16 ? h : part * 16 + 16;
-                            for (int r = part * 16; r < r1; ++r)
-                                for (int c = 0; c < w; ++c) e += scratch[(r - part * 16) * 32 + c];
-                        }
-                        wave_sync();
+                        e = __shfl(e, half << 5);
+                        if (unsure) stop = e * band.dist_scale < thr * bytes;
                     }
-                    e = __shfl(e, half << 5);
-                    stop_now = cand && e * band.dist_scale < thr * bytes;
+                    stop_now = stop;
                 }
             }
             if (act && (stop_now || np >= P.max_passes)) done = true;
@@ -520,28 +561,34 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
 }
 
 static hipError_t upload_zc_table() {
-    {
-        // Table D.1 (ISO/IEC 15444-1) for LL/LH, HL (h and v swapped) and HH.
-        uint8_t t[4 * 45];
-        for (int o = 0; o < 4; ++o)
-            for (int h = 0; h < 3; ++h)
-                for (int v = 0; v < 3; ++v)
-                    for (int d = 0; d < 5; ++d) {
-                        int hh = h, vv = v, c;
-                        if (o == 1) {
-                            hh = v;
-                            vv = h;
-                        }
-                        if (o == 3) {
-                            const int hv = hh + vv;
-                            c = d >= 3 ? 8 : d == 2 ? (hv ? 7 : 6) : d == 1 ? (hv >= 2 ? 5 : hv ? 4 : 3) : (hv >= 2 ? 2 : hv);
-                        } else {
-                            c = hh == 2 ? 8 : hh == 1 ? (vv ? 7 : d ? 6 : 5) : vv ? (vv == 2 ? 4 : 3) : (d >= 2 ? 2 : d);
-                        }
-                        t[o * 45 + h * 15 + v * 5 + d] = (uint8_t)c;
+    // Table D.1 (ISO/IEC 15444-1) for LL/LH, HL (h and v swapped) and HH by (h, v, d) ...
+    uint8_t t[4 * 45];
+    for (int o = 0; o < 4; ++o)
+        for (int h = 0; h < 3; ++h)
+            for (int v = 0; v < 3; ++v)
+                for (int d = 0; d < 5; ++d) {
+                    int hh = h, vv = v, c;
+                    if (o == 1) {
+                        hh = v;
+                        vv = h;
                     }
-        return hipMemcpyToSymbol(HIP_SYMBOL(kZcTable), t, sizeof t);
-    }
+                    if (o == 3) {
+                        const int hv = hh + vv;
+                        c = d >= 3 ? 8 : d == 2 ? (hv ? 7 : 6) : d == 1 ? (hv >= 2 ? 5 : hv ? 4 : 3) : (hv >= 2 ? 2 : hv);
+                    } else {
+                        c = hh == 2 ? 8 : hh == 1 ? (vv ? 7 : d ? 6 : 5) : vv ? (vv == 2 ? 4 : 3) : (d >= 2 ? 2 : d);
+                    }
+                    t[o * 45 + h * 15 + v * 5 + d] = (uint8_t)c;
+                }
+    // ... and by the 7-bit code of zc_codes (codes no neighbourhood produces clamp)
+    uint8_t z[4 * 128];
+    for (int o = 0; o < 4; ++o)
+        for (int i = 0; i < 128; ++i) {
+            const int h = std::min(2, (i & 1) + 2 * ((i >> 1) & 1)), v = std::min(2, ((i >> 2) & 1) + 2 * ((i >> 3) & 1));
+            const int d = std::min(4, ((i >> 4) & 1) + 2 * ((i >> 5) & 1) + 4 * ((i >> 6) & 1));
+            z[o * 128 + i] = t[o * 45 + h * 15 + v * 5 + d];
+        }
+    return hipMemcpyToSymbol(HIP_SYMBOL(kZcTable), z, sizeof z);
 }
 
 hipError_t launch_t1_cm(const CmParams& p, int waves, int groups, hipStream_t s) {
