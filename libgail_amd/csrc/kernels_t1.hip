@@ -188,7 +188,7 @@ template <int WAVES>
 __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
     __shared__ __attribute__((aligned(16))) uint8_t s_sym[WAVES][2][kPlaneSymCap];
     // Early-termination counts per context (d = 0 low half, d = 1 high half): one column per
-    // lane, so every lane updates its own words without atomics; summed at the plane end.
+    // lane, so lanes never add to the same word; the columns are summed at the plane end.
     __shared__ uint32_t s_hist[WAVES][2][20 * 32];
     __shared__ uint8_t s_zc[4 * 128];
     for (int i = threadIdx.x; i < 4 * 128; i += WAVES * 64) s_zc[i] = kZcTable[i];
@@ -317,7 +317,6 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                         for (uint32_t k = 0; k < em.n; ++k) {
                             const uint32_t sv = (uint32_t)(em.acc >> (6 * k)) & 63u;
                             sym[plane_count + excl + k] = (uint8_t)sv;
-                            if (track) atomicAdd(&hist[(sv >> 1) * 32 + x], (sv & 1u) ? 0x10000u : 1u);
                         }
                         plane_count += half ? all - lot : lot;
                     }
@@ -422,7 +421,6 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                         for (uint32_t k = 0; k < em.n; ++k) {
                             const uint32_t sv = (uint32_t)(em.acc >> (6 * k)) & 63u;
                             sym[plane_count + excl + k] = (uint8_t)sv;
-                            if (track) atomicAdd(&hist[(sv >> 1) * 32 + x], (sv & 1u) ? 0x10000u : 1u);
                         }
                         plane_count += half ? all - lot : lot;
                     }
@@ -441,6 +439,13 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
             }
             // ---- the plane's symbols to the pool (wave-uniform sub-allocation)
             wave_sync();
+            // early-termination counts of the plane's symbols, 32 lanes per block at a time
+            // (not one lane per column inside the divergent emission loops)
+            if (track && act)
+                for (uint32_t i = (uint32_t)x; i < plane_count; i += 32u) {
+                    const uint32_t sv = sym[i];
+                    atomicAdd(&hist[(sv >> 1) * 32 + x], (sv & 1u) ? 0x10000u : 1u);
+                }
             const uint32_t cnt_lo = __shfl((int)plane_count, 0), cnt_hi = __shfl((int)plane_count, 32);
             // 64-byte aligned: the MQ coder reads a plane in 64-byte groups
             const uint32_t al_lo = (cnt_lo + 63u) & ~63u, al_hi = (cnt_hi + 63u) & ~63u;
