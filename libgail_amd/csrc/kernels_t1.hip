@@ -187,9 +187,10 @@ hipError_t launch_quant_debug(const T1Block* blocks, const T1Band* bands, const 
 template <int WAVES>
 __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
     __shared__ __attribute__((aligned(16))) uint8_t s_sym[WAVES][2][kPlaneSymCap];
-    // Early-termination counts per context (d = 0 low half, d = 1 high half): one column per
-    // lane, so lanes never add to the same word; the columns are summed at the plane end.
-    __shared__ uint32_t s_hist[WAVES][2][20 * 32];
+    // Early-termination counts per context of the current plane (d = 0 low half, d = 1 high
+    // half) and the contexts' entropy terms.
+    __shared__ uint32_t s_hist[WAVES][2][32];
+    __shared__ __attribute__((aligned(16))) float s_term[WAVES][2][20];
     __shared__ uint8_t s_zc[4 * 128];
     for (int i = threadIdx.x; i < 4 * 128; i += WAVES * 64) s_zc[i] = kZcTable[i];
     __syncthreads();
@@ -197,6 +198,7 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, half = lane >> 5, x = lane & 31;
     uint8_t* const sym = s_sym[wv][half];
     uint32_t* const hist = s_hist[wv][half];
+    float* const terms = s_term[wv][half];
     unsigned long long chunk_cur = 0, chunk_end = 0;  // wave-uniform sub-allocator
 
     for (;;) {
@@ -217,16 +219,18 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
         uint32_t m[32];
         uint32_t sgn = 0, colmax = 0;
         {
-            const float* src = fr.coef + blk.coef_off + x;
+            // all 32 loads in flight at once: rows and columns outside the block read the
+            // block's first sample (a valid address) and are discarded
+            const float* src = fr.coef + blk.coef_off + (x < w ? x : 0);
+            float v[32];
+#pragma unroll
+            for (int r = 0; r < 32; ++r) v[r] = src[(size_t)(r < h ? r : 0) * fr.stride];
 #pragma unroll
             for (int r = 0; r < 32; ++r) {
-                uint32_t q = 0;
-                if (x < w && r < h) {
-                    const float v = src[(size_t)r * fr.stride];
-                    const float a = (v < 0.f ? -v : v) * band.inv_delta * 64.0f;
-                    q = a >= (float)band.max_mag ? band.max_mag : (uint32_t)a;
-                    sgn |= (v < 0.f && q >= 64u ? 1u : 0u) << r;
-                }
+                const float a = (v[r] < 0.f ? -v[r] : v[r]) * band.inv_delta * 64.0f;
+                uint32_t q = a >= (float)band.max_mag ? band.max_mag : (uint32_t)a;
+                q = x < w && r < h ? q : 0u;
+                sgn |= (v[r] < 0.f && q >= 64u ? 1u : 0u) << r;
                 m[r] = q;
                 colmax = q > colmax ? q : colmax;
             }
@@ -242,8 +246,7 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
         const uint32_t rowsValid = h >= 32 ? 0xFFFFFFFFu : ((1u << h) - 1u);
         const float thr = fr.slope_thresh;
         const bool track = thr > 0.f;
-#pragma unroll 4
-        for (int c = 0; c < 20; ++c) hist[c * 32 + x] = 0;
+        if (x < 20) hist[x] = 0;
 
         // Bit-plane masks of the column: pl[k] holds bit (p + k) of every row's magnitude
         // (bits p..p+6: the current magnitude bit is pl[6], the six below it feed the
@@ -444,7 +447,7 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
             if (track && act)
                 for (uint32_t i = (uint32_t)x; i < plane_count; i += 32u) {
                     const uint32_t sv = sym[i];
-                    atomicAdd(&hist[(sv >> 1) * 32 + x], (sv & 1u) ? 0x10000u : 1u);
+                    atomicAdd(&hist[sv >> 1], (sv & 1u) ? 0x10000u : 1u);
                 }
             const uint32_t cnt_lo = __shfl((int)plane_count, 0), cnt_hi = __shfl((int)plane_count, 32);
             // 64-byte aligned: the MQ coder reads a plane in 64-byte groups
@@ -476,9 +479,7 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
             if (__any(track)) {
                 float term = 0.f;
                 if (track && x < 19) {
-                    uint32_t hv = 0;  // the context's counts over the half-wave's 32 columns
-#pragma unroll 4
-                    for (int l = 0; l < 32; ++l) hv += hist[x * 32 + ((l + x) & 31)];
+                    const uint32_t hv = hist[x];
                     const uint32_t n0 = hv & 0xFFFFu, n1 = hv >> 16, n = n0 + n1;
                     if (n > 0 && n <= kLog2Max) {
                         const float* row = P.log2tab + (size_t)n * (n + 1) / 2;
@@ -487,13 +488,22 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                         term = -(a + b) + 2.f;
                     }
                 }
+                if (x < 20) terms[x] = term;
                 wave_sync();
-                if (track) {
-#pragma unroll 4
-                    for (int c = 0; c < 20; ++c) hist[c * 32 + x] = 0;
+                if (track && x < 20) hist[x] = 0;
+                float bits = 0.f;  // the contexts' terms summed in context order
+                {
+                    float4 tv[5];
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) tv[k] = ((const float4*)terms)[k];
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) {
+                        bits += tv[k].x;
+                        bits += tv[k].y;
+                        bits += tv[k].z;
+                        if (k < 4) bits += tv[k].w;
+                    }
                 }
-                float bits = 0.f;
-                for (int c = 0; c < 19; ++c) bits += __shfl(term, (half << 5) | c);
                 const float bytes = bits * 0.125f + 1.f;
                 bool cand = track && act && p > 0;
                 float dabs = 0.f;
