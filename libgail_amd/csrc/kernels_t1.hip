@@ -303,12 +303,15 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                         const int r0 = 4 * s;
                         const uint32_t coded = coded_all & (0xFu << r0);
                         if (!__any(coded != 0)) continue;
+                        // per row byte: bit | (context - 14) << 1, context 16 if refined before,
+                        // else 15 if a neighbour is significant, else 14; the symbol is 28 + it
+                        const uint32_t M = spread4(Bp, r0) | spread4(nAny & ~R, r0) << 1 | spread4(R, r0) << 2;
                         Emitter em{0, 0};
 #pragma unroll
                         for (int k = 0; k < 4; ++k) {
                             const int r = r0 + k;
                             if (!bitv(coded, r)) continue;
-                            em.put(bitv(R, r) ? 16u : (bitv(nAny, r) ? 15u : 14u), bitv(Bp, r));
+                            em.put_sym(28u + ((M >> (8 * k)) & 7u));
                         }
                         const unsigned long long b0 = __ballot(em.n & 1u), b1 = __ballot(em.n & 2u);
                         const unsigned long long b2 = __ballot(em.n & 4u);
@@ -445,9 +448,14 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
             // early-termination counts of the plane's symbols, 32 lanes per block at a time
             // (not one lane per column inside the divergent emission loops)
             if (track && act)
-                for (uint32_t i = (uint32_t)x; i < plane_count; i += 32u) {
-                    const uint32_t sv = sym[i];
-                    atomicAdd(&hist[sv >> 1], (sv & 1u) ? 0x10000u : 1u);
+                for (uint32_t i = 4u * (uint32_t)x; i < plane_count; i += 128u) {
+                    const uint32_t w4 = *(const uint32_t*)(sym + i);  // within the staging area
+#pragma unroll
+                    for (uint32_t j = 0; j < 4; ++j)
+                        if (i + j < plane_count) {
+                            const uint32_t sv = (w4 >> (8 * j)) & 0xFFu;
+                            atomicAdd(&hist[sv >> 1], (sv & 1u) ? 0x10000u : 1u);
+                        }
                 }
             const uint32_t cnt_lo = __shfl((int)plane_count, 0), cnt_hi = __shfl((int)plane_count, 32);
             // 64-byte aligned: the MQ coder reads a plane in 64-byte groups
