@@ -319,6 +319,219 @@ void or_color_pixel(const float* lut, const float* cp, uint16_t r, uint16_t g, u
     out[2] = (int)encode_code(Z);
 }
 
+/* ---- managed colour with tone map, gamut compression and 3D LUT ---- */
+/* SMPTE ST 2084 constants (color.cpp:14-19). */
+static const double kPqM1 = 2610.0 / 16384.0, kPqM2 = 2523.0 / 4096.0 * 128.0, kPqC1 = 3424.0 / 4096.0,
+                    kPqC2 = 2413.0 / 4096.0 * 32.0, kPqC3 = 2392.0 / 4096.0 * 32.0;
+
+static double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+static double maxd(double a, double b) { return a < b ? b : a; } /* std::max */
+
+/* pqEotf / pqInverseEotf in double (color.cpp:209-218). */
+static double pq_eotf_d(double e) {
+    e = clampd(e, 0.0, 1.0);
+    const double p = pow(e, 1.0 / kPqM2);
+    return 10000.0 * pow(maxd(p - kPqC1, 0.0) / (kPqC2 - kPqC3 * p), 1.0 / kPqM1);
+}
+static double pq_inv_d(double nits) {
+    const double y = pow(clampd(nits / 10000.0, 0.0, 1.0), kPqM1);
+    return pow((kPqC1 + kPqC2 * y) / (1.0 + kPqC3 * y), kPqM2);
+}
+/* BT.2100 HLG inverse OETF (color.cpp:81-84). */
+static double hlg_inv_oetf(double e) {
+    const double a = 0.17883277, b = 1.0 - 4.0 * a, c = 0.5 - a * log(4.0 * a);
+    return e <= 0.5 ? e * e / 3.0 : (exp((e - c) / a) + b) / 12.0;
+}
+
+/* ColorConverter's constructor (color.cpp:296-372): the EOTF table and the parameter block
+ * (color_params.h) for every tone map, tone norm and target gamut. Enumerations as in
+ * ColorParams: tone_map 0 signal, 1 shoulder, 2 BT.2390, 3 none; tone_norm 0 channel,
+ * 1 max(R,G,B), 2 luminance; gamut 0 P3-D65, 1 DCI-P3, 2 none. */
+void or_color_setup(const or_color_cfg* c, float* lut, float* cp) {
+    const int hdr = c->transfer == 2 || c->transfer == 3, passthrough = c->primaries == 4;
+    for (int i = 0; i < 65536; ++i) {
+        const double e = i / 65535.0;
+        double nits = 0;
+        switch (c->transfer) {
+        case 0: nits = pow(e, 2.4) * c->sdr_white; break;
+        case 1: nits = (e <= 0.04045 ? e / 12.92 : pow((e + 0.055) / 1.055, 2.4)) * c->sdr_white; break;
+        case 2: nits = c->tone_map == 0 ? pow(e, 2.4) * c->sdr_white : pq_eotf_d(e); break;
+        case 3: nits = c->tone_map == 0 ? pow(e, 2.4) * c->sdr_white : 1000.0 * pow(hlg_inv_oetf(e), 1.2); break;
+        case 4: nits = e * c->sdr_white; break;
+        case 5: nits = pow(e, 2.6) * 52.37; break;
+        }
+        lut[i] = (float)nits;
+    }
+    memset(cp, 0, 48 * sizeof(float));
+    double M[9];
+    rgb_to_xyz_matrix(c->primaries, M);
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k) cp[4 * r + k] = (float)M[3 * r + k];
+    cp[17] = (float)c->cinema_peak;
+    if (hdr && !passthrough && c->tone_map == 1) {
+        const double gain = c->cinema_ref_white / c->cinema_peak;
+        double knee = clampd(c->knee, 0.05, 0.99);
+        const double vmax = gain * c->hdr_peak / c->hdr_ref_white;
+        if (vmax <= 1.0) knee = 1.0;
+        cp[12] = 1.f;
+        cp[13] = (float)c->hdr_ref_white;
+        cp[14] = (float)gain;
+        cp[15] = (float)knee;
+        cp[16] = (float)maxd(1e-3, (vmax - knee) / (1.0 - knee));
+    } else if (hdr && !passthrough && c->tone_map == 2 && c->hdr_peak > c->cinema_peak) {
+        cp[12] = 2.f;
+        cp[39] = (float)pq_inv_d(c->hdr_peak);
+        cp[40] = (float)(pq_inv_d(c->cinema_peak) / pq_inv_d(c->hdr_peak));
+        const float ks = 1.5f * cp[40] - 0.5f;
+        cp[41] = 0.f < ks ? ks : 0.f;
+    }
+    cp[42] = c->tone_norm == 0 ? 2.f : c->tone_norm == 1 ? 1.f : 0.f;
+    if (!passthrough) {
+        double T[9], Ti[9];
+        rgb_to_xyz_matrix(c->gamut == 1 ? 3 : 2, T);
+        invert3(T, Ti);
+        for (int i = 0; i < 9; ++i) {
+            cp[20 + i] = (float)(Ti[i] / c->cinema_peak);
+            cp[29 + i] = (float)(T[i] * c->cinema_peak);
+        }
+        if (c->gamut != 2) {
+            cp[18] = 1.f;
+            cp[19] = (float)clampd(c->gamut_threshold, 0.0, 0.99);
+        }
+    }
+    cp[38] = (float)c->lut3d_size;
+}
+
+/* cm_pq_eotf / cm_pq_inv in float (color_math.h:27-40). */
+static float pq_eotf_f(float e) {
+    const float m1 = 2610.0f / 16384.0f, m2 = 2523.0f / 4096.0f * 128.0f, c1 = 3424.0f / 4096.0f,
+                c2 = 2413.0f / 4096.0f * 32.0f, c3 = 2392.0f / 4096.0f * 32.0f;
+    const float p = powf(fminf(fmaxf(e, 0.0f), 1.0f), 1.0f / m2);
+    return 10000.0f * powf(fmaxf(p - c1, 0.0f) / (c2 - c3 * p), 1.0f / m1);
+}
+static float pq_inv_f(float nits) {
+    const float m1 = 2610.0f / 16384.0f, m2 = 2523.0f / 4096.0f * 128.0f, c1 = 3424.0f / 4096.0f,
+                c2 = 2413.0f / 4096.0f * 32.0f, c3 = 2392.0f / 4096.0f * 32.0f;
+    const float y = powf(fminf(fmaxf(nits / 10000.0f, 0.0f), 1.0f), m1);
+    return powf((c1 + c2 * y) / (1.0f + c3 * y), m2);
+}
+
+/* cm_tone (color_math.h:42-65): target luminance for source luminance y. */
+static float tone(const float* cp, float y) {
+    if (y <= 0.0f) return 0.0f;
+    if (cp[12] > 1.5f) {
+        const float srcMax = cp[39], ks = cp[41];
+        const float e1 = pq_inv_f(y) / srcMax;
+        if (e1 < ks) return y;
+        const float t = (fminf(e1, 1.0f) - ks) / (1.0f - ks);
+        const float t2 = t * t, t3 = t2 * t;
+        const float e2 = (2.0f * t3 - 3.0f * t2 + 1.0f) * ks + (t3 - 2.0f * t2 + t) * (1.0f - ks) +
+                         (-2.0f * t3 + 3.0f * t2) * cp[40];
+        return pq_eotf_f(e2 * srcMax);
+    }
+    float v = cp[14] * y / cp[13];
+    const float k = cp[15];
+    if (v > k) {
+        const float t = (v - k) / (1.0f - k);
+        const float T = cp[16];
+        v = k + (1.0f - k) * fminf(t * (1.0f + t / (T * T)) / (1.0f + t), 1.0f);
+    }
+    return v * cp[17];
+}
+
+/* cm_gamut (color_math.h:67-105): toward neutral grey of the same luminance until every
+ * target-gamut channel lies in [0, 1], compressed smoothly beyond the threshold. */
+static void gamut(const float* cp, float* X, float* Y, float* Z) {
+    float r = cp[20] * *X + cp[21] * *Y + cp[22] * *Z;
+    float g = cp[23] * *X + cp[24] * *Y + cp[25] * *Z;
+    float b = cp[26] * *X + cp[27] * *Y + cp[28] * *Z;
+    const float n = *Y / cp[17];
+    if (n >= 1.0f) {
+        r = g = b = 1.0f;
+    } else {
+        float amax = 1.0f;
+        const float d0 = r - n, d1 = g - n, d2 = b - n;
+        if (d0 > 0.0f) amax = fminf(amax, (1.0f - n) / d0);
+        if (d0 < 0.0f) amax = fminf(amax, -n / d0);
+        if (d1 > 0.0f) amax = fminf(amax, (1.0f - n) / d1);
+        if (d1 < 0.0f) amax = fminf(amax, -n / d1);
+        if (d2 > 0.0f) amax = fminf(amax, (1.0f - n) / d2);
+        if (d2 < 0.0f) amax = fminf(amax, -n / d2);
+        const float dist = 1.0f / fmaxf(amax, 1e-6f);
+        const float th = cp[19];
+        float alpha = 1.0f;
+        if (dist > th) {
+            const float x = (dist - th) / (1.0f - th);
+            const float y = x / (1.0f + x);
+            alpha = (th + (1.0f - th) * y) / dist;
+        }
+        r = n + alpha * d0;
+        g = n + alpha * d1;
+        b = n + alpha * d2;
+    }
+    *X = cp[29] * r + cp[30] * g + cp[31] * b;
+    *Y = cp[32] * r + cp[33] * g + cp[34] * b;
+    *Z = cp[35] * r + cp[36] * g + cp[37] * b;
+}
+
+/* cm_rgb_to_xyz12 (color_math.h:114-179) for pixels given as 16-bit codes (code / 65535,
+ * color.cpp:400-406): the 3D LUT route or the managed route with tone map and gamut. */
+void or_color_pixels(const float* lut, const float* cp, const float* lut3, int n, const uint16_t* rgb, int* out) {
+    for (int p = 0; p < n; ++p) {
+        const float r = rgb[3 * p] / 65535.f, g = rgb[3 * p + 1] / 65535.f, b = rgb[3 * p + 2] / 65535.f;
+        int* o = out + 3 * p;
+        const int ln = (int)cp[38];
+        if (ln > 1) {
+            const float fr = fminf(fmaxf(r, 0.0f), 1.0f) * (float)(ln - 1);
+            const float fg = fminf(fmaxf(g, 0.0f), 1.0f) * (float)(ln - 1);
+            const float fb = fminf(fmaxf(b, 0.0f), 1.0f) * (float)(ln - 1);
+            const int i0 = imin((int)fr, ln - 2), j0 = imin((int)fg, ln - 2), k0 = imin((int)fb, ln - 2);
+            const float ar = fr - (float)i0, ag = fg - (float)j0, ab = fb - (float)k0;
+#define AT(i, j, k, c) lut3[(((k) * ln + (j)) * ln + (i)) * 3 + (c)]
+            for (int c = 0; c < 3; ++c) {
+                const float c00 = AT(i0, j0, k0, c) * (1.0f - ar) + AT(i0 + 1, j0, k0, c) * ar;
+                const float c10 = AT(i0, j0 + 1, k0, c) * (1.0f - ar) + AT(i0 + 1, j0 + 1, k0, c) * ar;
+                const float c01 = AT(i0, j0, k0 + 1, c) * (1.0f - ar) + AT(i0 + 1, j0, k0 + 1, c) * ar;
+                const float c11 = AT(i0, j0 + 1, k0 + 1, c) * (1.0f - ar) + AT(i0 + 1, j0 + 1, k0 + 1, c) * ar;
+                const float v = (c00 * (1.0f - ag) + c10 * ag) * (1.0f - ab) + (c01 * (1.0f - ag) + c11 * ag) * ab;
+                o[c] = (int)floorf(fminf(fmaxf(v, 0.0f), 1.0f) * 4095.0f + 0.5f);
+            }
+#undef AT
+            continue;
+        }
+        const float R = lut[lut_index(r)], G = lut[lut_index(g)], B = lut[lut_index(b)];
+        float X = cp[0] * R + cp[1] * G + cp[2] * B;
+        float Y = cp[4] * R + cp[5] * G + cp[6] * B;
+        float Z = cp[8] * R + cp[9] * G + cp[10] * B;
+        if (cp[12] > 0.5f && Y > 0.0f) {
+            float tr = cp[20] * X + cp[21] * Y + cp[22] * Z;
+            float tg = cp[23] * X + cp[24] * Y + cp[25] * Z;
+            float tb = cp[26] * X + cp[27] * Y + cp[28] * Z;
+            const float pk = cp[17];
+            if (cp[42] > 1.5f) {
+                const float mx = fmaxf(tr, fmaxf(tg, tb)) * pk;
+                const float sneg = mx > 0.0f ? tone(cp, mx) / mx : 1.0f;
+                tr = tr > 0.0f ? tone(cp, tr * pk) / pk : tr * sneg;
+                tg = tg > 0.0f ? tone(cp, tg * pk) / pk : tg * sneg;
+                tb = tb > 0.0f ? tone(cp, tb * pk) / pk : tb * sneg;
+                X = cp[29] * tr + cp[30] * tg + cp[31] * tb;
+                Y = cp[32] * tr + cp[33] * tg + cp[34] * tb;
+                Z = cp[35] * tr + cp[36] * tg + cp[37] * tb;
+            } else {
+                const float m = cp[42] > 0.5f ? fmaxf(Y, fmaxf(tr, fmaxf(tg, tb)) * pk) : Y;
+                const float s = tone(cp, m) / m;
+                X *= s;
+                Y *= s;
+                Z *= s;
+            }
+        }
+        if (cp[18] > 0.5f) gamut(cp, &X, &Y, &Z);
+        o[0] = (int)encode_code(X);
+        o[1] = (int)encode_code(Y);
+        o[2] = (int)encode_code(Z);
+    }
+}
+
 uint16_t or_xyz12_to_16(uint32_t v) {
     if (v > 4095u) v = 4095u;
     return (uint16_t)((v * 65535u + 2047u) / 4095u);

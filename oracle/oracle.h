@@ -66,6 +66,19 @@ void or_color_tables(int transfer, int primaries, double sdr_white, double cinem
                      float* params48);
 /* cm_rgb_to_xyz12 for one R'G'B' pixel given as 16-bit codes; codes out as ints. */
 void or_color_pixel(const float* lut, const float* params, uint16_t r, uint16_t g, uint16_t b, int out[3]);
+/* ColorParams of the managed route (color.h:84-100) in ColorParams' enumerations:
+ * tone_map 0 signal, 1 shoulder, 2 BT.2390, 3 none; tone_norm 0 channel, 1 max(R,G,B),
+ * 2 luminance; gamut 0 P3-D65, 1 DCI-P3, 2 none; lut3d_size: nodes per axis of an app 3D
+ * LUT (0: none). */
+typedef struct or_color_cfg {
+    int transfer, primaries, tone_map, tone_norm, gamut, lut3d_size;
+    double sdr_white, hdr_peak, cinema_peak, hdr_ref_white, cinema_ref_white, knee, gamut_threshold;
+} or_color_cfg;
+/* The EOTF LUT and the parameter block for any configuration. */
+void or_color_setup(const or_color_cfg* c, float* lut65536, float* params48);
+/* cm_rgb_to_xyz12 for n RGB48 pixels (tone map, gamut compression, 3D LUT route). */
+void or_color_pixels(const float* lut, const float* params, const float* lut3, int n, const uint16_t* rgb48,
+                     int* codes);
 /* planarToRgb48Row for a whole container frame (kind planar), into interleaved RGB48.
  * bits: sample depth; sx, sy: chroma subsampling log2; rgb: planes G',B',R';
  * chroma: 0 left, 1 center, 2 top-left; kr, kb: Y'CbCr matrix. */

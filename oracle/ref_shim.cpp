@@ -92,6 +92,51 @@ void ref_color_tables(int transfer, int primaries, double sdrWhite, double cinem
     std::memcpy(params, cc.params().data(), cc.params().size() * sizeof(float));
 }
 
+// The managed route with every colour parameter (or_color_cfg's layout), and an app 3D
+// LUT when lut3dSize > 1.
+struct RefColorCfg {
+    int transfer, primaries, toneMap, toneNorm, gamut, lut3dSize;
+    double sdrWhite, hdrPeak, cinemaPeak, hdrRefWhite, cinemaRefWhite, knee, gamutThreshold;
+};
+
+static ColorParams colorParamsFull(const RefColorCfg& c, const float* lut3) {
+    ColorParams p = colorParams(c.transfer, c.primaries, c.sdrWhite, c.cinemaPeak);
+    p.toneMap = (ToneMap)c.toneMap;
+    p.toneNorm = (ToneNorm)c.toneNorm;
+    p.gamut = (TargetGamut)c.gamut;
+    p.hdrPeak = c.hdrPeak;
+    p.hdrRefWhite = c.hdrRefWhite;
+    p.cinemaRefWhite = c.cinemaRefWhite;
+    p.knee = c.knee;
+    p.gamutThreshold = c.gamutThreshold;
+    if (c.lut3dSize > 1 && lut3) {
+        p.lut3dData.assign(lut3, lut3 + (size_t)c.lut3dSize * c.lut3dSize * c.lut3dSize * 3);
+        p.lut3dSize = c.lut3dSize;
+    }
+    return p;
+}
+
+int ref_color_setup(const RefColorCfg* c, const float* lut3, float* lut, float* params) {
+    try {
+        const ColorConverter cc(colorParamsFull(*c, lut3));
+        std::memcpy(lut, cc.lut().data(), 65536 * sizeof(float));
+        std::memcpy(params, cc.params().data(), cc.params().size() * sizeof(float));
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+int ref_color_pixels(const RefColorCfg* c, const float* lut3, int n, const uint16_t* rgb, int* out) {
+    try {
+        const ColorConverter cc(colorParamsFull(*c, lut3));
+        for (int i = 0; i < n; ++i) cc.toXYZ12(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2], out + 3 * i);
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
 void ref_convert_frame(int transfer, int primaries, double sdrWhite, double cinemaPeak, const uint16_t* rgb, int W,
                        int H, float* planes, uint16_t* xyz12) {
     const ColorConverter cc(colorParams(transfer, primaries, sdrWhite, cinemaPeak));

@@ -33,6 +33,22 @@ class CblkInfo(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("nbps", "num_passes", "data_offset", "data_len", "pass_base")]
 
 
+class ColorCfg(C.Structure):
+    """ColorParams of the managed route, in ColorParams' enumerations (or_color_cfg)."""
+    _fields_ = [(n, C.c_int32) for n in ("transfer", "primaries", "tone_map", "tone_norm", "gamut",
+                                          "lut3d_size")] + [
+        (n, C.c_double) for n in ("sdr_white", "hdr_peak", "cinema_peak", "hdr_ref_white", "cinema_ref_white",
+                                  "knee", "gamut_threshold")]
+
+
+def color_cfg(transfer=0, primaries=0, tone_map=0, tone_norm=0, gamut=2, lut3d_size=0, sdr_white=48.0,
+              hdr_peak=1000.0, cinema_peak=48.0, hdr_ref_white=203.0, cinema_ref_white=16.0, knee=0.25,
+              gamut_threshold=0.8):
+    """A ColorCfg with ColorParams' defaults (color.h:84-100) for the fields not given."""
+    return ColorCfg(transfer, primaries, tone_map, tone_norm, gamut, lut3d_size, sdr_white, hdr_peak,
+                    cinema_peak, hdr_ref_white, cinema_ref_white, knee, gamut_threshold)
+
+
 def _p(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
@@ -157,6 +173,8 @@ class Oracle(_Lib):
                                        C.c_void_p]
         L.or_xyz_planar_to_ict.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
         L.or_color_pixel.argtypes = [C.c_void_p, C.c_void_p, C.c_uint16, C.c_uint16, C.c_uint16, C.c_void_p]
+        L.or_color_setup.argtypes = [C.POINTER(ColorCfg), C.c_void_p, C.c_void_p]
+        L.or_color_pixels.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.or_xyz12_to_16.restype = C.c_uint16
         L.or_xyz12_to_16.argtypes = [C.c_uint32]
 
@@ -206,6 +224,18 @@ class Oracle(_Lib):
         self.lib.or_xyz_planar_to_ict(_p(xyz), W, H, _p(planes))
         return planes
 
+    def color_setup(self, cfg):
+        lut = np.zeros(65536, np.float32)
+        cp = np.zeros(48, np.float32)
+        self.lib.or_color_setup(C.byref(cfg), _p(lut), _p(cp))
+        return lut, cp
+
+    def color_pixels(self, lut, cp, lut3, rgb48):
+        rgb48 = np.ascontiguousarray(rgb48, dtype=np.uint16).reshape(-1, 3)
+        out = np.zeros((len(rgb48), 3), np.int32)
+        self.lib.or_color_pixels(_p(lut), _p(cp), _p(lut3), len(rgb48), _p(rgb48), _p(out))
+        return out
+
     def color_pixel(self, lut, cp, r, g, b):
         out = np.zeros(3, np.int32)
         self.lib.or_color_pixel(_p(lut), _p(cp), r, g, b, _p(out))
@@ -220,11 +250,29 @@ class Reference(_Lib):
         L.ref_synthesis_norm.argtypes = [C.c_int, C.c_int]
         L.ref_quantize_coeff.restype = C.c_uint32
         L.ref_quantize_coeff.argtypes = [C.c_float, C.c_float, C.c_uint32]
+        L.ref_color_setup.restype = C.c_int
+        L.ref_color_setup.argtypes = [C.POINTER(ColorCfg), C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ref_color_pixels.restype = C.c_int
+        L.ref_color_pixels.argtypes = [C.POINTER(ColorCfg), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.ref_convert_frame.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_int, C.c_int,
                                         C.c_void_p, C.c_void_p]
 
     def synthesis_norm(self, level, orient):
         return self.lib.ref_synthesis_norm(level, orient)
+
+    def color_setup(self, cfg, lut3=None):
+        lut = np.zeros(65536, np.float32)
+        cp = np.zeros(48, np.float32)
+        if self.lib.ref_color_setup(C.byref(cfg), _p(lut3), _p(lut), _p(cp)):
+            raise ValueError("reference rejected the colour parameters")
+        return lut, cp
+
+    def color_pixels(self, cfg, lut3, rgb48):
+        rgb48 = np.ascontiguousarray(rgb48, dtype=np.uint16).reshape(-1, 3)
+        out = np.zeros((len(rgb48), 3), np.int32)
+        if self.lib.ref_color_pixels(C.byref(cfg), _p(lut3), len(rgb48), _p(rgb48), _p(out)):
+            raise ValueError("reference rejected the colour parameters")
+        return out
 
     def quantize_coeff(self, v, inv_delta, max_mag):
         return self.lib.ref_quantize_coeff(v, inv_delta, max_mag)
