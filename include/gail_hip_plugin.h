@@ -80,12 +80,19 @@ typedef struct GailHipFrameFormat {
     double kr, kb;
 } GailHipFrameFormat;
 
-/* GailCudaColor (cuda_plugin.h:91-97), the members this backend honours: transfer
- * (0 BT.1886, 1 sRGB, 2 PQ, 3 HLG, 4 linear, 5 gamma 2.6) and primaries (0 BT.709,
- * 1 BT.2020, 2 P3-D65, 3 DCI-P3, 4 X'Y'Z'); HDR sources are taken in the "signal" mode. */
+/* GailCudaColor (cuda_plugin.h:91-97), gail::ColorParams (color.h:84-100) in its
+ * enumerations: transfer 0 BT.1886, 1 sRGB, 2 PQ, 3 HLG, 4 linear, 5 gamma 2.6; primaries
+ * 0 BT.709, 1 BT.2020, 2 P3-D65, 3 DCI-P3, 4 X'Y'Z'; toneMap (PQ / HLG sources) 0 signal,
+ * 1 shoulder, 2 BT.2390, 3 none; toneNorm 0 each channel, 1 max(R,G,B), 2 luminance; gamut
+ * 0 P3-D65, 1 DCI-P3, 2 none. An app 3D LUT (lut3dData: lut3dSize^3 x 3 floats, red
+ * fastest; or a .cube file named by lut3d) replaces the managed route. Non-positive
+ * luminances and knee take ColorParams' defaults. */
 typedef struct GailHipColor {
     int32_t transfer, primaries, toneMap, toneNorm, gamut, lut3dSize;
     double sdrWhite, hdrPeak, cinemaPeak, hdrRefWhite, cinemaRefWhite, knee, gamutThreshold;
+    const char* lut3d;        /* path of a .cube file, or NULL / "" */
+    const float* lut3dData;
+    uint64_t lut3dCount;      /* floats at lut3dData */
 } GailHipColor;
 
 /* GailCudaEncodeSettings (cuda_plugin.h:100-105). */

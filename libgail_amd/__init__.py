@@ -42,9 +42,27 @@ class FrameFormat(C.Structure):
 
 
 class Color(C.Structure):
+    """GailHipColor: ColorParams in its own enumerations (gamut 0 P3-D65, 1 DCI-P3, 2 none)."""
     _fields_ = [(n, C.c_int32) for n in ("transfer", "primaries", "toneMap", "toneNorm", "gamut", "lut3dSize")] + [
         (n, C.c_double) for n in ("sdrWhite", "hdrPeak", "cinemaPeak", "hdrRefWhite", "cinemaRefWhite", "knee",
-                                  "gamutThreshold")]
+                                  "gamutThreshold")] + [
+        ("lut3d", C.c_char_p), ("lut3dData", C.c_void_p), ("lut3dCount", C.c_uint64)]
+
+
+def set_color(color, transfer=0, primaries=0, tone_map=0, tone_norm=0, gamut=2, sdr_white=48.0, hdr_peak=1000.0,
+              cinema_peak=48.0, hdr_ref_white=203.0, cinema_ref_white=16.0, knee=0.25, gamut_threshold=0.8,
+              lut3d=None):
+    """Fills a Color with ColorParams' defaults for the members not given; lut3d: a float32
+    array of n^3 x 3 values (red fastest), kept alive by the caller until the frame is bound."""
+    color.transfer, color.primaries, color.toneMap, color.toneNorm, color.gamut = (transfer, primaries, tone_map,
+                                                                                  tone_norm, gamut)
+    color.sdrWhite, color.hdrPeak, color.cinemaPeak = sdr_white, hdr_peak, cinema_peak
+    color.hdrRefWhite, color.cinemaRefWhite, color.knee, color.gamutThreshold = (hdr_ref_white, cinema_ref_white,
+                                                                                 knee, gamut_threshold)
+    if lut3d is not None:
+        n = round((len(lut3d) // 3) ** (1.0 / 3.0))
+        color.lut3dSize, color.lut3dData, color.lut3dCount = n, lut3d.ctypes.data, len(lut3d)
+    return color
 
 
 class EncodeSettings(C.Structure):
@@ -241,7 +259,7 @@ def xyz_settings(layout):
     s = EncodeSettings()
     s.profile, s.width, s.height, s.baseStep = layout.fourk, layout.width, layout.height, layout.base_step
     s.input.kind, s.input.width, s.input.height = 2, layout.width, layout.height
-    s.color.transfer, s.color.primaries, s.color.sdrWhite, s.color.cinemaPeak = 5, 4, 48.0, 48.0
+    set_color(s.color, transfer=5, primaries=4)
     return s
 
 
@@ -254,7 +272,17 @@ def planar_settings(layout, bits, sx, sy, rgb, full_range, chroma=0, kr=0.2126, 
     aw, ah = active if active else (layout.width, layout.height)
     f.activeW, f.activeH, f.ox, f.oy = aw, ah, offset[0], offset[1]
     f.sx, f.sy, f.rgb, f.bits, f.fullRange, f.chroma, f.kr, f.kb = sx, sy, rgb, bits, full_range, chroma, kr, kb
-    s.color.transfer, s.color.primaries, s.color.sdrWhite, s.color.cinemaPeak = transfer, primaries, 48.0, 48.0
+    set_color(s.color, transfer=transfer, primaries=primaries)
+    return s
+
+
+def rgb48_settings(layout, **color):
+    """Interleaved 16-bit R'G'B' at the container size (the reference's CUDA slot input),
+    colour by set_color's keywords."""
+    s = EncodeSettings()
+    s.profile, s.width, s.height, s.baseStep = layout.fourk, layout.width, layout.height, layout.base_step
+    s.input.kind, s.input.width, s.input.height = 0, layout.width, layout.height
+    set_color(s.color, **color)
     return s
 
 

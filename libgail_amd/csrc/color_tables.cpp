@@ -1,7 +1,11 @@
 #include "color_tables.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
 
 namespace gail_amd {
 
@@ -55,24 +59,129 @@ float encodeCode(float v) {
 
 }  // namespace
 
-void buildColorTables(const ColorSpec& c, std::vector<float>& lut, float matrix[12]) {
+bool ColorSpec::sameTables(const ColorSpec& o) const {
+    if (transfer != o.transfer || primaries != o.primaries || toneMap != o.toneMap || toneNorm != o.toneNorm ||
+        gamut != o.gamut || sdrWhite != o.sdrWhite || hdrPeak != o.hdrPeak || cinemaPeak != o.cinemaPeak ||
+        hdrRefWhite != o.hdrRefWhite || cinemaRefWhite != o.cinemaRefWhite || knee != o.knee ||
+        gamutThreshold != o.gamutThreshold || lut3dSize != o.lut3dSize)
+        return false;
+    if (lut3d == o.lut3d) return true;
+    if (!lut3d || !o.lut3d) return false;
+    return *lut3d == *o.lut3d;
+}
+
+namespace {
+
+// SMPTE ST 2084 (PQ) and BT.2100 HLG in double, for the tables.
+constexpr double kM1 = 2610.0 / 16384.0, kM2 = 2523.0 / 4096.0 * 128.0, kC1 = 3424.0 / 4096.0,
+                 kC2 = 2413.0 / 4096.0 * 32.0, kC3 = 2392.0 / 4096.0 * 32.0;
+
+double pqToNits(double e) {
+    e = std::clamp(e, 0.0, 1.0);
+    const double p = std::pow(e, 1.0 / kM2);
+    return 10000.0 * std::pow(std::max(p - kC1, 0.0) / (kC2 - kC3 * p), 1.0 / kM1);
+}
+
+double nitsToPq(double nits) {
+    const double y = std::pow(std::clamp(nits / 10000.0, 0.0, 1.0), kM1);
+    return std::pow((kC1 + kC2 * y) / (1.0 + kC3 * y), kM2);
+}
+
+double hlgSceneLight(double e) {
+    const double a = 0.17883277, b = 1.0 - 4.0 * a, c = 0.5 - a * std::log(4.0 * a);
+    return e <= 0.5 ? e * e / 3.0 : (std::exp((e - c) / a) + b) / 12.0;
+}
+
+}  // namespace
+
+void buildColorTables(const ColorSpec& c, std::vector<float>& lut, float cp[kColorParams]) {
+    const bool hdr = c.transfer == 2 || c.transfer == 3, passthrough = c.primaries == 4;
     lut.resize(65536);
     for (int i = 0; i < 65536; ++i) {
         const double e = i / 65535.0;
         double nits;
         switch (c.transfer) {
         case 1: nits = (e <= 0.04045 ? e / 12.92 : std::pow((e + 0.055) / 1.055, 2.4)) * c.sdrWhite; break;
+        case 2: nits = c.toneMap == 0 ? std::pow(e, 2.4) * c.sdrWhite : pqToNits(e); break;
+        case 3: nits = c.toneMap == 0 ? std::pow(e, 2.4) * c.sdrWhite : 1000.0 * std::pow(hlgSceneLight(e), 1.2); break;
         case 4: nits = e * c.sdrWhite; break;
         case 5: nits = std::pow(e, 2.6) * 52.37; break;
         default: nits = std::pow(e, 2.4) * c.sdrWhite; break;
         }
         lut[i] = (float)nits;
     }
+    std::fill(cp, cp + kColorParams, 0.f);
     double M[9];
     rgbToXyz(c.primaries, M);
-    std::memset(matrix, 0, 12 * sizeof(float));
     for (int r = 0; r < 3; ++r)
-        for (int k = 0; k < 3; ++k) matrix[4 * r + k] = (float)M[3 * r + k];
+        for (int k = 0; k < 3; ++k) cp[4 * r + k] = (float)M[3 * r + k];
+    cp[17] = (float)c.cinemaPeak;
+    if (hdr && !passthrough && c.toneMap == 1) {  // shoulder (scene-relative)
+        const double gain = c.cinemaRefWhite / c.cinemaPeak;
+        double knee = std::clamp(c.knee, 0.05, 0.99);
+        const double vmax = gain * c.hdrPeak / c.hdrRefWhite;  // source peak on the output scale
+        if (vmax <= 1.0) knee = 1.0;                            // it already fits: no shoulder
+        cp[12] = 1.f;
+        cp[13] = (float)c.hdrRefWhite;
+        cp[14] = (float)gain;
+        cp[15] = (float)knee;
+        cp[16] = (float)std::max(1e-3, (vmax - knee) / (1.0 - knee));
+    } else if (hdr && !passthrough && c.toneMap == 2 && c.hdrPeak > c.cinemaPeak) {  // BT.2390 EETF
+        cp[12] = 2.f;
+        cp[39] = (float)nitsToPq(c.hdrPeak);
+        cp[40] = (float)(nitsToPq(c.cinemaPeak) / nitsToPq(c.hdrPeak));
+        cp[41] = std::max(0.f, 1.5f * cp[40] - 0.5f);
+    }
+    cp[42] = c.toneNorm == 0 ? 2.f : c.toneNorm == 1 ? 1.f : 0.f;
+    if (!passthrough) {
+        double T[9], Ti[9];
+        rgbToXyz(c.gamut == 1 ? 3 : 2, T);
+        invert3(T, Ti);
+        for (int i = 0; i < 9; ++i) {
+            cp[20 + i] = (float)(Ti[i] / c.cinemaPeak);
+            cp[29 + i] = (float)(T[i] * c.cinemaPeak);
+        }
+        if (c.gamut != 2) {
+            cp[18] = 1.f;
+            cp[19] = (float)std::clamp(c.gamutThreshold, 0.0, 0.99);
+        }
+    }
+    cp[38] = (float)(c.lut3d && c.lut3dSize > 1 ? c.lut3dSize : 0);
+}
+
+std::vector<float> readCubeLut(const std::string& path, int& size) {
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("cannot open LUT " + path);
+    std::vector<float> data;
+    size = 0;
+    std::string line;
+    while (std::getline(in, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream ss(line);
+        std::string key;
+        ss >> key;
+        if (key == "LUT_3D_SIZE") {
+            ss >> size;
+        } else if (key == "LUT_1D_SIZE") {
+            throw std::runtime_error("1D .cube LUTs are not supported: " + path);
+        } else if (key == "DOMAIN_MIN" || key == "DOMAIN_MAX") {
+            float a, b, c;
+            ss >> a >> b >> c;
+            const float want = key == "DOMAIN_MIN" ? 0.f : 1.f;
+            if (a != want || b != want || c != want) throw std::runtime_error("LUT domain must be 0..1: " + path);
+        } else if (key != "TITLE") {
+            std::istringstream vs(line);
+            float r, g, b;
+            if (vs >> r >> g >> b) {
+                data.push_back(r);
+                data.push_back(g);
+                data.push_back(b);
+            }
+        }
+    }
+    if (size < 2 || size > 256 || data.size() != (size_t)size * size * size * 3)
+        throw std::runtime_error("malformed 3D LUT (size / entry count) in " + path);
+    return data;
 }
 
 void buildCodeThresholds(std::vector<float>& thr) {

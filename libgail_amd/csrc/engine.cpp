@@ -211,11 +211,17 @@ void Job::release() {
     for (void* p : hostIn_)
         if (p) hipHostFree(p);
     hostIn_.clear();
-    for (SlotDev& d : sd_)
+    for (SlotDev& d : sd_) {
         if (d.hostLut) {
             hipHostFree(d.hostLut);
             d.hostLut = nullptr;
         }
+        if (d.lut3) {
+            hipFree(d.lut3);
+            d.lut3 = nullptr;
+            d.lut3Cap = 0;
+        }
+    }
     for (void** p : {(void**)&hFrames_, (void**)&hInfo_, (void**)&hPassLen_, (void**)&hPassDist_, (void**)&hData_,
                      (void**)&hSmall_, (void**)&hBlocks_, (void**)&hBands_, (void**)&hLevels_, (void**)&hBandScale_,
                      (void**)&hRc_})
@@ -354,10 +360,22 @@ void Job::bind(std::vector<FrameConfig> frames) {
         d.codes = fc.input.kind == InputFormat::XyzPlanar ? (uint16_t*)d.in : d.codesBuf;
         if (fc.input.kind != InputFormat::XyzPlanar) {
             const ColorSpec& cs = fc.color;
-            if (!d.lutValid || cs.transfer != d.lutSpec.transfer || cs.primaries != d.lutSpec.primaries ||
-                cs.sdrWhite != d.lutSpec.sdrWhite || cs.cinemaPeak != d.lutSpec.cinemaPeak) {
+            if (!d.lutValid || !cs.sameTables(d.lutSpec)) {
                 std::vector<float> lut;
-                buildColorTables(cs, lut, d.matrix);
+                buildColorTables(cs, lut, d.params);
+                if (cs.lut3d && cs.lut3dSize > 1) {
+                    // an app 3D LUT lives beside the slot, grown to the largest one seen
+                    const size_t n = cs.lut3d->size();
+                    if (n > d.lut3Cap) {
+                        if (d.lut3) hipFree(d.lut3);
+                        d.lut3 = nullptr;
+                        d.lut3Cap = 0;
+                        hipCheck(hipMalloc((void**)&d.lut3, n * sizeof(float)), "3D LUT");
+                        d.lut3Cap = n;
+                    }
+                    hipCheck(hipMemcpyAsync(d.lut3, cs.lut3d->data(), n * sizeof(float), hipMemcpyHostToDevice, stream_),
+                             "3D LUT upload");
+                }
                 std::memcpy(d.hostLut, lut.data(), 65536 * sizeof(float));
                 hipCheck(hipMemcpyAsync(d.lut, d.hostLut, 65536 * sizeof(float), hipMemcpyHostToDevice, stream_),
                          "EOTF table upload");
@@ -755,7 +773,9 @@ void Job::enqueueChain(bool uploadInputs) {
         c.codes = sd_[f].codes;
         c.lut = sd_[f].lut;
         c.thresholds = ctx_.thresholds();
-        std::memcpy(c.matrix, sd_[f].matrix, sizeof c.matrix);
+        std::memcpy(c.params, sd_[f].params, sizeof c.params);
+        c.lut3 = sd_[f].lut3;
+        c.lut3n = (int32_t)sd_[f].params[38];
         c.width = in.width;
         c.height = in.height;
         c.kind = in.kind == InputFormat::RGB48 ? 0 : 1;

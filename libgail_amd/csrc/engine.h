@@ -202,7 +202,9 @@ private:
         // bound frame
         size_t inBytes = 0, npix = 0;
         uint16_t* codes = nullptr;   // == in for X'Y'Z' planar input
-        float matrix[12] = {};
+        float params[kColorParams] = {};  // colour parameter block of the bound spec
+        float* lut3 = nullptr;             // app 3D LUT on the device (colour slots, on demand)
+        size_t lut3Cap = 0;                // its capacity in floats
         uint32_t firstBlock = 0, numBlocks = 0;
     };
 

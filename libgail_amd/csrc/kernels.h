@@ -29,7 +29,9 @@ struct ColorDesc {
     uint16_t* codes;          // 3 planes of width * height
     const float* lut;         // 65,536-entry EOTF table, cd/m^2
     const float* thresholds;  // 4,096 code thresholds in cd/m^2
-    float matrix[12];         // 3x4 row-major linear RGB -> XYZ
+    const float* lut3;        // app 3D LUT (lut3n^3 x 3 floats, red fastest) when lut3n > 1
+    int32_t lut3n;
+    float params[48];         // colour parameter block (RGB -> XYZ, tone map, gamut)
     int32_t width, height;
     int32_t kind;             // 0 interleaved 16-bit R'G'B', 1 planar
     int32_t aw, ah, ox, oy, sx, sy, bits, rgb, full_range, chroma;

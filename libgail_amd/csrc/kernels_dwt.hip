@@ -205,21 +205,97 @@ __global__ void __launch_bounds__(256) dwt_level_kernel(const DwtLevelDesc* __re
 }
 
 // ---------------------------------------------------------------- colour
-// Source R'G'B' sample (0..1 after range handling) to the EOTF table index, as the CPU
-// reference forms it: rounded to a 16-bit code (color.cpp:136-139), then code / 65535.
-__device__ __forceinline__ int lut_index16(float v) {
+// Source R'G'B' sample (0..1 after range handling) to the 16-bit code the CPU reference
+// rounds it to (color.cpp:136-139); the per-pixel path then works on code / 65535.
+__device__ __forceinline__ uint32_t code16(float v) {
     v = v < 0.f ? 0.f : (v > 1.f ? 1.f : v);
     const float s = v * 65535.f;
     const int r = (int)s;
-    const int code = r + ((s - (float)r) >= 0.5f ? 1 : 0);
-    const float e = (float)code / 65535.f;
-    const float q = (e < 0.f ? 0.f : (e > 1.f ? 1.f : e)) * 65535.0f + 0.5f;
-    return (int)floorf(q);
+    return (uint32_t)(r + ((s - (float)r) >= 0.5f ? 1 : 0));
 }
 
-__device__ __forceinline__ int lut_index_code(uint32_t code) {
-    const float e = (float)code / 65535.f;
+// EOTF table index of a source value e in 0..1 (color_math.h:140-142).
+__device__ __forceinline__ int lut_index(float e) {
     return (int)floorf((e < 0.f ? 0.f : (e > 1.f ? 1.f : e)) * 65535.0f + 0.5f);
+}
+
+// powf as the host C library computes it in practice: the power in double, rounded once to
+// float (the library's own double evaluation rounds to the same float except within a tiny
+// fraction of an ulp of a rounding boundary).
+__device__ __forceinline__ float pow_f(float x, float y) { return (float)pow((double)x, (double)y); }
+
+// SMPTE ST 2084 in float (color_math.h:27-40).
+__device__ __forceinline__ float pq_to_nits(float e) {
+    const float m1 = 2610.0f / 16384.0f, m2 = 2523.0f / 4096.0f * 128.0f, c1 = 3424.0f / 4096.0f,
+                c2 = 2413.0f / 4096.0f * 32.0f, c3 = 2392.0f / 4096.0f * 32.0f;
+    const float p = pow_f(fminf(fmaxf(e, 0.0f), 1.0f), 1.0f / m2);
+    return 10000.0f * pow_f(fmaxf(p - c1, 0.0f) / (c2 - c3 * p), 1.0f / m1);
+}
+__device__ __forceinline__ float nits_to_pq(float nits) {
+    const float m1 = 2610.0f / 16384.0f, m2 = 2523.0f / 4096.0f * 128.0f, c1 = 3424.0f / 4096.0f,
+                c2 = 2413.0f / 4096.0f * 32.0f, c3 = 2392.0f / 4096.0f * 32.0f;
+    const float y = pow_f(fminf(fmaxf(nits / 10000.0f, 0.0f), 1.0f), m1);
+    return pow_f((c1 + c2 * y) / (1.0f + c3 * y), m2);
+}
+
+// Tone curve: target luminance (cd/m^2) for source luminance y (color_math.h:42-65). The
+// shoulder is scene-relative (linear to the knee, then an extended-Reinhard roll-off that
+// reaches the peak at the source peak); BT.2390 is the EETF in the PQ domain.
+__device__ __forceinline__ float tone_curve(const float* cp, float y) {
+    if (y <= 0.0f) return 0.0f;
+    if (cp[12] > 1.5f) {
+        const float srcMax = cp[39], ks = cp[41];
+        const float e1 = nits_to_pq(y) / srcMax;
+        if (e1 < ks) return y;
+        const float t = (fminf(e1, 1.0f) - ks) / (1.0f - ks);
+        const float t2 = t * t, t3 = t2 * t;
+        const float e2 = (2.0f * t3 - 3.0f * t2 + 1.0f) * ks + (t3 - 2.0f * t2 + t) * (1.0f - ks) +
+                         (-2.0f * t3 + 3.0f * t2) * cp[40];
+        return pq_to_nits(e2 * srcMax);
+    }
+    float v = cp[14] * y / cp[13];
+    const float k = cp[15];
+    if (v > k) {
+        const float t = (v - k) / (1.0f - k);
+        const float T = cp[16];
+        v = k + (1.0f - k) * fminf(t * (1.0f + t / (T * T)) / (1.0f + t), 1.0f);
+    }
+    return v * cp[17];
+}
+
+// Gamut compression (color_math.h:67-105): toward neutral grey of the same luminance until
+// every target-gamut channel lies in [0, 1], smoothly beyond the threshold.
+__device__ __forceinline__ void compress_gamut(const float* cp, float& X, float& Y, float& Z) {
+    float r = cp[20] * X + cp[21] * Y + cp[22] * Z;
+    float g = cp[23] * X + cp[24] * Y + cp[25] * Z;
+    float b = cp[26] * X + cp[27] * Y + cp[28] * Z;
+    const float n = Y / cp[17];
+    if (n >= 1.0f) {
+        r = g = b = 1.0f;
+    } else {
+        float amax = 1.0f;
+        const float d0 = r - n, d1 = g - n, d2 = b - n;
+        if (d0 > 0.0f) amax = fminf(amax, (1.0f - n) / d0);
+        if (d0 < 0.0f) amax = fminf(amax, -n / d0);
+        if (d1 > 0.0f) amax = fminf(amax, (1.0f - n) / d1);
+        if (d1 < 0.0f) amax = fminf(amax, -n / d1);
+        if (d2 > 0.0f) amax = fminf(amax, (1.0f - n) / d2);
+        if (d2 < 0.0f) amax = fminf(amax, -n / d2);
+        const float dist = 1.0f / fmaxf(amax, 1e-6f);
+        const float th = cp[19];
+        float alpha = 1.0f;
+        if (dist > th) {
+            const float x = (dist - th) / (1.0f - th);
+            const float y = x / (1.0f + x);
+            alpha = (th + (1.0f - th) * y) / dist;
+        }
+        r = n + alpha * d0;
+        g = n + alpha * d1;
+        b = n + alpha * d2;
+    }
+    X = cp[29] * r + cp[30] * g + cp[31] * b;
+    Y = cp[32] * r + cp[33] * g + cp[34] * b;
+    Z = cp[35] * r + cp[36] * g + cp[37] * b;
 }
 
 // cd/m^2 -> 12-bit code: the number of code thresholds at or below v. The thresholds are
@@ -248,10 +324,10 @@ __global__ void __launch_bounds__(256) color_kernel(ColorDesc d) {
     const uint32_t npix = (uint32_t)d.width * d.height;
     if (i >= npix) return;
     const int x = (int)(i % (uint32_t)d.width), y = (int)(i / (uint32_t)d.width);
-    int idx[3] = {0, 0, 0};
+    uint32_t code[3] = {0, 0, 0};  // source R'G'B' as 16-bit codes
     if (d.kind == 0) {  // interleaved 16-bit R'G'B'
         const uint16_t* s = (const uint16_t*)d.src + 3 * (size_t)i;
-        for (int c = 0; c < 3; ++c) idx[c] = lut_index_code(s[c]);
+        for (int c = 0; c < 3; ++c) code[c] = s[c];
     } else {  // planar Y'CbCr or G'B'R' of the active picture, black around it
         const int ax = x - d.ox, ay = y - d.oy;
         float f[3] = {0.f, 0.f, 0.f};
@@ -294,16 +370,67 @@ __global__ void __launch_bounds__(256) color_kernel(ColorDesc d) {
                 f[2] = Y + d.cbB * U;
             }
         }
-        for (int c = 0; c < 3; ++c) idx[c] = lut_index16(f[c]);
+        for (int c = 0; c < 3; ++c) code[c] = code16(f[c]);
     }
-    const float R = d.lut[idx[0]], G = d.lut[idx[1]], B = d.lut[idx[2]];
-    const float* m = d.matrix;
-    const float X = m[0] * R + m[1] * G + m[2] * B;
-    const float Yv = m[4] * R + m[5] * G + m[6] * B;
-    const float Z = m[8] * R + m[9] * G + m[10] * B;
-    d.codes[i] = (uint16_t)encode_code(thr, X);
-    d.codes[npix + i] = (uint16_t)encode_code(thr, Yv);
-    d.codes[2 * (size_t)npix + i] = (uint16_t)encode_code(thr, Z);
+    // cm_rgb_to_xyz12 (color_math.h:114-179) on code / 65535
+    const float e0 = (float)code[0] / 65535.f, e1 = (float)code[1] / 65535.f, e2 = (float)code[2] / 65535.f;
+    const float* cp = d.params;
+    uint32_t out[3];
+    if (d.lut3n > 1) {
+        // app 3D LUT: trilinear between the 8 nodes around the source colour
+        const int n = d.lut3n;
+        const float fr = fminf(fmaxf(e0, 0.0f), 1.0f) * (float)(n - 1);
+        const float fg = fminf(fmaxf(e1, 0.0f), 1.0f) * (float)(n - 1);
+        const float fb = fminf(fmaxf(e2, 0.0f), 1.0f) * (float)(n - 1);
+        const int i0 = min((int)fr, n - 2), j0 = min((int)fg, n - 2), k0 = min((int)fb, n - 2);
+        const float ar = fr - (float)i0, ag = fg - (float)j0, ab = fb - (float)k0;
+        const float* L3 = d.lut3;
+        auto at = [&](int ii, int jj, int kk, int c) { return L3[(((size_t)kk * n + jj) * n + ii) * 3 + c]; };
+        for (int c = 0; c < 3; ++c) {
+            const float c00 = at(i0, j0, k0, c) * (1.0f - ar) + at(i0 + 1, j0, k0, c) * ar;
+            const float c10 = at(i0, j0 + 1, k0, c) * (1.0f - ar) + at(i0 + 1, j0 + 1, k0, c) * ar;
+            const float c01 = at(i0, j0, k0 + 1, c) * (1.0f - ar) + at(i0 + 1, j0, k0 + 1, c) * ar;
+            const float c11 = at(i0, j0 + 1, k0 + 1, c) * (1.0f - ar) + at(i0 + 1, j0 + 1, k0 + 1, c) * ar;
+            const float v = (c00 * (1.0f - ag) + c10 * ag) * (1.0f - ab) + (c01 * (1.0f - ag) + c11 * ag) * ab;
+            out[c] = (uint32_t)floorf(fminf(fmaxf(v, 0.0f), 1.0f) * 4095.0f + 0.5f);
+        }
+    } else {
+        const float R = d.lut[lut_index(e0)], G = d.lut[lut_index(e1)], B = d.lut[lut_index(e2)];
+        float X = cp[0] * R + cp[1] * G + cp[2] * B;
+        float Y = cp[4] * R + cp[5] * G + cp[6] * B;
+        float Z = cp[8] * R + cp[9] * G + cp[10] * B;
+        if (cp[12] > 0.5f && Y > 0.0f) {  // tone map (PQ / HLG sources)
+            float tr = cp[20] * X + cp[21] * Y + cp[22] * Z;
+            float tg = cp[23] * X + cp[24] * Y + cp[25] * Z;
+            float tb = cp[26] * X + cp[27] * Y + cp[28] * Z;
+            const float pk = cp[17];
+            if (cp[42] > 1.5f) {
+                // each target-gamut channel; channels below 0 scale like the brightest one
+                const float mx = fmaxf(tr, fmaxf(tg, tb)) * pk;
+                const float sneg = mx > 0.0f ? tone_curve(cp, mx) / mx : 1.0f;
+                tr = tr > 0.0f ? tone_curve(cp, tr * pk) / pk : tr * sneg;
+                tg = tg > 0.0f ? tone_curve(cp, tg * pk) / pk : tg * sneg;
+                tb = tb > 0.0f ? tone_curve(cp, tb * pk) / pk : tb * sneg;
+                X = cp[29] * tr + cp[30] * tg + cp[31] * tb;
+                Y = cp[32] * tr + cp[33] * tg + cp[34] * tb;
+                Z = cp[35] * tr + cp[36] * tg + cp[37] * tb;
+            } else {
+                // luminance, or max(R,G,B) (hue kept); XYZ scaled alike
+                const float m = cp[42] > 0.5f ? fmaxf(Y, fmaxf(tr, fmaxf(tg, tb)) * pk) : Y;
+                const float s = tone_curve(cp, m) / m;
+                X *= s;
+                Y *= s;
+                Z *= s;
+            }
+        }
+        if (cp[18] > 0.5f) compress_gamut(cp, X, Y, Z);
+        out[0] = encode_code(thr, X);
+        out[1] = encode_code(thr, Y);
+        out[2] = encode_code(thr, Z);
+    }
+    d.codes[i] = (uint16_t)out[0];
+    d.codes[npix + i] = (uint16_t)out[1];
+    d.codes[2 * (size_t)npix + i] = (uint16_t)out[2];
 }
 
 // ---------------------------------------------------------------- launchers

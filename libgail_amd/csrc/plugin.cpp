@@ -130,10 +130,34 @@ FrameConfig toFrame(const GailHipLayout& l, const GailHipEncodeSettings& s) {
     }
     f.rcMaxTotal = s.rcMaxTotal;
     f.rcMaxComp = s.rcMaxComp;
-    f.color.transfer = s.color.transfer;
-    f.color.primaries = s.color.primaries;
-    f.color.sdrWhite = s.color.sdrWhite > 0 ? s.color.sdrWhite : 48.0;
-    f.color.cinemaPeak = s.color.cinemaPeak > 0 ? s.color.cinemaPeak : 48.0;
+    const GailHipColor& c = s.color;
+    if (c.transfer < 0 || c.transfer > 5 || c.primaries < 0 || c.primaries > 4 || c.toneMap < 0 || c.toneMap > 3 ||
+        c.toneNorm < 0 || c.toneNorm > 2 || c.gamut < 0 || c.gamut > 2)
+        throw std::logic_error("colour enumeration out of range");
+    if (c.cinemaPeak > 52.37 || !(c.gamutThreshold >= 0 && c.gamutThreshold < 1) || (c.knee != 0 && !(c.knee > 0 && c.knee < 1)))
+        throw std::logic_error("colour parameter out of range");
+    f.color.transfer = c.transfer;
+    f.color.primaries = c.primaries;
+    f.color.toneMap = c.toneMap;
+    f.color.toneNorm = c.toneNorm;
+    f.color.gamut = c.gamut;
+    f.color.sdrWhite = c.sdrWhite > 0 ? c.sdrWhite : 48.0;
+    f.color.hdrPeak = c.hdrPeak > 0 ? c.hdrPeak : 1000.0;
+    f.color.cinemaPeak = c.cinemaPeak > 0 ? c.cinemaPeak : 48.0;
+    f.color.hdrRefWhite = c.hdrRefWhite > 0 ? c.hdrRefWhite : 203.0;
+    f.color.cinemaRefWhite = c.cinemaRefWhite > 0 ? c.cinemaRefWhite : 16.0;
+    f.color.knee = c.knee > 0 ? c.knee : 0.25;
+    f.color.gamutThreshold = c.gamutThreshold;
+    if (c.lut3dData && c.lut3dCount) {
+        if (c.lut3dSize < 2 || c.lut3dSize > 256 || c.lut3dCount != (uint64_t)c.lut3dSize * c.lut3dSize * c.lut3dSize * 3)
+            throw std::logic_error("3D LUT data does not match its size");
+        f.color.lut3dSize = c.lut3dSize;
+        f.color.lut3d = std::make_shared<const std::vector<float>>(c.lut3dData, c.lut3dData + c.lut3dCount);
+    } else if (c.lut3d && c.lut3d[0]) {
+        int n = 0;
+        f.color.lut3d = std::make_shared<const std::vector<float>>(readCubeLut(c.lut3d, n));
+        f.color.lut3dSize = n;
+    }
     return f;
 }
 

@@ -78,8 +78,8 @@ std::unique_ptr<StreamSpec> StreamSpec::make(const gail_config& c) {
         throw ApiError(GAIL_ERROR_INVALID, "fps or max_mbps out of range");
     if (!allZero(cfg.input.reserved, 4) || !allZero(cfg.color.reserved, 8))
         throw ApiError(GAIL_ERROR_INVALID, "reserved fields must be zero");
-    if (cfg.color.mode == GAIL_COLOR_LUT3D)
-        throw ApiError(GAIL_ERROR_UNSUPPORTED, "3D LUT colour is not supported by the HIP backend");
+    if (cfg.color.mode != GAIL_COLOR_MANAGED && cfg.color.mode != GAIL_COLOR_LUT3D)
+        throw ApiError(GAIL_ERROR_INVALID, "invalid color mode");
     if (gail_container_size(cfg.profile, cfg.container, &e->width, &e->height) != GAIL_OK)
         throw ApiError(GAIL_ERROR_INVALID, "invalid profile or container");
     e->picW = cfg.input.width ? cfg.input.width : e->width;
@@ -151,14 +151,39 @@ std::unique_ptr<StreamSpec> StreamSpec::make(const gail_config& c) {
     } else {
         throw ApiError(GAIL_ERROR_INVALID, "invalid pixel format");
     }
+    // colour (gail_api.cpp:158-198): gail_color in ColorParams' terms
+    const gail_color& gc = cfg.color;
+    if (!(gc.sdr_white > 0 && gc.hdr_peak > 0 && gc.hdr_ref_white > 0 && gc.cinema_ref_white > 0 &&
+          gc.cinema_peak > 0 && gc.cinema_peak <= 52.37 && gc.knee > 0 && gc.knee < 1 && gc.gamut_threshold >= 0 &&
+          gc.gamut_threshold < 1))
+        throw ApiError(GAIL_ERROR_INVALID, "color parameter out of range");
+    if (gc.transfer < GAIL_TRANSFER_BT1886 || gc.transfer > GAIL_TRANSFER_GAMMA26 || gc.primaries < GAIL_PRIMARIES_BT709 ||
+        gc.primaries > GAIL_PRIMARIES_DCIP3 || gc.tone_map < GAIL_TONE_MAP_SIGNAL || gc.tone_map > GAIL_TONE_MAP_NONE ||
+        gc.tone_norm < GAIL_TONE_NORM_CHANNEL || gc.tone_norm > GAIL_TONE_NORM_LUMINANCE || gc.gamut < GAIL_GAMUT_NONE ||
+        gc.gamut > GAIL_GAMUT_DCIP3)
+        throw ApiError(GAIL_ERROR_INVALID, "invalid colour enumeration");
     const bool xyz = s.input.kind == 2;
-    s.color.transfer = xyz ? 5 : (int)cfg.color.transfer;
-    s.color.primaries = xyz ? 4 : (int)cfg.color.primaries;
-    s.color.sdrWhite = cfg.color.sdr_white;
-    s.color.cinemaPeak = cfg.color.cinema_peak;
-    if (!xyz && (cfg.color.transfer == GAIL_TRANSFER_PQ || cfg.color.transfer == GAIL_TRANSFER_HLG) &&
-        cfg.color.tone_map != GAIL_TONE_MAP_SIGNAL)
-        throw ApiError(GAIL_ERROR_UNSUPPORTED, "HDR tone mapping other than SIGNAL is not supported by the HIP backend");
+    GailHipColor& col = s.color;
+    col.transfer = xyz ? 5 : (int)gc.transfer;
+    col.primaries = xyz ? 4 : (int)gc.primaries;
+    col.toneMap = (int)gc.tone_map;
+    col.toneNorm = (int)gc.tone_norm;
+    col.gamut = gc.gamut == GAIL_GAMUT_P3D65 ? 0 : gc.gamut == GAIL_GAMUT_DCIP3 ? 1 : 2;
+    col.sdrWhite = gc.sdr_white;
+    col.hdrPeak = gc.hdr_peak;
+    col.cinemaPeak = gc.cinema_peak;
+    col.hdrRefWhite = gc.hdr_ref_white;
+    col.cinemaRefWhite = gc.cinema_ref_white;
+    col.knee = gc.knee;
+    col.gamutThreshold = gc.gamut_threshold;
+    if (gc.mode == GAIL_COLOR_LUT3D && !xyz) {
+        if (!gc.lut3d || gc.lut3d_size < 2 || gc.lut3d_size > 256) throw ApiError(GAIL_ERROR_INVALID, "invalid 3D LUT");
+        const size_t n = (size_t)gc.lut3d_size * gc.lut3d_size * gc.lut3d_size * 3;
+        e->lut3d.assign(gc.lut3d, gc.lut3d + n);
+        col.lut3dSize = gc.lut3d_size;
+        col.lut3dData = e->lut3d.data();
+        col.lut3dCount = n;
+    }
     e->limits = dciLimits(e->eyeFps(), cfg.max_mbps, 200.0);
     if (emptyCodestreamBytes(e->L) > e->limits.maxTotal)
         throw ApiError(GAIL_ERROR_INVALID, "the frame budget is below the codestream headers");

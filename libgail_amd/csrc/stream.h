@@ -49,6 +49,7 @@ struct StreamSpec {
     int width = 0, height = 0, picW = 0, picH = 0, ox = 0, oy = 0;
     size_t inBytes = 0;
     double slopeFactor = 0.25;  // gail_api.cpp:607
+    std::vector<float> lut3d;   // the stream's copy of an app 3D LUT (settings point here)
 
     // Validates c and builds the stream (throws ApiError).
     static std::unique_ptr<StreamSpec> make(const gail_config& c);

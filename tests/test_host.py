@@ -86,6 +86,35 @@ def test_stream_configs_validate():
     assert st == -1 and "picture" in msg
 
 
+def test_color_settings_validate():
+    """HDR colour (tone maps, tone norms, target gamut, an app 3D LUT) is accepted for every
+    preset, and out-of-range colour parameters are refused with the reference API's checks
+    (gail_api.cpp:158-198), before any device is looked for."""
+    import ctypes as C
+    ok_codes = (0, -2)  # OK on a GPU, UNSUPPORTED (no device) here
+    lut = np.linspace(0, 1, 17 ** 3 * 3, dtype=np.float32)
+    for preset in range(4):
+        for tone_map in range(4):
+            for gamut in range(3):
+                c = g.stream_config("2k_flat", "gbrp12", picture=(1920, 1080), preset=preset)
+                c.color.tone_map, c.color.gamut, c.color.tone_norm = tone_map, gamut, tone_map % 3
+                st, msg = _encoder_create_status(c)
+                assert st in ok_codes, (preset, tone_map, gamut, st, msg)
+    c = g.stream_config("2k_flat", "gbrp12", picture=(1920, 1080))
+    c.color.mode, c.color.lut3d, c.color.lut3d_size = 1, lut.ctypes.data, 17
+    st, msg = _encoder_create_status(c)
+    assert st in ok_codes, msg
+    bad = [("knee", 1.0), ("knee", 0.0), ("gamut_threshold", 1.0), ("cinema_peak", 60.0), ("hdr_peak", 0.0),
+           ("gamut", 3), ("tone_map", 4), ("lut3d_size", 1)]
+    for field, value in bad:
+        c = g.stream_config("2k_flat", "gbrp12", picture=(1920, 1080))
+        if field == "lut3d_size":
+            c.color.mode, c.color.lut3d = 1, lut.ctypes.data
+        setattr(c.color, field, value)
+        st, msg = _encoder_create_status(c)
+        assert st == -1, (field, value, st, msg)
+
+
 def test_server_api_without_device():
     """The served form's entry points fail cleanly, with a message, where there is no GPU
     (or refuse bad arguments before touching one)."""
