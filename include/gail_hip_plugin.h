@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define GAIL_HIP_PLUGIN_ABI 1u
+#define GAIL_HIP_PLUGIN_ABI 2u
 #define GAIL_HIP_PLUGIN_ENTRY "gail_hip_plugin_get"
 
 /* GailCudaStatus (cuda_plugin.h:52-63). */
@@ -31,7 +31,7 @@ enum GailHipStatus {
     GAIL_HIP_HOST_MEMORY = 3,   /* std::bad_alloc */
     GAIL_HIP_MISUSE = 4,        /* std::logic_error */
     GAIL_HIP_NO_CODE = 5,       /* device_usable(): no code object for this GPU */
-    GAIL_HIP_UNSUPPORTED = 6    /* decoder entries: this backend encodes only */
+    GAIL_HIP_UNSUPPORTED = 6    /* a feature this backend does not have */
 };
 
 typedef void (*GailHipLog)(void* user, int level, const char* message);
@@ -39,6 +39,7 @@ typedef void (*GailHipLog)(void* user, int level, const char* message);
 typedef struct GailHipDevice GailHipDevice;
 typedef struct GailHipEncoder GailHipEncoder;
 typedef struct GailHipBatch GailHipBatch;
+typedef struct GailHipDecoder GailHipDecoder;
 
 /* GailCudaDeviceDesc (cuda_plugin.h:72-79); gfx arch instead of compute capability. */
 typedef struct GailHipDeviceDesc {
@@ -151,6 +152,32 @@ typedef struct GailHipDebug {
     uint64_t numXyz12, numCoeffs, numWavelet;
 } GailHipDebug;
 
+/* GailCudaDecodeSettings (cuda_plugin.h:108-110), gail::DecodeSettings (decode_backend.h:20-23):
+ * reduce = resolution levels left out (the picture is 1/2^reduce of the size); highBits != 0
+ * puts each 12-bit code in the high bits of its 16-bit word, else in the low bits. */
+typedef struct GailHipDecodeSettings {
+    int32_t reduce, highBits;
+} GailHipDecodeSettings;
+
+/* gail::CblkCode (t2_parse.h:17-22), 12 bytes: a code-block's bytes within the codestream,
+ * its coding passes (0: nothing coded) and magnitude bit-planes. */
+typedef struct GailHipCblkCode {
+    uint32_t offset, len;
+    uint8_t passes, nbps, pad[2];
+} GailHipCblkCode;
+
+/* GailCudaDecodeTimings (cuda_plugin.h:137-142): milliseconds of device time for codestream
+ * upload + Tier-1 + dequantisation, the inverse DWT, the inverse ICT + picture download, and
+ * the call's wall time. wantWavelet is set by the caller; wavelet[c] then points to the
+ * dequantised planes before the inverse DWT (numWavelet samples each), valid until the next
+ * call on the decoder. */
+typedef struct GailHipDecodeTimings {
+    double t1, dwt, color, total;
+    int32_t wantWavelet;
+    const float* wavelet[3];
+    uint64_t numWavelet;
+} GailHipDecodeTimings;
+
 /* Tuning of a batched job (no counterpart in the CUDA plugin). Zero fields take defaults. */
 typedef struct GailHipBatchOptions {
     int32_t dwtTile;      /* 0 64x64, 1 64x32, 2 32x64, 3 32x32 */
@@ -198,12 +225,21 @@ struct GailHipPlugin {
     int (*encoder_encode)(GailHipEncoder* e, const void* frame, GailHipTier1* out, GailHipFrameTimings* timings,
                           GailHipDebug* debug);
 
-    /* decoder entries (cuda_plugin.h:185-191): GAIL_HIP_UNSUPPORTED */
-    uint64_t (*decoder_bytes)(GailHipDevice* d, const GailHipLayout* layout, const void* settings);
-    int (*decoder_create)(GailHipDevice* d, const GailHipLayout* layout, const void* settings, void** out);
-    void (*decoder_destroy)(void* dec);
-    int (*decoder_decode)(void* dec, const GailHipLayout* frame, const uint8_t* data, size_t size, const void* blocks,
-                          size_t numBlocks, void* dst, ptrdiff_t strideBytes, void* timings);
+    /* decoder slots (cuda_plugin.h:183-190, the slot of decode_backend.h:42-51): one frame at
+     * a time per decoder; decoders on other threads run concurrently. decoder_decode takes
+     * the frame's layout (the decoder's geometry with the frame's own step sizes), the
+     * codestream (host or device memory) and one GailHipCblkCode per code-block of the
+     * layout; blocks with no passes or bytes past `size` decode as zero. dst receives the
+     * decoded picture (the layout's size halved `reduce` times, rounding up) as three 16-bit
+     * words per pixel (X', Y', Z'), rows strideBytes apart, in host or device memory. The
+     * call returns when the picture is there. timings may be NULL. */
+    uint64_t (*decoder_bytes)(GailHipDevice* d, const GailHipLayout* layout, const GailHipDecodeSettings* settings);
+    int (*decoder_create)(GailHipDevice* d, const GailHipLayout* layout, const GailHipDecodeSettings* settings,
+                          GailHipDecoder** out);
+    void (*decoder_destroy)(GailHipDecoder* dec);
+    int (*decoder_decode)(GailHipDecoder* dec, const GailHipLayout* frame, const uint8_t* data, size_t size,
+                          const GailHipCblkCode* blocks, size_t numBlocks, void* dst, ptrdiff_t strideBytes,
+                          GailHipDecodeTimings* timings);
 
     /* ---- batched form (no counterpart in the CUDA plugin) -------------------------------
      * A batch is a fixed set of frame slots of any mix of geometries and input formats,
@@ -286,7 +322,8 @@ static inline uint64_t gail_hip_plugin_layout(void) {
                               sizeof(GailHipFrameFormat), sizeof(GailHipColor),       sizeof(GailHipEncodeSettings),
                               sizeof(GailHipMemory),      sizeof(GailHipCblkInfo),    sizeof(GailHipTier1),
                               sizeof(GailHipFrameTimings), sizeof(GailHipDebug),      sizeof(GailHipBatchOptions),
-                              sizeof(GailHipSlot),        sizeof(GailHipRate),        sizeof(GailHipPlugin)};
+                              sizeof(GailHipSlot),        sizeof(GailHipRate),        sizeof(GailHipDecodeSettings),
+                              sizeof(GailHipCblkCode),    sizeof(GailHipDecodeTimings), sizeof(GailHipPlugin)};
     uint64_t h = 1469598103934665603ull;
     for (size_t i = 0; i < sizeof sizes / sizeof sizes[0]; ++i) h = (h ^ sizes[i]) * 1099511628211ull;
     return h;

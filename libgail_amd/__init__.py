@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(HERE, "lib")
 PLUGIN_PATH = os.path.join(LIB_DIR, "libgail_hip.so")
 HOST_PATH = os.path.join(LIB_DIR, "libgail.so")
-ABI = 1
+ABI = 2
 MAX_PASSES = 64
 
 
@@ -116,6 +116,24 @@ class Slot(C.Structure):
     _fields_ = [("maxWidth", C.c_int32), ("maxHeight", C.c_int32), ("colour", C.c_int32), ("reserved", C.c_int32)]
 
 
+class DecodeSettings(C.Structure):
+    _fields_ = [("reduce", C.c_int32), ("highBits", C.c_int32)]
+
+
+class CblkCode(C.Structure):
+    _fields_ = [("offset", C.c_uint32), ("len", C.c_uint32), ("passes", C.c_uint8), ("nbps", C.c_uint8),
+                ("pad", C.c_uint8 * 2)]
+
+
+# gail::CblkCode as a numpy record (12 bytes, GailHipCblkCode)
+CBLK_CODE = np.dtype([("offset", "<u4"), ("len", "<u4"), ("passes", "u1"), ("nbps", "u1"), ("pad", "u1", 2)])
+
+
+class DecodeTimings(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("t1", "dwt", "color", "total")] + [
+        ("wantWavelet", C.c_int32), ("wavelet", C.POINTER(C.c_float) * 3), ("numWavelet", C.c_uint64)]
+
+
 _vp = C.c_void_p
 _F = C.CFUNCTYPE
 
@@ -138,11 +156,11 @@ class Plugin(C.Structure):
         ("encoder_input", _F(_vp, _vp)),
         ("encoder_set_slope", _F(None, _vp, C.c_double)),
         ("encoder_encode", _F(C.c_int, _vp, _vp, C.POINTER(Tier1), C.POINTER(FrameTimings), C.POINTER(Debug))),
-        ("decoder_bytes", _F(C.c_uint64, _vp, C.POINTER(Layout), _vp)),
-        ("decoder_create", _F(C.c_int, _vp, C.POINTER(Layout), _vp, C.POINTER(_vp))),
+        ("decoder_bytes", _F(C.c_uint64, _vp, C.POINTER(Layout), C.POINTER(DecodeSettings))),
+        ("decoder_create", _F(C.c_int, _vp, C.POINTER(Layout), C.POINTER(DecodeSettings), C.POINTER(_vp))),
         ("decoder_destroy", _F(None, _vp)),
         ("decoder_decode", _F(C.c_int, _vp, C.POINTER(Layout), _vp, C.c_size_t, _vp, C.c_size_t, _vp, C.c_ssize_t,
-                              _vp)),
+                              C.POINTER(DecodeTimings))),
         ("batch_bytes", _F(C.c_uint64, _vp, C.POINTER(Layout), C.POINTER(EncodeSettings), C.c_int,
                            C.POINTER(BatchOptions))),
         ("batch_create", _F(C.c_int, _vp, C.POINTER(Layout), C.POINTER(EncodeSettings), C.c_int,
@@ -361,6 +379,70 @@ class Encoder:
     def close(self):
         if self.handle:
             plugin().encoder_destroy(self.handle)
+            self.handle = None
+
+
+def decoded_size(layout, reduce):
+    """decodedSize (decode_backend.h:33-40): the picture of a layout with `reduce` levels
+    left out."""
+    w, h = layout.width, layout.height
+    for _ in range(reduce):
+        w, h = (w + 1) // 2, (h + 1) // 2
+    return w, h
+
+
+class Decoder:
+    """One frame at a time (the reference's DecoderSlot) through the plugin's decoder entries:
+    code-block table + codestream in, X'Y'Z' picture (three 16-bit words per pixel) out."""
+
+    def __init__(self, device, layout, reduce=0, high_bits=True):
+        self.layout, self.reduce = layout, reduce
+        self.settings = DecodeSettings(reduce, 1 if high_bits else 0)
+        self._plain = layout.plain()
+        self.handle = _vp()
+        _check(plugin().decoder_create(device.handle, C.byref(self._plain), C.byref(self.settings),
+                                       C.byref(self.handle)), "decoder_create")
+
+    @staticmethod
+    def bytes_for(device, layout, reduce=0):
+        pl = layout.plain()
+        return plugin().decoder_bytes(device.handle, C.byref(pl), C.byref(DecodeSettings(reduce, 1)))
+
+    def decode(self, data, blocks, frame_layout=None, out=None, stride_bytes=None, wavelet=False):
+        """Decodes one frame. data: codestream bytes (numpy uint8 or an int device address
+        with its size as (address, size)); blocks: CBLK_CODE records, one per code-block.
+        Returns (picture as uint16 rows of 3 x width words, timings dict, wavelet planes or
+        None). out / stride_bytes: a host array or (device address, pitch) to decode into."""
+        fl = (frame_layout or self.layout).plain()
+        if isinstance(data, tuple):
+            ptr, size = data
+        else:
+            data = np.ascontiguousarray(data, dtype=np.uint8)
+            ptr, size = data.ctypes.data, data.size
+        blocks = np.ascontiguousarray(blocks, dtype=CBLK_CODE)
+        w, h = decoded_size(self.layout, self.reduce)
+        if out is None:
+            out = np.zeros((h, w * 3), np.uint16)
+            dst, pitch = out.ctypes.data, w * 6
+        elif isinstance(out, tuple):
+            dst, pitch = out
+        else:
+            dst, pitch = out.ctypes.data, stride_bytes or out.strides[0]
+        tm = DecodeTimings()
+        tm.wantWavelet = 1 if wavelet else 0
+        _check(plugin().decoder_decode(self.handle, C.byref(fl), _vp(ptr), size, blocks.ctypes.data_as(_vp),
+                                       len(blocks), _vp(dst), pitch, C.byref(tm)), "decoder_decode")
+        wav = None
+        if wavelet:
+            n = tm.numWavelet
+            wav = np.stack([np.ctypeslib.as_array(tm.wavelet[c], (n,)).copy() for c in range(3)])
+            wav = wav.reshape(3, self.layout.height, self.layout.width)
+        times = {k: getattr(tm, k) for k in ("t1", "dwt", "color", "total")}
+        return (out if not isinstance(out, tuple) else None), times, wav
+
+    def close(self):
+        if self.handle:
+            plugin().decoder_destroy(self.handle)
             self.handle = None
 
 

@@ -1,0 +1,329 @@
+// Decoder slot of the HIP backend (decoder.h). Host code compiled by hipcc.
+#include "decoder.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "engine.h"
+#include "errors.h"
+
+namespace gail_amd {
+
+namespace {
+
+// Bytes a lane may read past a codeword's end (the 16-byte window, aligned down to 8).
+constexpr size_t kBytePad = 64;
+// Length classes of the block ordering (bytes / 4, longer blocks share the top class).
+constexpr uint32_t kLenKeys = 4096;
+
+bool onDevice(const void* p) {
+    hipPointerAttribute_t a{};
+    if (!p || hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice;
+}
+
+double msSince(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+uint64_t Decoder::bytesFor(const Layout& L, int reduce) {
+    int w = L.width, h = L.height;
+    for (int i = 0; i < reduce; ++i) {
+        w = (w + 1) / 2;
+        h = (h + 1) / 2;
+    }
+    const uint64_t npix = (uint64_t)L.width * L.height;
+    // words + two float plane sets + the packed picture + codestream and block table room
+    return 3 * npix * 4 * 3 + (uint64_t)w * h * 6 + (4u << 20);
+}
+
+Decoder::Decoder(int device, const Layout& L, int reduce, bool highBits)
+    : dev_(device), L_(L), reduce_(reduce), highBits_(highBits) {
+    if (reduce < 0 || reduce > L.levels) throw std::runtime_error("more resolution levels left out than the stream has");
+    try {
+        hipCheck(hipSetDevice(dev_), "hipSetDevice");
+        cus_ = DeviceContext::get(dev_).computeUnits();
+        ws_.resize(L_.levels + 1);
+        hs_.resize(L_.levels + 1);
+        ws_[0] = L_.width;
+        hs_[0] = L_.height;
+        for (int l = 1; l <= L_.levels; ++l) {
+            ws_[l] = (ws_[l - 1] + 1) / 2;
+            hs_[l] = (hs_[l - 1] + 1) / 2;
+        }
+        outW_ = ws_[reduce];
+        outH_ = hs_[reduce];
+        for (int i = 0; i < L_.numCblks(); ++i)
+            if (L_.bands[L_.cblks[i].band].res <= L_.levels - reduce) wanted_.push_back(i);
+        hipCheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream");
+        for (auto& e : ev_) hipCheck(hipEventCreate(&e), "event");
+        const size_t npix = (size_t)L_.width * L_.height;
+        hipCheck(hipMalloc(&dWords_, 3 * npix * sizeof(uint32_t)), "decoder words");
+        hipCheck(hipMemset(dWords_, 0, 3 * npix * sizeof(uint32_t)), "decoder words");
+        for (auto& p : dPlane_) hipCheck(hipMalloc(&p, 3 * npix * sizeof(float)), "decoder planes");
+        const size_t outBytes = (size_t)outW_ * outH_ * 6;
+        hipCheck(hipMalloc(&dOut_, outBytes), "decoder picture");
+        hipCheck(hipHostMalloc((void**)&hOut_, outBytes, hipHostMallocDefault), "decoder picture (pinned)");
+        ensureBytes(1500000);  // a DCI frame is at most 250 Mbit/s over 24 frames
+        ensureDescs(((size_t)wanted_.size() / kDecLanes + 1) * kDecLanes);
+    } catch (...) {
+        release();
+        throw;
+    }
+}
+
+Decoder::~Decoder() { release(); }
+
+void Decoder::release() {
+    (void)hipSetDevice(dev_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (void* p : {(void*)dWords_, (void*)dPlane_[0], (void*)dPlane_[1], (void*)dWave_, (void*)dOut_, (void*)dBytes_,
+                    (void*)dDescs_})
+        if (p) (void)hipFree(p);
+    for (void* p : {(void*)hOut_, (void*)hBytes_, (void*)hDescs_})
+        if (p) (void)hipHostFree(p);
+    for (auto& e : ev_)
+        if (e) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamDestroy(stream_);
+    dWords_ = nullptr;
+    dPlane_[0] = dPlane_[1] = nullptr;
+    dWave_ = nullptr;
+    dOut_ = hOut_ = nullptr;
+    dBytes_ = hBytes_ = nullptr;
+    dDescs_ = hDescs_ = nullptr;
+    stream_ = nullptr;
+    (void)hipGetLastError();
+}
+
+void Decoder::ensureBytes(size_t n) {
+    if (n <= capBytes_) return;
+    if (dBytes_) (void)hipFree(dBytes_);
+    if (hBytes_) (void)hipHostFree(hBytes_);
+    dBytes_ = hBytes_ = nullptr;
+    capBytes_ = 0;
+    const size_t cap = n + n / 2;
+    hipCheck(hipMalloc(&dBytes_, cap + kBytePad), "codestream buffer");
+    hipCheck(hipMemset(dBytes_, 0, cap + kBytePad), "codestream buffer");
+    hipCheck(hipHostMalloc((void**)&hBytes_, cap, hipHostMallocDefault), "codestream buffer (pinned)");
+    capBytes_ = cap;
+}
+
+void Decoder::ensureDescs(size_t n) {
+    if (n <= capDescs_) return;
+    if (dDescs_) (void)hipFree(dDescs_);
+    if (hDescs_) (void)hipHostFree(hDescs_);
+    dDescs_ = hDescs_ = nullptr;
+    capDescs_ = 0;
+    const size_t cap = n + n / 2;
+    hipCheck(hipMalloc(&dDescs_, cap * sizeof(DecBlock)), "block table");
+    hipCheck(hipHostMalloc((void**)&hDescs_, cap * sizeof(DecBlock), hipHostMallocDefault), "block table (pinned)");
+    capDescs_ = cap;
+}
+
+uint32_t Decoder::buildDescs(const Layout& frame, const uint8_t* coded, const DecCode* blocks, size_t size) {
+    (void)frame;
+    (void)size;
+    // Coded blocks longest first (a counting sort over length classes).
+    count_.assign(kLenKeys + 1, 0u);
+    uint64_t total = 0;
+    size_t ncoded = 0;
+    auto key = [](uint32_t len) { return kLenKeys - 1u - std::min(len >> 2, kLenKeys - 1u); };
+    for (size_t k = 0; k < wanted_.size(); ++k)
+        if (coded[k]) {
+            ++count_[key(blocks[wanted_[k]].len) + 1];
+            total += blocks[wanted_[k]].len;
+            ++ncoded;
+        }
+    for (uint32_t k = 0; k < kLenKeys; ++k) count_[k + 1] += count_[k];
+    order_.resize(ncoded);
+    for (size_t k = 0; k < wanted_.size(); ++k)
+        if (coded[k]) order_[count_[key(blocks[wanted_[k]].len)]++] = (uint32_t)wanted_[k];
+    // Waves: a wave runs as long as its longest lane, so a long block takes a wave of its own
+    // and short ones share, up to a byte budget that spreads the frame over about eight
+    // waves per compute unit.
+    const uint64_t budget = std::max<uint64_t>(256, total / (uint64_t)(8 * cus_));
+    auto pack = [&](DecBlock* out) {
+        uint32_t n = 0, inWave = 0;
+        uint64_t waveBytes = 0;
+        for (uint32_t i : order_) {
+            const DecCode& c = blocks[i];
+            if (inWave && (inWave == (uint32_t)kDecLanes || waveBytes + c.len > budget)) {
+                if (out)
+                    for (; inWave < (uint32_t)kDecLanes; ++inWave) out[n++] = DecBlock{0, 0, 0, kDecEmpty};
+                else
+                    n += (uint32_t)kDecLanes - inWave;
+                inWave = 0;
+                waveBytes = 0;
+            }
+            if (out) {
+                const CodeBlock& cb = L_.cblks[i];
+                const Subband& b = L_.bands[cb.band];
+                DecBlock d;
+                d.off = c.offset;
+                d.len = c.len;
+                d.geom = (uint32_t)(cb.w - 1) | ((uint32_t)(cb.h - 1) << 5) | ((uint32_t)b.orient << 10) |
+                         ((uint32_t)c.nbps << 12) | ((uint32_t)c.passes << 18);
+                d.dst = (uint32_t)b.comp * (uint32_t)L_.width * (uint32_t)L_.height +
+                        (uint32_t)(b.py0 + cb.y0) * (uint32_t)L_.width + (uint32_t)(b.px0 + cb.x0);
+                out[n] = d;
+            }
+            ++n;
+            ++inWave;
+            waveBytes += c.len;
+        }
+        if (inWave) {
+            if (out)
+                for (; inWave < (uint32_t)kDecLanes; ++inWave) out[n++] = DecBlock{0, 0, 0, kDecEmpty};
+            else
+                n += (uint32_t)kDecLanes - inWave;
+        }
+        return n;
+    };
+    const uint32_t n = pack(nullptr);
+    ensureDescs(n);
+    pack(hDescs_);
+    return n;
+}
+
+void Decoder::decode(const Layout& frame, const uint8_t* data, size_t size, const DecCode* blocks, size_t n, void* dst,
+                     ptrdiff_t strideBytes, bool wantWavelet, DecodeTimes* tm) {
+    if (frame.width != L_.width || frame.height != L_.height || frame.levels != L_.levels ||
+        frame.bands.size() != L_.bands.size() || frame.numCblks() != L_.numCblks() || n != (size_t)L_.numCblks())
+        throw std::logic_error("frame does not match the layout the decoder was made for");
+    if (!dst || (size && !data) || (n && !blocks)) throw std::logic_error("null codestream, block table or picture");
+    if (size > 0xFFFFFFFFull - kBytePad) throw std::runtime_error("codestream too large");
+    const ptrdiff_t rowBytes = (ptrdiff_t)outW_ * 6;
+    if (strideBytes < rowBytes && strideBytes > -rowBytes) throw std::logic_error("row stride shorter than a picture row");
+    const auto t0 = std::chrono::steady_clock::now();
+    hipCheck(hipSetDevice(dev_), "hipSetDevice");
+    const hipStream_t s = stream_;
+    const uint32_t W = (uint32_t)L_.width, H = (uint32_t)L_.height;
+    const size_t npix = (size_t)W * H;
+
+    // Blocks decoded: of the resolutions kept, with passes, bit-planes the block coder carries
+    // and bytes inside the codestream; the rest stay zero.
+    std::vector<uint8_t> coded(wanted_.size());
+    for (size_t k = 0; k < wanted_.size(); ++k) {
+        const DecCode& c = blocks[wanted_[k]];
+        coded[k] = c.passes && c.nbps >= 1 && c.nbps <= 30 && (size_t)c.offset + c.len <= size;
+    }
+    const uint32_t nd = buildDescs(frame, coded.data(), blocks, size);
+    const bool devIn = onDevice(data), devOut = onDevice(dst);
+    ensureBytes(std::max<size_t>(size, 1));
+    hipCheck(hipEventRecord(ev_[0], s), "event");
+    if (size) {
+        if (devIn) {
+            hipCheck(hipMemcpyAsync(dBytes_, data, size, hipMemcpyDeviceToDevice, s), "codestream copy");
+        } else {
+            std::memcpy(hBytes_, data, size);
+            hipCheck(hipMemcpyAsync(dBytes_, hBytes_, size, hipMemcpyHostToDevice, s), "codestream upload");
+        }
+    }
+    if (nd) hipCheck(hipMemcpyAsync(dDescs_, hDescs_, (size_t)nd * sizeof(DecBlock), hipMemcpyHostToDevice, s), "block table upload");
+    hipCheck(hipMemsetAsync(dWords_, 0, 3 * npix * sizeof(uint32_t), s), "clear words");
+    T1DecodeParams tp;
+    tp.bytes = dBytes_;
+    tp.blocks = dDescs_;
+    tp.nblocks = nd;
+    tp.words = dWords_;
+    tp.nwords = (uint32_t)(3 * npix);
+    tp.stride = W;
+    hipCheck(launch_t1_decode(tp, s), "Tier-1 decode");
+
+    // half the step of every band, from this frame's own step sizes
+    auto half = [&](int comp, int level, int orient) {
+        const int res = orient == 0 ? 0 : L_.levels - level + 1;
+        return 0.5f * frame.bands[L_.bandIndex(comp, res, orient)].delta;
+    };
+    const bool dequantAll = wantWavelet || reduce_ == L_.levels;
+    if (dequantAll) {
+        if (wantWavelet && !dWave_) hipCheck(hipMalloc(&dWave_, 3 * npix * sizeof(float)), "wavelet planes");
+        DequantFrame dq{};
+        dq.words = dWords_;
+        dq.dst = wantWavelet ? dWave_ : dPlane_[0];
+        dq.width = W;
+        dq.height = H;
+        dq.levels = (uint32_t)L_.levels;
+        for (int c = 0; c < 3; ++c)
+            for (int l = 1; l <= L_.levels; ++l)
+                for (int o = 0; o < 4; ++o) dq.half[c][l][o] = (o == 0 && l != L_.levels) ? 0.f : half(c, l, o);
+        hipCheck(launch_dequant_frame(dq, s), "dequantisation");
+        if (reduce_ == L_.levels && wantWavelet) {
+            dq.dst = dPlane_[0];
+            hipCheck(launch_dequant_frame(dq, s), "dequantisation");
+        }
+    }
+    hipCheck(hipEventRecord(ev_[1], s), "event");
+
+    // inverse DWT, the top level first; each level's output is the next one's LL
+    const float* ll = reduce_ == L_.levels ? dPlane_[0] : nullptr;
+    int cur = 0;
+    int tx, ty;
+    idwt_tile_size(tx, ty);
+    for (int l = L_.levels; l > reduce_; --l) {
+        IdwtLevel p{};
+        p.words = dWords_;
+        p.ll = ll;
+        p.dst = dPlane_[cur];
+        p.stride = W;
+        p.plane = (uint32_t)npix;
+        p.w = ws_[l - 1];
+        p.h = hs_[l - 1];
+        p.tiles_x = (p.w + tx - 1) / tx;
+        p.tiles = p.tiles_x * ((p.h + ty - 1) / ty);
+        for (int c = 0; c < 3; ++c)
+            for (int o = 0; o < 4; ++o) p.half[c][o] = (o == 0 && l != L_.levels) ? 0.f : half(c, l, o);
+        hipCheck(launch_idwt_level(p, s), "inverse DWT");
+        ll = dPlane_[cur];
+        cur ^= 1;
+    }
+    hipCheck(hipEventRecord(ev_[2], s), "event");
+
+    IctPack ip;
+    ip.planes = ll;
+    ip.stride = W;
+    ip.plane = (uint32_t)npix;
+    ip.w = outW_;
+    ip.h = outH_;
+    ip.out = devOut ? (uint8_t*)dst : (uint8_t*)dOut_;
+    ip.pitch = devOut ? (int64_t)strideBytes : (int64_t)rowBytes;
+    ip.shift = highBits_ ? 4 : 0;
+    hipCheck(launch_ict_pack(ip, s), "inverse ICT");
+    const size_t outBytes = (size_t)outW_ * outH_ * 6;
+    if (!devOut) hipCheck(hipMemcpyAsync(hOut_, dOut_, outBytes, hipMemcpyDeviceToHost, s), "picture download");
+    hipCheck(hipEventRecord(ev_[3], s), "event");
+    if (wantWavelet) {
+        hWave_.resize(3 * npix);
+        hipCheck(hipMemcpyAsync(hWave_.data(), dWave_, 3 * npix * sizeof(float), hipMemcpyDeviceToHost, s), "wavelet download");
+    }
+    hipCheck(hipStreamSynchronize(s), "decode");
+    hipCheck(hipGetLastError(), "decode");
+    if (!devOut) {
+        if (strideBytes == rowBytes) {
+            std::memcpy(dst, hOut_, outBytes);
+        } else {
+            for (int y = 0; y < outH_; ++y)
+                std::memcpy((uint8_t*)dst + (ptrdiff_t)y * strideBytes, (const uint8_t*)hOut_ + (size_t)y * rowBytes, rowBytes);
+        }
+    }
+    if (tm) {
+        float a = 0, b = 0, c = 0;
+        (void)hipEventElapsedTime(&a, ev_[0], ev_[1]);
+        (void)hipEventElapsedTime(&b, ev_[1], ev_[2]);
+        (void)hipEventElapsedTime(&c, ev_[2], ev_[3]);
+        tm->t1 = a;
+        tm->dwt = b;
+        tm->color = c;
+        tm->total = msSince(t0);
+    }
+}
+
+}  // namespace gail_amd

@@ -1,0 +1,84 @@
+// Decoder slot of the HIP backend: one frame's located code-blocks (what a Tier-2 reader
+// hands over) to X'Y'Z' pixels on one device, the counterpart of the reference's CUDA
+// decode slot (cuda_decode.cu:131-256). Header and packet parsing stay with the caller.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "kernels_dec.h"
+#include "layout.h"
+
+namespace gail_amd {
+
+// gail::CblkCode (t2_parse.h:17-22): a code-block's bytes within the codestream, its coding
+// passes (0: nothing coded) and its magnitude bit-planes.
+struct DecCode {
+    uint32_t offset, len;
+    uint8_t passes, nbps, pad[2];
+};
+
+struct DecodeTimes {
+    double t1 = 0, dwt = 0, color = 0, total = 0;  // ms; t1 includes the codestream upload
+};
+
+class Decoder {
+public:
+    // reduce: resolution levels left out (0..levels); highBits: codes << 4 in each word.
+    Decoder(int device, const Layout& L, int reduce, bool highBits);
+    ~Decoder();
+    Decoder(const Decoder&) = delete;
+    Decoder& operator=(const Decoder&) = delete;
+
+    // Device memory a decoder for L takes (gpuDecoderSlotBytes, decode_backend.h:58-63,
+    // with this decoder's buffers).
+    static uint64_t bytesFor(const Layout& L, int reduce);
+
+    // frame: the layout of this frame (the geometry the decoder was made for, with the step
+    // sizes of the frame's own headers). data: the codestream (host or device memory) the
+    // blocks point into; blocks: one per code-block of the layout. dst: outWidth() x
+    // outHeight() pixels of three 16-bit words, rows strideBytes apart, in host or device
+    // memory. wantWavelet keeps the dequantised planes for wavelet().
+    void decode(const Layout& frame, const uint8_t* data, size_t size, const DecCode* blocks, size_t n, void* dst,
+                ptrdiff_t strideBytes, bool wantWavelet, DecodeTimes* tm);
+
+    int outWidth() const { return outW_; }
+    int outHeight() const { return outH_; }
+    // Plane c of the dequantised coefficients of the last decode(wantWavelet) (W x H floats).
+    const float* wavelet(int c) const { return hWave_.data() + (size_t)c * L_.width * L_.height; }
+    size_t waveletSamples() const { return (size_t)L_.width * L_.height; }
+
+private:
+    void release();
+    void ensureBytes(size_t n);
+    void ensureDescs(size_t n);
+    // Orders the coded blocks longest first and packs them into waves; returns the entries.
+    uint32_t buildDescs(const Layout& frame, const uint8_t* validity, const DecCode* blocks, size_t size);
+
+    int dev_;
+    Layout L_;
+    int reduce_;
+    bool highBits_;
+    int outW_ = 0, outH_ = 0;
+    std::vector<int> ws_, hs_, wanted_;
+    std::vector<uint32_t> order_, count_;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev_[4] = {};
+    uint32_t* dWords_ = nullptr;
+    float* dPlane_[2] = {};
+    float* dWave_ = nullptr;
+    uint16_t* dOut_ = nullptr;
+    uint16_t* hOut_ = nullptr;
+    uint8_t* dBytes_ = nullptr;
+    uint8_t* hBytes_ = nullptr;
+    size_t capBytes_ = 0;
+    DecBlock* dDescs_ = nullptr;
+    DecBlock* hDescs_ = nullptr;
+    size_t capDescs_ = 0;
+    int cus_ = 256;
+    std::vector<float> hWave_;
+};
+
+}  // namespace gail_amd

@@ -10,6 +10,7 @@
 
 #include "codestream.h"
 #include "errors.h"
+#include "kernels_dec.h"
 
 namespace gail_amd {
 
@@ -51,6 +52,7 @@ DeviceContext::DeviceContext(int device) : dev_(device) {
     name_ = prop.name;
     arch_ = prop.gcnArchName;
     hipCheck(t1_tables_init(), "Tier-1 tables");
+    hipCheck(dec_tables_init(), "Tier-1 decoder tables");
     std::vector<float> t;
     buildLog2Table(t, kLog2Max);
     hipCheck(hipMalloc(&log2_, t.size() * sizeof(float)), "log2 table");

@@ -23,6 +23,7 @@
 
 #include "device_types.h"
 #include "kernels.h"
+#include "t1_tables.h"
 
 namespace gail_amd {
 
@@ -584,33 +585,8 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
 }
 
 static hipError_t upload_zc_table() {
-    // Table D.1 (ISO/IEC 15444-1) for LL/LH, HL (h and v swapped) and HH by (h, v, d) ...
-    uint8_t t[4 * 45];
-    for (int o = 0; o < 4; ++o)
-        for (int h = 0; h < 3; ++h)
-            for (int v = 0; v < 3; ++v)
-                for (int d = 0; d < 5; ++d) {
-                    int hh = h, vv = v, c;
-                    if (o == 1) {
-                        hh = v;
-                        vv = h;
-                    }
-                    if (o == 3) {
-                        const int hv = hh + vv;
-                        c = d >= 3 ? 8 : d == 2 ? (hv ? 7 : 6) : d == 1 ? (hv >= 2 ? 5 : hv ? 4 : 3) : (hv >= 2 ? 2 : hv);
-                    } else {
-                        c = hh == 2 ? 8 : hh == 1 ? (vv ? 7 : d ? 6 : 5) : vv ? (vv == 2 ? 4 : 3) : (d >= 2 ? 2 : d);
-                    }
-                    t[o * 45 + h * 15 + v * 5 + d] = (uint8_t)c;
-                }
-    // ... and by the 7-bit code of zc_codes (codes no neighbourhood produces clamp)
     uint8_t z[4 * 128];
-    for (int o = 0; o < 4; ++o)
-        for (int i = 0; i < 128; ++i) {
-            const int h = std::min(2, (i & 1) + 2 * ((i >> 1) & 1)), v = std::min(2, ((i >> 2) & 1) + 2 * ((i >> 3) & 1));
-            const int d = std::min(4, ((i >> 4) & 1) + 2 * ((i >> 5) & 1) + 4 * ((i >> 6) & 1));
-            z[o * 128 + i] = t[o * 45 + h * 15 + v * 5 + d];
-        }
+    build_zc_table(z);
     return hipMemcpyToSymbol(HIP_SYMBOL(kZcTable), z, sizeof z);
 }
 
@@ -1036,23 +1012,9 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
 }
 
 static hipError_t upload_mq_table() {
-    {
-        static const uint16_t qe[47] = {
-            0x5601, 0x3401, 0x1801, 0x0AC1, 0x0521, 0x0221, 0x5601, 0x5401, 0x4801, 0x3801, 0x3001, 0x2401,
-            0x1C01, 0x1601, 0x5601, 0x5401, 0x5101, 0x4801, 0x3801, 0x3401, 0x3001, 0x2801, 0x2401, 0x2201,
-            0x1C01, 0x1801, 0x1601, 0x1401, 0x1201, 0x1101, 0x0AC1, 0x09C1, 0x08A1, 0x0521, 0x0441, 0x02A1,
-            0x0221, 0x0141, 0x0111, 0x0085, 0x0049, 0x0025, 0x0015, 0x0009, 0x0005, 0x0001, 0x5601};
-        static const uint8_t nm[47] = {1,  2,  3,  4,  5,  38, 7,  8,  9,  10, 11, 12, 13, 29, 15, 16,
-                                       17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31, 32,
-                                       33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 44, 45, 45, 46};
-        static const uint8_t nl[47] = {1,  6,  9,  12, 29, 33, 6,  14, 14, 14, 17, 18, 20, 21, 14, 14,
-                                       15, 16, 17, 18, 19, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29,
-                                       30, 31, 32, 33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 46};
-        uint32_t tab[47];
-        for (int i = 0; i < 47; ++i)
-            tab[i] = qe[i] | ((uint32_t)nm[i] << 16) | ((uint32_t)nl[i] << 22) | ((i == 0 || i == 6 || i == 14) ? 1u << 28 : 0u);
-        return hipMemcpyToSymbol(HIP_SYMBOL(kMqTable), tab, sizeof tab);
-    }
+    uint32_t tab[47];
+    build_mq_table(tab);
+    return hipMemcpyToSymbol(HIP_SYMBOL(kMqTable), tab, sizeof tab);
 }
 
 hipError_t t1_tables_init() {

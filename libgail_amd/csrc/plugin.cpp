@@ -13,6 +13,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "decoder.h"
 #include "engine.h"
 #include "errors.h"
 #include "gail_hip_plugin.h"
@@ -31,6 +32,10 @@ struct GailHipEncoder {
     std::vector<uint16_t> dbgCodes;
     std::vector<float> dbgWavelet;
     std::vector<uint32_t> dbgCoeffs;
+};
+
+struct GailHipDecoder {
+    std::unique_ptr<Decoder> dec;
 };
 
 struct GailHipBatch {
@@ -79,7 +84,7 @@ const Layout& cachedGeometry(int width, int height, bool fourk, int levels, int 
     return *p;
 }
 
-Layout toLayout(const GailHipLayout& p) {
+Layout toLayout(const GailHipLayout& p, int maxMb = 22) {
     if (!p.bands || p.numBands <= 0) throw std::logic_error("frame layout is empty");
     if (p.width <= 0 || p.height <= 0 || p.width > 65535 || p.height > 65535 || p.levels < 1 || p.levels > 8)
         throw std::logic_error("frame layout size or levels out of range");
@@ -92,7 +97,8 @@ Layout toLayout(const GailHipLayout& p) {
         Subband& b = L.bands[i];
         if (s.comp != b.comp || s.res != b.res || s.orient != b.orient || s.w != b.w || s.h != b.h)
             throw std::logic_error("subband does not match the frame geometry");
-        if (s.Mb < 0 || s.Mb > 22) throw std::logic_error("subband with more than 22 magnitude bit-planes");
+        if (s.Mb < 0 || s.Mb > maxMb)
+            throw std::logic_error("subband with more than " + std::to_string(maxMb) + " magnitude bit-planes");
         b.eps = s.eps;
         b.mu = s.mu;
         b.Mb = s.Mb;
@@ -317,15 +323,57 @@ int encoderEncode(GailHipEncoder* e, const void* frame, GailHipTier1* out, GailH
     });
 }
 
-uint64_t decoderBytes(GailHipDevice*, const GailHipLayout*, const void*) { return 0; }
-int decoderCreate(GailHipDevice*, const GailHipLayout*, const void*, void**) {
-    g_error = "the HIP backend does not decode";
-    return GAIL_HIP_UNSUPPORTED;
+// The decoder carries up to 30 magnitude bit-planes per block (the block table says how many
+// each has); the subbands' own Mb only bound them.
+constexpr int kDecodeMaxMb = 31;
+
+void checkDecodeSettings(const GailHipDecodeSettings* s, const GailHipLayout* l) {
+    if (!s || !l) throw std::logic_error("null layout or decode settings");
+    if (s->reduce < 0 || s->reduce > l->levels) throw std::runtime_error("more resolution levels left out than the stream has");
 }
-void decoderDestroy(void*) {}
-int decoderDecode(void*, const GailHipLayout*, const uint8_t*, size_t, const void*, size_t, void*, ptrdiff_t, void*) {
-    g_error = "the HIP backend does not decode";
-    return GAIL_HIP_UNSUPPORTED;
+
+uint64_t decoderBytes(GailHipDevice*, const GailHipLayout* l, const GailHipDecodeSettings* s) {
+    uint64_t r = 0;
+    const int st = guarded([&] {
+        checkDecodeSettings(s, l);
+        r = Decoder::bytesFor(toLayout(*l, kDecodeMaxMb), s->reduce);
+    });
+    return st == GAIL_HIP_OK ? r : 0;
+}
+
+int decoderCreate(GailHipDevice* d, const GailHipLayout* l, const GailHipDecodeSettings* s, GailHipDecoder** out) {
+    return guarded([&] {
+        if (!d || !out) throw std::logic_error("null device or output");
+        *out = nullptr;
+        checkDecodeSettings(s, l);
+        auto g = std::make_unique<GailHipDecoder>();
+        g->dec = std::make_unique<Decoder>(d->index, toLayout(*l, kDecodeMaxMb), s->reduce, s->highBits != 0);
+        *out = g.release();
+    });
+}
+
+void decoderDestroy(GailHipDecoder* d) { delete d; }
+
+int decoderDecode(GailHipDecoder* d, const GailHipLayout* frame, const uint8_t* data, size_t size,
+                  const GailHipCblkCode* blocks, size_t numBlocks, void* dst, ptrdiff_t strideBytes,
+                  GailHipDecodeTimings* t) {
+    static_assert(sizeof(GailHipCblkCode) == sizeof(DecCode), "block record layout");
+    return guarded([&] {
+        if (!d || !frame) throw std::logic_error("null decoder or frame layout");
+        const Layout L = toLayout(*frame, kDecodeMaxMb);
+        DecodeTimes tm;
+        const bool wave = t && t->wantWavelet;
+        d->dec->decode(L, data, size, reinterpret_cast<const DecCode*>(blocks), numBlocks, dst, strideBytes, wave,
+                       t ? &tm : nullptr);
+        if (t) {
+            t->t1 = tm.t1;
+            t->dwt = tm.dwt;
+            t->color = tm.color;
+            t->total = tm.total;
+            for (int c = 0; c < 3; ++c) t->wavelet[c] = wave ? d->dec->wavelet(c) : nullptr;
+            t->numWavelet = wave ? d->dec->waveletSamples() : 0;
+        }
+    });
 }
 
 std::vector<FrameConfig> toFrames(const GailHipLayout* l, const GailHipEncodeSettings* s, int count) {

@@ -1,0 +1,322 @@
+// EBCOT Tier-1 decoding of one code-block (ISO/IEC 15444-1 Annexes C.3 and D, no mode
+// switches), written once for the GPU kernel (one lane per block) and for a host build
+// that checks it against the CPU decoder.
+//
+// The block's state is four sets of 32 row masks (bit x = column x): significant, negative,
+// coded in this plane's significance pass, refined before. A stripe's six significance rows
+// (the row above, its four rows, the row below) and sign rows sit in registers while the
+// stripe is coded; contexts come from counts of significant neighbours. Each decoded sample
+// adds to its reconstruction word: on becoming significant in plane p the sign in bit 31 and
+// 3 * 2^p (the bit and the mid-point of the planes below, doubled), on each refinement
+// +-2^p. The words start at zero, so the sum is the word of t1_dec.cpp:85-97, 117-127.
+//
+// The caller supplies:
+//   State  rows(t, r) / set_rows(t, r, v) for t = 0 significant, 1 negative, 2 visited,
+//          3 refined; ctx(cx) / set_ctx(cx, v) for the 19 context states
+//   Sink   add(i, v): add v to the word of sample i = row * stride + column
+// The codeword is read 8 bytes at a time: src must be readable from (start & ~7) up to
+// 16 bytes past start + len.
+#pragma once
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#define T1D_FN __host__ __device__ __forceinline__
+#else
+#define T1D_FN inline
+#endif
+
+namespace gail_amd {
+
+constexpr uint32_t kDecCtxRl = 17, kDecCtxUni = 18;
+
+T1D_FN uint32_t t1d_popc(uint32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (uint32_t)__popc(v);
+#else
+    return (uint32_t)__builtin_popcount(v);
+#endif
+}
+T1D_FN uint32_t t1d_lowest(uint32_t v) {  // index of the lowest set bit, v != 0
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (uint32_t)(__ffs(v) - 1);
+#else
+    return (uint32_t)__builtin_ctz(v);
+#endif
+}
+T1D_FN int t1d_clz(uint32_t v) {  // v != 0
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __clz((int)v);
+#else
+    return __builtin_clz(v);
+#endif
+}
+
+// Sign coding (Table D.3) by k = 3 * (hc + 1) + (vc + 1), hc / vc the clamped horizontal
+// and vertical contributions: four bits each, (context - 9) | flip << 3.
+constexpr uint64_t kSignTable = 0x432109ABCull;
+
+// MQ decoder (C.3): A, C and CT as in the standard; the codeword's bytes come from a
+// 16-byte window (cur, nxt) that slides in 8-byte steps; bytes at or past `end` read as
+// 0xFF (C.3.4: a codeword that has run out is continued with ones).
+struct MqDec {
+    const uint8_t* src;
+    uint64_t cur, nxt;
+    uint32_t base, bp, end;
+    uint32_t A, C;
+    int32_t CT;
+
+    T1D_FN uint64_t load8(uint32_t at) const { return *reinterpret_cast<const uint64_t*>(src + at); }
+    T1D_FN uint32_t byte(uint32_t a) const {
+        if (a >= end) return 0xFFu;
+        const uint32_t k = a - base;
+        return (uint32_t)((k < 8u ? cur : nxt) >> (8u * (k & 7u))) & 0xFFu;
+    }
+    // BYTEIN (C.3.4): after 0xFF the next byte brings 7 bits, and one above 0x8F is a marker,
+    // which stays unread and feeds ones.
+    T1D_FN void bytein() {
+        const uint32_t b = byte(bp), b1 = byte(bp + 1u);
+        if (b != 0xFFu) {
+            ++bp;
+            C += b1 << 8;
+            CT = 8;
+        } else if (b1 <= 0x8Fu) {
+            ++bp;
+            C += b1 << 9;
+            CT = 7;
+        } else {
+            C += 0xFF00u;
+            CT = 8;
+        }
+        if (bp - base >= 8u) {  // keep bp and bp + 1 inside the window
+            cur = nxt;
+            base += 8u;
+            nxt = load8(base + 8u);
+        }
+    }
+    // INITDEC (C.3.5) for the codeword in src[start, start + len).
+    T1D_FN void init(const uint8_t* s, uint32_t start, uint32_t len) {
+        src = s;
+        bp = start;
+        end = start + len;
+        base = start & ~7u;
+        cur = load8(base);
+        nxt = load8(base + 8u);
+        C = byte(bp) << 16;
+        bytein();
+        C <<= 7;
+        CT -= 7;
+        A = 0x8000u;
+    }
+    // DECODE (C.3.2) with RENORMD (C.3.3) in context cx; a context state is its Table C.2
+    // word (Qe | NMPS << 16 | NLPS << 22 | SWITCH << 28) with the MPS in bit 31.
+    template <class State> T1D_FN uint32_t decode(State& st, uint32_t cx, const uint32_t* tab) {
+        const uint32_t s = st.ctx(cx);
+        const uint32_t qe = s & 0xFFFFu, mps = s >> 31;
+        A -= qe;
+        uint32_t d;
+        if ((C >> 16) >= qe) {
+            C -= qe << 16;
+            if (A & 0x8000u) return mps;
+            d = A < qe ? mps ^ 1u : mps;  // MPS exchange
+        } else {
+            d = A < qe ? mps : mps ^ 1u;  // LPS exchange
+            A = qe;
+        }
+        if (d == mps) {
+            st.set_ctx(cx, tab[(s >> 16) & 63u] | (mps << 31));
+        } else {
+            st.set_ctx(cx, tab[(s >> 22) & 63u] | ((mps ^ ((s >> 28) & 1u)) << 31));
+        }
+        // shift until A >= 0x8000, as many bits at a time as C holds
+        int n = t1d_clz(A) - 16;
+        while (n > 0) {
+            if (CT == 0) bytein();
+            const int k = n < CT ? n : CT;
+            A <<= k;
+            C <<= k;
+            CT -= k;
+            n -= k;
+        }
+        return d;
+    }
+};
+
+// Zero-coding code of the sample in row k (0..3) of the stripe whose significance rows are
+// S[0] (above) .. S[5] (below), column x: h | v << 2 | d << 4 (counts).
+T1D_FN uint32_t t1d_zc_code(const uint32_t (&S)[6], int k, uint32_t x) {
+    const uint32_t bit = 1u << x;
+    const uint32_t side = (bit << 1) | (bit >> 1);  // columns x - 1 and x + 1
+    const uint32_t h = t1d_popc(S[k + 1] & side);
+    const uint32_t v = ((S[k] >> x) & 1u) + ((S[k + 2] >> x) & 1u);
+    const uint32_t d = t1d_popc(S[k] & side) + t1d_popc(S[k + 2] & side);
+    return h | (v << 2) | (d << 4);
+}
+
+// The sample (row k of the stripe, column x) is significant from plane p on: decode its
+// sign (D.3.2) and add 3 * 2^p with the sign to its word.
+template <class State, class Sink>
+T1D_FN void t1d_become_sig(MqDec& mq, State& st, const uint32_t* tab, Sink& out, uint32_t (&S)[6], uint32_t (&G)[6],
+                           int k, uint32_t x, uint32_t p, uint32_t row_off) {
+    const uint32_t bit = 1u << x;
+    const uint32_t left = x ? bit >> 1 : 0u, right = bit << 1;  // right is 0 for x = 31
+    // contributions: +1 significant positive, -1 significant negative
+    const int cl = (S[k + 1] & left) ? ((G[k + 1] & left) ? -1 : 1) : 0;
+    const int cr = (S[k + 1] & right) ? ((G[k + 1] & right) ? -1 : 1) : 0;
+    const int cu = (S[k] & bit) ? ((G[k] & bit) ? -1 : 1) : 0;
+    const int cd = (S[k + 2] & bit) ? ((G[k + 2] & bit) ? -1 : 1) : 0;
+    int hc = cl + cr, vc = cu + cd;
+    hc = hc < -1 ? -1 : (hc > 1 ? 1 : hc);
+    vc = vc < -1 ? -1 : (vc > 1 ? 1 : vc);
+    const uint32_t e = (uint32_t)(kSignTable >> (4 * (3 * (hc + 1) + (vc + 1)))) & 15u;
+    const uint32_t neg = mq.decode(st, 9u + (e & 7u), tab) ^ (e >> 3);
+    S[k + 1] |= bit;
+    if (neg) G[k + 1] |= bit;
+    out.add(row_off + (uint32_t)k * out.stride + x, (neg << 31) | (3u << p));
+}
+
+// Decodes the first `passes` coding passes of a w x h block (1..32 each) with nbps
+// magnitude bit-planes (1..30); passes past the last plane are not decoded.
+template <class State, class Sink>
+T1D_FN void t1d_decode_block(State& st, const uint8_t* src, uint32_t start, uint32_t len, uint32_t w, uint32_t h,
+                             uint32_t orient, uint32_t nbps, uint32_t passes, const uint8_t* zc, const uint32_t* tab,
+                             Sink& out) {
+    if (w == 0u || h == 0u || w > 32u || h > 32u || nbps == 0u || nbps > 30u || passes == 0u) return;
+    if (passes > 3u * nbps - 2u) passes = 3u * nbps - 2u;
+    const uint8_t* zco = zc + 128u * (orient & 3u);
+    for (uint32_t r = 0; r < 32u; ++r)
+        for (int t = 0; t < 4; ++t) st.set_rows(t, r, 0u);
+    for (uint32_t c = 0; c < 19u; ++c) st.set_ctx(c, tab[0]);
+    st.set_ctx(0, tab[4]);
+    st.set_ctx(kDecCtxRl, tab[3]);
+    st.set_ctx(kDecCtxUni, tab[46]);
+    MqDec mq;
+    mq.init(src, start, len);
+    const uint32_t wmask = w == 32u ? 0xFFFFFFFFu : (1u << w) - 1u;
+    uint32_t p = nbps - 1u;
+    uint32_t type = 2u;  // 0 significance propagation, 1 magnitude refinement, 2 cleanup
+    for (uint32_t pass = 0; pass < passes; ++pass) {
+        for (uint32_t y0 = 0; y0 < h; y0 += 4u) {
+            const uint32_t rows = h - y0 < 4u ? h - y0 : 4u;
+            uint32_t S[6], G[6], M[4];  // M: columns present in each row of the stripe
+            S[0] = y0 ? st.rows(0, y0 - 1u) : 0u;
+            G[0] = y0 ? st.rows(1, y0 - 1u) : 0u;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const bool in = (uint32_t)k < rows;
+                S[k + 1] = in ? st.rows(0, y0 + k) : 0u;
+                G[k + 1] = in ? st.rows(1, y0 + k) : 0u;
+                M[k] = in ? wmask : 0u;
+            }
+            S[5] = y0 + 4u < h ? st.rows(0, y0 + 4u) : 0u;
+            G[5] = y0 + 4u < h ? st.rows(1, y0 + 4u) : 0u;
+            const uint32_t row_off = y0 * out.stride;
+            if (type == 0u) {
+                // D.3.1: insignificant samples with a significant neighbour
+                uint32_t V[4] = {0u, 0u, 0u, 0u};
+                uint32_t todo = 0u;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t n = S[k] | S[k + 1] | S[k + 2];
+                    todo |= (n | (n << 1) | (n >> 1)) & ~S[k + 1] & M[k];
+                }
+                while (todo) {
+                    const uint32_t x = t1d_lowest(todo);
+                    const uint32_t bit = 1u << x;
+                    todo &= ~bit;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if (!(M[k] & bit) || (S[k + 1] & bit)) continue;
+                        const uint32_t code = t1d_zc_code(S, k, x);
+                        if (!code) continue;
+                        V[k] |= bit;
+                        if (mq.decode(st, zco[code], tab)) {
+                            t1d_become_sig(mq, st, tab, out, S, G, k, x, p, row_off);
+                            todo |= (bit << 1) & wmask;  // the next column gains a neighbour
+                        }
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if ((uint32_t)k < rows) {
+                        st.set_rows(0, y0 + k, S[k + 1]);
+                        st.set_rows(1, y0 + k, G[k + 1]);
+                        st.set_rows(2, y0 + k, V[k]);
+                    }
+            } else if (type == 1u) {
+                // D.3.3: samples significant before this plane's significance pass
+                uint32_t V[4], R[4];
+                uint32_t todo = 0u;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    V[k] = (uint32_t)k < rows ? st.rows(2, y0 + k) : 0u;
+                    R[k] = (uint32_t)k < rows ? st.rows(3, y0 + k) : 0u;
+                    todo |= S[k + 1] & ~V[k];
+                }
+                while (todo) {
+                    const uint32_t x = t1d_lowest(todo);
+                    const uint32_t bit = 1u << x;
+                    todo &= ~bit;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if (!(S[k + 1] & ~V[k] & bit)) continue;
+                        const uint32_t cx = (R[k] & bit) ? 16u : (t1d_zc_code(S, k, x) ? 15u : 14u);
+                        const uint32_t b = mq.decode(st, cx, tab);
+                        R[k] |= bit;
+                        out.add(row_off + (uint32_t)k * out.stride + x, b ? (1u << p) : (0u - (1u << p)));
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if ((uint32_t)k < rows) st.set_rows(3, y0 + k, R[k]);
+            } else {
+                // D.3.4: every sample neither significant nor coded in this plane yet
+                uint32_t U[4];
+                uint32_t todo = 0u;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    U[k] = (uint32_t)k < rows ? ~S[k + 1] & ~st.rows(2, y0 + k) & M[k] : 0u;
+                    todo |= U[k];
+                }
+                const uint32_t all4 = rows == 4u ? U[0] & U[1] & U[2] & U[3] : 0u;
+                while (todo) {
+                    const uint32_t x = t1d_lowest(todo);
+                    const uint32_t bit = 1u << x;
+                    todo &= ~bit;
+                    int k0 = 0;
+                    const uint32_t near = ((bit << 1) | bit | (bit >> 1));
+                    if ((all4 & bit) && !((S[0] | S[1] | S[2] | S[3] | S[4] | S[5]) & near)) {
+                        // run-length mode: four untouched samples without significant neighbours
+                        if (!mq.decode(st, kDecCtxRl, tab)) continue;
+                        k0 = (int)(mq.decode(st, kDecCtxUni, tab) << 1);
+                        k0 |= (int)mq.decode(st, kDecCtxUni, tab);
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            if (k == k0) t1d_become_sig(mq, st, tab, out, S, G, k, x, p, row_off);
+                        ++k0;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if (k < k0 || !(U[k] & bit)) continue;
+                        if (mq.decode(st, zco[t1d_zc_code(S, k, x)], tab))
+                            t1d_become_sig(mq, st, tab, out, S, G, k, x, p, row_off);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if ((uint32_t)k < rows) {
+                        st.set_rows(0, y0 + k, S[k + 1]);
+                        st.set_rows(1, y0 + k, G[k + 1]);
+                        st.set_rows(2, y0 + k, 0u);  // the plane is finished for these rows
+                    }
+            }
+        }
+        if (type == 2u) {
+            type = 0u;
+            --p;
+        } else {
+            ++type;
+        }
+    }
+}
+
+}  // namespace gail_amd

@@ -1158,6 +1158,7 @@ typedef struct {
     int w, h, orient, stride;
     uint8_t* f; /* T_SIG, T_SGN (decoded), T_VIS, T_REF */
     uint32_t* mag;
+    uint32_t* word; /* optional: the reconstruction words of t1_dec.cpp, w x h */
     mq_dec m;
 } t1d;
 
@@ -1181,10 +1182,12 @@ static void dsign(t1d* t, int x, int y, int p) {
     const uint32_t s = mq_decode(&t->m, cx) ^ (uint32_t)xr;
     *DF(t, x, y) |= (uint8_t)(T_SIG | (s ? T_SGN : 0));
     t->mag[y * t->w + x] |= 1u << p;
+    /* twice the magnitude so far plus the mid-point of the planes below: 2 * 2^p + 2^p */
+    if (t->word) t->word[y * t->w + x] = (s << 31) | (3u << p);
 }
 
-int or_t1_decode(const uint8_t* data, uint32_t len, int w, int h, int orient, int nbps, int passes,
-                 uint32_t* mag, uint8_t* sign, uint8_t* sig) {
+static int t1_decode_core(const uint8_t* data, uint32_t len, int w, int h, int orient, int nbps, int passes,
+                          uint32_t* mag, uint32_t* word, uint8_t* sign, uint8_t* sig) {
     t1d t;
     t.w = w;
     t.h = h;
@@ -1192,7 +1195,9 @@ int or_t1_decode(const uint8_t* data, uint32_t len, int w, int h, int orient, in
     t.stride = w + 2;
     t.f = (uint8_t*)calloc((size_t)(w + 2) * (h + 2), 1);
     t.mag = mag;
+    t.word = word;
     memset(mag, 0, sizeof(uint32_t) * (size_t)w * h);
+    if (word) memset(word, 0, sizeof(uint32_t) * (size_t)w * h);
     mq_dec_start(&t.m, data, len);
     int k = 0;
     for (int p = nbps - 1; p >= 0 && k < passes; --p) {
@@ -1228,7 +1233,10 @@ int or_t1_decode(const uint8_t* data, uint32_t len, int w, int h, int orient, in
                         } else if (type == 1) {
                             if (!(*f & T_SIG) || (*f & T_VIS)) continue;
                             const int cx = (*f & T_REF) ? 16 : ((a | b | c) ? 15 : 14);
-                            t.mag[y * w + x] |= mq_decode(&t.m, cx) << p;
+                            const uint32_t bit = mq_decode(&t.m, cx);
+                            t.mag[y * w + x] |= bit << p;
+                            /* the bit replaces the mid-point guess: +-2^p in the doubled value */
+                            if (t.word) t.word[y * w + x] += bit ? (1u << p) : (0u - (1u << p));
                             *f |= T_REF;
                         } else {
                             if (*f & (T_SIG | T_VIS)) continue;
@@ -1240,14 +1248,29 @@ int or_t1_decode(const uint8_t* data, uint32_t len, int w, int h, int orient, in
         }
         for (size_t i = 0; i < (size_t)(w + 2) * (h + 2); ++i) t.f[i] &= (uint8_t)~T_VIS;
     }
-    for (int y = 0; y < h; ++y)
-        for (int x = 0; x < w; ++x) {
-            const uint8_t f = *DF(&t, x, y);
-            sign[y * w + x] = (f & T_SGN) ? 1 : 0;
-            sig[y * w + x] = (f & T_SIG) ? 1 : 0;
-        }
+    if (sign && sig)
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x) {
+                const uint8_t f = *DF(&t, x, y);
+                sign[y * w + x] = (f & T_SGN) ? 1 : 0;
+                sig[y * w + x] = (f & T_SIG) ? 1 : 0;
+            }
     free(t.f);
     return 0;
+}
+
+int or_t1_decode(const uint8_t* data, uint32_t len, int w, int h, int orient, int nbps, int passes,
+                 uint32_t* mag, uint8_t* sign, uint8_t* sig) {
+    return t1_decode_core(data, len, w, h, orient, nbps, passes, mag, NULL, sign, sig);
+}
+
+/* t1DecodeBlock (t1_dec.cpp:175-178): the words of t1_dec.cpp:85-97, 117-127 for w x h
+ * samples (row stride w). Passes past the last plane are not decoded. */
+int or_t1_decode_words(const uint8_t* data, uint32_t len, int w, int h, int orient, int nbps, int passes,
+                       uint32_t* words) {
+    if (w < 1 || h < 1 || w > 32 || h > 32) return -1;
+    uint32_t mag[1024];
+    return t1_decode_core(data, len, w, h, orient, nbps, passes, mag, words, NULL, NULL);
 }
 
 /* ======================================================================== frame */
@@ -1378,4 +1401,126 @@ int64_t or_encode_frame_xyz(const uint16_t* xyz, int W, int H, int fourk, double
     free(c.bufs);
     free(cblks);
     return c.failed ? -1 : total;
+}
+
+/* ======================================================================== decoder frame */
+/* Inverse 1-D transform in float (idwtRow / idwtColumns, dwt_cpu.cpp:96-135): the low
+ * half times K onto the even samples, the high half times 1/K onto the odd ones, then the
+ * lifting steps delta, gamma, beta, alpha with negated coefficients. */
+static void inv1d_f(float* x, int n, int stride, float* tmp) {
+    if (n < 2) return;
+    const int sn = (n + 1) / 2;
+    for (int i = 0; i < sn; ++i) tmp[2 * i] = x[(size_t)i * stride] * (float)L_K;
+    for (int i = 0; i < n / 2; ++i) tmp[2 * i + 1] = x[(size_t)(sn + i) * stride] * (float)(1.0 / L_K);
+    lift_f(tmp, n, 1, 0, (float)-L_DELTA);
+    lift_f(tmp, n, 1, 1, (float)-L_GAMMA);
+    lift_f(tmp, n, 1, 0, (float)-L_BETA);
+    lift_f(tmp, n, 1, 1, (float)-L_ALPHA);
+    for (int i = 0; i < n; ++i) x[(size_t)i * stride] = tmp[i];
+}
+
+/* idwt97_2d (dwt_cpu.cpp:213-240): levels down to skip + 1, each rows then columns of the
+ * level's region. */
+void or_idwt97_2d_f32(float* const* planes, int nplanes, int width, int height, int levels, int skip) {
+    int ws[34], hs[34];
+    ws[0] = width;
+    hs[0] = height;
+    for (int l = 1; l <= levels; ++l) {
+        ws[l] = (ws[l - 1] + 1) / 2;
+        hs[l] = (hs[l - 1] + 1) / 2;
+    }
+    float* tmp = (float*)malloc(sizeof(float) * (size_t)imax(width, height));
+    for (int l = levels; l > skip; --l) {
+        const int w = ws[l - 1], h = hs[l - 1];
+        for (int c = 0; c < nplanes; ++c) {
+            for (int y = 0; y < h; ++y) inv1d_f(planes[c] + (size_t)y * width, w, 1, tmp);
+            for (int x = 0; x < w; ++x) inv1d_f(planes[c] + x, h, width, tmp);
+        }
+    }
+    free(tmp);
+}
+
+typedef struct {
+    const uint8_t* data;
+    size_t size;
+    const or_cblk_code* blocks;
+    const or_band* bands;
+    const or_cblk* cblks;
+    const int* wanted;
+    float* planes[3];
+    int W;
+} dec_ctx;
+
+/* One code-block of cpu_decode_backend.cpp:46-68: Tier-1 words, then each word's value
+ * times half the band's step, negated for a set sign bit, into the block's place in the
+ * wavelet plane. A block with no passes, or whose bytes lie past the codestream, is zero. */
+static void job_decode_block(void* p, int k) {
+    dec_ctx* c = (dec_ctx*)p;
+    const int i = c->wanted[k];
+    const or_cblk* cb = &c->cblks[i];
+    const or_band* b = &c->bands[cb->band];
+    const or_cblk_code* code = &c->blocks[i];
+    float* plane = c->planes[b->comp] + (size_t)(b->py0 + cb->y0) * c->W + b->px0 + cb->x0;
+    if (!code->passes || (size_t)code->offset + code->len > c->size) {
+        for (int y = 0; y < cb->h; ++y) memset(plane + (size_t)y * c->W, 0, sizeof(float) * (size_t)cb->w);
+        return;
+    }
+    uint32_t q[1024];
+    or_t1_decode_words(c->data + code->offset, code->len, cb->w, cb->h, b->orient, code->nbps, code->passes, q);
+    const float half = 0.5f * b->delta;
+    for (int y = 0; y < cb->h; ++y)
+        for (int x = 0; x < cb->w; ++x) {
+            const uint32_t v = q[y * cb->w + x];
+            const float a = (float)(v & 0x7FFFFFFFu) * half;
+            plane[(size_t)y * c->W + x] = (v & 0x80000000u) ? -a : a;
+        }
+}
+
+int or_decode_frame(int width, int height, int fourk, double base_step, const uint8_t* data, size_t size,
+                    const or_cblk_code* blocks, int nblocks, int reduce, int high_bits, int threads, uint16_t* dst,
+                    ptrdiff_t stride_bytes, float* wavelet) {
+    or_band bands[OR_MAX_BANDS];
+    int nb = 0;
+    const int n = or_layout_make(width, height, fourk, base_step, bands, &nb, NULL, 0);
+    const int levels = fourk ? 6 : 5;
+    if (n < 0 || n != nblocks || reduce < 0 || reduce > levels) return -1;
+    or_cblk* cblks = (or_cblk*)malloc(sizeof(or_cblk) * (size_t)n);
+    or_layout_make(width, height, fourk, base_step, bands, &nb, cblks, n);
+    const size_t npix = (size_t)width * height;
+    float* planes = (float*)calloc(3 * npix, sizeof(float));
+    int* wanted = (int*)malloc(sizeof(int) * (size_t)n);
+    int nw = 0;
+    for (int i = 0; i < n; ++i)
+        if (bands[cblks[i].band].res <= levels - reduce) wanted[nw++] = i;
+    dec_ctx c = {data, size, blocks, bands, cblks, wanted, {planes, planes + npix, planes + 2 * npix}, width};
+    par_for(nw, threads, job_decode_block, &c);
+    if (wavelet) memcpy(wavelet, planes, sizeof(float) * 3 * npix);
+    or_idwt97_2d_f32(c.planes, 3, width, height, levels, reduce);
+    /* inverse ICT (G.3), level shift, rounding and clamping (cpu_decode_backend.cpp:78-96) */
+    int ow = width, oh = height;
+    for (int r = 0; r < reduce; ++r) {
+        ow = (ow + 1) / 2;
+        oh = (oh + 1) / 2;
+    }
+    const int shift = high_bits ? 4 : 0;
+    for (int y = 0; y < oh; ++y) {
+        uint16_t* out = (uint16_t*)((uint8_t*)dst + (ptrdiff_t)y * stride_bytes);
+        for (int x = 0; x < ow; ++x) {
+            const size_t i = (size_t)y * width + x;
+            const float y0 = c.planes[0][i], y1 = c.planes[1][i], y2 = c.planes[2][i];
+            float v[3];
+            v[0] = y0 + (float)ICT_INV_COLS[2][0] * y2;
+            v[1] = y0 + (float)ICT_INV_COLS[1][1] * y1 + (float)ICT_INV_COLS[2][1] * y2;
+            v[2] = y0 + (float)ICT_INV_COLS[1][2] * y1;
+            for (int k = 0; k < 3; ++k) {
+                int code = (int)floorf(v[k] + 2048.5f);
+                code = code < 0 ? 0 : (code > 4095 ? 4095 : code);
+                out[3 * x + k] = (uint16_t)(code << shift);
+            }
+        }
+    }
+    free(planes);
+    free(wanted);
+    free(cblks);
+    return 0;
 }

@@ -23,6 +23,13 @@
  *                         or_t1_decode at every truncation point
  *   or_t1_decode          brute-force/round-trip only: independent restatement of
  *                         Annexes C.3 and D (parity pinned by the encoder round trip)
+ *   or_t1_decode_words    reconstruction words equal to the compiled reference's
+ *                         t1DecodeBlock at every truncation point; closed form
+ *                         (2 * magnitude + 2^last plane) against or_t1_decode
+ *   or_idwt97_2d_f32      equal to the compiled reference's idwt97_2d; perfect
+ *                         reconstruction against or_fdwt97_2d_f32 within float error
+ *   or_decode_frame       pictures and wavelet planes equal to the compiled reference's
+ *                         CPU decode slot (cpu_decode_backend.cpp)
  */
 #ifndef GAIL_ORACLE_H
 #define GAIL_ORACLE_H
@@ -124,6 +131,32 @@ int or_t1_encode(const uint32_t* coeffs, int w, int h, int orient, int max_passe
  * (integer part, w*h, row-major), sign the sign bits, sig whether significant. */
 int or_t1_decode(const uint8_t* data, uint32_t len, int w, int h, int orient, int nbps, int passes,
                  uint32_t* mag, uint8_t* sign, uint8_t* sig);
+
+/* t1DecodeBlock (t1_dec.cpp:45-181): the reconstruction words of a w x h block (row
+ * stride w): bit 31 the sign, the rest twice the decoded magnitude plus the mid-point of
+ * the bit-planes not decoded. Returns 0, or -1 for a block size outside 1..32. */
+int or_t1_decode_words(const uint8_t* data, uint32_t len, int w, int h, int orient, int nbps, int passes,
+                       uint32_t* words);
+
+/* ---- decoder (cpu_decode_backend.cpp, dwt_cpu.cpp:96-135, 213-240) ---- */
+/* gail::CblkCode (t2_parse.h:17-22). */
+typedef struct or_cblk_code {
+    uint32_t offset, len;
+    uint8_t passes, nbps, pad[2];
+} or_cblk_code;
+
+/* idwt97_2d: levels down to skip + 1 of nplanes float planes. */
+void or_idwt97_2d_f32(float* const* planes, int nplanes, int width, int height, int levels, int skip);
+
+/* CpuDecoderSlot::decode (cpu_decode_backend.cpp:38-107) for the layout of (width,
+ * height, profile, base_step): Tier-1 and dequantisation of the code-blocks of the
+ * resolutions kept, the inverse DWT down to `reduce`, the inverse ICT to 12-bit codes
+ * (<< 4 when high_bits), three interleaved words per pixel, rows stride_bytes apart.
+ * wavelet (3 x W*H floats, may be NULL) receives the planes before the inverse DWT.
+ * Returns 0, or -1 when nblocks or reduce does not fit the layout. */
+int or_decode_frame(int width, int height, int fourk, double base_step, const uint8_t* data, size_t size,
+                    const or_cblk_code* blocks, int nblocks, int reduce, int high_bits, int threads, uint16_t* dst,
+                    ptrdiff_t stride_bytes, float* wavelet);
 
 /* ---- whole frame (cpu_backend.cpp:61-132) ---- */
 typedef struct or_cblk_info {

@@ -1,5 +1,6 @@
-// C entry points over the reference's own CPU hot-path sources (layout.cpp,
-// dwt_cpu.cpp, quant.cpp, t1_ref.cpp, color.cpp), compiled from where they lie by
+// C entry points over the reference's own CPU sources (layout.cpp, dwt_cpu.cpp, quant.cpp,
+// t1_ref.cpp, color.cpp; for decoding t1_dec.cpp, cpu_decode_backend.cpp,
+// codestream_parse.cpp, t2_parse.cpp), compiled from where they lie by
 // oracle/Makefile into oracle/_ref/libgailref.so. TEST INFRASTRUCTURE ONLY: used to
 // write golden vectors (tests/golden/make_golden.py), to cross-check the oracle, and
 // as the timed CPU baseline (the frame flow of cpu_backend.cpp:61-132 over the
@@ -12,11 +13,16 @@
 #include <thread>
 #include <vector>
 
+#include "core/codestream_parse.h"
 #include "core/color.h"
+#include "core/decode_backend.h"
 #include "core/dwt_cpu.h"
 #include "core/layout.h"
 #include "core/quant.h"
+#include "core/t1_dec.h"
 #include "core/t1_ref.h"
+#include "core/t2_parse.h"
+#include "core/thread_pool.h"
 
 using namespace gail;
 
@@ -266,6 +272,91 @@ int64_t ref_encode_frame_xyz(const uint16_t* xyz, int W, int H, int fourk, doubl
             off += r.data.size();
         }
         return (int64_t)off;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// t1DecodeBlock (t1_dec.cpp:175-178), words returned with row stride w.
+int ref_t1_decode_words(const uint8_t* data, uint32_t len, int w, int h, int orient, int nbps, int passes,
+                        uint32_t* words) {
+    if (w < 1 || h < 1 || w > kCblkSize || h > kCblkSize) return -1;
+    uint32_t q[kCblkArea];
+    t1DecodeBlock(data, len, w, h, orient, nbps, passes, q);
+    for (int y = 0; y < h; ++y) std::memcpy(words + (size_t)y * w, q + (size_t)y * kCblkSize, sizeof(uint32_t) * w);
+    return 0;
+}
+
+// idwt97_2d (dwt_cpu.cpp:213-240) of nplanes planes stored back to back.
+void ref_idwt97_2d(float* planes, int nplanes, int W, int H, int levels, int skip) {
+    std::vector<float*> pl;
+    for (int c = 0; c < nplanes; ++c) pl.push_back(planes + (size_t)c * W * H);
+    idwt97_2d(pl.data(), nplanes, W, H, levels, skip, nullptr);
+}
+
+struct RefCblkCode {
+    uint32_t offset, len;
+    uint8_t passes, nbps, pad[2];
+};
+
+// The reference's CPU decode slot (cpu_decode_backend.cpp) for the layout of (W, H,
+// profile, baseStep): blocks[nblocks] located in data[size]; the picture goes to dst (rows
+// strideBytes apart), the planes before the inverse wavelet to wavelet (may be NULL).
+int ref_decode_frame(int W, int H, int fourk, double baseStep, const uint8_t* data, size_t size,
+                     const RefCblkCode* blocks, int nblocks, int reduce, int highBits, int threads, uint16_t* dst,
+                     ptrdiff_t strideBytes, float* wavelet) {
+    try {
+        const FrameLayout L = makeLayout(W, H, fourk ? Profile::Cinema4K : Profile::Cinema2K, baseStep);
+        if (nblocks != L.numCblks()) return -1;
+        DecodeSettings s;
+        s.reduce = reduce;
+        s.highBits = highBits != 0;
+        auto pool = std::make_shared<ThreadPool>(threads, false);
+        auto slot = createCpuDecoderSlot(L, s, pool);
+        std::vector<CblkCode> codes(nblocks);
+        for (int i = 0; i < nblocks; ++i) {
+            codes[i].offset = blocks[i].offset;
+            codes[i].len = blocks[i].len;
+            codes[i].passes = blocks[i].passes;
+            codes[i].nbps = blocks[i].nbps;
+        }
+        std::vector<float> wv[3];
+        DecodeTimings tm;
+        if (wavelet) tm.wavelet = wv;
+        slot->decode(L, data, size, codes, dst, strideBytes, &tm);
+        if (wavelet)
+            for (int c = 0; c < 3; ++c) std::memcpy(wavelet + (size_t)c * W * H, wv[c].data(), sizeof(float) * W * H);
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// The reference's header parser and Tier-2 reader (codestream_parse.cpp, t2_parse.cpp) on
+// a codestream: its size, profile, signalled step sizes and every code-block's bytes.
+// Returns the number of code-blocks (blocks and bands filled up to cap / OR_MAX_BANDS),
+// or -1 with nothing filled when the stream is refused.
+int ref_parse_blocks(const uint8_t* data, size_t size, int* W, int* H, int* fourk, RefCblkCode* blocks, int cap,
+                     RefBand* bands, int* nbands) {
+    try {
+        StreamHeader h;
+        std::string err;
+        if (!parseCodestream(data, size, false, h, err) || !h.supported) return -1;
+        const FrameLayout L = layoutFor(h);
+        std::vector<CblkCode> out;
+        if (!parsePackets(data, h, L, L.levels, out, err)) return -1;
+        *W = L.width;
+        *H = L.height;
+        *fourk = h.rsiz == 4;
+        for (int i = 0; i < (int)out.size() && i < cap; ++i)
+            blocks[i] = {out[i].offset, out[i].len, out[i].passes, out[i].nbps, {0, 0}};
+        *nbands = (int)L.bands.size();
+        for (size_t i = 0; i < L.bands.size(); ++i) {
+            const Subband& b = L.bands[i];
+            bands[i] = {b.comp, b.res, b.level, b.orient, b.px0, b.py0, b.w, b.h, b.ncbx, b.ncby, b.cblkBase,
+                        b.eps, b.mu, b.Mb, b.delta, 0.f, b.weight};
+        }
+        return (int)out.size();
     } catch (...) {
         return -1;
     }

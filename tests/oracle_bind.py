@@ -53,6 +53,28 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
+# gail::CblkCode (t2_parse.h:17-22) as a numpy record: 12 bytes, as the C structure.
+CBLK_CODE = np.dtype([("offset", "<u4"), ("len", "<u4"), ("passes", "u1"), ("nbps", "u1"), ("pad", "u1", 2)])
+
+
+def block_codes(info, pass_len, passes=None):
+    """The code-block table a Tier-2 reader would hand the decoder for a frame's Tier-1
+    output coded into one codestream equal to its data buffer: each block's bytes from its
+    data offset, keeping `passes[i]` passes (all when None) and their truncation length."""
+    info = np.asarray(info, np.uint32).reshape(-1, 5)
+    n = len(info)
+    out = np.zeros(n, CBLK_CODE)
+    out["offset"] = info[:, 2]
+    out["nbps"] = info[:, 0]
+    keep = info[:, 1] if passes is None else np.minimum(np.asarray(passes), info[:, 1])
+    out["passes"] = keep
+    pl = np.asarray(pass_len, np.uint32)
+    base = info[:, 4].astype(np.int64)
+    lens = np.where(keep > 0, pl[np.maximum(base + keep.astype(np.int64) - 1, 0)], 0)
+    out["len"] = lens
+    return out
+
+
 class _Lib:
     """The oracle and the reference shim share one calling convention (prefix or_/ref_)."""
 
@@ -88,6 +110,46 @@ class _Lib:
         f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_float, C.c_int, C.c_void_p,
                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]
         self._frame = f
+        f = getattr(L, prefix + "t1_decode_words")
+        f.restype = C.c_int
+        f.argtypes = [C.c_void_p, C.c_uint32] + [C.c_int] * 5 + [C.c_void_p]
+        self._t1dw = f
+        f = getattr(L, prefix + "decode_frame")
+        f.restype = C.c_int
+        f.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_size_t, C.c_void_p, C.c_int, C.c_int,
+                      C.c_int, C.c_int, C.c_void_p, C.c_ssize_t, C.c_void_p]
+        self._decf = f
+
+    def t1_decode_words(self, data, w, h, orient, nbps, passes, length=None):
+        """Reconstruction words of a w x h block (sign in bit 31, twice the magnitude plus
+        the mid-point of the planes not decoded)."""
+        buf = np.frombuffer(bytes(data), np.uint8).copy() if len(data) else np.zeros(1, np.uint8)
+        n = len(data) if length is None else length
+        out = np.zeros(w * h, np.uint32)
+        if self._t1dw(_p(buf), n, w, h, orient, nbps, passes, _p(out)):
+            raise ValueError("block size out of range")
+        return out.reshape(h, w)
+
+    def decode_frame(self, W, H, fourk, data, blocks, reduce=0, high_bits=True, base_step=1.0, threads=8,
+                     pad_row=0, want_wavelet=False):
+        """The CPU decode of one frame: (picture as rows of 3 x 16-bit words, wavelet planes
+        before the inverse DWT or None)."""
+        data = np.ascontiguousarray(np.frombuffer(bytes(data), np.uint8) if not isinstance(data, np.ndarray)
+                                    else data, dtype=np.uint8)
+        if data.size == 0:
+            data = np.zeros(1, np.uint8)
+        blocks = np.ascontiguousarray(blocks, dtype=CBLK_CODE)
+        ow, oh = W, H
+        for _ in range(reduce):
+            ow, oh = (ow + 1) // 2, (oh + 1) // 2
+        row = ow * 3 + pad_row
+        out = np.zeros((oh, row), np.uint16)
+        wav = np.zeros((3, H, W), np.float32) if want_wavelet else None
+        rc = self._decf(W, H, int(fourk), base_step, _p(data), data.size, _p(blocks), len(blocks), reduce,
+                        int(high_bits), threads, _p(out), row * 2, _p(wav))
+        if rc:
+            raise ValueError("decode refused the frame")
+        return out, wav
 
     def layout(self, W, H, fourk, base_step=1.0):
         bands = (Band * 64)()
@@ -177,6 +239,16 @@ class Oracle(_Lib):
         L.or_color_pixels.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.or_xyz12_to_16.restype = C.c_uint16
         L.or_xyz12_to_16.argtypes = [C.c_uint32]
+        L.or_idwt97_2d_f32.restype = None
+        L.or_idwt97_2d_f32.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+
+    def idwt(self, planes, levels, skip=0):
+        """Float inverse DWT of a (n, H, W) stack, levels down to skip + 1."""
+        a = np.array(planes, dtype=np.float32, copy=True, order="C")
+        n, H, W = a.shape
+        ptrs = (C.c_void_p * n)(*[a[i].ctypes.data for i in range(n)])
+        self.lib.or_idwt97_2d_f32(ptrs, n, W, H, levels, skip)
+        return a
 
     def t1_decode(self, data, w, h, orient, nbps, passes, length=None):
         buf = np.frombuffer(bytes(data), np.uint8).copy() if len(data) else np.zeros(1, np.uint8)
@@ -256,6 +328,33 @@ class Reference(_Lib):
         L.ref_color_pixels.argtypes = [C.POINTER(ColorCfg), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.ref_convert_frame.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_int, C.c_int,
                                         C.c_void_p, C.c_void_p]
+        L.ref_idwt97_2d.restype = None
+        L.ref_idwt97_2d.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.ref_parse_blocks.restype = C.c_int
+        L.ref_parse_blocks.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int), C.c_void_p, C.c_int, C.POINTER(Band),
+                                       C.POINTER(C.c_int)]
+
+    def idwt(self, planes, levels, skip=0):
+        a = np.array(planes, dtype=np.float32, copy=True, order="C")
+        n, H, W = a.shape
+        self.lib.ref_idwt97_2d(_p(a), n, W, H, levels, skip)
+        return a
+
+    def parse_blocks(self, codestream):
+        """The reference's header and packet parsing of a codestream: (W, H, fourk, block
+        table, bands with the signalled steps), or None when it refuses the stream."""
+        data = np.frombuffer(bytes(codestream), np.uint8).copy()
+        W, H, fk, nb = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
+        bands = (Band * 64)()
+        n = self.lib.ref_parse_blocks(_p(data), data.size, C.byref(W), C.byref(H), C.byref(fk), None, 0, bands,
+                                      C.byref(nb))
+        if n < 0:
+            return None
+        blocks = np.zeros(n, CBLK_CODE)
+        self.lib.ref_parse_blocks(_p(data), data.size, C.byref(W), C.byref(H), C.byref(fk), _p(blocks), n, bands,
+                                  C.byref(nb))
+        return W.value, H.value, bool(fk.value), blocks, list(bands[:nb.value])
 
     def synthesis_norm(self, level, orient):
         return self.lib.ref_synthesis_norm(level, orient)

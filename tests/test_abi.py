@@ -28,7 +28,7 @@ def test_plugin_table():
     # layout hash over the plain structures, computed the way gail_hip_plugin_layout() does
     sizes = [C.sizeof(t) for t in (g.DeviceDesc, g.Subband, g.Layout, g.FrameFormat, g.Color, g.EncodeSettings,
                                    g.Memory, g.CblkInfo, g.Tier1, g.FrameTimings, g.Debug, g.BatchOptions, g.Slot,
-                                   g.Rate, g.Plugin)]
+                                   g.Rate, g.DecodeSettings, g.CblkCode, g.DecodeTimings, g.Plugin)]
     h = 1469598103934665603
     for s in sizes:
         h = ((h ^ s) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
@@ -53,7 +53,16 @@ def test_host_exports():
     assert not missing, missing
 
 
-def test_decoder_entries_unsupported():
+def test_decoder_entries_check_arguments():
+    """The decoder entries refuse missing arguments as misuse (GAIL_HIP_MISUSE), and a
+    reduction beyond the layout's levels as an error, before touching a device."""
     p = g.plugin()
     out = C.c_void_p()
-    assert p.decoder_create(None, None, None, C.byref(out)) == 6
+    assert p.decoder_create(None, None, None, C.byref(out)) == 4
+    assert not out.value
+    L = g.FrameLayout(1998, 1080, 0)
+    pl = L.plain()
+    assert p.decoder_bytes(None, C.byref(pl), C.byref(g.DecodeSettings(0, 1))) > 3 * 1998 * 1080 * 12
+    assert p.decoder_bytes(None, C.byref(pl), C.byref(g.DecodeSettings(6, 1))) == 0
+    assert p.decoder_decode(None, None, None, 0, None, 0, None, 0, None) == 4
+    assert C.sizeof(g.CblkCode) == 12 and g.CBLK_CODE.itemsize == 12
