@@ -16,6 +16,7 @@ flight; `served` the mixed-profile request loop at 300 fps of Poisson arrivals.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
 """
 import argparse
+import ctypes
 import json
 import os
 import statistics
@@ -57,6 +58,110 @@ def cpu_baseline(frames, fourk, thr, budget_s=20.0):
             "sample": f"{done} 4K flat frames, colour/ICT + DWT + quantisation + Tier-1 with the same early "
                       f"termination threshold, {cores} threads, {dt:.1f} s (no rate control or Tier-2: the "
                       f"reference's t2.cpp needs a generated header and does not build here)"}
+
+
+def decode_bench(g, np, dev, layout, frames, decoders=4, rounds=4, cpu=True):
+    """The HIP decoder on the reference's decode workload: 4K frames at 250 Mbit/s (24 fps
+    rate control on the device), each frame's code-blocks located (the Tier-1 output read as
+    its codestream: the same block bytes the codestream carries), decoded to 12-bit X'Y'Z'
+    words (highBits). `decoders` decoders on their own threads and streams; pictures return
+    to pinned host memory (the reference's setting), then the same with codestreams and
+    pictures resident in HBM. Stage times are medians of single frames in flight."""
+    p = g.plugin()
+    W, H = layout.width, layout.height
+    n = len(frames)
+    b = g.Batch(dev, [layout] * n, [g.with_rate_control(g.xyz_settings(layout))] * n)
+    for i, f in enumerate(frames):
+        b.upload(i, f)
+    res, _ = b.encode()
+    b.close()
+    items = [(np.ascontiguousarray(r.data), g.block_table(r)) for r in res]
+    cs_bytes = [len(g.build_codestream(layout, r)) for r in res]
+    mbps = statistics.mean(cs_bytes) * 8 * 24 / 1e6
+    pic = W * H * 6
+    hin = [p.host_alloc(max(d.size, 1)) for d, _ in items]
+    dinb = [p.device_alloc(dev.handle, max(d.size, 1)) for d, _ in items]
+    hout = [p.host_alloc(pic) for _ in range(decoders)]
+    dout = [p.device_alloc(dev.handle, pic) for _ in range(decoders)]
+    decs = [g.Decoder(dev, layout, 0, True) for _ in range(decoders)]
+    try:
+        if not all(hin) or not all(dinb) or not all(hout) or not all(dout):
+            raise RuntimeError("decode bench buffers could not be allocated")
+        for (d, _), h, dd in zip(items, hin, dinb):
+            ctypes.memmove(h, d.ctypes.data, d.size)
+            if p.copy(ctypes.c_void_p(dd), ctypes.c_void_p(h), d.size):
+                raise RuntimeError("copy failed")
+        # single frame in flight: stage times
+        stages = []
+        for k, (d, blocks) in enumerate(items):
+            _, tm, _ = decs[0].decode((hin[k], d.size), blocks, out=(hout[0], W * 6))
+            stages.append(tm)
+        stage_ms = {key: round(statistics.median(t[key] for t in stages[1:] or stages), 3)
+                    for key in ("t1", "dwt", "color", "total")}
+
+        def run(src, dst, total):
+            nxt = [0]
+            lock = threading.Lock()
+            errors = []
+
+            def worker(j):
+                try:
+                    while True:
+                        with lock:
+                            k = nxt[0]
+                            nxt[0] += 1
+                        if k >= total:
+                            return
+                        d, blocks = items[k % n]
+                        decs[j].decode((src[k % n], d.size), blocks, out=(dst[j], W * 6))
+                except Exception as e:  # noqa: BLE001
+                    errors.append(e)
+
+            ts = [threading.Thread(target=worker, args=(j,)) for j in range(decoders)]
+            t0 = time.perf_counter()
+            for t in ts:
+                t.start()
+            for t in ts:
+                t.join()
+            dt = time.perf_counter() - t0
+            if errors:
+                raise errors[0]
+            return total / dt
+
+        run(hin, hout, decoders)  # warm-up
+        total = n * rounds
+        host_fps = run(hin, hout, total)
+        hbm_fps = run(dinb, dout, total)
+    finally:
+        for dcd in decs:
+            dcd.close()
+        for h in hin + hout:
+            if h:
+                p.host_free(h)
+        for dd in dinb + dout:
+            if dd:
+                p.device_free(dd)
+    out = {"fps_host_memory": round(host_fps, 2), "fps_hbm_resident": round(hbm_fps, 2),
+           "decoders": decoders, "frames": total, "distinct_frames": n,
+           "stage_ms_one_in_flight": stage_ms, "mbit_per_s_at_24fps": round(mbps, 1),
+           "reference_published_fps": {"RTX 3090 CUDA 4K": 121.4, "RTX 5060 Ti CUDA 4K": 57.3},
+           "workload": "4K flat 3996x2160, 24 fps / 250 Mbit/s device rate control, decoded to xyz12 words"}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_bind import reference
+        ref = reference()
+        if ref is not None:
+            cores = len(os.sched_getaffinity(0))
+            done, t0 = 0, time.perf_counter()
+            while done < n and time.perf_counter() - t0 < 15.0:
+                d, blocks = items[done]
+                ref.decode_frame(W, H, 1, d, blocks, threads=cores)
+                done += 1
+            dt = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": round(done / dt, 3), "unit": "frames/s", "cores": cores,
+                                   "kind": "reference", "sample": f"{done} of the same frames through the "
+                                   f"reference's CPU decode slot (cpu_decode_backend.cpp), {cores} threads"}
+    return out
 
 
 def _planar_picture(g, np, seed, W, H, fmt):
@@ -305,6 +410,9 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=5.0)
     ap.add_argument("--mixed-steps", type=int, default=6)
     ap.add_argument("--pmc-probe", action="store_true", help="a 1 GiB copy and one batch, for counter runs")
+    ap.add_argument("--decode-frames", type=int, default=8, help="distinct 4K frames of the decoder measurement")
+    ap.add_argument("--decoders", type=int, default=8)
+    ap.add_argument("--decode-only", action="store_true", help="only the decoder measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU figures are not part of it "
@@ -324,6 +432,12 @@ def main():
         pmc_probe(g, np, W, H, fourk, frames)
         return
     dev = g.Device(0)
+    if args.decode_only:
+        dec = decode_bench(g, np, dev, layout, frames[:max(1, args.decode_frames)], decoders=args.decoders,
+                           cpu=not args.no_cpu)
+        print(json.dumps({"decode": dec}))
+        dev.close()
+        return
 
     # Early-termination threshold of the kernel measurement: 0.25 x lambda of a preceding
     # frame (pipeline.cpp:194), from the rate control of the first frame coded without it.
@@ -405,6 +519,11 @@ def main():
     if args.mixed_steps > 0:
         mixed = mixed_batch_run(g, np, dev, steps=args.mixed_steps)
         print(f"[bench] mixed 64-frame batch: {json.dumps(mixed)}", file=sys.stderr, flush=True)
+    decode = None
+    if args.decode_frames > 0:
+        decode = decode_bench(g, np, dev, layout, frames[:args.decode_frames], decoders=args.decoders,
+                              cpu=not args.no_cpu)
+        print(f"[bench] decoder: {json.dumps(decode)}", file=sys.stderr, flush=True)
     dev.close()
 
     # ---- served 4K flat streams (the headline) and the 48 fps stress stream
@@ -465,6 +584,7 @@ def main():
         "stress_4k_48fps": stress,
         "served": served,
         "cpu_2k_frame": cfg1,
+        "decode_4k": decode,
     }
     print(json.dumps(line))
 

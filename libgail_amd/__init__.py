@@ -391,6 +391,17 @@ def decoded_size(layout, reduce):
     return w, h
 
 
+def block_table(t1):
+    """The code-block table (CBLK_CODE records) of a frame's Tier-1 output read as its own
+    codestream: each block's bytes at its data offset with the passes the frame kept (all of
+    them, or those device rate control chose)."""
+    info = np.asarray(t1.info, np.uint32).reshape(-1, 5)
+    out = np.zeros(len(info), CBLK_CODE)
+    out["offset"], out["len"] = info[:, 2], info[:, 3]
+    out["passes"], out["nbps"] = np.minimum(info[:, 1], 255), info[:, 0]
+    return out
+
+
 class Decoder:
     """One frame at a time (the reference's DecoderSlot) through the plugin's decoder entries:
     code-block table + codestream in, X'Y'Z' picture (three 16-bit words per pixel) out."""

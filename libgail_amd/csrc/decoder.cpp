@@ -19,13 +19,19 @@ constexpr size_t kBytePad = 64;
 // Length classes of the block ordering (bytes / 4, longer blocks share the top class).
 constexpr uint32_t kLenKeys = 4096;
 
-bool onDevice(const void* p) {
+enum class Mem { Pageable, Pinned, Device };
+
+// Where a buffer lives: device memory is read or written in place, pinned host memory by DMA,
+// anything else through the decoder's own pinned staging buffers.
+Mem memKind(const void* p) {
     hipPointerAttribute_t a{};
     if (!p || hipPointerGetAttributes(&a, p) != hipSuccess) {
         (void)hipGetLastError();
-        return false;
+        return Mem::Pageable;
     }
-    return a.type == hipMemoryTypeDevice;
+    if (a.type == hipMemoryTypeDevice) return Mem::Device;
+    if (a.type == hipMemoryTypeHost) return Mem::Pinned;
+    return Mem::Pageable;
 }
 
 double msSince(std::chrono::steady_clock::time_point t0) {
@@ -73,7 +79,7 @@ Decoder::Decoder(int device, const Layout& L, int reduce, bool highBits)
         hipCheck(hipMalloc(&dOut_, outBytes), "decoder picture");
         hipCheck(hipHostMalloc((void**)&hOut_, outBytes, hipHostMallocDefault), "decoder picture (pinned)");
         ensureBytes(1500000);  // a DCI frame is at most 250 Mbit/s over 24 frames
-        ensureDescs(((size_t)wanted_.size() / kDecLanes + 1) * kDecLanes);
+        ensureDescs(((size_t)wanted_.size() / kDecSlots + 1) * kDecSlots);
     } catch (...) {
         release();
         throw;
@@ -147,19 +153,20 @@ uint32_t Decoder::buildDescs(const Layout& frame, const uint8_t* coded, const De
     for (size_t k = 0; k < wanted_.size(); ++k)
         if (coded[k]) order_[count_[key(blocks[wanted_[k]].len)]++] = (uint32_t)wanted_[k];
     // Waves: a wave runs as long as its longest lane, so a long block takes a wave of its own
-    // and short ones share, up to a byte budget that spreads the frame over about eight
-    // waves per compute unit.
-    const uint64_t budget = std::max<uint64_t>(256, total / (uint64_t)(8 * cus_));
+    // and short ones share, up to kDecWaveLanes blocks and a byte budget that spreads the frame
+    // over about eight waves per compute unit (measured best on 4K frames at 250 Mbit/s).
+    const uint64_t budget = std::max<uint64_t>(384, total / (uint64_t)(8 * cus_));
+    const uint32_t lanes = (uint32_t)kDecWaveLanes;
     auto pack = [&](DecBlock* out) {
         uint32_t n = 0, inWave = 0;
         uint64_t waveBytes = 0;
         for (uint32_t i : order_) {
             const DecCode& c = blocks[i];
-            if (inWave && (inWave == (uint32_t)kDecLanes || waveBytes + c.len > budget)) {
+            if (inWave && (inWave == lanes || waveBytes + c.len > budget)) {
                 if (out)
-                    for (; inWave < (uint32_t)kDecLanes; ++inWave) out[n++] = DecBlock{0, 0, 0, kDecEmpty};
+                    for (; inWave < lanes; ++inWave) out[n++] = DecBlock{0, 0, 0, kDecEmpty};
                 else
-                    n += (uint32_t)kDecLanes - inWave;
+                    n += lanes - inWave;
                 inWave = 0;
                 waveBytes = 0;
             }
@@ -181,9 +188,14 @@ uint32_t Decoder::buildDescs(const Layout& frame, const uint8_t* coded, const De
         }
         if (inWave) {
             if (out)
-                for (; inWave < (uint32_t)kDecLanes; ++inWave) out[n++] = DecBlock{0, 0, 0, kDecEmpty};
+                for (; inWave < lanes; ++inWave) out[n++] = DecBlock{0, 0, 0, kDecEmpty};
             else
-                n += (uint32_t)kDecLanes - inWave;
+                n += lanes - inWave;
+        }
+        // whole workgroups
+        while (n % (uint32_t)kDecSlots) {
+            if (out) out[n] = DecBlock{0, 0, 0, kDecEmpty};
+            ++n;
         }
         return n;
     };
@@ -216,12 +228,17 @@ void Decoder::decode(const Layout& frame, const uint8_t* data, size_t size, cons
         coded[k] = c.passes && c.nbps >= 1 && c.nbps <= 30 && (size_t)c.offset + c.len <= size;
     }
     const uint32_t nd = buildDescs(frame, coded.data(), blocks, size);
-    const bool devIn = onDevice(data), devOut = onDevice(dst);
+    const Mem in = memKind(data), outKind = memKind(dst);
+    const bool devOut = outKind == Mem::Device;
+    // pinned pictures are written by DMA when their rows go forwards
+    const bool dmaOut = outKind == Mem::Pinned && strideBytes > 0;
     ensureBytes(std::max<size_t>(size, 1));
     hipCheck(hipEventRecord(ev_[0], s), "event");
     if (size) {
-        if (devIn) {
+        if (in == Mem::Device) {
             hipCheck(hipMemcpyAsync(dBytes_, data, size, hipMemcpyDeviceToDevice, s), "codestream copy");
+        } else if (in == Mem::Pinned) {
+            hipCheck(hipMemcpyAsync(dBytes_, data, size, hipMemcpyHostToDevice, s), "codestream upload");
         } else {
             std::memcpy(hBytes_, data, size);
             hipCheck(hipMemcpyAsync(dBytes_, hBytes_, size, hipMemcpyHostToDevice, s), "codestream upload");
@@ -298,7 +315,13 @@ void Decoder::decode(const Layout& frame, const uint8_t* data, size_t size, cons
     ip.shift = highBits_ ? 4 : 0;
     hipCheck(launch_ict_pack(ip, s), "inverse ICT");
     const size_t outBytes = (size_t)outW_ * outH_ * 6;
-    if (!devOut) hipCheck(hipMemcpyAsync(hOut_, dOut_, outBytes, hipMemcpyDeviceToHost, s), "picture download");
+    if (dmaOut) {
+        hipCheck(hipMemcpy2DAsync(dst, (size_t)strideBytes, dOut_, (size_t)rowBytes, (size_t)rowBytes, (size_t)outH_,
+                                  hipMemcpyDeviceToHost, s),
+                 "picture download");
+    } else if (!devOut) {
+        hipCheck(hipMemcpyAsync(hOut_, dOut_, outBytes, hipMemcpyDeviceToHost, s), "picture download");
+    }
     hipCheck(hipEventRecord(ev_[3], s), "event");
     if (wantWavelet) {
         hWave_.resize(3 * npix);
@@ -306,7 +329,7 @@ void Decoder::decode(const Layout& frame, const uint8_t* data, size_t size, cons
     }
     hipCheck(hipStreamSynchronize(s), "decode");
     hipCheck(hipGetLastError(), "decode");
-    if (!devOut) {
+    if (!devOut && !dmaOut) {
         if (strideBytes == rowBytes) {
             std::memcpy(dst, hOut_, outBytes);
         } else {

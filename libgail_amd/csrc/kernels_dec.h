@@ -14,13 +14,17 @@ struct DecBlock {
     uint32_t dst;       // word index of the block's top-left sample in the Mallat planes
 };
 constexpr uint32_t kDecEmpty = 0xFFFFFFFFu;
-// Lanes per Tier-1 decode workgroup (one wave).
-constexpr int kDecLanes = 64;
+// Only the first kDecWaveLanes lanes of a wave decode blocks (waves are sparse anyway: a wave
+// runs as long as its longest lane, so long blocks go nearly alone), which lets a workgroup
+// of kDecWaves waves keep every decoding lane's state in the LDS a full wave would take.
+constexpr int kDecWaveLanes = 16;
+constexpr int kDecWaves = 4;
+constexpr int kDecSlots = kDecWaveLanes * kDecWaves;
 
 struct T1DecodeParams {
     const uint8_t* bytes;  // codestream, readable 64 bytes past its end
     const DecBlock* blocks;
-    uint32_t nblocks;      // a multiple of kDecLanes
+    uint32_t nblocks;      // kDecWaveLanes entries per wave, a multiple of kDecSlots
     uint32_t* words;       // 3 Mallat planes of reconstruction words, zero beforehand
     uint32_t nwords;
     uint32_t stride;       // words per plane row (the frame width)

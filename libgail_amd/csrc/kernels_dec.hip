@@ -1,11 +1,12 @@
 // Decoder kernels for CDNA4 (gfx950, wave64).
 //
 // Tier-1: one lane per code-block (t1_decode_block.h), blocks ordered longest first and
-// packed into waves by a byte budget on the host so a wave's lanes finish together. A lane's
-// row masks and context states live in LDS at [entry][lane] (conflict-free whatever row each
-// lane is on), its codeword comes through a 16-byte register window, and its reconstruction
-// words go out as atomic adds into zeroed Mallat planes, so no lane ever waits on a read of
-// its own output.
+// packed into waves of at most 16 blocks by a byte budget on the host so a wave's lanes finish
+// together. A lane's row masks and context states live in LDS at [entry][slot] (conflict-free
+// whatever row each lane is on); only 16 lanes of each wave hold a slot, so four waves share
+// the LDS of one and a SIMD keeps several waves to hide the serial chain's latencies. The
+// codeword comes through a 16-byte register window, and the reconstruction words go out as
+// atomic adds into zeroed Mallat planes, so no lane ever waits on a read of its own output.
 //
 // Inverse DWT: one launch per level; a workgroup owns a TX x TY tile of the level's output
 // and all three components. It gathers the tile plus a 4-sample mirrored halo from the four
@@ -43,8 +44,8 @@ constexpr float kR2 = (float)1.402, kG1 = (float)-0.34413, kG2 = (float)-0.71414
 
 // ---------------------------------------------------------------- Tier-1
 struct LdsState {
-    uint32_t (*masks)[32][kDecLanes];
-    uint32_t (*ctxs)[kDecLanes];
+    uint32_t (*masks)[32][kDecSlots];
+    uint32_t (*ctxs)[kDecSlots];
     int lane;
     __device__ __forceinline__ uint32_t rows(int t, uint32_t r) const { return masks[t][r][lane]; }
     __device__ __forceinline__ void set_rows(int t, uint32_t r, uint32_t v) { masks[t][r][lane] = v; }
@@ -60,16 +61,18 @@ struct AtomicSink {
     }
 };
 
-__global__ void __launch_bounds__(kDecLanes) t1_decode_kernel(T1DecodeParams P) {
-    __shared__ uint32_t s_rows[4][32][kDecLanes];
-    __shared__ uint32_t s_ctx[19][kDecLanes];
+__global__ void __launch_bounds__(64 * kDecWaves) t1_decode_kernel(T1DecodeParams P) {
+    __shared__ uint32_t s_rows[4][32][kDecSlots];
+    __shared__ uint32_t s_ctx[19][kDecSlots];
     __shared__ uint32_t s_tab[48];
     __shared__ uint8_t s_zc[4 * 128];
-    const int lane = threadIdx.x;
-    if (lane < 47) s_tab[lane] = kDecMq[lane];
-    for (int i = lane; i < 4 * 128; i += kDecLanes) s_zc[i] = kDecZc[i];
+    const int t = threadIdx.x;
+    if (t < 47) s_tab[t] = kDecMq[t];
+    for (int i = t; i < 4 * 128; i += 64 * kDecWaves) s_zc[i] = kDecZc[i];
     __syncthreads();
-    const uint32_t i = blockIdx.x * kDecLanes + lane;
+    const int lane = t & 63, wave = t >> 6;
+    if (lane >= kDecWaveLanes) return;
+    const uint32_t i = (blockIdx.x * kDecWaves + wave) * kDecWaveLanes + lane;
     if (i >= P.nblocks) return;
     const DecBlock d = P.blocks[i];
     if (d.dst == kDecEmpty) return;
@@ -77,7 +80,7 @@ __global__ void __launch_bounds__(kDecLanes) t1_decode_kernel(T1DecodeParams P) 
     const uint32_t nbps = (d.geom >> 12) & 63u, passes = (d.geom >> 18) & 255u;
     // the host builds the table from the layout; a wrong one still writes nothing outside
     if (d.dst >= P.nwords || P.nwords - d.dst < (h - 1u) * P.stride + w) return;
-    LdsState st{s_rows, s_ctx, lane};
+    LdsState st{s_rows, s_ctx, wave * kDecWaveLanes + lane};
     AtomicSink out{P.words + d.dst, P.stride};
     t1d_decode_block(st, P.bytes, d.off, d.len, w, h, orient, nbps, passes, s_zc, s_tab, out);
 }
@@ -261,7 +264,7 @@ hipError_t dec_tables_init() {
 
 hipError_t launch_t1_decode(const T1DecodeParams& p, hipStream_t s) {
     if (!p.nblocks) return hipSuccess;
-    hipLaunchKernelGGL(t1_decode_kernel, dim3(p.nblocks / kDecLanes), dim3(kDecLanes), 0, s, p);
+    hipLaunchKernelGGL(t1_decode_kernel, dim3(p.nblocks / kDecSlots), dim3(64 * kDecWaves), 0, s, p);
     return hipGetLastError();
 }
 

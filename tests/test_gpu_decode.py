@@ -102,6 +102,32 @@ def test_decoder_device_memory_and_reuse(device, oracle):
         dec.close()
 
 
+def test_decoder_pinned_memory(device, oracle):
+    """A codestream in pinned host memory is read by DMA and the picture written into pinned
+    host memory by DMA, with padded rows; the same as the oracle."""
+    W, H, enc = _encoded(oracle, 0, 61, 0.5)
+    p = g.plugin()
+    data = np.ascontiguousarray(enc["data"])
+    pitch = W * 6 + 64
+    hin = p.host_alloc(data.size)
+    hout = p.host_alloc(pitch * H)
+    assert hin and hout
+    dec = g.Decoder(device, g.FrameLayout(W, H, 0), 0, False)
+    try:
+        C.memmove(hin, data.ctypes.data, data.size)
+        C.memset(hout, 0xEE, pitch * H)
+        codes = block_codes(enc["info"], enc["pass_len"])
+        dec.decode((hin, data.size), codes, out=(hout, pitch))
+        got = np.ctypeslib.as_array((C.c_uint8 * (pitch * H)).from_address(hout)).reshape(H, pitch)
+        want, _ = oracle.decode_frame(W, H, 0, data, codes, 0, False)
+        _compare("pinned", got[:, :W * 6].copy().view(np.uint16), want)
+        assert np.all(got[:, W * 6:] == 0xEE)
+    finally:
+        dec.close()
+        p.host_free(C.c_void_p(hin))
+        p.host_free(C.c_void_p(hout))
+
+
 def test_decoders_on_threads(device, oracle):
     """Decoders on two threads at once (each on its own stream) give the oracle's pictures."""
     W, H, enc = _encoded(oracle, 0, 61, 0.5)
