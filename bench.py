@@ -128,6 +128,14 @@ def decode_bench(g, np, dev, layout, frames, decoders=4, rounds=4, cpu=True):
                 raise errors[0]
             return total / dt
 
+        # the floor of one frame's Tier-1 time: its longest code-block decoded alone
+        d0, blocks0 = items[0]
+        alone = blocks0.copy()
+        big = int(np.argmax(alone["len"]))
+        keep = alone[big].copy()
+        alone["passes"] = 0
+        alone[big] = keep
+        t_alone = min(decs[0].decode((dinb[0], d0.size), alone, out=(dout[0], W * 6))[1]["t1"] for _ in range(3))
         run(hin, hout, decoders)  # warm-up
         total = n * rounds
         host_fps = run(hin, hout, total)
@@ -144,6 +152,7 @@ def decode_bench(g, np, dev, layout, frames, decoders=4, rounds=4, cpu=True):
     out = {"fps_host_memory": round(host_fps, 2), "fps_hbm_resident": round(hbm_fps, 2),
            "decoders": decoders, "frames": total, "distinct_frames": n,
            "stage_ms_one_in_flight": stage_ms, "mbit_per_s_at_24fps": round(mbps, 1),
+           "longest_block_alone": {"bytes": int(keep["len"]), "tier1_ms": round(t_alone, 3)},
            "reference_published_fps": {"RTX 3090 CUDA 4K": 121.4, "RTX 5060 Ti CUDA 4K": 57.3},
            "workload": "4K flat 3996x2160, 24 fps / 250 Mbit/s device rate control, decoded to xyz12 words"}
     if cpu:
