@@ -157,10 +157,31 @@ uint32_t Decoder::buildDescs(const Layout& frame, const uint8_t* coded, const De
     // over about eight waves per compute unit (measured best on 4K frames at 250 Mbit/s).
     const uint64_t budget = std::max<uint64_t>(384, total / (uint64_t)(8 * cus_));
     const uint32_t lanes = (uint32_t)kDecWaveLanes;
+    // blocks over the budget would sit alone in a wave: they get a whole wave each and are
+    // decoded with the wave's scalar unit
+    uint32_t nlong = 0;
+    while (nlong < order_.size() && blocks[order_[nlong]].len > budget) ++nlong;
+    nlong_ = nlong;
+    auto entry = [&](uint32_t i) {
+        const DecCode& c = blocks[i];
+        const CodeBlock& cb = L_.cblks[i];
+        const Subband& b = L_.bands[cb.band];
+        DecBlock d;
+        d.off = c.offset;
+        d.len = c.len;
+        d.geom = (uint32_t)(cb.w - 1) | ((uint32_t)(cb.h - 1) << 5) | ((uint32_t)b.orient << 10) |
+                 ((uint32_t)c.nbps << 12) | ((uint32_t)c.passes << 18);
+        d.dst = (uint32_t)b.comp * (uint32_t)L_.width * (uint32_t)L_.height +
+                (uint32_t)(b.py0 + cb.y0) * (uint32_t)L_.width + (uint32_t)(b.px0 + cb.x0);
+        return d;
+    };
     auto pack = [&](DecBlock* out) {
         uint32_t n = 0, inWave = 0;
         uint64_t waveBytes = 0;
-        for (uint32_t i : order_) {
+        for (uint32_t k = 0; k < nlong; ++k, ++n)
+            if (out) out[n] = entry(order_[k]);
+        for (uint32_t k = nlong; k < order_.size(); ++k) {
+            const uint32_t i = order_[k];
             const DecCode& c = blocks[i];
             if (inWave && (inWave == lanes || waveBytes + c.len > budget)) {
                 if (out)
@@ -170,18 +191,7 @@ uint32_t Decoder::buildDescs(const Layout& frame, const uint8_t* coded, const De
                 inWave = 0;
                 waveBytes = 0;
             }
-            if (out) {
-                const CodeBlock& cb = L_.cblks[i];
-                const Subband& b = L_.bands[cb.band];
-                DecBlock d;
-                d.off = c.offset;
-                d.len = c.len;
-                d.geom = (uint32_t)(cb.w - 1) | ((uint32_t)(cb.h - 1) << 5) | ((uint32_t)b.orient << 10) |
-                         ((uint32_t)c.nbps << 12) | ((uint32_t)c.passes << 18);
-                d.dst = (uint32_t)b.comp * (uint32_t)L_.width * (uint32_t)L_.height +
-                        (uint32_t)(b.py0 + cb.y0) * (uint32_t)L_.width + (uint32_t)(b.px0 + cb.x0);
-                out[n] = d;
-            }
+            if (out) out[n] = entry(i);
             ++n;
             ++inWave;
             waveBytes += c.len;
@@ -193,9 +203,9 @@ uint32_t Decoder::buildDescs(const Layout& frame, const uint8_t* coded, const De
                 n += lanes - inWave;
         }
         // whole workgroups
-        while (n % (uint32_t)kDecSlots) {
-            if (out) out[n] = DecBlock{0, 0, 0, kDecEmpty};
-            ++n;
+        while (t1_decode_waves(n, nlong) % (uint32_t)kDecWaves) {
+            for (uint32_t j = 0; j < lanes; ++j, ++n)
+                if (out) out[n] = DecBlock{0, 0, 0, kDecEmpty};
         }
         return n;
     };
@@ -250,6 +260,7 @@ void Decoder::decode(const Layout& frame, const uint8_t* data, size_t size, cons
     tp.bytes = dBytes_;
     tp.blocks = dDescs_;
     tp.nblocks = nd;
+    tp.nlong = nlong_;
     tp.words = dWords_;
     tp.nwords = (uint32_t)(3 * npix);
     tp.stride = W;

@@ -77,6 +77,7 @@ private:
     DecBlock* dDescs_ = nullptr;
     DecBlock* hDescs_ = nullptr;
     size_t capDescs_ = 0;
+    uint32_t nlong_ = 0;  // leading table entries decoded a whole wave each
     int cus_ = 256;
     std::vector<float> hWave_;
 };

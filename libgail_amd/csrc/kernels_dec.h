@@ -23,8 +23,9 @@ constexpr int kDecSlots = kDecWaveLanes * kDecWaves;
 
 struct T1DecodeParams {
     const uint8_t* bytes;  // codestream, readable 64 bytes past its end
-    const DecBlock* blocks;
-    uint32_t nblocks;      // kDecWaveLanes entries per wave, a multiple of kDecSlots
+    const DecBlock* blocks;  // nlong long blocks (one wave each), then kDecWaveLanes per wave
+    uint32_t nblocks;      // entries, covering whole workgroups of kDecWaves waves
+    uint32_t nlong;
     uint32_t* words;       // 3 Mallat planes of reconstruction words, zero beforehand
     uint32_t nwords;
     uint32_t stride;       // words per plane row (the frame width)
@@ -60,6 +61,10 @@ struct IctPack {
 };
 
 hipError_t dec_tables_init();
+// Waves (and so workgroups) the table of p covers.
+inline uint32_t t1_decode_waves(uint32_t nblocks, uint32_t nlong) {
+    return nlong + (nblocks - nlong) / (uint32_t)kDecWaveLanes;
+}
 hipError_t launch_t1_decode(const T1DecodeParams& p, hipStream_t s);
 void idwt_tile_size(int& tx, int& ty);
 hipError_t launch_idwt_level(const IdwtLevel& p, hipStream_t s);

@@ -1,8 +1,9 @@
 // Decoder kernels for CDNA4 (gfx950, wave64).
 //
-// Tier-1: one lane per code-block (t1_decode_block.h), blocks ordered longest first and
-// packed into waves of at most 16 blocks by a byte budget on the host so a wave's lanes finish
-// together. A lane's row masks and context states live in LDS at [entry][slot] (conflict-free
+// Tier-1: the frame's longest code-blocks get a wave each, whose lanes all run the same
+// values, so the decode runs on scalar registers and branches; the rest go one lane per block
+// (t1_decode_block.h), ordered longest first and packed into waves of at most 16 blocks by a
+// byte budget on the host so a wave's lanes finish together. A lane's row masks and context states live in LDS at [entry][slot] (conflict-free
 // whatever row each lane is on); only 16 lanes of each wave hold a slot, so four waves share
 // the LDS of one and a SIMD keeps several waves to hide the serial chain's latencies. The
 // codeword comes through a 16-byte register window, and the reconstruction words go out as
@@ -29,7 +30,7 @@ namespace gail_amd {
 
 namespace {
 
-__constant__ uint8_t kDecZc[4 * 128];
+__constant__ __attribute__((aligned(16))) uint8_t kDecZc[4 * 128];
 __constant__ uint32_t kDecMq[47];
 
 // ISO/IEC 15444-1 Table F.4, rounded to float as the CPU reference rounds them.
@@ -46,11 +47,21 @@ constexpr float kR2 = (float)1.402, kG1 = (float)-0.34413, kG2 = (float)-0.71414
 struct LdsState {
     uint32_t (*masks)[32][kDecSlots];
     uint32_t (*ctxs)[kDecSlots];
+    const uint32_t* mq;   // Table C.2 in LDS
+    const uint8_t* zct;   // Table D.1 of the block's orientation in LDS
     int lane;
+    __device__ __forceinline__ void reset() {
+        for (uint32_t r = 0; r < 32u; ++r)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) masks[t][r][lane] = 0u;
+        for (uint32_t c = 0; c < 19u; ++c) ctxs[c][lane] = mq[t1d_initial_state(c)];
+    }
     __device__ __forceinline__ uint32_t rows(int t, uint32_t r) const { return masks[t][r][lane]; }
     __device__ __forceinline__ void set_rows(int t, uint32_t r, uint32_t v) { masks[t][r][lane] = v; }
     __device__ __forceinline__ uint32_t ctx(uint32_t c) const { return ctxs[c][lane]; }
     __device__ __forceinline__ void set_ctx(uint32_t c, uint32_t v) { ctxs[c][lane] = v; }
+    __device__ __forceinline__ uint32_t tab(uint32_t i) const { return mq[i]; }
+    __device__ __forceinline__ uint32_t zc(uint32_t code) const { return zct[code]; }
 };
 
 struct AtomicSink {
@@ -61,6 +72,54 @@ struct AtomicSink {
     }
 };
 
+// A whole wave on one block: every decoding value is the same in all lanes, so the compiler
+// keeps it in scalar registers and the control flow on scalar branches (no exec-mask work).
+// The block's state lives in vector registers used as per-lane cells, read and written with
+// readlane / writelane (a few cycles, no LDS round trip on the symbol chain): lane r of m[t]
+// holds row mask r of set t, lane k of c context k; lane i of mqv holds Table C.2 entry i,
+// lane j of zcv the zero-coding contexts of codes 4j..4j+3 for the block's orientation. A write
+// is a select on the lane number. Lane 0 alone adds to the words.
+struct WaveRegState {
+    uint32_t m0, m1, m2, m3, c, mqv, zcv, lid;
+    __device__ __forceinline__ void reset() {
+        m0 = m1 = m2 = m3 = 0u;
+        c = kDecMq[t1d_initial_state(threadIdx.x & 63)];  // lanes past 18 hold unused states
+    }
+    __device__ __forceinline__ uint32_t& set(int t) { return t == 0 ? m0 : (t == 1 ? m1 : (t == 2 ? m2 : m3)); }
+    __device__ __forceinline__ uint32_t rows(int t, uint32_t r) const {
+        return (uint32_t)__builtin_amdgcn_readlane((int)(t == 0 ? m0 : (t == 1 ? m1 : (t == 2 ? m2 : m3))), (int)r);
+    }
+    __device__ __forceinline__ void set_rows(int t, uint32_t r, uint32_t v) {
+        uint32_t& m = set(t);
+        m = lid == r ? v : m;
+    }
+    __device__ __forceinline__ uint32_t ctx(uint32_t k) const { return (uint32_t)__builtin_amdgcn_readlane((int)c, (int)k); }
+    __device__ __forceinline__ void set_ctx(uint32_t k, uint32_t v) { c = lid == k ? v : c; }
+    __device__ __forceinline__ uint32_t tab(uint32_t i) const { return (uint32_t)__builtin_amdgcn_readlane((int)mqv, (int)i); }
+    __device__ __forceinline__ uint32_t zc(uint32_t code) const {
+        return ((uint32_t)__builtin_amdgcn_readlane((int)zcv, (int)(code >> 2)) >> (8u * (code & 3u))) & 0xFFu;
+    }
+};
+
+struct WaveSink {
+    uint32_t* base;
+    uint32_t stride;
+    __device__ __forceinline__ void add(uint32_t i, uint32_t v) {
+        if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(base + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
+
+__device__ __forceinline__ bool block_fields(const T1DecodeParams& P, const DecBlock& d, uint32_t& w, uint32_t& h,
+                                             uint32_t& orient, uint32_t& nbps, uint32_t& passes) {
+    w = (d.geom & 31u) + 1u;
+    h = ((d.geom >> 5) & 31u) + 1u;
+    orient = (d.geom >> 10) & 3u;
+    nbps = (d.geom >> 12) & 63u;
+    passes = (d.geom >> 18) & 255u;
+    // the host builds the table from the layout; a wrong one still writes nothing outside
+    return d.dst < P.nwords && P.nwords - d.dst >= (h - 1u) * P.stride + w;
+}
+
 __global__ void __launch_bounds__(64 * kDecWaves) t1_decode_kernel(T1DecodeParams P) {
     __shared__ uint32_t s_rows[4][32][kDecSlots];
     __shared__ uint32_t s_ctx[19][kDecSlots];
@@ -70,19 +129,30 @@ __global__ void __launch_bounds__(64 * kDecWaves) t1_decode_kernel(T1DecodeParam
     if (t < 47) s_tab[t] = kDecMq[t];
     for (int i = t; i < 4 * 128; i += 64 * kDecWaves) s_zc[i] = kDecZc[i];
     __syncthreads();
-    const int lane = t & 63, wave = t >> 6;
+    const int lane = t & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const uint32_t gw = blockIdx.x * kDecWaves + wave;
+    uint32_t w, h, orient, nbps, passes;
+    if (gw < P.nlong) {
+        // one of the frame's longest blocks: the whole wave decodes it
+        const DecBlock d = P.blocks[gw];
+        if (d.dst == kDecEmpty || !block_fields(P, d, w, h, orient, nbps, passes)) return;
+        WaveRegState st;
+        st.lid = (uint32_t)lane;
+        st.mqv = lane < 47 ? kDecMq[lane] : 0u;
+        st.zcv = lane < 32 ? reinterpret_cast<const uint32_t*>(kDecZc + 128u * orient)[lane] : 0u;
+        WaveSink out{P.words + d.dst, P.stride};
+        t1d_decode_block(st, P.bytes, d.off, d.len, w, h, nbps, passes, out);
+        return;
+    }
     if (lane >= kDecWaveLanes) return;
-    const uint32_t i = (blockIdx.x * kDecWaves + wave) * kDecWaveLanes + lane;
+    const uint32_t i = P.nlong + (gw - P.nlong) * kDecWaveLanes + lane;
     if (i >= P.nblocks) return;
     const DecBlock d = P.blocks[i];
-    if (d.dst == kDecEmpty) return;
-    const uint32_t w = (d.geom & 31u) + 1u, h = ((d.geom >> 5) & 31u) + 1u, orient = (d.geom >> 10) & 3u;
-    const uint32_t nbps = (d.geom >> 12) & 63u, passes = (d.geom >> 18) & 255u;
-    // the host builds the table from the layout; a wrong one still writes nothing outside
-    if (d.dst >= P.nwords || P.nwords - d.dst < (h - 1u) * P.stride + w) return;
-    LdsState st{s_rows, s_ctx, wave * kDecWaveLanes + lane};
+    if (d.dst == kDecEmpty || !block_fields(P, d, w, h, orient, nbps, passes)) return;
+    LdsState st{s_rows, s_ctx, s_tab, s_zc + 128u * orient, (int)wave * kDecWaveLanes + lane};
     AtomicSink out{P.words + d.dst, P.stride};
-    t1d_decode_block(st, P.bytes, d.off, d.len, w, h, orient, nbps, passes, s_zc, s_tab, out);
+    t1d_decode_block(st, P.bytes, d.off, d.len, w, h, nbps, passes, out);
 }
 
 // ---------------------------------------------------------------- inverse DWT
@@ -264,7 +334,8 @@ hipError_t dec_tables_init() {
 
 hipError_t launch_t1_decode(const T1DecodeParams& p, hipStream_t s) {
     if (!p.nblocks) return hipSuccess;
-    hipLaunchKernelGGL(t1_decode_kernel, dim3(p.nblocks / kDecSlots), dim3(64 * kDecWaves), 0, s, p);
+    const uint32_t groups = (t1_decode_waves(p.nblocks, p.nlong) + kDecWaves - 1) / kDecWaves;
+    hipLaunchKernelGGL(t1_decode_kernel, dim3(groups), dim3(64 * kDecWaves), 0, s, p);
     return hipGetLastError();
 }
 

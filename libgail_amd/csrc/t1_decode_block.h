@@ -11,8 +11,11 @@
 // +-2^p. The words start at zero, so the sum is the word of t1_dec.cpp:85-97, 117-127.
 //
 // The caller supplies:
-//   State  rows(t, r) / set_rows(t, r, v) for t = 0 significant, 1 negative, 2 visited,
-//          3 refined; ctx(cx) / set_ctx(cx, v) for the 19 context states
+//   State  reset(): all row masks zero, the contexts in their Table D.7 states; rows(t, r) /
+//          set_rows(t, r, v) for t = 0 significant, 1 negative, 2 visited, 3 refined;
+//          ctx(cx) / set_ctx(cx, v) for the 19 context states; tab(i), entry i of Table C.2
+//          packed as below; zc(code), the zero-coding context of a neighbourhood code for the
+//          block's orientation (Table D.1)
 //   Sink   add(i, v): add v to the word of sample i = row * stride + column
 // The codeword is read 8 bytes at a time: src must be readable from (start & ~7) up to
 // 16 bytes past start + len.
@@ -28,6 +31,12 @@
 namespace gail_amd {
 
 constexpr uint32_t kDecCtxRl = 17, kDecCtxUni = 18;
+
+// Table C.2 entry a context starts from (Table D.7): context 0 at state 4, run length at 3,
+// uniform at 46, the rest at 0.
+T1D_FN uint32_t t1d_initial_state(uint32_t cx) {
+    return cx == 0u ? 4u : (cx == kDecCtxRl ? 3u : (cx == kDecCtxUni ? 46u : 0u));
+}
 
 T1D_FN uint32_t t1d_popc(uint32_t v) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -109,8 +118,11 @@ struct MqDec {
     }
     // DECODE (C.3.2) with RENORMD (C.3.3) in context cx; a context state is its Table C.2
     // word (Qe | NMPS << 16 | NLPS << 22 | SWITCH << 28) with the MPS in bit 31.
-    template <class State> T1D_FN uint32_t decode(State& st, uint32_t cx, const uint32_t* tab) {
+    template <class State> T1D_FN uint32_t decode(State& st, uint32_t cx) {
         const uint32_t s = st.ctx(cx);
+        // both successor states, read before they are known to be needed so the table reads
+        // overlap the interval arithmetic instead of following it
+        const uint32_t t_mps = st.tab((s >> 16) & 63u), t_lps = st.tab((s >> 22) & 63u);
         const uint32_t qe = s & 0xFFFFu, mps = s >> 31;
         A -= qe;
         uint32_t d;
@@ -122,11 +134,7 @@ struct MqDec {
             d = A < qe ? mps : mps ^ 1u;  // LPS exchange
             A = qe;
         }
-        if (d == mps) {
-            st.set_ctx(cx, tab[(s >> 16) & 63u] | (mps << 31));
-        } else {
-            st.set_ctx(cx, tab[(s >> 22) & 63u] | ((mps ^ ((s >> 28) & 1u)) << 31));
-        }
+        st.set_ctx(cx, d == mps ? t_mps | (mps << 31) : t_lps | ((mps ^ ((s >> 28) & 1u)) << 31));
         // shift until A >= 0x8000, as many bits at a time as C holds
         int n = t1d_clz(A) - 16;
         while (n > 0) {
@@ -155,8 +163,8 @@ T1D_FN uint32_t t1d_zc_code(const uint32_t (&S)[6], int k, uint32_t x) {
 // The sample (row k of the stripe, column x) is significant from plane p on: decode its
 // sign (D.3.2) and add 3 * 2^p with the sign to its word.
 template <class State, class Sink>
-T1D_FN void t1d_become_sig(MqDec& mq, State& st, const uint32_t* tab, Sink& out, uint32_t (&S)[6], uint32_t (&G)[6],
-                           int k, uint32_t x, uint32_t p, uint32_t row_off) {
+T1D_FN void t1d_become_sig(MqDec& mq, State& st, Sink& out, uint32_t (&S)[6], uint32_t (&G)[6], int k, uint32_t x,
+                           uint32_t p, uint32_t row_off) {
     const uint32_t bit = 1u << x;
     const uint32_t left = x ? bit >> 1 : 0u, right = bit << 1;  // right is 0 for x = 31
     // contributions: +1 significant positive, -1 significant negative
@@ -168,7 +176,7 @@ T1D_FN void t1d_become_sig(MqDec& mq, State& st, const uint32_t* tab, Sink& out,
     hc = hc < -1 ? -1 : (hc > 1 ? 1 : hc);
     vc = vc < -1 ? -1 : (vc > 1 ? 1 : vc);
     const uint32_t e = (uint32_t)(kSignTable >> (4 * (3 * (hc + 1) + (vc + 1)))) & 15u;
-    const uint32_t neg = mq.decode(st, 9u + (e & 7u), tab) ^ (e >> 3);
+    const uint32_t neg = mq.decode(st, 9u + (e & 7u)) ^ (e >> 3);
     S[k + 1] |= bit;
     if (neg) G[k + 1] |= bit;
     out.add(row_off + (uint32_t)k * out.stride + x, (neg << 31) | (3u << p));
@@ -178,17 +186,10 @@ T1D_FN void t1d_become_sig(MqDec& mq, State& st, const uint32_t* tab, Sink& out,
 // magnitude bit-planes (1..30); passes past the last plane are not decoded.
 template <class State, class Sink>
 T1D_FN void t1d_decode_block(State& st, const uint8_t* src, uint32_t start, uint32_t len, uint32_t w, uint32_t h,
-                             uint32_t orient, uint32_t nbps, uint32_t passes, const uint8_t* zc, const uint32_t* tab,
-                             Sink& out) {
+                             uint32_t nbps, uint32_t passes, Sink& out) {
     if (w == 0u || h == 0u || w > 32u || h > 32u || nbps == 0u || nbps > 30u || passes == 0u) return;
     if (passes > 3u * nbps - 2u) passes = 3u * nbps - 2u;
-    const uint8_t* zco = zc + 128u * (orient & 3u);
-    for (uint32_t r = 0; r < 32u; ++r)
-        for (int t = 0; t < 4; ++t) st.set_rows(t, r, 0u);
-    for (uint32_t c = 0; c < 19u; ++c) st.set_ctx(c, tab[0]);
-    st.set_ctx(0, tab[4]);
-    st.set_ctx(kDecCtxRl, tab[3]);
-    st.set_ctx(kDecCtxUni, tab[46]);
+    st.reset();
     MqDec mq;
     mq.init(src, start, len);
     const uint32_t wmask = w == 32u ? 0xFFFFFFFFu : (1u << w) - 1u;
@@ -229,8 +230,8 @@ T1D_FN void t1d_decode_block(State& st, const uint8_t* src, uint32_t start, uint
                         const uint32_t code = t1d_zc_code(S, k, x);
                         if (!code) continue;
                         V[k] |= bit;
-                        if (mq.decode(st, zco[code], tab)) {
-                            t1d_become_sig(mq, st, tab, out, S, G, k, x, p, row_off);
+                        if (mq.decode(st, st.zc(code))) {
+                            t1d_become_sig(mq, st, out, S, G, k, x, p, row_off);
                             todo |= (bit << 1) & wmask;  // the next column gains a neighbour
                         }
                     }
@@ -260,7 +261,7 @@ T1D_FN void t1d_decode_block(State& st, const uint8_t* src, uint32_t start, uint
                     for (int k = 0; k < 4; ++k) {
                         if (!(S[k + 1] & ~V[k] & bit)) continue;
                         const uint32_t cx = (R[k] & bit) ? 16u : (t1d_zc_code(S, k, x) ? 15u : 14u);
-                        const uint32_t b = mq.decode(st, cx, tab);
+                        const uint32_t b = mq.decode(st, cx);
                         R[k] |= bit;
                         out.add(row_off + (uint32_t)k * out.stride + x, b ? (1u << p) : (0u - (1u << p)));
                     }
@@ -286,19 +287,19 @@ T1D_FN void t1d_decode_block(State& st, const uint8_t* src, uint32_t start, uint
                     const uint32_t near = ((bit << 1) | bit | (bit >> 1));
                     if ((all4 & bit) && !((S[0] | S[1] | S[2] | S[3] | S[4] | S[5]) & near)) {
                         // run-length mode: four untouched samples without significant neighbours
-                        if (!mq.decode(st, kDecCtxRl, tab)) continue;
-                        k0 = (int)(mq.decode(st, kDecCtxUni, tab) << 1);
-                        k0 |= (int)mq.decode(st, kDecCtxUni, tab);
+                        if (!mq.decode(st, kDecCtxRl)) continue;
+                        k0 = (int)(mq.decode(st, kDecCtxUni) << 1);
+                        k0 |= (int)mq.decode(st, kDecCtxUni);
 #pragma unroll
                         for (int k = 0; k < 4; ++k)
-                            if (k == k0) t1d_become_sig(mq, st, tab, out, S, G, k, x, p, row_off);
+                            if (k == k0) t1d_become_sig(mq, st, out, S, G, k, x, p, row_off);
                         ++k0;
                     }
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
                         if (k < k0 || !(U[k] & bit)) continue;
-                        if (mq.decode(st, zco[t1d_zc_code(S, k, x)], tab))
-                            t1d_become_sig(mq, st, tab, out, S, G, k, x, p, row_off);
+                        if (mq.decode(st, st.zc(t1d_zc_code(S, k, x))))
+                            t1d_become_sig(mq, st, out, S, G, k, x, p, row_off);
                     }
                 }
 #pragma unroll

@@ -15,10 +15,18 @@ namespace {
 struct HostState {
     uint32_t m[4][32];
     uint32_t c[19];
+    const uint32_t* mq;
+    const uint8_t* zct;  // Table D.1 of the block's orientation
+    void reset() {
+        std::memset(m, 0, sizeof m);
+        for (uint32_t i = 0; i < 19; ++i) c[i] = mq[t1d_initial_state(i)];
+    }
     uint32_t rows(int t, uint32_t r) const { return m[t][r]; }
     void set_rows(int t, uint32_t r, uint32_t v) { m[t][r] = v; }
     uint32_t ctx(uint32_t i) const { return c[i]; }
     void set_ctx(uint32_t i, uint32_t v) { c[i] = v; }
+    uint32_t tab(uint32_t i) const { return mq[i]; }
+    uint32_t zc(uint32_t code) const { return zct[code]; }
 };
 
 struct HostSink {
@@ -59,8 +67,9 @@ int host_t1_decode_words(const uint8_t* data, uint32_t len, uint32_t start, int 
     uint32_t out[32 * 32] = {};
     HostSink sink{out, 32u, 32u * 32u, 0};
     HostState st;
-    t1d_decode_block(st, buf.data(), start, len, (uint32_t)w, (uint32_t)h, (uint32_t)orient, (uint32_t)nbps,
-                     (uint32_t)passes, g_tab.zc, g_tab.mq, sink);
+    st.mq = g_tab.mq;
+    st.zct = g_tab.zc + 128 * (orient & 3);
+    t1d_decode_block(st, buf.data(), start, len, (uint32_t)w, (uint32_t)h, (uint32_t)nbps, (uint32_t)passes, sink);
     for (int y = 0; y < h; ++y) std::memcpy(words + (size_t)y * w, out + (size_t)y * 32, sizeof(uint32_t) * w);
     return sink.bad;
 }
