@@ -72,6 +72,35 @@ def test_decoder_matches_oracle(device, oracle, case):
     assert tm["t1"] > 0 and tm["dwt"] >= 0 and tm["color"] > 0 and tm["total"] > 0
 
 
+@pytest.mark.parametrize("mbps", [40.0, 125.0])
+def test_decoder_on_rate_controlled_frames(device, oracle, mbps):
+    """The encoder's own output at a DCI rate (device rate control keeps each block's first
+    passes) through the decoder: equal to the oracle's decode of the same blocks. At the
+    lower rate few blocks are long enough to take a wave of their own."""
+    W, H = 1998, 1080
+    L = g.FrameLayout(W, H, 0)
+    frames = [g.synth_frame(70 + i, W, H) for i in range(2)]
+    b = g.Batch(device, [L] * 2, [g.with_rate_control(g.xyz_settings(L), fps=24.0, mbps=mbps)] * 2)
+    try:
+        for i, f in enumerate(frames):
+            b.upload(i, f)
+        res, _ = b.encode()
+    finally:
+        b.close()
+    dec = g.Decoder(device, L, 0, True)
+    try:
+        for r in res:
+            data = np.ascontiguousarray(r.data)
+            codes = g.block_table(r)
+            print(f"[decode rate {mbps}] {data.size} bytes, {int((codes['passes'] > 0).sum())} coded blocks, "
+                  f"longest {int(codes['len'].max())}", flush=True)
+            got, _, _ = dec.decode(data, codes)
+            want, _ = oracle.decode_frame(W, H, 0, data, codes, 0, True)
+            _compare(f"rate {mbps}", got, want)
+    finally:
+        dec.close()
+
+
 def test_decoder_device_memory_and_reuse(device, oracle):
     """The codestream and the picture in device memory (no host copies), and one decoder used
     for frames of other step sizes in turn (the frame's own layout)."""
