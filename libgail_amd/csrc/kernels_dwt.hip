@@ -4,7 +4,7 @@
 // workgroup owns a TX x TY tile of the level's LL region and all three components: it
 // stages the tile plus a 4-sample mirrored halo in LDS (coalesced row loads), lifts the
 // columns in registers (one thread per column, the whole column unrolled), then the rows
-// (one thread per row), and writes the de-interleaved results back with coalesced row
+// (two threads per row, each half with its own halo), and writes the de-interleaved results back with coalesced row
 // stores: the LL quarter to the next level's scratch plane, HL/LH/HH straight into the
 // frame's Mallat-layout coefficient planes. The order of operations on every sample is the
 // CPU reference's (columns, then rows; lifting steps alpha..delta; low * (1/K), high * K),
@@ -169,18 +169,29 @@ __global__ void __launch_bounds__(256) dwt_level_kernel(const DwtLevelDesc* __re
         }
         __syncthreads();
     }
-    // Rows (horizontal lifting), one thread per output row of each component.
+    // Rows (horizontal lifting): two threads per output row of each component, each lifting
+    // half the row with its own 4-sample halo (an output depends on the 4 samples either side
+    // only, so the halves compute exactly what the whole row would). Both halves of a row are
+    // in the same round, and every read of a round precedes its writes.
     if (w >= 2) {
-        for (int i = tid; i < 3 * TY; i += 256) {
-            const int c = i / TY, ry = 4 + (i - c * TY);
-            float v[RX];
+        constexpr int HX = TX / 2 + 8;
+        for (int r0 = 0; r0 < 6 * TY; r0 += 256) {
+            const int i = r0 + tid;
+            const bool act = i < 6 * TY;
+            const int c = i / (2 * TY), rr = i - c * (2 * TY), ry = 4 + (rr >> 1), xs = (rr & 1) * (TX / 2);
+            float v[HX];
+            if (act) {
 #pragma unroll
-            for (int k = 0; k < RX; ++k) v[k] = tile[c][ry][k];
-            lift97(v);
+                for (int k = 0; k < HX; ++k) v[k] = tile[c][ry][xs + k];
+            }
+            __syncthreads();
+            if (act) {
+                lift97(v);
 #pragma unroll
-            for (int k = 4; k < TX + 4; ++k) tile[c][ry][k] = v[k];
+                for (int k = 4; k < TX / 2 + 4; ++k) tile[c][ry][xs + k] = v[k];
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
     // De-interleaved stores: per tile row, the even samples (low) and the odd samples
     // (high) each form a contiguous run of TX / 2 outputs. Thread tid always writes output
