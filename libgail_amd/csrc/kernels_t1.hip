@@ -903,28 +903,31 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
                                 const bool add_c = is_mps != (a < qe);  // conditional exchange (C.2.5 / C.2.6)
                                 C += add_c ? qe : 0u;
                                 A = add_c ? a : qe;
-                                uint32_t st_new = st;
-                                if (!is_mps || !(a & 0x8000u)) {
-                                    const uint32_t nidx = is_mps ? ((st >> 16) & 63u) : ((st >> 22) & 63u);
-                                    const uint32_t nmps = is_mps ? mps : (mps ^ ((st >> 28) & 1u));
-                                    st_new = s_tab[nidx] | (nmps << 31);
-                                    s_ctx[cx][t] = st_new;
-                                    // RENORME: A and C shift until A >= 0x8000 (at most 15 bits), a
-                                    // byte out whenever CT runs out; a second one is rare
-                                    int nsh = __clz((int)A) - 16;
-                                    A <<= nsh;
-                                    int sh = nsh < CT ? nsh : CT;
+                                // The state changes and A, C renormalise unless the symbol is
+                                // the MPS with A still >= 0x8000; then the shift count below is
+                                // 0 and the state written back is the old one, so every lane
+                                // runs the same instructions and only byte-out branches.
+                                const bool renorm = !is_mps || !(a & 0x8000u);
+                                const uint32_t nidx = is_mps ? ((st >> 16) & 63u) : ((st >> 22) & 63u);
+                                const uint32_t nmps = is_mps ? mps : (mps ^ ((st >> 28) & 1u));
+                                const uint32_t t_new = s_tab[nidx] | (nmps << 31);
+                                const uint32_t st_new = renorm ? t_new : st;
+                                s_ctx[cx][t] = st_new;
+                                // RENORME: A and C shift until A >= 0x8000 (at most 15 bits), a
+                                // byte out whenever CT runs out; a second one is rare
+                                int nsh = __clz((int)A) - 16;
+                                A <<= nsh;
+                                int sh = nsh < CT ? nsh : CT;
+                                C <<= sh;
+                                CT -= sh;
+                                nsh -= sh;
+                                if (CT == 0) byteout();
+                                while (nsh > 0) {
+                                    sh = nsh < CT ? nsh : CT;
                                     C <<= sh;
                                     CT -= sh;
                                     nsh -= sh;
                                     if (CT == 0) byteout();
-                                    while (nsh > 0) {
-                                        sh = nsh < CT ? nsh : CT;
-                                        C <<= sh;
-                                        CT -= sh;
-                                        nsh -= sh;
-                                        if (CT == 0) byteout();
-                                    }
                                 }
                                 t_cur = cxn == cx ? st_new : t_next;
                                 ++pos;
