@@ -125,16 +125,17 @@ struct MqDec {
         const uint32_t t_mps = st.tab((s >> 16) & 63u), t_lps = st.tab((s >> 22) & 63u);
         const uint32_t qe = s & 0xFFFFu, mps = s >> 31;
         A -= qe;
-        uint32_t d;
-        if ((C >> 16) >= qe) {
-            C -= qe << 16;
-            if (A & 0x8000u) return mps;
-            d = A < qe ? mps ^ 1u : mps;  // MPS exchange
-        } else {
-            d = A < qe ? mps : mps ^ 1u;  // LPS exchange
-            A = qe;
-        }
-        st.set_ctx(cx, d == mps ? t_mps | (mps << 31) : t_lps | ((mps ^ ((s >> 28) & 1u)) << 31));
+        // upper sub-interval: the MPS unless A < Qe (MPS exchange); lower: the LPS unless
+        // A < Qe (LPS exchange). Without a renormalisation (the MPS with A >= 0x8000) the
+        // state is kept and the shift count below is 0, so lanes run the same instructions.
+        const bool upper = (C >> 16) >= qe;
+        const bool xchg = A < qe;
+        const uint32_t d = upper != xchg ? mps : mps ^ 1u;
+        C = upper ? C - (qe << 16) : C;
+        A = upper ? A : qe;
+        const bool renorm = !upper || !(A & 0x8000u);
+        const uint32_t s_new = d == mps ? t_mps | (mps << 31) : t_lps | ((mps ^ ((s >> 28) & 1u)) << 31);
+        st.set_ctx(cx, renorm ? s_new : s);
         // shift until A >= 0x8000, as many bits at a time as C holds
         int n = t1d_clz(A) - 16;
         while (n > 0) {
