@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes one DWT level-0 launch over 16 4K flat frames moves, from the FETCH_SIZE and
 # WRITE_SIZE counter runs in profiles/pmc_traffic_dwt_level0_4k_flat_batch16.md, each scaled by
 # what the counter reads for a 1 GiB device copy.
-DWT0_TRAFFIC_4K_FLAT_16 = 2498017159
+DWT0_TRAFFIC_4K_FLAT_16 = 2502515919
 
 
 def dwt_level0_bytes(W, H):
