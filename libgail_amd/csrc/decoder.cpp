@@ -134,9 +134,7 @@ void Decoder::ensureDescs(size_t n) {
     capDescs_ = cap;
 }
 
-uint32_t Decoder::buildDescs(const Layout& frame, const uint8_t* coded, const DecCode* blocks, size_t size) {
-    (void)frame;
-    (void)size;
+uint32_t Decoder::buildDescs(const uint8_t* coded, const DecCode* blocks) {
     // Coded blocks longest first (a counting sort over length classes).
     count_.assign(kLenKeys + 1, 0u);
     uint64_t total = 0;
@@ -237,7 +235,7 @@ void Decoder::decode(const Layout& frame, const uint8_t* data, size_t size, cons
         const DecCode& c = blocks[wanted_[k]];
         coded[k] = c.passes && c.nbps >= 1 && c.nbps <= 30 && (size_t)c.offset + c.len <= size;
     }
-    const uint32_t nd = buildDescs(frame, coded.data(), blocks, size);
+    const uint32_t nd = buildDescs(coded.data(), blocks);
     const Mem in = memKind(data), outKind = memKind(dst);
     const bool devOut = outKind == Mem::Device;
     // pinned pictures are written by DMA when their rows go forwards

@@ -54,8 +54,9 @@ private:
     void release();
     void ensureBytes(size_t n);
     void ensureDescs(size_t n);
-    // Orders the coded blocks longest first and packs them into waves; returns the entries.
-    uint32_t buildDescs(const Layout& frame, const uint8_t* validity, const DecCode* blocks, size_t size);
+    // Orders the blocks marked in coded[] (one flag per wanted block) longest first, gives the
+    // longest a wave each and packs the rest into waves; returns the table's entries.
+    uint32_t buildDescs(const uint8_t* coded, const DecCode* blocks);
 
     int dev_;
     Layout L_;
