@@ -101,6 +101,23 @@ def test_decoder_on_rate_controlled_frames(device, oracle, mbps):
         dec.close()
 
 
+@pytest.mark.parametrize("container", ["2k_scope", "2k_full", "4k_scope", "4k_full"])
+def test_decoder_other_containers(device, oracle, container):
+    """Scope and full DCI sizes (other band and edge-block shapes), reduce 1, damaged table."""
+    W, H, fourk = g.CONTAINERS[container]
+    x = g.synth_frame(90, W, H)
+    enc = oracle.encode_frame_xyz(x, fourk, thr=0.5, threads=8)
+    codes = _damaged(enc, 3)
+    print(f"[decode {container}] {W}x{H}", flush=True)
+    want, _ = oracle.decode_frame(W, H, fourk, enc["data"], codes, 1, True)
+    dec = g.Decoder(device, g.FrameLayout(W, H, fourk), 1, True)
+    try:
+        got, _, _ = dec.decode(enc["data"], codes)
+    finally:
+        dec.close()
+    _compare(container, got, want)
+
+
 def test_decoder_device_memory_and_reuse(device, oracle):
     """The codestream and the picture in device memory (no host copies), and one decoder used
     for frames of other step sizes in turn (the frame's own layout)."""

@@ -201,6 +201,17 @@ def test_decode_frame_matches_reference(oracle, ref, case):
     assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32))
 
 
+@pytest.mark.parametrize("container", ["2k_scope", "4k_full"])
+def test_decode_frame_other_containers_match_reference(oracle, ref, container):
+    W, H, fourk = g.CONTAINERS[container]
+    x = g.synth_frame(91, W, H)
+    enc = oracle.encode_frame_xyz(x, fourk, thr=0.5, threads=8)
+    codes = _codes_with_damage(enc, np.random.default_rng(5))
+    a, _ = oracle.decode_frame(W, H, fourk, enc["data"], codes, 1, True)
+    b, _ = ref.decode_frame(W, H, fourk, enc["data"], codes, 1, True)
+    assert np.array_equal(a, b)
+
+
 def test_codestream_blocks_match_tier1(oracle, ref):
     """The codestream written by libgail.so (rate control + Tier-2) read back by the
     reference's own header and packet parser: every code-block's bytes are the first
