@@ -60,6 +60,33 @@ def cpu_baseline(frames, fourk, thr, budget_s=20.0):
                       f"reference's t2.cpp needs a generated header and does not build here)"}
 
 
+def plain_parity(np, gpu, frame, fourk, thr, levels):
+    """Config-2 checks of one plain-path frame against the CPU path, run in the CPU-baseline
+    leg with the compiled reference (the oracle where it is not built) as the checker:
+    quantised-index mismatches, Tier-1 bit-exactness, and the largest difference between the
+    device's float32 DWT and a float64 DWT of the same ICT planes."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import Oracle, reference
+    from t1_compare import compare_tier1
+    oracle = Oracle()
+    ref = reference()
+    lib = ref if ref is not None else oracle
+    cores = len(os.sched_getaffinity(0))
+    want = lib.encode_frame_xyz(frame, fourk, thr=thr, threads=cores)
+    gi = (gpu.debug["coeffs"].reshape(-1) & 0x7FFFFFFF) >> 6
+    ri = (want["coeffs"].reshape(-1) & 0x7FFFFFFF) >> 6
+    d = np.abs(gi.astype(np.int64) - ri.astype(np.int64))
+    bad, _ = compare_tier1(gpu, want)
+    ict = oracle.xyz_to_ict(frame)
+    f64 = max(float(np.max(np.abs(oracle.fdwt_f64(ict[c], levels) - gpu.debug["wavelet"][c].astype(np.float64))))
+              for c in range(3))
+    return {"checker": "reference" if ref is not None else "port", "quant_mismatch_pm1": int(np.sum(d == 1)),
+            "quant_mismatch_gt1": int(np.sum(d > 1)), "sign_mismatches": int(np.sum(
+                (gpu.debug["coeffs"].reshape(-1) >> 31) != (want["coeffs"].reshape(-1) >> 31))),
+            "tier1_blocks_differing": int(bad), "tier1_bitexact": bad == 0,
+            "dwt_fp32_vs_fp64_max_abs": round(f64, 6)}
+
+
 def decode_bench(g, np, dev, layout, frames, decoders=4, rounds=4, cpu=True):
     """The HIP decoder on the reference's decode workload: 4K frames at 250 Mbit/s (24 fps
     rate control on the device), each frame's code-blocks located (the Tier-1 output read as
@@ -510,16 +537,22 @@ def main():
     # ---- plain path: one frame at a time, upload, Tier-1, rate control and Tier-2 in turn
     enc = g.Encoder(dev, layout, g.xyz_settings(layout))
     lam = 0.0
-    lat, lat_t1 = [], []
+    lat, lat_t1, plain_stages = [], [], []
     for i in range(max(args.plain_frames, 1) + 1):
         t1 = time.perf_counter()
         enc.set_slope(0.25 * lam)
         r = enc.encode(frames[i % B])
         t2 = time.perf_counter()
+        if i:
+            plain_stages.append(r.timings)
         _, lam, _ = g.encode_codestream(layout, r, fps=24.0, mbps=250.0, tag="hip")
         if i:
             lat.append(1000 * (time.perf_counter() - t1))
             lat_t1.append(1000 * (t2 - t1))
+    # one more plain frame with the debug taps, for the CPU-path checks
+    dbg_thr = 0.25 * lam
+    enc.set_slope(dbg_thr)
+    plain_dbg = enc.encode(frames[0], debug=True)
     enc.close()
     plain_fps = 1000.0 / statistics.mean(lat)
     print(f"[bench] plain: {plain_fps:.1f} frames/s (Tier-1 {statistics.mean(lat_t1):.2f} ms of "
@@ -557,9 +590,12 @@ def main():
         served = served_run(g, np, seconds=args.served_seconds)
         print(f"[bench] served: {json.dumps(served)}", file=sys.stderr, flush=True)
 
-    cpu = cfg1 = None
+    cpu = cfg1 = parity = None
     if not args.no_cpu:
         cpu = cpu_baseline(frames[:4], fourk, thr)
+        parity = plain_parity(np, plain_dbg, frames[0], fourk, dbg_thr, layout.levels)
+        parity["frame"] = f"{args.profile} seed 1000, early termination threshold {dbg_thr:.4g}"
+        print(f"[bench] plain-path checks: {json.dumps(parity)}", file=sys.stderr, flush=True)
         cfg1 = cpu_config1(g, np)
         print(f"[bench] CPU 2K frame: {json.dumps(cfg1)}", file=sys.stderr, flush=True)
 
@@ -581,6 +617,9 @@ def main():
         "batched_over_plain": round(value / plain_fps, 3),
         "plain_latency_ms": {"p50": _pct(np, lat, 50), "p99": _pct(np, lat, 99),
                              "tier1_p50": _pct(np, lat_t1, 50)},
+        "plain_stage_ms": {k: round(statistics.median(t[k] for t in plain_stages), 3)
+                           for k in ("upload", "color", "dwt", "t1cm", "t1mq", "download", "gpu", "total")},
+        "plain_checks": parity,
         "tier1_batched_fps": round(t1_fps, 2),
         "tier1_batch": {"frames": B, "batches_in_flight": nb, "ms_per_batch": round(ms_step, 3),
                         "slope_threshold": round(thr, 3)},
