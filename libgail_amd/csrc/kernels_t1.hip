@@ -813,32 +813,28 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
         s_ctx[18][t] = s_tab[46];
         uint32_t A = 0x8000u, C = 0u, B = 0u;
         int CT = 12, bp = -1;
-        uint32_t word = 0;              // finished bytes of the current aligned 4-byte group
-        uint4 ob = make_uint4(0u, 0u, 0u, 0u);  // finished words of the current 16 bytes
-        uint32_t flushed = 0;           // codeword bytes already written out (a multiple of 16)
+        // Finished bytes of the current aligned 8-byte group. The byte at bp = -1 is the
+        // coder's initial dummy: it is 0 and takes no carry (C + A stays within 2^27 until the
+        // first byte-out), so OR-ing it in changes nothing.
+        unsigned long long acc = 0ull;
+        uint32_t flushed = 0;           // codeword bytes already written out (a multiple of 8)
         // BYTEOUT (C.2.7) with the pending byte B kept in a register: it is final once the
-        // next byte starts. Branch-free but for the 16-byte flush: every lane of the wave
-        // runs it whenever one lane needs it. Finished bytes go out 16 at a time.
+        // next byte starts. Branch-free but for the 8-byte flush; the wave runs it whenever
+        // one lane needs it, so it is kept short.
         auto byteout = [&]() {
             // a carry into B unless B is 0xFF; a B of 0xFF (before or after the carry) is
             // followed by a stuffed byte of 7 bits
             const uint32_t carry = B != 0xFFu ? (C >> 27) & 1u : 0u;
             B += carry;
+            C -= carry << 27;
             const bool stuff = B == 0xFFu;
-            C = carry && stuff ? C & 0x7FFFFFFu : C;
-            word |= (bp >= 0 ? B : 0u) << (8 * (bp & 3));
-            const bool full = bp >= 0 && (bp & 3) == 3;
-            const int wi = (bp >> 2) & 3;
-            ob.x = full && wi == 0 ? word : ob.x;
-            ob.y = full && wi == 1 ? word : ob.y;
-            ob.z = full && wi == 2 ? word : ob.z;
-            ob.w = full && wi == 3 ? word : ob.w;
-            word = full ? 0u : word;
-            if (full && wi == 3) {
-                const uint32_t at = (uint32_t)bp & ~15u;
-                if (at + 16u <= cap) *(uint4*)(out + at) = ob;
+            acc |= (unsigned long long)B << (8 * (bp & 7));
+            if (((uint32_t)bp & 0x80000007u) == 7u) {  // bp >= 0 ends a group
+                const uint32_t at = (uint32_t)bp & ~7u;
+                if (at < cap) *(unsigned long long*)(out + at) = acc;  // cap: a multiple of 64
                 else overflow = 1u;
-                flushed = at + 16u;
+                acc = 0ull;
+                flushed = at + 8u;
             }
             ++bp;
             B = stuff ? C >> 20 : (C >> 19) & 0xFFu;
@@ -893,41 +889,44 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
                                 }
                                 const uint32_t sv = (w[b >> 2] >> (8 * (b & 3))) & 0xFFu;
                                 const uint32_t sn = b + 1 < m ? (w[(b + 1) >> 2] >> (8 * ((b + 1) & 3))) & 0xFFu : w[4];
-                                const uint32_t cx = sv >> 1, d = sv & 1u, cxn = sn >> 1;
+                                const uint32_t cx = sv >> 1, cxn = sn >> 1;
                                 // the next symbol's context state, read before this symbol may change it
                                 const uint32_t t_next = s_ctx[cxn][t];
                                 const uint32_t st = t_cur;
-                                const uint32_t qe = st & 0xFFFFu, mps = st >> 31;
-                                const uint32_t a = A - qe;
-                                const bool is_mps = d == mps;
+                                const uint32_t qe = st & 0xFFFFu;
+                                const uint32_t a = A - qe;  // A <= 0xFFFF, so a < 0x10000
+                                const bool is_mps = (int)(st ^ (sv << 31)) >= 0;  // the decision equals the MPS
                                 const bool add_c = is_mps != (a < qe);  // conditional exchange (C.2.5 / C.2.6)
                                 C += add_c ? qe : 0u;
                                 A = add_c ? a : qe;
                                 // The state changes and A, C renormalise unless the symbol is
                                 // the MPS with A still >= 0x8000; then the shift count below is
                                 // 0 and the state written back is the old one, so every lane
-                                // runs the same instructions and only byte-out branches.
-                                const bool renorm = !is_mps || !(a & 0x8000u);
+                                // runs the same instructions and only byte-out branches. After
+                                // an LPS the MPS flips where SWITCH (bit 28) is set.
+                                const bool renorm = !is_mps || a < 0x8000u;
                                 const uint32_t nidx = is_mps ? ((st >> 16) & 63u) : ((st >> 22) & 63u);
-                                const uint32_t nmps = is_mps ? mps : (mps ^ ((st >> 28) & 1u));
-                                const uint32_t t_new = s_tab[nidx] | (nmps << 31);
+                                const uint32_t msb = is_mps ? st : st ^ (st << 3);
+                                const uint32_t t_new = (msb & 0x80000000u) | s_tab[nidx];
                                 const uint32_t st_new = renorm ? t_new : st;
                                 s_ctx[cx][t] = st_new;
                                 // RENORME: A and C shift until A >= 0x8000 (at most 15 bits), a
-                                // byte out whenever CT runs out; a second one is rare
+                                // byte out whenever CT runs out; a second one is rare (an LPS
+                                // with a small Qe)
                                 int nsh = __clz((int)A) - 16;
                                 A <<= nsh;
                                 int sh = nsh < CT ? nsh : CT;
                                 C <<= sh;
                                 CT -= sh;
-                                nsh -= sh;
-                                if (CT == 0) byteout();
-                                while (nsh > 0) {
-                                    sh = nsh < CT ? nsh : CT;
-                                    C <<= sh;
-                                    CT -= sh;
+                                if (CT == 0) {
                                     nsh -= sh;
-                                    if (CT == 0) byteout();
+                                    do {
+                                        byteout();
+                                        sh = nsh < CT ? nsh : CT;
+                                        C <<= sh;
+                                        CT -= sh;
+                                        nsh -= sh;
+                                    } while (CT == 0);
                                 }
                                 t_cur = cxn == cx ? st_new : t_next;
                                 ++pos;
@@ -958,21 +957,16 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
         byteout();
         uint32_t len = (uint32_t)bp;
         if (B != 0xFFu) {
-            word |= B << (8 * (bp & 3));
+            acc |= (unsigned long long)B << (8 * (bp & 7));
             len = (uint32_t)bp + 1;
         }
-        // the words after the last full 16 bytes: the finished ones in ob, then the partial one
+        // the bytes after the last full group, as whole words
         if (len > cap) {
             overflow = 1u;
         } else {
             const uint32_t nw = (len + 3u) >> 2;
-            for (uint32_t wi = flushed >> 2; wi < nw; ++wi) {
-                const uint32_t r = wi & 3u;
-                const uint32_t v = wi + 1 == nw && (len & 3u || B != 0xFFu)
-                                       ? word
-                                       : (r == 0 ? ob.x : r == 1 ? ob.y : r == 2 ? ob.z : ob.w);
-                *(uint32_t*)(out + 4u * wi) = v;
-            }
+            for (uint32_t wi = flushed >> 2; wi < nw; ++wi)
+                *(uint32_t*)(out + 4u * wi) = (uint32_t)(acc >> (32u * (wi & 1u)));
         }
         if (overflow) {
             atomicOr(&P.counters->flags, 2u);
