@@ -660,6 +660,10 @@ void Job::uploadAsync(int i) {
     hipCheck(hipMemcpyAsync(sd_[i].in, hostInput(i), sd_[i].inBytes, hipMemcpyHostToDevice, stream_), "frame upload");
 }
 
+// Launches of at most this many blocks code their longest blocks one per wave: with a
+// batch's worth of blocks the lane waves fill the machine and the scalar coder only competes.
+constexpr uint32_t kMqUniformMaxBlocks = 65536;
+
 void Job::enqueueT1(hipStream_t s) {
     const uint32_t n = nblocks_;
     hipCheck(launch_clear_counters(dCounters_, dWork_, 4, s), "clear counters");
@@ -701,8 +705,12 @@ void Job::enqueueT1(hipStream_t s) {
     mq.counters = dCounters_;
     mq.nblocks = n;
     mq.work_next = dWork_ + 1;
+    mq.nlong = n <= kMqUniformMaxBlocks ? std::min<uint32_t>(n, (uint32_t)(ctx_.computeUnits() * opt_.mqLongPerCU)) : 0u;
+    // enough waves for one long block each, where the grid cap allows
+    const uint32_t mqWaves = (uint32_t)opt_.mqGroup / 64;
+    const uint32_t mqWant = std::max<uint32_t>((n + opt_.mqGroup - 1) / opt_.mqGroup, (mq.nlong + mqWaves - 1) / mqWaves);
     const int mqGroups = (int)std::max<uint32_t>(
-        1, std::min<uint32_t>((n + opt_.mqGroup - 1) / opt_.mqGroup, (uint32_t)(ctx_.computeUnits() * opt_.mqGroupsPerCU)));
+        1, std::min<uint32_t>(mqWant, (uint32_t)(ctx_.computeUnits() * opt_.mqGroupsPerCU)));
     hipCheck(launch_t1_mq(mq, opt_.mqGroup, opt_.mqRead, mqGroups, s), "MQ coding");
     if (!capturing_) hipCheck(hipEventRecord(ev_[5], s), "event");
     if (anyRc_) {

@@ -44,6 +44,7 @@ struct JobOptions {
     int mqGroup = 128;        // MQ workgroup size
     int mqRead = 16;          // symbol bytes each MQ lane reads at a time (16 or 64)
     int mqGroupsPerCU = 8;
+    int mqLongPerCU = 8;      // longest blocks per compute unit coded one per wave
     double poolBytesPerCoef = 8.0;  // symbol pool sizing (grown on overflow)
     bool useGraph = false;    // replay the launch chain as a HIP graph
 };

@@ -70,6 +70,7 @@ struct MqParams {
     JobCounters* counters;
     uint32_t nblocks;
     uint32_t* work_next;
+    uint32_t nlong;                     // leading blocks of perm coded one per wave (<= nblocks)
 };
 
 struct CompactParams {

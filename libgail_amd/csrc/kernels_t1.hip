@@ -774,6 +774,171 @@ __constant__ uint32_t kMqTable[47];
 
 }  // namespace
 
+// Truncation length of every pass but the last (mq_encoder.h:146-169): the shortest prefix,
+// never ending in 0xFF, that with the decoder's 1-fill stays inside the interval the coder had
+// at the pass end. out holds the block's len codeword bytes.
+__device__ __forceinline__ void mq_pass_lengths(const uint8_t* out, uint32_t len, const MqSnapshot* snaps,
+                                                uint32_t np, uint32_t* plen) {
+    uint32_t prev = 1;
+    for (uint32_t q = 0; q + 1 < np; ++q) {
+        const MqSnapshot sn = snaps[q];
+        const int ct = (int)(sn.b_ct >> 16);
+        const int base_sh = 43 - ct;
+        const unsigned long long low =
+            ((unsigned long long)(sn.b_ct & 0xFFFFu) << base_sh) + ((unsigned long long)sn.c << 16);
+        const unsigned long long high = low + ((unsigned long long)sn.a << 16);
+        unsigned long long prefix = 0;
+        int rel = 0;
+        uint32_t L = len;
+        for (int i = sn.bp; i < (int)len; ++i) {
+            const uint32_t byte = i >= 0 ? out[i] : 0u;
+            const int sh = base_sh - rel;
+            if (sh < 0) break;
+            prefix += (unsigned long long)byte << sh;
+            const unsigned long long tail = prefix + (1ull << sh);
+            if ((uint32_t)(i + 1) >= prev && byte != 0xFFu && tail <= high && tail > low) {
+                L = (uint32_t)(i + 1);
+                break;
+            }
+            rel += byte == 0xFFu ? 7 : 8;
+        }
+        plen[q] = L;
+        prev = L > prev ? L : prev;
+    }
+    plen[np - 1] = len;
+}
+
+// One code-block coded by a whole wave, for the longest blocks of a launch: a lane-per-block
+// wave advances all its blocks one symbol per step of about a hundred instructions, so its
+// longest block sets the launch's time. Here the coder's registers are wave-uniform (scalar)
+// values, the 19 context states sit in lanes 0-18 of one vector register and Table C.2 in
+// lanes 0-46 of another (tabv), both read with readlane, and a symbol is a short chain of
+// scalar instructions. The bytes are those of the lane coder (the same steps on the same
+// state). There is no lane-dependent branch: every lane runs the same code and stores the same
+// values to the same addresses, so the compiler can never run the wave's lanes apart (with a
+// lane-0-only tail it had sent the other lanes into the next block's loop without lane 0).
+__device__ void mq_code_wave(const MqParams& P, uint32_t cb, uint32_t tabv, int lane) {
+    const T1Info inf = P.info[cb];
+    const uint32_t np = inf.passes;
+    if (np == 0) {
+        P.code_len[cb] = 0;
+        return;
+    }
+    const unsigned long long obase = P.out_off[cb];
+    uint8_t* const out = P.out + obase;
+    uint32_t cap = out_capacity(inf);  // a multiple of 64, as every block's offset
+    bool overflow = obase + cap > P.out_cap;
+    if (overflow) cap = 0;
+    const uint32_t q0 = (uint32_t)__builtin_amdgcn_readlane((int)tabv, 0);
+    const uint32_t q3 = (uint32_t)__builtin_amdgcn_readlane((int)tabv, 3);
+    const uint32_t q4 = (uint32_t)__builtin_amdgcn_readlane((int)tabv, 4);
+    const uint32_t q46 = (uint32_t)__builtin_amdgcn_readlane((int)tabv, 46);
+    uint32_t ctxv = q0;
+    ctxv = lane == 0 ? q4 : ctxv;
+    ctxv = lane == 17 ? q3 : ctxv;
+    ctxv = lane == 18 ? q46 : ctxv;
+    uint32_t A = 0x8000u, C = 0u, B = 0u;
+    int CT = 12, bp = -1;
+    unsigned long long acc = 0ull;  // finished bytes of the current aligned 8-byte group
+    uint32_t flushed = 0;
+    auto byteout = [&]() {
+        const uint32_t carry = B != 0xFFu ? (C >> 27) & 1u : 0u;
+        B += carry;
+        C -= carry << 27;
+        const bool stuff = B == 0xFFu;
+        acc |= (unsigned long long)B << (8 * (bp & 7));
+        if (bp >= 0 && (bp & 7) == 7) {
+            const uint32_t at = (uint32_t)bp & ~7u;
+            if (at < cap) *(unsigned long long*)(out + at) = acc;
+            else overflow = true;
+            acc = 0ull;
+            flushed = at + 8u;
+        }
+        ++bp;
+        B = stuff ? C >> 20 : (C >> 19) & 0xFFu;
+        C &= stuff ? 0xFFFFFu : 0x7FFFFu;
+        CT = stuff ? 7 : 8;
+    };
+    const uint32_t* pend = P.pass_end + (size_t)cb * kMaxPasses;
+    MqSnapshot* snaps = P.snaps + (size_t)cb * kMaxPasses;
+    uint32_t k = 0, pos = 0, next_end = pend[0];
+    const uint2* chunk = P.chunks + (size_t)cb * kMaxPlanes;
+    for (uint32_t j = 0; j < inf.planes; ++j) {
+        const uint2 ch = chunk[j];
+        const uint8_t* src = P.pool + ((unsigned long long)ch.x << 4);
+        for (uint32_t base = 0; base < ch.y; base += 256u) {
+            // 256 symbols at a time, 4 per lane; lanes past the plane's end read its first
+            // word again (a plane's symbols start 64-byte aligned)
+            const uint32_t cnt = ch.y - base < 256u ? ch.y - base : 256u;
+            const uint32_t lo = 4u * (uint32_t)lane < cnt ? 4u * (uint32_t)lane : 0u;
+            const uint32_t wv = *(const uint32_t*)(src + base + lo);
+            for (uint32_t i = 0; i < cnt; ++i) {
+                while (pos == next_end) {  // pass ends: the truncation reference of the pass
+                    snaps[k] = MqSnapshot{bp, B | ((uint32_t)CT << 16), C, A};
+                    ++k;
+                    next_end = k < np ? pend[k] : 0xFFFFFFFFu;
+                }
+                const uint32_t word = (uint32_t)__builtin_amdgcn_readlane((int)wv, (int)(i >> 2));
+                const uint32_t sv = (word >> (8u * (i & 3u))) & 0xFFu;
+                const uint32_t cx = sv >> 1;
+                const uint32_t st = (uint32_t)__builtin_amdgcn_readlane((int)ctxv, (int)cx);
+                const uint32_t qe = st & 0xFFFFu;
+                const uint32_t a = A - qe;
+                const bool is_mps = (int)(st ^ (sv << 31)) >= 0;
+                const bool add_c = is_mps != (a < qe);
+                C += add_c ? qe : 0u;
+                A = add_c ? a : qe;
+                if (!is_mps || a < 0x8000u) {
+                    const uint32_t nidx = is_mps ? ((st >> 16) & 63u) : ((st >> 22) & 63u);
+                    const uint32_t msb = is_mps ? st : st ^ (st << 3);
+                    const uint32_t t_new =
+                        (msb & 0x80000000u) | (uint32_t)__builtin_amdgcn_readlane((int)tabv, (int)nidx);
+                    ctxv = lane == (int)cx ? t_new : ctxv;
+                    int nsh = __clz((int)A) - 16;
+                    A <<= nsh;
+                    while (nsh >= CT) {
+                        C <<= CT;
+                        nsh -= CT;
+                        byteout();
+                    }
+                    C <<= nsh;
+                    CT -= nsh;
+                }
+                ++pos;
+            }
+        }
+    }
+    while (pos == next_end) {
+        snaps[k] = MqSnapshot{bp, B | ((uint32_t)CT << 16), C, A};
+        ++k;
+        next_end = k < np ? pend[k] : 0xFFFFFFFFu;
+    }
+    // FLUSH (C.2.9)
+    const uint32_t tempc = C + A;
+    C |= 0xFFFFu;
+    if (C >= tempc) C -= 0x8000u;
+    C <<= CT;
+    byteout();
+    C <<= CT;
+    byteout();
+    uint32_t len = (uint32_t)bp;
+    if (B != 0xFFu) {
+        acc |= (unsigned long long)B << (8 * (bp & 7));
+        len = (uint32_t)bp + 1;
+    }
+    if (len > cap) overflow = true;
+    if (overflow) {
+        atomicOr(&P.counters->flags, 2u);
+        P.code_len[cb] = 0;
+        return;
+    }
+    const uint32_t nw = (len + 3u) >> 2;
+    for (uint32_t wi = flushed >> 2; wi < nw; ++wi)
+        *(uint32_t*)(out + 4u * wi) = (uint32_t)(acc >> (32u * (wi & 1u)));
+    mq_pass_lengths(out, len, snaps, np, P.pass_len + (size_t)cb * kMaxPasses);
+    P.code_len[cb] = len;
+}
+
 // GROUP lanes per workgroup; NQ: 16-byte words of symbols read per lane at a time (1 or 4),
 // with the following NQ in flight while these are coded.
 template <int GROUP, int NQ>
@@ -787,13 +952,24 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
     if (P.counters->flags & 1u) return;
     constexpr uint32_t kGroupBytes = 16u * NQ;
     const int t = threadIdx.x;
+    // The nlong longest blocks (the leading entries of the order) one per wave first, by a
+    // fixed stride over the grid's waves (the grid has at least nlong waves when it can).
+    {
+        const int lane = t & 63;
+        const uint32_t tabv = lane < 47 ? s_tab[lane] : 0u;
+        const uint32_t wave =
+            (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (GROUP / 64) + ((uint32_t)t >> 6)));
+        const uint32_t nwaves = gridDim.x * (GROUP / 64);
+        for (uint32_t i = wave; i < P.nlong; i += nwaves) mq_code_wave(P, P.perm[i], tabv, lane);
+    }
+    const uint32_t nrest = P.nblocks - P.nlong;
     for (;;) {
         // one wave-sized group of blocks at a time, in largest-first order
         uint32_t base = 0;
         if ((t & 63) == 0) base = atomicAdd(P.work_next, 64u);
-        base = __shfl((int)base, 0);
-        if (base >= P.nblocks) break;
-        const uint32_t gid = base + (t & 63);
+        base = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)base, 0));
+        if (base >= nrest) break;
+        const uint32_t gid = P.nlong + base + (t & 63);
         if (gid >= P.nblocks) continue;
         const uint32_t cb = P.perm[gid];
         const T1Info inf = P.info[cb];
@@ -973,37 +1149,7 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
             P.code_len[cb] = 0;
             continue;
         }
-        // Truncation length of every pass but the last (mq_encoder.h:146-169): the shortest
-        // prefix, never ending in 0xFF, that with the decoder's 1-fill stays inside the
-        // interval the coder had at the pass end.
-        uint32_t* plen = P.pass_len + (size_t)cb * kMaxPasses;
-        uint32_t prev = 1;
-        for (uint32_t q = 0; q + 1 < np; ++q) {
-            const MqSnapshot sn = snaps[q];
-            const int ct = (int)(sn.b_ct >> 16);
-            const int base_sh = 43 - ct;
-            const unsigned long long low =
-                ((unsigned long long)(sn.b_ct & 0xFFFFu) << base_sh) + ((unsigned long long)sn.c << 16);
-            const unsigned long long high = low + ((unsigned long long)sn.a << 16);
-            unsigned long long prefix = 0;
-            int rel = 0;
-            uint32_t L = len;
-            for (int i = sn.bp; i < (int)len; ++i) {
-                const uint32_t byte = i >= 0 ? out[i] : 0u;
-                const int sh = base_sh - rel;
-                if (sh < 0) break;
-                prefix += (unsigned long long)byte << sh;
-                const unsigned long long tail = prefix + (1ull << sh);
-                if ((uint32_t)(i + 1) >= prev && byte != 0xFFu && tail <= high && tail > low) {
-                    L = (uint32_t)(i + 1);
-                    break;
-                }
-                rel += byte == 0xFFu ? 7 : 8;
-            }
-            plen[q] = L;
-            prev = L > prev ? L : prev;
-        }
-        plen[np - 1] = len;
+        mq_pass_lengths(out, len, snaps, np, P.pass_len + (size_t)cb * kMaxPasses);
         P.code_len[cb] = len;
     }
 }
