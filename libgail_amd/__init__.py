@@ -471,8 +471,8 @@ class Batch:
     (a list of (max_width, max_height, colour)) the batch starts empty and bind() assigns
     frames to its slots for each run (the served form)."""
 
-    def __init__(self, device, layouts=None, settings=None, dwt_tile=0, cm_waves=4, mq_group=256, use_graph=False,
-                 pool_bytes_per_coef=0.0, slots=None, mq_read=16):
+    def __init__(self, device, layouts=None, settings=None, dwt_tile=1, cm_waves=4, mq_group=256, use_graph=False,
+                 pool_bytes_per_coef=0.0, slots=None, mq_read=64):
         self.options = BatchOptions(dwt_tile, cm_waves, mq_group, int(use_graph), mq_read, 0, pool_bytes_per_coef)
         self.handle = _vp()
         if slots is not None:
@@ -493,8 +493,8 @@ class Batch:
     @staticmethod
     def slots_bytes(device, slots, **options):
         arr = (Slot * len(slots))(*[Slot(w, h, int(c), 0) for w, h, c in slots])
-        opt = BatchOptions(options.get("dwt_tile", 0), options.get("cm_waves", 4), options.get("mq_group", 256),
-                           int(options.get("use_graph", False)), options.get("mq_read", 16), 0,
+        opt = BatchOptions(options.get("dwt_tile", 1), options.get("cm_waves", 4), options.get("mq_group", 256),
+                           int(options.get("use_graph", False)), options.get("mq_read", 64), 0,
                            options.get("pool_bytes_per_coef", 0.0))
         return plugin().slots_bytes(device.handle, arr, len(slots), C.byref(opt))
 

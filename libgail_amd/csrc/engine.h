@@ -41,8 +41,8 @@ struct JobOptions {
     int dwtTile = 1;          // tile shape index (see dwt_tile_size): 64 x 32
     int cmWaves = 4;          // waves per context-modelling workgroup
     int cmGroupsPerCU = 8;    // persistent grid size per compute unit
-    int mqGroup = 128;        // MQ workgroup size
-    int mqRead = 16;          // symbol bytes each MQ lane reads at a time (16 or 64)
+    int mqGroup = 256;        // MQ workgroup size
+    int mqRead = 64;          // symbol bytes each MQ lane reads at a time (16 or 64)
     int mqGroupsPerCU = 8;
     int mqLongPerCU = 8;      // longest blocks per compute unit coded one per wave
     double poolBytesPerCoef = 8.0;  // symbol pool sizing (grown on overflow)
