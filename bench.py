@@ -420,7 +420,7 @@ def pmc_probe(g, np, W, H, fourk, frames):
     dev = g.Device(0)
     layout = g.FrameLayout(W, H, fourk)
     bt = g.Batch(dev, [layout] * len(frames), [g.xyz_settings(layout)] * len(frames), dwt_tile=1, cm_waves=2,
-                 mq_group=128)
+                 mq_group=256, mq_read=64)
     for i, f in enumerate(frames):
         bt.upload(i, f)
         bt.set_slope(i, 1500.0)  # early termination on, as in the stream (0.25 lambda of a 24 fps frame)
