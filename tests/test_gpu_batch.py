@@ -42,6 +42,29 @@ def test_mixed_geometry_batch(device, oracle):
         _check_frame(outs[i], ref, n)
 
 
+def test_large_launch_lane_coder(device, oracle):
+    """Three 4K flat frames in one launch: 77,958 code-blocks, more than the 65,536 up to which
+    a launch codes its longest blocks one per wave, so here MQ codes every block one per lane.
+    Each frame equals the oracle, as the smaller launches of the other tests (which use both
+    coders) do."""
+    W, H, k = g.CONTAINERS["4k_flat"]
+    L = g.FrameLayout(W, H, k)
+    frames = [g.synth_frame(610 + i, W, H) for i in range(3)]
+    thr = [0.0, 1500.0, 300.0]
+    b = g.Batch(device, [L] * 3, [g.xyz_settings(L)] * 3)
+    try:
+        for i, f in enumerate(frames):
+            b.upload(i, f)
+            b.set_slope(i, thr[i])
+        print("[batch] encode 3 4K frames in one launch", flush=True)
+        outs, _ = b.encode()
+    finally:
+        b.close()
+    for i in range(3):
+        ref = oracle.encode_frame_xyz(frames[i], k, thr=thr[i], threads=16)
+        _check_frame(outs[i], ref, f"4k_flat #{i}")
+
+
 def test_same_alone_and_in_the_mix(device):
     """A frame's Tier-1 output is the same encoded alone and inside a batch of others."""
     W, H, k = g.CONTAINERS["2k_flat"]
