@@ -28,10 +28,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes one DWT level-0 launch over 16 4K flat frames moves, from the FETCH_SIZE and
-# WRITE_SIZE counter runs in profiles/pmc_traffic_dwt_level0_4k_flat_batch16.md, each scaled by
-# what the counter reads for a 1 GiB device copy.
-DWT0_TRAFFIC_4K_FLAT_16 = 2502515919
+# HBM bytes one DWT level-0 launch over 16 4K flat frames moves, per tile shape, from the
+# FETCH_SIZE and WRITE_SIZE counter runs in profiles/pmc_traffic_dwt_level0_4k_flat_batch16.md,
+# each scaled by what the counter reads for a 1 GiB device copy.
+DWT0_TRAFFIC_4K_FLAT_16 = {"64x16": 2491427489, "64x32": 2502515919}
 
 
 def dwt_level0_bytes(W, H):
@@ -287,7 +287,7 @@ def mixed_batch_run(g, np, dev, steps=6, per_profile=16):
         first.upload(i, f)
         first.set_slope(i, 0.0)
     chosen, _ = first.autotune(reps=1)
-    tiles = {"64x64": 0, "64x32": 1, "32x64": 2, "32x32": 3}
+    tiles = {"64x64": 0, "64x32": 1, "32x64": 2, "64x16": 3}
     second = g.Batch(dev, layouts, settings, use_graph=True, dwt_tile=tiles[chosen["dwt_tile"]],
                      cm_waves=chosen["cm_waves"], mq_group=chosen["mq_group"], mq_read=chosen["mq_read"])
     for i, f in enumerate(frames):
@@ -419,7 +419,7 @@ def pmc_probe(g, np, W, H, fourk, frames):
     del a, b
     dev = g.Device(0)
     layout = g.FrameLayout(W, H, fourk)
-    bt = g.Batch(dev, [layout] * len(frames), [g.xyz_settings(layout)] * len(frames), dwt_tile=1, cm_waves=2,
+    bt = g.Batch(dev, [layout] * len(frames), [g.xyz_settings(layout)] * len(frames), dwt_tile=3, cm_waves=2,
                  mq_group=256, mq_read=64)
     for i, f in enumerate(frames):
         bt.upload(i, f)
@@ -485,7 +485,7 @@ def main():
     # ---- GPU stages alone: batches of B frames resident in HBM, `inflight` batches on their
     # own streams, Tier-1 results copied to host memory
     nb = max(1, args.inflight)
-    tiles = {"64x64": 0, "64x32": 1, "32x64": 2, "32x32": 3}
+    tiles = {"64x64": 0, "64x32": 1, "32x64": 2, "64x16": 3}
     first = g.Batch(dev, [layout] * B, [g.xyz_settings(layout)] * B, use_graph=bool(args.graph))
     for i, f in enumerate(frames):
         first.upload(i, f)
@@ -528,9 +528,10 @@ def main():
     roof = None
     if med.get("dwt0", 0) > 0:
         achieved = B * dwt_level0_bytes(W, H) / (med["dwt0"] * 1e-3) / 1e9
-        traffic = DWT0_TRAFFIC_4K_FLAT_16 * B // 16 if args.profile == "4k_flat" else None
+        t16 = DWT0_TRAFFIC_4K_FLAT_16.get(chosen["dwt_tile"]) if args.profile == "4k_flat" else None
+        traffic = t16 * B // 16 if t16 else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "dwt_level_kernel<level 0>",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": f"dwt_level_kernel<level 0, {chosen['dwt_tile']}>",
                 "launch": f"{B} frames, {dwt_level0_bytes(W, H) * B} bytes algorithmic",
                 "traffic_source": "rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a 1 GiB copy"}
 

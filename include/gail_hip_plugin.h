@@ -180,7 +180,7 @@ typedef struct GailHipDecodeTimings {
 
 /* Tuning of a batched job (no counterpart in the CUDA plugin). Zero fields take defaults. */
 typedef struct GailHipBatchOptions {
-    int32_t dwtTile;      /* 0 64x64, 1 64x32, 2 32x64, 3 32x32 */
+    int32_t dwtTile;      /* 0 64x64, 1 64x32, 2 32x64, 3 64x16 */
     int32_t cmWaves;      /* waves per context-modelling workgroup: 1, 2, 4, 8 */
     int32_t mqGroup;      /* MQ workgroup size: 64, 128, 256 */
     int32_t useGraph;     /* replay the launch chain as a HIP graph */
@@ -291,7 +291,7 @@ struct GailHipPlugin {
     /* Pinned host memory on the plugin's runtime; NULL when none is left. */
     void* (*host_alloc)(uint64_t bytes);
     void (*host_free)(void* p);
-    /* Autotuner: times every DWT tile shape (64x64, 64x32, 32x64, 32x32), context-modelling
+    /* Autotuner: times every DWT tile shape (64x64, 64x32, 32x64, 64x16), context-modelling
      * waves per workgroup (1, 2, 4, 8) and MQ lanes per workgroup (64, 128, 256) on the bound
      * frames, whose inputs must already be on the device; keeps the fastest of each for the
      * batch's later runs, and the MQ symbol read width (16 or 64 bytes). chosen (may be NULL)

@@ -471,7 +471,7 @@ class Batch:
     (a list of (max_width, max_height, colour)) the batch starts empty and bind() assigns
     frames to its slots for each run (the served form)."""
 
-    def __init__(self, device, layouts=None, settings=None, dwt_tile=1, cm_waves=4, mq_group=256, use_graph=False,
+    def __init__(self, device, layouts=None, settings=None, dwt_tile=3, cm_waves=4, mq_group=256, use_graph=False,
                  pool_bytes_per_coef=0.0, slots=None, mq_read=64):
         self.options = BatchOptions(dwt_tile, cm_waves, mq_group, int(use_graph), mq_read, 0, pool_bytes_per_coef)
         self.handle = _vp()
@@ -493,7 +493,7 @@ class Batch:
     @staticmethod
     def slots_bytes(device, slots, **options):
         arr = (Slot * len(slots))(*[Slot(w, h, int(c), 0) for w, h, c in slots])
-        opt = BatchOptions(options.get("dwt_tile", 1), options.get("cm_waves", 4), options.get("mq_group", 256),
+        opt = BatchOptions(options.get("dwt_tile", 3), options.get("cm_waves", 4), options.get("mq_group", 256),
                            int(options.get("use_graph", False)), options.get("mq_read", 64), 0,
                            options.get("pool_bytes_per_coef", 0.0))
         return plugin().slots_bytes(device.handle, arr, len(slots), C.byref(opt))
@@ -507,7 +507,7 @@ class Batch:
         _check(plugin().batch_bind(self.handle, self._plain, self._settings, n), "batch_bind")
         self.n = n
 
-    TUNE = ("dwt_64x64", "dwt_64x32", "dwt_32x64", "dwt_32x32", "cm_1", "cm_2", "cm_4", "cm_8", "mq_64", "mq_128",
+    TUNE = ("dwt_64x64", "dwt_64x32", "dwt_32x64", "dwt_64x16", "cm_1", "cm_2", "cm_4", "cm_8", "mq_64", "mq_128",
             "mq_256", "mq_64_r64", "mq_128_r64", "mq_256_r64")
 
     def autotune(self, reps=2):
@@ -516,7 +516,7 @@ class Batch:
         o = BatchOptions()
         t = (C.c_double * 14)()
         _check(plugin().batch_autotune(self.handle, reps, C.byref(o), t, 14), "batch_autotune")
-        chosen = {"dwt_tile": ["64x64", "64x32", "32x64", "32x32"][o.dwtTile], "cm_waves": o.cmWaves,
+        chosen = {"dwt_tile": ["64x64", "64x32", "32x64", "64x16"][o.dwtTile], "cm_waves": o.cmWaves,
                   "mq_group": o.mqGroup, "mq_read": o.mqRead}
         return chosen, {k: round(v, 4) for k, v in zip(self.TUNE, list(t))}
 

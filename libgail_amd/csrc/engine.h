@@ -38,7 +38,7 @@ struct FrameConfig {
 };
 
 struct JobOptions {
-    int dwtTile = 1;          // tile shape index (see dwt_tile_size): 64 x 32
+    int dwtTile = 3;          // tile shape index (see dwt_tile_size): 64 x 16
     int cmWaves = 4;          // waves per context-modelling workgroup
     int cmGroupsPerCU = 8;    // persistent grid size per compute unit
     int mqGroup = 256;        // MQ workgroup size

@@ -456,12 +456,12 @@ hipError_t launch_dwt_level(bool level0, int tile_shape, const DwtLevelDesc* des
     case 0: return level0 ? launch_dwt<true, 64, 64>(descs, ndesc, tiles, s) : launch_dwt<false, 64, 64>(descs, ndesc, tiles, s);
     case 1: return level0 ? launch_dwt<true, 64, 32>(descs, ndesc, tiles, s) : launch_dwt<false, 64, 32>(descs, ndesc, tiles, s);
     case 2: return level0 ? launch_dwt<true, 32, 64>(descs, ndesc, tiles, s) : launch_dwt<false, 32, 64>(descs, ndesc, tiles, s);
-    default: return level0 ? launch_dwt<true, 32, 32>(descs, ndesc, tiles, s) : launch_dwt<false, 32, 32>(descs, ndesc, tiles, s);
+    default: return level0 ? launch_dwt<true, 64, 16>(descs, ndesc, tiles, s) : launch_dwt<false, 64, 16>(descs, ndesc, tiles, s);
     }
 }
 
 void dwt_tile_size(int tile_shape, int& tx, int& ty) {
-    static const int shapes[4][2] = {{64, 64}, {64, 32}, {32, 64}, {32, 32}};
+    static const int shapes[4][2] = {{64, 64}, {64, 32}, {32, 64}, {64, 16}};
     const int k = tile_shape >= 0 && tile_shape < 4 ? tile_shape : 3;
     tx = shapes[k][0];
     ty = shapes[k][1];

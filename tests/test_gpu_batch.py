@@ -213,7 +213,7 @@ def test_slot_job_rebinds(device, oracle):
         b.close()
 
 
-@pytest.mark.parametrize("opts", [dict(dwt_tile=0), dict(dwt_tile=2), dict(dwt_tile=3), dict(cm_waves=1),
+@pytest.mark.parametrize("opts", [dict(dwt_tile=0), dict(dwt_tile=1), dict(dwt_tile=2), dict(cm_waves=1),
                                   dict(cm_waves=2), dict(cm_waves=8), dict(mq_group=64), dict(mq_group=128),
                                   dict(mq_read=16), dict(mq_group=64, mq_read=16), dict(mq_group=128, mq_read=16)])
 def test_kernel_variants_bit_exact(device, oracle, opts):
