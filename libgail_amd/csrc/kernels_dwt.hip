@@ -4,7 +4,7 @@
 // workgroup owns a TX x TY tile of the level's LL region and all three components: it
 // stages the tile plus a 4-sample mirrored halo in LDS (coalesced row loads), lifts the
 // columns in registers (one thread per column, the whole column unrolled), then the rows
-// (two threads per row, each half with its own halo), and writes the de-interleaved results back with coalesced row
+// (two or four threads per row, each piece with its own halo), and writes the de-interleaved results back with coalesced row
 // stores: the LL quarter to the next level's scratch plane, HL/LH/HH straight into the
 // frame's Mallat-layout coefficient planes. The order of operations on every sample is the
 // CPU reference's (columns, then rows; lifting steps alpha..delta; low * (1/K), high * K),
@@ -169,16 +169,19 @@ __global__ void __launch_bounds__(256) dwt_level_kernel(const DwtLevelDesc* __re
         }
         __syncthreads();
     }
-    // Rows (horizontal lifting): two threads per output row of each component, each lifting
-    // half the row with its own 4-sample halo (an output depends on the 4 samples either side
-    // only, so the halves compute exactly what the whole row would). Both halves of a row are
-    // in the same round, and every read of a round precedes its writes.
+    // Rows (horizontal lifting): SPLIT threads per output row of each component, each lifting
+    // an even-aligned piece of the row with its own 4-sample halo (an output depends on the 4
+    // samples either side only, so the pieces compute exactly what the whole row would). All
+    // pieces of a row are in the same round, and every read of a round precedes its writes.
+    // Short tiles split rows in four so that most of the 256 threads have a piece.
     if (w >= 2) {
-        constexpr int HX = TX / 2 + 8;
-        for (int r0 = 0; r0 < 6 * TY; r0 += 256) {
+        constexpr int SPLIT = 3 * TY * 4 <= 256 ? 4 : 2;
+        constexpr int PW = TX / SPLIT, HX = PW + 8;
+        static_assert(PW % 2 == 0 && PW >= 8, "row pieces start at even samples");
+        for (int r0 = 0; r0 < 3 * SPLIT * TY; r0 += 256) {
             const int i = r0 + tid;
-            const bool act = i < 6 * TY;
-            const int c = i / (2 * TY), rr = i - c * (2 * TY), ry = 4 + (rr >> 1), xs = (rr & 1) * (TX / 2);
+            const bool act = i < 3 * SPLIT * TY;
+            const int c = i / (SPLIT * TY), rr = i - c * (SPLIT * TY), ry = 4 + rr / SPLIT, xs = (rr % SPLIT) * PW;
             float v[HX];
             if (act) {
 #pragma unroll
@@ -188,7 +191,7 @@ __global__ void __launch_bounds__(256) dwt_level_kernel(const DwtLevelDesc* __re
             if (act) {
                 lift97(v);
 #pragma unroll
-                for (int k = 4; k < TX / 2 + 4; ++k) tile[c][ry][xs + k] = v[k];
+                for (int k = 4; k < PW + 4; ++k) tile[c][ry][xs + k] = v[k];
             }
             __syncthreads();
         }
