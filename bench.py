@@ -61,7 +61,7 @@ def cpu_baseline(frames, fourk, thr, budget_s=20.0):
 
 
 def plain_parity(np, gpu, frame, fourk, thr, levels):
-    """Config-2 checks of one plain-path frame against the CPU path, run in the CPU-baseline
+    """Checks of one plain-path 4K frame against the CPU path, run in the CPU-baseline
     leg with the compiled reference (the oracle where it is not built) as the checker:
     quantised-index mismatches, Tier-1 bit-exactness, and the largest difference between the
     device's float32 DWT and a float64 DWT of the same ICT planes."""
@@ -452,7 +452,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU figures are not part of it "
-              "(frames shard across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+              "(frames are independent end to end and shard across GPUs with no exchange)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -17,9 +17,9 @@ def _check_frame(res, ref, label):
 
 
 def test_mixed_geometry_batch(device, oracle):
-    """Config 3 in small: 2K flat, 2K scope, 4K scope and 4K full frames with their own
-    seeds and early-termination thresholds in one batch (one DWT launch per level, one
-    Tier-1 launch for all code-blocks)."""
+    """The mixed batched workload in small: 2K flat, 2K scope, 4K scope and 4K full frames
+    with their own seeds and early-termination thresholds in one batch (one DWT launch per
+    level, one Tier-1 launch for all code-blocks)."""
     names = ["2k_flat", "2k_scope", "4k_scope", "4k_full"]
     thr = [0.0, 40.0, 0.0, 400.0]
     layouts, frames = [], []
@@ -142,7 +142,7 @@ def test_pool_overflow_reruns_on_gpu(device, oracle):
 
 @pytest.mark.parametrize("kind", ["gbrp12_bt709", "yuv420p10_bt709"])
 def test_colour_inputs(device, oracle, kind):
-    """Config 4 inputs: 12-bit G'B'R' and 10-bit Y'CbCr 4:2:0 (BT.709, limited range, chroma
+    """The served workload's inputs: 12-bit G'B'R' and 10-bit Y'CbCr 4:2:0 (BT.709, limited range, chroma
     left), a 1920x1080 picture centred in the 2K flat container."""
     W, H, k = g.CONTAINERS["2k_flat"]
     aw, ah, ox, oy = 1920, 1080, 39, 0

@@ -51,6 +51,6 @@ def test_plain_2k_flat_early_stop(device, oracle):
 
 
 def test_plain_4k_flat(device, oracle):
-    """Config 2: one 4K flat 3996x2160 frame, 6 levels, ragged edge blocks."""
+    """The plain workload: one 4K flat 3996x2160 frame, 6 levels, ragged edge blocks."""
     res = _frame_parity(device, oracle, "4k_flat", 3)
     assert len(res.info) == 25986

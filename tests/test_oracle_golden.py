@@ -80,7 +80,7 @@ def test_color(oracle):
 
 @pytest.mark.parametrize("key", ["2k_flat_seed1_thr0.0", "2k_flat_seed1_thr50.0"])
 def test_frame_2k(oracle, key):
-    """Config 1: one 2K flat 1998x1080 frame of the seeded generator, NL 5, whole Tier-1."""
+    """The CPU workload: one 2K flat 1998x1080 frame of the seeded generator, NL 5, whole Tier-1."""
     import libgail_amd as g
     with open(os.path.join(GOLDEN, "frames.json")) as f:
         gold = json.load(f)[key]
