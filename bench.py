@@ -40,6 +40,47 @@ def dwt_level0_bytes(W, H):
     return 3 * (2 + 4) * W * H
 
 
+def hbm_copy_gbs(g, dev):
+    """HBM rate a plain device-to-device copy reaches on this GPU: 1 GiB copied (2 GiB moved)
+    with hipMemcpyAsync, timed with HIP events on the null stream, median of 10. It puts a
+    kernel's fraction of the published 8 TB/s beside what a copy kernel achieves."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    vp = ctypes.c_void_p
+    hip.hipMemcpyAsync.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int, vp]
+    hip.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
+    hip.hipEventRecord.argtypes = [vp, vp]
+    hip.hipEventSynchronize.argtypes = [vp]
+    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+    hip.hipEventDestroy.argtypes = [vp]
+    hip.hipDeviceSynchronize.argtypes = []
+    n = 1 << 30
+    p = g.plugin()
+    a, b = p.device_alloc(dev.handle, n), p.device_alloc(dev.handle, n)
+    if not a or not b:
+        return None
+    e0, e1 = vp(), vp()
+    try:
+        if hip.hipEventCreate(ctypes.byref(e0)) or hip.hipEventCreate(ctypes.byref(e1)):
+            return None
+        ts = []
+        for i in range(11):
+            if hip.hipEventRecord(e0, None) or hip.hipMemcpyAsync(vp(b), vp(a), n, 3, None) \
+                    or hip.hipEventRecord(e1, None) or hip.hipEventSynchronize(e1):
+                return None
+            ms = ctypes.c_float()
+            hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+            if i:
+                ts.append(ms.value)
+        return 2 * n / (statistics.median(ts) * 1e-3) / 1e9
+    finally:
+        hip.hipDeviceSynchronize()
+        for e in (e0, e1):
+            if e:
+                hip.hipEventDestroy(e)
+        p.device_free(vp(a))
+        p.device_free(vp(b))
+
+
 def cpu_baseline(frames, fourk, thr, budget_s=20.0):
     """The reference's CPU flow (compiled from its sources into oracle/_ref when present,
     else the oracle port) on this host's cores, over a bounded sample of the same frames."""
@@ -534,6 +575,12 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": f"dwt_level_kernel<level 0, {chosen['dwt_tile']}>",
                 "launch": f"{B} frames, {dwt_level0_bytes(W, H) * B} bytes algorithmic",
                 "traffic_source": "rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a 1 GiB copy"}
+        copy = hbm_copy_gbs(g, dev)
+        if copy:
+            roof["copy_gbs"] = round(copy, 1)
+            roof["frac_of_copy"] = round(achieved / copy, 4)
+            print(f"[bench] 1 GiB device copy: {copy:.1f} GB/s; DWT level 0 {achieved:.1f} GB/s", file=sys.stderr,
+                  flush=True)
 
     # ---- plain path: one frame at a time, upload, Tier-1, rate control and Tier-2 in turn
     enc = g.Encoder(dev, layout, g.xyz_settings(layout))
