@@ -1029,7 +1029,9 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
         uint4 q[NQ];
 #pragma unroll
         for (int e = 0; e < NQ; ++e) q[e] = *(const uint4*)(src + 16 * e);
-        uint32_t t_cur = s_ctx[(q[0].x & 0xFFu) >> 1][t];
+        // the state of the next symbol's context and that context's slot
+        uint32_t* cslot = &s_ctx[(q[0].x & 0xFFu) >> 1][t];
+        uint32_t t_cur = *cslot;
         for (uint32_t j = 0; j < planes; ++j) {
             const uint2 chn = j + 1 < planes ? chunk[j + 1] : make_uint2(0u, 0u);
             const uint8_t* srcn = P.pool + ((unsigned long long)chn.x << 4);
@@ -1065,9 +1067,9 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
                                 }
                                 const uint32_t sv = (w[b >> 2] >> (8 * (b & 3))) & 0xFFu;
                                 const uint32_t sn = b + 1 < m ? (w[(b + 1) >> 2] >> (8 * ((b + 1) & 3))) & 0xFFu : w[4];
-                                const uint32_t cx = sv >> 1, cxn = sn >> 1;
                                 // the next symbol's context state, read before this symbol may change it
-                                const uint32_t t_next = s_ctx[cxn][t];
+                                uint32_t* const nslot = &s_ctx[sn >> 1][t];
+                                const uint32_t t_next = *nslot;
                                 const uint32_t st = t_cur;
                                 const uint32_t qe = st & 0xFFFFu;
                                 const uint32_t a = A - qe;  // A <= 0xFFFF, so a < 0x10000
@@ -1085,11 +1087,11 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
                                 const uint32_t msb = is_mps ? st : st ^ (st << 3);
                                 const uint32_t t_new = (msb & 0x80000000u) | s_tab[nidx];
                                 const uint32_t st_new = renorm ? t_new : st;
-                                s_ctx[cx][t] = st_new;
+                                *cslot = st_new;
                                 // RENORME: A and C shift until A >= 0x8000 (at most 15 bits), a
                                 // byte out whenever CT runs out; a second one is rare (an LPS
-                                // with a small Qe)
-                                int nsh = __clz((int)A) - 16;
+                                // with a small Qe). A is never 0 here.
+                                int nsh = __builtin_clz(A) - 16;
                                 A <<= nsh;
                                 int sh = nsh < CT ? nsh : CT;
                                 C <<= sh;
@@ -1104,7 +1106,8 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
                                         nsh -= sh;
                                     } while (CT == 0);
                                 }
-                                t_cur = cxn == cx ? st_new : t_next;
+                                t_cur = nslot == cslot ? st_new : t_next;
+                                cslot = nslot;
                                 ++pos;
                             }
                         }
