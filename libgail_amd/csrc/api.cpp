@@ -91,6 +91,11 @@ struct gail_encoder {
     std::vector<std::thread> workers;
     std::thread writer;
 
+    std::string errorText() {
+        std::lock_guard<std::mutex> l(om);
+        return error;
+    }
+
     void log(gail_log_level lvl, const char* fmt, ...) {
         if (!cfg.log || lvl > cfg.log_level) return;
         char buf[512];
@@ -101,18 +106,45 @@ struct gail_encoder {
         cfg.log(cfg.log_user, lvl, buf);
     }
 
+    // Marks the stream failed (the first reason is kept) and fails every blocked encode call.
     void stop(const std::string& why) {
         {
             std::lock_guard<std::mutex> l(om);
             if (!failed.exchange(true)) error = why;
         }
+        failWaiters(GAIL_ERROR_FAILED);
+    }
+
+    // Every frame still queued is dropped; a gail_encoder_encode caller waiting on one
+    // returns `status`, as does one whose frame was coded but not yet delivered. Frames a
+    // worker holds are failed by that worker when it hands its result over (deliverable()).
+    // `failed` or `canceled` is set before this runs, so push() cannot queue a frame after it.
+    void failWaiters(int status) {
+        {
+            std::lock_guard<std::mutex> l(qm);
+            for (Item& it : queue)
+                if (it.waiter) finishWaiter(it.waiter, status, nullptr);
+            queue.clear();
+        }
         qcv.notify_all();
+        {
+            std::lock_guard<std::mutex> l(om);
+            for (auto& kv : pending)
+                if (kv.second.waiter) finishWaiter(kv.second.waiter, status, nullptr);
+            pending.clear();
+        }
         ocv.notify_all();
     }
 
+    int stoppedStatus() const { return canceled.load() ? GAIL_ERROR_CANCELLED : GAIL_ERROR_FAILED; }
+
     int push(const gail_frame* f, Waiter* w) {
         if (canceled.load()) return fail(GAIL_ERROR_CANCELLED, "canceled");
-        if (failed.load()) return fail(GAIL_ERROR_FAILED, error);
+        if (failed.load()) return fail(GAIL_ERROR_FAILED, errorText());
+        {
+            std::lock_guard<std::mutex> l(qm);
+            if (ended) return fail(GAIL_ERROR_STATE, "the stream has already ended");
+        }
         std::string why;
         const int st = spec->checkFrame(*f, why);
         if (st != GAIL_OK) return fail(st, why);
@@ -124,8 +156,9 @@ struct gail_encoder {
         std::unique_lock<std::mutex> l(qm);
         qcv.wait(l, [&] { return queue.size() < qcap || failed.load() || canceled.load() || ended; });
         if (canceled.load()) return fail(GAIL_ERROR_CANCELLED, "canceled");
-        if (failed.load()) return fail(GAIL_ERROR_FAILED, error);
-        if (ended && !w) return fail(GAIL_ERROR_STATE, "the stream has already ended");
+        if (failed.load()) return fail(GAIL_ERROR_FAILED, errorText());
+        // after send(NULL) or finish the workers may have gone: nothing queued now would run
+        if (ended) return fail(GAIL_ERROR_STATE, "the stream has already ended");
         it.index = nextIndex++;
         queue.push_back(std::move(it));
         qcv.notify_all();
@@ -181,6 +214,12 @@ struct gail_encoder {
             }
             {
                 std::lock_guard<std::mutex> l(om);
+                if (!deliverable()) {
+                    // canceled or failed while this frame was on the device: the writer
+                    // delivers nothing more, so its waiter is answered here
+                    if (d.waiter) finishWaiter(d.waiter, stoppedStatus(), nullptr);
+                    continue;
+                }
                 pending.emplace(it.index, std::move(d));
             }
             ocv.notify_all();
@@ -191,6 +230,9 @@ struct gail_encoder {
         }
         ocv.notify_all();
     }
+
+    // With om held: whether the writer will still take a frame put into `pending`.
+    bool deliverable() const { return !writerDone && !failed.load() && !canceled.load(); }
 
     static void finishWaiter(Waiter* w, int status, const gail_packet* p) {
         std::lock_guard<std::mutex> l(w->m);
@@ -240,12 +282,18 @@ struct gail_encoder {
             out.push_back(p);
             ocv.notify_all();
         }
-        // wake blocked waiters of frames that will not be delivered
-        std::lock_guard<std::mutex> l(om);
-        for (auto& kv : pending)
-            if (kv.second.waiter) finishWaiter(kv.second.waiter, canceled.load() ? GAIL_ERROR_CANCELLED : GAIL_ERROR_FAILED, nullptr);
-        writerDone = true;
+        // wake blocked waiters of frames that will not be delivered; once writerDone is set
+        // (under om) no worker adds to `pending` (deliverable())
+        {
+            std::lock_guard<std::mutex> l(om);
+            for (auto& kv : pending)
+                if (kv.second.waiter) finishWaiter(kv.second.waiter, stoppedStatus(), nullptr);
+            pending.clear();
+            writerDone = true;
+        }
         ocv.notify_all();
+        // a frame queued behind a failure that no worker will take
+        if (failed.load() || canceled.load()) failWaiters(stoppedStatus());
     }
 
     void shutdown() {
@@ -330,9 +378,10 @@ GAIL_API void gail_color_preset(gail_preset preset, gail_color* c) {
 }
 
 GAIL_API void gail_config_init(gail_config* c, size_t size) {
+    if (!c || size < sizeof(size_t)) return;
     gail_config d;
     std::memset(&d, 0, sizeof d);
-    d.struct_size = std::min(size, sizeof d);
+    d.struct_size = size;  // the caller's header: fields past this library's end stay zero
     d.profile = GAIL_PROFILE_2K;
     d.container = GAIL_CONTAINER_FLAT;
     d.fps = 24.0;
@@ -342,6 +391,7 @@ GAIL_API void gail_config_init(gail_config* c, size_t size) {
     d.backend = GAIL_BACKEND_AUTO;
     d.check = 1;
     d.log_level = GAIL_LOG_WARNING;
+    std::memset(c, 0, size);
     std::memcpy(c, &d, std::min(size, sizeof d));
 }
 
@@ -353,8 +403,19 @@ GAIL_API void gail_packet_free(gail_packet* p) {
 }
 
 GAIL_API int gail_encoder_create(const gail_config* c, gail_encoder** encoder) {
-    if (!c || !encoder) return fail(GAIL_ERROR_INVALID, "null argument");
+    if (!encoder) return fail(GAIL_ERROR_INVALID, "encoder pointer is NULL");
     *encoder = nullptr;
+    // gail.h's compatibility rules: a config at least as large as this header's, and any
+    // fields of a newer header left at zero
+    if (!c || c->struct_size < sizeof(gail_config))
+        return fail(GAIL_ERROR_INVALID, "config not initialized with gail_config_defaults()");
+    if (c->struct_size > sizeof(gail_config)) {
+        const uint8_t* extra = reinterpret_cast<const uint8_t*>(c) + sizeof(gail_config);
+        for (size_t i = 0; i < c->struct_size - sizeof(gail_config); ++i)
+            if (extra[i])
+                return fail(GAIL_ERROR_UNSUPPORTED,
+                            std::string("the config uses settings this libgail (") + gail_version_string() + ") lacks");
+    }
     auto e = std::make_unique<gail_encoder>();
     try {
         e->spec = StreamSpec::make(*c);
@@ -364,7 +425,8 @@ GAIL_API int gail_encoder_create(const gail_config* c, gail_encoder** encoder) {
         if (!ph.table) throw ApiError(GAIL_ERROR_UNSUPPORTED, ph.error);
         e->P = ph.table;
         const int n = e->P->device_count();
-        if (n <= 0) throw ApiError(GAIL_ERROR_UNSUPPORTED, "no HIP device");
+        if (n < 0) throw ApiError(GAIL_ERROR_UNSUPPORTED, std::string("HIP runtime unavailable: ") + e->P->last_error());
+        if (n == 0) throw ApiError(GAIL_ERROR_UNSUPPORTED, "no HIP device");
         const int index = (cfg.devices && cfg.num_devices > 0) ? cfg.devices[0] : 0;
         if (index < 0 || index >= n) throw ApiError(GAIL_ERROR_INVALID, "no such device");
         if (e->P->device_usable(index) != GAIL_HIP_OK) throw ApiError(GAIL_ERROR_UNSUPPORTED, e->P->last_error());
@@ -422,6 +484,8 @@ GAIL_API const char* gail_encoder_devices(const gail_encoder* e) { return e ? e-
 
 GAIL_API int gail_encoder_send(gail_encoder* e, const gail_frame* f) {
     if (!e) return fail(GAIL_ERROR_INVALID, "null encoder");
+    if (e->canceled.load()) return fail(GAIL_ERROR_CANCELLED, "canceled");
+    if (e->failed.load()) return fail(GAIL_ERROR_FAILED, e->errorText());
     if (!f) {
         {
             std::lock_guard<std::mutex> l(e->qm);
@@ -429,10 +493,6 @@ GAIL_API int gail_encoder_send(gail_encoder* e, const gail_frame* f) {
         }
         e->qcv.notify_all();
         return GAIL_OK;
-    }
-    {
-        std::lock_guard<std::mutex> l(e->qm);
-        if (e->ended) return fail(GAIL_ERROR_STATE, "the stream has already ended");
     }
     return e->push(f, nullptr);
 }
@@ -442,12 +502,13 @@ GAIL_API int gail_encoder_receive(gail_encoder* e, gail_packet* pkt, int wait) {
     if (e->cfg.on_packet) return fail(GAIL_ERROR_STATE, "packets go to on_packet");
     std::unique_lock<std::mutex> l(e->om);
     for (;;) {
+        // after a cancel nothing more is handed out; after a failure, what was coded is
+        if (e->canceled.load()) return fail(GAIL_ERROR_CANCELLED, "canceled");
         if (!e->out.empty()) {
             *pkt = e->out.front();
             e->out.pop_front();
             return GAIL_OK;
         }
-        if (e->canceled.load()) return fail(GAIL_ERROR_CANCELLED, "canceled");
         if (e->failed.load()) return fail(GAIL_ERROR_FAILED, e->error);
         if (e->writerDone) return GAIL_EOF;
         if (!wait) return GAIL_EAGAIN;
@@ -460,9 +521,15 @@ GAIL_API int gail_encoder_encode(gail_encoder* e, const gail_frame* f, gail_pack
     Waiter w;
     const int r = e->push(f, &w);
     if (r != GAIL_OK) return r;
-    std::unique_lock<std::mutex> l(w.m);
-    w.cv.wait(l, [&] { return w.done; });
-    if (w.status != GAIL_OK) return fail(w.status, e->failed.load() ? e->error : "canceled");
+    int status;
+    {
+        std::unique_lock<std::mutex> l(w.m);
+        w.cv.wait(l, [&] { return w.done; });
+        status = w.status;
+    }
+    // w.m is released before om is taken (errorText): the writer and workers take them the
+    // other way round
+    if (status != GAIL_OK) return fail(status, status == GAIL_ERROR_CANCELLED ? "canceled" : e->errorText());
     *pkt = w.pkt;
     return GAIL_OK;
 }
@@ -483,12 +550,8 @@ GAIL_API int gail_encoder_finish(gail_encoder* e) {
 
 GAIL_API void gail_encoder_cancel(gail_encoder* e) {
     if (!e || e->canceled.exchange(true)) return;
-    {
-        std::lock_guard<std::mutex> l(e->qm);
-        e->queue.clear();
-    }
-    e->qcv.notify_all();
-    e->ocv.notify_all();
+    // blocked sends, encodes and receives return CANCELLED (gail.h:354-358)
+    e->failWaiters(GAIL_ERROR_CANCELLED);
 }
 
 GAIL_API const char* gail_encoder_error(const gail_encoder* e) {

@@ -31,13 +31,23 @@ const PluginHandle& loadPlugin() {
             return;
         }
         auto get = (const GailHipPlugin* (*)(uint32_t))dlsym(lib, GAIL_HIP_PLUGIN_ENTRY);
-        const GailHipPlugin* t = get ? get(GAIL_HIP_PLUGIN_ABI) : nullptr;
-        if (!t || t->abi != GAIL_HIP_PLUGIN_ABI || t->size != sizeof(GailHipPlugin) ||
-            t->layout != gail_hip_plugin_layout()) {
-            h.error = "libgail_hip.so is from another build";
+        if (!get) {
+            h.error = "libgail_hip.so has no " GAIL_HIP_PLUGIN_ENTRY " entry point";
             return;
         }
-        h.table = t;
+        const GailHipPlugin* t = get(GAIL_HIP_PLUGIN_ABI);
+        // the table is read only once abi and size say it has the fields read
+        if (!t) h.error = "libgail_hip.so does not support plugin ABI " + std::to_string(GAIL_HIP_PLUGIN_ABI);
+        else if (t->abi != GAIL_HIP_PLUGIN_ABI)
+            h.error = "libgail_hip.so is from another build (plugin ABI " + std::to_string(t->abi) + ", this library " +
+                      std::to_string(GAIL_HIP_PLUGIN_ABI) + ")";
+        else if (t->size != sizeof(GailHipPlugin))
+            h.error = "libgail_hip.so is from another build (plugin table of " + std::to_string(t->size) + " bytes, " +
+                      std::to_string(sizeof(GailHipPlugin)) + " expected)";
+        else if (t->layout != gail_hip_plugin_layout())
+            h.error = "libgail_hip.so is from another build (its structure layout differs)";
+        else
+            h.table = t;
     });
     return h;
 }
