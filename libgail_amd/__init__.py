@@ -619,6 +619,55 @@ def build_codestream(layout, t1, tag="hip", cap=8 << 20):
     return bytes(out[:n])
 
 
+def rate_allocate(layout, t1, max_total, max_comp):
+    """Rate control of a Tier-1 frame against explicit byte budgets (libgail.so): a dict of
+    the passes kept per block, the per-component lambdas, the codestream's total and
+    component sizes, the remaining distortion and the iterations."""
+    info, pl, pd, data = _t1_arrays(t1)
+    h = host()
+    h.gail_rate_allocate.restype = C.c_int
+    h.gail_rate_allocate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, _vp, C.c_uint64,
+                                     C.c_uint64, C.c_uint64, _vp, _vp, _vp, _vp]
+    passes = np.zeros(len(info), np.uint8)
+    lam = np.zeros(3, np.float64)
+    sizes = np.zeros(4, np.uint64)
+    dist = C.c_double(0)
+    it = h.gail_rate_allocate(layout.width, layout.height, layout.fourk, layout.base_step, info.ctypes.data_as(_vp),
+                              pl.ctypes.data_as(_vp), pd.ctypes.data_as(_vp), data.ctypes.data_as(_vp), len(info),
+                              int(max_total), int(max_comp), passes.ctypes.data_as(_vp), lam.ctypes.data_as(_vp),
+                              sizes.ctypes.data_as(_vp), C.byref(dist))
+    if it < 0:
+        raise GailError(it, "rate control failed")
+    return dict(passes=passes, lambdas=lam, total=int(sizes[0]), comp=[int(v) for v in sizes[1:]],
+                distortion=dist.value, iterations=it)
+
+
+def with_passes(t1, passes):
+    """A copy of a Tier-1 frame (dict of arrays) whose blocks keep only `passes` passes each,
+    for gail_build_codestream."""
+    info, pl, pd, data = _t1_arrays(t1)
+    info = info.copy()
+    info[:, 1] = np.minimum(info[:, 1], np.asarray(passes, np.uint32))
+    return dict(info=info, pass_len=pl, pass_dist=pd, data=data)
+
+
+def dci_limits(fps, mbps=230.0, comp_mbps=200.0):
+    """libgail.so's dciLimits: (frame budget, component budget) in bytes."""
+    h = host()
+    h.gail_dci_limits.restype = None
+    h.gail_dci_limits.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    t, c = C.c_uint64(0), C.c_uint64(0)
+    h.gail_dci_limits(fps, mbps, comp_mbps, C.byref(t), C.byref(c))
+    return t.value, c.value
+
+
+def empty_codestream_bytes(layout):
+    h = host()
+    h.gail_empty_codestream_bytes.restype = C.c_uint64
+    h.gail_empty_codestream_bytes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double]
+    return h.gail_empty_codestream_bytes(layout.width, layout.height, layout.fourk, layout.base_step)
+
+
 def dci_budget(fps, mbps=250.0, comp_mbps=200.0):
     """(frame budget, component budget) in bytes: floor(Mbit/s * 1e6 / (8 fps)) - 16
     (rate_control.cpp:11-26)."""

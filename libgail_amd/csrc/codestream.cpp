@@ -6,241 +6,96 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
+#include <utility>
 
 namespace gail_amd {
 
 namespace {
 
-// ---------------------------------------------------------------- packet-header bits
-// B.10.1: bits fill each byte MSB first; a byte after 0xFF carries 7 bits, so no marker
-// can appear in a header. With a null destination it only counts.
-class HeaderBits {
-public:
-    explicit HeaderBits(std::vector<uint8_t>* dst) : dst_(dst) {}
-    void bit(uint32_t b) { bits(b & 1u, 1); }
-    void bits(uint32_t v, int n) {
-        while (n > 0) {
-            const int take = std::min(n, room_ - used_);
-            n -= take;
-            acc_ = (acc_ << take) | ((v >> n) & ((1u << take) - 1u));
-            used_ += take;
-            if (used_ == room_) emit();
-        }
-    }
-    void zeros(int n) {
-        for (; n > 24; n -= 24) bits(0, 24);
-        bits(0, n);
-    }
-    // Pads the last byte; a header ending in 0xFF still owes the stuffed zero byte.
-    size_t finish() {
-        if (used_ > 0 || count_ == 0) {
-            acc_ <<= room_ - used_;
-            emit();
-        }
-        if (last_ == 0xFFu) {
-            acc_ = 0;
-            emit();
-        }
-        return count_;
-    }
-
-private:
-    void emit() {
-        last_ = acc_ & 0xFFu;
-        if (dst_) dst_->push_back((uint8_t)last_);
-        ++count_;
-        room_ = last_ == 0xFFu ? 7 : 8;
-        acc_ = 0;
-        used_ = 0;
-    }
-    std::vector<uint8_t>* dst_;
-    uint32_t acc_ = 0, last_ = 0;
-    int used_ = 0, room_ = 8;
-    size_t count_ = 0;
-};
-
-// ---------------------------------------------------------------- tag trees (B.10.2)
-class TagTree {
-public:
-    void reset(int w, int h) {
-        if (w != w_ || h != h_) {
-            w_ = w;
-            h_ = h;
-            parent_.clear();
-            int lw = w, lh = h, base = 0;
-            std::vector<std::array<int, 3>> levels;
-            for (;;) {
-                levels.push_back({base, lw, lh});
-                base += lw * lh;
-                if (lw == 1 && lh == 1) break;
-                lw = (lw + 1) / 2;
-                lh = (lh + 1) / 2;
-            }
-            parent_.assign(base, -1);
-            for (size_t l = 0; l + 1 < levels.size(); ++l) {
-                const auto& a = levels[l];
-                const auto& b = levels[l + 1];
-                for (int y = 0; y < a[2]; ++y)
-                    for (int x = 0; x < a[1]; ++x) parent_[a[0] + y * a[1] + x] = b[0] + (y / 2) * b[1] + x / 2;
-            }
-            value_.resize(base);
-            low_.resize(base);
-            known_.resize(base);
-        }
-        std::fill(value_.begin(), value_.end(), 1 << 30);
-        std::fill(low_.begin(), low_.end(), 0);
-        std::fill(known_.begin(), known_.end(), 0);
-    }
-    // Leaf values, then every inner node takes the minimum of its children.
-    void set(int leaf, int v) {
-        for (int i = leaf; i >= 0; i = parent_[i]) value_[i] = std::min(value_[i], v);
-    }
-    void encode(HeaderBits& bw, int leaf, int threshold) {
-        int path[32], depth = 0;
-        for (int i = leaf; i >= 0; i = parent_[i]) path[depth++] = i;
-        int low = 0;
-        while (depth > 0) {
-            const int n = path[--depth];
-            low = std::max(low, low_[n]);
-            while (low < threshold) {
-                if (low >= value_[n]) {
-                    if (!known_[n]) {
-                        bw.bit(1);
-                        known_[n] = 1;
-                    }
-                    break;
-                }
-                bw.bit(0);
-                ++low;
-            }
-            low_[n] = low;
-        }
-    }
-
-private:
-    int w_ = 0, h_ = 0;
-    std::vector<int> parent_, value_, low_;
-    std::vector<uint8_t> known_;
-};
-
-int floorLog2(uint32_t v) { return 31 - __builtin_clz(v); }
-
-// Table B.4: codewords for the number of coding passes.
-void passCount(HeaderBits& bw, int n) {
-    if (n == 1) bw.bits(0, 1);
-    else if (n == 2) bw.bits(2, 2);
-    else if (n <= 5) bw.bits(0xC | (n - 3), 4);
-    else if (n <= 36) bw.bits(0x1E0 | (n - 6), 9);
-    else if (n <= 164) bw.bits(0xFF80 | (n - 37), 16);
-    else throw std::runtime_error("too many coding passes");
+// ---------------------------------------------------------------- packets
+// Resolutions a tile-part carries: the 2K frame puts all of a component in one tile-part;
+// the 4K frame's POC sends the 2K resolutions of every component first, then the top
+// resolution of every component (SMPTE 429-4 / progression.h:20-26), six tile-parts.
+void tilePartResolutions(const Layout& L, int part, int& r0, int& r1) {
+    r0 = (L.fourk && part == 1) ? L.levels : 0;
+    r1 = (L.fourk && part == 0) ? L.levels : L.levels + 1;
 }
 
-struct Packets {
-    const Layout& L;
-    const Tier1Ref& t1;
-    const uint8_t* passes;
-    TagTree incl, zbp;
+// log2 of the precinct size of resolution r (COD: 2^7 at r = 0, 2^8 above).
+int precinctLog2(int r) { return r == 0 ? 7 : 8; }
 
-    // One packet (one precinct of one resolution, single layer); appends it to dst when
-    // given and returns its size.
-    size_t write(int comp, int res, int prec, std::vector<uint8_t>* dst) {
-        const Precinct& P = L.resolution(comp, res).precincts[prec];
-        bool any = false;
-        for (int k = 0; k < P.nb && !any; ++k) {
-            const PrecinctBand& pb = P.bands[k];
-            const Subband& b = L.bands[pb.band];
-            for (int j = 0; j < pb.ncby && !any; ++j)
-                for (int i = 0; i < pb.ncbx && !any; ++i)
-                    any = passes[b.cblkBase + (pb.cby0 + j) * b.ncbx + pb.cbx0 + i] != 0;
+// The packet table of a layout: every packet in codestream order with its precinct-bands.
+// Within a tile-part (one component, a range of resolutions) the order is CPRL with the tile
+// at the origin (B.12.1.5): a precinct of resolution r covers 2^(PP_r + NL - r) image samples
+// each way, and the packets come in raster order of their precincts' image-domain origins,
+// lower resolutions first at a shared origin. A band-domain precinct of resolution r > 0 is
+// half the resolution's (B.6), so every precinct-band spans 4 x 4 code-blocks at most.
+RcTables buildPacketTables(const Layout& L) {
+    RcTables T;
+    const int ntp = L.fourk ? 6 : 3;
+    T.ntp = (uint32_t)ntp;
+    struct Entry {
+        int y, x, r, px, py;
+    };
+    std::vector<Entry> order;
+    for (int part = 0; part < (L.fourk ? 2 : 1); ++part) {
+        int r0, r1;
+        tilePartResolutions(L, part, r0, r1);
+        order.clear();
+        for (int r = r0; r < r1; ++r) {
+            const int pp = precinctLog2(r), shift = pp + L.levels - r;
+            const int wr = (L.width + (1 << (L.levels - r)) - 1) >> (L.levels - r);
+            const int hr = (L.height + (1 << (L.levels - r)) - 1) >> (L.levels - r);
+            const int npx = (wr + (1 << pp) - 1) >> pp, npy = (hr + (1 << pp) - 1) >> pp;
+            for (int py = 0; py < npy; ++py)
+                for (int px = 0; px < npx; ++px) order.push_back(Entry{py << shift, px << shift, r, px, py});
         }
-        HeaderBits bw(dst);
-        if (!any) {
-            bw.bit(0);
-            return bw.finish();
-        }
-        bw.bit(1);
-        size_t body = 0;
-        for (int k = 0; k < P.nb; ++k) {
-            const PrecinctBand& pb = P.bands[k];
-            if (!pb.ncbx) continue;
-            const Subband& b = L.bands[pb.band];
-            incl.reset(pb.ncbx, pb.ncby);
-            zbp.reset(pb.ncbx, pb.ncby);
-            for (int j = 0; j < pb.ncby; ++j)
-                for (int i = 0; i < pb.ncbx; ++i) {
-                    const int ci = b.cblkBase + (pb.cby0 + j) * b.ncbx + pb.cbx0 + i;
-                    incl.set(j * pb.ncbx + i, passes[ci] ? 0 : 1);
-                    zbp.set(j * pb.ncbx + i, b.Mb - (int)t1.nbps(ci));
+        std::sort(order.begin(), order.end(), [](const Entry& a, const Entry& b) {
+            return a.y != b.y ? a.y < b.y : a.x != b.x ? a.x < b.x : a.r < b.r;
+        });
+        for (int c = 0; c < kComps; ++c) {
+            const int tp = part * kComps + c;
+            T.tpComp[tp] = c;
+            for (const Entry& e : order) {
+                const int nbands = e.r == 0 ? 1 : 3;
+                const int span = 1 << ((e.r == 0 ? precinctLog2(e.r) : precinctLog2(e.r) - 1) - kCblkLog2);
+                RcPacket pk{(uint32_t)T.pbands.size(), (uint16_t)nbands, (uint16_t)tp};
+                for (int k = 0; k < nbands; ++k) {
+                    const Subband& b = L.bands[L.bandIndex(c, e.r, e.r == 0 ? 0 : k + 1)];
+                    if (b.Mb > 31) throw std::logic_error("subband with more than 31 magnitude bit-planes");
+                    const int cx = e.px * span, cy = e.py * span;
+                    const int w = std::max(0, std::min(span, b.ncbx - cx)), h = std::max(0, std::min(span, b.ncby - cy));
+                    RcPBand q{};
+                    q.block0 = (uint32_t)(b.cblkBase + cy * b.ncbx + cx);
+                    q.stride = (uint16_t)b.ncbx;
+                    q.ncbx = (uint8_t)(w && h ? w : 0);
+                    q.ncby = (uint8_t)(w && h ? h : 0);
+                    q.mb = (uint8_t)b.Mb;
+                    T.pbands.push_back(q);
                 }
-            for (int j = 0; j < pb.ncby; ++j)
-                for (int i = 0; i < pb.ncbx; ++i) {
-                    const int leaf = j * pb.ncbx + i;
-                    const int ci = b.cblkBase + (pb.cby0 + j) * b.ncbx + pb.cbx0 + i;
-                    incl.encode(bw, leaf, 1);
-                    const int np = passes[ci];
-                    if (!np) continue;
-                    zbp.encode(bw, leaf, 1 << 20);
-                    passCount(bw, np);
-                    const uint32_t len = t1.len(ci, np - 1);
-                    const int extra = floorLog2((uint32_t)np);
-                    const int need = len ? floorLog2(len) + 1 : 1;
-                    const int incr = std::max(0, need - (3 + extra));
-                    bw.bits(((1u << incr) - 1u) << 1, incr + 1);  // Lblock increment: incr ones, then 0
-                    bw.bits(len, 3 + incr + extra);
-                    body += len;
-                }
-        }
-        const size_t hdr = bw.finish();
-        if (dst) {
-            for (int k = 0; k < P.nb; ++k) {
-                const PrecinctBand& pb = P.bands[k];
-                if (!pb.ncbx) continue;
-                const Subband& b = L.bands[pb.band];
-                for (int j = 0; j < pb.ncby; ++j)
-                    for (int i = 0; i < pb.ncbx; ++i) {
-                        const int ci = b.cblkBase + (pb.cby0 + j) * b.ncbx + pb.cbx0 + i;
-                        if (!passes[ci]) continue;
-                        const uint8_t* c = t1.code(ci);
-                        dst->insert(dst->end(), c, c + t1.len(ci, passes[ci] - 1));
-                    }
+                T.packets.push_back(pk);
             }
         }
-        return hdr + body;
     }
-};
-
-// CPRL progression volumes (progression.h:20-26): one for 2K; for 4K the POC puts the 2K
-// resolutions first and the 4K resolution last.
-struct Volume {
-    int rs, re;
-};
-int volumes(const Layout& L, Volume v[2]) {
-    v[0] = {0, L.fourk ? L.levels : L.levels + 1};
-    v[1] = {L.levels, L.levels + 1};
-    return L.fourk ? 2 : 1;
+    return T;
 }
 
-// Packets of component comp within a volume, CPRL order (B.12.1.5, origin 0, one tile).
-template <class F> void forEachPacket(const Layout& L, Volume v, int comp, F&& f) {
-    const int NL = L.levels;
-    int xs = 1 << 30, ys = 1 << 30;
-    for (int r = v.rs; r < v.re; ++r) {
-        const Resolution& R = L.resolution(0, r);
-        xs = std::min(xs, 1 << (R.ppx + NL - r));
-        ys = std::min(ys, 1 << (R.ppy + NL - r));
-    }
-    for (int y = 0; y < L.height; y += ys)
-        for (int x = 0; x < L.width; x += xs)
-            for (int r = v.rs; r < v.re; ++r) {
-                const Resolution& R = L.resolution(comp, r);
-                const int xr = 1 << (R.ppx + NL - r), yr = 1 << (R.ppy + NL - r);
-                if ((x & (xr - 1)) || (y & (yr - 1))) continue;
-                const int px = x / xr, py = y / yr;
-                if (px >= R.npx || py >= R.npy) continue;
-                f(r, py * R.npx + px);
-            }
-}
+// A frame's blocks as the packet coder reads them: passes[i] of block i kept.
+struct KeptBlocks {
+    const Tier1Ref* t1;
+    const uint8_t* p;
+    uint32_t passes(uint32_t i) const { return p[i]; }
+    uint32_t nbps(uint32_t i) const { return t1->nbps((int)i); }
+    uint32_t len(uint32_t i) const { return p[i] ? t1->len((int)i, p[i] - 1) : 0u; }
+};
+
+struct AppendByte {
+    std::vector<uint8_t>* v;
+    void operator()(uint32_t b) const { v->push_back((uint8_t)b); }
+};
 
 void put16(std::vector<uint8_t>& v, uint32_t x) {
     v.push_back((uint8_t)(x >> 8));
@@ -321,101 +176,83 @@ void mainHeader(const Layout& L, std::vector<uint8_t>& v, int ntp, const std::st
 
 const char* codestreamLabel() { return "GAIL 1.1.0"; }
 
+static const RcTables& packetTables(const Layout& L) {
+    // one table per frame geometry (the six DCI containers, and any other size asked for)
+    static std::mutex m;
+    static std::vector<std::pair<std::array<int, 3>, std::unique_ptr<RcTables>>> cache;
+    const std::array<int, 3> key{L.width, L.height, L.fourk ? 1 : 0};
+    std::lock_guard<std::mutex> lock(m);
+    for (auto& e : cache)
+        if (e.first == key) return *e.second;
+    auto t = std::make_unique<RcTables>(makeRcTables(L));
+    cache.emplace_back(key, std::move(t));
+    return *cache.back().second;
+}
+
 void buildCodestream(const Layout& L, const Tier1Ref& t1, const uint8_t* passes, std::vector<uint8_t>* out,
                      CodestreamSizes* sizes, const std::string& tag) {
     if (tag.size() > kMaxDeviceTagBytes) throw std::logic_error("device tag too long");
-    const int ntp = L.fourk ? 6 : 3;
-    Packets pw{L, t1, passes, {}, {}};
+    const RcTables& T = packetTables(L);
+    const CodestreamSizes sz = rcMeasure(L, T, t1, passes);
+    if (sizes) *sizes = sz;
+    if (!out) return;
     const std::string comment = tag.empty() ? std::string(codestreamLabel()) : std::string(codestreamLabel()) + " " + tag;
     const size_t slack = std::strlen(codestreamLabel()) + 1 + kMaxDeviceTagBytes - comment.size();
-    std::vector<uint8_t> hdr;
-    mainHeader(L, hdr, ntp, comment);
-    const size_t tlmPos = hdr.size() - 5 * ntp;
-    if (out) *out = hdr;
-    Volume vols[2];
-    const int nv = volumes(L, vols);
-    CodestreamSizes sz;
-    sz.numTileParts = ntp;
-    size_t total = hdr.size() + slack;
-    for (int vi = 0; vi < nv; ++vi)
-        for (int c = 0; c < kComps; ++c) {
-            const int tp = vi * 3 + c;
-            size_t sotPos = 0;
-            if (out) {
-                sotPos = out->size();
-                put16(*out, SOT);
-                put16(*out, 10);
-                put16(*out, 0);
-                put32(*out, 0);
-                out->push_back((uint8_t)tp);
-                out->push_back((uint8_t)ntp);
-                put16(*out, SOD);
+    std::vector<uint8_t>& v = *out;
+    v.clear();
+    v.reserve(sz.total);
+    mainHeader(L, v, (int)T.ntp, comment);
+    const size_t tlmPos = v.size() - 5 * T.ntp;
+    const KeptBlocks view{&t1, passes};
+    std::vector<uint8_t> header;
+    std::vector<std::pair<uint32_t, uint32_t>> bodies;
+    size_t next = 0;
+    for (uint32_t tp = 0; tp < T.ntp; ++tp) {
+        const size_t sot = v.size();
+        put16(v, SOT);
+        put16(v, 10);
+        put16(v, 0);  // tile index
+        put32(v, 0);  // Psot, patched below
+        v.push_back((uint8_t)tp);
+        v.push_back((uint8_t)T.ntp);
+        put16(v, SOD);
+        for (; next < T.packets.size() && T.packets[next].tp == tp; ++next) {
+            header.clear();
+            bodies.clear();
+            RcBitsT<AppendByte> bw{AppendByte{&header}};
+            bw.init();
+            const bool any = rc_packet_code(T.packets[next], T.pbands.data(), view, bw,
+                                            [&](uint32_t i, uint32_t len) { bodies.emplace_back(i, len); });
+            if (!any) {
+                v.push_back(0);
+                continue;
             }
-            size_t bytes = 14;
-            forEachPacket(L, vols[vi], c, [&](int r, int prec) { bytes += pw.write(c, r, prec, out); });
-            if (out) {
-                const uint32_t psot = (uint32_t)(out->size() - sotPos);
-                if (psot != bytes) throw std::logic_error("tile-part size mismatch");
-                for (int b = 0; b < 4; ++b) {
-                    (*out)[sotPos + 6 + b] = (uint8_t)(psot >> (24 - 8 * b));
-                    (*out)[tlmPos + 5 * tp + 1 + b] = (uint8_t)(psot >> (24 - 8 * b));
-                }
-            }
-            sz.tilePart[tp] = bytes;
-            sz.comp[c] += bytes;
-            total += bytes;
+            bw.finish();
+            v.insert(v.end(), header.begin(), header.end());
+            for (const auto& [i, len] : bodies) v.insert(v.end(), t1.code((int)i), t1.code((int)i) + len);
         }
-    total += 2;
-    if (out) put16(*out, EOC);
-    sz.total = total;
-    if (out && out->size() + slack != total) throw std::logic_error("codestream size mismatch");
-    if (sizes) *sizes = sz;
+        const uint32_t psot = (uint32_t)(v.size() - sot);
+        if (psot != sz.tilePart[tp]) throw std::logic_error("tile-part size differs from its measurement");
+        for (int b = 0; b < 4; ++b) {
+            v[sot + 6 + b] = (uint8_t)(psot >> (24 - 8 * b));
+            v[tlmPos + 5 * tp + 1 + b] = (uint8_t)(psot >> (24 - 8 * b));
+        }
+    }
+    put16(v, EOC);
+    if (v.size() + slack != sz.total) throw std::logic_error("codestream size differs from its measurement");
 }
 
 RcTables makeRcTables(const Layout& L) {
-    RcTables T;
-    const int ntp = L.fourk ? 6 : 3;
-    T.ntp = (uint32_t)ntp;
+    RcTables T = buildPacketTables(L);
     std::vector<uint8_t> hdr;
-    mainHeader(L, hdr, ntp, codestreamLabel());
+    mainHeader(L, hdr, (int)T.ntp, codestreamLabel());
     const size_t slack = 1 + kMaxDeviceTagBytes;  // the COM text at its longest
     T.fixedBytes = hdr.size() + slack + 2;
-    Volume vols[2];
-    const int nv = volumes(L, vols);
-    for (int vi = 0; vi < nv; ++vi)
-        for (int c = 0; c < kComps; ++c) {
-            const int tp = vi * 3 + c;
-            T.tpComp[tp] = c;
-            forEachPacket(L, vols[vi], c, [&](int r, int prec) {
-                const Precinct& P = L.resolution(c, r).precincts[prec];
-                RcPacket pk{(uint32_t)T.pbands.size(), (uint16_t)P.nb, (uint16_t)tp};
-                for (int k = 0; k < P.nb; ++k) {
-                    const PrecinctBand& pb = P.bands[k];
-                    const Subband& b = L.bands[pb.band];
-                    if (pb.ncbx > 4 || pb.ncby > 4) throw std::logic_error("precinct-band wider than 4 code-blocks");
-                    if (b.Mb > 31) throw std::logic_error("subband with more than 31 magnitude bit-planes");
-                    RcPBand q{};
-                    q.block0 = (uint32_t)(b.cblkBase + pb.cby0 * b.ncbx + pb.cbx0);
-                    q.stride = (uint16_t)b.ncbx;
-                    q.ncbx = (uint8_t)pb.ncbx;
-                    q.ncby = (uint8_t)pb.ncby;
-                    q.mb = (uint8_t)b.Mb;
-                    T.pbands.push_back(q);
-                }
-                T.packets.push_back(pk);
-            });
-        }
     return T;
 }
 
 CodestreamSizes rcMeasure(const Layout& L, const RcTables& T, const Tier1Ref& t1, const uint8_t* passes) {
-    struct View {
-        const Tier1Ref* t1;
-        const uint8_t* p;
-        uint32_t passes(uint32_t i) const { return p[i]; }
-        uint32_t nbps(uint32_t i) const { return t1->nbps((int)i); }
-        uint32_t len(uint32_t i) const { return p[i] ? t1->len((int)i, p[i] - 1) : 0u; }
-    } view{&t1, passes};
+    const KeptBlocks view{&t1, passes};
     (void)L;
     CodestreamSizes sz;
     sz.numTileParts = (int)T.ntp;

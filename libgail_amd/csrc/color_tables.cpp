@@ -1,11 +1,14 @@
 #include "color_tables.h"
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
-#include <sstream>
+#include <iterator>
 #include <stdexcept>
+#include <string>
 
 namespace gail_amd {
 
@@ -149,38 +152,81 @@ void buildColorTables(const ColorSpec& c, std::vector<float>& lut, float cp[kCol
     cp[38] = (float)(c.lut3d && c.lut3dSize > 1 ? c.lut3dSize : 0);
 }
 
+// A 3D LUT in the .cube text format (Adobe Cube LUT Specification 1.0): keyword lines
+// (TITLE, LUT_3D_SIZE, DOMAIN_MIN / DOMAIN_MAX) ahead of size^3 data lines of three numbers
+// each, red varying fastest; '#' starts a comment; blank lines and keywords this reader does
+// not use are skipped. Only the unit domain is accepted, and 1D LUTs are not.
 std::vector<float> readCubeLut(const std::string& path, int& size) {
-    std::ifstream in(path);
-    if (!in) throw std::runtime_error("cannot open LUT " + path);
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error(".cube LUT " + path + ": cannot be opened");
+    const std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    auto bad = [&](int lineNo, const std::string& why) {
+        return std::runtime_error(".cube LUT " + path + ", line " + std::to_string(lineNo) + ": " + why);
+    };
+    // up to `want` numbers from the text at p (strtof), how many were read; *end past them
+    auto numbers = [](const char* p, float* out, int want, const char** end) {
+        int n = 0;
+        for (; n < want; ++n) {
+            char* e;
+            const float v = std::strtof(p, &e);
+            if (e == p) break;
+            out[n] = v;
+            p = e;
+        }
+        *end = p;
+        return n;
+    };
+    auto blank = [](const char* p) {
+        while (*p == ' ' || *p == '\t') ++p;
+        return *p == 0;
+    };
     std::vector<float> data;
     size = 0;
-    std::string line;
-    while (std::getline(in, line)) {
-        if (line.empty() || line[0] == '#') continue;
-        std::istringstream ss(line);
-        std::string key;
-        ss >> key;
-        if (key == "LUT_3D_SIZE") {
-            ss >> size;
-        } else if (key == "LUT_1D_SIZE") {
-            throw std::runtime_error("1D .cube LUTs are not supported: " + path);
-        } else if (key == "DOMAIN_MIN" || key == "DOMAIN_MAX") {
-            float a, b, c;
-            ss >> a >> b >> c;
-            const float want = key == "DOMAIN_MIN" ? 0.f : 1.f;
-            if (a != want || b != want || c != want) throw std::runtime_error("LUT domain must be 0..1: " + path);
-        } else if (key != "TITLE") {
-            std::istringstream vs(line);
-            float r, g, b;
-            if (vs >> r >> g >> b) {
-                data.push_back(r);
-                data.push_back(g);
-                data.push_back(b);
+    int lineNo = 0;
+    size_t pos = 0;
+    while (pos < text.size()) {
+        size_t eol = text.find('\n', pos);
+        if (eol == std::string::npos) eol = text.size();
+        std::string line = text.substr(pos, eol - pos);
+        pos = eol + 1;
+        ++lineNo;
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line.resize(hash);
+        while (!line.empty() && (line.back() == '\r' || line.back() == ' ' || line.back() == '\t')) line.pop_back();
+        const size_t first = line.find_first_not_of(" \t");
+        if (first == std::string::npos) continue;
+        const char* p = line.c_str() + first;
+        if (std::isalpha((unsigned char)*p)) {  // a keyword line
+            const char* q = p;
+            while (*q && *q != ' ' && *q != '\t') ++q;
+            const std::string key(p, q);
+            if (!data.empty()) throw bad(lineNo, "keyword " + key + " after the table");
+            float v[3];
+            const char* end;
+            if (key == "LUT_3D_SIZE") {
+                if (numbers(q, v, 1, &end) != 1 || !blank(end) || v[0] != std::floor(v[0]))
+                    throw bad(lineNo, "LUT_3D_SIZE needs one whole number");
+                size = (int)v[0];
+                if (size < 2 || size > 256) throw bad(lineNo, "LUT_3D_SIZE must be 2 to 256");
+            } else if (key == "LUT_1D_SIZE") {
+                throw bad(lineNo, "1D LUTs are not supported");
+            } else if (key == "DOMAIN_MIN" || key == "DOMAIN_MAX") {
+                const float unit = key == "DOMAIN_MIN" ? 0.f : 1.f;
+                if (numbers(q, v, 3, &end) != 3 || !blank(end)) throw bad(lineNo, key + " needs three numbers");
+                if (v[0] != unit || v[1] != unit || v[2] != unit) throw bad(lineNo, "only the domain 0..1 is supported");
             }
+            continue;  // TITLE and keywords of other tools
         }
+        float v[3];
+        const char* end;
+        if (numbers(p, v, 3, &end) != 3 || !blank(end)) throw bad(lineNo, "a table line needs exactly three numbers");
+        data.insert(data.end(), v, v + 3);
     }
-    if (size < 2 || size > 256 || data.size() != (size_t)size * size * size * 3)
-        throw std::runtime_error("malformed 3D LUT (size / entry count) in " + path);
+    if (size == 0) throw std::runtime_error(".cube LUT " + path + ": no LUT_3D_SIZE");
+    const size_t want = (size_t)size * size * size * 3;
+    if (data.size() != want)
+        throw std::runtime_error(".cube LUT " + path + ": " + std::to_string(data.size() / 3) + " table lines, " +
+                                 std::to_string(want / 3) + " expected");
     return data;
 }
 

@@ -97,6 +97,38 @@ extern "C" GAIL_AMD_EXPORT int gail_rc_measure_check(int W, int H, int fourk, do
     }
 }
 
+// Rate control of one frame against explicit byte budgets (maxTotal for the whole codestream,
+// maxComp per component): passes[nblocks] receives the passes kept per block, lambda[3] the
+// per-component thresholds, sizes[4] the codestream's total and component bytes as Tier-2 will
+// write them, distortion the weighted squared error left. Returns the iterations, or -1.
+extern "C" GAIL_AMD_EXPORT int gail_rate_allocate(int W, int H, int fourk, double baseStep, const uint32_t* info5,
+                                                  const uint32_t* passLen, const int32_t* passDist, const uint8_t* data,
+                                                  uint64_t nblocks, uint64_t maxTotal, uint64_t maxComp, uint8_t* passes,
+                                                  double* lambda, uint64_t* sizes, double* distortion) {
+    try {
+        const Layout& L = cachedLayout(W, H, fourk != 0, baseStep);
+        if ((uint64_t)L.numCblks() != nblocks) return -1;
+        const Tier1Ref t1{info5, passLen, passDist, data, (size_t)nblocks};
+        const RateResult rr = allocateRate(L, t1, RateLimits{(size_t)maxTotal, (size_t)maxComp});
+        std::memcpy(passes, rr.passes.data(), nblocks);
+        for (int c = 0; c < 3; ++c) lambda[c] = rr.lambdaComp[c];
+        sizes[0] = rr.sizes.total;
+        for (int c = 0; c < 3; ++c) sizes[1 + c] = rr.sizes.comp[c];
+        *distortion = rr.distortion;
+        return rr.iterations;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// dciLimits(fps, maxMbps, compMbps) as two byte budgets.
+extern "C" GAIL_AMD_EXPORT void gail_dci_limits(double fps, double maxMbps, double compMbps, uint64_t* total,
+                                                uint64_t* comp) {
+    const RateLimits l = dciLimits(fps, maxMbps, compMbps);
+    *total = l.maxTotal;
+    *comp = l.maxComp;
+}
+
 // The rate-control slope grid (floor(1024 log2 s) and a bin's lowest double), for tests.
 extern "C" GAIL_AMD_EXPORT int64_t gail_slope_bin(double s) { return s > 0 && std::isfinite(s) ? slopeBin(s) : 0; }
 extern "C" GAIL_AMD_EXPORT double gail_slope_bin_edge(int64_t k) { return binEdge(k); }

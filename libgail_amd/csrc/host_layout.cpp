@@ -1,8 +1,11 @@
 // C entry points of libgail.so for frame geometry: what the library computes before it
 // creates an encoder (the reference's makeLayout, layout.cpp:181-210, and
 // containerSize, layout.cpp:221-228), in the plain form the plugin boundary takes.
+#include <algorithm>
 #include <cstring>
+#include <vector>
 
+#include "color_tables.h"
 #include "gail_amd_export.h"
 #include "gail_hip_plugin.h"
 #include "layout.h"
@@ -56,6 +59,25 @@ extern "C" GAIL_AMD_EXPORT int gail_layout_cblks(int width, int height, int four
         }
         return L.numCblks();
     } catch (...) {
+        return -1;
+    }
+}
+
+// The .cube reader the plugin uses for an app LUT file: fills up to cap floats at out and
+// the LUT's size; returns the float count, or -1 with the reason in msg.
+extern "C" GAIL_AMD_EXPORT int64_t gail_read_cube_lut(const char* path, float* out, uint64_t cap, int* size, char* msg,
+                                                      int msgCap) {
+    try {
+        int n = 0;
+        const std::vector<float> v = readCubeLut(path, n);
+        *size = n;
+        std::memcpy(out, v.data(), sizeof(float) * std::min<uint64_t>(cap, v.size()));
+        return (int64_t)v.size();
+    } catch (const std::exception& e) {
+        if (msg && msgCap > 0) {
+            std::strncpy(msg, e.what(), (size_t)msgCap - 1);
+            msg[msgCap - 1] = 0;
+        }
         return -1;
     }
 }

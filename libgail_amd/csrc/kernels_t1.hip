@@ -120,6 +120,22 @@ __device__ __forceinline__ SignMasks sign_masks(uint32_t CL, uint32_t CR, uint32
 }
 
 // Rows r0..r0+3 of mask m moved to bits 0, 8, 16 and 24 (one byte per row of a stripe).
+// Significance propagation down one column of a stripe (D.3.1 scan order): a candidate row
+// (insignificant, inside the stripe) is coded when it has a significant neighbour, and a coded
+// row that turns significant (its bit in Bp) makes the row below a neighbour in turn. seed:
+// the rows coded on their own neighbours. The rows reached from a seed through a chain of
+// such links are filled with one carry: in each run of link rows (a candidate below a
+// candidate that would turn significant) the rows from the first one whose upper neighbour is
+// coded to the end of the run are coded; adding the run's lowest bit to the still-unreached
+// rows of the run clears exactly the rows below that first one.
+__device__ __forceinline__ uint32_t column_fill(uint32_t seed, uint32_t cand, uint32_t Bp) {
+    const uint32_t link = cand & ((cand & Bp) << 1);  // row r can be reached from row r - 1
+    const uint32_t unseeded = link & ~(seed << 1);
+    const uint32_t runStart = link & ~(link << 1);
+    const uint32_t below = ((unseeded + (runStart & unseeded)) ^ unseeded) & unseeded;
+    return seed | (link & ~below);
+}
+
 __device__ __forceinline__ uint32_t spread4(uint32_t m, int r0) { return (((m >> r0) & 0xFu) * 0x204081u) & 0x01010101u; }
 
 // Zero-coding neighbourhoods (D.3.1) of the four rows of stripe r0, one 7-bit code per byte:
@@ -340,26 +356,22 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                         bool coded = false;  // wave-uniform: some lane codes in this stripe
                         if (type == 0) {
                             // ---- significance propagation (D.3.1)
-                            const uint32_t cand = ~S & SM;
-                            const uint32_t nS = neighbour_any(S, SL, SR);
-                            if (__any((cand & nS) != 0)) {
+                            const uint32_t open = ~S & SM;  // rows this pass may code
+                            const uint32_t sigAround = neighbour_any(S, SL, SR);
+                            if (__any((open & sigAround) != 0)) {
                                 coded = true;
-                                // least fixpoint of "eligible given the news to the left and above"
-                                uint32_t NEW = 0, E = 0;
+                                // Columns are scanned left to right, so a column also sees the
+                                // samples its left neighbour turned significant in this pass:
+                                // each round every lane codes its column given the left lane's
+                                // current turned set, until no lane's set changes (the least
+                                // fixed point, reached within one round per column of a chain)
+                                uint32_t NEW = 0, E;
                                 for (;;) {
-                                    const uint32_t NL = from_left(NEW, x);
-                                    uint32_t e = cand & (nS | NL | (NL << 1) | (NL >> 1));
-                                    uint32_t nn = e & Bp;
-                                    e |= cand & (nn << 1);
-                                    nn = e & Bp;
-                                    e |= cand & (nn << 1);
-                                    nn = e & Bp;
-                                    e |= cand & (nn << 1);
-                                    nn = e & Bp;
-                                    const bool changed = nn != NEW;
-                                    NEW = nn;
-                                    E = e;
-                                    if (!__any(changed)) break;
+                                    const uint32_t left = from_left(NEW, x);
+                                    E = column_fill(open & (sigAround | left | (left << 1) | (left >> 1)), open, Bp);
+                                    const uint32_t turned = E & Bp;
+                                    if (!__any(turned != NEW)) break;
+                                    NEW = turned;
                                 }
                                 const uint32_t NL = from_left(NEW, x);
                                 const uint32_t CL = SL | NL, CR = SR, CU = S | NEW, CD = S;

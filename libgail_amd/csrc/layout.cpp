@@ -1,83 +1,111 @@
-// Frame geometry and quantisation steps (the reference's layout.cpp:21-210, restated).
+// Frame geometry and quantisation steps of a DCI frame, from ISO/IEC 15444-1 B.5-B.7 (one
+// tile at the origin) and E.1; the resulting tables equal the reference's layout.cpp:21-210.
 #include "layout.h"
 
 #include <algorithm>
 #include <cmath>
-#include <map>
-#include <mutex>
 #include <stdexcept>
+#include <vector>
 
 namespace gail_amd {
 
 namespace {
 
-// 9/7 synthesis lifting in double precision on n samples (dwt_cpu.cpp:162-178): undo the
-// scaling, then delta, gamma, beta, alpha with whole-sample symmetric extension.
-void inverse97(std::vector<double>& x, int n) {
+constexpr int kMaxNormLevel = 8;  // synthesis norms are tabled for decomposition levels 1..8
+
+// ceil(v / 2^n)
+int ceilShift(int v, int n) { return (v + (1 << n) - 1) >> n; }
+
+// One axis of a subband of decomposition level n (B.5, tile origin 0): the low band holds
+// ceil(len / 2^n) samples from 0, the high band the rest of level n-1's low band, placed
+// right after the low band in the Mallat arrangement.
+void bandAxis(int len, int n, bool high, int& start, int& size) {
+    const int low = ceilShift(len, n);
+    start = high ? low : 0;
+    size = high ? ceilShift(len, n - 1) - low : low;
+}
+
+// R_b of E.1: the nominal dynamic range of a band, the component depth plus the band's
+// analysis gain in bits (0 for LL, 1 for HL and LH, 2 for HH: one per high-pass axis).
+int nominalRange(int orient) { return kBitDepth + __builtin_popcount((unsigned)orient); }
+
+// One level of 9/7 synthesis on the n samples at x (interleaved low / high order on output),
+// in double precision: low coefficients scaled by K, high by 1/K, then the four lifting
+// steps undone in reverse order with whole-sample symmetric extension (dwt_cpu.cpp:162-178).
+void synthesize97(double* x, int n) {
     if (n < 2) return;
     const double K = 1.230174104914001;
-    const double steps[4] = {-0.443506852043971, -0.882911075530934, 0.052980118572961, 1.586134342059924};
+    const double undo[4] = {-0.443506852043971, -0.882911075530934, 0.052980118572961, 1.586134342059924};
     std::vector<double> t(n);
-    const int sn = (n + 1) / 2;
-    for (int i = 0; i < sn; ++i) t[2 * i] = x[i] * K;
-    for (int i = 0; i < n / 2; ++i) t[2 * i + 1] = x[sn + i] * (1.0 / K);
+    const int nLow = (n + 1) / 2;
+    for (int i = 0; i < nLow; ++i) t[2 * i] = x[i] * K;
+    for (int i = 0; i < n / 2; ++i) t[2 * i + 1] = x[nLow + i] * (1.0 / K);
     for (int s = 0; s < 4; ++s) {
-        const int parity = (s & 1) ? 1 : 0;  // delta / beta on even samples, gamma / alpha on odd
-        for (int i = parity; i < n; i += 2) {
-            const double l = i > 0 ? t[i - 1] : t[1];
-            const double r = i + 1 < n ? t[i + 1] : l;
-            t[i] += steps[s] * (l + r);
+        for (int i = s & 1; i < n; i += 2) {  // even samples first, then odd, alternating
+            const double left = i > 0 ? t[i - 1] : t[1];
+            const double right = i + 1 < n ? t[i + 1] : left;
+            t[i] += undo[s] * (left + right);
         }
     }
-    for (int i = 0; i < n; ++i) x[i] = t[i];
+    std::copy(t.begin(), t.end(), x);
 }
 
-// L2 norm of a band's 1-D synthesis basis function: a unit impulse mid-band, inverse
-// transformed level by level.
-double synth1d(int level, bool high) {
-    const int n = 64 << level;
-    std::vector<double> sig(n, 0.0);
-    const int bandLen = n >> level;
-    sig[high ? bandLen + bandLen / 2 : bandLen / 2] = 1.0;
-    for (int l = level; l >= 1; --l) {
-        const int len = n >> (l - 1);
-        std::vector<double> part(sig.begin(), sig.begin() + len);
-        inverse97(part, len);
-        std::copy(part.begin(), part.end(), sig.begin());
+// L2 norm of the 1-D synthesis basis function of a level-n band: a unit coefficient in the
+// middle of the band, synthesised through n levels (far from the signal's ends, so that the
+// extension plays no part), and the root of the sum of squares.
+double basisNorm(int n, bool high) {
+    const int len = 64 << n;
+    std::vector<double> sig(len, 0.0);
+    const int bandLen = len >> n;
+    sig[(high ? bandLen : 0) + bandLen / 2] = 1.0;
+    for (int l = n; l >= 1; --l) synthesize97(sig.data(), len >> (l - 1));
+    double energy = 0;
+    for (double v : sig) energy += v * v;
+    return std::sqrt(energy);
+}
+
+struct NormTable {
+    double low[kMaxNormLevel + 1], high[kMaxNormLevel + 1];
+    NormTable() {
+        low[0] = high[0] = 1.0;
+        for (int n = 1; n <= kMaxNormLevel; ++n) {
+            low[n] = basisNorm(n, false);
+            high[n] = basisNorm(n, true);
+        }
     }
+};
+
+const NormTable& norms() {
+    static const NormTable table;  // built once, on first use (thread-safe initialisation)
+    return table;
+}
+
+// Squared L2 norm of component c's column of the inverse ICT (G.3: R = Y + 1.402 Cr,
+// G = Y - 0.34413 Cb - 0.71414 Cr, B = Y + 1.772 Cb), formed as the square of the root, the
+// way the weights are compared bit for bit with the oracle's.
+double ictColumnWeight(int c) {
+    static const double inverseIct[3][3] = {{1.0, 0.0, 1.402}, {1.0, -0.34413, -0.71414}, {1.0, 1.772, 0.0}};
     double e = 0;
-    for (double v : sig) e += v * v;
+    for (int row = 0; row < 3; ++row) e += inverseIct[row][c] * inverseIct[row][c];
     return std::sqrt(e);
 }
 
-double ictSynthesisNorm(int comp) {
-    static const double cols[3][3] = {{1, 1, 1}, {0, -0.34413, 1.772}, {1.402, -0.71414, 0}};
-    double e = 0;
-    for (int i = 0; i < 3; ++i) e += cols[comp][i] * cols[comp][i];
-    return std::sqrt(e);
-}
-
-int nominalRange(const Subband& b) { return kBitDepth + (b.orient == 0 ? 0 : (b.orient == 3 ? 2 : 1)); }
-
-void applyStep(Subband& b, int guardBits, int eps, int mu) {
+// Step of a band from its exponent / mantissa pair (E-3) and its magnitude bit-planes (E-2).
+void setStep(Subband& b, int guardBits, int eps, int mu) {
     b.eps = eps;
     b.mu = mu;
-    b.delta = (float)(std::ldexp(1.0, nominalRange(b) - eps) * (1.0 + mu / 2048.0));
+    b.delta = (float)(std::ldexp(1.0 + mu / 2048.0, nominalRange(b.orient) - eps));
     b.Mb = guardBits + eps - 1;
-    if (b.Mb > 22) throw std::runtime_error("quantization step too fine (more than 22 bitplanes)");
+    if (b.Mb > 22) throw std::runtime_error("a subband would need " + std::to_string(b.Mb) +
+                                            " magnitude bit-planes (22 at most): base step too small");
 }
 
 }  // namespace
 
 double synthesisNorm(int level, int orient) {
-    static std::mutex m;
-    static std::map<std::pair<int, int>, double> cache;
-    std::lock_guard<std::mutex> lock(m);
-    const auto key = std::make_pair(level, orient);
-    if (auto it = cache.find(key); it != cache.end()) return it->second;
-    const double v = synth1d(level, orient == 1 || orient == 3) * synth1d(level, orient == 2 || orient == 3);
-    cache[key] = v;
-    return v;
+    if (level < 1 || level > kMaxNormLevel) throw std::out_of_range("decomposition level out of range");
+    const NormTable& t = norms();
+    return ((orient & 1) ? t.high[level] : t.low[level]) * ((orient & 2) ? t.high[level] : t.low[level]);
 }
 
 Layout makeGeometry(int width, int height, bool fourk, int levels, int guardBits) {
@@ -87,122 +115,79 @@ Layout makeGeometry(int width, int height, bool fourk, int levels, int guardBits
     L.height = height;
     L.levels = levels;
     L.guardBits = guardBits;
-    std::vector<int> ws(levels + 1), hs(levels + 1);
-    ws[0] = width;
-    hs[0] = height;
-    for (int l = 1; l <= levels; ++l) {
-        ws[l] = (ws[l - 1] + 1) / 2;
-        hs[l] = (hs[l - 1] + 1) / 2;
-    }
-    int count = 0;
+    L.bands.reserve(kComps * (3 * levels + 1));
+    int blocks = 0;
     for (int c = 0; c < kComps; ++c)
         for (int r = 0; r <= levels; ++r)
-            for (int o = 0; o < (r == 0 ? 1 : 3); ++o) {
+            for (int orient = (r == 0 ? 0 : 1); orient <= (r == 0 ? 0 : 3); ++orient) {
                 Subband b;
                 b.comp = c;
                 b.res = r;
-                b.orient = r == 0 ? 0 : o + 1;
-                b.level = r == 0 ? levels : levels - r + 1;
-                const int n = b.level;
-                const bool hx = b.orient == 1 || b.orient == 3, hy = b.orient == 2 || b.orient == 3;
-                b.px0 = hx ? ws[n] : 0;
-                b.py0 = hy ? hs[n] : 0;
-                b.w = hx ? ws[n - 1] - ws[n] : ws[n];
-                b.h = hy ? hs[n - 1] - hs[n] : hs[n];
-                b.ncbx = (b.w + kCblk - 1) / kCblk;
-                b.ncby = (b.h + kCblk - 1) / kCblk;
-                b.cblkBase = count;
-                count += b.ncbx * b.ncby;
+                b.orient = orient;
+                b.level = r == 0 ? levels : levels + 1 - r;
+                bandAxis(width, b.level, orient & 1, b.px0, b.w);
+                bandAxis(height, b.level, orient & 2, b.py0, b.h);
+                b.ncbx = ceilShift(b.w, kCblkLog2);
+                b.ncby = ceilShift(b.h, kCblkLog2);
+                b.cblkBase = blocks;
+                blocks += b.ncbx * b.ncby;
                 L.bands.push_back(b);
             }
-    L.cblks.resize(count);
+    // code-blocks (B.7): a 32 x 32 grid anchored at each band's origin, cut at its far edges
+    L.cblks.reserve(blocks);
     for (int bi = 0; bi < (int)L.bands.size(); ++bi) {
         const Subband& b = L.bands[bi];
-        for (int cy = 0; cy < b.ncby; ++cy)
-            for (int cx = 0; cx < b.ncbx; ++cx) {
-                CodeBlock& cb = L.cblks[b.cblkBase + cy * b.ncbx + cx];
-                cb.band = bi;
-                cb.x0 = cx * kCblk;
-                cb.y0 = cy * kCblk;
-                cb.w = std::min(kCblk, b.w - cb.x0);
-                cb.h = std::min(kCblk, b.h - cb.y0);
-            }
+        for (int y0 = 0; y0 < b.h; y0 += kCblk)
+            for (int x0 = 0; x0 < b.w; x0 += kCblk)
+                L.cblks.push_back(CodeBlock{bi, x0, y0, std::min(kCblk, b.w - x0), std::min(kCblk, b.h - y0)});
     }
-    for (int c = 0; c < kComps; ++c)
-        for (int r = 0; r <= levels; ++r) {
-            Resolution R;
-            R.comp = c;
-            R.res = r;
-            R.w = ws[levels - r];
-            R.h = hs[levels - r];
-            R.ppx = R.ppy = r == 0 ? 7 : 8;
-            R.npx = (R.w + (1 << R.ppx) - 1) >> R.ppx;
-            R.npy = (R.h + (1 << R.ppy) - 1) >> R.ppy;
-            const int bandPrecLog2 = r == 0 ? R.ppx : R.ppx - 1;
-            const int cbPerPrec = 1 << (bandPrecLog2 - kCblkLog2);
-            R.precincts.resize((size_t)R.npx * R.npy);
-            for (int py = 0; py < R.npy; ++py)
-                for (int px = 0; px < R.npx; ++px) {
-                    Precinct& P = R.precincts[(size_t)py * R.npx + px];
-                    P.nb = r == 0 ? 1 : 3;
-                    for (int k = 0; k < P.nb; ++k) {
-                        const int bi = L.bandIndex(c, r, r == 0 ? 0 : k + 1);
-                        const Subband& b = L.bands[bi];
-                        PrecinctBand& pb = P.bands[k];
-                        pb.band = bi;
-                        pb.cbx0 = px * cbPerPrec;
-                        pb.cby0 = py * cbPerPrec;
-                        pb.ncbx = std::max(0, std::min(cbPerPrec, b.ncbx - pb.cbx0));
-                        pb.ncby = std::max(0, std::min(cbPerPrec, b.ncby - pb.cby0));
-                        if (pb.ncbx == 0 || pb.ncby == 0) pb.ncbx = pb.ncby = 0;
-                    }
-                }
-            L.resolutions.push_back(std::move(R));
-        }
     return L;
 }
 
 void setSignaledSteps(Layout& L, const int* eps, const int* mu) {
-    const int nb = L.bandsPerComp();
-    for (int bi = 0; bi < (int)L.bands.size(); ++bi) {
-        const int i = bi % nb;
-        if (eps[i] < 0 || eps[i] > 31 || mu[i] < 0 || mu[i] > 2047) throw std::runtime_error("quantization step out of range");
-        applyStep(L.bands[bi], L.guardBits, eps[i], mu[i]);
+    const int perComp = L.bandsPerComp();
+    for (size_t bi = 0; bi < L.bands.size(); ++bi) {
+        const int k = (int)(bi % perComp);
+        if (eps[k] < 0 || eps[k] > 31 || mu[k] < 0 || mu[k] > 2047)
+            throw std::runtime_error("signalled quantisation step out of range");
+        setStep(L.bands[bi], L.guardBits, eps[k], mu[k]);
     }
 }
 
 Layout makeLayout(int width, int height, bool fourk, double baseStep) {
-    if (fourk ? (width > 4096 || height > 2160) : (width > 2048 || height > 1080))
-        throw std::runtime_error("image size exceeds the DCI container for this profile");
-    if (width < 64 || height < 64) throw std::runtime_error("image too small");
+    const int maxW = fourk ? 4096 : 2048, maxH = fourk ? 2160 : 1080;
+    if (width > maxW || height > maxH)
+        throw std::runtime_error("a " + std::to_string(width) + "x" + std::to_string(height) + " picture does not fit the " +
+                                 (fourk ? "4K" : "2K") + " DCI container");
+    if (width < 64 || height < 64) throw std::runtime_error("pictures smaller than 64x64 are not coded");
     Layout L = makeGeometry(width, height, fourk, fourk ? 6 : 5, fourk ? 2 : 1);
     L.baseStep = baseStep;
     for (Subband& b : L.bands) {
-        // equal image-domain MSE per unit coefficient error: delta_b ~ baseStep / ||synthesis||
+        // the same image-domain error per unit of coefficient error in every band: the step
+        // is baseStep over the band's synthesis norm, written as 2^(R_b - eps)(1 + mu/2^11)
         const double norm = synthesisNorm(b.level, b.orient);
-        const double x = baseStep / norm / std::ldexp(1.0, nominalRange(b));
-        int eps = (int)std::ceil(-std::log2(x));
-        if (std::ldexp(x, eps) < 1.0) ++eps;
-        if (std::ldexp(x, eps) >= 2.0) --eps;
-        int mu = (int)std::lround((std::ldexp(x, eps) - 1.0) * 2048.0);
-        if (mu >= 2048) {
+        int e2;
+        const double m = std::frexp(baseStep / norm / std::ldexp(1.0, nominalRange(b.orient)), &e2);  // m in [0.5, 1)
+        int eps = 1 - e2;
+        int mu = (int)std::lround((2.0 * m - 1.0) * 2048.0);
+        if (mu == 2048) {  // the mantissa rounded up to 2: one exponent step coarser
             mu = 0;
             --eps;
         }
-        if (eps < 0 || eps > 31) throw std::runtime_error("quantization step out of range");
-        applyStep(b, L.guardBits, eps, mu);
-        const double wc = ictSynthesisNorm(b.comp);
+        if (eps < 0 || eps > 31) throw std::runtime_error("base step out of range for the quantisation exponent");
+        setStep(b, L.guardBits, eps, mu);
+        const double wc = ictColumnWeight(b.comp);
         b.weight = norm * norm * wc * wc;
     }
     return L;
 }
 
 void containerSize(bool fourk, int container, int& w, int& h) {
-    switch (container) {
-    case 1: w = fourk ? 4096 : 2048; h = fourk ? 1716 : 858; break;
-    case 2: w = fourk ? 4096 : 2048; h = fourk ? 2160 : 1080; break;
-    default: w = fourk ? 3996 : 1998; h = fourk ? 2160 : 1080; break;
-    }
+    // flat 1.85:1, scope 2.39:1, full container; 4K is twice 2K in each direction
+    static const int size2k[3][2] = {{1998, 1080}, {2048, 858}, {2048, 1080}};
+    const int* s = size2k[container == 1 || container == 2 ? container : 0];
+    w = fourk ? 2 * s[0] : s[0];
+    h = fourk ? 2 * s[1] : s[1];
 }
 
 }  // namespace gail_amd

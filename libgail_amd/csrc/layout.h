@@ -1,6 +1,6 @@
 // Frame geometry of a single-tile DCI JPEG 2000 frame: subbands in Mallat layout, 32x32
-// code-blocks with ragged edges, precincts (128 at r = 0, 256 elsewhere) and the
-// quantisation step of every subband (layout.cpp:77-210 of the reference).
+// code-blocks with ragged edges and the quantisation step of every subband (the values of
+// the reference's layout.cpp:77-210; precincts are the packet tables' business).
 #pragma once
 #include <cstdint>
 #include <string>
@@ -32,28 +32,12 @@ struct CodeBlock {
     int x0 = 0, y0 = 0, w = 0, h = 0;
 };
 
-struct PrecinctBand {
-    int band = -1;
-    int cbx0 = 0, cby0 = 0, ncbx = 0, ncby = 0;
-};
-
-struct Precinct {
-    int nb = 0;
-    PrecinctBand bands[3];
-};
-
-struct Resolution {
-    int comp = 0, res = 0, w = 0, h = 0, ppx = 0, ppy = 0, npx = 0, npy = 0;
-    std::vector<Precinct> precincts;
-};
-
 struct Layout {
     bool fourk = false;
     int width = 0, height = 0, levels = 5, guardBits = 1;
     double baseStep = 1.0;
     std::vector<Subband> bands;
     std::vector<CodeBlock> cblks;
-    std::vector<Resolution> resolutions;  // comp * (levels + 1) + res
 
     int numRes() const { return levels + 1; }
     int bandsPerComp() const { return 3 * levels + 1; }
@@ -61,10 +45,10 @@ struct Layout {
     int bandIndex(int comp, int res, int orient) const {
         return comp * bandsPerComp() + (res == 0 ? 0 : 1 + 3 * (res - 1) + (orient - 1));
     }
-    const Resolution& resolution(int comp, int res) const { return resolutions[comp * numRes() + res]; }
 };
 
-// Geometry only (no step sizes).
+// Geometry only (no step sizes). Bands are ordered component by component, LL first, then
+// HL, LH, HH of each resolution from the coarsest; blocks band by band in raster order.
 Layout makeGeometry(int width, int height, bool fourk, int levels, int guardBits);
 // Full layout with the profile's levels / guard bits and steps chosen from baseStep.
 // Throws std::runtime_error on an invalid size.

@@ -133,9 +133,15 @@ GAIL_HD int rc_bin_of(double s, int64_t kLo, int nb, const unsigned long long* m
 }
 
 // ---------------------------------------------------------------- packet headers
-// B.10.1 bit writer that only counts: bytes fill MSB first and a byte after 0xFF carries 7
-// bits (t2.cpp / bitio.h).
-struct RcBits {
+// B.10.1 packet-header bits: bytes fill MSB first and a byte after 0xFF carries 7 bits, so no
+// marker can appear in a header. Each finished byte goes to `out` (RcCount drops it: the
+// device only needs the count; the host's Tier-2 writer appends it to the codestream).
+struct RcCount {
+    GAIL_HD void operator()(uint32_t) const {}
+};
+
+template <class Out> struct RcBitsT {
+    Out out;
     uint32_t acc, last, count;
     int used, room;
     GAIL_HD void init() {
@@ -145,6 +151,7 @@ struct RcBits {
     }
     GAIL_HD void emit() {
         last = acc & 0xFFu;
+        out(last);
         ++count;
         room = last == 0xFFu ? 7 : 8;
         acc = 0;
@@ -159,6 +166,7 @@ struct RcBits {
             if (used == room) emit();
         }
     }
+    // Pads the last byte with zeros; a header ending in 0xFF still owes its stuffed byte.
     GAIL_HD uint32_t finish() {
         if (used > 0 || count == 0) {
             acc <<= room - used;
@@ -171,11 +179,12 @@ struct RcBits {
         return count;
     }
 };
+using RcBits = RcBitsT<RcCount>;
 
 GAIL_HD int rc_floor_log2(uint32_t v) { return 31 - __builtin_clz(v); }
 
 // Table B.4: codewords for the number of coding passes.
-GAIL_HD void rc_pass_count(RcBits& bw, int n) {
+template <class Sink> GAIL_HD void rc_pass_count(Sink& bw, int n) {
     if (n == 1) bw.bits(0, 1);
     else if (n == 2) bw.bits(2, 2);
     else if (n <= 5) bw.bits(0xCu | (uint32_t)(n - 3), 4);
@@ -183,9 +192,11 @@ GAIL_HD void rc_pass_count(RcBits& bw, int n) {
     else bw.bits(0xFF80u | (uint32_t)(n - 37), 16);
 }
 
-// Bytes of one packet (header and body) with blk.passes(i) passes of frame-local block i
-// (one layer; t2.cpp:31-115). blk gives passes(i), nbps(i) and len(i), the truncation
-// length after its passes.
+// Codes one packet (one layer; B.10) with blk.passes(i) passes of frame-local block i: the
+// header bits go to bw (after a leading 1: the packet is not empty), and body(i, len) is
+// called for each included block in the order its bytes follow the header. Returns whether
+// any block is included; an empty packet is instead the single byte 0x00, which the caller
+// writes. blk gives passes(i), nbps(i) and len(i), the truncation length after its passes.
 //
 // The two tag trees of a precinct-band (B.10.2) are coded without node arrays. A DCI
 // precinct-band holds at most 4 x 4 code-blocks, so a tree has at most three levels: the
@@ -199,13 +210,10 @@ GAIL_HD void rc_pass_count(RcBits& bw, int n) {
 // That is what the general tag-tree coder emits for these thresholds. Everything else
 // lives in registers (the leaves of a band are unrolled), so the device sizes a packet in
 // one thread without memory traffic beyond its blocks.
-template <class Blocks>
-GAIL_HD unsigned long long rc_packet_bytes(const RcPacket& P, const RcPBand* pbands, const Blocks& blk) {
-    RcBits bw;
-    bw.init();
-    bw.bits(1, 1);  // non-empty; an empty packet is the single byte 0x00 (below)
+template <class Blocks, class Sink, class Body>
+GAIL_HD bool rc_packet_code(const RcPacket& P, const RcPBand* pbands, const Blocks& blk, Sink& bw, const Body& body) {
+    bw.bits(1, 1);
     bool any = false;
-    unsigned long long body = 0;
     for (int k = 0; k < P.nband; ++k) {
         const RcPBand pb = pbands[P.first + k];
         const int w = pb.ncbx, h = pb.ncby;
@@ -292,10 +300,24 @@ GAIL_UNROLL
             const int incr = need - (3 + extra) > 0 ? need - (3 + extra) : 0;
             bw.bits(((1u << incr) - 1u) << 1, incr + 1);  // Lblock increment: incr ones, then 0
             bw.bits(len[q], 3 + incr + extra);
-            body += len[q];
+            body(pb.block0 + (uint32_t)(q >> 2) * pb.stride + (uint32_t)(q & 3), len[q]);
         }
     }
-    if (!any) return 1;
+    return any;
+}
+
+struct RcBodySum {
+    unsigned long long* bytes;
+    GAIL_HD void operator()(uint32_t, uint32_t len) const { *bytes += len; }
+};
+
+// Bytes of one packet, header and body.
+template <class Blocks>
+GAIL_HD unsigned long long rc_packet_bytes(const RcPacket& P, const RcPBand* pbands, const Blocks& blk) {
+    RcBits bw;
+    bw.init();
+    unsigned long long body = 0;
+    if (!rc_packet_code(P, pbands, blk, bw, RcBodySum{&body})) return 1;
     return bw.finish() + body;
 }
 

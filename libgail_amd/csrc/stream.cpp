@@ -59,22 +59,15 @@ bool allZero(const uint64_t* p, int n) {
 }
 
 bool planarInfo(gail_pixel_format f, PlanarInfo& p) {
-    switch (f) {
-    case GAIL_PIX_YUV420P: p = {8, 1, 1, 0}; return true;
-    case GAIL_PIX_YUV420P10: p = {10, 1, 1, 0}; return true;
-    case GAIL_PIX_YUV420P12: p = {12, 1, 1, 0}; return true;
-    case GAIL_PIX_YUV422P: p = {8, 1, 0, 0}; return true;
-    case GAIL_PIX_YUV422P10: p = {10, 1, 0, 0}; return true;
-    case GAIL_PIX_YUV422P12: p = {12, 1, 0, 0}; return true;
-    case GAIL_PIX_YUV444P: p = {8, 0, 0, 0}; return true;
-    case GAIL_PIX_YUV444P10: p = {10, 0, 0, 0}; return true;
-    case GAIL_PIX_YUV444P12: p = {12, 0, 0, 0}; return true;
-    case GAIL_PIX_GBRP: p = {8, 0, 0, 1}; return true;
-    case GAIL_PIX_GBRP10: p = {10, 0, 0, 1}; return true;
-    case GAIL_PIX_GBRP12: p = {12, 0, 0, 1}; return true;
-    case GAIL_PIX_GBRP16: p = {16, 0, 0, 1}; return true;
-    default: return false;
-    }
+    // the planar formats of gail_pixel_format, in enumeration order from GAIL_PIX_YUV420P:
+    // three Y'CbCr depths each at 4:2:0, 4:2:2 and 4:4:4, then G'B'R' at 8, 10, 12 and 16 bits
+    static const PlanarInfo table[] = {
+        {8, 1, 1, 0}, {10, 1, 1, 0}, {12, 1, 1, 0}, {8, 1, 0, 0}, {10, 1, 0, 0}, {12, 1, 0, 0}, {8, 0, 0, 0},
+        {10, 0, 0, 0}, {12, 0, 0, 0}, {8, 0, 0, 1}, {10, 0, 0, 1}, {12, 0, 0, 1}, {16, 0, 0, 1}};
+    const int i = (int)f - (int)GAIL_PIX_YUV420P;
+    if (i < 0 || i >= (int)(sizeof table / sizeof table[0])) return false;
+    p = table[i];
+    return true;
 }
 
 std::unique_ptr<StreamSpec> StreamSpec::make(const gail_config& c) {
@@ -150,8 +143,18 @@ std::unique_ptr<StreamSpec> StreamSpec::make(const gail_config& c) {
             throw ApiError(GAIL_ERROR_INVALID, "invalid matrix");
         s.input.kr = kr[cfg.input.matrix];
         s.input.kb = kb[cfg.input.matrix];
+        // chroma planes of ceil(w / 2^sx) x ceil(h / 2^sy) samples, one or two bytes each
+        const size_t bytesPer = pi.bits > 8 ? 2 : 1;
         const int cw = (e->picW + pi.sx) >> pi.sx, ch = (e->picH + pi.sy) >> pi.sy;
-        e->inBytes = ((size_t)e->picW * e->picH + 2 * (size_t)cw * ch) * (pi.bits > 8 ? 2 : 1);
+        e->planar = true;
+        size_t offset = 0;
+        for (int c = 0; c < 3; ++c) {
+            e->planes[c].offset = offset;
+            e->planes[c].rowBytes = (size_t)(c ? cw : e->picW) * bytesPer;
+            e->planes[c].rows = c ? ch : e->picH;
+            offset += e->planes[c].rowBytes * e->planes[c].rows;
+        }
+        e->inBytes = offset;
     } else if (fmt == GAIL_PIX_RGB48) {
         s.input.kind = 0;
         e->inBytes = (size_t)e->width * e->height * 6;
@@ -217,18 +220,17 @@ int StreamSpec::checkFrame(const gail_frame& f, std::string& why) const {
 
 void StreamSpec::fill(const gail_frame& f, uint8_t* dst) const {
     const gail_pixel_format fmt = cfg.input.format;
-    PlanarInfo pi;
-    if (planarInfo(fmt, pi)) {
-        const size_t bps = pi.bits > 8 ? 2 : 1;
-        const int cw = (picW + pi.sx) >> pi.sx, ch = (picH + pi.sy) >> pi.sy;
-        uint8_t* plane = dst;
+    if (planar) {
         for (int c = 0; c < 3; ++c) {
-            const int w = c ? cw : picW, h = c ? ch : picH;
-            const size_t row = (size_t)w * bps;
-            const ptrdiff_t stride = f.stride[c] ? f.stride[c] : (ptrdiff_t)row;
-            const uint8_t* s = static_cast<const uint8_t*>(f.data[c]);
-            for (int y = 0; y < h; ++y) std::memcpy(plane + y * row, s + y * stride, row);
-            plane += row * h;
+            const Plane& pl = planes[c];
+            const uint8_t* src = static_cast<const uint8_t*>(f.data[c]);
+            const ptrdiff_t stride = f.stride[c] ? f.stride[c] : (ptrdiff_t)pl.rowBytes;
+            uint8_t* out = dst + pl.offset;
+            if (stride == (ptrdiff_t)pl.rowBytes) {  // unpadded rows: the plane in one copy
+                std::memcpy(out, src, pl.rowBytes * pl.rows);
+                continue;
+            }
+            for (int y = 0; y < pl.rows; ++y, out += pl.rowBytes, src += stride) std::memcpy(out, src, pl.rowBytes);
         }
         return;
     }
@@ -268,17 +270,10 @@ const void* StreamSpec::directInput(const gail_frame& f) const {
     const gail_pixel_format fmt = cfg.input.format;
     const uint8_t* base = static_cast<const uint8_t*>(f.data[0]);
     if (!base) return nullptr;
-    PlanarInfo pi;
-    if (planarInfo(fmt, pi)) {
-        const size_t bps = pi.bits > 8 ? 2 : 1;
-        const size_t cw = (size_t)((picW + pi.sx) >> pi.sx), ch = (size_t)((picH + pi.sy) >> pi.sy);
-        const size_t rows[3] = {(size_t)picW * bps, cw * bps, cw * bps};
-        const size_t sizes[3] = {(size_t)picW * picH * bps, cw * ch * bps, cw * ch * bps};
-        const uint8_t* expect = base;
+    if (planar) {
         for (int c = 0; c < 3; ++c) {
-            if (f.data[c] != expect) return nullptr;
-            if (f.stride[c] && (size_t)f.stride[c] != rows[c]) return nullptr;
-            expect += sizes[c];
+            if (f.data[c] != base + planes[c].offset) return nullptr;
+            if (f.stride[c] && (size_t)f.stride[c] != planes[c].rowBytes) return nullptr;
         }
         return base;
     }

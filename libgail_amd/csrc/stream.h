@@ -48,6 +48,14 @@ struct StreamSpec {
     RateLimits limits;
     int width = 0, height = 0, picW = 0, picH = 0, ox = 0, oy = 0;
     size_t inBytes = 0;
+    // planar inputs: where each plane lies in the plugin's input layout (planes back to back,
+    // rows unpadded) and its rows
+    struct Plane {
+        size_t offset = 0, rowBytes = 0;
+        int rows = 0;
+    };
+    bool planar = false;
+    Plane planes[3];
     double slopeFactor = 0.25;  // gail_api.cpp:607
     std::vector<float> lut3d;   // the stream's copy of an app 3D LUT (settings point here)
 

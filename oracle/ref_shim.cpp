@@ -1,6 +1,6 @@
 // C entry points over the reference's own CPU sources (layout.cpp, dwt_cpu.cpp, quant.cpp,
 // t1_ref.cpp, color.cpp; for decoding t1_dec.cpp, cpu_decode_backend.cpp,
-// codestream_parse.cpp, t2_parse.cpp), compiled from where they lie by
+// codestream_parse.cpp, t2_parse.cpp; the DCI checker codestream_check.cpp), compiled from where they lie by
 // oracle/Makefile into oracle/_ref/libgailref.so. TEST INFRASTRUCTURE ONLY: used to
 // write golden vectors (tests/golden/make_golden.py), to cross-check the oracle, and
 // as the timed CPU baseline (the frame flow of cpu_backend.cpp:61-132 over the
@@ -13,6 +13,7 @@
 #include <thread>
 #include <vector>
 
+#include "core/codestream_check.h"
 #include "core/codestream_parse.h"
 #include "core/color.h"
 #include "core/decode_backend.h"
@@ -357,6 +358,30 @@ int ref_parse_blocks(const uint8_t* data, size_t size, int* W, int* H, int* four
                         b.eps, b.mu, b.Mb, b.delta, 0.f, b.weight};
         }
         return (int)out.size();
+    } catch (...) {
+        return -1;
+    }
+}
+
+// The reference's DCI compliance checker (codestream_check.cpp:43-280) on a codestream.
+// Returns 1 when it passes, 0 when it does not (the errors, "; "-joined, in msg), -1 on an
+// exception. facts (may be NULL) receives total bytes, the three component byte counts, the
+// tile-part count, rsiz, width, height and decomposition levels.
+int ref_check_codestream(const uint8_t* data, size_t size, double fps, char* msg, int cap, uint64_t* facts) {
+    try {
+        const CheckReport r = checkDciCodestream(data, size, fps);
+        std::string all;
+        for (const std::string& e : r.errors) all += (all.empty() ? "" : "; ") + e;
+        if (msg && cap > 0) {
+            std::strncpy(msg, all.c_str(), (size_t)cap - 1);
+            msg[cap - 1] = 0;
+        }
+        if (facts) {
+            const uint64_t f[9] = {r.totalBytes, r.compBytes[0], r.compBytes[1], r.compBytes[2], (uint64_t)r.tileParts,
+                                   (uint64_t)r.rsiz, (uint64_t)r.width, (uint64_t)r.height, (uint64_t)r.levels};
+            std::memcpy(facts, f, sizeof f);
+        }
+        return r.ok ? 1 : 0;
     } catch (...) {
         return -1;
     }

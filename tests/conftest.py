@@ -24,3 +24,13 @@ def device():
     d = g.Device(0)
     yield d
     d.close()
+
+
+@pytest.fixture(scope="session")
+def ref():
+    """The reference's own sources compiled into oracle/_ref (skipped where not built)."""
+    from oracle_bind import reference
+    r = reference()
+    if r is None:
+        pytest.skip("compiled reference not built (oracle/_ref)")
+    return r

@@ -330,10 +330,24 @@ class Reference(_Lib):
                                         C.c_void_p, C.c_void_p]
         L.ref_idwt97_2d.restype = None
         L.ref_idwt97_2d.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.ref_check_codestream.restype = C.c_int
+        L.ref_check_codestream.argtypes = [C.c_void_p, C.c_size_t, C.c_double, C.c_char_p, C.c_int, C.c_void_p]
         L.ref_parse_blocks.restype = C.c_int
         L.ref_parse_blocks.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int), C.c_void_p, C.c_int, C.POINTER(Band),
                                        C.POINTER(C.c_int)]
+
+    def check_codestream(self, cs, fps):
+        """The reference's checkDciCodestream (codestream_check.cpp:43-280): (ok, errors, facts),
+        facts a dict of total / comp bytes, tile-parts, rsiz, width, height and levels."""
+        buf = np.frombuffer(cs, np.uint8)
+        msg = C.create_string_buffer(4096)
+        f = np.zeros(9, np.uint64)
+        r = self.lib.ref_check_codestream(buf.ctypes.data, len(buf), fps, msg, 4096, f.ctypes.data)
+        if r < 0:
+            raise RuntimeError("the reference's checker raised an exception")
+        keys = ("total", "comp0", "comp1", "comp2", "tile_parts", "rsiz", "width", "height", "levels")
+        return r == 1, msg.value.decode(), {k: int(v) for k, v in zip(keys, f)}
 
     def idwt(self, planes, levels, skip=0):
         a = np.array(planes, dtype=np.float32, copy=True, order="C")

@@ -44,3 +44,36 @@ def planar_from_synth(x, bits, sx, sy, rgb):
         p = p[::1 << sy, ::1 << sx]
         return (p >> shift).astype(np.uint16)
     return np.concatenate([Y.ravel(), sub(x[0]).ravel(), sub(x[2]).ravel()])
+
+
+def _gather(base, counts):
+    """Indices base[i] + 0..counts[i]-1 for every i, concatenated."""
+    counts = np.asarray(counts, np.int64)
+    total = int(counts.sum())
+    if total == 0:
+        return np.zeros(0, np.int64)
+    starts = np.repeat(np.asarray(base, np.int64) - np.concatenate([[0], np.cumsum(counts)[:-1]]), counts)
+    return starts + np.arange(total, dtype=np.int64)
+
+
+def tier1_mismatch(gpu, ref):
+    """Vectorised form of compare_tier1 for large frames: the number of code-blocks whose
+    bit-planes, pass count, pass lengths, pass distortions or codeword bytes differ."""
+    gi, ri = np.asarray(gpu.info, np.int64), np.asarray(ref["info"], np.int64)
+    if gi.shape != ri.shape:
+        return max(len(gi), len(ri))
+    bad = (gi[:, 0] != ri[:, 0]) | (gi[:, 1] != ri[:, 1]) | (gi[:, 3] != ri[:, 3])
+    ok = np.flatnonzero(~bad)
+    npass = gi[ok, 1]
+    gp, rp = _gather(gi[ok, 4], npass), _gather(ri[ok, 4], npass)
+    owner = np.repeat(np.arange(len(ok)), npass)
+    diff = (np.asarray(gpu.pass_len)[gp] != np.asarray(ref["pass_len"])[rp]) | \
+           (np.asarray(gpu.pass_dist)[gp] != np.asarray(ref["pass_dist"])[rp])
+    bad[ok[np.unique(owner[diff])]] = True
+    ok = np.flatnonzero(~bad)
+    ln = gi[ok, 3]
+    gd, rd = _gather(gi[ok, 2], ln), _gather(ri[ok, 2], ln)
+    owner = np.repeat(np.arange(len(ok)), ln)
+    diff = np.asarray(gpu.data)[gd] != np.asarray(ref["data"])[rd]
+    bad[ok[np.unique(owner[diff])]] = True
+    return int(bad.sum())

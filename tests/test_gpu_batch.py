@@ -250,3 +250,44 @@ def test_autotune_then_encode(device, oracle):
         b.close()
     for i in range(2):
         _check_frame(outs[i], oracle.encode_frame_xyz(frames[i], k, threads=16), f"autotuned frame {i}")
+
+
+def test_full_mixed_batch_64(device, oracle):
+    """The batched workload at its full size: 64 frames, 16 each of 2K flat, 2K scope, 4K scope
+    and 4K full, 950,400 code-blocks in one Tier-1 launch, the DWT chain replayed as a HIP
+    graph with autotuned tile shapes and workgroup sizes, and an early-termination threshold
+    of its own per frame. Every frame equals the oracle's Tier-1 of the same frame."""
+    from t1_compare import tier1_mismatch
+    names = ["2k_flat", "2k_scope", "4k_scope", "4k_full"]
+    thr_choices = [0.0, 150.0, 800.0, 1600.0]
+    layouts, specs = [], []
+    for i in range(64):
+        n = names[i % 4]
+        W, H, k = g.CONTAINERS[n]
+        layouts.append(g.FrameLayout(W, H, k))
+        specs.append((n, 5000 + i, thr_choices[(i // 4) % 4], k))
+    b = g.Batch(device, layouts, [g.xyz_settings(L) for L in layouts], use_graph=True)
+    try:
+        for i, (n, seed, thr, _) in enumerate(specs):
+            W, H, _ = g.CONTAINERS[n]
+            b.upload(i, g.synth_frame(seed, W, H))
+            b.set_slope(i, thr)
+        print("[batch64] autotune", flush=True)
+        chosen = b.autotune()
+        print(f"[batch64] chose {chosen}; encode 64 frames", flush=True)
+        outs, _ = b.encode()
+    finally:
+        b.close()
+    blocks = sum(len(o.info) for o in outs)
+    print(f"[batch64] {blocks} code-blocks in one launch", flush=True)
+    assert blocks == 950400
+    bad_frames = []
+    for i, (n, seed, thr, k) in enumerate(specs):
+        W, H, _ = g.CONTAINERS[n]
+        ref = oracle.encode_frame_xyz(g.synth_frame(seed, W, H), k, thr=thr, threads=16)
+        bad = tier1_mismatch(outs[i], ref)
+        if bad:
+            bad_frames.append((i, n, thr, bad))
+        if i % 8 == 7:
+            print(f"[batch64] frames up to {i} checked, {len(bad_frames)} differ", flush=True)
+    assert not bad_frames, f"frames differing from the oracle (index, size, threshold, blocks): {bad_frames}"

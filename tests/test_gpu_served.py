@@ -166,3 +166,44 @@ def test_served_device_resident_frames(device):
     finally:
         srv.close()
         dev_frame.close()
+
+
+def test_stress_4k_48fps_matches_oracle(device, oracle, ref):
+    """The high-frame-rate stress stream: 4K flat 3996x2160 frames at 48 fps rate control (cap
+    651,041 bytes, budget 651,025), PCRD truncation on every frame, twelve frames in flight in
+    three device jobs with rate control on the GPU. Every codestream passes the reference's own
+    DCI checker at 48 fps within the budget, and equals the oracle's Tier-1 of the same frame
+    (at the frame's early-termination threshold) through the host rate control and Tier-2."""
+    W, H, k = g.CONTAINERS["4k_flat"]
+    L = g.FrameLayout(W, H, k)
+    budget, comp_budget = g.dci_limits(48.0, 250.0, 200.0)
+    assert budget == 651025
+    cfg = g.stream_config("4k_flat", "xyz12_planar", fps=48.0, mbps=250.0)
+    slopes = [0.0, 1800.0, 900.0, 2500.0] * 3
+    frames = [g.synth_frame(7000 + i, W, H) for i in range(len(slopes))]
+    srv = g.Server(slots_per_job=4, max_jobs=3, check=True, keep=True)
+    try:
+        print(f"[stress] {len(frames)} 4K flat frames at 48 fps", flush=True)
+        for i, x in enumerate(frames):
+            srv.submit(i, 48, cfg, [x[0], x[1], x[2]], slopes[i])
+        srv.drain()
+        st = srv.stats()
+        results = {i: srv.results[i] for i in range(len(frames))}
+    finally:
+        srv.close()
+    print("[stress] stats", {k2: st[k2] for k2 in ("frames", "fps", "p50_ms", "p99_ms", "max_frames_in_flight",
+                                                   "jobs")}, flush=True)
+    assert st["max_frames_in_flight"] > 4 and st["jobs"] == 3
+    for i, x in enumerate(frames):
+        r = results[i]
+        assert r["status"] == 0 and r["compliant"] == 1, (i, r)
+        cs = r["codestream"]
+        ok, errors, facts = ref.check_codestream(cs, 48.0)
+        assert ok, (i, errors)
+        assert facts["total"] == len(cs) <= budget
+        assert max(facts["comp0"], facts["comp1"], facts["comp2"]) <= comp_budget
+        t1 = oracle.encode_frame_xyz(x, k, thr=slopes[i], threads=16)
+        want, lam, _ = g.encode_codestream(L, t1, fps=48.0, mbps=250.0, tag="hip")
+        print(f"[stress] frame {i}: threshold {slopes[i]:g}, {len(cs)} bytes (budget {budget}), lambda "
+              f"{r['lambda']:.1f} / oracle path {lam:.1f}, equal {want == cs}", flush=True)
+        assert want == cs, f"frame {i} differs from the oracle path"

@@ -92,3 +92,39 @@ def test_t1_planted_mistake_is_caught(oracle):
     bad[len(bad) // 2] ^= 0x10
     mag, _, _ = oracle.t1_decode(bytes(bad), w, h, orient, r["nbps"], r["num_passes"])
     assert not np.array_equal(mag, (c[:h, :w] & 0x7FFFFFFF) >> 6)
+
+
+def test_tier1_mismatch_counts_planted_differences(oracle):
+    """The vectorised Tier-1 comparison the large GPU tests use finds one block changed in
+    each of its fields, and nothing in an identical frame."""
+    import libgail_amd as g
+    from t1_compare import compare_tier1, tier1_mismatch
+
+    class Res:
+        def __init__(self, d):
+            self.info, self.pass_len, self.pass_dist, self.data = (np.array(d[k]) for k in
+                                                                  ("info", "pass_len", "pass_dist", "data"))
+
+        def block(self, i):
+            nbps, np_, off, ln, pb = (int(v) for v in self.info[i])
+            return dict(nbps=nbps, num_passes=np_, data=bytes(self.data[off:off + ln]),
+                        pass_len=[int(v) for v in self.pass_len[pb:pb + np_]],
+                        pass_dist=[int(v) for v in self.pass_dist[pb:pb + np_]])
+
+    ref = oracle.encode_frame_xyz(g.synth_frame(3, 256, 128), 0, threads=4)
+    assert tier1_mismatch(Res(ref), ref) == 0
+    coded = np.flatnonzero(ref["info"][:, 3] > 2)
+    for field in ("nbps", "pass_len", "pass_dist", "data"):
+        r = Res(ref)
+        i = int(coded[len(coded) // 2])
+        nbps, np_, off, ln, pb = (int(v) for v in r.info[i])
+        if field == "nbps":
+            r.info[i, 0] += 1
+        elif field == "pass_len":
+            r.pass_len[pb + np_ - 1] += 1
+        elif field == "pass_dist":
+            r.pass_dist[pb] -= 1
+        else:
+            r.data[off + ln - 1] ^= 1
+        assert tier1_mismatch(r, ref) == 1, field
+        assert compare_tier1(r, ref)[0] == 1, field
