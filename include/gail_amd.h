@@ -262,9 +262,14 @@ typedef void (*gail_server_cb)(void* user, const gail_server_result* result);
 typedef struct gail_server_config {
     size_t struct_size;
     int device;
-    int slots_per_job;      /* frames per device job (default 8) */
-    int max_jobs;           /* device jobs (default 4); fewer if the budget is short */
-    uint64_t memory_budget; /* device bytes for the jobs; 0 = free memory less a reserve */
+    int slots_per_job;      /* frame slots per device job (default 8), fewer when one job of
+                               a slot class would not fit the memory there is */
+    int max_jobs;           /* most device jobs held at once (default 4). Jobs have 2K or 4K
+                               slots; they are made as frames of their class wait, while the
+                               device has their memory free beyond a headroom of max(512 MiB,
+                               6 %), and retired when idle to make room for the other class
+                               or when the device runs short of memory */
+    uint64_t memory_budget; /* most device bytes the jobs may hold; 0 = no limit of its own */
     int cpu_threads;        /* rate control + Tier-2 + input threads (0 = usable cores - 1) */
     int check;              /* run the DCI codestream check on every frame */
     int batch_window_us;    /* an idle job waits up to this long for more frames (0: launch at once) */
@@ -281,9 +286,14 @@ typedef struct gail_server_stats {
     double p50_ms, p99_ms, max_ms;          /* latency */
     double queued_p50_ms, queued_p99_ms;
     double mean_batch;    /* frames per device job */
-    int jobs, slots_per_job, max_frames_in_flight;
-    uint64_t job_bytes;   /* device memory of one job */
-    uint64_t slot_bytes;  /* device memory per frame slot (one 4K frame) */
+    int jobs, slots_per_job, max_frames_in_flight;  /* jobs held now; 4K slots per job */
+    uint64_t job_bytes;   /* device memory the jobs hold now */
+    uint64_t slot_bytes;  /* device memory of one 4K frame slot */
+    int jobs_2k, jobs_4k; /* jobs held now, by slot class */
+    int jobs_made, jobs_retired, max_jobs;  /* jobs made, retired, most held at once */
+    int slots_per_job_2k;
+    uint64_t peak_job_bytes;  /* most device memory the jobs held at once */
+    uint64_t slot_bytes_2k;   /* device memory of one 2K frame slot */
 } gail_server_stats;
 
 GAIL_API void gail_server_config_init(gail_server_config* c, size_t struct_size);

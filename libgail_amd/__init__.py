@@ -7,11 +7,14 @@ never falls back to anything else.
 """
 import ctypes as C
 import os
+import threading
+import weakref
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_DIR = os.path.join(HERE, "lib")
+# libgail.so and the plugin beside it; GAIL_AMD_LIB_DIR names another build of the pair
+LIB_DIR = os.environ.get("GAIL_AMD_LIB_DIR") or os.path.join(HERE, "lib")
 PLUGIN_PATH = os.path.join(LIB_DIR, "libgail_hip.so")
 HOST_PATH = os.path.join(LIB_DIR, "libgail.so")
 ABI = 2
@@ -748,7 +751,9 @@ class ServerStats(C.Structure):
                 ("fps", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
                 ("queued_p50_ms", C.c_double), ("queued_p99_ms", C.c_double), ("mean_batch", C.c_double),
                 ("jobs", C.c_int), ("slots_per_job", C.c_int), ("max_frames_in_flight", C.c_int),
-                ("job_bytes", C.c_uint64), ("slot_bytes", C.c_uint64)]
+                ("job_bytes", C.c_uint64), ("slot_bytes", C.c_uint64), ("jobs_2k", C.c_int), ("jobs_4k", C.c_int),
+                ("jobs_made", C.c_int), ("jobs_retired", C.c_int), ("max_jobs", C.c_int),
+                ("slots_per_job_2k", C.c_int), ("peak_job_bytes", C.c_uint64), ("slot_bytes_2k", C.c_uint64)]
 
 
 PIX = {"rgb48": 0, "xyz12": 1, "xyz12_msb": 2, "yuv420p": 3, "yuv420p10": 4, "yuv420p12": 5, "yuv422p": 6,
@@ -827,7 +832,8 @@ def pinned_empty(nbytes):
 
 class DeviceFrame:
     """A frame's planes uploaded, back to back, into device memory (gail_device_alloc), for
-    submitting frames that are already resident in HBM."""
+    submitting frames that are already resident in HBM. The memory is freed by close() or
+    when the object is collected."""
 
     def __init__(self, planes):
         h = _lib_api()
@@ -836,6 +842,7 @@ class DeviceFrame:
         self.ptr = h.gail_device_alloc(total)
         if not self.ptr:
             raise ServerError(f"device allocation failed: {h.gail_last_error().decode()}")
+        self._free = weakref.finalize(self, h.gail_device_free, self.ptr)
         self.ptrs, self.strides, off = [], [], 0
         for a in arrs:
             if h.gail_copy(self.ptr + off, a.ctypes.data, a.nbytes) != 0:
@@ -846,37 +853,22 @@ class DeviceFrame:
         self.nbytes = total
 
     def close(self):
-        if self.ptr:
-            _lib_api().gail_device_free(self.ptr)
-            self.ptr = None
+        self._free()
+        self.ptr = None
 
 
-class Server:
-    """The served form (gail_server_*): frames of many streams admitted into device jobs as
-    slots free up. Results land in self.results[id] as dicts; with keep=True each also holds
-    the codestream bytes."""
+class _ServerSink:
+    """The part of a Server its result callback uses. It is kept apart from the Server object:
+    the C++ pool threads call the callback thunk until gail_server_destroy returns, so the
+    thunk, this sink and the server handle are owned by a finaliser that destroys the server
+    first, whether close() runs or the Server is collected without it."""
 
-    def __init__(self, device=0, slots_per_job=8, max_jobs=4, memory_budget=0, cpu_threads=0, check=False,
-                 keep=False, batch_window_us=0, on_done=None, host_rate_control=False):
-        h = _lib_api()
-        self.keep = keep
-        self.on_done = on_done  # called with (id, result dict) after each result is stored
-        self.results = {}
-        self._frames = {}
-        self._lock = __import__("threading").Lock()
-        cfg = ServerConfig()
-        h.gail_server_config_init(C.byref(cfg), C.sizeof(cfg))
-        cfg.device, cfg.slots_per_job, cfg.max_jobs = device, slots_per_job, max_jobs
-        cfg.memory_budget, cfg.cpu_threads, cfg.check = memory_budget, cpu_threads, int(check)
-        cfg.batch_window_us = int(batch_window_us)
-        cfg.host_rate_control = int(host_rate_control)
-        self._cb = SERVER_CB(self._on_result)
-        cfg.on_result = self._cb
-        self.handle = _vp()
-        if h.gail_server_create(C.byref(cfg), C.byref(self.handle)) != 0:
-            raise ServerError(h.gail_last_error().decode())
+    def __init__(self, keep, on_done):
+        self.keep, self.on_done = keep, on_done
+        self.results, self.frames = {}, {}
+        self.lock = threading.Lock()
 
-    def _on_result(self, _user, rp):
+    def on_result(self, _user, rp):
         r = rp.contents
         d = {"status": r.status, "latency_ms": r.latency_ms, "queued_ms": r.queued_ms, "batch": r.batch_frames,
              "recoded": r.recoded, "lambda": r.lambda_, "stream": r.stream}
@@ -885,11 +877,45 @@ class Server:
             d["size"], d["compliant"], d["psnr"] = p.size, p.compliant, p.psnr_estimate
             if self.keep:
                 d["codestream"] = C.string_at(p.data, p.size)
-        with self._lock:
+        with self.lock:
             self.results[r.id] = d
-            self._frames.pop(r.id, None)
+            self.frames.pop(r.id, None)
         if self.on_done:
             self.on_done(r.id, d)
+
+
+def _destroy_server(handle, thunk, sink):
+    """Waits for every frame and every pool thread of the server (gail_server_destroy), and
+    only then lets the callback thunk, the sink and the frames it keeps alive go."""
+    if handle.value:
+        _lib_api().gail_server_destroy(handle)
+        handle.value = None
+    del thunk, sink
+
+
+class Server:
+    """The served form (gail_server_*): frames of many streams admitted into device jobs as
+    slots free up. Results land in self.results[id] as dicts; with keep=True each also holds
+    the codestream bytes. close() waits for every frame; a Server collected without it is
+    closed the same way."""
+
+    def __init__(self, device=0, slots_per_job=8, max_jobs=4, memory_budget=0, cpu_threads=0, check=False,
+                 keep=False, batch_window_us=0, on_done=None, host_rate_control=False):
+        h = _lib_api()
+        self._sink = _ServerSink(keep, on_done)  # on_done: called with (id, result dict)
+        self.results = self._sink.results
+        cfg = ServerConfig()
+        h.gail_server_config_init(C.byref(cfg), C.sizeof(cfg))
+        cfg.device, cfg.slots_per_job, cfg.max_jobs = device, slots_per_job, max_jobs
+        cfg.memory_budget, cfg.cpu_threads, cfg.check = memory_budget, cpu_threads, int(check)
+        cfg.batch_window_us = int(batch_window_us)
+        cfg.host_rate_control = int(host_rate_control)
+        thunk = SERVER_CB(self._sink.on_result)
+        cfg.on_result = thunk
+        self.handle = _vp()
+        if h.gail_server_create(C.byref(cfg), C.byref(self.handle)) != 0:
+            raise ServerError(h.gail_last_error().decode())
+        self._close = weakref.finalize(self, _destroy_server, self.handle, thunk, self._sink)
 
     def submit(self, fid, stream, cfg, planes, slope=-1.0):
         """planes: list of three numpy arrays (planar formats) or one array; kept alive until
@@ -904,12 +930,14 @@ class Server:
             for i, a in enumerate(arrs):
                 f.data[i] = a.ctypes.data
                 f.stride[i] = a.strides[0] if a.ndim > 1 else 0
-        with self._lock:
-            self._frames[fid] = (arrs, cfg)
+        if not self.handle:
+            raise ServerError("submit: the server is closed")
+        with self._sink.lock:  # the planes live until the frame's result arrives
+            self._sink.frames[fid] = (arrs, cfg)
         st = _lib_api().gail_server_submit(self.handle, fid, stream, C.byref(cfg), C.byref(f), slope)
         if st != 0:
-            with self._lock:
-                self._frames.pop(fid, None)
+            with self._sink.lock:
+                self._sink.frames.pop(fid, None)
             raise ServerError(f"submit: {_lib_api().gail_last_error().decode()}")
 
     def drain(self):
@@ -921,6 +949,4 @@ class Server:
         return {k: getattr(s, k) for k, _ in ServerStats._fields_}
 
     def close(self):
-        if self.handle:
-            _lib_api().gail_server_destroy(self.handle)
-            self.handle = None
+        self._close()

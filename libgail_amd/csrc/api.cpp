@@ -434,15 +434,11 @@ GAIL_API int gail_encoder_create(const gail_config* c, gail_encoder** encoder) {
             throw ApiError(GAIL_ERROR_FAILED, e->P->last_error());
         e->devices = e->P->device_name(e->dev);
         // frames in flight: as asked (default 2), fewer when device memory is short
-        int want = cfg.frames_in_flight > 0 ? cfg.frames_in_flight : 2;
         const uint64_t per = e->P->encoder_bytes(e->dev, &e->spec->plainLayout, &e->spec->settings);
         GailHipMemory mem{};
-        if (per && e->P->device_memory(e->dev, &mem) == GAIL_HIP_OK && mem.budgeted) {
-            const uint64_t headroom = std::max<uint64_t>(512ull << 20, mem.total / 16);
-            uint64_t avail = mem.available > headroom ? mem.available - headroom : 0;
-            if (cfg.max_device_memory) avail = std::min<uint64_t>(avail, cfg.max_device_memory);
-            want = (int)std::max<uint64_t>(1, std::min<uint64_t>(want, avail / (per + per / 10)));
-        }
+        const bool known = e->P->device_memory(e->dev, &mem) == GAIL_HIP_OK && mem.budgeted;
+        const int want = fitFrames(known ? mem.available : 0, mem.total, per,
+                                   cfg.frames_in_flight > 0 ? cfg.frames_in_flight : 2, cfg.max_device_memory);
         for (int i = 0; i < want; ++i) {
             GailHipEncoder* slot = nullptr;
             const int st = e->P->encoder_create(e->dev, &e->spec->plainLayout, &e->spec->settings, &slot);

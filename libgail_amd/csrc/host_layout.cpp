@@ -9,6 +9,7 @@
 #include "gail_amd_export.h"
 #include "gail_hip_plugin.h"
 #include "layout.h"
+#include "stream.h"
 
 using namespace gail_amd;
 
@@ -80,4 +81,11 @@ extern "C" GAIL_AMD_EXPORT int64_t gail_read_cube_lut(const char* path, float* o
         }
         return -1;
     }
+}
+
+// The frames-in-flight rule shared by the encoder and the server (memoryHeadroom, fitFrames).
+extern "C" GAIL_AMD_EXPORT uint64_t gail_memory_headroom(uint64_t total) { return memoryHeadroom(total); }
+extern "C" GAIL_AMD_EXPORT int gail_fit_frames(uint64_t available, uint64_t total, uint64_t perFrame, int requested,
+                                               uint64_t limit) {
+    return fitFrames(available, total, perFrame, requested, limit);
 }

@@ -1,16 +1,20 @@
 // Served form of libgail.so (gail_server_*, include/gail_amd.h): a request loop on one device
 // for frames of many streams with mixed profiles and input formats.
 //
-// Device memory is held by a few jobs of the plugin (batches of frame slots, each slot room
-// for one 4K frame of any input); their number comes from the memory budget. A scheduler
-// thread admits waiting frames (oldest first) into whichever job is idle, as many as it has
-// slots, binds them, queues their inputs (DMA straight from pinned app memory when the frame
-// already has the device layout, else a host copy on the pool) and launches the job; jobs run
-// concurrently on their own streams. A waiter thread per job collects its Tier-1 results and
-// hands each frame to the host pool for rate control, Tier-2 and delivery; the job is idle
-// again once all of its frames are done. A frame whose lambda falls below its early-
-// termination threshold is queued again, first in line, to be coded without it
-// (pipeline.cpp:194-218).
+// Device memory is held by jobs of the plugin: batches of frame slots of one class, 2K
+// (2048 x 1080) or 4K (4096 x 2160), each slot room for one frame of its class in any input
+// format. A scheduler thread admits waiting frames, the oldest first, into an idle job of
+// their class (a 4K job also takes 2K frames), binds them, queues their inputs (DMA straight
+// from pinned app memory when the frame already has the device layout, else a host copy on
+// the pool) and launches the job; jobs run concurrently on their own streams. Jobs are made
+// when frames wait and none of their class is idle, as far as the server's memory budget and
+// the memory the device has free at that moment (less the headroom of pipeline.cpp:19,
+// max(512 MiB, 6 %)) allow, up to max_jobs; idle jobs are retired to make room for the other
+// class, and one at a time when the device runs short of memory (pipeline.cpp:105-138). A
+// waiter thread per job collects its Tier-1 results and hands each frame to the host pool for
+// rate control, Tier-2 and delivery; the job is idle again once all of its frames are done.
+// A frame whose lambda falls below its early-termination threshold is queued again, first in
+// line, to be coded without it (pipeline.cpp:194-218).
 #include <sched.h>
 
 #include <algorithm>
@@ -109,13 +113,22 @@ struct Request {
     int batchFrames = 0;
 };
 
+// Slot classes: a job's slots all take frames up to one of these sizes.
+constexpr int kClasses = 2;
+const GailHipSlot kClassSlot[kClasses] = {{2048, 1080, 1, 0}, {4096, 2160, 1, 0}};
+const char* const kClassName[kClasses] = {"2K", "4K"};
+
 struct JobState {
+    int cls = 0;
+    uint64_t bytes = 0;  // device memory the job holds
     GailHipBatch* batch = nullptr;
     std::vector<std::unique_ptr<Request>> reqs;
     std::vector<GailHipTier1> t1;
     int rcLeft = 0;
-    bool busy = false;
+    bool busy = false;      // frames taken by the scheduler and not all delivered
     bool launched = false;  // waiter: a run is on the device
+    bool retire = false;    // waiter: leave
+    Clock::time_point lastUsed;
     std::thread waiter;
 };
 
@@ -126,14 +139,17 @@ struct gail_server {
     const GailHipPlugin* P = nullptr;
     GailHipDevice* dev = nullptr;
     std::unique_ptr<Pool> pool;
-    std::vector<std::unique_ptr<JobState>> jobs;
-    int slotsPerJob = 0;
-    uint64_t jobBytes = 0, slotBytes = 0;
+    int slotsPerJob[kClasses] = {0, 0};
+    uint64_t jobBytes[kClasses] = {0, 0}, slotBytes[kClasses] = {0, 0};
 
     std::mutex m;  // everything below
     std::condition_variable cv;
     std::deque<std::unique_ptr<Request>> pending;
     std::map<uint32_t, std::unique_ptr<StreamState>> streams;
+    std::vector<std::unique_ptr<JobState>> jobs;
+    uint64_t heldBytes = 0, peakBytes = 0;
+    int jobsMade = 0, jobsRetired = 0, peakJobs = 0;
+    Clock::time_point nextPressureCheck;
     bool stop = false;
     uint64_t submitted = 0, delivered = 0, failedCount = 0, recodedCount = 0, jobRuns = 0, jobFrames = 0;
     int inFlight = 0, maxInFlight = 0;
@@ -228,7 +244,7 @@ struct gail_server {
         for (;;) {
             {
                 std::unique_lock<std::mutex> l(m);
-                cv.wait(l, [&] { return j->launched || (stop && !j->busy); });
+                cv.wait(l, [&] { return j->launched || j->retire; });
                 if (!j->launched) return;
                 j->launched = false;
             }
@@ -329,40 +345,214 @@ struct gail_server {
         cv.notify_all();
     }
 
-    void schedulerLoop() {
+    static int classOf(const Request& r) { return r.st->spec->fourk() ? 1 : 0; }
+
+    // With m held: whether a new job of class k fits the server's budget and the memory the
+    // device has free now, less the headroom for other users of the GPU.
+    bool roomFor(int k) {
+        if ((int)jobs.size() >= cfg.max_jobs) return false;
+        if (cfg.memory_budget && heldBytes + jobBytes[k] > cfg.memory_budget) return false;
+        GailHipMemory mem{};
+        if (P->device_memory(dev, &mem) == GAIL_HIP_OK && mem.budgeted &&
+            mem.available < jobBytes[k] + memoryHeadroom(mem.total))
+            return false;
+        return true;
+    }
+
+    // Whether a job of class k could ever be held: within the budget and, with nothing else
+    // held, within the device's memory less the headroom.
+    bool everFits(int k) {
+        if (cfg.memory_budget && jobBytes[k] > cfg.memory_budget) return false;
+        GailHipMemory mem{};
+        if (P->device_memory(dev, &mem) == GAIL_HIP_OK && mem.budgeted && mem.total < jobBytes[k] + memoryHeadroom(mem.total))
+            return false;
+        return true;
+    }
+
+    // Makes a job of class k (m held on entry and exit, released while the plugin allocates).
+    JobState* makeJob(std::unique_lock<std::mutex>& l, int k) {
+        std::vector<GailHipSlot> slots(slotsPerJob[k], kClassSlot[k]);
+        GailHipBatch* b = nullptr;
+        l.unlock();
+        const int st = P->slots_create(dev, slots.data(), slotsPerJob[k], nullptr, &b);
+        l.lock();
+        if (st != GAIL_HIP_OK) return nullptr;
+        auto j = std::make_unique<JobState>();
+        j->cls = k;
+        j->bytes = jobBytes[k];
+        j->batch = b;
+        j->lastUsed = Clock::now();
+        JobState* jp = j.get();
+        j->waiter = std::thread([this, jp] { waiterLoop(jp); });
+        jobs.push_back(std::move(j));
+        heldBytes += jobBytes[k];
+        peakBytes = std::max(peakBytes, heldBytes);
+        peakJobs = std::max(peakJobs, (int)jobs.size());
+        ++jobsMade;
+        return jp;
+    }
+
+    // Frees idle job j (m held on entry and exit, released while its waiter leaves and the
+    // plugin frees its memory).
+    void retireJob(std::unique_lock<std::mutex>& l, JobState* j) {
+        auto it = std::find_if(jobs.begin(), jobs.end(), [&](const std::unique_ptr<JobState>& p) { return p.get() == j; });
+        std::unique_ptr<JobState> owned = std::move(*it);
+        jobs.erase(it);
+        heldBytes -= owned->bytes;
+        ++jobsRetired;
+        owned->retire = true;
+        cv.notify_all();
+        l.unlock();
+        owned->waiter.join();
+        P->batch_destroy(owned->batch);
+        owned.reset();
+        l.lock();
+    }
+
+    JobState* idleJob(int k) {
+        for (auto& j : jobs)
+            if (!j->busy && j->cls == k) return j.get();
+        return nullptr;
+    }
+
+    // The least recently used idle job not of class `keep` (-1: any class).
+    JobState* idleToRetire(int keep) {
+        JobState* best = nullptr;
+        for (auto& j : jobs)
+            if (!j->busy && j->cls != keep && (!best || j->lastUsed < best->lastUsed)) best = j.get();
+        return best;
+    }
+
+    // Fails the waiting frames of class k (m held on entry and exit).
+    void failClass(std::unique_lock<std::mutex>& l, int k, const std::string& why) {
+        std::vector<std::unique_ptr<Request>> gone;
+        for (auto it = pending.begin(); it != pending.end();) {
+            if (classOf(**it) == k) {
+                gone.push_back(std::move(*it));
+                it = pending.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        l.unlock();
+        for (auto& r : gone) {
+            r->tAdmit = Clock::now();
+            lastErrorText() = why;
+            deliver(*r, GAIL_ERROR_NOMEM, nullptr, 0);
+        }
+        l.lock();
+    }
+
+    // A job for the waiting frames, reserved (busy), or nullptr when every usable job is busy
+    // and no other can be made now. Classes are served in the order of their oldest frame.
+    JobState* chooseJob(std::unique_lock<std::mutex>& l) {
         for (;;) {
-            JobState* j = nullptr;
-            {
-                std::unique_lock<std::mutex> l(m);
-                auto idle = [&]() -> JobState* {
-                    for (auto& p : jobs)
-                        if (!p->busy) return p.get();
-                    return nullptr;
-                };
-                cv.wait(l, [&] { return (stop && pending.empty()) || (!pending.empty() && idle()); });
-                if (pending.empty()) return;
-                // batching window: a part-full job waits for more frames, up to the window
-                // counted from the oldest waiting frame's arrival
-                if (cfg.batch_window_us > 0 && !stop) {
-                    const Clock::time_point until = pending.front()->tSubmit + std::chrono::microseconds(cfg.batch_window_us);
-                    cv.wait_until(l, until, [&] { return stop || (int)pending.size() >= slotsPerJob; });
-                    if (pending.empty()) continue;
+            int order[kClasses], n = 0;
+            for (const auto& r : pending) {
+                const int k = classOf(*r);
+                if (std::find(order, order + n, k) == order + n) order[n++] = k;
+                if (n == kClasses) break;
+            }
+            bool again = false;
+            for (int o = 0; o < n && !again; ++o) {
+                const int k = order[o];
+                if (!everFits(k) && !(k == 0 && everFits(1))) {
+                    failClass(l, k, std::string("a job of ") + kClassName[k] + " frame slots exceeds the memory budget");
+                    again = true;
+                    continue;
                 }
-                j = idle();
-                if (!j) continue;
-                j->busy = true;
-                const Clock::time_point now = Clock::now();
-                const int take = std::min<int>(slotsPerJob, (int)pending.size());
-                for (int i = 0; i < take; ++i) {
-                    auto r = std::move(pending.front());
-                    pending.pop_front();
-                    r->tAdmit = now;
-                    r->batchFrames = take;
-                    if (r->autoSlope && !r->recoded) r->slope = r->st->spec->slopeFactor * r->st->lastLambda.load();
-                    j->reqs.push_back(std::move(r));
+                JobState* j = idleJob(k);
+                if (!j && roomFor(k)) j = makeJob(l, k);
+                if (!j && k == 0) j = idleJob(1);  // a 4K job's slots take 2K frames
+                if (j) {
+                    j->busy = true;
+                    return j;
+                }
+                // make room: the least recently used idle job of another class goes
+                if (JobState* r = idleToRetire(k)) {
+                    retireJob(l, r);
+                    again = true;
+                } else if (jobs.empty()) {
+                    failClass(l, k, std::string("no device memory for a job of ") + kClassName[k] + " frame slots");
+                    again = true;
                 }
             }
+            if (!again) return nullptr;
+            if (pending.empty()) return nullptr;
+        }
+    }
+
+    // Under memory pressure from other users of the device (free memory below half the
+    // headroom) an idle job is given up, one at a time, the last one kept (pipeline.cpp:121-138).
+    void relievePressure(std::unique_lock<std::mutex>& l) {
+        const Clock::time_point now = Clock::now();
+        if (now < nextPressureCheck || jobs.size() <= 1) return;
+        nextPressureCheck = now + std::chrono::milliseconds(500);
+        GailHipMemory mem{};
+        if (P->device_memory(dev, &mem) != GAIL_HIP_OK || !mem.budgeted || mem.available >= memoryHeadroom(mem.total) / 2)
+            return;
+        if (JobState* j = idleToRetire(-1)) {
+            retireJob(l, j);
+            nextPressureCheck = Clock::now() + std::chrono::milliseconds(1000);  // let the freed memory show
+        }
+    }
+
+    void schedulerLoop() {
+        std::unique_lock<std::mutex> l(m);
+        for (;;) {
+            if (stop && pending.empty()) return;
+            if (pending.empty()) {
+                cv.wait(l);
+                continue;
+            }
+            relievePressure(l);
+            JobState* j = chooseJob(l);
+            if (!j) {
+                // every usable job busy and none can be made: a job going idle, a new frame or
+                // memory coming free (checked again at least every 100 ms) wakes this
+                cv.wait_for(l, std::chrono::milliseconds(100));
+                continue;
+            }
+            const int cap = slotsPerJob[j->cls];
+            auto fits = [&](const Request& r) { return classOf(r) <= j->cls; };
+            auto waiting = [&] {
+                int c = 0;
+                for (const auto& r : pending) c += fits(*r) ? 1 : 0;
+                return c;
+            };
+            // batching window: a part-full job waits for more frames, up to the window counted
+            // from the arrival of the oldest frame it can take
+            if (cfg.batch_window_us > 0 && !stop && waiting() < cap) {
+                Clock::time_point oldest = Clock::now();
+                for (const auto& r : pending)
+                    if (fits(*r)) {
+                        oldest = r->tSubmit;
+                        break;
+                    }
+                cv.wait_until(l, oldest + std::chrono::microseconds(cfg.batch_window_us),
+                              [&] { return stop || waiting() >= cap; });
+            }
+            const Clock::time_point now = Clock::now();
+            for (auto it = pending.begin(); it != pending.end() && (int)j->reqs.size() < cap;) {
+                if (!fits(**it)) {
+                    ++it;
+                    continue;
+                }
+                std::unique_ptr<Request> r = std::move(*it);
+                it = pending.erase(it);
+                r->tAdmit = now;
+                if (r->autoSlope && !r->recoded) r->slope = r->st->spec->slopeFactor * r->st->lastLambda.load();
+                j->reqs.push_back(std::move(r));
+            }
+            if (j->reqs.empty()) {  // the frames it was reserved for went to another job meanwhile
+                j->busy = false;
+                continue;
+            }
+            for (auto& r : j->reqs) r->batchFrames = (int)j->reqs.size();
+            j->lastUsed = now;
+            l.unlock();
             admit(*j);
+            l.lock();
         }
     }
 
@@ -373,14 +563,17 @@ struct gail_server {
         }
         cv.notify_all();
         if (scheduler.joinable()) scheduler.join();
-        for (auto& j : jobs) {
-            {
-                std::unique_lock<std::mutex> l(m);
-                cv.wait(l, [&] { return !j->busy; });
-            }
-            cv.notify_all();
+        std::unique_lock<std::mutex> l(m);
+        cv.wait(l, [&] {
+            for (auto& j : jobs)
+                if (j->busy) return false;
+            return true;
+        });
+        for (auto& j : jobs) j->retire = true;
+        cv.notify_all();
+        l.unlock();
+        for (auto& j : jobs)
             if (j->waiter.joinable()) j->waiter.join();
-        }
         pool.reset();
         for (auto& j : jobs)
             if (j->batch) P->batch_destroy(j->batch);
@@ -393,12 +586,13 @@ struct gail_server {
 extern "C" {
 
 GAIL_API void gail_server_config_init(gail_server_config* c, size_t size) {
-    if (!c) return;
+    if (!c || size < sizeof(size_t)) return;
     gail_server_config d{};
-    d.struct_size = sizeof d;
+    d.struct_size = size;
     d.slots_per_job = 8;
     d.max_jobs = 4;
-    std::memcpy(c, &d, std::min(size ? size : sizeof d, sizeof d));
+    std::memset(c, 0, size);
+    std::memcpy(c, &d, std::min(size, sizeof d));
 }
 
 GAIL_API int gail_server_create(const gail_server_config* c, gail_server** out) {
@@ -420,39 +614,26 @@ GAIL_API int gail_server_create(const gail_server_config* c, gail_server** out) 
     if (cfg.device < 0 || cfg.device >= n) return sfail(GAIL_ERROR_INVALID, "no such device");
     if (P->device_usable(cfg.device) != GAIL_HIP_OK) return sfail(GAIL_ERROR_UNSUPPORTED, P->last_error());
     if (P->device_create(cfg.device, nullptr, nullptr, &s->dev) != GAIL_HIP_OK) return sfail(GAIL_ERROR_FAILED, P->last_error());
-    // every slot takes any DCI frame (up to 4096 x 2160) and any input format
-    const GailHipSlot one{4096, 2160, 1, 0};
-    s->slotBytes = P->slots_bytes(s->dev, &one, 1, nullptr);
+    // jobs of each slot class: as many slots as asked, fewer when one job would not fit the
+    // memory budget (the server's own, or what the device has free now less the headroom)
     uint64_t budget = cfg.memory_budget;
     GailHipMemory mem{};
-    if (P->device_memory(s->dev, &mem) == GAIL_HIP_OK) {
-        const uint64_t reserve = std::max<uint64_t>(2ull << 30, mem.total / 16);
-        const uint64_t avail = mem.available > reserve ? mem.available - reserve : 0;
-        budget = budget ? std::min(budget, avail) : avail;
+    if (P->device_memory(s->dev, &mem) == GAIL_HIP_OK && mem.budgeted) {
+        const uint64_t head = memoryHeadroom(mem.total), usable = mem.available > head ? mem.available - head : 0;
+        budget = budget ? std::min(budget, usable) : usable;
     }
-    s->slotsPerJob = cfg.slots_per_job;
-    std::vector<GailHipSlot> slots(s->slotsPerJob, one);
-    s->jobBytes = P->slots_bytes(s->dev, slots.data(), s->slotsPerJob, nullptr);
-    while (s->slotsPerJob > 1 && s->jobBytes > budget) {
-        --s->slotsPerJob;
-        s->jobBytes = P->slots_bytes(s->dev, slots.data(), s->slotsPerJob, nullptr);
+    for (int k = 0; k < kClasses; ++k) {
+        s->slotBytes[k] = P->slots_bytes(s->dev, &kClassSlot[k], 1, nullptr);
+        int n = cfg.slots_per_job;
+        std::vector<GailHipSlot> slots(n, kClassSlot[k]);
+        uint64_t bytes = P->slots_bytes(s->dev, slots.data(), n, nullptr);
+        while (n > 1 && budget && bytes > budget) bytes = P->slots_bytes(s->dev, slots.data(), --n, nullptr);
+        s->slotsPerJob[k] = n;
+        s->jobBytes[k] = bytes;
     }
-    const int njobs = (int)std::min<uint64_t>(cfg.max_jobs, s->jobBytes ? budget / s->jobBytes : 0);
-    if (njobs < 1) {
+    if (budget && s->jobBytes[0] > budget) {
         s->shutdown();
         return sfail(GAIL_ERROR_NOMEM, "the device memory budget holds no frame slot");
-    }
-    for (int i = 0; i < njobs; ++i) {
-        auto j = std::make_unique<JobState>();
-        if (P->slots_create(s->dev, slots.data(), s->slotsPerJob, nullptr, &j->batch) != GAIL_HIP_OK) {
-            if (s->jobs.empty()) {
-                const std::string why = P->last_error();
-                s->shutdown();
-                return sfail(GAIL_ERROR_NOMEM, why);
-            }
-            break;
-        }
-        s->jobs.push_back(std::move(j));
     }
     int threads = cfg.cpu_threads;
     if (threads <= 0) {  // the cores this process may run on (usableCpuCount, cpu_backend.cpp:180-192)
@@ -463,7 +644,6 @@ GAIL_API int gail_server_create(const gail_server_config* c, gail_server** out) 
         threads = std::max(2, cores - 1);
     }
     s->pool = std::make_unique<Pool>(threads);
-    for (auto& j : s->jobs) j->waiter = std::thread([sp = s.get(), jp = j.get()] { sp->waiterLoop(jp); });
     s->scheduler = std::thread([sp = s.get()] { sp->schedulerLoop(); });
     *out = s.release();
     return GAIL_OK;
@@ -553,10 +733,17 @@ GAIL_API int gail_server_get_stats(gail_server* s, gail_server_stats* out) {
     r.queued_p99_ms = pct(s->queued, 0.99);
     r.mean_batch = s->jobRuns ? (double)s->jobFrames / (double)s->jobRuns : 0;
     r.jobs = (int)s->jobs.size();
-    r.slots_per_job = s->slotsPerJob;
+    r.slots_per_job = s->slotsPerJob[1];
     r.max_frames_in_flight = s->maxInFlight;
-    r.job_bytes = s->jobBytes;
-    r.slot_bytes = s->slotBytes;
+    r.job_bytes = s->heldBytes;
+    r.slot_bytes = s->slotBytes[1];
+    for (auto& j : s->jobs) (j->cls ? r.jobs_4k : r.jobs_2k)++;
+    r.jobs_made = s->jobsMade;
+    r.jobs_retired = s->jobsRetired;
+    r.max_jobs = s->peakJobs;
+    r.slots_per_job_2k = s->slotsPerJob[0];
+    r.peak_job_bytes = s->peakBytes;
+    r.slot_bytes_2k = s->slotBytes[0];
     *out = r;
     return GAIL_OK;
 }

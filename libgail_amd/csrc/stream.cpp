@@ -58,6 +58,23 @@ bool allZero(const uint64_t* p, int n) {
     return true;
 }
 
+uint64_t memoryHeadroom(uint64_t total) {
+    const uint64_t floor = 512ull << 20, share = total / 100 * 6 + total % 100 * 6 / 100;
+    return share > floor ? share : floor;
+}
+
+int fitFrames(uint64_t available, uint64_t total, uint64_t perFrame, int requested, uint64_t limit) {
+    const int want = requested > 1 ? requested : 1;
+    if (!perFrame || (!available && !limit)) return want;
+    uint64_t room = limit;
+    if (available) {
+        const uint64_t head = memoryHeadroom(total), usable = available > head ? available - head : 0;
+        room = limit && limit < usable ? limit : usable;
+    }
+    const uint64_t fit = room / (perFrame + perFrame / 10);
+    return fit < 1 ? 1 : (fit < (uint64_t)want ? (int)fit : want);
+}
+
 bool planarInfo(gail_pixel_format f, PlanarInfo& p) {
     // the planar formats of gail_pixel_format, in enumeration order from GAIL_PIX_YUV420P:
     // three Y'CbCr depths each at 4:2:0, 4:2:2 and 4:4:4, then G'B'R' at 8, 10, 12 and 16 bits

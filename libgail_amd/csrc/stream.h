@@ -32,6 +32,13 @@ const PluginHandle& loadPlugin();
 
 bool allZero(const uint64_t* p, int n);
 
+// Device memory left to other users of the GPU: max(512 MiB, 6 % of it) (pipeline.cpp:19).
+uint64_t memoryHeadroom(uint64_t total);
+// Frames in flight that fit: as many as requested (at least one), fewer when `available`
+// less the headroom, or `limit` (0: none), holds fewer of perFrame bytes plus a tenth of
+// slack; available 0 means unknown (pipeline.cpp:21-34).
+int fitFrames(uint64_t available, uint64_t total, uint64_t perFrame, int requested, uint64_t limit);
+
 struct PlanarInfo {
     int bits, sx, sy, rgb;
 };

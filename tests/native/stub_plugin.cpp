@@ -16,12 +16,19 @@
 //   6  a working table whose runtime finds no driver (device_count() < 0)
 //
 // A working stand-in also reads, at each call, what the test asks of it:
-//   GAIL_STUB_FAIL_AT=n    the n-th encode on a device (1-based) and every later one fail as
-//                          a lost device would (GAIL_HIP_ERROR)
+//   GAIL_STUB_FAIL_AT=n    the n-th frame coded on a device (1-based, by an encoder or a
+//                          batch) and every later one fail as a lost device would
 //   GAIL_STUB_SLOTS=k      encoder_create reports device memory exhausted after k encoders
-//   GAIL_STUB_DELAY_MS=d   every encode takes d ms longer
+//   GAIL_STUB_DELAY_MS=d   every encode, and every batch run, takes d ms longer
+//   GAIL_STUB_MEM_MB=m     the device has m MiB (default 288 GiB), of which frame slots take
+//                          what slots_bytes says; slots_create fails beyond it
+//   GAIL_STUB_MEM_USED_MB=u  another process holds u MiB of it
+// Each is read at every call; gail_stub_control() sets one from the test program.
+// Batches (the served form's jobs) code on the CPU like the encoder: one frame slot holds
+// any frame up to its size; slots_bytes charges 40 bytes per pixel of a slot.
 #include <stdlib.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cstdio>
@@ -41,8 +48,26 @@ namespace {
 thread_local std::string g_error = "no error";
 
 long envLong(const char* name, long dflt) {
+    // a value set through gail_stub_control() wins over the environment
+    extern long controlValue(const char* name, long dflt);
     const char* v = std::getenv(name);
-    return v && *v ? std::strtol(v, nullptr, 10) : dflt;
+    return controlValue(name, v && *v ? std::strtol(v, nullptr, 10) : dflt);
+}
+
+// Knobs a test program sets while the plugin runs (getenv is not safe beside setenv).
+struct Knob {
+    const char* name;
+    std::atomic<long> value;
+    std::atomic<bool> set;
+};
+Knob g_knobs[] = {{"GAIL_STUB_FAIL_AT", {0}, {false}}, {"GAIL_STUB_SLOTS", {0}, {false}},
+                  {"GAIL_STUB_DELAY_MS", {0}, {false}}, {"GAIL_STUB_MEM_MB", {0}, {false}},
+                  {"GAIL_STUB_MEM_USED_MB", {0}, {false}}};
+
+long controlValue(const char* name, long dflt) {
+    for (Knob& k : g_knobs)
+        if (!std::strcmp(k.name, name)) return k.set.load() ? k.value.load() : dflt;
+    return dflt;
 }
 
 int failWith(int status, const char* msg) {
@@ -54,8 +79,18 @@ struct Device {
     std::string name = "Stand-in MI355X (HIP test plugin)";
     std::atomic<long> encodes{0};
     std::atomic<int> encoders{0};
+    std::atomic<uint64_t> slotBytes{0};  // held by live batches
 };
 
+uint64_t deviceTotal() { return (uint64_t)envLong("GAIL_STUB_MEM_MB", 288 * 1024) << 20; }
+
+uint64_t deviceAvailable(const Device* d) {
+    const uint64_t used = d->slotBytes.load() + ((uint64_t)envLong("GAIL_STUB_MEM_USED_MB", 0) << 20);
+    const uint64_t total = deviceTotal();
+    return used < total ? total - used : 0;
+}
+
+struct FrameResult;
 struct Encoder {
     Device* dev;
     int numCblks;
@@ -99,14 +134,52 @@ int deviceCreate(int index, GailHipLog log, void* user, GailHipDevice** out) {
 void deviceDestroy(GailHipDevice* d) { delete reinterpret_cast<Device*>(d); }
 const char* deviceName(GailHipDevice* d) { return reinterpret_cast<Device*>(d)->name.c_str(); }
 
-int deviceMemory(GailHipDevice*, GailHipMemory* out) {
-    std::memset(out, 0, sizeof *out);  // not budgeted: the library takes frames_in_flight as asked
+int deviceMemory(GailHipDevice* d, GailHipMemory* out) {
+    std::memset(out, 0, sizeof *out);
+    out->total = deviceTotal();
+    out->available = deviceAvailable(reinterpret_cast<Device*>(d));
+    out->budgeted = 1;
     return GAIL_HIP_OK;
 }
 
 uint64_t encoderBytes(GailHipDevice*, const GailHipLayout* L, const GailHipEncodeSettings*) {
     return (uint64_t)L->width * L->height * 16;
 }
+
+// One frame's stand-in Tier-1 result: a coding pass in every seventh block, its bytes from a
+// hash of the input.
+struct FrameResult {
+    std::vector<GailHipCblkInfo> info;
+    std::vector<uint32_t> passLen;
+    std::vector<int32_t> passDist;
+    std::vector<uint8_t> data;
+    void code(int numCblks, uint64_t h) {
+        info.assign(numCblks, GailHipCblkInfo{0, 0, 0, 0, 0});
+        passLen.clear();
+        passDist.clear();
+        data.clear();
+        for (int i = 0; i < numCblks; i += 7) {
+            GailHipCblkInfo& b = info[i];
+            b.nbps = 1;
+            b.numPasses = 1;
+            b.dataOffset = (uint32_t)data.size();
+            b.dataLen = 3;
+            b.passBase = (uint32_t)passLen.size();
+            for (int k = 0; k < 3; ++k) data.push_back((uint8_t)((h >> (8 * ((i + k) % 8))) & 0x7F));
+            passLen.push_back(3);
+            passDist.push_back(100 + (int32_t)((h >> (i % 48)) & 63));
+        }
+    }
+    void view(GailHipTier1* out) const {
+        out->info = info.data();
+        out->passLen = passLen.data();
+        out->passDist = passDist.data();
+        out->data = data.data();
+        out->numInfo = info.size();
+        out->numPasses = passLen.size();
+        out->numData = data.size();
+    }
+};
 
 // Bytes of one frame in the input layout the settings describe.
 size_t inputBytes(const GailHipEncodeSettings& s) {
@@ -187,7 +260,130 @@ int encoderEncode(GailHipEncoder* eh, const void* frame, GailHipTier1* out, Gail
     return GAIL_HIP_OK;
 }
 
-int unsupported() { return failWith(GAIL_HIP_UNSUPPORTED, "a stand-in plugin has no decoder or batches"); }
+// ---- batches of frame slots (the served form's jobs)
+struct Batch {
+    Device* dev;
+    uint64_t bytes;
+    std::vector<GailHipSlot> slots;
+    std::vector<std::vector<uint8_t>> staging;  // one buffer per slot
+    struct Frame {
+        int numCblks = 0;
+        size_t inBytes = 0;
+        const void* src = nullptr;
+        double slope = 0;
+        bool rate = false;
+        FrameResult result;
+    };
+    std::vector<Frame> frames;
+    bool launched = false, failed = false;
+};
+
+uint64_t slotsBytes(GailHipDevice*, const GailHipSlot* s, int count, const GailHipBatchOptions*) {
+    uint64_t b = 0;
+    for (int i = 0; i < count; ++i) b += (uint64_t)s[i].maxWidth * s[i].maxHeight * 40;
+    return b;
+}
+
+int slotsCreate(GailHipDevice* dh, const GailHipSlot* s, int count, const GailHipBatchOptions* o, GailHipBatch** out) {
+    Device* d = reinterpret_cast<Device*>(dh);
+    const uint64_t need = slotsBytes(dh, s, count, o);
+    if (need > deviceAvailable(d)) return failWith(GAIL_HIP_DEVICE_MEMORY, "out of device memory (stand-in)");
+    Batch* b = new Batch();
+    b->dev = d;
+    b->bytes = need;
+    b->slots.assign(s, s + count);
+    for (int i = 0; i < count; ++i) b->staging.emplace_back((size_t)s[i].maxWidth * s[i].maxHeight * 6);
+    d->slotBytes += need;
+    *out = reinterpret_cast<GailHipBatch*>(b);
+    return GAIL_HIP_OK;
+}
+
+void batchDestroy(GailHipBatch* bh) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    b->dev->slotBytes -= b->bytes;
+    delete b;
+}
+
+int batchBind(GailHipBatch* bh, const GailHipLayout* l, const GailHipEncodeSettings* st, int count) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    if (count < 1 || count > (int)b->slots.size()) return failWith(GAIL_HIP_MISUSE, "more frames than slots");
+    b->frames.assign(count, Batch::Frame{});
+    for (int i = 0; i < count; ++i) {
+        if (l[i].width > b->slots[i].maxWidth || l[i].height > b->slots[i].maxHeight)
+            return failWith(GAIL_HIP_MISUSE, "a frame larger than its slot");
+        b->frames[i].numCblks = l[i].numCblks;
+        b->frames[i].inBytes = inputBytes(st[i]);
+        b->frames[i].rate = st[i].rcMaxTotal != 0;
+    }
+    return GAIL_HIP_OK;
+}
+
+void* batchInput(GailHipBatch* bh, int i) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    if (i < 0 || i >= (int)b->staging.size()) {
+        failWith(GAIL_HIP_MISUSE, "no such slot");
+        return nullptr;
+    }
+    return b->staging[i].data();
+}
+
+int batchUploadFrom(GailHipBatch* bh, int i, const void* src) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    if (i < 0 || i >= (int)b->frames.size() || !src) return failWith(GAIL_HIP_MISUSE, "no such frame");
+    b->frames[i].src = src;
+    return GAIL_HIP_OK;
+}
+
+void batchSetSlope(GailHipBatch* bh, int i, double t) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    if (i >= 0 && i < (int)b->frames.size()) b->frames[i].slope = t;
+}
+
+int batchLaunch(GailHipBatch* bh, int) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    const long delay = envLong("GAIL_STUB_DELAY_MS", 0);
+    if (delay > 0) std::this_thread::sleep_for(std::chrono::milliseconds(delay));
+    const long failAt = envLong("GAIL_STUB_FAIL_AT", 0);
+    b->failed = false;
+    for (size_t i = 0; i < b->frames.size(); ++i) {
+        Batch::Frame& f = b->frames[i];
+        const long n = ++b->dev->encodes;
+        if (failAt > 0 && n >= failAt) b->failed = true;
+        if (!f.src) return failWith(GAIL_HIP_MISUSE, "a bound frame without input");
+        std::memcpy(b->staging[i].data(), f.src, std::min(f.inBytes, b->staging[i].size()));  // the "upload"
+        f.result.code(f.numCblks, inputHash(b->staging[i].data(), f.inBytes));
+    }
+    b->launched = true;
+    return GAIL_HIP_OK;
+}
+
+int batchWait(GailHipBatch* bh, GailHipTier1* outs, GailHipFrameTimings* timings) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    if (!b->launched) return failWith(GAIL_HIP_MISUSE, "nothing launched");
+    b->launched = false;
+    if (b->failed) return failWith(GAIL_HIP_ERROR, "device lost (stand-in)");
+    for (size_t i = 0; i < b->frames.size(); ++i) b->frames[i].result.view(&outs[i]);
+    if (timings) std::memset(timings, 0, sizeof *timings);
+    return GAIL_HIP_OK;
+}
+
+int batchQuery(GailHipBatch*) { return 1; }
+
+// Device rate control of a stand-in frame: every pass is kept (its few bytes fit any budget).
+int batchRate(GailHipBatch* bh, int i, GailHipRate* out) {
+    Batch* b = reinterpret_cast<Batch*>(bh);
+    if (i < 0 || i >= (int)b->frames.size() || !b->frames[i].rate)
+        return failWith(GAIL_HIP_MISUSE, "the frame had no rate control on the device");
+    std::memset(out, 0, sizeof *out);
+    out->lambda[0] = out->lambda[1] = out->lambda[2] = 1.0;
+    return GAIL_HIP_OK;
+}
+
+void* hostAlloc(uint64_t bytes) { return std::malloc(bytes ? bytes : 16); }
+void hostFree(void* p) { std::free(p); }
+int isDevicePointer(const void*) { return 0; }
+
+int unsupported() { return failWith(GAIL_HIP_UNSUPPORTED, "a stand-in plugin has no decoder"); }
 uint64_t decoderBytes(GailHipDevice*, const GailHipLayout*, const GailHipDecodeSettings*) { return 0; }
 int decoderCreate(GailHipDevice*, const GailHipLayout*, const GailHipDecodeSettings*, GailHipDecoder**) { return unsupported(); }
 void decoderDestroy(GailHipDecoder*) {}
@@ -223,12 +419,38 @@ GailHipPlugin makeTable() {
     t.decoder_create = decoderCreate;
     t.decoder_destroy = decoderDestroy;
     t.decoder_decode = decoderDecode;
-    return t;  // batch entries stay NULL: libgail's encoder does not use them
+    t.batch_destroy = batchDestroy;
+    t.batch_input = batchInput;
+    t.batch_set_slope = batchSetSlope;
+    t.batch_launch = batchLaunch;
+    t.batch_wait = batchWait;
+    t.slots_bytes = slotsBytes;
+    t.slots_create = slotsCreate;
+    t.batch_bind = batchBind;
+    t.batch_query = batchQuery;
+    t.batch_upload_from = batchUploadFrom;
+    t.host_alloc = hostAlloc;
+    t.host_free = hostFree;
+    t.is_device_pointer = isDevicePointer;
+    t.batch_rate = batchRate;
+    return t;  // the entries the served form does not use stay NULL
 }
 
 const GailHipPlugin kTable = makeTable();
 
 }  // namespace
+
+// Sets one of the GAIL_STUB_* knobs from the test program (found with dlopen RTLD_NOLOAD on
+// the plugin libgail.so loaded); returns 0, or -1 for an unknown name.
+extern "C" __attribute__((visibility("default"))) int gail_stub_control(const char* name, long value) {
+    for (Knob& k : g_knobs)
+        if (!std::strcmp(k.name, name)) {
+            k.value.store(value);
+            k.set.store(true);
+            return 0;
+        }
+    return -1;
+}
 
 #if STUB_MODE != 5
 extern "C" __attribute__((visibility("default"))) const GailHipPlugin* gail_hip_plugin_get(uint32_t abi) {

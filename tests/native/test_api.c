@@ -11,6 +11,7 @@
  * input and codestream to a directory, so that a test can recompute the codestream from the
  * same pixels on another path. Exit status 0 when every check held; each failed check prints
  * a FAIL line. A watchdog ends the process (exit 3) if a call does not return. */
+#define _GNU_SOURCE 1
 #include <pthread.h>
 #include <stddef.h>
 #include <signal.h>
@@ -533,6 +534,215 @@ static int test_create(int argc, char** argv) {
     return 0;
 }
 
+/* ---- served form ------------------------------------------------------------------------ */
+#include <dlfcn.h>
+
+typedef int (*stub_control_fn)(const char*, long);
+
+/* The stand-in plugin's knob setter, when the plugin next to libgail.so is the stand-in. */
+static stub_control_fn stub_control(void) {
+    Dl_info info;
+    if (!dladdr((void*)&gail_version, &info) || !info.dli_fname) return NULL;
+    char path[1024];
+    snprintf(path, sizeof path, "%s", info.dli_fname);
+    char* slash = strrchr(path, '/');
+    if (!slash) return NULL;
+    snprintf(slash + 1, sizeof path - (size_t)(slash + 1 - path), "libgail_hip.so");
+    void* h = dlopen(path, RTLD_NOW | RTLD_NOLOAD);
+    return h ? (stub_control_fn)dlsym(h, "gail_stub_control") : NULL;
+}
+
+typedef struct {
+    pthread_mutex_t m;
+    int done, ok, nomem, failed, compliant;
+    int order[256];
+} server_sink;
+
+static void on_result(void* user, const gail_server_result* r) {
+    server_sink* s = (server_sink*)user;
+    pthread_mutex_lock(&s->m);
+    if (s->done < 256) s->order[s->done] = (int)r->id;
+    s->done++;
+    if (r->status == GAIL_OK) {
+        s->ok++;
+        if (r->packet && r->packet->compliant && is_codestream(r->packet->data, r->packet->size)) s->compliant++;
+    } else if (r->status == GAIL_ERROR_NOMEM) {
+        s->nomem++;
+    } else {
+        s->failed++;
+    }
+    pthread_mutex_unlock(&s->m);
+}
+
+static void stream_config(gail_config* c, int fourk) {
+    gail_config_defaults(c);
+    c->profile = fourk ? GAIL_PROFILE_4K : GAIL_PROFILE_2K;
+    c->container = GAIL_CONTAINER_FLAT;
+    c->fps = 24.0;
+    c->max_mbps = 250.0;
+    c->input.format = GAIL_PIX_XYZ12_PLANAR;
+}
+
+/* One X'Y'Z' planar frame per size, shared by every submit (the server reads it before the
+   frame's result arrives; nothing writes it). */
+static uint16_t* g_frames[2];
+
+static int submit(gail_server* srv, uint64_t id, int fourk) {
+    gail_config c;
+    stream_config(&c, fourk);
+    int w, h;
+    gail_container_size(c.profile, c.container, &w, &h);
+    if (!g_frames[fourk]) {
+        g_frames[fourk] = (uint16_t*)malloc((size_t)w * h * 6);
+        for (size_t i = 0; i < (size_t)w * h * 3; ++i) g_frames[fourk][i] = (uint16_t)(noise((uint32_t)i, 7, 1, fourk) & 4095u);
+    }
+    gail_frame fr;
+    memset(&fr, 0, sizeof fr);
+    for (int p = 0; p < 3; ++p) fr.data[p] = g_frames[fourk] + (size_t)p * w * h;
+    return gail_server_submit(srv, id, fourk ? 2 : 1, &c, &fr, 0.0);
+}
+
+static gail_server* make_server(server_sink* sink, int slots, int jobs, uint64_t budget) {
+    gail_server_config sc;
+    gail_server_config_init(&sc, sizeof sc);
+    sc.slots_per_job = slots;
+    sc.max_jobs = jobs;
+    sc.memory_budget = budget;
+    sc.check = 1;
+    sc.cpu_threads = 3;
+    sc.host_rate_control = 1;
+    sc.on_result = on_result;
+    sc.user = sink;
+    memset(sink, 0, sizeof *sink);
+    pthread_mutex_init(&sink->m, NULL);
+    gail_server* srv = NULL;
+    const int r = gail_server_create(&sc, &srv);
+    CHECK(r == GAIL_OK && srv, "server create: %s (%s)", gail_status_string(r), gail_last_error());
+    return srv;
+}
+
+static void print_stats(const char* name, gail_server* srv) {
+    gail_server_stats st;
+    memset(&st, 0, sizeof st);
+    gail_server_get_stats(srv, &st);
+    printf("  %-16s %llu frames (%llu failed); jobs now %d (2K %d, 4K %d), made %d, retired %d, most %d; "
+           "in flight at most %d; peak %.0f MiB\n",
+           name, (unsigned long long)st.frames, (unsigned long long)st.failed, st.jobs, st.jobs_2k, st.jobs_4k,
+           st.jobs_made, st.jobs_retired, st.max_jobs, st.max_frames_in_flight, st.peak_job_bytes / 1048576.0);
+}
+
+/* server <case>: mixed | classes | pressure | nomem | lost (stand-in plugin only, but for mixed) */
+static int test_server(const char* which) {
+    const uint64_t MB = 1ull << 20;
+    server_sink sink;
+    gail_device_info info[1];
+    gail_enumerate_devices(GAIL_BACKEND_AUTO, info, 1);  /* loads the plugin */
+    stub_control_fn knob = stub_control();
+    if (!strcmp(which, "mixed")) {
+        /* frames of both sizes arriving together: each class gets jobs of its own slot size */
+        gail_server* srv = make_server(&sink, 2, 4, 0);
+        if (!srv) return 1;
+        for (int i = 0; i < 16; ++i) CHECK(submit(srv, i, i % 2) == GAIL_OK, "submit %d: %s", i, gail_last_error());
+        gail_server_drain(srv);
+        gail_server_stats st;
+        gail_server_get_stats(srv, &st);
+        print_stats("mixed", srv);
+        CHECK(sink.done == 16 && sink.ok == 16 && sink.compliant == 16, "mixed: %d results, %d ok, %d compliant",
+              sink.done, sink.ok, sink.compliant);
+        CHECK(st.jobs_2k >= 1 && st.jobs_4k >= 1, "mixed: no job of each slot class");
+        CHECK(st.slot_bytes_2k > 0 && st.slot_bytes_2k < st.slot_bytes, "mixed: 2K slots not smaller than 4K slots");
+        gail_server_destroy(srv);
+        return 0;
+    }
+    if (!knob) {
+        printf("  server %s needs the stand-in plugin\n", which);
+        return 2;
+    }
+    if (!strcmp(which, "classes")) {
+        /* a budget of two 2K jobs: a 4K frame retires the idle 2K jobs to make its own */
+        gail_server* probe = make_server(&sink, 2, 4, 0);
+        if (!probe) return 1;
+        gail_server_stats st;
+        gail_server_get_stats(probe, &st);
+        gail_server_destroy(probe);
+        const uint64_t job2k = 2 * st.slot_bytes_2k, job4k = 2 * st.slot_bytes;
+        gail_server* srv = make_server(&sink, 2, 4, job4k + job2k / 2);
+        if (!srv) return 1;
+        for (int i = 0; i < 6; ++i) CHECK(submit(srv, i, 0) == GAIL_OK, "submit 2K %d", i);
+        gail_server_drain(srv);
+        gail_server_get_stats(srv, &st);
+        CHECK(st.jobs_2k >= 2, "classes: 2K frames made %d 2K jobs", st.jobs_2k);
+        for (int i = 0; i < 2; ++i) CHECK(submit(srv, 100 + i, 1) == GAIL_OK, "submit 4K %d", i);
+        gail_server_drain(srv);
+        gail_server_get_stats(srv, &st);
+        print_stats("classes", srv);
+        CHECK(sink.ok == 8 && sink.compliant == 8, "classes: %d ok of 8", sink.ok);
+        CHECK(st.jobs_retired >= 1 && st.jobs_4k == 1, "classes: %d retired, %d 4K jobs", st.jobs_retired, st.jobs_4k);
+        CHECK(st.peak_job_bytes <= job4k + job2k / 2, "classes: %llu bytes held over the budget",
+              (unsigned long long)st.peak_job_bytes);
+        gail_server_destroy(srv);
+        return 0;
+    }
+    if (!strcmp(which, "pressure")) {
+        /* another process takes the device's memory: idle jobs are given up one at a time,
+           the last one kept, and frames still code */
+        knob("GAIL_STUB_MEM_MB", 16 * 1024);
+        gail_server* srv = make_server(&sink, 2, 4, 0);
+        if (!srv) return 1;
+        for (int i = 0; i < 16; ++i) CHECK(submit(srv, i, 1) == GAIL_OK, "submit %d", i);
+        gail_server_drain(srv);
+        gail_server_stats st;
+        gail_server_get_stats(srv, &st);
+        const int before = st.jobs;
+        CHECK(before >= 2, "pressure: only %d jobs made", before);
+        knob("GAIL_STUB_MEM_USED_MB", (long)(16 * 1024 - st.job_bytes / MB - 100));
+        for (int i = 0; i < 12; ++i) {
+            CHECK(submit(srv, 100 + i, 1) == GAIL_OK, "submit under pressure %d", i);
+            gail_server_drain(srv);
+            sleep_ms(300);
+        }
+        gail_server_get_stats(srv, &st);
+        print_stats("pressure", srv);
+        CHECK(st.jobs_retired >= 1 && st.jobs >= 1 && st.jobs < before, "pressure: %d of %d jobs left, %d retired",
+              st.jobs, before, st.jobs_retired);
+        CHECK(sink.ok == 28, "pressure: %d of 28 frames ok", sink.ok);
+        gail_server_destroy(srv);
+        knob("GAIL_STUB_MEM_USED_MB", 0);
+        return 0;
+    }
+    if (!strcmp(which, "nomem")) {
+        /* a budget that holds 2K jobs but no 4K job: 4K frames fail with NOMEM, 2K frames code */
+        gail_server* probe = make_server(&sink, 1, 2, 0);
+        if (!probe) return 1;
+        gail_server_stats st;
+        gail_server_get_stats(probe, &st);
+        gail_server_destroy(probe);
+        gail_server* srv = make_server(&sink, 1, 2, st.slot_bytes_2k * 3 / 2);
+        if (!srv) return 1;
+        for (int i = 0; i < 4; ++i) CHECK(submit(srv, i, i % 2) == GAIL_OK, "submit %d", i);
+        gail_server_drain(srv);
+        print_stats("nomem", srv);
+        CHECK(sink.ok == 2 && sink.nomem == 2, "nomem: %d ok, %d out of memory", sink.ok, sink.nomem);
+        gail_server_destroy(srv);
+        return 0;
+    }
+    if (!strcmp(which, "lost")) {
+        /* the device fails mid-stream: every frame still gets a result, and destroy returns */
+        knob("GAIL_STUB_FAIL_AT", 5);
+        gail_server* srv = make_server(&sink, 2, 2, 0);
+        if (!srv) return 1;
+        for (int i = 0; i < 12; ++i) CHECK(submit(srv, i, i % 2) == GAIL_OK, "submit %d", i);
+        gail_server_drain(srv);
+        print_stats("lost", srv);
+        CHECK(sink.done == 12 && sink.failed >= 1 && sink.ok >= 1, "lost: %d results, %d ok, %d failed", sink.done,
+              sink.ok, sink.failed);
+        gail_server_destroy(srv);
+        knob("GAIL_STUB_FAIL_AT", 0);
+        return 0;
+    }
+    return 2;
+}
+
 /* ---- configuration checks (binary compatibility rules of gail.h) -------------------------- */
 static int test_config(void) {
     gail_encoder* e = NULL;
@@ -583,7 +793,7 @@ static int test_config(void) {
 int main(int argc, char** argv) {
     setvbuf(stdout, NULL, _IOLBF, 0);
     if (argc < 2) {
-        fprintf(stderr, "usage: test_api pull|callback|threads|cancel|lost|create|config [arguments]\n");
+        fprintf(stderr, "usage: test_api pull|callback|threads|cancel|lost|create|config|server [arguments]\n");
         return 2;
     }
     signal(SIGALRM, on_alarm);
@@ -597,6 +807,7 @@ int main(int argc, char** argv) {
     else if (!strcmp(c, "lost")) r = test_lost();
     else if (!strcmp(c, "create")) r = test_create(argc - 2, argv + 2);
     else if (!strcmp(c, "config")) r = test_config();
+    else if (!strcmp(c, "server") && argc > 2) r = test_server(argv[2]);
     else r = 2;
     if (r == 2) {
         fprintf(stderr, "test_api: bad arguments\n");
