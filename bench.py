@@ -3,13 +3,14 @@
 Headline (`value`): sustained 4K DCI encode rate of the served form, frames/s of finished
 codestreams: DCI 4K flat 3996x2160 12-bit X'Y'Z' frames of the seeded synthetic generator
 (NL 6, 32x32 code-blocks), 24 fps rate control at 250 Mbit/s (1,302,067-byte budget), every
-codestream built and DCI-checked. The input frames are resident in HBM when timing starts
-(each is copied within HBM into a job slot), 128 frames in flight; the same run with the
-frames in pinned host memory (PCIe in the timed loop) is reported beside it. Colour/ICT, 9/7
-DWT, quantisation and Tier-1 run on the GPU, PCRD rate control
-and Tier-2 on the host pool, with early termination at 0.25 x the stream's last lambda as
-the reference pipeline does. `plain_fps` is the same encode one frame at a time (the
-reference's slot flow); `tier1_batched_fps` the GPU stages alone on batches of 16 frames
+codestream built and DCI-checked. The input frames are in pinned host memory and each crosses
+PCIe inside the timed region, as the reference's pipeline uploads every frame; 128 frames in
+flight. A step is --step-frames (32) frames of the stream: --warmup untimed steps, then
+--steps timed ones. The same run from frames resident in HBM (copied within HBM into the job
+slots) is reported beside it. Colour/ICT, 9/7 DWT, quantisation, Tier-1 and PCRD rate control
+run on the GPU, Tier-2 on the host pool, with early termination at 0.25 x the stream's last
+lambda as the reference pipeline does. `plain_fps` is the same encode one frame at a time
+(the reference's slot flow); `tier1_batched_fps` the GPU stages alone on batches of 16 frames
 resident in HBM. `stress_4k_48fps` is the 48 fps rate-control stream with up to 256 frames in
 flight; `served` the mixed-profile request loop at 300 fps of Poisson arrivals.
 
@@ -302,9 +303,15 @@ def served_run(g, np, offered_fps=300.0, seconds=5.0, seed=7):
         return {"offered_fps": offered_fps, "frames": st["frames"], "fps": round(st["fps"], 2),
                 "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
                 "queued_p99_ms": round(st["queued_p99_ms"], 2), "mean_batch": round(st["mean_batch"], 2),
-                "jobs": st["jobs"], "slots_per_job": st["slots_per_job"],
+                "jobs_at_end": {"2k": st["jobs_2k"], "4k": st["jobs_4k"]}, "jobs_made": st["jobs_made"],
+                "jobs_retired": st["jobs_retired"], "max_jobs": st["max_jobs"],
+                "slots_per_job": {"2k": st["slots_per_job_2k"], "4k": st["slots_per_job"]},
                 "max_frames_in_flight": st["max_frames_in_flight"],
-                "slot_gb": round(st["slot_bytes"] / 1e9, 3), "recoded": st["recoded"], "failed_or_noncompliant": bad}
+                "slot_gb": {"2k": round(st["slot_bytes_2k"] / 1e9, 3), "4k": round(st["slot_bytes"] / 1e9, 3)},
+                "peak_job_gb": round(st["peak_job_bytes"] / 1e9, 2),
+                "admission": "jobs of 2K or 4K slots made as frames of their class wait, against the device's "
+                             "free memory less max(512 MiB, 6 %)",
+                "recoded": st["recoded"], "failed_or_noncompliant": bad}
     finally:
         srv.close()
 
@@ -395,9 +402,10 @@ class StreamBench:
             self.src.append(_pin(g, np, [x[0], x[1], x[2]]))
             self.dev.append(g.DeviceFrame([x[0], x[1], x[2]]))
         self.next_id = 0
-        # every job runs once before anything is timed
+        # the server makes its jobs as frames wait: 256 frames at once make all 16 (and run
+        # each once) before anything is timed
         cfg = g.stream_config("4k_flat", "xyz12_planar", fps=24.0)
-        for i in self._ids(self.srv.stats()["jobs"] * 16):
+        for i in self._ids(16 * 16):
             self.srv.submit(i, 100, cfg, self.src[i % distinct], -1.0)
         self.srv.drain()
 
@@ -411,16 +419,19 @@ class StreamBench:
         if sem is not None:
             sem.release()
 
-    def run(self, fps_rc, frames, in_flight, stream, resident=True):
+    def run(self, fps_rc, frames, in_flight, stream, resident=True, warm_frames=4):
         """frames of a 4K flat stream at fps_rc rate control, at most in_flight at once, from
-        frames resident in HBM (resident) or from pinned host memory."""
+        frames resident in HBM (resident) or from pinned host memory, after warm_frames
+        untimed frames of the same stream (they also settle its early-termination threshold)."""
         g, np = self.g, self.np
         pool = self.dev if resident else self.src
         cfg = g.stream_config("4k_flat", "xyz12_planar", fps=fps_rc)
-        # settle the stream's early-termination threshold on a few frames first
-        for i in self._ids(4):
+        self._sem = threading.Semaphore(in_flight)
+        for i in self._ids(max(warm_frames, 4)):
+            self._sem.acquire()
             self.srv.submit(i, stream, cfg, pool[i % len(pool)], -1.0)
         self.srv.drain()
+        self._sem = None
         st0 = self.srv.stats()
         ids = self._ids(frames)
         self._sem = threading.Semaphore(in_flight)
@@ -436,7 +447,8 @@ class StreamBench:
         bad = sum(1 for r in res if r["status"] != 0 or not r.get("compliant", 0))
         cap = int(250e6 / (8 * fps_rc)) - 16
         st = self.srv.stats()
-        return {"frames": frames, "fps": round(frames / dt, 2), "p50_ms": _pct(np, lat, 50), "p99_ms": _pct(np, lat, 99),
+        return {"frames": frames, "warm_frames": max(warm_frames, 4), "seconds": round(dt, 3),
+                "fps": round(frames / dt, 2), "p50_ms": _pct(np, lat, 50), "p99_ms": _pct(np, lat, 99),
                 "inputs": "resident in HBM" if resident else "pinned host memory (PCIe in the timed loop)",
                 "max_frames_in_flight": in_flight, "max_bytes": max(r.get("size", 0) for r in res), "cap_bytes": cap,
                 "mean_psnr_db": round(float(np.mean([r.get("psnr", 0) for r in res])), 2),
@@ -474,15 +486,17 @@ def pmc_probe(g, np, W, H, fourk, frames):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10, help="batches of the Tier-1 kernel measurement")
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=24,
+                    help="timed steps of the headline stream, each --step-frames frames")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed steps of the stream before the timed ones")
+    ap.add_argument("--step-frames", type=int, default=32)
+    ap.add_argument("--t1-batches", type=int, default=10, help="batches of the Tier-1 kernel measurement")
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--profile", default="4k_flat")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-frames", type=int, default=8)
     ap.add_argument("--graph", type=int, default=0)
     ap.add_argument("--inflight", type=int, default=2)
-    ap.add_argument("--stream-frames", type=int, default=768)
     ap.add_argument("--stress-frames", type=int, default=512)
     ap.add_argument("--served-seconds", type=float, default=5.0)
     ap.add_argument("--mixed-steps", type=int, default=6)
@@ -550,18 +564,18 @@ def main():
           flush=True)
     launched = done = 0
     t0 = time.perf_counter()
-    while launched < min(nb, args.steps):
+    while launched < min(nb, args.t1_batches):
         batches[launched % nb].launch()
         launched += 1
-    while done < args.steps:
+    while done < args.t1_batches:
         batches[done % nb].wait(copy=False)
         done += 1
-        if launched < args.steps:
+        if launched < args.t1_batches:
             batches[launched % nb].launch()
             launched += 1
     dt = time.perf_counter() - t0
-    ms_step = 1000 * dt / args.steps
-    t1_fps = B * args.steps / dt
+    ms_step = 1000 * dt / args.t1_batches
+    t1_fps = B * args.t1_batches / dt
     print(f"[bench] Tier-1 batched: {t1_fps:.1f} frames/s, {ms_step:.2f} ms per batch of {B}", file=sys.stderr,
           flush=True)
     for b in batches:
@@ -616,15 +630,17 @@ def main():
         print(f"[bench] decoder: {json.dumps(decode)}", file=sys.stderr, flush=True)
     dev.close()
 
-    # ---- served 4K flat streams (the headline) and the 48 fps stress stream
+    # ---- served 4K flat streams (the headline: frames handed over from host memory, as the
+    # reference's pipeline uploads every frame) and the 48 fps stress stream
+    steps, warm, per = max(args.steps, 1), max(args.warmup, 0), max(args.step_frames, 1)
     sb = StreamBench(g, np)
     try:
-        print(f"[bench] 4K flat stream, 24 fps rate control, {args.stream_frames} frames, 128 in flight",
-              file=sys.stderr, flush=True)
-        main_run = sb.run(24.0, args.stream_frames, 128, stream=1, resident=True)
+        print(f"[bench] 4K flat stream from pinned host memory, 24 fps rate control, {warm} + {steps} steps of "
+              f"{per} frames, 128 in flight", file=sys.stderr, flush=True)
+        main_run = sb.run(24.0, steps * per, 128, stream=3, resident=False, warm_frames=warm * per)
         print(f"[bench] stream: {json.dumps(main_run)}", file=sys.stderr, flush=True)
-        pcie_run = sb.run(24.0, args.stream_frames, 128, stream=3, resident=False)
-        print(f"[bench] stream from host memory: {json.dumps(pcie_run)}", file=sys.stderr, flush=True)
+        hbm_run = sb.run(24.0, steps * per, 128, stream=1, resident=True, warm_frames=warm * per)
+        print(f"[bench] stream from HBM-resident frames: {json.dumps(hbm_run)}", file=sys.stderr, flush=True)
         stress = None
         if args.stress_frames > 0:
             print(f"[bench] stress: 4K flat, 48 fps rate control, {args.stress_frames} frames, 256 in flight",
@@ -650,17 +666,18 @@ def main():
     value = main_run["fps"]
     line = {
         "metric": "4K DCI encode fps, batched/served vs plain, 1 MI355X; p50/p99 frame latency",
-        "value": value, "unit": "frames/s", "n_gpus": 1, "steps": args.stream_frames, "warmup": 4,
-        "ms_per_step": round(1000.0 / value, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+int", "data": "synthetic",
+        "value": value, "unit": "frames/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": round(1000.0 * per / value, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+int", "data": "synthetic",
         "config": {"workload": "served 4K flat 3996x2160 12-bit X'Y'Z' stream, 24 fps / 250 Mbit/s rate control, "
                                "codestreams built and DCI-checked",
-                   "frames_in_flight": 128, "levels": layout.levels, "code_blocks_per_frame": layout.num_cblks,
-                   "input": "resident in HBM when timing starts (copied within HBM into job slots)",
+                   "frames_per_step": per, "frames_in_flight": 128, "levels": layout.levels,
+                   "code_blocks_per_frame": layout.num_cblks,
+                   "input": "pinned host memory: every frame crosses PCIe inside the timed region",
                    "autotuned": chosen},
         "latency_ms": {"p50": main_run["p50_ms"], "p99": main_run["p99_ms"]},
         "stream": main_run,
-        "stream_inputs_over_pcie": pcie_run,
+        "stream_hbm_resident": hbm_run,
         "plain_fps": round(plain_fps, 2),
         "batched_over_plain": round(value / plain_fps, 3),
         "plain_latency_ms": {"p50": _pct(np, lat, 50), "p99": _pct(np, lat, 99),
