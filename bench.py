@@ -335,7 +335,7 @@ def mixed_batch_run(g, np, dev, steps=6, per_profile=16):
         first.upload(i, f)
         first.set_slope(i, 0.0)
     chosen, _ = first.autotune(reps=1)
-    tiles = {"64x64": 0, "64x32": 1, "32x64": 2, "64x16": 3}
+    tiles = {name: i for i, name in enumerate(g.Batch.DWT_SHAPES)}
     second = g.Batch(dev, layouts, settings, use_graph=True, dwt_tile=tiles[chosen["dwt_tile"]],
                      cm_waves=chosen["cm_waves"], mq_group=chosen["mq_group"], mq_read=chosen["mq_read"])
     for i, f in enumerate(frames):
@@ -461,7 +461,7 @@ class StreamBench:
             d.close()
 
 
-def pmc_probe(g, np, W, H, fourk, frames):
+def pmc_probe(g, np, W, H, fourk, frames, dwt_tile=3):
     """For HBM traffic counters: one plain device-to-device copy of 1 GiB (the reference for
     scaling FETCH_SIZE / WRITE_SIZE), then one batch of the frames through the whole chain."""
     import torch
@@ -472,7 +472,7 @@ def pmc_probe(g, np, W, H, fourk, frames):
     del a, b
     dev = g.Device(0)
     layout = g.FrameLayout(W, H, fourk)
-    bt = g.Batch(dev, [layout] * len(frames), [g.xyz_settings(layout)] * len(frames), dwt_tile=3, cm_waves=2,
+    bt = g.Batch(dev, [layout] * len(frames), [g.xyz_settings(layout)] * len(frames), dwt_tile=dwt_tile, cm_waves=2,
                  mq_group=256, mq_read=64)
     for i, f in enumerate(frames):
         bt.upload(i, f)
@@ -501,6 +501,7 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=5.0)
     ap.add_argument("--mixed-steps", type=int, default=6)
     ap.add_argument("--pmc-probe", action="store_true", help="a 1 GiB copy and one batch, for counter runs")
+    ap.add_argument("--pmc-tile", type=int, default=3, help="DWT tile shape of the counter probe's batch")
     ap.add_argument("--decode-frames", type=int, default=8, help="distinct 4K frames of the decoder measurement")
     ap.add_argument("--decoders", type=int, default=8)
     ap.add_argument("--decode-only", action="store_true", help="only the decoder measurement")
@@ -520,7 +521,7 @@ def main():
     print(f"[bench] generating {B} frames {W}x{H}", file=sys.stderr, flush=True)
     frames = [g.synth_frame(1000 + i, W, H, threads=16) for i in range(B)]
     if args.pmc_probe:
-        pmc_probe(g, np, W, H, fourk, frames)
+        pmc_probe(g, np, W, H, fourk, frames, dwt_tile=args.pmc_tile)
         return
     dev = g.Device(0)
     if args.decode_only:
@@ -540,7 +541,7 @@ def main():
     # ---- GPU stages alone: batches of B frames resident in HBM, `inflight` batches on their
     # own streams, Tier-1 results copied to host memory
     nb = max(1, args.inflight)
-    tiles = {"64x64": 0, "64x32": 1, "32x64": 2, "64x16": 3}
+    tiles = {name: i for i, name in enumerate(g.Batch.DWT_SHAPES)}
     first = g.Batch(dev, [layout] * B, [g.xyz_settings(layout)] * B, use_graph=bool(args.graph))
     for i, f in enumerate(frames):
         first.upload(i, f)

@@ -511,15 +511,17 @@ class Batch:
         self.n = n
 
     TUNE = ("dwt_64x64", "dwt_64x32", "dwt_32x64", "dwt_64x16", "cm_1", "cm_2", "cm_4", "cm_8", "mq_64", "mq_128",
-            "mq_256", "mq_64_r64", "mq_128_r64", "mq_256_r64")
+            "mq_256", "mq_64_r64", "mq_128_r64", "mq_256_r64", "dwt_strip_120x64", "dwt_strip_120x32",
+            "dwt_strip_120x128")
+    DWT_SHAPES = ("64x64", "64x32", "32x64", "64x16", "strip_120x64", "strip_120x32", "strip_120x128")
 
     def autotune(self, reps=2):
         """Times the kernel variants on the bound, uploaded frames and keeps the fastest.
         Returns (chosen options dict, stage times dict in ms)."""
         o = BatchOptions()
-        t = (C.c_double * 14)()
-        _check(plugin().batch_autotune(self.handle, reps, C.byref(o), t, 14), "batch_autotune")
-        chosen = {"dwt_tile": ["64x64", "64x32", "32x64", "64x16"][o.dwtTile], "cm_waves": o.cmWaves,
+        t = (C.c_double * len(self.TUNE))()
+        _check(plugin().batch_autotune(self.handle, reps, C.byref(o), t, len(self.TUNE)), "batch_autotune")
+        chosen = {"dwt_tile": self.DWT_SHAPES[o.dwtTile], "cm_waves": o.cmWaves,
                   "mq_group": o.mqGroup, "mq_read": o.mqRead}
         return chosen, {k: round(v, 4) for k, v in zip(self.TUNE, list(t))}
 

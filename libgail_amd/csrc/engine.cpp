@@ -477,11 +477,21 @@ void Job::growT1(bool pool, bool out) {
 // DWT level descriptors of the bound frames for the current tile shape: level l of every
 // frame that has it, tiles numbered across frames; queued to the device on the stream.
 void Job::planLevels() {
-    int tx, ty;
-    dwt_tile_size(opt_.dwtTile, tx, ty);
     const int nf = numFrames();
     levelCount_.assign(numLevels_, 0);
     levelTiles_.assign(numLevels_, 0);
+    // each level's shape: the job's, but the strips need LL regions of 2 x 2 samples at least
+    levelShape_.assign(numLevels_, opt_.dwtTile);
+    if (opt_.dwtTile >= 4)
+        for (int f = 0; f < nf; ++f) {
+            const Layout& L = frames_[f].layout;
+            int w = L.width, h = L.height;
+            for (int l = 0; l < L.levels; ++l) {
+                if (w < 2 || h < 2) levelShape_[l] = 3;
+                w = (w + 1) / 2;
+                h = (h + 1) / 2;
+            }
+        }
     for (int f = 0; f < nf; ++f) {
         const Layout& L = frames_[f].layout;
         SlotDev& d = sd_[f];
@@ -512,6 +522,8 @@ void Job::planLevels() {
             }
             ds.w = w;
             ds.h = h;
+            int tx, ty;
+            dwt_tile_size(levelShape_[l], tx, ty);
             ds.tiles_x = (w + tx - 1) / tx;
             ds.tile_base = (uint32_t)levelTiles_[l];
             levelTiles_[l] += ds.tiles_x * ((h + ty - 1) / ty);
@@ -531,7 +543,7 @@ void Job::planLevels() {
 
 void Job::enqueueDwt(hipStream_t s) {
     for (int l = 0; l < numLevels_; ++l) {
-        hipCheck(launch_dwt_level(l == 0, opt_.dwtTile, dLevels_ + (size_t)l * numSlots(), levelCount_[l], levelTiles_[l], s),
+        hipCheck(launch_dwt_level(l == 0, levelShape_[l], dLevels_ + (size_t)l * numSlots(), levelCount_[l], levelTiles_[l], s),
                  "DWT level");
         if (l == 0 && !capturing_) hipCheck(hipEventRecord(ev_[8], s), "event");
     }
@@ -556,7 +568,7 @@ JobOptions Job::autotune(int reps, TuneTimes* out) {
     // DWT tile shape: the whole DWT of the bound frames, best of reps
     int bestTile = opt_.dwtTile;
     double bestDwt = 1e30;
-    for (int shape = 0; shape < 4; ++shape) {
+    for (int shape = 0; shape < kDwtShapes; ++shape) {
         opt_.dwtTile = shape;
         planLevels();
         double best = 1e30;

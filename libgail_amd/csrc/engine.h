@@ -101,11 +101,12 @@ private:
     std::string name_, arch_;
 };
 
-// Stage times (ms) the autotuner measured for each choice: DWT tile shapes 64x64, 64x32,
-// 32x64, 32x32; context-modelling waves per workgroup 1, 2, 4, 8; MQ lanes per workgroup
-// 64, 128, 256 with 16-byte symbol reads, then the same with 64-byte reads.
+// Stage times (ms) the autotuner measured for each choice: the DWT tile shapes of
+// dwt_tile_size (64x64, 64x32, 32x64, 64x16 LDS tiles, 120x64, 120x32, 120x128 strips);
+// context-modelling waves per workgroup 1, 2, 4, 8; MQ lanes per workgroup 64, 128, 256 with
+// 16-byte symbol reads, then the same with 64-byte reads.
 struct TuneTimes {
-    double dwt[4] = {}, cm[4] = {}, mq[6] = {};  // mq: 16-byte reads x 64/128/256 lanes, then 64-byte
+    double dwt[kDwtShapes] = {}, cm[4] = {}, mq[6] = {};  // mq: 16-byte reads x 64/128/256 lanes, then 64-byte
 };
 
 // Room one frame slot of a job holds: any frame up to maxWidth x maxHeight, and (colour)
@@ -244,7 +245,7 @@ private:
     // bound composition
     uint32_t nblocks_ = 0;
     int numLevels_ = 0;
-    std::vector<int> levelCount_, levelTiles_;
+    std::vector<int> levelCount_, levelTiles_, levelShape_;
     // job-wide tables (device, and pinned staging for their upload)
     T1Block* dBlocks_ = nullptr;
     T1Block* hBlocks_ = nullptr;

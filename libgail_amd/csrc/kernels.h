@@ -123,6 +123,10 @@ hipError_t launch_rc(const RcParams& p, int frames, uint32_t max_frame_blocks, h
 
 hipError_t launch_dwt_level(bool level0, int tile_shape, const DwtLevelDesc* descs, int ndesc, int tiles,
                             hipStream_t s);
+// Tile shapes: 0 64x64, 1 64x32, 2 32x64, 3 64x16 (LDS tiles, 256 threads each), 4 120x64,
+// 5 120x32, 6 120x128 (register-streaming strips, one wave each). Levels whose LL region is
+// narrower or lower than 2 samples take the LDS tiles.
+constexpr int kDwtShapes = 7;
 void dwt_tile_size(int tile_shape, int& tx, int& ty);
 hipError_t launch_color(const ColorDesc& d, hipStream_t s);
 

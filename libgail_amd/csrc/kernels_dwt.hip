@@ -218,6 +218,203 @@ __global__ void __launch_bounds__(256) dwt_level_kernel(const DwtLevelDesc* __re
     }
 }
 
+// ---------------------------------------------------------------- DWT, streaming strips
+// One wave per strip of kStripOut output columns and TY output rows, all three components,
+// with no LDS: lane l holds the even / odd column pair starting at X0 - 4 + 2l, so the wave
+// covers the strip plus a 4-column halo each side. It streams the rows down from Y0 - 4 to
+// Y0 + TY + 2 (mirrored at the frame's edges), two at a time, each column lifting in
+// registers through a four-value pipeline (below); every pair of rows it finishes is lifted
+// across the lanes, the neighbours' values moved one lane by DPP, and stored de-interleaved
+// with coalesced row stores. The work per output sample is the lifting itself plus the
+// 128 / 120 and (TY + 7) / TY halos, against 1.5 x 1.5 for the LDS tiles above.
+//
+// Column pipeline. After the rows up to even row k, a column holds E0 = x[k] (raw), O1 =
+// x1[k-1] (odd, after the alpha step), E2 = x2[k-2] (even, after beta), O3 = x3[k-3] (odd,
+// after gamma). Rows k+1 and k+2 advance it: x1[k+1], then x2[k], x3[k-1] and x4[k-2] --
+// each sample's update is v + c * (left + right), in the steps' order, as in the whole-column
+// lifting -- leaving even row k-2 and odd row k-1 final. The values that depend on rows
+// above the first one loaded are wrong, and only rows above Y0 get them.
+constexpr int kStripOut = 120;
+
+__device__ __forceinline__ float lane_from_right(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));  // wave_shl:1
+}
+__device__ __forceinline__ float lane_from_left(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));  // wave_shr:1
+}
+
+// Horizontal lifting of one row held as (even, odd) column pairs across the wave's lanes,
+// then the low / high scaling. Lanes at the wave's ends (the halo) come out wrong.
+__device__ __forceinline__ void lift_row_lanes(float& e, float& o) {
+    o += kAlpha * (e + lane_from_right(e));
+    e += kBeta * (lane_from_left(o) + o);
+    o += kGamma * (e + lane_from_right(e));
+    e += kDelta * (lane_from_left(o) + o);
+    e *= kLowScale;
+    o *= kHighScale;
+}
+
+struct StripColumn {
+    float E0, O1, E2, O3;
+    // Advances by rows k+1 (odd, ro) and k+2 (even, re); returns even row k-2 and odd row
+    // k-1, final and scaled.
+    __device__ __forceinline__ void step(float ro, float re, float& outEven, float& outOdd) {
+        const float O1n = ro + kAlpha * (E0 + re);
+        const float E2n = E0 + kBeta * (O1 + O1n);
+        const float O3n = O1 + kGamma * (E2 + E2n);
+        const float E4 = E2 + kDelta * (O3 + O3n);
+        outEven = E4 * kLowScale;
+        outOdd = O3n * kHighScale;
+        E0 = re;
+        O1 = O1n;
+        E2 = E2n;
+        O3 = O3n;
+    }
+};
+
+template <bool kLevel0> struct StripLoader {
+    const void* src;
+    uint32_t stride, plane;
+    int w, h, ce;        // the lane's even column (may be outside [0, w): mirrored)
+    bool inside, pairs;  // columns need no mirroring / a pair is one aligned load
+    int cx0, cx1;        // mirrored columns
+    __device__ __forceinline__ void init(const DwtLevelDesc& d, int x0, int lane) {
+        src = d.src;
+        stride = d.src_stride;
+        plane = d.src_plane;
+        w = d.w;
+        h = d.h;
+        ce = x0 - 4 + 2 * lane;
+        inside = x0 >= 4 && x0 + kStripOut + 4 <= w;
+        pairs = inside && (stride & 1u) == 0;
+        cx0 = mirror(ce, w);
+        cx1 = mirror(ce + 1, w);
+    }
+    // Samples (even column, odd column) of row y (mirrored) of every component; level 0
+    // also applies the DC shift and the forward ICT.
+    __device__ __forceinline__ void load(int y, float (&e)[3], float (&o)[3]) const {
+        const uint32_t row = (uint32_t)mirror(y, h) * stride;
+        if constexpr (kLevel0) {
+            const uint16_t* s = (const uint16_t*)src;
+            uint32_t a[3], b[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const uint16_t* r = s + (size_t)c * plane + row;
+                if (pairs) {
+                    const uint32_t v = *(const uint32_t*)(r + ce);
+                    a[c] = v & 0xFFFFu;
+                    b[c] = v >> 16;
+                } else {
+                    a[c] = r[cx0];
+                    b[c] = r[cx1];
+                }
+            }
+            // 12-bit codes (words above 4095 read as 4095, as the host copy clips them)
+            const float x0 = (float)((int)min(a[0], 4095u) - 2048), y0 = (float)((int)min(a[1], 4095u) - 2048),
+                        z0 = (float)((int)min(a[2], 4095u) - 2048);
+            const float x1 = (float)((int)min(b[0], 4095u) - 2048), y1 = (float)((int)min(b[1], 4095u) - 2048),
+                        z1 = (float)((int)min(b[2], 4095u) - 2048);
+            e[0] = kY0[0] * x0 + kY0[1] * y0 + kY0[2] * z0;
+            e[1] = kY1[0] * x0 + kY1[1] * y0 + kY1[2] * z0;
+            e[2] = kY2[0] * x0 + kY2[1] * y0 + kY2[2] * z0;
+            o[0] = kY0[0] * x1 + kY0[1] * y1 + kY0[2] * z1;
+            o[1] = kY1[0] * x1 + kY1[1] * y1 + kY1[2] * z1;
+            o[2] = kY2[0] * x1 + kY2[1] * y1 + kY2[2] * z1;
+        } else {
+            const float* s = (const float*)src;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float* r = s + (size_t)c * plane + row;
+                if (pairs) {
+                    const float2 v = *(const float2*)(r + ce);
+                    e[c] = v.x;
+                    o[c] = v.y;
+                } else {
+                    e[c] = r[cx0];
+                    o[c] = r[cx1];
+                }
+            }
+        }
+    }
+};
+
+template <bool kLevel0, int TY>
+__global__ void __launch_bounds__(256) dwt_strip_kernel(const DwtLevelDesc* __restrict__ descs, int ndesc, int ntiles) {
+    static_assert(TY % 2 == 0, "strips start at even rows");
+    // Workgroup b runs on XCD b % 8; each XCD takes a contiguous range of workgroups, so
+    // strips that share halo rows and columns meet in the same L2.
+    const int nb = (int)gridDim.x, q = nb >> 3, rem = nb & 7;
+    const int xcd = (int)blockIdx.x & 7, j = (int)blockIdx.x >> 3;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int tileIdx = (xcd * q + min(xcd, rem) + j) * 4 + wave;
+    if (tileIdx >= ntiles) return;
+    const int lane = (int)threadIdx.x & 63;
+    const int di = find_desc(descs, ndesc, tileIdx);
+    const DwtLevelDesc d = descs[di];
+    const int t = tileIdx - (int)d.tile_base;
+    const int x0 = (t % d.tiles_x) * kStripOut, y0 = (t / d.tiles_x) * TY;
+    const int w = d.w, h = d.h;
+    StripLoader<kLevel0> ld;
+    ld.init(d, x0, lane);
+
+    // outputs: the lane's low and high columns of this level, where they are in the strip
+    const int snx = (w + 1) >> 1, sny = (h + 1) >> 1;
+    const int xo = (x0 >> 1) - 2 + lane;               // low column; the high one is snx + xo
+    const bool inStrip = lane >= 2 && lane < 2 + kStripOut / 2;
+    const bool lowOk = inStrip && ld.ce < w, highOk = inStrip && ld.ce + 1 < w;
+
+    StripColumn col[3][2];
+    {
+        float e[3], o[3];
+        ld.load(y0 - 4, e, o);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            col[c][0] = StripColumn{e[c], 0.f, 0.f, 0.f};
+            col[c][1] = StripColumn{o[c], 0.f, 0.f, 0.f};
+        }
+    }
+    // steps k = y0 - 4, y0 - 2, ...: the step at k loads rows k + 1, k + 2 and finishes rows
+    // k - 2 and k - 1, the strip's from k = y0 + 2 on
+    float nextOddE[3], nextOddO[3], nextEvenE[3], nextEvenO[3];
+    ld.load(y0 - 3, nextOddE, nextOddO);
+    ld.load(y0 - 2, nextEvenE, nextEvenO);
+    for (int k = y0 - 4; k <= y0 + TY; k += 2) {
+        float oddE[3], oddO[3], evenE[3], evenO[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            oddE[c] = nextOddE[c];
+            oddO[c] = nextOddO[c];
+            evenE[c] = nextEvenE[c];
+            evenO[c] = nextEvenO[c];
+        }
+        if (k + 2 <= y0 + TY) {  // the next step's rows, in flight while this one computes
+            ld.load(k + 3, nextOddE, nextOddO);
+            ld.load(k + 4, nextEvenE, nextEvenO);
+        }
+        float lowE[3], lowO[3], highE[3], highO[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            col[c][0].step(oddE[c], evenE[c], lowE[c], highE[c]);
+            col[c][1].step(oddO[c], evenO[c], lowO[c], highO[c]);
+        }
+        const int gy = k - 2;  // the even row finished; gy + 1 the odd one
+        if (gy < y0 || gy >= h) continue;  // wave-uniform
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            lift_row_lanes(lowE[c], lowO[c]);
+            lift_row_lanes(highE[c], highO[c]);
+            const int oyL = gy >> 1;
+            if (lowOk) d.ll_dst[(size_t)c * d.ll_plane + (size_t)oyL * d.ll_stride + xo] = lowE[c];
+            if (highOk) d.coef[(size_t)c * d.coef_plane + (size_t)oyL * d.coef_stride + snx + xo] = lowO[c];
+            if (gy + 1 < h) {
+                const int oyH = sny + (gy >> 1);
+                if (lowOk) d.coef[(size_t)c * d.coef_plane + (size_t)oyH * d.coef_stride + xo] = highE[c];
+                if (highOk) d.coef[(size_t)c * d.coef_plane + (size_t)oyH * d.coef_stride + snx + xo] = highO[c];
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- colour
 // Source R'G'B' sample (0..1 after range handling) to the 16-bit code the CPU reference
 // rounds it to (color.cpp:136-139); the per-pixel path then works on code / 65535.
@@ -453,19 +650,28 @@ template <bool L0, int TX, int TY> static hipError_t launch_dwt(const DwtLevelDe
     return hipGetLastError();
 }
 
+template <bool L0, int TY> static hipError_t launch_strip(const DwtLevelDesc* descs, int n, int tiles, hipStream_t s) {
+    hipLaunchKernelGGL((dwt_strip_kernel<L0, TY>), dim3((tiles + 3) / 4), dim3(256), 0, s, descs, n, tiles);
+    return hipGetLastError();
+}
+
 hipError_t launch_dwt_level(bool level0, int tile_shape, const DwtLevelDesc* descs, int ndesc, int tiles,
                             hipStream_t s) {
     switch (tile_shape) {
     case 0: return level0 ? launch_dwt<true, 64, 64>(descs, ndesc, tiles, s) : launch_dwt<false, 64, 64>(descs, ndesc, tiles, s);
     case 1: return level0 ? launch_dwt<true, 64, 32>(descs, ndesc, tiles, s) : launch_dwt<false, 64, 32>(descs, ndesc, tiles, s);
     case 2: return level0 ? launch_dwt<true, 32, 64>(descs, ndesc, tiles, s) : launch_dwt<false, 32, 64>(descs, ndesc, tiles, s);
+    case 4: return level0 ? launch_strip<true, 64>(descs, ndesc, tiles, s) : launch_strip<false, 64>(descs, ndesc, tiles, s);
+    case 5: return level0 ? launch_strip<true, 32>(descs, ndesc, tiles, s) : launch_strip<false, 32>(descs, ndesc, tiles, s);
+    case 6: return level0 ? launch_strip<true, 128>(descs, ndesc, tiles, s) : launch_strip<false, 128>(descs, ndesc, tiles, s);
     default: return level0 ? launch_dwt<true, 64, 16>(descs, ndesc, tiles, s) : launch_dwt<false, 64, 16>(descs, ndesc, tiles, s);
     }
 }
 
 void dwt_tile_size(int tile_shape, int& tx, int& ty) {
-    static const int shapes[4][2] = {{64, 64}, {64, 32}, {32, 64}, {64, 16}};
-    const int k = tile_shape >= 0 && tile_shape < 4 ? tile_shape : 3;
+    static const int shapes[kDwtShapes][2] = {{64, 64}, {64, 32}, {32, 64}, {64, 16},
+                                              {kStripOut, 64}, {kStripOut, 32}, {kStripOut, 128}};
+    const int k = tile_shape >= 0 && tile_shape < kDwtShapes ? tile_shape : 3;
     tx = shapes[k][0];
     ty = shapes[k][1];
 }

@@ -170,7 +170,7 @@ FrameConfig toFrame(const GailHipLayout& l, const GailHipEncodeSettings& s) {
 JobOptions toOptions(const GailHipBatchOptions* o) {
     JobOptions j;
     if (!o) return j;
-    if (o->dwtTile >= 0 && o->dwtTile <= 3) j.dwtTile = o->dwtTile;
+    if (o->dwtTile >= 0 && o->dwtTile < kDwtShapes) j.dwtTile = o->dwtTile;
     if (o->cmWaves == 1 || o->cmWaves == 2 || o->cmWaves == 4 || o->cmWaves == 8) j.cmWaves = o->cmWaves;
     if (o->mqGroup == 64 || o->mqGroup == 128 || o->mqGroup == 256) j.mqGroup = o->mqGroup;
     if (o->mqRead == 16 || o->mqRead == 64) j.mqRead = o->mqRead;
@@ -514,9 +514,10 @@ int batchAutotune(GailHipBatch* b, int reps, GailHipBatchOptions* chosen, double
             chosen->useGraph = o.useGraph ? 1 : 0;
             chosen->poolBytesPerCoef = o.poolBytesPerCoef;
         }
-        const double v[14] = {t.dwt[0], t.dwt[1], t.dwt[2], t.dwt[3], t.cm[0], t.cm[1], t.cm[2],
-                              t.cm[3],  t.mq[0],  t.mq[1],  t.mq[2],  t.mq[3], t.mq[4], t.mq[5]};
-        for (int i = 0; times && i < ntimes && i < 14; ++i) times[i] = v[i];
+        // the four LDS tile shapes first and the strips last, as the table grew
+        const double v[17] = {t.dwt[0], t.dwt[1], t.dwt[2], t.dwt[3], t.cm[0], t.cm[1], t.cm[2], t.cm[3], t.mq[0],
+                              t.mq[1],  t.mq[2],  t.mq[3],  t.mq[4],  t.mq[5], t.dwt[4], t.dwt[5], t.dwt[6]};
+        for (int i = 0; times && i < ntimes && i < 17; ++i) times[i] = v[i];
     });
 }
 

@@ -213,7 +213,8 @@ def test_slot_job_rebinds(device, oracle):
         b.close()
 
 
-@pytest.mark.parametrize("opts", [dict(dwt_tile=0), dict(dwt_tile=1), dict(dwt_tile=2), dict(cm_waves=1),
+@pytest.mark.parametrize("opts", [dict(dwt_tile=0), dict(dwt_tile=1), dict(dwt_tile=2), dict(dwt_tile=4),
+                                  dict(dwt_tile=5), dict(dwt_tile=6), dict(cm_waves=1),
                                   dict(cm_waves=2), dict(cm_waves=8), dict(mq_group=64), dict(mq_group=128),
                                   dict(mq_read=16), dict(mq_group=64, mq_read=16), dict(mq_group=128, mq_read=16)])
 def test_kernel_variants_bit_exact(device, oracle, opts):
@@ -291,3 +292,26 @@ def test_full_mixed_batch_64(device, oracle):
         if i % 8 == 7:
             print(f"[batch64] frames up to {i} checked, {len(bad_frames)} differ", flush=True)
     assert not bad_frames, f"frames differing from the oracle (index, size, threshold, blocks): {bad_frames}"
+
+
+@pytest.mark.parametrize("shape", [3, 4, 5, 6])
+def test_dwt_planes_every_shape(device, oracle, shape):
+    """The wavelet planes of every DWT tile shape, LDS tiles and register-streaming strips,
+    equal the oracle's float32 DWT bit for bit on frames whose sizes leave ragged strips and
+    tiles at both edges (odd widths and heights, LL regions down to a few samples)."""
+    sizes = [(257, 131, 0), (1998, 1080, 0), (1001, 515, 1), (2048, 858, 0)]
+    layouts = [g.FrameLayout(W, H, k) for W, H, k in sizes]
+    frames = [g.synth_frame(900 + i, W, H) for i, (W, H, _) in enumerate(sizes)]
+    b = g.Batch(device, layouts, [g.xyz_settings(L) for L in layouts], dwt_tile=shape)
+    try:
+        for i, f in enumerate(frames):
+            b.upload(i, f)
+        b.encode()
+        for i, (W, H, k) in enumerate(sizes):
+            got = b.debug(i)["wavelet"]
+            want = oracle.encode_frame_xyz(frames[i], k, threads=16, want_planes=True)["planes"]
+            bad = int(np.count_nonzero(got.view(np.uint32) != want.view(np.uint32)))
+            print(f"[dwt] shape {shape} {W}x{H}: {bad} samples differ", flush=True)
+            assert bad == 0
+    finally:
+        b.close()
