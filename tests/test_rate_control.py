@@ -190,3 +190,26 @@ def test_budgets_at_and_below_the_headers(frames, ref):
     cs = g.build_codestream(L, g.with_passes(t1, res["passes"]), tag="")
     ok, errors, facts = ref.check_codestream(cs, 24.0)
     assert ok, errors
+
+
+def test_pcrd_check_catches_planted_errors(frames):
+    """The condition check above is not vacuous: a result with one block keeping a pass
+    more or a pass fewer than the hull gives, or a remaining distortion off by one block's
+    gain, fails it."""
+    L, t1 = frames["2k_flat"]
+    max_total, max_comp = g.dci_limits(24.0, 250.0, 200.0)
+    res = g.rate_allocate(L, t1, max_total, max_comp)
+    _check_pcrd(L, t1, res)
+    info = np.asarray(t1["info"])
+    kept = np.flatnonzero((res["passes"] > 0) & (res["passes"] < info[:, 1]))
+    assert len(kept) > 0
+    for delta in (+1, -1):
+        bad = dict(res)
+        bad["passes"] = res["passes"].astype(np.int64)
+        bad["passes"][kept[len(kept) // 2]] += delta
+        with pytest.raises(AssertionError):
+            _check_pcrd(L, t1, bad)
+    bad = dict(res)
+    bad["distortion"] = res["distortion"] * (1 + 1e-6) + 1.0
+    with pytest.raises(AssertionError):
+        _check_pcrd(L, t1, bad)
