@@ -317,6 +317,23 @@ typedef struct GailHipPlugin GailHipPlugin;
  * Returns the table, or NULL for another GAIL_HIP_PLUGIN_ABI. */
 __attribute__((visibility("default"))) const GailHipPlugin* gail_hip_plugin_get(uint32_t abi);
 
+/* decoder_decode without a decoder object and without device memory of its own. It has no
+ * counterpart in the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Its intermediate buffers are
+ * a fixed 80 MiB array in the plugin's device code, which each GPU the plugin runs on holds
+ * until the process ends; so it takes frames up to 2048 x 1080 (any 2K DCI frame) with at
+ * most 1,572,864 bytes of codestream, and refuses larger ones with GAIL_HIP_MISUSE. frame,
+ * reduce, highBits, data, size, blocks and strideBytes mean what they mean for
+ * decoder_decode; data may be host or device memory, dst must be device memory. stream: a
+ * hipStream_t to work on (NULL: the default stream). Synchronous: returns when the picture is
+ * in dst. Calls on one GPU share the array (it keeps the last call's contents, and the words
+ * it must start from zero are cleared by a kernel on every call); the plugin runs them one at
+ * a time. The first call on a device also loads the decoder's constant tables there. */
+__attribute__((visibility("default"))) int gail_hip_decode_noalloc(int device, const GailHipLayout* frame, int reduce,
+                                                                   int highBits, const uint8_t* data, size_t size,
+                                                                   const GailHipCblkCode* blocks, size_t numBlocks,
+                                                                   void* dst, ptrdiff_t strideBytes, void* stream);
+
 /* Sizes of every structure above folded into one number (cuda_plugin.h:208-217), which
  * the loader compares with the table's layout field. */
 static inline uint64_t gail_hip_plugin_layout(void) {

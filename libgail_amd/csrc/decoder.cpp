@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -134,83 +135,169 @@ void Decoder::ensureDescs(size_t n) {
     capDescs_ = cap;
 }
 
-uint32_t Decoder::buildDescs(const uint8_t* coded, const DecCode* blocks) {
+void DescPlan::plan(const Layout& L, const std::vector<int>& wanted, const uint8_t* coded, const DecCode* blocks,
+                    int cus) {
     // Coded blocks longest first (a counting sort over length classes).
-    count_.assign(kLenKeys + 1, 0u);
+    count.assign(kLenKeys + 1, 0u);
     uint64_t total = 0;
     size_t ncoded = 0;
     auto key = [](uint32_t len) { return kLenKeys - 1u - std::min(len >> 2, kLenKeys - 1u); };
-    for (size_t k = 0; k < wanted_.size(); ++k)
+    for (size_t k = 0; k < wanted.size(); ++k)
         if (coded[k]) {
-            ++count_[key(blocks[wanted_[k]].len) + 1];
-            total += blocks[wanted_[k]].len;
+            ++count[key(blocks[wanted[k]].len) + 1];
+            total += blocks[wanted[k]].len;
             ++ncoded;
         }
-    for (uint32_t k = 0; k < kLenKeys; ++k) count_[k + 1] += count_[k];
-    order_.resize(ncoded);
-    for (size_t k = 0; k < wanted_.size(); ++k)
-        if (coded[k]) order_[count_[key(blocks[wanted_[k]].len)]++] = (uint32_t)wanted_[k];
+    for (uint32_t k = 0; k < kLenKeys; ++k) count[k + 1] += count[k];
+    order.resize(ncoded);
+    for (size_t k = 0; k < wanted.size(); ++k)
+        if (coded[k]) order[count[key(blocks[wanted[k]].len)]++] = (uint32_t)wanted[k];
     // Waves: a wave runs as long as its longest lane, so a long block takes a wave of its own
     // and short ones share, up to kDecWaveLanes blocks and a byte budget that spreads the frame
     // over about eight waves per compute unit (measured best on 4K frames at 250 Mbit/s).
-    const uint64_t budget = std::max<uint64_t>(384, total / (uint64_t)(8 * cus_));
-    const uint32_t lanes = (uint32_t)kDecWaveLanes;
+    budget = std::max<uint64_t>(384, total / (uint64_t)(8 * cus));
     // blocks over the budget would sit alone in a wave: they get a whole wave each and are
     // decoded with the wave's scalar unit
-    uint32_t nlong = 0;
-    while (nlong < order_.size() && blocks[order_[nlong]].len > budget) ++nlong;
-    nlong_ = nlong;
+    nlong = 0;
+    while (nlong < order.size() && blocks[order[nlong]].len > budget) ++nlong;
+    (void)L;
+}
+
+uint32_t DescPlan::pack(const Layout& L, const DecCode* blocks, DecBlock* out) const {
+    const uint32_t lanes = (uint32_t)kDecWaveLanes;
     auto entry = [&](uint32_t i) {
         const DecCode& c = blocks[i];
-        const CodeBlock& cb = L_.cblks[i];
-        const Subband& b = L_.bands[cb.band];
+        const CodeBlock& cb = L.cblks[i];
+        const Subband& b = L.bands[cb.band];
         DecBlock d;
         d.off = c.offset;
         d.len = c.len;
         d.geom = (uint32_t)(cb.w - 1) | ((uint32_t)(cb.h - 1) << 5) | ((uint32_t)b.orient << 10) |
                  ((uint32_t)c.nbps << 12) | ((uint32_t)c.passes << 18);
-        d.dst = (uint32_t)b.comp * (uint32_t)L_.width * (uint32_t)L_.height +
-                (uint32_t)(b.py0 + cb.y0) * (uint32_t)L_.width + (uint32_t)(b.px0 + cb.x0);
+        d.dst = (uint32_t)b.comp * (uint32_t)L.width * (uint32_t)L.height +
+                (uint32_t)(b.py0 + cb.y0) * (uint32_t)L.width + (uint32_t)(b.px0 + cb.x0);
         return d;
     };
-    auto pack = [&](DecBlock* out) {
-        uint32_t n = 0, inWave = 0;
-        uint64_t waveBytes = 0;
-        for (uint32_t k = 0; k < nlong; ++k, ++n)
-            if (out) out[n] = entry(order_[k]);
-        for (uint32_t k = nlong; k < order_.size(); ++k) {
-            const uint32_t i = order_[k];
-            const DecCode& c = blocks[i];
-            if (inWave && (inWave == lanes || waveBytes + c.len > budget)) {
-                if (out)
-                    for (; inWave < lanes; ++inWave) out[n++] = DecBlock{0, 0, 0, kDecEmpty};
-                else
-                    n += lanes - inWave;
-                inWave = 0;
-                waveBytes = 0;
-            }
-            if (out) out[n] = entry(i);
-            ++n;
-            ++inWave;
-            waveBytes += c.len;
-        }
-        if (inWave) {
+    uint32_t n = 0, inWave = 0;
+    uint64_t waveBytes = 0;
+    for (uint32_t k = 0; k < nlong; ++k, ++n)
+        if (out) out[n] = entry(order[k]);
+    for (uint32_t k = nlong; k < order.size(); ++k) {
+        const uint32_t i = order[k];
+        const DecCode& c = blocks[i];
+        if (inWave && (inWave == lanes || waveBytes + c.len > budget)) {
             if (out)
                 for (; inWave < lanes; ++inWave) out[n++] = DecBlock{0, 0, 0, kDecEmpty};
             else
                 n += lanes - inWave;
+            inWave = 0;
+            waveBytes = 0;
         }
-        // whole workgroups
-        while (t1_decode_waves(n, nlong) % (uint32_t)kDecWaves) {
-            for (uint32_t j = 0; j < lanes; ++j, ++n)
-                if (out) out[n] = DecBlock{0, 0, 0, kDecEmpty};
-        }
-        return n;
-    };
-    const uint32_t n = pack(nullptr);
-    ensureDescs(n);
-    pack(hDescs_);
+        if (out) out[n] = entry(i);
+        ++n;
+        ++inWave;
+        waveBytes += c.len;
+    }
+    if (inWave) {
+        if (out)
+            for (; inWave < lanes; ++inWave) out[n++] = DecBlock{0, 0, 0, kDecEmpty};
+        else
+            n += lanes - inWave;
+    }
+    // whole workgroups
+    while (t1_decode_waves(n, nlong) % (uint32_t)kDecWaves) {
+        for (uint32_t j = 0; j < lanes; ++j, ++n)
+            if (out) out[n] = DecBlock{0, 0, 0, kDecEmpty};
+    }
     return n;
+}
+
+uint32_t Decoder::buildDescs(const uint8_t* coded, const DecCode* blocks) {
+    plan_.plan(L_, wanted_, coded, blocks, cus_);
+    const uint32_t n = plan_.pack(L_, blocks, nullptr);
+    ensureDescs(n);
+    plan_.pack(L_, blocks, hDescs_);
+    nlong_ = plan_.nlong;
+    return n;
+}
+
+void queueDecode(const Layout& L, const Layout& frame, int reduce, bool highBits, const DecodeBuffers& b, hipStream_t s,
+                 hipEvent_t* ev) {
+    const uint32_t W = (uint32_t)L.width, H = (uint32_t)L.height;
+    const size_t npix = (size_t)W * H;
+    T1DecodeParams tp;
+    tp.bytes = b.bytes;
+    tp.blocks = b.descs;
+    tp.nblocks = b.nd;
+    tp.nlong = b.nlong;
+    tp.words = b.words;
+    tp.nwords = (uint32_t)(3 * npix);
+    tp.stride = W;
+    hipCheck(launch_t1_decode(tp, s), "Tier-1 decode");
+
+    // half the step of every band, from this frame's own step sizes
+    auto half = [&](int comp, int level, int orient) {
+        const int res = orient == 0 ? 0 : L.levels - level + 1;
+        return 0.5f * frame.bands[L.bandIndex(comp, res, orient)].delta;
+    };
+    if (b.wave || reduce == L.levels) {
+        DequantFrame dq{};
+        dq.words = b.words;
+        dq.dst = b.wave ? b.wave : b.plane[0];
+        dq.width = W;
+        dq.height = H;
+        dq.levels = (uint32_t)L.levels;
+        for (int c = 0; c < 3; ++c)
+            for (int l = 1; l <= L.levels; ++l)
+                for (int o = 0; o < 4; ++o) dq.half[c][l][o] = (o == 0 && l != L.levels) ? 0.f : half(c, l, o);
+        hipCheck(launch_dequant_frame(dq, s), "dequantisation");
+        if (reduce == L.levels && b.wave) {
+            dq.dst = b.plane[0];
+            hipCheck(launch_dequant_frame(dq, s), "dequantisation");
+        }
+    }
+    if (ev) hipCheck(hipEventRecord(ev[0], s), "event");
+
+    // inverse DWT, the top level first; each level's output is the next one's LL
+    int ws[kMaxDecLevels + 1], hs[kMaxDecLevels + 1];
+    ws[0] = L.width;
+    hs[0] = L.height;
+    for (int l = 1; l <= L.levels; ++l) {
+        ws[l] = (ws[l - 1] + 1) / 2;
+        hs[l] = (hs[l - 1] + 1) / 2;
+    }
+    const float* ll = reduce == L.levels ? b.plane[0] : nullptr;
+    int cur = 0, tx, ty;
+    idwt_tile_size(tx, ty);
+    for (int l = L.levels; l > reduce; --l) {
+        IdwtLevel p{};
+        p.words = b.words;
+        p.ll = ll;
+        p.dst = b.plane[cur];
+        p.stride = W;
+        p.plane = (uint32_t)npix;
+        p.w = ws[l - 1];
+        p.h = hs[l - 1];
+        p.tiles_x = (p.w + tx - 1) / tx;
+        p.tiles = p.tiles_x * ((p.h + ty - 1) / ty);
+        for (int c = 0; c < 3; ++c)
+            for (int o = 0; o < 4; ++o) p.half[c][o] = (o == 0 && l != L.levels) ? 0.f : half(c, l, o);
+        hipCheck(launch_idwt_level(p, s), "inverse DWT");
+        ll = b.plane[cur];
+        cur ^= 1;
+    }
+    if (ev) hipCheck(hipEventRecord(ev[1], s), "event");
+
+    IctPack ip;
+    ip.planes = ll;
+    ip.stride = W;
+    ip.plane = (uint32_t)npix;
+    ip.w = ws[reduce];
+    ip.h = hs[reduce];
+    ip.out = b.out;
+    ip.pitch = b.pitch;
+    ip.shift = highBits ? 4 : 0;
+    hipCheck(launch_ict_pack(ip, s), "inverse ICT");
 }
 
 void Decoder::decode(const Layout& frame, const uint8_t* data, size_t size, const DecCode* blocks, size_t n, void* dst,
@@ -254,75 +341,19 @@ void Decoder::decode(const Layout& frame, const uint8_t* data, size_t size, cons
     }
     if (nd) hipCheck(hipMemcpyAsync(dDescs_, hDescs_, (size_t)nd * sizeof(DecBlock), hipMemcpyHostToDevice, s), "block table upload");
     hipCheck(hipMemsetAsync(dWords_, 0, 3 * npix * sizeof(uint32_t), s), "clear words");
-    T1DecodeParams tp;
-    tp.bytes = dBytes_;
-    tp.blocks = dDescs_;
-    tp.nblocks = nd;
-    tp.nlong = nlong_;
-    tp.words = dWords_;
-    tp.nwords = (uint32_t)(3 * npix);
-    tp.stride = W;
-    hipCheck(launch_t1_decode(tp, s), "Tier-1 decode");
-
-    // half the step of every band, from this frame's own step sizes
-    auto half = [&](int comp, int level, int orient) {
-        const int res = orient == 0 ? 0 : L_.levels - level + 1;
-        return 0.5f * frame.bands[L_.bandIndex(comp, res, orient)].delta;
-    };
-    const bool dequantAll = wantWavelet || reduce_ == L_.levels;
-    if (dequantAll) {
-        if (wantWavelet && !dWave_) hipCheck(hipMalloc(&dWave_, 3 * npix * sizeof(float)), "wavelet planes");
-        DequantFrame dq{};
-        dq.words = dWords_;
-        dq.dst = wantWavelet ? dWave_ : dPlane_[0];
-        dq.width = W;
-        dq.height = H;
-        dq.levels = (uint32_t)L_.levels;
-        for (int c = 0; c < 3; ++c)
-            for (int l = 1; l <= L_.levels; ++l)
-                for (int o = 0; o < 4; ++o) dq.half[c][l][o] = (o == 0 && l != L_.levels) ? 0.f : half(c, l, o);
-        hipCheck(launch_dequant_frame(dq, s), "dequantisation");
-        if (reduce_ == L_.levels && wantWavelet) {
-            dq.dst = dPlane_[0];
-            hipCheck(launch_dequant_frame(dq, s), "dequantisation");
-        }
-    }
-    hipCheck(hipEventRecord(ev_[1], s), "event");
-
-    // inverse DWT, the top level first; each level's output is the next one's LL
-    const float* ll = reduce_ == L_.levels ? dPlane_[0] : nullptr;
-    int cur = 0;
-    int tx, ty;
-    idwt_tile_size(tx, ty);
-    for (int l = L_.levels; l > reduce_; --l) {
-        IdwtLevel p{};
-        p.words = dWords_;
-        p.ll = ll;
-        p.dst = dPlane_[cur];
-        p.stride = W;
-        p.plane = (uint32_t)npix;
-        p.w = ws_[l - 1];
-        p.h = hs_[l - 1];
-        p.tiles_x = (p.w + tx - 1) / tx;
-        p.tiles = p.tiles_x * ((p.h + ty - 1) / ty);
-        for (int c = 0; c < 3; ++c)
-            for (int o = 0; o < 4; ++o) p.half[c][o] = (o == 0 && l != L_.levels) ? 0.f : half(c, l, o);
-        hipCheck(launch_idwt_level(p, s), "inverse DWT");
-        ll = dPlane_[cur];
-        cur ^= 1;
-    }
-    hipCheck(hipEventRecord(ev_[2], s), "event");
-
-    IctPack ip;
-    ip.planes = ll;
-    ip.stride = W;
-    ip.plane = (uint32_t)npix;
-    ip.w = outW_;
-    ip.h = outH_;
-    ip.out = devOut ? (uint8_t*)dst : (uint8_t*)dOut_;
-    ip.pitch = devOut ? (int64_t)strideBytes : (int64_t)rowBytes;
-    ip.shift = highBits_ ? 4 : 0;
-    hipCheck(launch_ict_pack(ip, s), "inverse ICT");
+    if (wantWavelet && !dWave_) hipCheck(hipMalloc(&dWave_, 3 * npix * sizeof(float)), "wavelet planes");
+    DecodeBuffers buf;
+    buf.bytes = dBytes_;
+    buf.descs = dDescs_;
+    buf.nd = nd;
+    buf.nlong = nlong_;
+    buf.words = dWords_;
+    buf.plane[0] = dPlane_[0];
+    buf.plane[1] = dPlane_[1];
+    buf.wave = wantWavelet ? dWave_ : nullptr;
+    buf.out = devOut ? (uint8_t*)dst : (uint8_t*)dOut_;
+    buf.pitch = devOut ? (int64_t)strideBytes : (int64_t)rowBytes;
+    queueDecode(L_, frame, reduce_, highBits_, buf, s, &ev_[1]);
     const size_t outBytes = (size_t)outW_ * outH_ * 6;
     if (dmaOut) {
         hipCheck(hipMemcpy2DAsync(dst, (size_t)strideBytes, dOut_, (size_t)rowBytes, (size_t)rowBytes, (size_t)outH_,
@@ -356,6 +387,122 @@ void Decoder::decode(const Layout& frame, const uint8_t* data, size_t size, cons
         tm->color = c;
         tm->total = msSince(t0);
     }
+}
+
+// ---------------------------------------------------------------- decoding without allocation
+namespace {
+
+// Intermediate buffers of one frame of up to kNoAllocMaxWidth x kNoAllocMaxHeight: the
+// reconstruction words and the two float plane sets (3 x 2048 x 1080 x 4 bytes each), the
+// codestream with its read-ahead pad, and the Tier-1 table (up to 16 entries per block when
+// every wave holds one block, for the 6,708 blocks of a 2K frame).
+constexpr size_t kNoAllocPlane = (size_t)3 * kNoAllocMaxWidth * kNoAllocMaxHeight * 4;
+constexpr size_t kNoAllocCodeBytes = (kNoAllocMaxBytes + kBytePad + 255) / 256 * 256;
+constexpr size_t kNoAllocTableBytes = 2u << 20;
+constexpr size_t kNoAllocBytes = 3 * kNoAllocPlane + kNoAllocCodeBytes + kNoAllocTableBytes;
+
+// The array every GPU holds (in this file's code object) until the process ends. Host code
+// uses its per-device address from hipGetSymbolAddress, never the name.
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+constexpr int kNoAllocDevices = 64;
+
+struct NoAllocDevice {
+    std::once_flag once;
+    hipError_t setup = hipSuccess;
+    uint8_t* base = nullptr;  // scratch's address on this device
+    int cus = 256;
+    std::mutex m;             // one call at a time on the shared array
+};
+NoAllocDevice g_noAlloc[kNoAllocDevices];
+
+// With the device current: the array's address, the decoder's constant tables and the
+// compute-unit count, once per device (no allocation: the tables go to constant memory).
+void noAllocSetup(NoAllocDevice& d, int device) {
+    std::call_once(d.once, [&] {
+        void* p = nullptr;
+        d.setup = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (d.setup == hipSuccess) d.base = static_cast<uint8_t*>(p);
+        if (d.setup == hipSuccess) d.setup = dec_tables_init();
+        int cus = 0;
+        if (d.setup == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+            cus > 0)
+            d.cus = cus;
+    });
+    hipCheck(d.setup, "no-allocation decoder set-up");
+}
+
+}  // namespace
+
+void decodeNoAlloc(int device, const Layout& frame, int reduce, bool highBits, const uint8_t* data, size_t size,
+                   const DecCode* blocks, size_t numBlocks, void* dst, ptrdiff_t strideBytes, hipStream_t stream) {
+    if (device < 0 || device >= kNoAllocDevices) throw std::logic_error("no such device");
+    if (frame.width > kNoAllocMaxWidth || frame.height > kNoAllocMaxHeight)
+        throw std::logic_error("a " + std::to_string(frame.width) + "x" + std::to_string(frame.height) +
+                               " frame exceeds the no-allocation decoder's " + std::to_string(kNoAllocMaxWidth) + "x" +
+                               std::to_string(kNoAllocMaxHeight) + " limit");
+    if (size > kNoAllocMaxBytes)
+        throw std::logic_error("a codestream of " + std::to_string(size) + " bytes exceeds the no-allocation decoder's " +
+                               std::to_string(kNoAllocMaxBytes) + "-byte limit");
+    if (frame.levels < 1 || frame.levels > kMaxDecLevels || reduce < 0 || reduce > frame.levels)
+        throw std::logic_error("more resolution levels left out than the stream has");
+    if (numBlocks != (size_t)frame.numCblks()) throw std::logic_error("one block record per code-block is needed");
+    if (!dst || (size && !data) || (numBlocks && !blocks)) throw std::logic_error("null codestream, block table or picture");
+    int outW = frame.width, outH = frame.height;
+    for (int i = 0; i < reduce; ++i) {
+        outW = (outW + 1) / 2;
+        outH = (outH + 1) / 2;
+    }
+    const ptrdiff_t rowBytes = (ptrdiff_t)outW * 6;
+    if (strideBytes < rowBytes && strideBytes > -rowBytes) throw std::logic_error("row stride shorter than a picture row");
+    hipCheck(hipSetDevice(device), "hipSetDevice");
+    if (memKind(dst) != Mem::Device) throw std::logic_error("the no-allocation decoder writes device memory only");
+    NoAllocDevice& d = g_noAlloc[device];
+    noAllocSetup(d, device);
+    std::lock_guard<std::mutex> lock(d.m);
+
+    // the table on the host, in storage each thread keeps from call to call
+    thread_local std::vector<int> wanted;
+    thread_local std::vector<uint8_t> coded;
+    thread_local DescPlan plan;
+    thread_local std::vector<DecBlock> table;
+    wanted.clear();
+    coded.clear();
+    for (int i = 0; i < frame.numCblks(); ++i)
+        if (frame.bands[frame.cblks[i].band].res <= frame.levels - reduce) {
+            const DecCode& c = blocks[i];
+            wanted.push_back(i);
+            coded.push_back(c.passes && c.nbps >= 1 && c.nbps <= 30 && (size_t)c.offset + c.len <= size);
+        }
+    plan.plan(frame, wanted, coded.data(), blocks, d.cus);
+    const uint32_t nd = plan.pack(frame, blocks, nullptr);
+    if ((size_t)nd * sizeof(DecBlock) > kNoAllocTableBytes) throw std::runtime_error("Tier-1 table larger than its room");
+    table.resize(nd);
+    plan.pack(frame, blocks, table.data());
+
+    // the array's regions
+    const size_t npix = (size_t)frame.width * frame.height;
+    DecodeBuffers b;
+    b.words = reinterpret_cast<uint32_t*>(d.base);
+    b.plane[0] = reinterpret_cast<float*>(d.base + kNoAllocPlane);
+    b.plane[1] = reinterpret_cast<float*>(d.base + 2 * kNoAllocPlane);
+    uint8_t* bytes = d.base + 3 * kNoAllocPlane;
+    DecBlock* descs = reinterpret_cast<DecBlock*>(bytes + kNoAllocCodeBytes);
+    b.bytes = bytes;
+    b.descs = descs;
+    b.nd = nd;
+    b.nlong = plan.nlong;
+    b.out = static_cast<uint8_t*>(dst);
+    b.pitch = (int64_t)strideBytes;
+
+    const hipStream_t s = stream;
+    if (size) hipCheck(hipMemcpyAsync(bytes, data, size, hipMemcpyDefault, s), "codestream copy");
+    if (nd) hipCheck(hipMemcpyAsync(descs, table.data(), (size_t)nd * sizeof(DecBlock), hipMemcpyHostToDevice, s), "table upload");
+    // the words start at zero: the last call left its own in the array
+    hipCheck(launch_zero_words(b.words, 3 * npix, s), "clear words");
+    queueDecode(frame, frame, reduce, highBits, b, s, nullptr);
+    hipCheck(hipStreamSynchronize(s), "decode");
+    hipCheck(hipGetLastError(), "decode");
 }
 
 }  // namespace gail_amd

@@ -24,6 +24,50 @@ struct DecodeTimes {
     double t1 = 0, dwt = 0, color = 0, total = 0;  // ms; t1 includes the codestream upload
 };
 
+constexpr int kMaxDecLevels = 8;
+
+// The Tier-1 decode table of one frame: the blocks to decode, longest first; the longest get
+// a wave each, the rest are packed into waves of up to kDecWaveLanes blocks by a byte budget
+// (a wave runs as long as its longest lane).
+struct DescPlan {
+    std::vector<uint32_t> order, count;
+    uint32_t nlong = 0;
+    uint64_t budget = 0;
+    // wanted: the layout's blocks of the resolutions kept; coded[k]: wanted[k] has data.
+    void plan(const Layout& L, const std::vector<int>& wanted, const uint8_t* coded, const DecCode* blocks, int cus);
+    // Writes the table to out (nullptr: only counts); returns its entries (whole workgroups).
+    uint32_t pack(const Layout& L, const DecCode* blocks, DecBlock* out) const;
+};
+
+// Device buffers of one frame's decode.
+struct DecodeBuffers {
+    const uint8_t* bytes = nullptr;   // codestream, readable 64 bytes past its end
+    const DecBlock* descs = nullptr;  // nd table entries, the first nlong a wave each
+    uint32_t nd = 0, nlong = 0;
+    uint32_t* words = nullptr;        // 3 W x H reconstruction words, zero beforehand
+    float* plane[2] = {nullptr, nullptr};  // 3 W x H floats each, the inverse DWT's ping-pong
+    float* wave = nullptr;            // dequantised planes wanted (3 W x H floats), or null
+    uint8_t* out = nullptr;           // the picture, rows pitch bytes apart
+    int64_t pitch = 0;
+};
+
+// Queues the decode of one frame on s: Tier-1, dequantisation, the inverse DWT down to
+// `reduce` levels left out, the inverse ICT into b.out. L: the geometry, frame: the frame's own
+// step sizes. ev (may be null): two events, recorded after Tier-1 and after the inverse DWT.
+void queueDecode(const Layout& L, const Layout& frame, int reduce, bool highBits, const DecodeBuffers& b, hipStream_t s,
+                 hipEvent_t* ev);
+
+// Decodes one frame with no device memory of its own: the intermediate buffers are a region of
+// a fixed array in the plugin's device code (kNoAllocBytes per GPU, held until the process
+// ends), for frames up to kNoAllocMaxWidth x kNoAllocMaxHeight with at most kNoAllocMaxBytes
+// of codestream. data may be host or device memory; dst must be device memory. Synchronous:
+// returns when the picture is in dst. Calls on one GPU share the array; the library takes
+// them one at a time.
+constexpr int kNoAllocMaxWidth = 2048, kNoAllocMaxHeight = 1080;
+constexpr size_t kNoAllocMaxBytes = 1536 * 1024;
+void decodeNoAlloc(int device, const Layout& frame, int reduce, bool highBits, const uint8_t* data, size_t size,
+                   const DecCode* blocks, size_t numBlocks, void* dst, ptrdiff_t strideBytes, hipStream_t stream);
+
 class Decoder {
 public:
     // reduce: resolution levels left out (0..levels); highBits: codes << 4 in each word.
@@ -64,7 +108,7 @@ private:
     bool highBits_;
     int outW_ = 0, outH_ = 0;
     std::vector<int> ws_, hs_, wanted_;
-    std::vector<uint32_t> order_, count_;
+    DescPlan plan_;
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_[4] = {};
     uint32_t* dWords_ = nullptr;

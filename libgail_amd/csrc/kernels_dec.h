@@ -70,5 +70,7 @@ void idwt_tile_size(int& tx, int& ty);
 hipError_t launch_idwt_level(const IdwtLevel& p, hipStream_t s);
 hipError_t launch_dequant_frame(const DequantFrame& p, hipStream_t s);
 hipError_t launch_ict_pack(const IctPack& p, hipStream_t s);
+// Zeroes n 32-bit words at p (16-byte aligned) with a kernel rather than a memset node.
+hipError_t launch_zero_words(uint32_t* p, size_t n, hipStream_t s);
 
 }  // namespace gail_amd

@@ -20,6 +20,7 @@
 // Inverse ICT: one thread per pixel, level shift, rounding, clamping to 12 bits, packing.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "kernels_dec.h"
@@ -330,6 +331,24 @@ hipError_t dec_tables_init() {
     uint32_t tab[47];
     build_mq_table(tab);
     return hipMemcpyToSymbol(HIP_SYMBOL(kDecMq), tab, sizeof tab);
+}
+
+// Zeroes n words at p (16-byte aligned): four words per store, a grid-stride loop.
+__global__ void __launch_bounds__(256) zero_words_kernel(uint32_t* p, uint64_t n) {
+    const uint64_t quads = n / 4, step = (uint64_t)gridDim.x * blockDim.x;
+    uint4* q = reinterpret_cast<uint4*>(p);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < quads; i += step) q[i] = make_uint4(0, 0, 0, 0);
+    const uint64_t t = quads * 4 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && t < n) p[t] = 0u;
+}
+
+hipError_t launch_zero_words(uint32_t* p, size_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    if ((uintptr_t)p % 16) return hipErrorInvalidValue;
+    const uint64_t quads = n / 4;
+    const unsigned groups = (unsigned)std::min<uint64_t>(4096, (quads + 255) / 256 + 1);
+    hipLaunchKernelGGL(zero_words_kernel, dim3(groups), dim3(256), 0, s, p, (uint64_t)n);
+    return hipGetLastError();
 }
 
 hipError_t launch_t1_decode(const T1DecodeParams& p, hipStream_t s) {

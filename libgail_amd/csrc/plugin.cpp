@@ -620,3 +620,13 @@ const GailHipPlugin g_plugin = {
 }  // namespace
 
 extern "C" const GailHipPlugin* gail_hip_plugin_get(uint32_t abi) { return abi == GAIL_HIP_PLUGIN_ABI ? &g_plugin : nullptr; }
+
+extern "C" __attribute__((visibility("default"))) int gail_hip_decode_noalloc(
+    int device, const GailHipLayout* frame, int reduce, int highBits, const uint8_t* data, size_t size,
+    const GailHipCblkCode* blocks, size_t numBlocks, void* dst, ptrdiff_t strideBytes, void* stream) {
+    return guarded([&] {
+        if (!frame) throw std::logic_error("null frame layout");
+        decodeNoAlloc(device, toLayout(*frame, kDecodeMaxMb), reduce, highBits != 0, data, size,
+                      reinterpret_cast<const DecCode*>(blocks), numBlocks, dst, strideBytes, (hipStream_t)stream);
+    });
+}
