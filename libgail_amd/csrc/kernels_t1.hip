@@ -23,6 +23,7 @@
 
 #include "device_types.h"
 #include "kernels.h"
+#include "t1_cm_masks.h"
 #include "t1_tables.h"
 
 namespace gail_amd {
@@ -30,11 +31,6 @@ namespace gail_amd {
 namespace {
 
 // ---------------------------------------------------------------- helpers
-__device__ __forceinline__ uint32_t bitv(uint32_t m, int r) { return (m >> r) & 1u; }
-// Bit r-1 and r+1 of m, zero outside 0..31.
-__device__ __forceinline__ uint32_t above(uint32_t m, int r) { return ((m << 1) >> r) & 1u; }
-__device__ __forceinline__ uint32_t below(uint32_t m, int r) { return ((m >> 1) >> r) & 1u; }
-
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -52,12 +48,6 @@ __device__ __forceinline__ uint32_t from_left(uint32_t v, int x) {
 __device__ __forceinline__ uint32_t from_right(uint32_t v, int x) {
     const uint32_t r = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xF, 0xF, true);
     return x == 31 ? 0u : r;
-}
-
-// Any set bit among the 8 neighbours of each row, from the column's own mask S and the
-// left / right columns' masks.
-__device__ __forceinline__ uint32_t neighbour_any(uint32_t S, uint32_t L, uint32_t R) {
-    return (S << 1) | (S >> 1) | L | (L << 1) | (L >> 1) | R | (R << 1) | (R >> 1);
 }
 
 __device__ __forceinline__ int half_sum_i(int v) {
@@ -79,80 +69,24 @@ __device__ __forceinline__ uint32_t half_max_u(uint32_t v) {
     return v;
 }
 
-__device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+// Inclusive prefix sum over the lanes of each half-wave (lanes 0-31 and 32-63 separately):
+// four row shifts within each 16-lane row, then row 0's (row 2's) last lane added to row 1
+// (row 3). Lanes without a source add zero.
+__device__ __forceinline__ uint32_t half_scan(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15, rows 1, 3
+    return v;
 }
 
-// Table D.1 for the four orientations, indexed [orient][code] by the 7-bit neighbourhood
-// code of zc_codes (h = h1 + 2 h2, v = v1 + 2 v2, d = d0 + 2 d1 + 4 d2).
-__constant__ uint8_t kZcTable[4 * 128];
-
-struct Emitter {
-    unsigned long long acc;
-    uint32_t n;
-    __device__ __forceinline__ void put(uint32_t cx, uint32_t d) { put_sym((cx << 1) | d); }
-    __device__ __forceinline__ void put_sym(uint32_t sym) {
-        acc |= (unsigned long long)sym << (6 * n);
-        ++n;
-    }
-};
-
-// Sign-coding contexts (D.3.2, Table D.3) of every row of a column at once, from the
-// neighbours' significance at the time of coding (CL, CR, CU, CD: row masks of the left,
-// right, above and below neighbours; CU / CD are read one row up / down) and the signs of
-// the left, right and own columns (sU / sD: own signs moved one row down / up). Per row,
-// with hc, vc the clamped horizontal and vertical contributions:
-//   context 9 + (c0 | c1 << 1 | c2 << 2): 13 when hc = vc != 0, 12 when hc != 0 = vc,
-//   11 when hc = -vc != 0, 10 when hc = 0 != vc, 9 when both are 0;
-//   coded bit d = sign ^ (hc < 0 or (hc = 0 and vc < 0)).
-struct SignMasks {
-    uint32_t c0, c1, c2, d;
-};
-__device__ __forceinline__ SignMasks sign_masks(uint32_t CL, uint32_t CR, uint32_t CU, uint32_t CD, uint32_t sL,
-                                                uint32_t sR, uint32_t sC, uint32_t sU, uint32_t sD) {
-    const uint32_t lp = CL & ~sL, ln = CL & sL, rp = CR & ~sR, rn = CR & sR;
-    const uint32_t hP = (lp & ~rn) | (rp & ~ln), hN = (ln & ~rp) | (rn & ~lp);
-    const uint32_t U = CU << 1, D = CD >> 1;
-    const uint32_t up = U & ~sU, un = U & sU, dp = D & ~sD, dn = D & sD;
-    const uint32_t vP = (up & ~dn) | (dp & ~un), vN = (un & ~dp) | (dn & ~up);
-    const uint32_t H = hP | hN, V = vP | vN, same = (hP & vP) | (hN & vN);
-    return SignMasks{H ^ V, H & ~same, same, sC ^ (hN | (~H & vN))};
-}
-
-// Rows r0..r0+3 of mask m moved to bits 0, 8, 16 and 24 (one byte per row of a stripe).
-// Significance propagation down one column of a stripe (D.3.1 scan order): a candidate row
-// (insignificant, inside the stripe) is coded when it has a significant neighbour, and a coded
-// row that turns significant (its bit in Bp) makes the row below a neighbour in turn. seed:
-// the rows coded on their own neighbours. The rows reached from a seed through a chain of
-// such links are filled with one carry: in each run of link rows (a candidate below a
-// candidate that would turn significant) the rows from the first one whose upper neighbour is
-// coded to the end of the run are coded; adding the run's lowest bit to the still-unreached
-// rows of the run clears exactly the rows below that first one.
-__device__ __forceinline__ uint32_t column_fill(uint32_t seed, uint32_t cand, uint32_t Bp) {
-    const uint32_t link = cand & ((cand & Bp) << 1);  // row r can be reached from row r - 1
-    const uint32_t unseeded = link & ~(seed << 1);
-    const uint32_t runStart = link & ~(link << 1);
-    const uint32_t below = ((unseeded + (runStart & unseeded)) ^ unseeded) & unseeded;
-    return seed | (link & ~below);
-}
-
-__device__ __forceinline__ uint32_t spread4(uint32_t m, int r0) { return (((m >> r0) & 0xFu) * 0x204081u) & 0x01010101u; }
-
-// Zero-coding neighbourhoods (D.3.1) of the four rows of stripe r0, one 7-bit code per byte:
-// h1 | h2 << 1 (horizontal count h), v1 | v2 << 2 (vertical v), d0 | d1 << 4 | d2 << 5
-// (diagonal d), each a bit-sliced sum over whole row masks.
-__device__ __forceinline__ uint32_t zc_codes(uint32_t CL, uint32_t CR, uint32_t CU, uint32_t CD, int r0) {
-    const uint32_t U = CU << 1, D = CD >> 1;
-    const uint32_t a = CL << 1, b = CL >> 1, c = CR << 1, e = CR >> 1;
-    const uint32_t s1 = a ^ b, c1 = a & b, s2 = c ^ e, c2 = c & e;  // d = s1 + s2 + 2 (c1 + c2)
-    return spread4(CL ^ CR, r0) | spread4(CL & CR, r0) << 1 | spread4(U ^ D, r0) << 2 | spread4(U & D, r0) << 3 |
-           spread4(s1 ^ s2, r0) << 4 | spread4(c1 ^ c2 ^ (s1 & s2), r0) << 5 | spread4(c1 & c2, r0) << 6;
-}
-
-// Sign-coding symbols of the four rows of stripe r0, one per byte: 18 + the byte is the
-// symbol (context 9 + c, bit d).
-__device__ __forceinline__ uint32_t sign_codes(const SignMasks& m, int r0) {
-    return spread4(m.d, r0) | spread4(m.c0, r0) << 1 | spread4(m.c1, r0) << 2 | spread4(m.c2, r0) << 3;
+// Rows r0..r0+3 of mask m as bit 0 of bytes 0..3. The product is kept opaque so that a shift
+// applied afterwards is not folded into the multiplier (a 24-bit multiply stays one).
+__device__ __forceinline__ uint32_t stripe_bytes(uint32_t m, int r0) {
+    uint32_t t = __umul24((m >> r0) & 0xFu, 0x204081u) & 0x01010101u;
+    asm("" : "+v"(t));
+    return t;
 }
 
 constexpr int kChunkBytes = 65536;  // symbol pool grabbed per wave at a time
@@ -203,17 +137,17 @@ hipError_t launch_quant_debug(const T1Block* blocks, const T1Band* bands, const 
 // ---------------------------------------------------------------- context modelling
 template <int WAVES>
 __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
-    __shared__ __attribute__((aligned(16))) uint8_t s_sym[WAVES][2][kPlaneSymCap];
+    // a plane's symbols per block, then 128 bytes where each lane writes the symbols its rows
+    // do not have (so that every lane stores on every step)
+    __shared__ __attribute__((aligned(16))) uint8_t s_sym[WAVES][2][kPlaneSymCap + 128];
     // Early-termination counts per context of the current plane (d = 0 low half, d = 1 high
     // half) and the contexts' entropy terms.
     __shared__ uint32_t s_hist[WAVES][2][32];
     __shared__ __attribute__((aligned(16))) float s_term[WAVES][2][20];
-    __shared__ uint8_t s_zc[4 * 128];
-    for (int i = threadIdx.x; i < 4 * 128; i += WAVES * 64) s_zc[i] = kZcTable[i];
-    __syncthreads();
 
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, half = lane >> 5, x = lane & 31;
     uint8_t* const sym = s_sym[wv][half];
+    uint8_t* const junk = sym + kPlaneSymCap + 4 * x;
     uint32_t* const hist = s_hist[wv][half];
     float* const terms = s_term[wv][half];
     unsigned long long chunk_cur = 0, chunk_end = 0;  // wave-uniform sub-allocator
@@ -230,7 +164,7 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
         const int w = has ? blk.w : 0, h = has ? blk.h : 0;
         const T1Band band = P.bands[has ? blk.band : 0];
         const T1FrameRef fr = P.frames[has ? blk.frame : 0];
-        const uint8_t* zc = s_zc + 128 * (has ? blk.orient : 0);
+        const bool hl = has && blk.orient == 1, hh = has && blk.orient == 3;
 
         // Fused dead-zone quantisation of the block's column (quant.h:20-25).
         uint32_t m[32];
@@ -310,141 +244,148 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
             for (int type = (j == 0 ? 2 : 0); type < 3; ++type) {
                 const bool pact = act && np < P.max_passes;
                 const uint32_t colMask = (pact && x < w) ? rowsValid : 0u;
-                uint32_t new_sig = 0;  // samples that became significant in this pass
+                const uint32_t SL = from_left(S, x), SR = from_right(S, x);
+                // The pass over the whole column: keep0 the rows whose zero-coding (or
+                // refinement) symbol is written, keep1 those whose sign symbol is written,
+                // rl the run-length symbols of each stripe (a nibble per stripe); m0 / m1 the
+                // bits of those symbol bytes, one mask per bit.
+                uint32_t keep0 = 0, keep1 = 0, runs = 0, rl = 0, add0 = 0;
+                uint32_t m0[5] = {Bp, 0, 0, 0, 0}, m1[4] = {0, 0, 0, 0};
                 int dist = 0;
                 if (type == 1) {
-                    // ---- magnitude refinement (D.3.3): samples significant before this plane
-                    const uint32_t nAny = neighbour_any(S, from_left(S, x), from_right(S, x));
-                    const uint32_t coded_all = sig0 & colMask;
-                    for (int s = 0; s < nstripes; ++s) {
-                        const int r0 = 4 * s;
-                        const uint32_t coded = coded_all & (0xFu << r0);
-                        if (!__any(coded != 0)) continue;
-                        // per row byte: bit | (context - 14) << 1, context 16 if refined before,
-                        // else 15 if a neighbour is significant, else 14; the symbol is 28 + it
-                        const uint32_t M = spread4(Bp, r0) | spread4(nAny & ~R, r0) << 1 | spread4(R, r0) << 2;
-                        Emitter em{0, 0};
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            const int r = r0 + k;
-                            if (!bitv(coded, r)) continue;
-                            em.put_sym(28u + ((M >> (8 * k)) & 7u));
-                        }
-                        const unsigned long long b0 = __ballot(em.n & 1u), b1 = __ballot(em.n & 2u);
-                        const unsigned long long b2 = __ballot(em.n & 4u);
-                        const uint32_t lot = __popcll(b0 & 0xFFFFFFFFull) + 2 * __popcll(b1 & 0xFFFFFFFFull) +
-                                             4 * __popcll(b2 & 0xFFFFFFFFull);
-                        const uint32_t all = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
-                        uint32_t excl = lanes_below(b0) + 2 * lanes_below(b1) + 4 * lanes_below(b2);
-                        if (half) excl -= lot;
-                        for (uint32_t k = 0; k < em.n; ++k) {
-                            const uint32_t sv = (uint32_t)(em.acc >> (6 * k)) & 63u;
-                            sym[plane_count + excl + k] = (uint8_t)sv;
-                        }
-                        plane_count += half ? all - lot : lot;
-                    }
+                    // ---- magnitude refinement (D.3.3): samples significant before this plane;
+                    // symbol 28 + (bit | (context - 14) << 1), context 16 if refined before,
+                    // else 15 if a neighbour is significant, else 14
+                    keep0 = sig0 & colMask;
+                    m0[1] = neighbour_any(S, SL, SR) & ~R;
+                    m0[2] = R;
+                    add0 = 0x1C1C1C1Cu;
                     // t1_dist_ref summed: 64 w - 5120 where bit p+6 is set, 3072 - 64 w elsewhere
-                    const uint32_t s1 = coded_all & Bp, s0 = coded_all & ~Bp;
+                    const uint32_t s1 = keep0 & Bp, s0 = keep0 & ~Bp;
                     dist = 64 * window_sum(s1) - 5120 * __popc(s1) + 3072 * __popc(s0) - 64 * window_sum(s0);
-                    R |= coded_all;
+                    R |= keep0;
                 } else {
+                    uint32_t N = 0;  // samples the pass turns significant
+                    if (type == 0) {
+                        // ---- significance propagation (D.3.1): the least fixed point of "an
+                        // insignificant row is coded when a neighbour is significant as the scan
+                        // sees it"; each round every lane fills its column given the other
+                        // columns' current new significance, until no lane's set changes
+                        const uint32_t open = ~S & colMask;
+                        const uint32_t around = neighbour_any(S, SL, SR);
+                        uint32_t E = 0;
+                        if (__any((open & around) != 0)) {
+                            for (;;) {
+                                const uint32_t seen = around | new_seen(from_left(N, x), from_right(N, x));
+                                E = column_fill(open & seen, open, Bp);
+                                const uint32_t turned = E & Bp;
+                                if (!__any(turned != N)) break;
+                                N = turned;
+                            }
+                        }
+                        keep0 = E;
+                        V |= E;
+                    } else {
+                        // ---- cleanup with run-length mode (D.3.4): every remaining sample
+                        const uint32_t Cc = colMask & ~S & ~V;
+                        N = Cc & Bp;
+                        runs = run_stripes(Cc, S, SL, SR, N, from_left(N, x), from_right(N, x));
+                        cleanup_run_symbols(Cc, N, runs, keep0, rl);
+                    }
+                    keep1 = N;
+                    if (__any((keep0 | keep1 | rl) != 0)) {
+                        const Seen v = seen_at_coding(S, SL, SR, N, from_left(N, x), from_right(N, x));
+                        const ZcCtx z = zc_context(v, hl, hh);
+                        const ScCtx g = sign_context(v, sgnL, sgnR, sgn, sgnU, sgnD);
+                        m0[1] = z.c0;
+                        m0[2] = z.c1;
+                        m0[3] = z.c2;
+                        m0[4] = z.c3;
+                        m1[0] = g.d;
+                        m1[1] = g.c0;
+                        m1[2] = g.c1;
+                        m1[3] = g.c2;
+                    }
+                    S |= N;
+                    // t1_dist_sig summed over the new significances: 192 w - 9216 each
+                    dist = 192 * window_sum(N) - 9216 * __popc(N);
+                }
+                if (__any((keep0 | keep1 | rl) != 0)) {
+                    // symbols per stripe (a nibble each), placed stripe after stripe and, within
+                    // a stripe, column after column by an exclusive prefix over the half-wave;
+                    // stripes k and k + 4 share a register, 16 bits each
+                    const uint32_t cnt = nibble_counts(keep0) + nibble_counts(keep1) + rl;
+                    uint32_t e0, e1, e2, e3, t0, t1, t2, t3, u0, u1, u2, u3;
+                    {
+                        uint32_t f = cnt & 0x000F000Fu, inc = half_scan(f);
+                        e0 = inc - f;
+                        t0 = (uint32_t)__builtin_amdgcn_readlane((int)inc, 31);
+                        u0 = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+                        f = (cnt >> 4) & 0x000F000Fu;
+                        inc = half_scan(f);
+                        e1 = inc - f;
+                        t1 = (uint32_t)__builtin_amdgcn_readlane((int)inc, 31);
+                        u1 = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+                        f = (cnt >> 8) & 0x000F000Fu;
+                        inc = half_scan(f);
+                        e2 = inc - f;
+                        t2 = (uint32_t)__builtin_amdgcn_readlane((int)inc, 31);
+                        u2 = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+                        f = (cnt >> 12) & 0x000F000Fu;
+                        inc = half_scan(f);
+                        e3 = inc - f;
+                        t3 = (uint32_t)__builtin_amdgcn_readlane((int)inc, 31);
+                        u3 = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+                    }
+                    uint32_t base = plane_count;
                     for (int s = 0; s < nstripes; ++s) {
+                        // this stripe's prefix and half-wave totals, then move the next ones up
+                        const uint32_t ex = e0 & 0xFFFFu, tlo = t0 & 0xFFFFu, thi = u0 & 0xFFFFu;
+                        {
+                            const uint32_t a = e0 >> 16, b = t0 >> 16, c = u0 >> 16;
+                            e0 = e1; e1 = e2; e2 = e3; e3 = a;
+                            t0 = t1; t1 = t2; t2 = t3; t3 = b;
+                            u0 = u1; u1 = u2; u2 = u3; u3 = c;
+                        }
+                        if (tlo + thi == 0) continue;
+                        uint32_t pos = base + ex;
+                        base += half ? thi : tlo;
                         const int r0 = 4 * s;
-                        const uint32_t SM = (0xFu << r0) & colMask;
-                        const uint32_t SL = from_left(S, x), SR = from_right(S, x);
-                        Emitter em{0, 0};
-                        bool coded = false;  // wave-uniform: some lane codes in this stripe
-                        if (type == 0) {
-                            // ---- significance propagation (D.3.1)
-                            const uint32_t open = ~S & SM;  // rows this pass may code
-                            const uint32_t sigAround = neighbour_any(S, SL, SR);
-                            if (__any((open & sigAround) != 0)) {
-                                coded = true;
-                                // Columns are scanned left to right, so a column also sees the
-                                // samples its left neighbour turned significant in this pass:
-                                // each round every lane codes its column given the left lane's
-                                // current turned set, until no lane's set changes (the least
-                                // fixed point, reached within one round per column of a chain)
-                                uint32_t NEW = 0, E;
-                                for (;;) {
-                                    const uint32_t left = from_left(NEW, x);
-                                    E = column_fill(open & (sigAround | left | (left << 1) | (left >> 1)), open, Bp);
-                                    const uint32_t turned = E & Bp;
-                                    if (!__any(turned != NEW)) break;
-                                    NEW = turned;
-                                }
-                                const uint32_t NL = from_left(NEW, x);
-                                const uint32_t CL = SL | NL, CR = SR, CU = S | NEW, CD = S;
-                                const uint32_t Z = zc_codes(CL, CR, CU, CD, r0);
-                                const uint32_t G = sign_codes(sign_masks(CL, CR, CU, CD, sgnL, sgnR, sgn, sgnU, sgnD), r0);
+                        const uint32_t z = (keep0 >> r0) & 0xFu, gn = (keep1 >> r0) & 0xFu;
+                        const uint32_t w0 = add0 + (stripe_bytes(m0[0], r0) | stripe_bytes(m0[1], r0) << 1 |
+                                                    stripe_bytes(m0[2], r0) << 2 | stripe_bytes(m0[3], r0) << 3 |
+                                                    stripe_bytes(m0[4], r0) << 4);
+                        if (type == 2 && __any(((runs >> r0) & 1u) != 0)) {
+                            // run-length symbols: context 17 with whether a row turns
+                            // significant, then that row's position in two context-18 bits
+                            const bool run = (runs >> r0) & 1u, hit = run && gn != 0;
+                            const uint32_t first = gn ? (uint32_t)__builtin_ctz(gn) : 0u;
+                            *(run ? sym + pos : junk) = (uint8_t)(34u | (gn != 0));
+                            *(hit ? sym + pos + 1 : junk) = (uint8_t)(36u | (first >> 1));
+                            *(hit ? sym + pos + 2 : junk) = (uint8_t)(36u | (first & 1u));
+                            pos += run ? (hit ? 3u : 1u) : 0u;
+                        }
+                        if (type == 1) {
 #pragma unroll
-                                for (int k = 0; k < 4; ++k) {
-                                    const int r = r0 + k;
-                                    if (!bitv(E, r)) continue;
-                                    const uint32_t b = bitv(Bp, r);
-                                    em.put(zc[(Z >> (8 * k)) & 0x7Fu], b);
-                                    if (b) em.put_sym(18u + ((G >> (8 * k)) & 0xFu));
-                                }
-                                S |= NEW;
-                                V |= E;
-                                new_sig |= NEW;
+                            for (int k = 0; k < 4; ++k) {
+                                const uint32_t zk = (z >> k) & 1u;
+                                *(zk ? sym + pos : junk) = (uint8_t)(w0 >> (8 * k));
+                                pos += zk;
                             }
                         } else {
-                            // ---- cleanup with run-length mode (D.3.4)
-                            const uint32_t Cc = SM & ~S & ~V;
-                            if (__any(Cc != 0)) {
-                                coded = true;
-                                const uint32_t NEWc = Cc & Bp;
-                                const uint32_t NLc = from_left(NEWc, x);
-                                const uint32_t CL = SL | NLc, CR = SR, CU = S | NEWc, CD = S;
-                                const uint32_t Z = zc_codes(CL, CR, CU, CD, r0);
-                                const uint32_t G = sign_codes(sign_masks(CL, CR, CU, CD, sgnL, sgnR, sgn, sgnU, sgnD), r0);
-                                bool run = ((Cc >> r0) & 0xFu) == 0xFu && r0 + 3 < h;
-                                if (run) {
-                                    // no significant neighbour in the 3 x 6 window around the column's stripe
-                                    const uint32_t win = (r0 == 0 ? 0x1Fu : (0x3Fu << (r0 - 1)));
-                                    run = ((CL | CR) & win) == 0 && above(S, r0) == 0 && below(S, r0 + 3) == 0;
-                                }
-                                int first = 4;
-                                if (run) {
-                                    const uint32_t nb = (NEWc >> r0) & 0xFu;
-                                    if (nb) first = __builtin_ctz(nb);
-                                    em.put(17u, nb ? 1u : 0u);
-                                    if (nb) {
-                                        em.put(18u, (uint32_t)first >> 1);
-                                        em.put(18u, (uint32_t)first & 1u);
-                                    }
-                                }
+                            const uint32_t w1 = 0x12121212u + (stripe_bytes(m1[0], r0) | stripe_bytes(m1[1], r0) << 1 |
+                                                               stripe_bytes(m1[2], r0) << 2 | stripe_bytes(m1[3], r0) << 3);
 #pragma unroll
-                                for (int k = 0; k < 4; ++k) {
-                                    const int r = r0 + k;
-                                    if (run && k < first) continue;
-                                    if (!bitv(Cc, r)) continue;
-                                    const uint32_t b = bitv(Bp, r);
-                                    if (!(run && k == first)) em.put(zc[(Z >> (8 * k)) & 0x7Fu], b);
-                                    if (b) em.put_sym(18u + ((G >> (8 * k)) & 0xFu));
-                                }
-                                S |= NEWc;
-                                new_sig |= NEWc;
+                            for (int k = 0; k < 4; ++k) {
+                                const uint32_t zk = (z >> k) & 1u, gk = (gn >> k) & 1u;
+                                *(zk ? sym + pos : junk) = (uint8_t)(w0 >> (8 * k));
+                                pos += zk;
+                                *(gk ? sym + pos : junk) = (uint8_t)(w1 >> (8 * k));
+                                pos += gk;
                             }
                         }
-                        if (!coded) continue;
-                        // the stripe's symbols, in column order, after the plane's earlier ones
-                        const unsigned long long b0 = __ballot(em.n & 1u), b1 = __ballot(em.n & 2u);
-                        const unsigned long long b2 = __ballot(em.n & 4u), b3 = __ballot(em.n & 8u);
-                        const uint32_t lot = __popcll(b0 & 0xFFFFFFFFull) + 2 * __popcll(b1 & 0xFFFFFFFFull) +
-                                             4 * __popcll(b2 & 0xFFFFFFFFull) + 8 * __popcll(b3 & 0xFFFFFFFFull);
-                        const uint32_t all = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2) + 8 * __popcll(b3);
-                        uint32_t excl = lanes_below(b0) + 2 * lanes_below(b1) + 4 * lanes_below(b2) + 8 * lanes_below(b3);
-                        if (half) excl -= lot;
-                        for (uint32_t k = 0; k < em.n; ++k) {
-                            const uint32_t sv = (uint32_t)(em.acc >> (6 * k)) & 63u;
-                            sym[plane_count + excl + k] = (uint8_t)sv;
-                        }
-                        plane_count += half ? all - lot : lot;
                     }
-                    // t1_dist_sig summed over the new significances: 192 w - 9216 each
-                    dist = 192 * window_sum(new_sig) - 9216 * __popc(new_sig);
+                    plane_count = base;
                 }
                 const int total = half_sum_i(dist);
                 if (pact) {
@@ -594,12 +535,6 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
             if (nbps > kMaxPlanes) atomicOr(&P.counters->flags, 4u);
         }
     }
-}
-
-static hipError_t upload_zc_table() {
-    uint8_t z[4 * 128];
-    build_zc_table(z);
-    return hipMemcpyToSymbol(HIP_SYMBOL(kZcTable), z, sizeof z);
 }
 
 hipError_t launch_t1_cm(const CmParams& p, int waves, int groups, hipStream_t s) {
@@ -1175,10 +1110,7 @@ static hipError_t upload_mq_table() {
     return hipMemcpyToSymbol(HIP_SYMBOL(kMqTable), tab, sizeof tab);
 }
 
-hipError_t t1_tables_init() {
-    const hipError_t e = upload_zc_table();
-    return e != hipSuccess ? e : upload_mq_table();
-}
+hipError_t t1_tables_init() { return upload_mq_table(); }
 
 hipError_t launch_t1_mq(const MqParams& p, int group, int read_bytes, int groups, hipStream_t s) {
     const bool wide = read_bytes >= 64;
