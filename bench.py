@@ -505,6 +505,9 @@ def main():
     ap.add_argument("--decode-frames", type=int, default=8, help="distinct 4K frames of the decoder measurement")
     ap.add_argument("--decoders", type=int, default=8)
     ap.add_argument("--decode-only", action="store_true", help="only the decoder measurement")
+    ap.add_argument("--stream-inflight", type=int, default=128, help="frames in flight in the headline stream")
+    ap.add_argument("--stream-only", action="store_true",
+                    help="only the headline stream (pinned host memory) and its HBM-resident twin")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU figures are not part of it "
@@ -522,6 +525,19 @@ def main():
     frames = [g.synth_frame(1000 + i, W, H, threads=16) for i in range(B)]
     if args.pmc_probe:
         pmc_probe(g, np, W, H, fourk, frames, dwt_tile=args.pmc_tile)
+        return
+    if args.stream_only:
+        steps, warm, per = max(args.steps, 1), max(args.warmup, 0), max(args.step_frames, 1)
+        sb = StreamBench(g, np)
+        try:
+            out = {"inflight": args.stream_inflight,
+                   "stream": sb.run(24.0, steps * per, args.stream_inflight, stream=3, resident=False,
+                                    warm_frames=warm * per),
+                   "stream_hbm_resident": sb.run(24.0, steps * per, args.stream_inflight, stream=1, resident=True,
+                                                 warm_frames=warm * per)}
+        finally:
+            sb.close()
+        print(json.dumps(out))
         return
     dev = g.Device(0)
     if args.decode_only:
@@ -637,10 +653,10 @@ def main():
     sb = StreamBench(g, np)
     try:
         print(f"[bench] 4K flat stream from pinned host memory, 24 fps rate control, {warm} + {steps} steps of "
-              f"{per} frames, 128 in flight", file=sys.stderr, flush=True)
-        main_run = sb.run(24.0, steps * per, 128, stream=3, resident=False, warm_frames=warm * per)
+              f"{per} frames, {args.stream_inflight} in flight", file=sys.stderr, flush=True)
+        main_run = sb.run(24.0, steps * per, args.stream_inflight, stream=3, resident=False, warm_frames=warm * per)
         print(f"[bench] stream: {json.dumps(main_run)}", file=sys.stderr, flush=True)
-        hbm_run = sb.run(24.0, steps * per, 128, stream=1, resident=True, warm_frames=warm * per)
+        hbm_run = sb.run(24.0, steps * per, args.stream_inflight, stream=1, resident=True, warm_frames=warm * per)
         print(f"[bench] stream from HBM-resident frames: {json.dumps(hbm_run)}", file=sys.stderr, flush=True)
         stress = None
         if args.stress_frames > 0:
@@ -672,7 +688,7 @@ def main():
         "vs_baseline": None, "dtype": "f32+int", "data": "synthetic",
         "config": {"workload": "served 4K flat 3996x2160 12-bit X'Y'Z' stream, 24 fps / 250 Mbit/s rate control, "
                                "codestreams built and DCI-checked",
-                   "frames_per_step": per, "frames_in_flight": 128, "levels": layout.levels,
+                   "frames_per_step": per, "frames_in_flight": args.stream_inflight, "levels": layout.levels,
                    "code_blocks_per_frame": layout.num_cblks,
                    "input": "pinned host memory: every frame crosses PCIe inside the timed region",
                    "autotuned": chosen},

@@ -310,6 +310,10 @@ struct GailHipPlugin {
     /* Device rate control result of frame i of the last run (GAIL_HIP_MISUSE when the frame's
      * settings left rate control to the host). */
     int (*batch_rate)(GailHipBatch* b, int i, GailHipRate* out);
+    /* Blocks until every frame queued by batch_upload_from since the last run has arrived in
+     * the batch's slots. A caller that launches only then keeps the device's queues free of
+     * waits on copies, and can hand the copy engines one batch's frames at a time. */
+    int (*batch_inputs_ready)(GailHipBatch* b);
 };
 typedef struct GailHipPlugin GailHipPlugin;
 

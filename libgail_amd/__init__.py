@@ -190,6 +190,7 @@ class Plugin(C.Structure):
         ("copy", _F(C.c_int, _vp, _vp, C.c_uint64)),
         ("is_device_pointer", _F(C.c_int, _vp)),
         ("batch_rate", _F(C.c_int, _vp, C.c_int, C.POINTER(Rate))),
+        ("batch_inputs_ready", _F(C.c_int, _vp)),
     ]
 
 

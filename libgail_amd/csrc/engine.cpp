@@ -666,6 +666,11 @@ void Job::uploadFrom(int i, const void* src) {
     hipCheck(hipMemcpyAsync(sd_[i].in, src, sd_[i].inBytes, hipMemcpyDefault, stream_), "frame upload");
 }
 
+void Job::inputsReady() {
+    hipCheck(hipSetDevice(dev_), "hipSetDevice");
+    hipCheck(hipStreamSynchronize(stream_), "frame uploads");
+}
+
 void Job::uploadAsync(int i) {
     if (i < 0 || i >= numFrames()) throw std::logic_error("no frame bound to this slot");
     hipCheck(hipSetDevice(dev_), "hipSetDevice");

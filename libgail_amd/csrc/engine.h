@@ -154,6 +154,9 @@ public:
     // The same from any host or device buffer (kept valid by the caller until the run is
     // done); a device buffer is copied within HBM.
     void uploadFrom(int i, const void* src);
+    // Waits on the host until the copies queued by uploadFrom / uploadAsync have arrived
+    // (the job's stream holds nothing else before a launch).
+    void inputsReady();
     // Early termination threshold of frame i (0: code every pass).
     void setSlope(int i, double t) { slope_[i] = (float)t; }
 

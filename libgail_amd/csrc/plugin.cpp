@@ -547,6 +547,13 @@ int isDevicePointer(const void* p) {
     return a.type == hipMemoryTypeDevice ? 1 : 0;
 }
 
+int batchInputsReady(GailHipBatch* b) {
+    return guarded([&] {
+        if (!b) throw std::logic_error("null batch");
+        b->job->inputsReady();
+    });
+}
+
 int batchRate(GailHipBatch* b, int i, GailHipRate* out) {
     return guarded([&] {
         const RcFrame* r = b->job->rateResult(i);
@@ -615,6 +622,7 @@ const GailHipPlugin g_plugin = {
     copyEntry,
     isDevicePointer,
     batchRate,
+    batchInputsReady,
 };
 
 }  // namespace

@@ -157,6 +157,14 @@ struct gail_server {
     Clock::time_point tFirst, tLast;
     std::vector<double> latency, queued;
     std::thread scheduler;
+    // Jobs bound and filled, in the order the scheduler took them, with their frames' sources:
+    // the launcher thread queues one job's uploads at a time, waits until they have arrived,
+    // then launches that job, so each job's frames cross PCIe at the link's full rate and its
+    // kernels start at once while the next job's frames follow, and no device queue ever
+    // holds a wait on a copy.
+    std::deque<std::pair<JobState*, std::vector<const void*>>> toLaunch;
+    bool launcherStop = false;
+    std::thread launcher;
     std::mutex deliverMutex;  // one callback at a time
 
     void deliver(Request& r, int status, const gail_packet* pkt, double lambda) {
@@ -284,16 +292,7 @@ struct gail_server {
                 settings[i].rcMaxComp = sp.limits.maxComp;
             }
         }
-        auto failAll = [&](const std::string& why) {
-            for (auto& r : j.reqs) {
-                lastErrorText() = why;
-                deliver(*r, GAIL_ERROR_FAILED, nullptr, 0);
-            }
-            std::lock_guard<std::mutex> l(m);
-            j.reqs.clear();
-            j.busy = false;
-            cv.notify_all();
-        };
+        auto failAll = [&](const std::string& why) { failJob(j, why); };
         if (P->batch_bind(j.batch, layouts.data(), settings.data(), n) != GAIL_HIP_OK) return failAll(P->last_error());
         // host copies of frames not already in the device layout, in parallel on the pool
         std::vector<const void*> src(n, nullptr);
@@ -333,16 +332,52 @@ struct gail_server {
             fcv.wait(l, [&] { return fills == 0; });
         }
         if (!stagingError.empty()) return failAll(stagingError);
-        for (int i = 0; i < n; ++i) {
-            if (P->batch_upload_from(j.batch, i, src[i]) != GAIL_HIP_OK) return failAll(P->last_error());
-            P->batch_set_slope(j.batch, i, j.reqs[i]->slope);
-        }
-        if (P->batch_launch(j.batch, 0) != GAIL_HIP_OK) return failAll(P->last_error());
         std::lock_guard<std::mutex> l(m);
-        j.launched = true;
-        ++jobRuns;
-        jobFrames += n;
+        toLaunch.emplace_back(&j, std::move(src));
         cv.notify_all();
+    }
+
+    // Every frame of a job fails with the same error and the job is free again.
+    void failJob(JobState& j, const std::string& why) {
+        for (auto& r : j.reqs) {
+            lastErrorText() = why;
+            deliver(*r, GAIL_ERROR_FAILED, nullptr, 0);
+        }
+        std::lock_guard<std::mutex> l(m);
+        j.reqs.clear();
+        j.busy = false;
+        cv.notify_all();
+    }
+
+    void launcherLoop() {
+        for (;;) {
+            std::pair<JobState*, std::vector<const void*>> next;
+            {
+                std::unique_lock<std::mutex> l(m);
+                cv.wait(l, [&] { return launcherStop || !toLaunch.empty(); });
+                if (toLaunch.empty()) return;
+                next = std::move(toLaunch.front());
+                toLaunch.pop_front();
+            }
+            JobState& j = *next.first;
+            const int n = (int)j.reqs.size();
+            bool ok = true;
+            for (int i = 0; i < n && ok; ++i) {
+                ok = P->batch_upload_from(j.batch, i, next.second[i]) == GAIL_HIP_OK;
+                if (ok) P->batch_set_slope(j.batch, i, j.reqs[i]->slope);
+            }
+            if (ok && P->batch_inputs_ready) ok = P->batch_inputs_ready(j.batch) == GAIL_HIP_OK;
+            if (ok) ok = P->batch_launch(j.batch, 0) == GAIL_HIP_OK;
+            if (!ok) {
+                failJob(j, P->last_error());
+                continue;
+            }
+            std::lock_guard<std::mutex> l(m);
+            j.launched = true;
+            ++jobRuns;
+            jobFrames += n;
+            cv.notify_all();
+        }
     }
 
     static int classOf(const Request& r) { return r.st->spec->fourk() ? 1 : 0; }
@@ -569,9 +604,11 @@ struct gail_server {
                 if (j->busy) return false;
             return true;
         });
+        launcherStop = true;
         for (auto& j : jobs) j->retire = true;
         cv.notify_all();
         l.unlock();
+        if (launcher.joinable()) launcher.join();
         for (auto& j : jobs)
             if (j->waiter.joinable()) j->waiter.join();
         pool.reset();
@@ -644,6 +681,7 @@ GAIL_API int gail_server_create(const gail_server_config* c, gail_server** out) 
         threads = std::max(2, cores - 1);
     }
     s->pool = std::make_unique<Pool>(threads);
+    s->launcher = std::thread([sp = s.get()] { sp->launcherLoop(); });
     s->scheduler = std::thread([sp = s.get()] { sp->schedulerLoop(); });
     *out = s.release();
     return GAIL_OK;

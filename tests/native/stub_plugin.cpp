@@ -368,6 +368,7 @@ int batchWait(GailHipBatch* bh, GailHipTier1* outs, GailHipFrameTimings* timings
 }
 
 int batchQuery(GailHipBatch*) { return 1; }
+int batchInputsReady(GailHipBatch*) { return GAIL_HIP_OK; }  // stand-in uploads are synchronous
 
 // Device rate control of a stand-in frame: every pass is kept (its few bytes fit any budget).
 int batchRate(GailHipBatch* bh, int i, GailHipRate* out) {
@@ -433,6 +434,7 @@ GailHipPlugin makeTable() {
     t.host_free = hostFree;
     t.is_device_pointer = isDevicePointer;
     t.batch_rate = batchRate;
+    t.batch_inputs_ready = batchInputsReady;
     return t;  // the entries the served form does not use stay NULL
 }
 

@@ -639,8 +639,9 @@ static int test_server(const char* which) {
     gail_enumerate_devices(GAIL_BACKEND_AUTO, info, 1);  /* loads the plugin */
     stub_control_fn knob = stub_control();
     if (!strcmp(which, "mixed")) {
-        /* frames of both sizes arriving together: each class gets jobs of its own slot size */
-        gail_server* srv = make_server(&sink, 2, 4, 0);
+        /* frames of both sizes arriving together, with room for a job per two frames: each
+           class gets jobs of its own slot size (none has to be retired for the other) */
+        gail_server* srv = make_server(&sink, 2, 8, 0);
         if (!srv) return 1;
         for (int i = 0; i < 16; ++i) CHECK(submit(srv, i, i % 2) == GAIL_OK, "submit %d: %s", i, gail_last_error());
         gail_server_drain(srv);
