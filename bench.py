@@ -4,8 +4,9 @@ Headline (`value`): sustained 4K DCI encode rate of the served form, frames/s of
 codestreams: DCI 4K flat 3996x2160 12-bit X'Y'Z' frames of the seeded synthetic generator
 (NL 6, 32x32 code-blocks), 24 fps rate control at 250 Mbit/s (1,302,067-byte budget), every
 codestream built and DCI-checked. The input frames are in pinned host memory and each crosses
-PCIe inside the timed region, as the reference's pipeline uploads every frame; 128 frames in
-flight. A step is --step-frames (32) frames of the stream: --warmup untimed steps, then
+PCIe inside the timed region, as the reference's pipeline uploads every frame; up to 256
+frames in flight (--stream-inflight: every frame slot of the server's 16 jobs, 170 GB of
+HBM). A step is --step-frames (32) frames of the stream: --warmup untimed steps, then
 --steps timed ones. The same run from frames resident in HBM (copied within HBM into the job
 slots) is reported beside it. Colour/ICT, 9/7 DWT, quantisation, Tier-1 and PCRD rate control
 run on the GPU, Tier-2 on the host pool, with early termination at 0.25 x the stream's last
@@ -387,7 +388,7 @@ def cpu_config1(g, np):
 
 
 class StreamBench:
-    """One server with 16 jobs x 16 frame slots (256 4K frames, ~160 GB of HBM) for the 4K
+    """One server with 16 jobs x 16 frame slots (256 4K frames, ~170 GB of HBM) for the 4K
     flat streams: a closed loop with at most a given number of frames in flight."""
 
     def __init__(self, g, np, distinct=8):
@@ -486,7 +487,7 @@ def pmc_probe(g, np, W, H, fourk, frames, dwt_tile=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=24,
+    ap.add_argument("--steps", type=int, default=48,
                     help="timed steps of the headline stream, each --step-frames frames")
     ap.add_argument("--warmup", type=int, default=2, help="untimed steps of the stream before the timed ones")
     ap.add_argument("--step-frames", type=int, default=32)
@@ -505,7 +506,8 @@ def main():
     ap.add_argument("--decode-frames", type=int, default=8, help="distinct 4K frames of the decoder measurement")
     ap.add_argument("--decoders", type=int, default=8)
     ap.add_argument("--decode-only", action="store_true", help="only the decoder measurement")
-    ap.add_argument("--stream-inflight", type=int, default=128, help="frames in flight in the headline stream")
+    ap.add_argument("--stream-inflight", type=int, default=256,
+                    help="frames in flight in the headline stream (256: every slot of the 16 jobs)")
     ap.add_argument("--stream-only", action="store_true",
                     help="only the headline stream (pinned host memory) and its HBM-resident twin")
     args = ap.parse_args()

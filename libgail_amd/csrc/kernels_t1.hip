@@ -50,11 +50,6 @@ __device__ __forceinline__ uint32_t from_right(uint32_t v, int x) {
     return x == 31 ? 0u : r;
 }
 
-__device__ __forceinline__ int half_sum_i(int v) {
-#pragma unroll
-    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
-    return v;
-}
 __device__ __forceinline__ double half_sum_d(double v) {
 #pragma unroll
     for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
@@ -79,6 +74,14 @@ __device__ __forceinline__ uint32_t half_scan(uint32_t v) {
     v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
     v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15, rows 1, 3
     return v;
+}
+
+// Sum over the lane's half-wave, in every lane of that half (the scan's last lanes hold the
+// two sums).
+__device__ __forceinline__ int half_total(int v) {
+    const int inc = (int)half_scan((uint32_t)v);
+    const int lo = __builtin_amdgcn_readlane(inc, 31), hi = __builtin_amdgcn_readlane(inc, 63);
+    return (threadIdx.x & 32) ? hi : lo;
 }
 
 // Rows r0..r0+3 of mask m as bit 0 of bytes 0..3. The product is kept opaque so that a shift
@@ -188,9 +191,10 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
         }
         const uint32_t bmax = half_max_u(colmax) >> 6;
         const int nbps = bmax ? 32 - __clz((int)bmax) : 0;
-        const int nbps_lo = __shfl(nbps, 0), nbps_hi = __shfl(nbps, 32);
+        // the pair's plane and stripe counts, in scalar registers (wave-uniform loop bounds)
+        const int nbps_lo = __builtin_amdgcn_readlane(nbps, 0), nbps_hi = __builtin_amdgcn_readlane(nbps, 32);
         const int jmax = nbps_lo > nbps_hi ? nbps_lo : nbps_hi;
-        const int h_lo = __shfl(h, 0), h_hi = __shfl(h, 32);
+        const int h_lo = __builtin_amdgcn_readlane(h, 0), h_hi = __builtin_amdgcn_readlane(h, 32);
         const int nstripes = ((h_lo > h_hi ? h_lo : h_hi) + 3) >> 2;
         const uint32_t sgnL = from_left(sgn, x), sgnR = from_right(sgn, x);
         const uint32_t sgnU = sgn << 1, sgnD = sgn >> 1;
@@ -387,7 +391,7 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                     }
                     plane_count = base;
                 }
-                const int total = half_sum_i(dist);
+                const int total = half_total(dist);
                 if (pact) {
                     if (x == 0) {
                         P.pass_end[(size_t)cb * kMaxPasses + np] = sym_total + plane_count;
