@@ -33,7 +33,16 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes one DWT level-0 launch over 16 4K flat frames moves, per tile shape, from the
 # FETCH_SIZE and WRITE_SIZE counter runs in profiles/pmc_traffic_dwt_level0_4k_flat_batch16.md,
 # each scaled by what the counter reads for a 1 GiB device copy.
-DWT0_TRAFFIC_4K_FLAT_16 = {"64x16": 2491427489, "64x32": 2502515919}
+DWT0_TRAFFIC_4K_FLAT_16 = {"64x16": 2491427489, "64x32": 2502515919, "strip0_120x8": 2493219461}
+
+
+def dwt_level0_kernel(shape):
+    """The kernel that runs DWT level 0 for an autotuner tile shape name."""
+    if shape.startswith("strip0_"):
+        return f"dwt_strip0_kernel<level 0, {shape[len('strip0_'):]} strips, input rows loaded ahead>"
+    if shape.startswith("strip_"):
+        return f"dwt_strip_kernel<level 0, {shape[len('strip_'):]} strips>"
+    return f"dwt_level_kernel<level 0, {shape} tiles>"
 
 
 def dwt_level0_bytes(W, H):
@@ -502,12 +511,13 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=5.0)
     ap.add_argument("--mixed-steps", type=int, default=6)
     ap.add_argument("--pmc-probe", action="store_true", help="a 1 GiB copy and one batch, for counter runs")
-    ap.add_argument("--pmc-tile", type=int, default=3, help="DWT tile shape of the counter probe's batch")
+    ap.add_argument("--pmc-tile", type=int, default=8, help="DWT tile shape of the counter probe's batch")
     ap.add_argument("--decode-frames", type=int, default=8, help="distinct 4K frames of the decoder measurement")
     ap.add_argument("--decoders", type=int, default=8)
     ap.add_argument("--decode-only", action="store_true", help="only the decoder measurement")
     ap.add_argument("--stream-inflight", type=int, default=256,
                     help="frames in flight in the headline stream (256: every slot of the 16 jobs)")
+    ap.add_argument("--tune-only", action="store_true", help="only the autotuner's kernel times on one batch")
     ap.add_argument("--stream-only", action="store_true",
                     help="only the headline stream (pinned host memory) and its HBM-resident twin")
     args = ap.parse_args()
@@ -566,6 +576,12 @@ def main():
         first.set_slope(i, thr)
     chosen, tune_ms = first.autotune(reps=2)
     print(f"[bench] autotuner: {chosen} {json.dumps(tune_ms)}", file=sys.stderr, flush=True)
+    if args.tune_only:
+        for _ in range(2):  # again, for the run-to-run spread
+            print(json.dumps({"chosen": chosen, "ms": first.autotune(reps=2)[1]}))
+        first.close()
+        dev.close()
+        return
     batches = [first] + [g.Batch(dev, [layout] * B, [g.xyz_settings(layout)] * B, use_graph=bool(args.graph),
                                  dwt_tile=tiles[chosen["dwt_tile"]], cm_waves=chosen["cm_waves"],
                                  mq_group=chosen["mq_group"], mq_read=chosen["mq_read"]) for _ in range(nb - 1)]
@@ -605,7 +621,7 @@ def main():
         t16 = DWT0_TRAFFIC_4K_FLAT_16.get(chosen["dwt_tile"]) if args.profile == "4k_flat" else None
         traffic = t16 * B // 16 if t16 else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": f"dwt_level_kernel<level 0, {chosen['dwt_tile']}>",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dwt_level0_kernel(chosen["dwt_tile"]),
                 "launch": f"{B} frames, {dwt_level0_bytes(W, H) * B} bytes algorithmic",
                 "traffic_source": "rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a 1 GiB copy"}
         copy = hbm_copy_gbs(g, dev)

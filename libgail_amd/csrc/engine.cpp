@@ -482,6 +482,8 @@ void Job::planLevels() {
     levelTiles_.assign(numLevels_, 0);
     // each level's shape: the job's, but the strips need LL regions of 2 x 2 samples at least
     levelShape_.assign(numLevels_, opt_.dwtTile);
+    if (opt_.dwtTile >= 7)  // strips with rows loaded ahead at level 0 only, LDS tiles below it
+        for (int l = 1; l < numLevels_; ++l) levelShape_[l] = 3;
     if (opt_.dwtTile >= 4)
         for (int f = 0; f < nf; ++f) {
             const Layout& L = frames_[f].layout;

@@ -38,7 +38,7 @@ struct FrameConfig {
 };
 
 struct JobOptions {
-    int dwtTile = 3;          // tile shape index (see dwt_tile_size): 64 x 16
+    int dwtTile = kDwtDefaultShape;  // tile shape index (see dwt_tile_size): level 0 in 120 x 8 strips
     int cmWaves = 4;          // waves per context-modelling workgroup
     int cmGroupsPerCU = 8;    // persistent grid size per compute unit
     int mqGroup = 256;        // MQ workgroup size
@@ -102,7 +102,8 @@ private:
 };
 
 // Stage times (ms) the autotuner measured for each choice: the DWT tile shapes of
-// dwt_tile_size (64x64, 64x32, 32x64, 64x16 LDS tiles, 120x64, 120x32, 120x128 strips);
+// dwt_tile_size (64x64, 64x32, 32x64, 64x16 LDS tiles, 120x64, 120x32, 120x128 strips,
+// 120x16 preloaded strips at level 0);
 // context-modelling waves per workgroup 1, 2, 4, 8; MQ lanes per workgroup 64, 128, 256 with
 // 16-byte symbol reads, then the same with 64-byte reads.
 struct TuneTimes {

@@ -124,9 +124,12 @@ hipError_t launch_rc(const RcParams& p, int frames, uint32_t max_frame_blocks, h
 hipError_t launch_dwt_level(bool level0, int tile_shape, const DwtLevelDesc* descs, int ndesc, int tiles,
                             hipStream_t s);
 // Tile shapes: 0 64x64, 1 64x32, 2 32x64, 3 64x16 (LDS tiles, 256 threads each), 4 120x64,
-// 5 120x32, 6 120x128 (register-streaming strips, one wave each). Levels whose LL region is
-// narrower or lower than 2 samples take the LDS tiles.
-constexpr int kDwtShapes = 7;
+// 5 120x32, 6 120x128 (register-streaming strips, one wave each); 7, 8 level 0 as strips whose
+// input rows are loaded ahead (7: 120x16, 10 rows ahead; 8: 120x8, every row up front) and the
+// other levels as 64x16 LDS tiles. Levels whose LL region is narrower or lower than 2 samples
+// take the LDS tiles.
+constexpr int kDwtShapes = 9;
+constexpr int kDwtDefaultShape = 8;
 void dwt_tile_size(int tile_shape, int& tx, int& ty);
 hipError_t launch_color(const ColorDesc& d, hipStream_t s);
 

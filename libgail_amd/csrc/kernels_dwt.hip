@@ -415,6 +415,129 @@ __global__ void __launch_bounds__(256) dwt_strip_kernel(const DwtLevelDesc* __re
     }
 }
 
+// Level 0 as strips of kStripOut x TY outputs with PRE input rows loaded ahead: a lane issues
+// the loads of its column pair (the 12-bit codes of both columns in one 32-bit word per
+// component) for the first PRE rows before it converts or lifts anything, and keeps PRE rows
+// in flight from then on, so a wave has much of its input on the way at once, and short
+// strips give a 4K batch several times as many waves as the streaming strips. The lifting is
+// the streaming strips' own, operation for operation: the ICT on each row as it enters the
+// column pipelines, the same pipeline steps, the same row lifting across lanes.
+// Global-memory views of the frame buffers (the descriptors hold generic pointers; through
+// these the compiler issues global rather than flat loads and stores, which the scalar-memory
+// wait counter does not track).
+typedef __attribute__((address_space(1))) const uint16_t GlobalU16;
+typedef __attribute__((address_space(1))) const uint32_t GlobalU32;
+typedef __attribute__((address_space(1))) float GlobalF32;
+__device__ __forceinline__ void store_global(float* p, float v) { *(GlobalF32*)p = v; }
+
+template <int TY, int PRE, int MINW>
+__global__ void __launch_bounds__(256, MINW) dwt_strip0_kernel(const DwtLevelDesc* __restrict__ descs, int ndesc, int ntiles) {
+    static_assert(TY % 2 == 0, "strips start at even rows");
+    constexpr int RY = TY + 7;  // rows y0 - 4 .. y0 + TY + 2
+    static_assert(PRE >= 3, "rows 2i + 1 and 2i + 2 must be loaded by step i");
+    const int nb = (int)gridDim.x, q = nb >> 3, rem = nb & 7;
+    const int xcd = (int)blockIdx.x & 7, j = (int)blockIdx.x >> 3;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int tileIdx = (xcd * q + min(xcd, rem) + j) * 4 + wave;
+    if (tileIdx >= ntiles) return;
+    const int lane = (int)threadIdx.x & 63;
+    const int di = find_desc(descs, ndesc, tileIdx);
+    const DwtLevelDesc d = descs[di];
+    const int t = tileIdx - (int)d.tile_base;
+    const int x0 = (t % d.tiles_x) * kStripOut, y0 = (t / d.tiles_x) * TY;
+    const int w = d.w, h = d.h;
+    const int ce = x0 - 4 + 2 * lane;
+    const bool pairs = x0 >= 4 && x0 + kStripOut + 4 <= w && ((d.src_stride | d.src_plane) & 1u) == 0;
+    const bool rowsInside = y0 >= 4 && y0 - 4 + RY <= h;  // no loaded row needs mirroring
+    const int cx0 = mirror(ce, w), cx1 = mirror(ce + 1, w);
+
+    // the codes of both columns of every row, each component's pair in one word; the row's
+    // start is the same for the whole wave, the lane's column an offset from it
+    const uint16_t* const s0 = (const uint16_t*)d.src;
+    uint32_t code[RY][3];
+    auto loadRow = [&](int r) {
+        const int y = rowsInside ? y0 - 4 + r : mirror(y0 - 4 + r, h);
+        const size_t row = (size_t)y * d.src_stride;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            GlobalU16* const rowp = (GlobalU16*)(s0 + (size_t)c * d.src_plane + row);
+            code[r][c] = pairs ? ((GlobalU32*)rowp)[(uint32_t)ce >> 1]
+                               : ((uint32_t)rowp[(uint32_t)cx0] | (uint32_t)rowp[(uint32_t)cx1] << 16);
+        }
+    };
+#pragma unroll
+    for (int r = 0; r < (PRE < RY ? PRE : RY); ++r) loadRow(r);
+    // DC shift and forward ICT of row r, as StripLoader<true>::load computes them
+    auto rowSamples = [&](int r, float (&e)[3], float (&o)[3]) {
+        // 12-bit codes (words above 4095 read as 4095, as the host copy clips them)
+        const float x0f = (float)((int)min(code[r][0] & 0xFFFFu, 4095u) - 2048);
+        const float y0f = (float)((int)min(code[r][1] & 0xFFFFu, 4095u) - 2048);
+        const float z0f = (float)((int)min(code[r][2] & 0xFFFFu, 4095u) - 2048);
+        const float x1f = (float)((int)min(code[r][0] >> 16, 4095u) - 2048);
+        const float y1f = (float)((int)min(code[r][1] >> 16, 4095u) - 2048);
+        const float z1f = (float)((int)min(code[r][2] >> 16, 4095u) - 2048);
+        e[0] = kY0[0] * x0f + kY0[1] * y0f + kY0[2] * z0f;
+        e[1] = kY1[0] * x0f + kY1[1] * y0f + kY1[2] * z0f;
+        e[2] = kY2[0] * x0f + kY2[1] * y0f + kY2[2] * z0f;
+        o[0] = kY0[0] * x1f + kY0[1] * y1f + kY0[2] * z1f;
+        o[1] = kY1[0] * x1f + kY1[1] * y1f + kY1[2] * z1f;
+        o[2] = kY2[0] * x1f + kY2[1] * y1f + kY2[2] * z1f;
+    };
+
+    const int snx = (w + 1) >> 1, sny = (h + 1) >> 1;
+    const int xo = (x0 >> 1) - 2 + lane;
+    const bool inStrip = lane >= 2 && lane < 2 + kStripOut / 2;
+    const bool lowOk = inStrip && ce < w, highOk = inStrip && ce + 1 < w;
+
+    StripColumn col[3][2];
+    {
+        float e[3], o[3];
+        rowSamples(0, e, o);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            col[c][0] = StripColumn{e[c], 0.f, 0.f, 0.f};
+            col[c][1] = StripColumn{o[c], 0.f, 0.f, 0.f};
+        }
+    }
+    // the step at k = y0 - 4 + 2 i takes rows k + 1, k + 2 (preloaded rows 2 i + 1, 2 i + 2)
+    // and finishes rows k - 2, k - 1
+#pragma unroll
+    for (int i = 0; i <= (TY + 4) / 2; ++i) {
+        const int k = y0 - 4 + 2 * i;
+        if (PRE + 2 * i < RY) loadRow(PRE + 2 * i);  // keep PRE rows ahead in flight
+        if (PRE + 2 * i + 1 < RY) loadRow(PRE + 2 * i + 1);
+        float oddE[3], oddO[3], evenE[3], evenO[3];
+        rowSamples(2 * i + 1, oddE, oddO);
+        rowSamples(2 * i + 2, evenE, evenO);
+        float lowE[3], lowO[3], highE[3], highO[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            col[c][0].step(oddE[c], evenE[c], lowE[c], highE[c]);
+            col[c][1].step(oddO[c], evenO[c], lowO[c], highO[c]);
+        }
+        const int gy = k - 2;
+        if (gy < y0 || gy >= h) continue;  // wave-uniform
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            lift_row_lanes(lowE[c], lowO[c]);
+            lift_row_lanes(highE[c], highO[c]);
+            // the rows' starts are the same for the whole wave, the lane's column an offset
+            const int oyL = gy >> 1;
+            float* const llRow = d.ll_dst + (size_t)c * d.ll_plane + (size_t)oyL * d.ll_stride;
+            float* const hlRow = d.coef + (size_t)c * d.coef_plane + (size_t)oyL * d.coef_stride + snx;
+            // (an unsigned offset: lanes left of the frame store nothing)
+            const uint32_t xu = (uint32_t)xo;
+            if (lowOk) store_global(llRow + xu, lowE[c]);
+            if (highOk) store_global(hlRow + xu, lowO[c]);
+            if (gy + 1 < h) {
+                float* const lhRow = d.coef + (size_t)c * d.coef_plane + (size_t)(sny + (gy >> 1)) * d.coef_stride;
+                if (lowOk) store_global(lhRow + xu, highE[c]);
+                if (highOk) store_global(lhRow + snx + xu, highO[c]);
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- colour
 // Source R'G'B' sample (0..1 after range handling) to the 16-bit code the CPU reference
 // rounds it to (color.cpp:136-139); the per-pixel path then works on code / 65535.
@@ -655,6 +778,12 @@ template <bool L0, int TY> static hipError_t launch_strip(const DwtLevelDesc* de
     return hipGetLastError();
 }
 
+template <int TY, int PRE, int MINW>
+static hipError_t launch_strip0(const DwtLevelDesc* descs, int n, int tiles, hipStream_t s) {
+    hipLaunchKernelGGL((dwt_strip0_kernel<TY, PRE, MINW>), dim3((tiles + 3) / 4), dim3(256), 0, s, descs, n, tiles);
+    return hipGetLastError();
+}
+
 hipError_t launch_dwt_level(bool level0, int tile_shape, const DwtLevelDesc* descs, int ndesc, int tiles,
                             hipStream_t s) {
     switch (tile_shape) {
@@ -664,14 +793,17 @@ hipError_t launch_dwt_level(bool level0, int tile_shape, const DwtLevelDesc* des
     case 4: return level0 ? launch_strip<true, 64>(descs, ndesc, tiles, s) : launch_strip<false, 64>(descs, ndesc, tiles, s);
     case 5: return level0 ? launch_strip<true, 32>(descs, ndesc, tiles, s) : launch_strip<false, 32>(descs, ndesc, tiles, s);
     case 6: return level0 ? launch_strip<true, 128>(descs, ndesc, tiles, s) : launch_strip<false, 128>(descs, ndesc, tiles, s);
+    case 7: return level0 ? launch_strip0<16, 10, 5>(descs, ndesc, tiles, s) : launch_dwt<false, 64, 16>(descs, ndesc, tiles, s);
+    case 8: return level0 ? launch_strip0<8, 15, 5>(descs, ndesc, tiles, s) : launch_dwt<false, 64, 16>(descs, ndesc, tiles, s);
     default: return level0 ? launch_dwt<true, 64, 16>(descs, ndesc, tiles, s) : launch_dwt<false, 64, 16>(descs, ndesc, tiles, s);
     }
 }
 
 void dwt_tile_size(int tile_shape, int& tx, int& ty) {
     static const int shapes[kDwtShapes][2] = {{64, 64}, {64, 32}, {32, 64}, {64, 16},
-                                              {kStripOut, 64}, {kStripOut, 32}, {kStripOut, 128}};
-    const int k = tile_shape >= 0 && tile_shape < kDwtShapes ? tile_shape : 3;
+                                              {kStripOut, 64}, {kStripOut, 32}, {kStripOut, 128}, {kStripOut, 16},
+                                              {kStripOut, 8}};
+    const int k = tile_shape >= 0 && tile_shape < kDwtShapes ? tile_shape : kDwtDefaultShape;
     tx = shapes[k][0];
     ty = shapes[k][1];
 }

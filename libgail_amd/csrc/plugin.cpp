@@ -515,9 +515,10 @@ int batchAutotune(GailHipBatch* b, int reps, GailHipBatchOptions* chosen, double
             chosen->poolBytesPerCoef = o.poolBytesPerCoef;
         }
         // the four LDS tile shapes first and the strips last, as the table grew
-        const double v[17] = {t.dwt[0], t.dwt[1], t.dwt[2], t.dwt[3], t.cm[0], t.cm[1], t.cm[2], t.cm[3], t.mq[0],
-                              t.mq[1],  t.mq[2],  t.mq[3],  t.mq[4],  t.mq[5], t.dwt[4], t.dwt[5], t.dwt[6]};
-        for (int i = 0; times && i < ntimes && i < 17; ++i) times[i] = v[i];
+        const double v[19] = {t.dwt[0], t.dwt[1], t.dwt[2], t.dwt[3], t.cm[0],  t.cm[1], t.cm[2],
+                              t.cm[3],  t.mq[0],  t.mq[1],  t.mq[2],  t.mq[3],  t.mq[4], t.mq[5],
+                              t.dwt[4], t.dwt[5], t.dwt[6], t.dwt[7], t.dwt[8]};
+        for (int i = 0; times && i < ntimes && i < 19; ++i) times[i] = v[i];
     });
 }
 

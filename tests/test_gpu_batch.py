@@ -214,7 +214,8 @@ def test_slot_job_rebinds(device, oracle):
 
 
 @pytest.mark.parametrize("opts", [dict(dwt_tile=0), dict(dwt_tile=1), dict(dwt_tile=2), dict(dwt_tile=4),
-                                  dict(dwt_tile=5), dict(dwt_tile=6), dict(cm_waves=1),
+                                  dict(dwt_tile=5), dict(dwt_tile=6), dict(dwt_tile=7), dict(dwt_tile=3),
+                                  dict(cm_waves=1),
                                   dict(cm_waves=2), dict(cm_waves=8), dict(mq_group=64), dict(mq_group=128),
                                   dict(mq_read=16), dict(mq_group=64, mq_read=16), dict(mq_group=128, mq_read=16)])
 def test_kernel_variants_bit_exact(device, oracle, opts):
@@ -294,7 +295,7 @@ def test_full_mixed_batch_64(device, oracle):
     assert not bad_frames, f"frames differing from the oracle (index, size, threshold, blocks): {bad_frames}"
 
 
-@pytest.mark.parametrize("shape", [3, 4, 5, 6])
+@pytest.mark.parametrize("shape", [3, 4, 5, 6, 7, 8])
 def test_dwt_planes_every_shape(device, oracle, shape):
     """The wavelet planes of every DWT tile shape, LDS tiles and register-streaming strips,
     equal the oracle's float32 DWT bit for bit on frames whose sizes leave ragged strips and
