@@ -970,7 +970,9 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
         };
         const uint32_t* pend = P.pass_end + (size_t)cb * kMaxPasses;
         MqSnapshot* snaps = P.snaps + (size_t)cb * kMaxPasses;
-        uint32_t k = 0, pos = 0, next_end = pend[0];
+        // the next pass end and the one after it: the load of each pass end is issued a
+        // whole pass before the coder reaches it
+        uint32_t k = 0, pos = 0, next_end = pend[0], after_end = np > 1 ? pend[1] : 0xFFFFFFFFu;
         const uint2* chunk = P.chunks + (size_t)cb * kMaxPlanes;
         const uint32_t planes = inf.planes;
         uint2 ch = chunk[0];
@@ -1014,7 +1016,8 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
                                 while (pos == next_end) {
                                     snaps[k] = MqSnapshot{bp, B | ((uint32_t)CT << 16), C, A};
                                     ++k;
-                                    next_end = k < np ? pend[k] : 0xFFFFFFFFu;
+                                    next_end = after_end;
+                                    after_end = k + 1 < np ? pend[k + 1] : 0xFFFFFFFFu;
                                 }
                                 const uint32_t sv = (w[b >> 2] >> (8 * (b & 3))) & 0xFFu;
                                 const uint32_t sn = b + 1 < m ? (w[(b + 1) >> 2] >> (8 * ((b + 1) & 3))) & 0xFFu : w[4];
@@ -1075,7 +1078,8 @@ __global__ void __launch_bounds__(GROUP) t1_mq_kernel(MqParams P) {
         while (pos == next_end) {
             snaps[k] = MqSnapshot{bp, B | ((uint32_t)CT << 16), C, A};
             ++k;
-            next_end = k < np ? pend[k] : 0xFFFFFFFFu;
+            next_end = after_end;
+            after_end = k + 1 < np ? pend[k + 1] : 0xFFFFFFFFu;
         }
         // FLUSH (C.2.9)
         const uint32_t tempc = C + A;
