@@ -430,39 +430,30 @@ typedef __attribute__((address_space(1))) const uint32_t GlobalU32;
 typedef __attribute__((address_space(1))) float GlobalF32;
 __device__ __forceinline__ void store_global(float* p, float v) { *(GlobalF32*)p = v; }
 
-template <int TY, int PRE, int MINW>
-__global__ void __launch_bounds__(256, MINW) dwt_strip0_kernel(const DwtLevelDesc* __restrict__ descs, int ndesc, int ntiles) {
-    static_assert(TY % 2 == 0, "strips start at even rows");
+// One strip. kInner: every loaded row and column lies inside the frame and its rows are
+// even-aligned in memory (pairs of columns in one load, rows without mirroring, every output
+// column present), so the inner strips of a frame run without any per-row or per-store test.
+template <int TY, int PRE, bool kInner>
+__device__ __forceinline__ void strip0_body(const DwtLevelDesc& d, int x0, int y0, int lane) {
     constexpr int RY = TY + 7;  // rows y0 - 4 .. y0 + TY + 2
-    static_assert(PRE >= 3, "rows 2i + 1 and 2i + 2 must be loaded by step i");
-    const int nb = (int)gridDim.x, q = nb >> 3, rem = nb & 7;
-    const int xcd = (int)blockIdx.x & 7, j = (int)blockIdx.x >> 3;
-    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    const int tileIdx = (xcd * q + min(xcd, rem) + j) * 4 + wave;
-    if (tileIdx >= ntiles) return;
-    const int lane = (int)threadIdx.x & 63;
-    const int di = find_desc(descs, ndesc, tileIdx);
-    const DwtLevelDesc d = descs[di];
-    const int t = tileIdx - (int)d.tile_base;
-    const int x0 = (t % d.tiles_x) * kStripOut, y0 = (t / d.tiles_x) * TY;
     const int w = d.w, h = d.h;
     const int ce = x0 - 4 + 2 * lane;
-    const bool pairs = x0 >= 4 && x0 + kStripOut + 4 <= w && ((d.src_stride | d.src_plane) & 1u) == 0;
-    const bool rowsInside = y0 >= 4 && y0 - 4 + RY <= h;  // no loaded row needs mirroring
-    const int cx0 = mirror(ce, w), cx1 = mirror(ce + 1, w);
+    const int cx0 = kInner ? ce : mirror(ce, w), cx1 = kInner ? ce + 1 : mirror(ce + 1, w);
 
     // the codes of both columns of every row, each component's pair in one word; the row's
     // start is the same for the whole wave, the lane's column an offset from it
     const uint16_t* const s0 = (const uint16_t*)d.src;
     uint32_t code[RY][3];
     auto loadRow = [&](int r) {
-        const int y = rowsInside ? y0 - 4 + r : mirror(y0 - 4 + r, h);
+        const int y = kInner ? y0 - 4 + r : mirror(y0 - 4 + r, h);
         const size_t row = (size_t)y * d.src_stride;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             GlobalU16* const rowp = (GlobalU16*)(s0 + (size_t)c * d.src_plane + row);
-            code[r][c] = pairs ? ((GlobalU32*)rowp)[(uint32_t)ce >> 1]
-                               : ((uint32_t)rowp[(uint32_t)cx0] | (uint32_t)rowp[(uint32_t)cx1] << 16);
+            if constexpr (kInner)
+                code[r][c] = ((GlobalU32*)rowp)[(uint32_t)ce >> 1];
+            else
+                code[r][c] = (uint32_t)rowp[(uint32_t)cx0] | (uint32_t)rowp[(uint32_t)cx1] << 16;
         }
     };
 #pragma unroll
@@ -487,7 +478,7 @@ __global__ void __launch_bounds__(256, MINW) dwt_strip0_kernel(const DwtLevelDes
     const int snx = (w + 1) >> 1, sny = (h + 1) >> 1;
     const int xo = (x0 >> 1) - 2 + lane;
     const bool inStrip = lane >= 2 && lane < 2 + kStripOut / 2;
-    const bool lowOk = inStrip && ce < w, highOk = inStrip && ce + 1 < w;
+    const bool lowOk = inStrip && (kInner || ce < w), highOk = inStrip && (kInner || ce + 1 < w);
 
     StripColumn col[3][2];
     {
@@ -516,26 +507,66 @@ __global__ void __launch_bounds__(256, MINW) dwt_strip0_kernel(const DwtLevelDes
             col[c][1].step(oddO[c], evenO[c], lowO[c], highO[c]);
         }
         const int gy = k - 2;
-        if (gy < y0 || gy >= h) continue;  // wave-uniform
+        if (2 * i < 6) continue;                 // the rows above the strip: gy < y0
+        if (!kInner && gy >= h) continue;        // wave-uniform
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             lift_row_lanes(lowE[c], lowO[c]);
             lift_row_lanes(highE[c], highO[c]);
-            // the rows' starts are the same for the whole wave, the lane's column an offset
-            const int oyL = gy >> 1;
-            float* const llRow = d.ll_dst + (size_t)c * d.ll_plane + (size_t)oyL * d.ll_stride;
-            float* const hlRow = d.coef + (size_t)c * d.coef_plane + (size_t)oyL * d.coef_stride + snx;
-            // (an unsigned offset: lanes left of the frame store nothing)
-            const uint32_t xu = (uint32_t)xo;
-            if (lowOk) store_global(llRow + xu, lowE[c]);
-            if (highOk) store_global(hlRow + xu, lowO[c]);
-            if (gy + 1 < h) {
-                float* const lhRow = d.coef + (size_t)c * d.coef_plane + (size_t)(sny + (gy >> 1)) * d.coef_stride;
-                if (lowOk) store_global(lhRow + xu, highE[c]);
-                if (highOk) store_global(lhRow + snx + xu, highO[c]);
+        }
+        // the rows' starts are the same for the whole wave, the lane's column an offset
+        // (unsigned: lanes left of the frame store nothing)
+        const uint32_t xu = (uint32_t)xo;
+        const int oyL = gy >> 1, oyH = sny + (gy >> 1);
+        const bool oddRow = kInner || gy + 1 < h;
+        if (kInner) {
+            if (inStrip) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    float* const llRow = d.ll_dst + (size_t)c * d.ll_plane + (size_t)oyL * d.ll_stride;
+                    float* const hlRow = d.coef + (size_t)c * d.coef_plane + (size_t)oyL * d.coef_stride + snx;
+                    float* const lhRow = d.coef + (size_t)c * d.coef_plane + (size_t)oyH * d.coef_stride;
+                    store_global(llRow + xu, lowE[c]);
+                    store_global(hlRow + xu, lowO[c]);
+                    store_global(lhRow + xu, highE[c]);
+                    store_global(lhRow + snx + xu, highO[c]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float* const llRow = d.ll_dst + (size_t)c * d.ll_plane + (size_t)oyL * d.ll_stride;
+                float* const hlRow = d.coef + (size_t)c * d.coef_plane + (size_t)oyL * d.coef_stride + snx;
+                if (lowOk) store_global(llRow + xu, lowE[c]);
+                if (highOk) store_global(hlRow + xu, lowO[c]);
+                if (oddRow) {
+                    float* const lhRow = d.coef + (size_t)c * d.coef_plane + (size_t)oyH * d.coef_stride;
+                    if (lowOk) store_global(lhRow + xu, highE[c]);
+                    if (highOk) store_global(lhRow + snx + xu, highO[c]);
+                }
             }
         }
     }
+}
+
+template <int TY, int PRE, int MINW>
+__global__ void __launch_bounds__(256, MINW) dwt_strip0_kernel(const DwtLevelDesc* __restrict__ descs, int ndesc, int ntiles) {
+    static_assert(TY % 2 == 0, "strips start at even rows");
+    static_assert(PRE >= 3, "rows 2i + 1 and 2i + 2 must be loaded by step i");
+    const int nb = (int)gridDim.x, q = nb >> 3, rem = nb & 7;
+    const int xcd = (int)blockIdx.x & 7, j = (int)blockIdx.x >> 3;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int tileIdx = (xcd * q + min(xcd, rem) + j) * 4 + wave;
+    if (tileIdx >= ntiles) return;
+    const int lane = (int)threadIdx.x & 63;
+    const int di = find_desc(descs, ndesc, tileIdx);
+    const DwtLevelDesc d = descs[di];
+    const int t = tileIdx - (int)d.tile_base;
+    const int x0 = (t % d.tiles_x) * kStripOut, y0 = (t / d.tiles_x) * TY;
+    const bool inner = x0 >= 4 && x0 + kStripOut + 4 <= d.w && y0 >= 4 && y0 + TY + 3 <= d.h &&
+                       ((d.src_stride | d.src_plane) & 1u) == 0;
+    if (inner) strip0_body<TY, PRE, true>(d, x0, y0, lane);
+    else strip0_body<TY, PRE, false>(d, x0, y0, lane);
 }
 
 // ---------------------------------------------------------------- colour
