@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
             for (int k = 0; k < 7; ++k) {
                 uint32_t mk = 0;
 #pragma unroll
-                for (int r = 0; r < 32; ++r) mk |= ((m[r] >> (p0 + k)) & 1u) << r;
+                for (int r = 0; r < 32; ++r) mk |= __builtin_amdgcn_ubfe(m[r], (uint32_t)(p0 + k), 1u) << r;
                 pl[k] = mk;
             }
         }
@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(WAVES * 64, 3) t1_cm_kernel(CmParams P) {
                 const int sh = p > 0 ? p : 0;
                 uint32_t mk = 0;
 #pragma unroll
-                for (int r = 0; r < 32; ++r) mk |= ((m[r] >> sh) & 1u) << r;
+                for (int r = 0; r < 32; ++r) mk |= __builtin_amdgcn_ubfe(m[r], (uint32_t)sh, 1u) << r;
                 pl[0] = act ? mk : 0u;
             }
             const uint32_t Bp = pl[6];
