@@ -38,6 +38,10 @@ DWT0_TRAFFIC_4K_FLAT_16 = {"64x16": 2491427489, "64x32": 2502515919, "strip0_120
 
 def dwt_level0_kernel(shape):
     """The kernel that runs DWT level 0 for an autotuner tile shape name."""
+    if shape.startswith("stage0_"):
+        return f"dwt_stage0_kernel<level 0, {shape[len('stage0_'):]} groups of 120x8 strips, line stores through LDS>"
+    if shape.startswith("strip0b_"):
+        return f"dwt_strip0b_kernel<level 0, {shape[len('strip0b_'):]} strips, input rows loaded ahead, buffer addressing>"
     if shape.startswith("strip0_"):
         return f"dwt_strip0_kernel<level 0, {shape[len('strip0_'):]} strips, input rows loaded ahead>"
     if shape.startswith("strip_"):
@@ -511,7 +515,7 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=5.0)
     ap.add_argument("--mixed-steps", type=int, default=6)
     ap.add_argument("--pmc-probe", action="store_true", help="a 1 GiB copy and one batch, for counter runs")
-    ap.add_argument("--pmc-tile", type=int, default=8, help="DWT tile shape of the counter probe's batch")
+    ap.add_argument("--pmc-tile", type=int, default=10, help="DWT tile shape of the counter probe's batch")
     ap.add_argument("--decode-frames", type=int, default=8, help="distinct 4K frames of the decoder measurement")
     ap.add_argument("--decoders", type=int, default=8)
     ap.add_argument("--decode-only", action="store_true", help="only the decoder measurement")

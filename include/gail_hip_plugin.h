@@ -182,8 +182,10 @@ typedef struct GailHipDecodeTimings {
 typedef struct GailHipBatchOptions {
     int32_t dwtTile;      /* 0 64x64, 1 64x32, 2 32x64, 3 64x16 (LDS tiles); 4 120x64, 5 120x32,
                              6 120x128 (register-streaming strips, one wave each); 7, 8 level 0
-                             as strips with their input rows loaded ahead (7 120x16, 8 120x8,
-                             the default), the other levels as 64x16 LDS tiles */
+                             as strips with their input rows loaded ahead (7 120x16, 8 120x8),
+                             9 the 120x8 strips addressed as buffers, 10 groups of four such
+                             strips writing whole 128-byte lines through LDS (the default);
+                             the other levels as 64x16 LDS tiles */
     int32_t cmWaves;      /* waves per context-modelling workgroup: 1, 2, 4, 8 */
     int32_t mqGroup;      /* MQ workgroup size: 64, 128, 256 */
     int32_t useGraph;     /* replay the launch chain as a HIP graph */
@@ -294,13 +296,13 @@ struct GailHipPlugin {
     /* Pinned host memory on the plugin's runtime; NULL when none is left. */
     void* (*host_alloc)(uint64_t bytes);
     void (*host_free)(void* p);
-    /* Autotuner: times every DWT tile shape (dwtTile 0-8), context-modelling waves per
+    /* Autotuner: times every DWT tile shape (dwtTile 0-10), context-modelling waves per
      * workgroup (1, 2, 4, 8) and MQ lanes per workgroup (64, 128, 256) on the bound frames,
      * whose inputs must already be on the device; keeps the fastest of each for the batch's
      * later runs, and the MQ symbol read width (16 or 64 bytes). chosen (may be NULL)
-     * receives them; times (may be NULL) up to 19 measured stage times in ms: tile shapes
+     * receives them; times (may be NULL) up to 21 measured stage times in ms: tile shapes
      * 0-3, 4 workgroup sizes, 3 MQ workgroup sizes with 16-byte reads, the same with 64-byte,
-     * then tile shapes 4-8. */
+     * then tile shapes 4-10. */
     int (*batch_autotune)(GailHipBatch* b, int reps, GailHipBatchOptions* chosen, double* times, int ntimes);
     /* Device memory for frames kept resident in HBM (a frame whose planes lie there is
      * copied device to device into its slot), a synchronous copy in any direction, and

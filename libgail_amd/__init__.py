@@ -475,7 +475,7 @@ class Batch:
     (a list of (max_width, max_height, colour)) the batch starts empty and bind() assigns
     frames to its slots for each run (the served form)."""
 
-    def __init__(self, device, layouts=None, settings=None, dwt_tile=8, cm_waves=4, mq_group=256, use_graph=False,
+    def __init__(self, device, layouts=None, settings=None, dwt_tile=10, cm_waves=4, mq_group=256, use_graph=False,
                  pool_bytes_per_coef=0.0, slots=None, mq_read=64):
         self.options = BatchOptions(dwt_tile, cm_waves, mq_group, int(use_graph), mq_read, 0, pool_bytes_per_coef)
         self.handle = _vp()
@@ -513,9 +513,9 @@ class Batch:
 
     TUNE = ("dwt_64x64", "dwt_64x32", "dwt_32x64", "dwt_64x16", "cm_1", "cm_2", "cm_4", "cm_8", "mq_64", "mq_128",
             "mq_256", "mq_64_r64", "mq_128_r64", "mq_256_r64", "dwt_strip_120x64", "dwt_strip_120x32",
-            "dwt_strip_120x128", "dwt_strip0_120x16", "dwt_strip0_120x8")
+            "dwt_strip_120x128", "dwt_strip0_120x16", "dwt_strip0_120x8", "dwt_strip0b_120x8", "dwt_stage0_480x8")
     DWT_SHAPES = ("64x64", "64x32", "32x64", "64x16", "strip_120x64", "strip_120x32", "strip_120x128",
-                  "strip0_120x16", "strip0_120x8")
+                  "strip0_120x16", "strip0_120x8", "strip0b_120x8", "stage0_480x8")
 
     def autotune(self, reps=2):
         """Times the kernel variants on the bound, uploaded frames and keeps the fastest.

@@ -153,7 +153,9 @@ size_t Job::estimateBytes(const std::vector<SlotCapacity>& slots, const JobOptio
     for (const SlotCapacity& c : slots) {
         const size_t W = c.maxWidth, H = c.maxHeight, npix = W * H;
         total += alignUp(npix * 6) + (c.colour ? alignUp(npix * 6) + alignUp(65536 * 4) : 0);
-        total += alignUp(npix * 12) + alignUp(3 * ((W + 1) / 2) * ((H + 1) / 2) * 4) + alignUp(3 * ((W + 3) / 4) * ((H + 3) / 4) * 4);
+        const CoefLayout cl = CoefLayout::of(W);
+        total += alignUp(3 * cl.stride * H * 4) + alignUp(3 * cl.llStride * ((H + 1) / 2) * 4) +
+                 alignUp(3 * ((W + 3) / 4) * ((H + 3) / 4) * 4);
         coefs += 3 * npix;
         const SlotLimits lim = slotLimits(c);
         n += lim.blocks;
@@ -268,8 +270,10 @@ void Job::allocate() {
             d.lut = devAlloc<float>(65536);
             hostAlloc((void**)&d.hostLut, 65536 * sizeof(float));
         }
-        d.coef = devAlloc<float>(3 * d.pixCap);
-        d.llA = devAlloc<float>(3 * ((W + 1) / 2) * ((H + 1) / 2));
+        // the padded layout of the widest frame is the widest of every frame that fits
+        const CoefLayout cl = CoefLayout::of(W);
+        d.coef = devAlloc<float>(3 * cl.stride * H);
+        d.llA = devAlloc<float>(3 * cl.llStride * ((H + 1) / 2));
         d.llB = devAlloc<float>(3 * ((W + 3) / 4) * ((H + 3) / 4));
         const SlotLimits lim = slotLimits(c);
         blockCap_ += lim.blocks;
@@ -358,6 +362,11 @@ void Job::bind(std::vector<FrameConfig> frames) {
         SlotDev& d = sd_[f];
         const size_t W = L.width, H = L.height;
         d.npix = W * H;
+        const CoefLayout cl = CoefLayout::of(W);
+        d.coefStride = cl.stride;
+        d.coefPlane = cl.stride * H;
+        d.hiOff = cl.hiOff;
+        d.llStride = cl.llStride;
         d.inBytes = fc.input.bytes();
         d.codes = fc.input.kind == InputFormat::XyzPlanar ? (uint16_t*)d.in : d.codesBuf;
         if (fc.input.kind != InputFormat::XyzPlanar) {
@@ -399,7 +408,10 @@ void Job::bind(std::vector<FrameConfig> frames) {
             const CodeBlock& cb = L.cblks[i];
             const Subband& b = L.bands[cb.band];
             T1Block t{};
-            t.coef_off = (uint32_t)((size_t)b.comp * d.npix + (size_t)(b.py0 + cb.y0) * W + (size_t)(b.px0 + cb.x0));
+            // level 0's HL and HH are the only bands right of column ceil(W / 2)
+            size_t x = (size_t)(b.px0 + cb.x0);
+            if (x >= (W + 1) / 2) x += d.hiOff - (W + 1) / 2;
+            t.coef_off = (uint32_t)((size_t)b.comp * d.coefPlane + (size_t)(b.py0 + cb.y0) * d.coefStride + x);
             t.frame = (uint16_t)f;
             t.band = (uint16_t)(bandBase + cb.band);
             t.w = (uint8_t)cb.w;
@@ -427,7 +439,7 @@ void Job::bind(std::vector<FrameConfig> frames) {
         }
         nblocks += d.numBlocks;
         maxLevels = std::max(maxLevels, L.levels);
-        hFrames_[f] = T1FrameRef{d.coef, (uint32_t)W, 0.f};
+        hFrames_[f] = T1FrameRef{d.coef, (uint32_t)d.coefStride, 0.f};
     }
     if (maxLevels > levelCap_) throw std::logic_error("frame has more levels than its slot");
     hipStream_t s = stream_;
@@ -488,6 +500,11 @@ void Job::planLevels() {
         for (int f = 0; f < nf; ++f) {
             const Layout& L = frames_[f].layout;
             int w = L.width, h = L.height;
+            // the buffer-addressed strips mirror rows by one reflection (frames of 11 rows or
+            // more) and address a frame's three planes with 32-bit byte offsets
+            if (levelShape_[0] >= 9 &&
+                (h < 11 || (uint64_t)3 * (uint64_t)sd_[f].coefPlane * sizeof(float) >= (uint64_t)INT32_MAX))
+                levelShape_[0] = 8;
             for (int l = 0; l < L.levels; ++l) {
                 if (w < 2 || h < 2) levelShape_[l] = 3;
                 w = (w + 1) / 2;
@@ -501,26 +518,31 @@ void Job::planLevels() {
         for (int l = 0; l < L.levels; ++l) {
             DwtLevelDesc ds{};
             const int w2 = (w + 1) / 2, h2 = (h + 1) / 2;
+            // level 0's LL goes to llA in rows of d.llStride; the deeper levels' LL planes
+            // are packed (row stride = width)
             if (l == 0) {
                 ds.src = d.codes;
                 ds.src_stride = (uint32_t)L.width;
                 ds.src_plane = (uint32_t)d.npix;
             } else {
+                const uint32_t st = l == 1 ? (uint32_t)d.llStride : (uint32_t)w;
                 ds.src = ((l - 1) % 2 == 0) ? d.llA : d.llB;
-                ds.src_stride = (uint32_t)w;
-                ds.src_plane = (uint32_t)(w * h);
+                ds.src_stride = st;
+                ds.src_plane = st * (uint32_t)h;
             }
             ds.coef = d.coef;
-            ds.coef_stride = (uint32_t)L.width;
-            ds.coef_plane = (uint32_t)d.npix;
+            ds.coef_stride = (uint32_t)d.coefStride;
+            ds.coef_plane = (uint32_t)d.coefPlane;
+            ds.hi_off = l == 0 ? (uint32_t)d.hiOff : (uint32_t)w2;
             if (l == L.levels - 1) {
                 ds.ll_dst = d.coef;
-                ds.ll_stride = (uint32_t)L.width;
-                ds.ll_plane = (uint32_t)d.npix;
+                ds.ll_stride = (uint32_t)d.coefStride;
+                ds.ll_plane = (uint32_t)d.coefPlane;
             } else {
+                const uint32_t st = l == 0 ? (uint32_t)d.llStride : (uint32_t)w2;
                 ds.ll_dst = (l % 2 == 0) ? d.llA : d.llB;
-                ds.ll_stride = (uint32_t)w2;
-                ds.ll_plane = (uint32_t)(w2 * h2);
+                ds.ll_stride = st;
+                ds.ll_plane = st * (uint32_t)h2;
             }
             ds.w = w;
             ds.h = h;
@@ -989,9 +1011,20 @@ void Job::debugCodes(int i, std::vector<uint16_t>& out) {
 }
 
 void Job::debugWavelet(int i, std::vector<float>& out) {
-    out.resize(3 * sd_[i].npix);
+    // the padded planes, repacked into W x H Mallat planes
+    const SlotDev& d = sd_[i];
+    const size_t W = frames_[i].layout.width, H = frames_[i].layout.height, snx = (W + 1) / 2;
+    std::vector<float> dev(3 * d.coefPlane);
     hipCheck(hipStreamSynchronize(stream_), "sync");
-    hipCheck(hipMemcpy(out.data(), sd_[i].coef, out.size() * 4, hipMemcpyDeviceToHost), "wavelet download");
+    hipCheck(hipMemcpy(dev.data(), d.coef, dev.size() * 4, hipMemcpyDeviceToHost), "wavelet download");
+    out.resize(3 * d.npix);
+    for (size_t c = 0; c < 3; ++c)
+        for (size_t y = 0; y < H; ++y) {
+            const float* src = dev.data() + c * d.coefPlane + y * d.coefStride;
+            float* dst = out.data() + c * d.npix + y * W;
+            std::memcpy(dst, src, snx * 4);
+            std::memcpy(dst + snx, src + d.hiOff, (W - snx) * 4);
+        }
 }
 
 void Job::debugCoeffs(int i, std::vector<uint32_t>& out) {

@@ -38,7 +38,7 @@ struct FrameConfig {
 };
 
 struct JobOptions {
-    int dwtTile = kDwtDefaultShape;  // tile shape index (see dwt_tile_size): level 0 in 120 x 8 strips
+    int dwtTile = kDwtDefaultShape;  // tile shape index (see dwt_tile_size): level 0 in groups of 4 120 x 8 strips
     int cmWaves = 4;          // waves per context-modelling workgroup
     int cmGroupsPerCU = 8;    // persistent grid size per compute unit
     int mqGroup = 256;        // MQ workgroup size
@@ -103,11 +103,30 @@ private:
 
 // Stage times (ms) the autotuner measured for each choice: the DWT tile shapes of
 // dwt_tile_size (64x64, 64x32, 32x64, 64x16 LDS tiles, 120x64, 120x32, 120x128 strips,
-// 120x16 preloaded strips at level 0);
+// 120x16 and 120x8 preloaded strips, the 120x8 strips addressed as buffers, groups of four
+// of them writing whole lines, all at level 0);
 // context-modelling waves per workgroup 1, 2, 4, 8; MQ lanes per workgroup 64, 128, 256 with
 // 16-byte symbol reads, then the same with 64-byte reads.
 struct TuneTimes {
     double dwt[kDwtShapes] = {}, cm[4] = {}, mq[6] = {};  // mq: 16-byte reads x 64/128/256 lanes, then 64-byte
+};
+
+// Where a W x H frame's wavelet coefficients lie in its device planes: Mallat order (the LL
+// quadrant of level 0 holds the deeper levels at their Mallat positions), but every row
+// starts on a 128-byte line and level 0's high bands (HL, HH) start at a line too, so the
+// level-0 transform writes whole lines. Widths in floats.
+struct CoefLayout {
+    size_t hiOff;     // column of level 0's HL / HH: ceil(W / 2) rounded up to 32
+    size_t stride;    // row stride: hiOff + floor(W / 2) rounded up to 32
+    size_t llStride;  // row stride of level 0's LL output: ceil(W / 2) rounded up to 32
+    static CoefLayout of(size_t W) {
+        auto up32 = [](size_t v) { return (v + 31) & ~(size_t)31; };
+        CoefLayout c;
+        c.hiOff = up32((W + 1) / 2);
+        c.stride = up32(c.hiOff + W / 2);
+        c.llStride = c.hiOff;
+        return c;
+    }
 };
 
 // Room one frame slot of a job holds: any frame up to maxWidth x maxHeight, and (colour)
@@ -198,8 +217,8 @@ private:
         size_t inCap = 0, pixCap = 0;
         void* in = nullptr;          // device input
         uint16_t* codesBuf = nullptr;  // 3 planes of codes for colour inputs
-        float* coef = nullptr;       // 3 Mallat planes
-        float* llA = nullptr;        // LL scratch, 3 x ceil(W/2) x ceil(H/2)
+        float* coef = nullptr;       // 3 coefficient planes (CoefLayout)
+        float* llA = nullptr;        // LL scratch, 3 x CoefLayout::llStride x ceil(H/2)
         float* llB = nullptr;        // LL scratch, 3 x ceil(W/4) x ceil(H/4)
         float* lut = nullptr;        // EOTF table (colour slots only)
         float* hostLut = nullptr;    // pinned copy of the table last uploaded
@@ -207,6 +226,7 @@ private:
         ColorSpec lutSpec;
         // bound frame
         size_t inBytes = 0, npix = 0;
+        size_t coefStride = 0, coefPlane = 0, hiOff = 0, llStride = 0;  // CoefLayout of the frame
         uint16_t* codes = nullptr;   // == in for X'Y'Z' planar input
         float params[kColorParams] = {};  // colour parameter block of the bound spec
         float* lut3 = nullptr;             // app 3D LUT on the device (colour slots, on demand)

@@ -14,8 +14,9 @@ struct DwtLevelDesc {
     const void* src;        // level 0: 3 planes of u16 codes; else 3 float LL planes
     uint32_t src_stride;    // elements per row
     uint32_t src_plane;     // elements per plane
-    float* coef;            // Mallat planes of the frame
+    float* coef;            // coefficient planes of the frame (Mallat order, rows padded)
     uint32_t coef_stride, coef_plane;
+    uint32_t hi_off;        // column of this level's high band in coef (level 0: padded)
     float* ll_dst;          // where this level's LL quarter goes
     uint32_t ll_stride, ll_plane;
     int32_t w, h;           // LL region transformed by this level
@@ -128,8 +129,8 @@ hipError_t launch_dwt_level(bool level0, int tile_shape, const DwtLevelDesc* des
 // input rows are loaded ahead (7: 120x16, 10 rows ahead; 8: 120x8, every row up front) and the
 // other levels as 64x16 LDS tiles. Levels whose LL region is narrower or lower than 2 samples
 // take the LDS tiles.
-constexpr int kDwtShapes = 9;
-constexpr int kDwtDefaultShape = 8;
+constexpr int kDwtShapes = 11;
+constexpr int kDwtDefaultShape = 10;
 void dwt_tile_size(int tile_shape, int& tx, int& ty);
 hipError_t launch_color(const ColorDesc& d, hipStream_t s);
 

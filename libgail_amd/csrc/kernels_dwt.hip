@@ -199,11 +199,11 @@ __global__ void __launch_bounds__(256) dwt_level_kernel(const DwtLevelDesc* __re
     // De-interleaved stores: per tile row, the even samples (low) and the odd samples
     // (high) each form a contiguous run of TX / 2 outputs. Thread tid always writes output
     // k of segment seg; the 256 threads cover 256 / TX rows per step.
-    const int snx = (w + 1) >> 1, sny = (h + 1) >> 1;
+    const int sny = (h + 1) >> 1;
     constexpr int HALF = TX / 2, ROWS = 256 / TX, STEPS = 3 * TY / ROWS;
     const int k = tid % HALF, seg = (tid / HALF) & 1, r0 = tid / TX;
     const int gx = x0 + 2 * k + seg;
-    const int ox = seg ? snx + (gx >> 1) : (gx >> 1);
+    const int ox = seg ? (int)d.hi_off + (gx >> 1) : (gx >> 1);
     const bool colOk = gx < w;
 #pragma unroll 4
     for (int j = 0; j < STEPS; ++j) {
@@ -358,8 +358,8 @@ __global__ void __launch_bounds__(256) dwt_strip_kernel(const DwtLevelDesc* __re
     ld.init(d, x0, lane);
 
     // outputs: the lane's low and high columns of this level, where they are in the strip
-    const int snx = (w + 1) >> 1, sny = (h + 1) >> 1;
-    const int xo = (x0 >> 1) - 2 + lane;               // low column; the high one is snx + xo
+    const int sny = (h + 1) >> 1;
+    const int xo = (x0 >> 1) - 2 + lane;               // low column; the high one is hi_off + xo
     const bool inStrip = lane >= 2 && lane < 2 + kStripOut / 2;
     const bool lowOk = inStrip && ld.ce < w, highOk = inStrip && ld.ce + 1 < w;
 
@@ -405,11 +405,11 @@ __global__ void __launch_bounds__(256) dwt_strip_kernel(const DwtLevelDesc* __re
             lift_row_lanes(highE[c], highO[c]);
             const int oyL = gy >> 1;
             if (lowOk) d.ll_dst[(size_t)c * d.ll_plane + (size_t)oyL * d.ll_stride + xo] = lowE[c];
-            if (highOk) d.coef[(size_t)c * d.coef_plane + (size_t)oyL * d.coef_stride + snx + xo] = lowO[c];
+            if (highOk) d.coef[(size_t)c * d.coef_plane + (size_t)oyL * d.coef_stride + d.hi_off + xo] = lowO[c];
             if (gy + 1 < h) {
                 const int oyH = sny + (gy >> 1);
                 if (lowOk) d.coef[(size_t)c * d.coef_plane + (size_t)oyH * d.coef_stride + xo] = highE[c];
-                if (highOk) d.coef[(size_t)c * d.coef_plane + (size_t)oyH * d.coef_stride + snx + xo] = highO[c];
+                if (highOk) d.coef[(size_t)c * d.coef_plane + (size_t)oyH * d.coef_stride + d.hi_off + xo] = highO[c];
             }
         }
     }
@@ -475,7 +475,7 @@ __device__ __forceinline__ void strip0_body(const DwtLevelDesc& d, int x0, int y
         o[2] = kY2[0] * x1f + kY2[1] * y1f + kY2[2] * z1f;
     };
 
-    const int snx = (w + 1) >> 1, sny = (h + 1) >> 1;
+    const int sny = (h + 1) >> 1;
     const int xo = (x0 >> 1) - 2 + lane;
     const bool inStrip = lane >= 2 && lane < 2 + kStripOut / 2;
     const bool lowOk = inStrip && (kInner || ce < w), highOk = inStrip && (kInner || ce + 1 < w);
@@ -524,25 +524,25 @@ __device__ __forceinline__ void strip0_body(const DwtLevelDesc& d, int x0, int y
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     float* const llRow = d.ll_dst + (size_t)c * d.ll_plane + (size_t)oyL * d.ll_stride;
-                    float* const hlRow = d.coef + (size_t)c * d.coef_plane + (size_t)oyL * d.coef_stride + snx;
+                    float* const hlRow = d.coef + (size_t)c * d.coef_plane + (size_t)oyL * d.coef_stride + d.hi_off;
                     float* const lhRow = d.coef + (size_t)c * d.coef_plane + (size_t)oyH * d.coef_stride;
                     store_global(llRow + xu, lowE[c]);
                     store_global(hlRow + xu, lowO[c]);
                     store_global(lhRow + xu, highE[c]);
-                    store_global(lhRow + snx + xu, highO[c]);
+                    store_global(lhRow + d.hi_off + xu, highO[c]);
                 }
             }
         } else {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 float* const llRow = d.ll_dst + (size_t)c * d.ll_plane + (size_t)oyL * d.ll_stride;
-                float* const hlRow = d.coef + (size_t)c * d.coef_plane + (size_t)oyL * d.coef_stride + snx;
+                float* const hlRow = d.coef + (size_t)c * d.coef_plane + (size_t)oyL * d.coef_stride + d.hi_off;
                 if (lowOk) store_global(llRow + xu, lowE[c]);
                 if (highOk) store_global(hlRow + xu, lowO[c]);
                 if (oddRow) {
                     float* const lhRow = d.coef + (size_t)c * d.coef_plane + (size_t)oyH * d.coef_stride;
                     if (lowOk) store_global(lhRow + xu, highE[c]);
-                    if (highOk) store_global(lhRow + snx + xu, highO[c]);
+                    if (highOk) store_global(lhRow + d.hi_off + xu, highO[c]);
                 }
             }
         }
@@ -567,6 +567,324 @@ __global__ void __launch_bounds__(256, MINW) dwt_strip0_kernel(const DwtLevelDes
                        ((d.src_stride | d.src_plane) & 1u) == 0;
     if (inner) strip0_body<TY, PRE, true>(d, x0, y0, lane);
     else strip0_body<TY, PRE, false>(d, x0, y0, lane);
+}
+
+// Level 0 in 120 x TY strips with the rows loaded ahead, as dwt_strip0_kernel, but every
+// load and store a raw buffer access: the frame's base in a buffer resource, the row and
+// component a scalar byte offset, the lane's column a fixed vector offset, so no vector
+// instruction computes an address. A lane with nothing to store (outside the strip or the
+// frame) stores to offset kDiscard, beyond the resource's range, which the hardware drops:
+// the stores of the inner strips run with no branch and no per-step vector work. Byte
+// offsets are 32-bit: a frame's three planes stay below 2 GiB (the host checks it). The
+// codes' DC shift runs on both halves of a word at once. The arithmetic is otherwise the
+// streaming strips' own, operation for operation.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t frame_rsrc(const void* p) {
+    // raw (stride 0) resource of 2^31 - 1 bytes; word 3 is the gfx9 default data format
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, 0x7FFFFFFF, 0x00020000);
+}
+// A vector offset beyond every frame_rsrc range: stores to it are dropped.
+constexpr uint32_t kDiscard = 0x80000000u;
+
+// Row y of a strip mirrored into [0, h) for h >= 11: one reflection is exact for the rows
+// -4 .. h + 9 a strip of up to 8 rows reads; rows further down land on some valid row.
+__device__ __forceinline__ int mirror_row(int y, int h) {
+    y = y < 0 ? -y : y;
+    y = y < h ? y : 2 * (h - 1) - y;
+    return y > 0 ? y : 0;
+}
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+// The DC-shifted samples of the two 12-bit codes of a word (low half first): both halves
+// clipped to 4095 (words above it read as 4095, as the host copy clips them) and shifted by
+// -2048 with one packed 16-bit operation each, then converted from the sign-extended halves.
+// The values are (float)((int)min(code, 4095) - 2048) of each half.
+__device__ __forceinline__ void dc_shift_pair(uint32_t code, float& lo, float& hi) {
+    u16x2 v = __builtin_bit_cast(u16x2, code);
+    v = __builtin_elementwise_min(v, (u16x2){4095, 4095});
+    const s16x2 s = __builtin_bit_cast(s16x2, v) - (s16x2){2048, 2048};
+    lo = (float)s.x;
+    hi = (float)s.y;
+}
+
+template <int TY, int PRE, bool kInner>
+__device__ __forceinline__ void strip0b_body(const DwtLevelDesc& d, int x0, int y0, int lane) {
+    constexpr int RY = TY + 7;  // rows y0 - 4 .. y0 + TY + 2
+    static_assert(TY <= 8, "mirror_row covers strips of up to 8 rows");
+    const int w = d.w, h = d.h;
+    const int ce = x0 - 4 + 2 * lane;
+    const __amdgpu_buffer_rsrc_t src = frame_rsrc(d.src);
+    // kInner: the pair (ce, ce + 1) is one aligned word; else two mirrored columns
+    const uint32_t voE = 2u * (uint32_t)(kInner ? ce : mirror(ce, w));
+    const uint32_t voO = kInner ? 0u : 2u * (uint32_t)mirror(ce + 1, w);
+    const uint32_t sRow = 2u * d.src_stride, sPlane = 2u * d.src_plane;
+    uint32_t code[RY][3];
+    auto loadRow = [&](int r) {
+        const int y = kInner ? y0 - 4 + r : mirror_row(y0 - 4 + r, h);
+        const uint32_t so = (uint32_t)y * sRow;  // wave-uniform
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const uint32_t soc = so + (uint32_t)c * sPlane;
+            if constexpr (kInner)
+                code[r][c] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(src, (int)voE, (int)soc, 0);
+            else
+                code[r][c] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(src, (int)voE, (int)soc, 0) |
+                             (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(src, (int)voO, (int)soc, 0) << 16;
+        }
+    };
+#pragma unroll
+    for (int r = 0; r < (PRE < RY ? PRE : RY); ++r) loadRow(r);
+    // DC shift and forward ICT of row r, as StripLoader<true>::load computes them
+    auto rowSamples = [&](int r, float (&e)[3], float (&o)[3]) {
+        float x0f, y0f, z0f, x1f, y1f, z1f;
+        dc_shift_pair(code[r][0], x0f, x1f);
+        dc_shift_pair(code[r][1], y0f, y1f);
+        dc_shift_pair(code[r][2], z0f, z1f);
+        e[0] = kY0[0] * x0f + kY0[1] * y0f + kY0[2] * z0f;
+        e[1] = kY1[0] * x0f + kY1[1] * y0f + kY1[2] * z0f;
+        e[2] = kY2[0] * x0f + kY2[1] * y0f + kY2[2] * z0f;
+        o[0] = kY0[0] * x1f + kY0[1] * y1f + kY0[2] * z1f;
+        o[1] = kY1[0] * x1f + kY1[1] * y1f + kY1[2] * z1f;
+        o[2] = kY2[0] * x1f + kY2[1] * y1f + kY2[2] * z1f;
+    };
+
+    const __amdgpu_buffer_rsrc_t coef = frame_rsrc(d.coef), llr = frame_rsrc(d.ll_dst);
+    const int sny = (h + 1) >> 1;
+    const int xo = (x0 >> 1) - 2 + lane;  // the lane's low column
+    const bool inStrip = lane >= 2 && lane < 2 + kStripOut / 2;
+    const uint32_t voL = inStrip && (kInner || ce < w) ? 4u * (uint32_t)xo : kDiscard;
+    const uint32_t voH = inStrip && (kInner || ce + 1 < w) ? 4u * (d.hi_off + (uint32_t)xo) : kDiscard;
+    const uint32_t llRow = 4u * d.ll_stride, llPlane = 4u * d.ll_plane;
+    const uint32_t cRow = 4u * d.coef_stride, cPlane = 4u * d.coef_plane;
+
+    StripColumn col[3][2];
+    {
+        float e[3], o[3];
+        rowSamples(0, e, o);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            col[c][0] = StripColumn{e[c], 0.f, 0.f, 0.f};
+            col[c][1] = StripColumn{o[c], 0.f, 0.f, 0.f};
+        }
+    }
+    // the step at k = y0 - 4 + 2 i takes rows k + 1, k + 2 (preloaded rows 2 i + 1, 2 i + 2)
+    // and finishes rows k - 2, k - 1
+#pragma unroll
+    for (int i = 0; i <= (TY + 4) / 2; ++i) {
+        const int k = y0 - 4 + 2 * i;
+        if (PRE + 2 * i < RY) loadRow(PRE + 2 * i);  // keep PRE rows ahead in flight
+        if (PRE + 2 * i + 1 < RY) loadRow(PRE + 2 * i + 1);
+        float oddE[3], oddO[3], evenE[3], evenO[3];
+        rowSamples(2 * i + 1, oddE, oddO);
+        rowSamples(2 * i + 2, evenE, evenO);
+        float lowE[3], lowO[3], highE[3], highO[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            col[c][0].step(oddE[c], evenE[c], lowE[c], highE[c]);
+            col[c][1].step(oddO[c], evenO[c], lowO[c], highO[c]);
+        }
+        const int gy = k - 2;
+        if (2 * i < 6) continue;           // the rows above the strip: gy < y0
+        if (!kInner && gy >= h) continue;  // wave-uniform
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            lift_row_lanes(lowE[c], lowO[c]);
+            lift_row_lanes(highE[c], highO[c]);
+        }
+        const bool oddIn = kInner || gy + 1 < h;
+        const uint32_t vLo = oddIn ? voL : kDiscard, vHo = oddIn ? voH : kDiscard;
+        const uint32_t oyL = (uint32_t)gy >> 1, oyH = (uint32_t)sny + oyL;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const uint32_t soLL = (uint32_t)c * llPlane + oyL * llRow;
+            const uint32_t soHL = (uint32_t)c * cPlane + oyL * cRow;
+            const uint32_t soLH = (uint32_t)c * cPlane + oyH * cRow;
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(lowE[c]), llr, (int)voL, (int)soLL, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(lowO[c]), coef, (int)voH, (int)soHL, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(highE[c]), coef, (int)vLo, (int)soLH, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(highO[c]), coef, (int)vHo, (int)soLH, 0);
+        }
+    }
+}
+
+// WG waves per workgroup, one strip each.
+template <int TY, int PRE, int MINW, int WG>
+__global__ void __launch_bounds__(64 * WG, MINW) dwt_strip0b_kernel(const DwtLevelDesc* __restrict__ descs, int ndesc, int ntiles) {
+    static_assert(TY % 2 == 0, "strips start at even rows");
+    static_assert(PRE >= 3, "rows 2i + 1 and 2i + 2 must be loaded by step i");
+    const int nb = (int)gridDim.x, q = nb >> 3, rem = nb & 7;
+    const int xcd = (int)blockIdx.x & 7, j = (int)blockIdx.x >> 3;
+    const int wave = WG == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int tileIdx = (xcd * q + min(xcd, rem) + j) * WG + wave;
+    if (tileIdx >= ntiles) return;
+    const int lane = (int)threadIdx.x & 63;
+    const int di = find_desc(descs, ndesc, tileIdx);
+    const DwtLevelDesc d = descs[di];
+    const int t = tileIdx - (int)d.tile_base;
+    const int x0 = (t % d.tiles_x) * kStripOut, y0 = (t / d.tiles_x) * TY;
+    const bool inner = x0 >= 4 && x0 + kStripOut + 4 <= d.w && y0 >= 4 && y0 + TY + 3 <= d.h &&
+                       ((d.src_stride | d.src_plane) & 1u) == 0;
+    if (inner) strip0b_body<TY, PRE, true>(d, x0, y0, lane);
+    else strip0b_body<TY, PRE, false>(d, x0, y0, lane);
+}
+
+// Level 0 in workgroups of NW adjacent 120 x 8 strips, one per wave with its rows loaded
+// ahead as in dwt_strip0b_kernel, whose outputs leave the CU as whole 128-byte lines: each
+// pair of finished rows is staged in LDS, then the workgroup writes the group's run of
+// NW * 60 columns of every band row with 16-byte stores. Coefficient rows, level 0's LL rows
+// and its high bands all start on a line (the padded coefficient layout) and a group's first
+// output column is a multiple of 32, so every run starts on a line; a strip's own 240-byte
+// pieces would start anywhere in one, and HBM takes partial lines far more slowly. The
+// staging is double-buffered: the pair finished at step i goes to buffer (i - 3) & 1, so one
+// barrier per pair orders the writes of a buffer after the reads of its previous pair.
+template <int NW, int MINW>
+__global__ void __launch_bounds__(64 * NW, MINW) dwt_stage0_kernel(const DwtLevelDesc* __restrict__ descs, int ndesc, int ngroups) {
+    constexpr int TY = 8, RY = TY + 7, RUN = NW * (kStripOut / 2), Q = RUN / 4;
+    static_assert(RUN % 8 == 0 && 12 % NW == 0 && Q <= 64, "runs start on 32-byte boundaries, whole runs per wave");
+    __shared__ float4 stage[2][12][Q];  // buffer, run (component c, band b: c + 3 b), quads
+    const int nb = (int)gridDim.x, q = nb >> 3, rem = nb & 7;
+    const int xcd = (int)blockIdx.x & 7, j = (int)blockIdx.x >> 3;
+    const int g = xcd * q + min(xcd, rem) + j;
+    if (g >= ngroups) return;  // the whole workgroup
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
+    const int di = find_desc(descs, ndesc, g);
+    const DwtLevelDesc d = descs[di];
+    const int t = g - (int)d.tile_base;
+    const int gx = t % d.tiles_x, y0 = (t / d.tiles_x) * TY;
+    const int x0 = gx * (NW * kStripOut) + wave * kStripOut;
+    const int w = d.w, h = d.h;
+    const bool act = x0 < w;  // waves right of the frame only keep the barriers
+    const bool inner = act && x0 >= 4 && x0 + kStripOut + 4 <= w && y0 >= 4 && y0 + TY + 3 <= h &&
+                       ((d.src_stride | d.src_plane) & 1u) == 0;
+    const int ce = x0 - 4 + 2 * lane;
+
+    uint32_t code[RY][3];
+    {
+        const __amdgpu_buffer_rsrc_t src = frame_rsrc(d.src);
+        const uint32_t sRow = 2u * d.src_stride, sPlane = 2u * d.src_plane;
+        if (inner) {
+            const uint32_t voE = 2u * (uint32_t)ce;
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                const uint32_t so = (uint32_t)(y0 - 4 + r) * sRow;
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    code[r][c] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(src, (int)voE, (int)(so + (uint32_t)c * sPlane), 0);
+            }
+        } else if (act) {
+            const uint32_t voE = 2u * (uint32_t)mirror(ce, w), voO = 2u * (uint32_t)mirror(ce + 1, w);
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                const uint32_t so = (uint32_t)mirror_row(y0 - 4 + r, h) * sRow;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const int soc = (int)(so + (uint32_t)c * sPlane);
+                    code[r][c] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(src, (int)voE, soc, 0) |
+                                 (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(src, (int)voO, soc, 0) << 16;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < RY; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) code[r][c] = 0;
+        }
+    }
+    // DC shift and forward ICT of row r, as StripLoader<true>::load computes them
+    auto rowSamples = [&](int r, float (&e)[3], float (&o)[3]) {
+        float x0f, y0f, z0f, x1f, y1f, z1f;
+        dc_shift_pair(code[r][0], x0f, x1f);
+        dc_shift_pair(code[r][1], y0f, y1f);
+        dc_shift_pair(code[r][2], z0f, z1f);
+        e[0] = kY0[0] * x0f + kY0[1] * y0f + kY0[2] * z0f;
+        e[1] = kY1[0] * x0f + kY1[1] * y0f + kY1[2] * z0f;
+        e[2] = kY2[0] * x0f + kY2[1] * y0f + kY2[2] * z0f;
+        o[0] = kY0[0] * x1f + kY0[1] * y1f + kY0[2] * z1f;
+        o[1] = kY1[0] * x1f + kY1[1] * y1f + kY1[2] * z1f;
+        o[2] = kY2[0] * x1f + kY2[1] * y1f + kY2[2] * z1f;
+    };
+
+    const bool inStrip = lane >= 2 && lane < 2 + kStripOut / 2;
+    const bool lowOk = act && inStrip && ce < w, highOk = act && inStrip && ce + 1 < w;
+    float* const slot = (float*)&stage[0][0][0] + wave * (kStripOut / 2) + lane - 2;  // run-relative
+    const int sny = (h + 1) >> 1, xo0 = gx * RUN;
+    const int nLow = min(RUN, ((w + 1) >> 1) - xo0), nHigh = min(RUN, (w >> 1) - xo0);
+    const __amdgpu_buffer_rsrc_t coef = frame_rsrc(d.coef), llr = frame_rsrc(d.ll_dst);
+    const uint32_t llRow = 4u * d.ll_stride, llPlane = 4u * d.ll_plane;
+    const uint32_t cRow = 4u * d.coef_stride, cPlane = 4u * d.coef_plane, hiB = 4u * d.hi_off;
+    const uint32_t voQ = 4u * (uint32_t)(xo0 + 4 * lane);  // this lane's quad in every run
+
+    StripColumn col[3][2];
+    {
+        float e[3], o[3];
+        rowSamples(0, e, o);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            col[c][0] = StripColumn{e[c], 0.f, 0.f, 0.f};
+            col[c][1] = StripColumn{o[c], 0.f, 0.f, 0.f};
+        }
+    }
+#pragma unroll
+    for (int i = 0; i <= (TY + 4) / 2; ++i) {
+        float oddE[3], oddO[3], evenE[3], evenO[3];
+        rowSamples(2 * i + 1, oddE, oddO);
+        rowSamples(2 * i + 2, evenE, evenO);
+        float lowE[3], lowO[3], highE[3], highO[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            col[c][0].step(oddE[c], evenE[c], lowE[c], highE[c]);
+            col[c][1].step(oddO[c], evenO[c], lowO[c], highO[c]);
+        }
+        if (i < 3) continue;               // the rows above the group
+        const int gy = y0 - 6 + 2 * i;     // even row finished; gy + 1 the odd one
+        if (gy >= h) continue;             // the same for the whole workgroup
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            lift_row_lanes(lowE[c], lowO[c]);
+            lift_row_lanes(highE[c], highO[c]);
+        }
+        const int buf = (i - 3) & 1;
+        float* const sb = slot + buf * (12 * RUN);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (lowOk) {
+                sb[(c + 0) * RUN] = lowE[c];   // LL
+                sb[(c + 6) * RUN] = highE[c];  // LH
+            }
+            if (highOk) {
+                sb[(c + 3) * RUN] = lowO[c];   // HL
+                sb[(c + 9) * RUN] = highO[c];  // HH
+            }
+        }
+        __syncthreads();
+        // wave v writes runs v, v + NW, ...: lane l the quad at columns 4l .. 4l + 3 of the
+        // run, a 16-byte store at a fixed vector offset; the run's row, component and band
+        // are the scalar offset
+        const bool oddIn = gy + 1 < h;  // LH and HH only where the odd row exists
+        const uint32_t oyL = (uint32_t)gy >> 1, oyH = (uint32_t)sny + oyL;
+#pragma unroll
+        for (int k = 0; k < 12 / NW; ++k) {
+            const int run = wave + NW * k;  // wave-uniform
+            const int c = run % 3, band = run / 3;
+            if (band >= 2 && !oddIn) continue;
+            const int n = (band & 1) ? nHigh : nLow;
+            const uint32_t so = band == 0 ? (uint32_t)c * llPlane + oyL * llRow
+                                          : (uint32_t)c * cPlane + (band >= 2 ? oyH : oyL) * cRow + ((band & 1) ? hiB : 0u);
+            const __amdgpu_buffer_rsrc_t dst = band == 0 ? llr : coef;
+            if (lane < Q && 4 * lane < n) {
+                const float4 v = stage[buf][run][lane];
+                if (4 * lane + 4 <= n) {
+                    const u32x4 q4 = {__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z), __float_as_uint(v.w)};
+                    __builtin_amdgcn_raw_buffer_store_b128(q4, dst, (int)voQ, (int)so, 0);
+                } else {  // the frame's last columns
+                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v.x), dst, (int)voQ, (int)so, 0);
+                    if (4 * lane + 1 < n) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v.y), dst, (int)voQ + 4, (int)so, 0);
+                    if (4 * lane + 2 < n) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v.z), dst, (int)voQ + 8, (int)so, 0);
+                }
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------- colour
@@ -815,6 +1133,18 @@ static hipError_t launch_strip0(const DwtLevelDesc* descs, int n, int tiles, hip
     return hipGetLastError();
 }
 
+template <int NW, int MINW>
+static hipError_t launch_stage0(const DwtLevelDesc* descs, int n, int groups, hipStream_t s) {
+    hipLaunchKernelGGL((dwt_stage0_kernel<NW, MINW>), dim3(groups), dim3(64 * NW), 0, s, descs, n, groups);
+    return hipGetLastError();
+}
+
+template <int TY, int PRE, int MINW, int WG>
+static hipError_t launch_strip0b(const DwtLevelDesc* descs, int n, int tiles, hipStream_t s) {
+    hipLaunchKernelGGL((dwt_strip0b_kernel<TY, PRE, MINW, WG>), dim3((tiles + WG - 1) / WG), dim3(64 * WG), 0, s, descs, n, tiles);
+    return hipGetLastError();
+}
+
 hipError_t launch_dwt_level(bool level0, int tile_shape, const DwtLevelDesc* descs, int ndesc, int tiles,
                             hipStream_t s) {
     switch (tile_shape) {
@@ -826,6 +1156,8 @@ hipError_t launch_dwt_level(bool level0, int tile_shape, const DwtLevelDesc* des
     case 6: return level0 ? launch_strip<true, 128>(descs, ndesc, tiles, s) : launch_strip<false, 128>(descs, ndesc, tiles, s);
     case 7: return level0 ? launch_strip0<16, 10, 5>(descs, ndesc, tiles, s) : launch_dwt<false, 64, 16>(descs, ndesc, tiles, s);
     case 8: return level0 ? launch_strip0<8, 15, 5>(descs, ndesc, tiles, s) : launch_dwt<false, 64, 16>(descs, ndesc, tiles, s);
+    case 9: return level0 ? launch_strip0b<8, 15, 5, 4>(descs, ndesc, tiles, s) : launch_dwt<false, 64, 16>(descs, ndesc, tiles, s);
+    case 10: return level0 ? launch_stage0<4, 5>(descs, ndesc, tiles, s) : launch_dwt<false, 64, 16>(descs, ndesc, tiles, s);
     default: return level0 ? launch_dwt<true, 64, 16>(descs, ndesc, tiles, s) : launch_dwt<false, 64, 16>(descs, ndesc, tiles, s);
     }
 }
@@ -833,7 +1165,7 @@ hipError_t launch_dwt_level(bool level0, int tile_shape, const DwtLevelDesc* des
 void dwt_tile_size(int tile_shape, int& tx, int& ty) {
     static const int shapes[kDwtShapes][2] = {{64, 64}, {64, 32}, {32, 64}, {64, 16},
                                               {kStripOut, 64}, {kStripOut, 32}, {kStripOut, 128}, {kStripOut, 16},
-                                              {kStripOut, 8}};
+                                              {kStripOut, 8}, {kStripOut, 8}, {4 * kStripOut, 8}};
     const int k = tile_shape >= 0 && tile_shape < kDwtShapes ? tile_shape : kDwtDefaultShape;
     tx = shapes[k][0];
     ty = shapes[k][1];

@@ -515,10 +515,12 @@ int batchAutotune(GailHipBatch* b, int reps, GailHipBatchOptions* chosen, double
             chosen->poolBytesPerCoef = o.poolBytesPerCoef;
         }
         // the four LDS tile shapes first and the strips last, as the table grew
-        const double v[19] = {t.dwt[0], t.dwt[1], t.dwt[2], t.dwt[3], t.cm[0],  t.cm[1], t.cm[2],
-                              t.cm[3],  t.mq[0],  t.mq[1],  t.mq[2],  t.mq[3],  t.mq[4], t.mq[5],
-                              t.dwt[4], t.dwt[5], t.dwt[6], t.dwt[7], t.dwt[8]};
-        for (int i = 0; times && i < ntimes && i < 19; ++i) times[i] = v[i];
+        const double v[] = {t.dwt[0], t.dwt[1], t.dwt[2], t.dwt[3], t.cm[0],  t.cm[1], t.cm[2],
+                            t.cm[3],  t.mq[0],  t.mq[1],  t.mq[2],  t.mq[3],  t.mq[4], t.mq[5],
+                            t.dwt[4], t.dwt[5], t.dwt[6], t.dwt[7], t.dwt[8], t.dwt[9], t.dwt[10]};
+        static_assert(sizeof(v) / sizeof(v[0]) == 10 + kDwtShapes, "every stage time once");
+        const int nv = (int)(sizeof(v) / sizeof(v[0]));
+        for (int i = 0; times && i < ntimes && i < nv; ++i) times[i] = v[i];
     });
 }
 

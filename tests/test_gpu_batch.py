@@ -214,7 +214,8 @@ def test_slot_job_rebinds(device, oracle):
 
 
 @pytest.mark.parametrize("opts", [dict(dwt_tile=0), dict(dwt_tile=1), dict(dwt_tile=2), dict(dwt_tile=4),
-                                  dict(dwt_tile=5), dict(dwt_tile=6), dict(dwt_tile=7), dict(dwt_tile=3),
+                                  dict(dwt_tile=5), dict(dwt_tile=6), dict(dwt_tile=7), dict(dwt_tile=9),
+                                  dict(dwt_tile=8), dict(dwt_tile=9), dict(dwt_tile=3),
                                   dict(cm_waves=1),
                                   dict(cm_waves=2), dict(cm_waves=8), dict(mq_group=64), dict(mq_group=128),
                                   dict(mq_read=16), dict(mq_group=64, mq_read=16), dict(mq_group=128, mq_read=16)])
@@ -295,12 +296,13 @@ def test_full_mixed_batch_64(device, oracle):
     assert not bad_frames, f"frames differing from the oracle (index, size, threshold, blocks): {bad_frames}"
 
 
-@pytest.mark.parametrize("shape", [3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("shape", [3, 4, 5, 6, 7, 8, 9, 10])
 def test_dwt_planes_every_shape(device, oracle, shape):
     """The wavelet planes of every DWT tile shape, LDS tiles and register-streaming strips,
     equal the oracle's float32 DWT bit for bit on frames whose sizes leave ragged strips and
-    tiles at both edges (odd widths and heights, LL regions down to a few samples)."""
-    sizes = [(257, 131, 0), (1998, 1080, 0), (1001, 515, 1), (2048, 858, 0)]
+    tiles at both edges (odd widths and heights, LL regions down to a few samples; a frame
+    67 rows high, whose second tall strip lies mostly below the frame's last row)."""
+    sizes = [(257, 131, 0), (1998, 1080, 0), (1001, 515, 1), (2048, 858, 0), (301, 67, 0)]
     layouts = [g.FrameLayout(W, H, k) for W, H, k in sizes]
     frames = [g.synth_frame(900 + i, W, H) for i, (W, H, _) in enumerate(sizes)]
     b = g.Batch(device, layouts, [g.xyz_settings(L) for L in layouts], dwt_tile=shape)
