@@ -33,7 +33,8 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes one DWT level-0 launch over 16 4K flat frames moves, per tile shape, from the
 # FETCH_SIZE and WRITE_SIZE counter runs in profiles/pmc_traffic_dwt_level0_4k_flat_batch16.md,
 # each scaled by what the counter reads for a 1 GiB device copy.
-DWT0_TRAFFIC_4K_FLAT_16 = {"64x16": 2491427489, "64x32": 2502515919, "strip0_120x8": 2493219461}
+DWT0_TRAFFIC_4K_FLAT_16 = {"64x16": 2491427489, "64x32": 2502515919, "strip0_120x8": 2493219461,
+                           "stage0_480x8": 2498583259}
 
 
 def dwt_level0_kernel(shape):
@@ -96,9 +97,10 @@ def hbm_copy_gbs(g, dev):
         p.device_free(vp(b))
 
 
-def cpu_baseline(frames, fourk, thr, budget_s=20.0):
+def cpu_baseline(frames, fourk, thr, budget_s=12.0):
     """The reference's CPU flow (compiled from its sources into oracle/_ref when present,
-    else the oracle port) on this host's cores, over a bounded sample of the same frames."""
+    else the oracle port) on this host's cores, over a bounded sample of the same frames:
+    the given frames in turn, again and again, until budget_s has passed (at least each once)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_bind import Oracle, reference
     ref = reference()
@@ -106,12 +108,13 @@ def cpu_baseline(frames, fourk, thr, budget_s=20.0):
     lib = ref if ref is not None else Oracle()
     cores = len(os.sched_getaffinity(0))
     done, t0 = 0, time.perf_counter()
-    while done < len(frames) and (time.perf_counter() - t0) < budget_s:
-        lib.encode_frame_xyz(frames[done], fourk, thr=thr, threads=cores)
+    while done < len(frames) or (time.perf_counter() - t0) < budget_s:
+        lib.encode_frame_xyz(frames[done % len(frames)], fourk, thr=thr, threads=cores)
         done += 1
     dt = time.perf_counter() - t0
     return {"value": round(done / dt, 3), "unit": "frames/s", "cores": cores, "kind": kind,
-            "sample": f"{done} 4K flat frames, colour/ICT + DWT + quantisation + Tier-1 with the same early "
+            "sample": f"{done} 4K flat frames ({len(frames)} distinct, in turn), colour/ICT + DWT + quantisation + "
+                      f"Tier-1 with the same early "
                       f"termination threshold, {cores} threads, {dt:.1f} s (no rate control or Tier-2: the "
                       f"reference's t2.cpp needs a generated header and does not build here)"}
 
