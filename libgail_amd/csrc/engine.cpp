@@ -208,6 +208,7 @@ Job::~Job() { release(); }
 void Job::release() {
     hipSetDevice(dev_);
     if (stream_) hipStreamSynchronize(stream_);
+    if (upStream_) hipStreamSynchronize(upStream_);
     if (graph_) hipGraphExecDestroy(graph_);
     graph_ = nullptr;
     for (void* p : allocs_) hipFree(p);
@@ -240,8 +241,13 @@ void Job::release() {
         }
     if (evDone_) hipEventDestroy(evDone_);
     evDone_ = nullptr;
+    if (evUp_) hipEventDestroy(evUp_);
+    evUp_ = nullptr;
     if (stream_) hipStreamDestroy(stream_);
     stream_ = nullptr;
+    if (upStream_) hipStreamDestroy(upStream_);
+    upStream_ = nullptr;
+    upPending_ = false;
     bytes_ = 0;
 }
 
@@ -252,8 +258,10 @@ static void hostAlloc(void** p, size_t bytes) {
 void Job::allocate() {
     hipCheck(hipSetDevice(dev_), "hipSetDevice");
     hipCheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream");
+    hipCheck(hipStreamCreateWithFlags(&upStream_, hipStreamNonBlocking), "stream");
     for (auto& e : ev_) hipCheck(hipEventCreate(&e), "event");
     hipCheck(hipEventCreateWithFlags(&evDone_, hipEventDisableTiming), "event");
+    hipCheck(hipEventCreateWithFlags(&evUp_, hipEventDisableTiming), "event");
     const int ns = (int)slots_.size();
     sd_.resize(ns);
     hostIn_.assign(ns, nullptr);
@@ -577,6 +585,8 @@ JobOptions Job::autotune(int reps, TuneTimes* out) {
     if (frames_.empty()) throw std::logic_error("no frames bound to the job");
     hipCheck(hipSetDevice(dev_), "hipSetDevice");
     hipCheck(hipStreamSynchronize(stream_), "previous run");
+    hipCheck(hipStreamSynchronize(upStream_), "frame uploads");
+    upPending_ = false;
     launched_ = false;
     reps = std::max(1, reps);
     hipStream_t s = stream_;
@@ -687,12 +697,16 @@ void Job::uploadFrom(int i, const void* src) {
     hipCheck(hipSetDevice(dev_), "hipSetDevice");
     if (launched_) hipCheck(hipStreamSynchronize(stream_), "previous run");
     // host or device source: the runtime tells them apart (a device source stays in HBM)
-    hipCheck(hipMemcpyAsync(sd_[i].in, src, sd_[i].inBytes, hipMemcpyDefault, stream_), "frame upload");
+    const bool second = (i & 1) != 0;
+    hipCheck(hipMemcpyAsync(sd_[i].in, src, sd_[i].inBytes, hipMemcpyDefault, second ? upStream_ : stream_),
+             "frame upload");
+    if (second) upPending_ = true;
 }
 
 void Job::inputsReady() {
     hipCheck(hipSetDevice(dev_), "hipSetDevice");
     hipCheck(hipStreamSynchronize(stream_), "frame uploads");
+    hipCheck(hipStreamSynchronize(upStream_), "frame uploads");
 }
 
 void Job::uploadAsync(int i) {
@@ -858,6 +872,11 @@ void Job::launch(bool uploadInputs) {
     hipCheck(hipSetDevice(dev_), "hipSetDevice");
     // hFrames_ is staged from the host: a run still in flight may be reading it
     if (launched_) hipCheck(hipStreamSynchronize(stream_), "previous run");
+    if (upPending_) {  // the run reads what uploadFrom queued on the second stream
+        hipCheck(hipEventRecord(evUp_, upStream_), "event");
+        hipCheck(hipStreamWaitEvent(stream_, evUp_, 0), "stream wait");
+        upPending_ = false;
+    }
     for (int f = 0; f < numFrames(); ++f) hFrames_[f].slope_thresh = slope_[f];
     launched_ = true;
     if (uploadInputs)  // staging buffers exist before any capture records copies from them

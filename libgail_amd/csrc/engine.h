@@ -254,6 +254,11 @@ private:
     std::vector<void*> allocs_;
     size_t bytes_ = 0;
     hipStream_t stream_ = nullptr;
+    // uploadFrom puts odd slots' copies on a second stream: two copy queues keep the link busy
+    // while the other turns around between copies; the next launch waits for evUp_
+    hipStream_t upStream_ = nullptr;
+    hipEvent_t evUp_ = nullptr;
+    bool upPending_ = false;
     hipEvent_t ev_[9] = {};  // ev_[8]: after DWT level 0
     hipEvent_t evDone_ = nullptr;
     hipGraphExec_t graph_ = nullptr;
