@@ -404,13 +404,15 @@ def cpu_config1(g, np):
 
 
 class StreamBench:
-    """One server with 16 jobs x 16 frame slots (256 4K frames, ~170 GB of HBM) for the 4K
-    flat streams: a closed loop with at most a given number of frames in flight."""
+    """One server with jobs of 16 frame slots, as many as frames_in_flight needs (16 jobs for
+    256 4K frames, ~170 GB of HBM), for the 4K flat streams: a closed loop with at most a given
+    number of frames in flight."""
 
-    def __init__(self, g, np, distinct=8):
+    def __init__(self, g, np, distinct=8, frames_in_flight=256):
         self.g, self.np = g, np
         self._sem = None
-        self.srv = g.Server(slots_per_job=16, max_jobs=16, check=True, on_done=self._done)
+        self.jobs = max(16, -(-frames_in_flight // 16))
+        self.srv = g.Server(slots_per_job=16, max_jobs=self.jobs, check=True, on_done=self._done)
         W, H, _ = g.CONTAINERS["4k_flat"]
         self.src = []  # the same frames in pinned host memory (PCIe in the timed loop)
         self.dev = []  # and resident in HBM (copied within HBM into the slots)
@@ -419,10 +421,10 @@ class StreamBench:
             self.src.append(_pin(g, np, [x[0], x[1], x[2]]))
             self.dev.append(g.DeviceFrame([x[0], x[1], x[2]]))
         self.next_id = 0
-        # the server makes its jobs as frames wait: 256 frames at once make all 16 (and run
-        # each once) before anything is timed
+        # the server makes its jobs as frames wait: 16 frames per job at once make them all
+        # (and run each once) before anything is timed
         cfg = g.stream_config("4k_flat", "xyz12_planar", fps=24.0)
-        for i in self._ids(16 * 16):
+        for i in self._ids(16 * self.jobs):
             self.srv.submit(i, 100, cfg, self.src[i % distinct], -1.0)
         self.srv.drain()
 
@@ -547,7 +549,7 @@ def main():
         return
     if args.stream_only:
         steps, warm, per = max(args.steps, 1), max(args.warmup, 0), max(args.step_frames, 1)
-        sb = StreamBench(g, np)
+        sb = StreamBench(g, np, frames_in_flight=args.stream_inflight)
         try:
             out = {"inflight": args.stream_inflight,
                    "stream": sb.run(24.0, steps * per, args.stream_inflight, stream=3, resident=False,
@@ -675,7 +677,7 @@ def main():
     # ---- served 4K flat streams (the headline: frames handed over from host memory, as the
     # reference's pipeline uploads every frame) and the 48 fps stress stream
     steps, warm, per = max(args.steps, 1), max(args.warmup, 0), max(args.step_frames, 1)
-    sb = StreamBench(g, np)
+    sb = StreamBench(g, np, frames_in_flight=args.stream_inflight)
     try:
         print(f"[bench] 4K flat stream from pinned host memory, 24 fps rate control, {warm} + {steps} steps of "
               f"{per} frames, {args.stream_inflight} in flight", file=sys.stderr, flush=True)
