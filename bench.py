@@ -285,6 +285,58 @@ def _pct(np, v, q):
     return round(float(np.percentile(v, q)), 2) if len(v) else None
 
 
+def api_encoder_run(g, np, frames, in_flight, count, warm=8):
+    """The drop-in encoder API (gail_encoder_send / receive, the reference's gail.h:329-362)
+    on a 4K flat X'Y'Z' stream at 24 fps rate control with in_flight frames in flight (the
+    reference's benchmark runs 4): one thread sends frames from pinned host memory, the
+    calling thread receives the codestreams in order; every frame DCI-checked."""
+    cfg = g.stream_config("4k_flat", "xyz12_planar", fps=24.0)
+    cfg.frames_in_flight = in_flight
+    cfg.check = 1
+    enc = g.ApiEncoder(cfg)
+    total = warm + count
+    sent = [0.0] * total
+    err = []
+
+    def sender():
+        try:
+            for i in range(total):
+                sent[i] = time.perf_counter()
+                enc.send(frames[i % len(frames)])
+            enc.end()
+        except Exception as e:  # reported below; the receiver then sees the end or a failure
+            err.append(str(e))
+            enc.end()
+
+    th = threading.Thread(target=sender)
+    th.start()
+    lat, bad, t0, n, sizes = [], 0, None, 0, []
+    try:
+        while True:
+            r = enc.receive()
+            if r is None:
+                break
+            data, idx, compliant = r
+            now = time.perf_counter()
+            if n == warm - 1:
+                t0 = now
+            if n >= warm:
+                lat.append((now - sent[idx]) * 1e3)
+                sizes.append(len(data))
+                bad += 0 if compliant else 1
+            n += 1
+        th.join()
+    finally:
+        enc.close()
+    if err or n != total or t0 is None:
+        return {"error": err[0] if err else f"{n} of {total} frames came back"}
+    dt = time.perf_counter() - t0
+    return {"frames": count, "warm_frames": warm, "frames_in_flight": in_flight, "seconds": round(dt, 3),
+            "fps": round(count / dt, 2), "p50_ms": _pct(np, lat, 50), "p99_ms": _pct(np, lat, 99),
+            "max_bytes": max(sizes), "noncompliant": bad,
+            "inputs": "pinned host memory, copied by gail_encoder_send as the reference's API does"}
+
+
 def served_run(g, np, offered_fps=300.0, seconds=5.0, seed=7):
     """Served form: Poisson arrivals at offered_fps, profiles uniform over the six DCI sizes,
     inputs alternating 12-bit G'B'R' BT.709 and 10-bit Y'CbCr 4:2:0 (limited range, chroma
@@ -526,6 +578,11 @@ def main():
     ap.add_argument("--decode-only", action="store_true", help="only the decoder measurement")
     ap.add_argument("--stream-inflight", type=int, default=256,
                     help="frames in flight in the headline stream (256: every slot of the 16 jobs)")
+    ap.add_argument("--api-frames", type=int, default=96,
+                    help="timed frames of the drop-in encoder API measurement (0: skip it)")
+    ap.add_argument("--api-inflight", type=int, default=4, help="frames in flight in that measurement")
+    ap.add_argument("--api-only", action="store_true",
+                    help="only the drop-in encoder API, at --api-inflight, twice and four times as many in flight")
     ap.add_argument("--tune-only", action="store_true", help="only the autotuner's kernel times on one batch")
     ap.add_argument("--stream-only", action="store_true",
                     help="only the headline stream (pinned host memory) and its HBM-resident twin")
@@ -546,6 +603,18 @@ def main():
     frames = [g.synth_frame(1000 + i, W, H, threads=16) for i in range(B)]
     if args.pmc_probe:
         pmc_probe(g, np, W, H, fourk, frames, dwt_tile=args.pmc_tile)
+        return
+    if args.api_only:
+        W4, H4, _ = g.CONTAINERS["4k_flat"]
+        pinned = []
+        for k in range(8):
+            x = g.synth_frame(1000 + k, W4, H4, threads=16)
+            pinned.append(_pin(g, np, [x[0], x[1], x[2]]))
+        out = {}
+        for n in (args.api_inflight, 2 * args.api_inflight, 4 * args.api_inflight):
+            out[n] = api_encoder_run(g, np, pinned, n, args.api_frames)
+            print(f"[bench] drop-in encoder API, {n} in flight: {json.dumps(out[n])}", file=sys.stderr, flush=True)
+        print(json.dumps({"api_encoder": out}))
         return
     if args.stream_only:
         steps, warm, per = max(args.steps, 1), max(args.warmup, 0), max(args.step_frames, 1)
@@ -693,6 +762,10 @@ def main():
             print(f"[bench] stress: {json.dumps(stress)}", file=sys.stderr, flush=True)
     finally:
         sb.close()
+    api_enc = None
+    if args.api_frames > 0:
+        api_enc = api_encoder_run(g, np, sb.src, args.api_inflight, args.api_frames)
+        print(f"[bench] drop-in encoder API: {json.dumps(api_enc)}", file=sys.stderr, flush=True)
     served = None
     if args.served_seconds > 0:
         served = served_run(g, np, seconds=args.served_seconds)
@@ -740,6 +813,7 @@ def main():
         "batched_mixed_64": mixed,
         "stress_4k_48fps": stress,
         "served": served,
+        "api_encoder": api_enc,
         "cpu_2k_frame": cfg1,
         "decode_4k": decode,
     }

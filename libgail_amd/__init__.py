@@ -822,6 +822,74 @@ class ServerError(RuntimeError):
     pass
 
 
+def _lib_encoder():
+    h = _lib_api()
+    if not hasattr(h, "_encoder_bound"):
+        h.gail_encoder_create.restype = C.c_int
+        h.gail_encoder_create.argtypes = [C.POINTER(GailConfig), C.POINTER(_vp)]
+        h.gail_encoder_send.restype = C.c_int
+        h.gail_encoder_send.argtypes = [_vp, C.POINTER(GailFrame)]
+        h.gail_encoder_receive.restype = C.c_int
+        h.gail_encoder_receive.argtypes = [_vp, C.POINTER(GailPacket), C.c_int]
+        h.gail_encoder_destroy.restype = None
+        h.gail_encoder_destroy.argtypes = [_vp]
+        h.gail_packet_free.restype = None
+        h.gail_packet_free.argtypes = [C.POINTER(GailPacket)]
+        h._encoder_bound = True
+    return h
+
+
+class ApiEncoder:
+    """The drop-in encoder (gail_encoder_*, include/gail_amd.h): frames sent in order come
+    back as codestreams in the same order (pull model, gail.h:329-362)."""
+
+    def __init__(self, cfg):
+        h = _lib_encoder()
+        self.handle = _vp()
+        if h.gail_encoder_create(C.byref(cfg), C.byref(self.handle)) != 0:
+            self.handle = _vp()
+            raise ServerError(f"gail_encoder_create: {h.gail_last_error().decode()}")
+
+    def send(self, planes):
+        """planes: three numpy arrays (planar formats) or one array; copied before the call
+        returns, as gail_encoder_send does."""
+        f = GailFrame()
+        arrs = planes if isinstance(planes, (list, tuple)) else [planes]
+        for i, a in enumerate(arrs):
+            f.data[i] = a.ctypes.data
+            f.stride[i] = a.strides[0] if a.ndim > 1 else 0
+        if _lib_encoder().gail_encoder_send(self.handle, C.byref(f)) != 0:
+            raise ServerError(f"gail_encoder_send: {_lib_encoder().gail_last_error().decode()}")
+
+    def end(self):
+        _lib_encoder().gail_encoder_send(self.handle, None)
+
+    def receive(self):
+        """(codestream bytes, frame index, DCI-compliant) of the next frame, or None at the end."""
+        h = _lib_encoder()
+        pkt = GailPacket()
+        st = h.gail_encoder_receive(self.handle, C.byref(pkt), 1)
+        if st == 2:  # GAIL_EOF
+            return None
+        if st != 0:
+            raise ServerError(f"gail_encoder_receive: {h.gail_last_error().decode()}")
+        data = C.string_at(pkt.data, pkt.size)
+        out = (data, pkt.index, pkt.compliant)
+        h.gail_packet_free(C.byref(pkt))
+        return out
+
+    def close(self):
+        if self.handle:
+            _lib_encoder().gail_encoder_destroy(self.handle)
+            self.handle = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def pinned_empty(nbytes):
     """A uint8 numpy array over pinned host memory (gail_host_alloc), freed with the array."""
     import weakref

@@ -26,6 +26,7 @@
 #include "codestream.h"
 #include "gail_amd.h"
 #include "gail_hip_plugin.h"
+#include "host_pool.h"
 #include "layout.h"
 #include "stream.h"
 
@@ -48,7 +49,7 @@ struct Waiter {
 
 struct Item {
     long long index = 0;
-    std::vector<uint8_t> buf;
+    uint8_t* buf = nullptr;  // from the encoder's buffer pool
     void* opaque = nullptr;
     Waiter* waiter = nullptr;
 };
@@ -90,6 +91,65 @@ struct gail_encoder {
     bool writerDone = false;
     std::vector<std::thread> workers;
     std::thread writer;
+    // Host buffers frames are filled into, reused from frame to frame: pinned (the plugin's
+    // host allocator) so that a worker's upload is a single DMA, and never zero-filled or
+    // faulted in again. Every queued frame, every frame a worker holds and every send in
+    // progress has one; they are freed with the encoder.
+    std::mutex bm;
+    std::vector<uint8_t*> freeBufs;
+    std::vector<std::pair<uint8_t*, bool>> allBufs;  // buffer, pinned
+
+    uint8_t* takeBuffer() {
+        {
+            std::lock_guard<std::mutex> l(bm);
+            if (!freeBufs.empty()) {
+                uint8_t* b = freeBufs.back();
+                freeBufs.pop_back();
+                return b;
+            }
+        }
+        bool pinned = true;
+        uint8_t* b = P->host_alloc ? (uint8_t*)P->host_alloc(spec->inBytes) : nullptr;
+        if (!b) {
+            pinned = false;
+            b = (uint8_t*)std::malloc(std::max<size_t>(1, spec->inBytes));
+        }
+        if (!b) return nullptr;
+        std::lock_guard<std::mutex> l(bm);
+        allBufs.emplace_back(b, pinned);
+        return b;
+    }
+
+    void giveBuffer(uint8_t* b) {
+        if (!b) return;
+        std::lock_guard<std::mutex> l(bm);
+        freeBufs.push_back(b);
+    }
+
+    // Threads that copy parts of a frame beside the sending thread: a 4K frame is 51.8 MB,
+    // more than one thread copies at the rate the device takes frames.
+    std::unique_ptr<Pool> fillPool;
+
+    void fillFrame(const gail_frame& f, uint8_t* dst) {
+        const int helpers = fillPool ? fillPool->size() : 0;
+        if (helpers == 0 || !spec->splitParts()) {
+            spec->fill(f, dst);
+            return;
+        }
+        const int parts = helpers + 1;
+        std::mutex m;
+        std::condition_variable cv;
+        int left = helpers;
+        for (int p = 1; p < parts; ++p)
+            fillPool->post([&, p] {
+                spec->fill(f, dst, p, parts);
+                std::lock_guard<std::mutex> l(m);
+                if (--left == 0) cv.notify_all();
+            });
+        spec->fill(f, dst, 0, parts);
+        std::unique_lock<std::mutex> l(m);
+        cv.wait(l, [&] { return left == 0; });
+    }
 
     std::string errorText() {
         std::lock_guard<std::mutex> l(om);
@@ -149,16 +209,21 @@ struct gail_encoder {
         const int st = spec->checkFrame(*f, why);
         if (st != GAIL_OK) return fail(st, why);
         Item it;
-        it.buf.resize(spec->inBytes);
-        spec->fill(*f, it.buf.data());
+        it.buf = takeBuffer();
+        if (!it.buf) return fail(GAIL_ERROR_NOMEM, "no host memory for the frame");
+        fillFrame(*f, it.buf);
         it.opaque = f->opaque;
         it.waiter = w;
         std::unique_lock<std::mutex> l(qm);
         qcv.wait(l, [&] { return queue.size() < qcap || failed.load() || canceled.load() || ended; });
-        if (canceled.load()) return fail(GAIL_ERROR_CANCELLED, "canceled");
-        if (failed.load()) return fail(GAIL_ERROR_FAILED, errorText());
+        auto refuse = [&](int status, const std::string& why) {
+            giveBuffer(it.buf);
+            return fail(status, why);
+        };
+        if (canceled.load()) return refuse(GAIL_ERROR_CANCELLED, "canceled");
+        if (failed.load()) return refuse(GAIL_ERROR_FAILED, errorText());
         // after send(NULL) or finish the workers may have gone: nothing queued now would run
-        if (ended) return fail(GAIL_ERROR_STATE, "the stream has already ended");
+        if (ended) return refuse(GAIL_ERROR_STATE, "the stream has already ended");
         it.index = nextIndex++;
         queue.push_back(std::move(it));
         qcv.notify_all();
@@ -184,7 +249,7 @@ struct gail_encoder {
                 const double thr = lastLambda.load() * spec->slopeFactor;
                 P->encoder_set_slope(slot, thr);
                 GailHipTier1 t1{};
-                if (P->encoder_encode(slot, it.buf.data(), &t1, nullptr, nullptr) != GAIL_HIP_OK)
+                if (P->encoder_encode(slot, it.buf, &t1, nullptr, nullptr) != GAIL_HIP_OK)
                     throw std::runtime_error(P->last_error());
                 Tier1Ref ref{(const uint32_t*)t1.info, t1.passLen, t1.passDist, t1.data, (size_t)t1.numInfo};
                 RateResult rr = allocateRate(spec->L, ref, spec->limits);
@@ -193,7 +258,7 @@ struct gail_encoder {
                     // an easier frame than the last: passes the rate control would keep may
                     // have been skipped; code it again without early termination
                     P->encoder_set_slope(slot, 0.0);
-                    if (P->encoder_encode(slot, it.buf.data(), &t1, nullptr, nullptr) != GAIL_HIP_OK)
+                    if (P->encoder_encode(slot, it.buf, &t1, nullptr, nullptr) != GAIL_HIP_OK)
                         throw std::runtime_error(P->last_error());
                     ref = Tier1Ref{(const uint32_t*)t1.info, t1.passLen, t1.passDist, t1.data, (size_t)t1.numInfo};
                     rr = allocateRate(spec->L, ref, spec->limits);
@@ -208,10 +273,12 @@ struct gail_encoder {
                     if (!d.compliant) log(GAIL_LOG_WARNING, "frame %lld failed the DCI check: %s", it.index, why.c_str());
                 }
             } catch (const std::exception& e) {
+                giveBuffer(it.buf);
                 stop(e.what());
                 if (it.waiter) finishWaiter(it.waiter, GAIL_ERROR_FAILED, nullptr);
                 break;
             }
+            giveBuffer(it.buf);
             {
                 std::lock_guard<std::mutex> l(om);
                 if (!deliverable()) {
@@ -306,10 +373,20 @@ struct gail_encoder {
             if (t.joinable()) t.join();
         ocv.notify_all();
         if (writer.joinable()) writer.join();
+        fillPool.reset();
         for (auto* s : slots) P->encoder_destroy(s);
         slots.clear();
         if (dev) P->device_destroy(dev);
         dev = nullptr;
+        {
+            std::lock_guard<std::mutex> lb(bm);
+            for (auto& b : allBufs) {
+                if (b.second) P->host_free(b.first);
+                else std::free(b.first);
+            }
+            allBufs.clear();
+            freeBufs.clear();
+        }
         std::lock_guard<std::mutex> l(om);
         for (auto& p : out) std::free(p.data);
         out.clear();
@@ -451,6 +528,7 @@ GAIL_API int gail_encoder_create(const gail_config* c, gail_encoder** encoder) {
             e->slots.push_back(slot);
         }
         e->qcap = e->slots.size() + 2;
+        e->fillPool.reset(new Pool(3));
         e->activeWorkers = (int)e->slots.size();
         for (auto* slot : e->slots) e->workers.emplace_back([ep = e.get(), slot] { ep->workerLoop(slot); });
         e->writer = std::thread([ep = e.get()] { ep->writerLoop(); });

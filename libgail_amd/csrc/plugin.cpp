@@ -309,14 +309,31 @@ void* encoderInput(GailHipEncoder* e) {
 }
 void encoderSetSlope(GailHipEncoder* e, double t) { e->job->setSlope(0, t); }
 
+// Whether the device can copy from p by DMA: pinned host memory or device memory.
+static bool dmaReadable(const void* p) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // pageable memory is not an error here
+        return false;
+    }
+    return a.type == hipMemoryTypeHost || a.type == hipMemoryTypeDevice;
+}
+
 int encoderEncode(GailHipEncoder* e, const void* frame, GailHipTier1* out, GailHipFrameTimings* timings,
                   GailHipDebug* debug) {
     return guarded([&] {
         if (!frame || !out) throw std::logic_error("encode needs a frame and an output");
         Job& j = *e->job;
-        if (frame != j.hostInput(0)) std::memcpy(j.hostInput(0), frame, j.frame(0).input.bytes());
         StageTimes t;
-        j.encode(true, &t);
+        if (frame == j.hostInput(0)) {
+            j.encode(true, &t);
+        } else if (dmaReadable(frame)) {  // one copy straight into the slot (not timed as upload)
+            j.uploadFrom(0, frame);
+            j.encode(false, &t);
+        } else {  // pageable memory: through the slot's pinned staging buffer
+            std::memcpy(j.hostInput(0), frame, j.frame(0).input.bytes());
+            j.encode(true, &t);
+        }
         fillTier1(j, 0, out);
         if (timings) fillTimings(t, timings);
         if (debug) debugTaps(e, 0, debug);

@@ -36,6 +36,7 @@
 #include "codestream.h"
 #include "gail_amd.h"
 #include "gail_hip_plugin.h"
+#include "host_pool.h"
 #include "stream.h"
 
 using namespace gail_amd;
@@ -50,50 +51,6 @@ int sfail(int status, const std::string& m) {
 }
 
 double msSince(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); }
-
-// Fixed set of host threads running queued tasks.
-class Pool {
-public:
-    explicit Pool(int n) {
-        for (int i = 0; i < n; ++i) threads_.emplace_back([this] { run(); });
-    }
-    ~Pool() {
-        {
-            std::lock_guard<std::mutex> l(m_);
-            stop_ = true;
-        }
-        cv_.notify_all();
-        for (auto& t : threads_) t.join();
-    }
-    void post(std::function<void()> f, bool urgent = false) {
-        {
-            std::lock_guard<std::mutex> l(m_);
-            if (urgent) q_.push_front(std::move(f));
-            else q_.push_back(std::move(f));
-        }
-        cv_.notify_one();
-    }
-
-private:
-    void run() {
-        for (;;) {
-            std::function<void()> f;
-            {
-                std::unique_lock<std::mutex> l(m_);
-                cv_.wait(l, [&] { return stop_ || !q_.empty(); });
-                if (q_.empty()) return;
-                f = std::move(q_.front());
-                q_.pop_front();
-            }
-            f();
-        }
-    }
-    std::mutex m_;
-    std::condition_variable cv_;
-    std::deque<std::function<void()>> q_;
-    std::vector<std::thread> threads_;
-    bool stop_ = false;
-};
 
 struct StreamState {
     std::unique_ptr<StreamSpec> spec;

@@ -235,40 +235,49 @@ int StreamSpec::checkFrame(const gail_frame& f, std::string& why) const {
     return GAIL_OK;
 }
 
-void StreamSpec::fill(const gail_frame& f, uint8_t* dst) const {
+void StreamSpec::fill(const gail_frame& f, uint8_t* dst, int part, int parts) const {
     const gail_pixel_format fmt = cfg.input.format;
+    auto rowsOf = [&](int n, int& y0, int& y1) {
+        y0 = (int)((long long)n * part / parts);
+        y1 = (int)((long long)n * (part + 1) / parts);
+    };
     if (planar) {
         for (int c = 0; c < 3; ++c) {
             const Plane& pl = planes[c];
-            const uint8_t* src = static_cast<const uint8_t*>(f.data[c]);
             const ptrdiff_t stride = f.stride[c] ? f.stride[c] : (ptrdiff_t)pl.rowBytes;
-            uint8_t* out = dst + pl.offset;
-            if (stride == (ptrdiff_t)pl.rowBytes) {  // unpadded rows: the plane in one copy
-                std::memcpy(out, src, pl.rowBytes * pl.rows);
+            int y0, y1;
+            rowsOf(pl.rows, y0, y1);
+            const uint8_t* src = static_cast<const uint8_t*>(f.data[c]) + y0 * stride;
+            uint8_t* out = dst + pl.offset + (size_t)y0 * pl.rowBytes;
+            if (stride == (ptrdiff_t)pl.rowBytes) {  // unpadded rows: the part in one copy
+                std::memcpy(out, src, pl.rowBytes * (size_t)(y1 - y0));
                 continue;
             }
-            for (int y = 0; y < pl.rows; ++y, out += pl.rowBytes, src += stride) std::memcpy(out, src, pl.rowBytes);
+            for (int y = y0; y < y1; ++y, out += pl.rowBytes, src += stride) std::memcpy(out, src, pl.rowBytes);
         }
         return;
     }
+    int y0, y1;
+    rowsOf(picH, y0, y1);
     if (fmt == GAIL_PIX_RGB48) {
-        if (picW != width || picH != height) std::memset(dst, 0, inBytes);
+        if (picW != width || picH != height) std::memset(dst, 0, inBytes);  // parts == 1 here
         const size_t row = (size_t)picW * 6;
         const ptrdiff_t stride = f.stride[0] ? f.stride[0] : (ptrdiff_t)row;
-        for (int y = 0; y < picH; ++y)
+        for (int y = y0; y < y1; ++y)
             std::memcpy(dst + ((size_t)(y + oy) * width + ox) * 6, (const uint8_t*)f.data[0] + y * stride, row);
         return;
     }
     // X'Y'Z': planar 12-bit codes, the picture centred, code 0 around it
-    if (picW != width || picH != height) std::memset(dst, 0, inBytes);
+    if (picW != width || picH != height) std::memset(dst, 0, inBytes);  // parts == 1 here
     uint16_t* planes[3] = {(uint16_t*)dst, (uint16_t*)dst + (size_t)width * height, (uint16_t*)dst + 2 * (size_t)width * height};
-    for (int y = 0; y < picH; ++y) {
+    for (int y = y0; y < y1; ++y) {
         const size_t o = (size_t)(y + oy) * width + ox;
         if (fmt == GAIL_PIX_XYZ12_PLANAR) {
+            // the rows as they are: the device reads codes above 4095 as 4095 (the DWT's DC
+            // shift clips on load), as it does for frames the served form takes directly
             for (int c = 0; c < 3; ++c) {
                 const ptrdiff_t stride = f.stride[c] ? f.stride[c] : (ptrdiff_t)picW * 2;
-                const uint16_t* s = (const uint16_t*)((const uint8_t*)f.data[c] + y * stride);
-                for (int x = 0; x < picW; ++x) planes[c][o + x] = (uint16_t)std::min<uint32_t>(s[x], 4095u);
+                std::memcpy(planes[c] + o, (const uint8_t*)f.data[c] + y * stride, (size_t)picW * 2);
             }
             continue;
         }

@@ -73,7 +73,12 @@ struct StreamSpec {
     // Checks that a frame has the planes this stream reads.
     int checkFrame(const gail_frame& f, std::string& why) const;
     // Copies an app frame into the plugin's input layout (inBytes at dst).
-    void fill(const gail_frame& f, uint8_t* dst) const;
+    void fill(const gail_frame& f, uint8_t* dst) const { fill(f, dst, 0, 1); }
+    // Part `part` of `parts` of that copy: the rows [rows * part / parts, rows * (part + 1) /
+    // parts) of every plane. Parts of one frame may run at once on different threads when the
+    // picture fills the container (splitParts()); otherwise only fill(f, dst, 0, 1).
+    void fill(const gail_frame& f, uint8_t* dst, int part, int parts) const;
+    bool splitParts() const { return picW == width && picH == height; }
     // When the frame already is the plugin's input layout in one block of memory, that
     // block (inBytes long); else nullptr.
     const void* directInput(const gail_frame& f) const;
