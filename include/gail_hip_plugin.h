@@ -318,6 +318,9 @@ struct GailHipPlugin {
      * the batch's slots. A caller that launches only then keeps the device's queues free of
      * waits on copies, and can hand the copy engines one batch's frames at a time. */
     int (*batch_inputs_ready)(GailHipBatch* b);
+    /* Device rate control result of the encoder's last frame (GAIL_HIP_MISUSE when its
+     * settings left rate control to the host); as batch_rate, for encoder_encode. */
+    int (*encoder_rate)(GailHipEncoder* e, GailHipRate* out);
 };
 typedef struct GailHipPlugin GailHipPlugin;
 

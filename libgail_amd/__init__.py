@@ -191,6 +191,7 @@ class Plugin(C.Structure):
         ("is_device_pointer", _F(C.c_int, _vp)),
         ("batch_rate", _F(C.c_int, _vp, C.c_int, C.POINTER(Rate))),
         ("batch_inputs_ready", _F(C.c_int, _vp)),
+        ("encoder_rate", _F(C.c_int, _vp, _vp)),
     ]
 
 

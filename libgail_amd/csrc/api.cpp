@@ -251,8 +251,22 @@ struct gail_encoder {
                 GailHipTier1 t1{};
                 if (P->encoder_encode(slot, it.buf, &t1, nullptr, nullptr) != GAIL_HIP_OK)
                     throw std::runtime_error(P->last_error());
+                // the device's rate control when the plugin ran it (the Tier-1 output then
+                // holds each block's chosen passes and no more), else PCRD here
+                auto rate = [&](const Tier1Ref& r) {
+                    GailHipRate dr{};
+                    if (P->encoder_rate && P->encoder_rate(slot, &dr) == GAIL_HIP_OK) {
+                        RateResult x;
+                        x.passes.resize(r.numBlocks);
+                        for (size_t b = 0; b < r.numBlocks; ++b) x.passes[b] = (uint8_t)r.passes((int)b);
+                        for (int c = 0; c < 3; ++c) x.lambdaComp[c] = dr.lambda[c];
+                        x.distortion = dr.distortion;
+                        return x;
+                    }
+                    return allocateRate(spec->L, r, spec->limits);
+                };
                 Tier1Ref ref{(const uint32_t*)t1.info, t1.passLen, t1.passDist, t1.data, (size_t)t1.numInfo};
-                RateResult rr = allocateRate(spec->L, ref, spec->limits);
+                RateResult rr = rate(ref);
                 double lam = std::min({rr.lambdaComp[0], rr.lambdaComp[1], rr.lambdaComp[2]});
                 if (thr > 0 && lam < thr) {
                     // an easier frame than the last: passes the rate control would keep may
@@ -261,7 +275,7 @@ struct gail_encoder {
                     if (P->encoder_encode(slot, it.buf, &t1, nullptr, nullptr) != GAIL_HIP_OK)
                         throw std::runtime_error(P->last_error());
                     ref = Tier1Ref{(const uint32_t*)t1.info, t1.passLen, t1.passDist, t1.data, (size_t)t1.numInfo};
-                    rr = allocateRate(spec->L, ref, spec->limits);
+                    rr = rate(ref);
                     lam = std::min({rr.lambdaComp[0], rr.lambdaComp[1], rr.lambdaComp[2]});
                 }
                 buildCodestream(spec->L, ref, rr.passes.data(), &d.cs, nullptr, tag);
@@ -510,15 +524,20 @@ GAIL_API int gail_encoder_create(const gail_config* c, gail_encoder** encoder) {
         if (e->P->device_create(index, nullptr, nullptr, &e->dev) != GAIL_HIP_OK)
             throw ApiError(GAIL_ERROR_FAILED, e->P->last_error());
         e->devices = e->P->device_name(e->dev);
+        // rate control on the device, as the served form runs it (byte-identical to PCRD on
+        // the host): a worker then only writes the packets of the passes the device kept
+        GailHipEncodeSettings es = e->spec->settings;
+        es.rcMaxTotal = e->spec->limits.maxTotal;
+        es.rcMaxComp = e->spec->limits.maxComp;
         // frames in flight: as asked (default 2), fewer when device memory is short
-        const uint64_t per = e->P->encoder_bytes(e->dev, &e->spec->plainLayout, &e->spec->settings);
+        const uint64_t per = e->P->encoder_bytes(e->dev, &e->spec->plainLayout, &es);
         GailHipMemory mem{};
         const bool known = e->P->device_memory(e->dev, &mem) == GAIL_HIP_OK && mem.budgeted;
         const int want = fitFrames(known ? mem.available : 0, mem.total, per,
                                    cfg.frames_in_flight > 0 ? cfg.frames_in_flight : 2, cfg.max_device_memory);
         for (int i = 0; i < want; ++i) {
             GailHipEncoder* slot = nullptr;
-            const int st = e->P->encoder_create(e->dev, &e->spec->plainLayout, &e->spec->settings, &slot);
+            const int st = e->P->encoder_create(e->dev, &e->spec->plainLayout, &es, &slot);
             if (st != GAIL_HIP_OK) {
                 if (e->slots.empty())
                     throw ApiError(st == GAIL_HIP_DEVICE_MEMORY ? GAIL_ERROR_NOMEM : GAIL_ERROR_FAILED, e->P->last_error());

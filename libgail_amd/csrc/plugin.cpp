@@ -574,20 +574,28 @@ int batchInputsReady(GailHipBatch* b) {
     });
 }
 
-int batchRate(GailHipBatch* b, int i, GailHipRate* out) {
+void fillRate(const RcFrame* r, GailHipRate* out) {
+    if (!r || !out) throw std::logic_error("the frame had no rate control on the device");
+    for (int c = 0; c < 3; ++c) {
+        out->lambda[c] = r->lambda[c];
+        out->comp[c] = r->comp[c];
+    }
+    out->total = r->total;
+    for (int k = 0; k < 6; ++k) out->tilePart[k] = r->tile_part[k];
+    out->distortion = r->distortion;
+    out->iterations = r->iterations;
+    out->numTileParts = r->ntp;
+}
+
+int encoderRate(GailHipEncoder* e, GailHipRate* out) {
     return guarded([&] {
-        const RcFrame* r = b->job->rateResult(i);
-        if (!r || !out) throw std::logic_error("the frame had no rate control on the device");
-        for (int c = 0; c < 3; ++c) {
-            out->lambda[c] = r->lambda[c];
-            out->comp[c] = r->comp[c];
-        }
-        out->total = r->total;
-        for (int k = 0; k < 6; ++k) out->tilePart[k] = r->tile_part[k];
-        out->distortion = r->distortion;
-        out->iterations = r->iterations;
-        out->numTileParts = r->ntp;
+        if (!e) throw std::logic_error("null encoder");
+        fillRate(e->job->rateResult(0), out);
     });
+}
+
+int batchRate(GailHipBatch* b, int i, GailHipRate* out) {
+    return guarded([&] { fillRate(b->job->rateResult(i), out); });
 }
 
 #define GAIL_STR2(x) #x
@@ -643,6 +651,7 @@ const GailHipPlugin g_plugin = {
     isDevicePointer,
     batchRate,
     batchInputsReady,
+    encoderRate,
 };
 
 }  // namespace
