@@ -1,12 +1,17 @@
 // Colour conversion and the irreversible 9/7 DWT for CDNA4 (gfx950, wave64).
 //
-// DWT: one kernel launch per decomposition level covers every frame of a job. A
-// workgroup owns a TX x TY tile of the level's LL region and all three components: it
-// stages the tile plus a 4-sample mirrored halo in LDS (coalesced row loads), lifts the
-// columns in registers (one thread per column, the whole column unrolled), then the rows
-// (two or four threads per row, each piece with its own halo), and writes the de-interleaved results back with coalesced row
-// stores: the LL quarter to the next level's scratch plane, HL/LH/HH straight into the
-// frame's Mallat-layout coefficient planes. The order of operations on every sample is the
+// DWT: one kernel launch per decomposition level covers every frame of a job, in one of
+// three forms. LDS tiles (dwt_level_kernel): a workgroup owns a TX x TY tile of the level's
+// LL region and all three components, stages it with a 4-sample mirrored halo in LDS, lifts
+// the columns in registers, then the rows, and writes the de-interleaved results with
+// coalesced row stores; levels 1 and deeper always run this way. Register strips
+// (dwt_strip_kernel, dwt_strip0_kernel, dwt_strip0b_kernel): one wave per 120-column strip,
+// the rows lifted through register pipelines and the row lifting's neighbours exchanged by
+// DPP. Line-writing groups (dwt_stage0_kernel, level 0 by default): four such strips per
+// workgroup stage their finished rows in LDS and write every band row of the group as whole
+// 128-byte lines. The LL quarter goes to the next level's scratch plane, HL/LH/HH into the
+// frame's coefficient planes (Mallat order; rows and level 0's high bands start on lines, at
+// the column the level's descriptor gives). The order of operations on every sample is the
 // CPU reference's (columns, then rows; lifting steps alpha..delta; low * (1/K), high * K),
 // and the kernels are built with -ffp-contract=off, so results are bit-identical to it.
 // Level 0 reads the frame's 12-bit X'Y'Z' codes and applies the DC shift and forward ICT
