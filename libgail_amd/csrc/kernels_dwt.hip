@@ -739,8 +739,9 @@ __global__ void __launch_bounds__(64 * WG, MINW) dwt_strip0b_kernel(const DwtLev
 // pair of finished rows is staged in LDS, then the workgroup writes the group's run of
 // NW * 60 columns of every band row with 16-byte stores. Coefficient rows, level 0's LL rows
 // and its high bands all start on a line (the padded coefficient layout) and a group's first
-// output column is a multiple of 32, so every run starts on a line; a strip's own 240-byte
-// pieces would start anywhere in one, and HBM takes partial lines far more slowly. The
+// output column is 60 NW times its index (for four strips a multiple of 16 floats), so a run
+// starts on a line or half-way into one and covers whole lines from there; a strip's own
+// 240-byte pieces would start anywhere in a line, and HBM takes partial lines far more slowly. The
 // staging is double-buffered: the pair finished at step i goes to buffer (i - 3) & 1, so one
 // barrier per pair orders the writes of a buffer after the reads of its previous pair.
 template <int NW, int MINW>
