@@ -1,5 +1,6 @@
 // Tier-2, rate control and the DCI structural check (see codestream.h).
 #include "codestream.h"
+#include "host_pool.h"
 #include "rc_shared.h"
 
 #include <algorithm>
@@ -9,6 +10,7 @@
 #include <memory>
 #include <mutex>
 #include <stdexcept>
+#include <thread>
 #include <utility>
 
 namespace gail_amd {
@@ -251,14 +253,13 @@ RcTables makeRcTables(const Layout& L) {
     return T;
 }
 
-CodestreamSizes rcMeasure(const Layout& L, const RcTables& T, const Tier1Ref& t1, const uint8_t* passes) {
-    const KeptBlocks view{&t1, passes};
-    (void)L;
+namespace {
+
+// Sizes from the packet bytes of each tile-part (its 14 header bytes added here).
+CodestreamSizes sizesOf(const RcTables& T, unsigned long long tp[6]) {
     CodestreamSizes sz;
     sz.numTileParts = (int)T.ntp;
-    unsigned long long tp[6] = {};
-    for (uint32_t k = 0; k < T.ntp; ++k) tp[k] = 14;
-    for (const RcPacket& pk : T.packets) tp[pk.tp] += rc_packet_bytes(pk, T.pbands.data(), view);
+    for (uint32_t k = 0; k < T.ntp; ++k) tp[k] += 14;
     sz.total = T.fixedBytes;
     for (uint32_t k = 0; k < T.ntp; ++k) {
         sz.tilePart[k] = tp[k];
@@ -266,6 +267,16 @@ CodestreamSizes rcMeasure(const Layout& L, const RcTables& T, const Tier1Ref& t1
         sz.total += tp[k];
     }
     return sz;
+}
+
+}  // namespace
+
+CodestreamSizes rcMeasure(const Layout& L, const RcTables& T, const Tier1Ref& t1, const uint8_t* passes) {
+    const KeptBlocks view{&t1, passes};
+    (void)L;
+    unsigned long long tp[6] = {};
+    for (const RcPacket& pk : T.packets) tp[pk.tp] += rc_packet_bytes(pk, T.pbands.data(), view);
+    return sizesOf(T, tp);
 }
 
 size_t emptyCodestreamBytes(const Layout& L) {
@@ -288,41 +299,111 @@ RateLimits dciLimits(double fps, double maxMbps, double compMbps) {
 // ---------------------------------------------------------------- rate control
 namespace {
 
-// Lower convex hull of each block's (rate, distortion) points (rate_control.cpp:38-86), as
-// start offsets into one array of hull points.
+// Helper threads shared by every rate-control call in the process. The calling thread takes
+// chunks of its own work as well, so a call completes even while the helpers serve another.
+Pool& rcHelpers() {
+    static Pool* const p = new Pool(std::clamp((int)std::thread::hardware_concurrency() - 1, 1, 4));
+    return *p;
+}
+
+int rcSlots() { return rcHelpers().size() + 1; }
+
+constexpr int kChunks = 32;
+
+std::pair<int, int> chunkRange(int n, int chunk) {
+    return {(int)((int64_t)n * chunk / kChunks), (int)((int64_t)n * (chunk + 1) / kChunks)};
+}
+
+// fn(chunk, slot) for every chunk in [0, kChunks) on the caller and the helpers. Partial sums
+// kept per slot are integers or minima/maxima, so their totals do not depend on the split.
+template <class F>
+void rcParallel(const F& fn) {
+    parallelChunks(rcHelpers(), kChunks, fn);
+}
+
+// Lower convex hull of each block's (rate, distortion) points (rate_control.cpp:38-86). Block
+// i's count[i] points start at pts[start[i]]; start[] leaves room for one point per coding
+// pass, so the blocks are hulled independently.
 struct Hull {
-    std::vector<uint32_t> start;
+    std::vector<uint32_t> start, count;
     std::vector<RcPoint> pts;
+    double minSlope = 0, maxSlope = 0;  // over the finite positive slopes; 0 if there are none
+    const RcPoint* at(int i) const { return pts.data() + start[i]; }
 };
 
 // Fills H (its buffers are reused from call to call, so a worker thread that codes many
 // frames does not fault fresh pages in for every one).
 void buildHull(const Layout& L, const Tier1Ref& t1, Hull& H) {
     const int n = L.numCblks();
-    size_t total = 0;
-    for (int i = 0; i < n; ++i) total += t1.passes(i);
     H.start.resize(n + 1);
-    H.pts.resize(total);
-    size_t out = 0;
+    H.count.resize(n);
+    uint32_t total = 0;
     for (int i = 0; i < n; ++i) {
-        H.start[i] = (uint32_t)out;
-        const int np = (int)t1.passes(i);
-        if (!np) continue;
-        if (np > kMaxPasses) throw std::runtime_error("code-block with more than 64 coding passes");
-        const Subband& b = L.bands[L.cblks[i].band];
-        const double scale = (double)b.delta * b.delta * b.weight;
-        out += rc_block_hull(np, (int)t1.nbps(i), scale, &t1.passLen[t1.info5[5 * i + 4]],
-                             &t1.passDist[t1.info5[5 * i + 4]], &H.pts[out]);
+        const uint32_t np = t1.passes(i);
+        if (np > (uint32_t)kMaxPasses) throw std::runtime_error("code-block with more than 64 coding passes");
+        H.start[i] = total;
+        total += np;
     }
-    H.start[n] = (uint32_t)out;
-    H.pts.resize(out);
+    H.start[n] = total;
+    H.pts.resize(total);
+    struct alignas(64) Range {
+        double lo = INFINITY, hi = 0;
+    };
+    std::vector<Range> range(rcSlots());
+    rcParallel([&](int chunk, int s) {
+        const auto [i0, i1] = chunkRange(n, chunk);
+        Range& r = range[s];
+        for (int i = i0; i < i1; ++i) {
+            const int np = (int)t1.passes(i);
+            H.count[i] = 0;
+            if (!np) continue;
+            const Subband& b = L.bands[L.cblks[i].band];
+            const double scale = (double)b.delta * b.delta * b.weight;
+            RcPoint* q = H.pts.data() + H.start[i];
+            const int m = rc_block_hull(np, (int)t1.nbps(i), scale, &t1.passLen[t1.info5[5 * i + 4]],
+                                        &t1.passDist[t1.info5[5 * i + 4]], q);
+            H.count[i] = (uint32_t)m;
+            for (int h = 0; h < m; ++h)
+                if (std::isfinite(q[h].slope) && q[h].slope > 0) {
+                    r.hi = std::max(r.hi, q[h].slope);
+                    r.lo = std::min(r.lo, q[h].slope);
+                }
+        }
+    });
+    H.minSlope = INFINITY;
+    H.maxSlope = 0;
+    for (const Range& r : range) {
+        H.maxSlope = std::max(H.maxSlope, r.hi);
+        H.minSlope = std::min(H.minSlope, r.lo);
+    }
 }
 
 void select(const Layout& L, const Hull& H, const double lambda[3], std::vector<uint8_t>& passes) {
-    for (int i = 0; i < L.numCblks(); ++i) {
-        const int c = L.bands[L.cblks[i].band].comp;
-        passes[i] = (uint8_t)rc_select(&H.pts[H.start[i]], (int)(H.start[i + 1] - H.start[i]), lambda[c]);
-    }
+    const int n = L.numCblks();
+    rcParallel([&](int chunk, int) {
+        const auto [i0, i1] = chunkRange(n, chunk);
+        for (int i = i0; i < i1; ++i)
+            passes[i] = (uint8_t)rc_select(H.at(i), (int)H.count[i], lambda[L.bands[L.cblks[i].band].comp]);
+    });
+}
+
+// rcMeasure over packet ranges on the helpers: the same integer sums.
+CodestreamSizes measureParallel(const RcTables& T, const Tier1Ref& t1, const uint8_t* passes) {
+    const KeptBlocks view{&t1, passes};
+    struct alignas(64) Part {
+        unsigned long long tp[6] = {};
+    };
+    std::vector<Part> part(rcSlots());
+    const int np = (int)T.packets.size();
+    rcParallel([&](int chunk, int s) {
+        const auto [k0, k1] = chunkRange(np, chunk);
+        for (int k = k0; k < k1; ++k)
+            part[s].tp[T.packets[k].tp] += rc_packet_bytes(T.packets[k], T.pbands.data(), view);
+    });
+    unsigned long long tp[6] = {};
+    for (const Part& p : part)
+        for (int k = 0; k < 6; ++k) tp[k] += p.tp[k];
+    return sizesOf(T, tp);
 }
 
 }  // namespace
@@ -366,37 +447,66 @@ namespace {
 // counts for any bin threshold in O(1) (rate_control.cpp:117-162). Integer sums.
 class SlopeHistogram {
 public:
-    // Buffers are reused across frames by the thread that owns the histogram.
+    // Buffers are reused across frames by the thread that owns the histogram. Each helper
+    // slot counts its blocks into counts of its own, summed per bin afterwards.
     void build(const Layout& L, const Hull& H, int64_t kLo, int nb) {
         const OctaveSteps& t = octaveSteps();
         nb_ = nb;
+        const size_t bins = 3 * (size_t)(nb + 1);
+        part_.resize(rcSlots());
+        for (Part& p : part_) p.used = false;
+        const int n = L.numCblks();
+        rcParallel([&](int chunk, int s) {
+            Part& p = part_[s];
+            if (!p.used) {  // a slot clears its counts when it first takes a chunk
+                p.used = true;
+                p.rate.assign(bins, 0ull);
+                p.incl.assign(bins, 0ull);
+            }
+            const auto [i0, i1] = chunkRange(n, chunk);
+            for (int i = i0; i < i1; ++i) {
+                const size_t c = (size_t)L.bands[L.cblks[i].band].comp * (nb + 1);
+                const RcPoint* q = H.at(i);
+                uint32_t prev = 0;
+                for (uint32_t h = 0; h < H.count[i]; ++h) {
+                    const int b = rc_bin_of(q[h].slope, kLo, nb, t.mantissa, t.first);
+                    p.rate[c + b] += q[h].rate - prev;
+                    if (h == 0) p.incl[c + b] += 1;
+                    prev = q[h].rate;
+                }
+            }
+        });
+        rate_.assign(bins, 0ull);
+        incl_.assign(bins, 0ull);
+        rcParallel([&](int chunk, int) {
+            const auto [b0, b1] = chunkRange((int)bins, chunk);
+            for (const Part& p : part_)
+                if (p.used)
+                    for (int b = b0; b < b1; ++b) {
+                        rate_[b] += p.rate[b];
+                        incl_[b] += p.incl[b];
+                    }
+        });
         for (int c = 0; c < 3; ++c) {
-            rate_[c].assign(nb_ + 1, 0ull);
-            incl_[c].assign(nb_ + 1, 0ull);
-        }
-        for (int i = 0; i < L.numCblks(); ++i) {
-            const int c = L.bands[L.cblks[i].band].comp;
-            unsigned long long* rate = rate_[c].data();
-            uint32_t prev = 0;
-            for (uint32_t h = H.start[i]; h < H.start[i + 1]; ++h) {
-                const int b = rc_bin_of(H.pts[h].slope, kLo, nb, t.mantissa, t.first);
-                rate[b] += H.pts[h].rate - prev;
-                if (h == H.start[i]) incl_[c][b] += 1;
-                prev = H.pts[h].rate;
+            unsigned long long* r = &rate_[(size_t)c * (nb + 1)];
+            unsigned long long* k = &incl_[(size_t)c * (nb + 1)];
+            for (int b = nb - 1; b >= 0; --b) {
+                r[b] += r[b + 1];
+                k[b] += k[b + 1];
             }
         }
-        for (int c = 0; c < 3; ++c)
-            for (int b = nb_ - 1; b >= 0; --b) {
-                rate_[c][b] += rate_[c][b + 1];
-                incl_[c][b] += incl_[c][b + 1];
-            }
     }
-    double data(int c, int b) const { return (double)rate_[c][b]; }
-    double incl(int c, int b) const { return (double)incl_[c][b]; }
+    double data(int c, int b) const { return (double)rate_[(size_t)c * (nb_ + 1) + b]; }
+    double incl(int c, int b) const { return (double)incl_[(size_t)c * (nb_ + 1) + b]; }
 
 private:
+    struct Part {
+        bool used = false;
+        std::vector<unsigned long long> rate, incl;
+    };
     int nb_ = 0;
-    std::vector<unsigned long long> rate_[3], incl_[3];
+    std::vector<unsigned long long> rate_, incl_;  // [component][bin], nb + 1 bins each
+    std::vector<Part> part_;
 };
 
 struct RateScratch {
@@ -413,12 +523,7 @@ RateResult allocateRate(const Layout& L, const Tier1Ref& t1, const RateLimits& l
     RateResult res;
     res.passes.assign(L.numCblks(), 0);
     const OctaveSteps& t = octaveSteps();
-    double maxSlope = 0, minSlope = INFINITY;
-    for (const RcPoint& p : H.pts)
-        if (std::isfinite(p.slope) && p.slope > 0) {
-            maxSlope = std::max(maxSlope, p.slope);
-            minSlope = std::min(minSlope, p.slope);
-        }
+    double maxSlope = H.maxSlope, minSlope = H.minSlope;
     if (maxSlope == 0) maxSlope = minSlope = 1;
     RcSearch S;
     S.init();
@@ -431,9 +536,10 @@ RateResult allocateRate(const Layout& L, const Tier1Ref& t1, const RateLimits& l
     S.first = t.first;
     SlopeHistogram& hist = scratch.hist;
     hist.build(L, H, S.kLo, S.nb);
+    const RcTables& T = packetTables(L);
     auto measure = [&](const double l[3]) {
         select(L, H, l, res.passes);
-        buildCodestream(L, t1, res.passes.data(), nullptr, &res.sizes);
+        res.sizes = measureParallel(T, t1, res.passes.data());
         return RcSizes{res.sizes.total, {res.sizes.comp[0], res.sizes.comp[1], res.sizes.comp[2]}};
     };
     RcSizes sz = measure(S.lambda);
@@ -441,10 +547,12 @@ RateResult allocateRate(const Layout& L, const Tier1Ref& t1, const RateLimits& l
     res.iterations = (int)S.iterations;
     res.lambda = std::max({S.lambda[0], S.lambda[1], S.lambda[2]});
     for (int c = 0; c < 3; ++c) res.lambdaComp[c] = S.lambda[c];
-    double dist = 0;
-    for (int i = 0; i < L.numCblks(); ++i)
-        for (uint32_t h = H.start[i]; h < H.start[i + 1]; ++h)
-            if (H.pts[h].passes > res.passes[i]) dist += H.pts[h].gain;
+    double dist = 0;  // one running sum in block order
+    for (int i = 0; i < L.numCblks(); ++i) {
+        const RcPoint* q = H.at(i);
+        for (uint32_t h = 0; h < H.count[i]; ++h)
+            if (q[h].passes > res.passes[i]) dist += q[h].gain;
+    }
     res.distortion = dist;
     return res;
 }

@@ -1,9 +1,12 @@
 // A fixed set of host threads running queued tasks (the served form's Tier-2 pool, the
-// encoder's frame copies).
+// encoder's frame copies, host rate control).
 #pragma once
+#include <algorithm>
+#include <atomic>
 #include <condition_variable>
 #include <deque>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -56,5 +59,37 @@ private:
     std::vector<std::thread> threads_;
     bool stop_ = false;
 };
+
+// fn(chunk, slot) for every chunk in [0, chunks), spread over the calling thread and the
+// pool's threads. slot (below pool.size() + 1) is the running thread's own for this call, so
+// per-slot partial results need no locks. The caller takes chunks as well, so a call
+// completes even while the pool's threads serve other work. fn must not throw.
+template <class F>
+void parallelChunks(Pool& pool, int chunks, const F& fn) {
+    struct Job {
+        std::atomic<int> next{0}, done{0}, slot{0};
+        int chunks = 0;
+        const F* fn = nullptr;
+        std::mutex m;
+        std::condition_variable cv;
+    };
+    auto job = std::make_shared<Job>();
+    job->chunks = chunks;
+    job->fn = &fn;
+    // a pool thread that starts after the last chunk was taken finds none and never calls fn
+    auto work = [](Job& j) {
+        const int s = j.slot.fetch_add(1);
+        int k = 0;
+        for (int c; (c = j.next.fetch_add(1)) < j.chunks; ++k) (*j.fn)(c, s);
+        if (k && j.done.fetch_add(k) + k == j.chunks) {
+            std::lock_guard<std::mutex> l(j.m);
+            j.cv.notify_all();
+        }
+    };
+    for (int h = 0; h < std::min(pool.size(), chunks - 1); ++h) pool.post([job, work] { work(*job); });
+    work(*job);
+    std::unique_lock<std::mutex> l(job->m);
+    job->cv.wait(l, [&] { return job->done.load() == job->chunks; });
+}
 
 }  // namespace gail_amd
