@@ -278,7 +278,7 @@ struct gail_encoder {
                     rr = rate(ref);
                     lam = std::min({rr.lambdaComp[0], rr.lambdaComp[1], rr.lambdaComp[2]});
                 }
-                buildCodestream(spec->L, ref, rr.passes.data(), &d.cs, nullptr, tag);
+                buildCodestream(spec->L, ref, rr.passes.data(), &d.cs, nullptr, tag, true);
                 lastLambda.store(lam);
                 d.mse = rr.distortion / ((double)spec->width * spec->height * 3);
                 if (cfg.check) {

@@ -191,11 +191,40 @@ static const RcTables& packetTables(const Layout& L) {
     return *cache.back().second;
 }
 
+namespace {
+
+// Helper threads shared by every rate-control and Tier-2 call in the process that asks for
+// them. The calling thread takes chunks of its own work as well, so a call completes even
+// while the helpers serve another.
+Pool& rcHelpers() {
+    static Pool* const p = new Pool(std::clamp((int)std::thread::hardware_concurrency() - 1, 1, 4));
+    return *p;
+}
+
+int rcSlots() { return rcHelpers().size() + 1; }
+
+constexpr int kChunks = 32;
+
+std::pair<int, int> chunkRange(int n, int chunk) {
+    return {(int)((int64_t)n * chunk / kChunks), (int)((int64_t)n * (chunk + 1) / kChunks)};
+}
+
+// fn(chunk, slot) for every chunk in [0, kChunks) on the caller and the helpers. Partial sums
+// kept per slot are integers or minima/maxima, so their totals do not depend on the split.
+template <class F>
+void rcParallel(const F& fn) {
+    parallelChunks(rcHelpers(), kChunks, fn);
+}
+
+CodestreamSizes measureParallel(const RcTables& T, const Tier1Ref& t1, const uint8_t* passes);
+
+}  // namespace
+
 void buildCodestream(const Layout& L, const Tier1Ref& t1, const uint8_t* passes, std::vector<uint8_t>* out,
-                     CodestreamSizes* sizes, const std::string& tag) {
+                     CodestreamSizes* sizes, const std::string& tag, bool parallel) {
     if (tag.size() > kMaxDeviceTagBytes) throw std::logic_error("device tag too long");
     const RcTables& T = packetTables(L);
-    const CodestreamSizes sz = rcMeasure(L, T, t1, passes);
+    const CodestreamSizes sz = parallel ? measureParallel(T, t1, passes) : rcMeasure(L, T, t1, passes);
     if (sizes) *sizes = sz;
     if (!out) return;
     const std::string comment = tag.empty() ? std::string(codestreamLabel()) : std::string(codestreamLabel()) + " " + tag;
@@ -206,8 +235,39 @@ void buildCodestream(const Layout& L, const Tier1Ref& t1, const uint8_t* passes,
     mainHeader(L, v, (int)T.ntp, comment);
     const size_t tlmPos = v.size() - 5 * T.ntp;
     const KeptBlocks view{&t1, passes};
-    std::vector<uint8_t> header;
-    std::vector<std::pair<uint32_t, uint32_t>> bodies;
+    // every packet coded into its chunk's bytes (chunks of consecutive packets, coded in
+    // parallel or in turn), then copied out in codestream order
+    const int npk = (int)T.packets.size();
+    std::vector<std::vector<uint8_t>> chunkBytes(kChunks);
+    std::vector<uint32_t> pkBegin(npk), pkEnd(npk);
+    std::vector<uint8_t> pkChunk(npk);
+    auto code = [&](int chunk, int) {
+        const auto [k0, k1] = chunkRange(npk, chunk);
+        std::vector<uint8_t>& o = chunkBytes[chunk];
+        std::vector<uint8_t> header;
+        std::vector<std::pair<uint32_t, uint32_t>> bodies;
+        for (int k = k0; k < k1; ++k) {
+            pkChunk[k] = (uint8_t)chunk;
+            pkBegin[k] = (uint32_t)o.size();
+            header.clear();
+            bodies.clear();
+            RcBitsT<AppendByte> bw{AppendByte{&header}};
+            bw.init();
+            const bool any = rc_packet_code(T.packets[k], T.pbands.data(), view, bw,
+                                            [&](uint32_t i, uint32_t len) { bodies.emplace_back(i, len); });
+            if (!any) {
+                o.push_back(0);
+            } else {
+                bw.finish();
+                o.insert(o.end(), header.begin(), header.end());
+                for (const auto& [i, len] : bodies) o.insert(o.end(), t1.code((int)i), t1.code((int)i) + len);
+            }
+            pkEnd[k] = (uint32_t)o.size();
+        }
+    };
+    if (parallel) parallelChunks(rcHelpers(), kChunks, code);
+    else
+        for (int c = 0; c < kChunks; ++c) code(c, 0);
     size_t next = 0;
     for (uint32_t tp = 0; tp < T.ntp; ++tp) {
         const size_t sot = v.size();
@@ -219,19 +279,8 @@ void buildCodestream(const Layout& L, const Tier1Ref& t1, const uint8_t* passes,
         v.push_back((uint8_t)T.ntp);
         put16(v, SOD);
         for (; next < T.packets.size() && T.packets[next].tp == tp; ++next) {
-            header.clear();
-            bodies.clear();
-            RcBitsT<AppendByte> bw{AppendByte{&header}};
-            bw.init();
-            const bool any = rc_packet_code(T.packets[next], T.pbands.data(), view, bw,
-                                            [&](uint32_t i, uint32_t len) { bodies.emplace_back(i, len); });
-            if (!any) {
-                v.push_back(0);
-                continue;
-            }
-            bw.finish();
-            v.insert(v.end(), header.begin(), header.end());
-            for (const auto& [i, len] : bodies) v.insert(v.end(), t1.code((int)i), t1.code((int)i) + len);
+            const uint8_t* o = chunkBytes[pkChunk[next]].data();
+            v.insert(v.end(), o + pkBegin[next], o + pkEnd[next]);
         }
         const uint32_t psot = (uint32_t)(v.size() - sot);
         if (psot != sz.tilePart[tp]) throw std::logic_error("tile-part size differs from its measurement");
@@ -298,28 +347,6 @@ RateLimits dciLimits(double fps, double maxMbps, double compMbps) {
 
 // ---------------------------------------------------------------- rate control
 namespace {
-
-// Helper threads shared by every rate-control call in the process. The calling thread takes
-// chunks of its own work as well, so a call completes even while the helpers serve another.
-Pool& rcHelpers() {
-    static Pool* const p = new Pool(std::clamp((int)std::thread::hardware_concurrency() - 1, 1, 4));
-    return *p;
-}
-
-int rcSlots() { return rcHelpers().size() + 1; }
-
-constexpr int kChunks = 32;
-
-std::pair<int, int> chunkRange(int n, int chunk) {
-    return {(int)((int64_t)n * chunk / kChunks), (int)((int64_t)n * (chunk + 1) / kChunks)};
-}
-
-// fn(chunk, slot) for every chunk in [0, kChunks) on the caller and the helpers. Partial sums
-// kept per slot are integers or minima/maxima, so their totals do not depend on the split.
-template <class F>
-void rcParallel(const F& fn) {
-    parallelChunks(rcHelpers(), kChunks, fn);
-}
 
 // Lower convex hull of each block's (rate, distortion) points (rate_control.cpp:38-86). Block
 // i's count[i] points start at pts[start[i]]; start[] leaves room for one point per coding

@@ -42,9 +42,10 @@ const char* codestreamLabel();
 
 // Writes (out != nullptr) or only measures the codestream with passes[i] coding passes of
 // block i. The sizes count the COM marker at its longest, so the coded data never depends
-// on the tag.
+// on the tag. parallel: packets are coded on the rate-control helper threads as well (one
+// frame at a time callers); the bytes are the same either way.
 void buildCodestream(const Layout& L, const Tier1Ref& t1, const uint8_t* passes, std::vector<uint8_t>* out,
-                     CodestreamSizes* sizes, const std::string& tag = std::string());
+                     CodestreamSizes* sizes, const std::string& tag = std::string(), bool parallel = false);
 // Smallest codestream this layout can produce (no pass anywhere).
 size_t emptyCodestreamBytes(const Layout& L);
 

@@ -50,7 +50,7 @@ extern "C" GAIL_AMD_EXPORT int64_t gail_encode_codestream(int W, int H, int four
         if (mse) *mse = rr.distortion / ((double)W * H * 3);
         if (!out) return (int64_t)rr.sizes.total;
         std::vector<uint8_t> cs;
-        buildCodestream(L, t1, rr.passes.data(), &cs, nullptr, tag ? tag : "");
+        buildCodestream(L, t1, rr.passes.data(), &cs, nullptr, tag ? tag : "", true);
         if (cs.size() > cap) return -2;
         std::memcpy(out, cs.data(), cs.size());
         return (int64_t)cs.size();
@@ -146,7 +146,7 @@ extern "C" GAIL_AMD_EXPORT int64_t gail_build_codestream(int W, int H, int fourk
         std::vector<uint8_t> passes(nblocks);
         for (uint64_t i = 0; i < nblocks; ++i) passes[i] = (uint8_t)t1.passes((int)i);
         std::vector<uint8_t> cs;
-        buildCodestream(L, t1, passes.data(), &cs, nullptr, tag ? tag : "");
+        buildCodestream(L, t1, passes.data(), &cs, nullptr, tag ? tag : "", true);
         if (cs.size() > cap) return -2;
         std::memcpy(out, cs.data(), cs.size());
         return (int64_t)cs.size();
