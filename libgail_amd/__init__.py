@@ -315,15 +315,17 @@ def _check(status, what):
 
 
 class Tier1Result:
-    """A copy of one frame's Tier-1 output (gail::Tier1Frame)."""
+    """One frame's Tier-1 output (gail::Tier1Frame): a copy, or with copy=False views of the
+    encoder's own buffers, valid until its next encode or close (the plugin's contract)."""
 
-    def __init__(self, t):
+    def __init__(self, t, copy=True):
         n, npass, nd = t.numInfo, t.numPasses, t.numData
-        self.info = np.ctypeslib.as_array(C.cast(t.info, C.POINTER(C.c_uint32)), (n * 5,)).reshape(n, 5).copy() \
+        own = (lambda a: a.copy()) if copy else (lambda a: a)
+        self.info = own(np.ctypeslib.as_array(C.cast(t.info, C.POINTER(C.c_uint32)), (n * 5,)).reshape(n, 5)) \
             if n else np.zeros((0, 5), np.uint32)
-        self.pass_len = np.ctypeslib.as_array(t.passLen, (npass,)).copy() if npass else np.zeros(0, np.uint32)
-        self.pass_dist = np.ctypeslib.as_array(t.passDist, (npass,)).copy() if npass else np.zeros(0, np.int32)
-        self.data = np.ctypeslib.as_array(t.data, (nd,)).copy() if nd else np.zeros(0, np.uint8)
+        self.pass_len = own(np.ctypeslib.as_array(t.passLen, (npass,))) if npass else np.zeros(0, np.uint32)
+        self.pass_dist = own(np.ctypeslib.as_array(t.passDist, (npass,))) if npass else np.zeros(0, np.int32)
+        self.data = own(np.ctypeslib.as_array(t.data, (nd,))) if nd else np.zeros(0, np.uint8)
 
     def block(self, i):
         nbps, np_, off, ln, pb = (int(v) for v in self.info[i])
@@ -369,13 +371,14 @@ class Encoder:
     def set_slope(self, t):
         plugin().encoder_set_slope(self.handle, t)
 
-    def encode(self, frame, debug=False, full=True):
+    def encode(self, frame, debug=False, full=True, copy=True):
+        """Tier-1 of one frame; copy=False returns views valid until the next encode."""
         frame = np.ascontiguousarray(frame)
         out, tm = Tier1(), FrameTimings()
         dbg = Debug(1 if full else 0) if debug else None
         _check(plugin().encoder_encode(self.handle, frame.ctypes.data_as(_vp), C.byref(out), C.byref(tm),
                                        C.byref(dbg) if dbg else None), "encoder_encode")
-        res = Tier1Result(out)
+        res = Tier1Result(out, copy=copy)
         res.timings = {k: getattr(tm, k) for k, _ in FrameTimings._fields_}
         if dbg:
             res.debug = _debug_arrays(dbg, self.layout)
